@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: simulated days per second of the 513^2 3-layer FP64 ocean box.
+
+One "step" is one QG timestep (step_qg) of the 513 x 513 x 3 double-gyre box
+at dx = 7.5 km, dt = 20 min, with the leapfrog suppression every 25 steps, on
+synthetic seeded input resident in HBM.  The default K = 2160 steps is the
+full 30-day run.  Prints one JSON line.
+
+Also reported:
+  roofline      the dominant kernel's algorithmic bytes per launch / its mean
+                duration (HIP events on the library's stream), against 8 TB/s;
+  cpu_baseline  the CPU oracle (a C restatement of the reference step) on the
+                host cores, on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(w, kernel: str) -> float:
+    """Compulsory HBM bytes of one launch of `kernel` (F = one layer field)."""
+    F = 8.0 * w.nxp * w.nyp
+    nl = w.nl
+    return {
+        "k_vort": (5 * nl + 2) * F,      # read q, p, pm, qm, ent1, wekp; write q_new
+        "k_fwd": 2 * nl * F,             # read q_new; write x spectra
+        "k_ysolve": 2 * nl * F,          # read + write spectra
+        "k_inv": (3 * nl - 1) * F,       # read spectra + hom solutions; write p_new
+        "k_cyc": 0.0,
+        "step": (6 * nl + 2) * F,        # whole step: q, qm, p, pm, wekp, ent1 in; q, p out
+    }[kernel]
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(workload, {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(w, seconds: float):
+    from oracle import cpuref as cr
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    cr.set_threads(cores)
+    b = cr.Basin(w.params())
+    p = w.initial_p()
+    b.set_state(p=p, pm=p)
+    b.init_state()
+    ent, wekp, txis, txin = w.forcing()
+    # calibrate, then time a bounded sample
+    t0 = time.perf_counter()
+    b.step(w.tdt, ent, wekp, txis, txin)
+    b.suppress()
+    one = max(time.perf_counter() - t0, 1e-4)
+    n = int(max(3, min(2000, seconds / one)))
+    t0 = time.perf_counter()
+    for nt in range(2, n + 2):
+        b.step(w.tdt, ent, wekp, txis, txin)
+        if (nt - 1) % w.supp_every == 0:
+            b.suppress()
+    el = time.perf_counter() - t0
+    return {
+        "value": n * w.dt / 86400.0 / el,
+        "unit": "simulated days/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{n} steps of {w.name} (dt={w.dt:g} s) after init, OpenMP {cores} threads, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2160)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="box513")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--flags", type=int, default=0, help="oqg_desc.flags (1: per-thread Thomas y-solve)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from openqg_amd import workloads as W
+
+    w = W.WORKLOADS[args.workload]()
+    qg = w.make_qg(flags=args.flags)
+    p = w.initial_p()
+    qg.set_state(p=p, pm=p)
+    qg.init_state()
+    ent, wekp, txis, txin = w.forcing()
+    qg.set_forcing(ent, wekp, txis, txin)
+
+    qg.run(w.tdt, args.warmup, 0, w.supp_every)
+    qg.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    qg.run(w.tdt, args.steps, args.warmup, w.supp_every)
+    qg.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = el / args.steps * 1e3
+    days_per_s = args.steps * w.dt / 86400.0 / el
+
+    # per-kernel durations from HIP events on the library's stream
+    kt = qg.kernel_times(w.tdt, 50)
+    dom = max((k for k in kt if kt[k] > 0), key=lambda k: kt[k])
+    ach = algorithmic_bytes(w, dom) / (kt[dom] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(ach, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(ach / HBM_PEAK_GBS, 4),
+        "traffic": load_traffic(dom, w.name),
+        "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
+        "step_algorithmic_GBs": round(algorithmic_bytes(w, "step") / (ms_per_step * 1e-3) / 1e9, 1),
+    }
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(w, args.cpu_seconds)
+    out = {
+        "metric": "simulated days/s, 513^2 3-layer ocean FP64",
+        "value": round(days_per_s, 3),
+        "unit": "simulated days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded k^-3 PV field, double-gyre wind)",
+        "config": {"workload": w.name, "nxp": w.nxp, "nyp": w.nyp, "nl": w.nl, "dx_m": w.dx, "dt_s": w.dt,
+                   "supp_every": w.supp_every, "y_solver": "thomas" if args.flags & 1 else "wavefront-pcr"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

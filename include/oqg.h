@@ -1,0 +1,168 @@
+/*
+ * oqg.h -- C ABI of the MI355X (gfx950) QG timestep library, liboqg.so.
+ *
+ * Drop-in boundary for OpenQG's step_qg (src/qg/core.F90:16-33).  A context
+ * holds one basin (ocean box or atmosphere channel) resident in device memory;
+ * the caller steps it and reads fields back when its diagnostics, coupling or
+ * restart code needs them.
+ *
+ * Layout: every field argument is a Fortran (nxp,nyp,nl) array, i.e. the C
+ * array a[k][j][i] with x fastest, exactly as qg%q / qg%p are stored
+ * (src/qg/qg.F90:84-87).  All arithmetic is IEEE FP64.
+ *
+ * Conventions: functions return 0 on success and a nonzero OQG_ERR_* code on
+ * failure; oqg_last_error() gives the message.  A context is not thread safe;
+ * use one context per basin.  Host pointers are borrowed only for the call.
+ * `kind` selects host (OQG_HOST) or device (OQG_DEVICE) memory for pointer
+ * arguments that accept either.
+ */
+#ifndef OQG_H
+#define OQG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OQG_MAX_LAYERS 8
+
+enum { OQG_HOST = 0, OQG_DEVICE = 1 };
+enum { OQG_FIELD_Q = 0, OQG_FIELD_QM = 1, OQG_FIELD_P = 2, OQG_FIELD_PM = 3 };
+enum {
+    OQG_OK = 0,
+    OQG_ERR_ARG = 1,         /* invalid argument or unsupported shape */
+    OQG_ERR_HIP = 2,         /* HIP runtime error (message has the HIP error string) */
+    OQG_ERR_NUMERIC = 3,     /* failed precompute check (mode identity, singular matrix) */
+    OQG_ERR_STATE = 4        /* call made in the wrong state (e.g. step before state set) */
+};
+/* oqg_desc.flags */
+enum {
+    OQG_FLAG_YSOLVE_THOMAS = 1   /* y-tridiagonal by per-thread Thomas instead of the
+                                    wavefront partition + PCR solver */
+};
+
+/*
+ * Basin description: the fields of box_type (src/glam/box.F90:11-41),
+ * qg_type (src/qg/qg.F90:17-53) and topog_type (src/glam/topog.F90:16-28)
+ * that the step reads.  Fixed-size arrays keep the struct interoperable with
+ * Fortran bind(C).  The optional pointers let a Fortran caller hand over the
+ * precompute its unchanged init_qg already did (qg%mod, qg%hom); when NULL the
+ * library derives them itself.
+ */
+typedef struct oqg_desc {
+    int nxp, nyp, nl;             /* p-grid points and layers (box.F90:132-133) */
+    int cyclic;                   /* zonally periodic channel (box.F90:75) */
+    double dx, dy;                /* grid spacing (m) */
+    double fnot, beta;            /* f0 (1/s), beta (1/(m s)) */
+    double h[OQG_MAX_LAYERS];     /* layer thicknesses (m) */
+    double gp[OQG_MAX_LAYERS];    /* reduced gravities, nl-1 used (m/s^2) */
+    double ah2[OQG_MAX_LAYERS];   /* Del^4 coefficients (m^2/s) */
+    double ah4[OQG_MAX_LAYERS];   /* Del^6 coefficients (m^4/s) */
+    double bcco;                  /* mixed boundary coefficient */
+    double delek;                 /* bottom Ekman layer thickness (m) */
+    int dz_sign, tau_sign;        /* basin.F90:36,62 (ocean -1,+1) / :129,158 (atmos +1,-1) */
+    int k_topo;                   /* 1-based; 0 = derive from dz_sign (topog.F90:52-56) */
+    int flags;                    /* OQG_FLAG_* */
+    const double* ddyn;           /* (nxp,nyp) dynamic topography, NULL = flat */
+    /* optional precompute, Fortran (nl,nl) column-major; NULL = derive */
+    const double* amat;           /* modes_type%amat  (modes.F90:13) */
+    const double* ctl2m;          /* modes_type%ctl2m (modes.F90:17) */
+    const double* ctm2l;          /* modes_type%ctm2l (modes.F90:18) */
+    const double* rdm2;           /* modes_type%rdm2 (nl)           */
+    /* optional homogeneous solutions; NULL = derive (requires the modes above
+       to be consistent with them when given) */
+    const double* hom_sol_bc;     /* box: (nxp,nyp,nl-1)  homog_box_type%hom_sol_bc */
+    const double* hom_bt;         /* cyclic: (nyp)        hom_sol_bt */
+    const double* hom_bc1;        /* cyclic: (nyp,nl-1)   hom_sol_bc1 */
+    const double* hom_bc2;        /* cyclic: (nyp,nl-1)   hom_sol_bc2 */
+    const double* hom_coef;       /* cyclic: hc1s,hc2s,hc1n,hc2n per mode 2..nl, then hbsi */
+} oqg_desc;
+
+typedef struct oqg_ctx oqg_ctx;
+
+/* init_qg (src/qg/qg.F90:60-87): allocate device state, derive modes
+ * (eigmod, modes.F90:46), Helmholtz tables (init_inhomog, inhomog.F90:71)
+ * and homogeneous solutions (homsol, homog.F90:108).  State starts at zero. */
+int oqg_create(const oqg_desc* desc, oqg_ctx** out);
+void oqg_destroy(oqg_ctx* ctx);
+
+/* Message for the last failure on ctx (or of oqg_create when ctx is NULL). */
+const char* oqg_last_error(const oqg_ctx* ctx);
+
+/* Replace or read the four time levels q, qm, p, pm (qg%q .. qg%pm).  NULL
+ * pointers are skipped.  In a cyclic basin column nxp must equal column 1. */
+int oqg_set_state(oqg_ctx* ctx, const double* q, const double* qm, const double* p,
+                  const double* pm, int kind);
+int oqg_get_state(oqg_ctx* ctx, double* q, double* qm, double* p, double* pm, int kind);
+
+/* One layer (k = 1..nl) of q/qm/p/pm (OQG_FIELD_*): the cheap exchanges of
+ * driver.F90:100,127 and windstress.F90:129-130. */
+int oqg_get_layer(oqg_ctx* ctx, int which, int k, double* out, int kind);
+
+/* Constraint scalars (constraint.F90:22-67), packed as
+ * dpi[nl-1], dpip[nl-1], cs[nl], cn[nl], csp[nl], cnp[nl]. */
+int oqg_constr_len(const oqg_ctx* ctx);
+int oqg_set_constr(oqg_ctx* ctx, const double* v);
+int oqg_get_constr(oqg_ctx* ctx, double* v);
+
+/* init_pv (vorsubs.F90:17-25), init_constraint (constraint.F90:80-143, cyclic)
+ * and init_mass_constr (constraint.F90:145-165) from the current p and pm,
+ * in the order of openqg.F90:239-243. */
+int oqg_init_state(oqg_ctx* ctx);
+
+/* Upload forcing that stays resident for later steps: ent (nxp,nyp,n_ent),
+ * n_ent >= 1 (layer 1 everywhere; layer 2 edge rows in the cyclic case),
+ * wekp (nxp,nyp), and the cyclic boundary stress integrals txis, txin. */
+int oqg_set_forcing(oqg_ctx* ctx, const double* ent, int n_ent, const double* wekp,
+                    double txis, double txin, int kind);
+
+/* step_qg (core.F90:16-33): vort_step, solve_pressure, ocqbdy.  ent/wekp are
+ * host arrays as in the reference call; pass NULL for both to use the
+ * resident forcing (then txis/txin are also taken from oqg_set_forcing).
+ * Asynchronous on the context's stream when the forcing is resident. */
+int oqg_step(oqg_ctx* ctx, double tdt, const double* ent, int n_ent, const double* wekp,
+             double txis, double txin);
+
+/* suppress_comp_mode (driver.F90:174-189). */
+int oqg_suppress(oqg_ctx* ctx);
+
+/* Ocean-only time loop (openqg.F90:483-509 with driver.F90:89-110) on the
+ * resident forcing: for nt = step0+1 .. step0+nsteps, one step_qg, then
+ * suppress_comp_mode when mod(nt-1, supp_every) == 0 (supp_every <= 0: never).
+ * Replays cached HIP graphs; asynchronous. */
+int oqg_run(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp_every);
+
+/* Wait for all work queued on the context's stream. */
+int oqg_sync(oqg_ctx* ctx);
+
+/* solve_inhomog_eqn (inhomog.F90:156-168) for mode m = 1..nl: rhs and out are
+ * (nxp,nyp) arrays.  Synchronous. */
+int oqg_solve_inhomog(oqg_ctx* ctx, int m, const double* rhs, double* out, int kind);
+
+/* The mode matrices in use, Fortran (nl,nl) column-major, and rdm2 (nl). */
+int oqg_get_modes(oqg_ctx* ctx, double* amat, double* ctl2m, double* ctm2l, double* rdm2);
+
+/* Stream used for all device work of ctx (a hipStream_t). Setting NULL
+ * restores the context's own stream. */
+int oqg_set_stream(oqg_ctx* ctx, void* stream);
+void* oqg_get_stream(oqg_ctx* ctx);
+
+/* Device pointers of the current q, qm, p, pm and the row pitch (in doubles)
+ * of the resident layout a[k][j][i] at a + (k*nyp + j)*pitch + i.  Valid until
+ * the next step or suppression, which rotate the time levels. */
+int oqg_state_ptrs(oqg_ctx* ctx, double** q, double** qm, double** p, double** pm,
+                   size_t* pitch);
+
+/* Advances the basin by nsteps steps of step_qg on the resident forcing,
+ * recording HIP events on the context's stream around each kernel, and
+ * returns in ms[] the average duration of each kernel of the step (names in
+ * `names`, '\n' separated; a kernel a basin does not use reports 0).
+ * Returns the number of kernels reported (<= max) or a negative error. */
+int oqg_kernel_times(oqg_ctx* ctx, double tdt, int nsteps, double* ms, int max, char* names,
+                     int names_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

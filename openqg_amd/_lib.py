@@ -1,0 +1,97 @@
+"""ctypes binding of liboqg.so (include/oqg.h).
+
+The library is the only compute path: importing this module on a machine with
+a GPU but without the built library raises, and no operation falls back to
+CPU code.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboqg.so")
+MAX_LAYERS = 8
+
+OQG_HOST, OQG_DEVICE = 0, 1
+FIELD_Q, FIELD_QM, FIELD_P, FIELD_PM = 0, 1, 2, 3
+FLAG_YSOLVE_THOMAS = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class OqgDesc(ctypes.Structure):
+    """Mirror of `oqg_desc` (include/oqg.h)."""
+
+    _fields_ = [
+        ("nxp", ctypes.c_int), ("nyp", ctypes.c_int), ("nl", ctypes.c_int), ("cyclic", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("fnot", ctypes.c_double), ("beta", ctypes.c_double),
+        ("h", ctypes.c_double * MAX_LAYERS), ("gp", ctypes.c_double * MAX_LAYERS),
+        ("ah2", ctypes.c_double * MAX_LAYERS), ("ah4", ctypes.c_double * MAX_LAYERS),
+        ("bcco", ctypes.c_double), ("delek", ctypes.c_double),
+        ("dz_sign", ctypes.c_int), ("tau_sign", ctypes.c_int),
+        ("k_topo", ctypes.c_int), ("flags", ctypes.c_int),
+        ("ddyn", _dp),
+        ("amat", _dp), ("ctl2m", _dp), ("ctm2l", _dp), ("rdm2", _dp),
+        ("hom_sol_bc", _dp), ("hom_bt", _dp), ("hom_bc1", _dp), ("hom_bc2", _dp), ("hom_coef", _dp),
+    ]
+
+
+# (name, restype, argtypes) for every entry point of include/oqg.h
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+SIGNATURES = [
+    ("oqg_create", _i, [ctypes.POINTER(OqgDesc), ctypes.POINTER(_vp)]),
+    ("oqg_destroy", None, [_vp]),
+    ("oqg_last_error", ctypes.c_char_p, [_vp]),
+    ("oqg_set_state", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    ("oqg_get_state", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    ("oqg_get_layer", _i, [_vp, _i, _i, _vp, _i]),
+    ("oqg_constr_len", _i, [_vp]),
+    ("oqg_set_constr", _i, [_vp, _dp]),
+    ("oqg_get_constr", _i, [_vp, _dp]),
+    ("oqg_init_state", _i, [_vp]),
+    ("oqg_set_forcing", _i, [_vp, _vp, _i, _vp, _d, _d, _i]),
+    ("oqg_step", _i, [_vp, _d, _vp, _i, _vp, _d, _d]),
+    ("oqg_suppress", _i, [_vp]),
+    ("oqg_run", _i, [_vp, _d, _i, _i, _i]),
+    ("oqg_sync", _i, [_vp]),
+    ("oqg_solve_inhomog", _i, [_vp, _i, _vp, _vp, _i]),
+    ("oqg_get_modes", _i, [_vp, _dp, _dp, _dp, _dp]),
+    ("oqg_set_stream", _i, [_vp, _vp]),
+    ("oqg_get_stream", _vp, [_vp]),
+    ("oqg_state_ptrs", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                            ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_size_t)]),
+    ("oqg_kernel_times", _i, [_vp, _d, _i, _dp, _i, ctypes.c_char_p, _i]),
+]
+
+_lib = None
+
+
+class OqgError(RuntimeError):
+    pass
+
+
+def load():
+    """Load liboqg.so (built by `python -m openqg_amd.build`); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OqgError(f"{LIB_PATH} is not built; run `python -m openqg_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, ctx=None, what: str = ""):
+    if rc != 0:
+        msg = load().oqg_last_error(ctx)
+        msg = msg.decode() if msg else ""
+        raise OqgError(f"{what} failed (code {rc}): {msg}")

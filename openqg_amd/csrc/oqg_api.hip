@@ -1,0 +1,1104 @@
+// oqg_api.hip -- host side of liboqg: the C ABI of include/oqg.h, the init-time
+// precompute (modes, Helmholtz tables, homogeneous solutions) and step scheduling
+// (eager launches or replayed HIP graphs) on one stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "oqg_internal.h"
+
+using namespace oqg;
+
+namespace {
+
+thread_local std::string g_create_err;
+constexpr double TWOPI_REF = 2.0 * 3.14159265358979324;   // constants.F90: TWOPI
+
+struct GraphKey {
+    double tdt;
+    int nsteps, supp, cur;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(tdt, nsteps, supp, cur) < std::tie(o.tdt, o.nsteps, o.supp, o.cur);
+    }
+};
+
+}  // namespace
+
+struct oqg_ctx {
+    oqg_desc d{};
+    Geo g{};
+    Spec s{};
+    Mix mx{};
+    Phys ph{};
+    BoxHom bh{};
+    CycHom ch{};
+    int variant = 0;       // y-solver: 0 wavefront partition+PCR, 1 per-thread Thomas
+    int cur = 0;           // time-level parity
+    int tiles_x = 0, tiles_y = 0;
+    size_t fbytes = 0;     // bytes of one nl-layer field (pitch layout)
+    // device memory
+    double* qbuf[2] = {nullptr, nullptr};
+    double* pbuf[2] = {nullptr, nullptr};
+    double* cbuf[2] = {nullptr, nullptr};
+    double* spec = nullptr;
+    double* gamw = nullptr;
+    double* colsum = nullptr;
+    double* colw = nullptr;
+    double* entpart = nullptr;
+    double* aux = nullptr;
+    double* ent1 = nullptr;
+    double* ent2s = nullptr;
+    double* ent2n = nullptr;
+    double* wekp = nullptr;
+    double* ddyn = nullptr;
+    double* yprel = nullptr;
+    double* hom = nullptr;
+    double* bcol = nullptr;
+    double* tw = nullptr;
+    double* raw[2] = {nullptr, nullptr};
+    DevConsts* dconst = nullptr;
+    std::vector<void*> allocs;
+    bool has_ent2 = false;
+    double txis = 0.0, txin = 0.0;
+    hipStream_t own = nullptr;
+    hipStream_t st = nullptr;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    std::string err;
+    // host copies
+    std::vector<double> yprel_h, ddyn_h;
+};
+
+// ------------------------------------------------------------------ error helpers
+
+static int fail(oqg_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                    \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ctx, OQG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+static int dalloc(oqg_ctx* c, T** p, size_t n) {
+    void* v = nullptr;
+    HIPCHK(c, hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPCHK(c, hipMemsetAsync(v, 0, std::max<size_t>(n, 1) * sizeof(T), c->st));
+    c->allocs.push_back(v);
+    *p = static_cast<T*>(v);
+    return 0;
+}
+
+// ------------------------------------------------------------------ host precompute
+
+// exp(-2 pi i t / n): quadrant reduction, long-double sin/cos of the reduced angle.
+static void twiddle_ld(long t, long n, double* re, double* im) {
+    t %= n;
+    const long q = (4 * t) / n, r = 4 * t - q * n;   // angle = (q + r/n) pi/2
+    const long double ang = 1.5707963267948966192313216916397514L * (long double)r / (long double)n;
+    long double c = cosl(ang), s = sinl(ang);
+    if (r == 0) { c = 1.0L; s = 0.0L; }
+    long double C, S;   // cos, sin of the full angle
+    switch (q) {
+        case 0: C = c; S = s; break;
+        case 1: C = -s; S = c; break;
+        case 2: C = -c; S = -s; break;
+        default: C = s; S = -c; break;
+    }
+    *re = (double)C;
+    *im = (double)-S;
+}
+
+// Mode matrices: compute_A (modes.F90:221-246) and derive_qg_modes (:76-161).
+// Eigenpairs of A come from the similar symmetric tridiagonal H^1/2 A H^-1/2:
+// eigenvalues by Sturm-sequence bisection, eigenvectors by inverse iteration.
+static int derive_modes(oqg_ctx* c) {
+    const int nl = c->d.nl;
+    const double* h = c->d.h;
+    const double* gp = c->d.gp;
+    double A[MAXL][MAXL] = {};
+    A[0][1] = -1.0 / (gp[0] * h[0]);
+    A[0][0] = -A[0][1];
+    for (int k = 1; k < nl - 1; k++) {
+        A[k][k - 1] = -1.0 / (gp[k - 1] * h[k]);
+        A[k][k + 1] = -1.0 / (gp[k] * h[k]);
+        A[k][k] = -A[k][k - 1] - A[k][k + 1];
+    }
+    A[nl - 1][nl - 2] = -1.0 / (gp[nl - 2] * h[nl - 1]);
+    A[nl - 1][nl - 1] = -A[nl - 1][nl - 2];
+    double dg[MAXL], of[MAXL];
+    for (int k = 0; k < nl; k++) dg[k] = A[k][k];
+    for (int k = 0; k < nl - 1; k++) of[k] = -1.0 / (gp[k] * std::sqrt(h[k] * h[k + 1]));
+    double lo = 0, hi = 0;
+    for (int k = 0; k < nl; k++) {
+        const double rad = (k > 0 ? std::fabs(of[k - 1]) : 0.0) + (k < nl - 1 ? std::fabs(of[k]) : 0.0);
+        lo = std::min(lo, dg[k] - rad);
+        hi = std::max(hi, dg[k] + rad);
+    }
+    auto count_below = [&](double x) {
+        int cnt = 0;
+        double dd = dg[0] - x;
+        if (dd < 0) cnt++;
+        for (int k = 1; k < nl; k++) {
+            if (dd == 0.0) dd = 1e-300;
+            dd = (dg[k] - x) - of[k - 1] * of[k - 1] / dd;
+            if (dd < 0) cnt++;
+        }
+        return cnt;
+    };
+    double lam[MAXL], vec[MAXL][MAXL];   // vec[m][k] eigenvector of the symmetric matrix
+    for (int i = 0; i < nl; i++) {
+        double a = lo - 1.0, b = hi + 1.0;
+        for (int it = 0; it < 200; it++) {
+            const double mid = 0.5 * (a + b);
+            if (mid <= a || mid >= b) break;
+            if (count_below(mid) > i) b = mid;
+            else a = mid;
+        }
+        lam[i] = 0.5 * (a + b);
+        // inverse iteration on (T - sigma I)
+        const double scale = std::max(std::fabs(lo), std::fabs(hi));
+        const double sigma = lam[i] + 1e-13 * scale;
+        double v[MAXL];
+        for (int k = 0; k < nl; k++) v[k] = 1.0 + 0.01 * k;
+        for (int it = 0; it < 4; it++) {
+            double M[MAXL][MAXL + 1] = {};
+            for (int k = 0; k < nl; k++) {
+                M[k][k] = dg[k] - sigma;
+                if (k > 0) M[k][k - 1] = of[k - 1];
+                if (k < nl - 1) M[k][k + 1] = of[k];
+                M[k][nl] = v[k];
+            }
+            for (int col = 0; col < nl; col++) {
+                int piv = col;
+                for (int r = col + 1; r < nl; r++)
+                    if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
+                if (piv != col)
+                    for (int t = 0; t <= nl; t++) std::swap(M[col][t], M[piv][t]);
+                if (M[col][col] == 0.0) M[col][col] = 1e-300;
+                for (int r = col + 1; r < nl; r++) {
+                    const double f = M[r][col] / M[col][col];
+                    for (int t = col; t <= nl; t++) M[r][t] -= f * M[col][t];
+                }
+            }
+            for (int r = nl - 1; r >= 0; r--) {
+                double t = M[r][nl];
+                for (int q = r + 1; q < nl; q++) t -= M[r][q] * v[q];
+                v[r] = t / M[r][r];
+            }
+            double nrm = 0.0;
+            for (int k = 0; k < nl; k++) nrm += v[k] * v[k];
+            nrm = std::sqrt(nrm);
+            for (int k = 0; k < nl; k++) v[k] /= nrm;
+        }
+        for (int k = 0; k < nl; k++) vec[i][k] = v[k];
+    }
+    // right eigenvectors of A: H^-1/2 v (unit 2-norm, first component positive); left: H^1/2 v
+    double er[MAXL][MAXL], el[MAXL][MAXL];
+    for (int i = 0; i < nl; i++) {
+        double nrm = 0.0;
+        for (int k = 0; k < nl; k++) {
+            er[i][k] = vec[i][k] / std::sqrt(h[k]);
+            el[i][k] = vec[i][k] * std::sqrt(h[k]);
+            nrm += er[i][k] * er[i][k];
+        }
+        nrm = std::sqrt(nrm);
+        const double sg = er[i][0] < 0 ? -1.0 : 1.0;
+        for (int k = 0; k < nl; k++) { er[i][k] = sg * er[i][k] / nrm; el[i][k] *= sg; }
+    }
+    // order by |lambda| ascending, stable (modes.F90:99-115)
+    int idx[MAXL];
+    double ab[MAXL];
+    for (int i = 0; i < nl; i++) { idx[i] = i; ab[i] = std::fabs(lam[i]); }
+    for (int m = 1; m < nl; m++) {
+        const int t = idx[m];
+        int i = m - 1;
+        while (i >= 0 && ab[idx[i]] > ab[t]) { idx[i + 1] = idx[i]; i--; }
+        idx[i + 1] = t;
+    }
+    std::memset(&c->mx, 0, sizeof(c->mx));
+    c->mx.rdm2[0] = 0.0;
+    for (int m = 1; m < nl; m++) c->mx.rdm2[m] = c->d.fnot * c->d.fnot * ab[idx[m]];
+    for (int m = 0; m < nl; m++) {
+        const int im = idx[m];
+        double elder = 0.0;
+        for (int k = 0; k < nl; k++) elder += el[im][k] * er[im][k];
+        for (int k = 0; k < nl; k++) {
+            c->mx.ctl2m[k * MAXL + m] = el[im][k] / elder;
+            c->mx.ctm2l[m * MAXL + k] = er[im][k];
+        }
+    }
+    for (int k = 0; k < nl; k++)
+        for (int l = 0; l < nl; l++) c->mx.amat[k * MAXL + l] = A[k][l];
+    return 0;
+}
+
+static int check_identity(oqg_ctx* c) {
+    const int nl = c->d.nl;
+    for (int l = 0; l < nl; l++)
+        for (int k = 0; k < nl; k++) {
+            double s = 0.0;
+            for (int m = 0; m < nl; m++) s += c->mx.ctm2l[m * MAXL + k] * c->mx.ctl2m[l * MAXL + m];
+            if (std::fabs(s - (l == k ? 1.0 : 0.0)) > 5.0e-15) {
+                char buf[160];
+                std::snprintf(buf, sizeof buf, "mode conversion matrices are not inverse: (%d,%d) = %.17g", k + 1, l + 1, s);
+                return fail(c, OQG_ERR_NUMERIC, buf);
+            }
+        }
+    return 0;
+}
+
+// Host trapezoid integrals (numerics.F90:188-212) on a dense (nxp,nyp) array.
+static double host_int_dA(const oqg_ctx* c, const double* f) {
+    const int nx = c->g.nxp, ny = c->g.nyp;
+    double all = 0, s1 = 0, sn = 0, w1 = 0, wn = 0;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) all += f[(size_t)j * nx + i];
+    for (int i = 0; i < nx; i++) { s1 += f[i]; sn += f[(size_t)(ny - 1) * nx + i]; }
+    for (int j = 0; j < ny; j++) { w1 += f[(size_t)j * nx]; wn += f[(size_t)j * nx + nx - 1]; }
+    double r = all;
+    r = r - 0.5 * (s1 + sn);
+    r = r - 0.5 * (w1 + wn);
+    r = r + 0.25 * (f[0] + f[(size_t)(ny - 1) * nx] + f[nx - 1] + f[(size_t)(ny - 1) * nx + nx - 1]);
+    return r * c->g.dx * c->g.dy;
+}
+
+static double host_int_dx(const oqg_ctx* c, const double* row) {
+    const int nx = c->g.nxp;
+    double s = 0.0;
+    for (int i = 1; i < nx - 1; i++) s += row[i];
+    return (0.5 * row[0] + s + 0.5 * row[nx - 1]) * c->g.dx;
+}
+
+// Partial-pivot LU of the (nl-1)^2 matrix (LU_factor, linalg.F90:24-55).
+static int host_lu(oqg_ctx* c, int n) {
+    BoxHom& b = c->bh;
+    for (int i = 0; i < MAXL * MAXL; i++) b.LU[i] = b.cdhoc[i];
+    for (int k = 0; k < n; k++) {
+        int piv = k;
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(b.LU[i * MAXL + k]) > std::fabs(b.LU[piv * MAXL + k])) piv = i;
+        b.ipiv[k] = piv;
+        if (b.LU[piv * MAXL + k] == 0.0) return fail(c, OQG_ERR_NUMERIC, "mass constraint matrix cdhoc is singular");
+        if (piv != k)
+            for (int j = 0; j < n; j++) std::swap(b.LU[k * MAXL + j], b.LU[piv * MAXL + j]);
+        for (int i = k + 1; i < n; i++) b.LU[i * MAXL + k] /= b.LU[k * MAXL + k];
+        for (int j = k + 1; j < n; j++)
+            for (int i = k + 1; i < n; i++) b.LU[i * MAXL + j] -= b.LU[i * MAXL + k] * b.LU[k * MAXL + j];
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ copies
+
+static int upload_field(oqg_ctx* c, double* dst, const double* src, int nlayers, int kind) {
+    const size_t w = (size_t)c->g.nxp * sizeof(double);
+    HIPCHK(c, hipMemcpy2DAsync(dst, (size_t)c->g.pitch * sizeof(double), src, w, w, (size_t)nlayers * c->g.nyp,
+                               kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+    return 0;
+}
+
+static int download_field(oqg_ctx* c, double* dst, const double* src, int nlayers, int kind) {
+    const size_t w = (size_t)c->g.nxp * sizeof(double);
+    HIPCHK(c, hipMemcpy2DAsync(dst, w, src, (size_t)c->g.pitch * sizeof(double), w, (size_t)nlayers * c->g.nyp,
+                               kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->st));
+    return 0;
+}
+
+// ------------------------------------------------------------------ launches
+
+static void fill_inv_common(oqg_ctx* c, InvArgs& ia) {
+    std::memset(&ia, 0, sizeof ia);
+    ia.g = c->g;
+    ia.s = c->s;
+    ia.ph = c->ph;
+    ia.mx = &c->dconst->mx;
+    ia.bh = &c->dconst->bh;
+    ia.ch = &c->dconst->ch;
+    ia.spec = c->spec;
+    ia.tw = c->tw;
+    ia.colsum = c->colsum;
+    ia.colw = c->colw;
+    ia.entpart = c->entpart;
+    ia.nentpart = c->tiles_x * c->tiles_y;
+    ia.hom = c->hom;
+    ia.aux = c->aux;
+    ia.yprel = c->yprel;
+    ia.ddyn = c->ddyn;
+    ia.raw_mode = -1;
+}
+
+// Raw Helmholtz solve of mode m (0-based) from raw[0] into raw[1].
+static int enqueue_raw_solve(oqg_ctx* c, int m) {
+    FwdArgs fa{};
+    fa.g = c->g;
+    fa.s = c->s;
+    fa.mx = c->mx;
+    fa.q = c->raw[0];
+    fa.yprel = c->yprel;
+    fa.tw = c->tw;
+    fa.spec = c->spec;
+    fa.raw_mode = m;
+    launch_fwd(fa, 1, c->st);
+    YArgs ya{};
+    ya.s = c->s;
+    ya.spec = c->spec;
+    ya.bcol = c->bcol;
+    ya.gam = c->gamw;
+    ya.mode0 = m;
+    ya.nmodes = 1;
+    launch_ysolve(ya, c->variant, c->st);
+    InvArgs ia;
+    fill_inv_common(c, ia);
+    ia.raw_mode = m;
+    ia.out_raw = c->raw[1];
+    launch_inv(ia, c->st);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+enum { KV = 0, KC, KF, KY, KI, NK };
+static const char* kNames[NK] = {"k_vort", "k_cyc", "k_fwd", "k_ysolve", "k_inv"};
+
+// One step_qg on resident forcing.  ev (optional) brackets each kernel.
+static int enqueue_step(oqg_ctx* c, double tdt, hipEvent_t* ev) {
+    const int cur = c->cur, nxt = 1 - cur;
+    int e = 0;
+    auto mark = [&]() { if (ev) hipEventRecord(ev[e++], c->st); };
+    VortArgs va{};
+    va.g = c->g;
+    va.ph = c->ph;
+    va.q = c->qbuf[cur];
+    va.p = c->pbuf[cur];
+    va.pm = c->pbuf[nxt];
+    va.qnew = c->qbuf[nxt];
+    va.ent1 = c->ent1;
+    va.wekp = c->wekp;
+    va.entpart = c->entpart;
+    va.tdt = tdt;
+    va.tiles_x = c->tiles_x;
+    va.tiles_y = c->tiles_y;
+    mark();
+    launch_vort(va, c->st);
+    mark();
+    if (c->g.cyclic) {
+        CycArgs ca{};
+        ca.g = c->g;
+        ca.ph = c->ph;
+        ca.q = c->qbuf[cur];
+        ca.p = c->pbuf[cur];
+        ca.pm = c->pbuf[nxt];
+        ca.ent = c->ent1;
+        ca.ent2s = c->has_ent2 ? c->ent2s : nullptr;
+        ca.ent2n = c->has_ent2 ? c->ent2n : nullptr;
+        ca.aux = c->aux;
+        ca.txis = c->txis;
+        ca.txin = c->txin;
+        launch_cyc(ca, c->st);
+    }
+    mark();
+    FwdArgs fa{};
+    fa.g = c->g;
+    fa.s = c->s;
+    fa.mx = c->mx;
+    fa.q = c->qbuf[nxt];
+    fa.ddyn = c->ddyn;
+    fa.yprel = c->yprel;
+    fa.tw = c->tw;
+    fa.spec = c->spec;
+    fa.raw_mode = -1;
+    launch_fwd(fa, c->g.nl, c->st);
+    mark();
+    YArgs ya{};
+    ya.s = c->s;
+    ya.spec = c->spec;
+    ya.bcol = c->bcol;
+    ya.colsum = c->g.cyclic ? nullptr : c->colsum;
+    ya.gam = c->gamw;
+    ya.mode0 = 0;
+    ya.nmodes = c->g.nl;
+    launch_ysolve(ya, c->variant, c->st);
+    mark();
+    InvArgs ia;
+    fill_inv_common(c, ia);
+    ia.cin = c->cbuf[cur];
+    ia.cout = c->cbuf[nxt];
+    ia.pnew = c->pbuf[nxt];
+    ia.qnew = c->qbuf[nxt];
+    ia.tdt = tdt;
+    launch_inv(ia, c->st);
+    mark();
+    HIPCHK(c, hipGetLastError());
+    c->cur = nxt;
+    return 0;
+}
+
+static int enqueue_suppress(oqg_ctx* c) {
+    SuppArgs sa{};
+    sa.g = c->g;
+    sa.q = c->qbuf[c->cur];
+    sa.qm = c->qbuf[1 - c->cur];
+    sa.p = c->pbuf[c->cur];
+    sa.pm = c->pbuf[1 - c->cur];
+    sa.c = c->cbuf[c->cur];
+    launch_suppress(sa, c->st);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ create
+
+static int setup(oqg_ctx* c) {
+    const oqg_desc& d = c->d;
+    const int nl = d.nl, nxp = d.nxp, nyp = d.nyp;
+    Geo& g = c->g;
+    g.nxp = nxp;
+    g.nyp = nyp;
+    g.nl = nl;
+    g.pitch = (nxp + 15) / 16 * 16;
+    g.cyclic = d.cyclic ? 1 : 0;
+    const int kt1 = d.k_topo > 0 ? d.k_topo : (d.dz_sign == 1 ? 1 : nl);
+    g.kt = kt1 - 1;
+    g.fs = (long long)g.pitch * nyp;
+    g.dx = d.dx;
+    g.dy = d.dy;
+    g.fnot = d.fnot;
+    g.beta = d.beta;
+    g.fx = 1.0 / (d.dx * d.dx);
+    g.fy = 1.0 / (d.dy * d.dy);
+    const double al = d.bcco;
+    g.cxl = g.fx * al / (0.5 * al + 1.0);
+    g.cxr = -g.fx * al / (0.5 * al + 1.0);
+    g.cyl = g.fy * al / (0.5 * al + 1.0);
+    g.cyr = -g.fy * al / (0.5 * al + 1.0);
+    g.jfac = 1.0 / (3.0 * (2.0 * d.dx) * (2.0 * d.dy) * d.fnot);
+    c->fbytes = (size_t)nl * g.fs * sizeof(double);
+
+    Phys& ph = c->ph;
+    for (int k = 0; k < nl; k++) {
+        ph.c2[k] = d.ah2[k] / d.fnot;
+        ph.c4[k] = d.ah4[k] / d.fnot;
+        ph.fohfac[k] = d.fnot / d.h[k];
+        ph.ah2[k] = d.ah2[k];
+        ph.ah4[k] = d.ah4[k];
+        ph.h[k] = d.h[k];
+    }
+    for (int k = 0; k < nl - 1; k++) ph.gp[k] = d.gp[k];
+    ph.cent1 = d.dz_sign * ph.fohfac[0];
+    ph.cwek = d.tau_sign * ph.fohfac[0];
+    ph.cent2 = d.dz_sign * ph.fohfac[1];
+    ph.cdrag = (ph.fohfac[g.kt] * d.delek / (2.0 * std::fabs(d.fnot)));
+    ph.cd_bdr = d.fnot * d.delek / (2.0 * std::fabs(d.fnot));
+    ph.qbfac = d.bcco * (1.0 / (d.dx * d.dx)) / (0.5 * d.bcco + 1.0) / d.fnot;
+    ph.dz_sign = d.dz_sign;
+    ph.tau_sign = d.tau_sign;
+
+    // spectral layout and FFT plan (init_inhomog, inhomog.F90:83-124)
+    Spec& s = c->s;
+    int Nref;
+    if (g.cyclic) {
+        s.nin = nxp - 1;
+        s.L = s.nin;
+        s.ncol = 2 * (s.nin / 2 + 1);
+        Nref = s.nin;
+    } else {
+        s.nin = nxp - 2;
+        s.L = 2 * (s.nin + 1);
+        s.ncol = s.nin;
+        Nref = s.L;
+    }
+    if (s.L > MAX_FFT) return fail(c, OQG_ERR_ARG, "x transform longer than 4096 points");
+    {
+        const size_t inv_lds = sizeof(double2) * 2 * (size_t)s.L + sizeof(double) * 2 * (size_t)nl * g.pitch;
+        if (inv_lds > 150 * 1024) return fail(c, OQG_ERR_ARG, "grid too wide for the LDS-staged x transform");
+    }
+    s.nyi = nyp - 2;
+    s.R = (s.nyi + WAVE - 1) / WAVE;
+    if (s.R > MAX_YROWS_PER_LANE) return fail(c, OQG_ERR_ARG, "nyp too large for the y solver (nyp-2 > 1024)");
+    s.nypad = WAVE * s.R;
+    s.a = 1.0 / (d.dy * d.dy);
+    s.ftnorm = 1.0 / Nref;
+    {
+        int m = s.L;
+        s.nstages = 0;
+        while (m % 4 == 0) { s.radix[s.nstages++] = 4; m /= 4; }
+        while (m % 2 == 0) { s.radix[s.nstages++] = 2; m /= 2; }
+        while (m % 3 == 0) { s.radix[s.nstages++] = 3; m /= 3; }
+        while (m % 5 == 0) { s.radix[s.nstages++] = 5; m /= 5; }
+        if (m != 1) return fail(c, OQG_ERR_ARG, "x transform length must factor into 2, 3 and 5");
+    }
+
+    // modes
+    if (d.ctl2m && d.ctm2l && d.rdm2 && d.amat) {
+        std::memset(&c->mx, 0, sizeof(c->mx));
+        for (int k = 0; k < nl; k++)
+            for (int m = 0; m < nl; m++) {
+                c->mx.ctl2m[k * MAXL + m] = d.ctl2m[m * nl + k];   // Fortran (k,m)
+                c->mx.ctm2l[m * MAXL + k] = d.ctm2l[k * nl + m];   // Fortran (m,k)
+                c->mx.amat[k * MAXL + m] = d.amat[m * nl + k];
+            }
+        for (int m = 0; m < nl; m++) c->mx.rdm2[m] = d.rdm2[m];
+    } else {
+        int rc = derive_modes(c);
+        if (rc) return rc;
+    }
+    if (int rc = check_identity(c)) return rc;
+
+    // device buffers
+    const size_t fl = (size_t)nl * g.fs, f1 = (size_t)g.fs;
+    const size_t nspec = (size_t)nl * s.ncol * s.nypad;
+    c->tiles_x = (nxp + VORT_TX - 1) / VORT_TX;
+    c->tiles_y = (nyp + VORT_TY - 1) / VORT_TY;
+    int rc = 0;
+    for (int t = 0; t < 2 && !rc; t++) {
+        rc |= dalloc(c, &c->qbuf[t], fl);
+        rc |= dalloc(c, &c->pbuf[t], fl);
+        rc |= dalloc(c, &c->cbuf[t], C_LEN);
+        rc |= dalloc(c, &c->raw[t], f1);
+    }
+    if (!rc) rc = dalloc(c, &c->spec, nspec);
+    if (!rc && c->variant == 1) rc = dalloc(c, &c->gamw, nspec);
+    if (!rc) rc = dalloc(c, &c->colsum, (size_t)nl * s.ncol);
+    if (!rc) rc = dalloc(c, &c->colw, (size_t)s.ncol);
+    if (!rc) rc = dalloc(c, &c->entpart, (size_t)c->tiles_x * c->tiles_y);
+    if (!rc) rc = dalloc(c, &c->aux, A_LEN);
+    if (!rc) rc = dalloc(c, &c->ent1, f1);
+    if (!rc) rc = dalloc(c, &c->ent2s, (size_t)nxp);
+    if (!rc) rc = dalloc(c, &c->ent2n, (size_t)nxp);
+    if (!rc) rc = dalloc(c, &c->wekp, f1);
+    if (!rc) rc = dalloc(c, &c->yprel, (size_t)nyp);
+    if (!rc) rc = dalloc(c, &c->bcol, (size_t)nl * s.ncol);
+    if (!rc) rc = dalloc(c, &c->tw, 2 * (size_t)s.L);
+    if (!rc && d.ddyn) rc = dalloc(c, &c->ddyn, f1);
+    if (!rc) rc = dalloc(c, &c->hom, g.cyclic ? (size_t)nyp * (1 + 2 * (nl - 1)) : (size_t)(nl - 1) * f1);
+    if (!rc) rc = dalloc(c, &c->dconst, 1);
+    if (rc) return rc;
+    auto upload_consts = [&]() -> int {
+        DevConsts h;
+        h.mx = c->mx;
+        h.bh = c->bh;
+        h.ch = c->ch;
+        HIPCHK(c, hipMemcpyAsync(c->dconst, &h, sizeof h, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        return 0;
+    };
+    if (int r = upload_consts()) return r;
+
+    // constants
+    const double xl = (nxp - 1) * d.dx, yl = (nyp - 1) * d.dy;   // box.F90:135-136
+    c->yprel_h.resize(nyp);
+    for (int j = 0; j < nyp; j++) c->yprel_h[j] = -0.5 * yl + j * d.dy;
+    HIPCHK(c, hipMemcpyAsync(c->yprel, c->yprel_h.data(), nyp * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (d.ddyn) {
+        c->ddyn_h.assign(d.ddyn, d.ddyn + (size_t)nxp * nyp);
+        if (int r = upload_field(c, c->ddyn, d.ddyn, 1, OQG_HOST)) return r;
+    }
+    std::vector<double> tw(2 * (size_t)s.L);
+    for (int t = 0; t < s.L; t++) twiddle_ld(t, s.L, &tw[2 * t], &tw[2 * t + 1]);
+    HIPCHK(c, hipMemcpyAsync(c->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    // tridiagonal diagonals bb(k,m) (inhomog.F90:127-141)
+    std::vector<double> bcol((size_t)nl * s.ncol);
+    const double dxm2 = 1.0 / (d.dx * d.dx);
+    for (int col = 0; col < s.ncol; col++) {
+        double arg;
+        if (g.cyclic) arg = ((double)(col / 2) * TWOPI_REF) / (double)Nref;
+        else arg = ((double)(col + 1) * TWOPI_REF) / (double)Nref;
+        const double bd2 = -2.0 * s.a + 2.0 * dxm2 * (std::cos(arg) - 1.0);
+        for (int m = 0; m < nl; m++) bcol[(size_t)m * s.ncol + col] = bd2 - c->mx.rdm2[m];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->bcol, bcol.data(), bcol.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (!g.cyclic) {
+        // sum over interior i of the inverse DST basis: 2*ftnorm*cot(pi K / (2N)) for odd K
+        std::vector<double> colw(s.ncol);
+        const long double N = (long double)(nxp - 1);
+        for (int K = 1; K <= s.ncol; K++) {
+            long double v = 0.0L;
+            if (K % 2 == 1) {
+                const long double th = 3.14159265358979323846264338327950288L * K / (2.0L * N);
+                v = 2.0L * (long double)s.ftnorm * cosl(th) / sinl(th);
+            }
+            colw[K - 1] = (double)v;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->colw, colw.data(), colw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    }
+    set_kernel_attrs(sizeof(double2) * 2 * (size_t)s.L,
+                     sizeof(double2) * 2 * (size_t)s.L + sizeof(double) * 2 * (size_t)nl * g.pitch);
+
+    // homogeneous solutions (homsol, homog.F90:108-278) from this library's own solver
+    const size_t nxy = (size_t)nxp * nyp;
+    std::vector<double> L1(nxy), sol(nxy);
+    if (!g.cyclic) {
+        std::vector<double> homh((size_t)(nl - 1) * nxy);
+        double aipohs[MAXL] = {};
+        if (d.hom_sol_bc) {
+            std::memcpy(homh.data(), d.hom_sol_bc, homh.size() * sizeof(double));
+        } else {
+            std::fill(L1.begin(), L1.end(), 1.0);
+            if (int r = upload_field(c, c->raw[0], L1.data(), 1, OQG_HOST)) return r;
+            for (int m = 1; m < nl; m++) {
+                if (int r = enqueue_raw_solve(c, m)) return r;
+                if (int r = download_field(c, sol.data(), c->raw[1], 1, OQG_HOST)) return r;
+                HIPCHK(c, hipStreamSynchronize(c->st));
+                double* hm = homh.data() + (size_t)(m - 1) * nxy;
+                for (size_t t = 0; t < nxy; t++) hm[t] = L1[t] + c->mx.rdm2[m] * sol[t];
+            }
+        }
+        for (int m = 1; m < nl; m++) aipohs[m] = host_int_dA(c, homh.data() + (size_t)(m - 1) * nxy);
+        std::memset(&c->bh, 0, sizeof(c->bh));
+        for (int k = 0; k < nl - 1; k++)
+            for (int m = 1; m < nl; m++)
+                c->bh.cdhoc[k * MAXL + (m - 1)] =
+                    (c->mx.ctm2l[m * MAXL + k + 1] - c->mx.ctm2l[m * MAXL + k]) * aipohs[m];
+        if (int r = host_lu(c, nl - 1)) return r;
+        if (int r = upload_field(c, c->hom, homh.data(), nl - 1, OQG_HOST)) return r;
+    } else {
+        std::vector<double> hb((size_t)nyp * (1 + 2 * (nl - 1)));
+        double* bt = hb.data();
+        double* b1 = bt + nyp;
+        double* b2 = b1 + (size_t)(nl - 1) * nyp;
+        std::memset(&c->ch, 0, sizeof(c->ch));
+        if (d.hom_bt && d.hom_bc1 && d.hom_bc2 && d.hom_coef) {
+            std::memcpy(bt, d.hom_bt, nyp * sizeof(double));
+            std::memcpy(b1, d.hom_bc1, (size_t)(nl - 1) * nyp * sizeof(double));
+            std::memcpy(b2, d.hom_bc2, (size_t)(nl - 1) * nyp * sizeof(double));
+            for (int m = 1; m < nl; m++) {
+                c->ch.hc1s[m] = d.hom_coef[4 * (m - 1) + 0];
+                c->ch.hc2s[m] = d.hom_coef[4 * (m - 1) + 1];
+                c->ch.hc1n[m] = d.hom_coef[4 * (m - 1) + 2];
+                c->ch.hc2n[m] = d.hom_coef[4 * (m - 1) + 3];
+            }
+            c->ch.hbsi = d.hom_coef[4 * (nl - 1)];
+        } else {
+            for (int j = 0; j < nyp; j++) bt[j] = (double)(nyp - 1 - j) / (double)(nyp - 1);
+            c->ch.hbsi = yl / xl;
+            std::vector<double> r2(nxy);
+            const double ypn = (nyp - 1) * d.dy, yp1 = 0.0;
+            for (int j = 0; j < nyp; j++)
+                for (int i = 0; i < nxp; i++) {
+                    const double ypj = j * d.dy;
+                    L1[(size_t)j * nxp + i] = (ypn - ypj) / yl;
+                    r2[(size_t)j * nxp + i] = (ypj - yp1) / yl;
+                }
+            for (int m = 1; m < nl; m++) {
+                for (int which = 0; which < 2; which++) {
+                    const std::vector<double>& rr = which ? r2 : L1;
+                    if (int r = upload_field(c, c->raw[0], rr.data(), 1, OQG_HOST)) return r;
+                    if (int r = enqueue_raw_solve(c, m)) return r;
+                    if (int r = download_field(c, sol.data(), c->raw[1], 1, OQG_HOST)) return r;
+                    HIPCHK(c, hipStreamSynchronize(c->st));
+                    double* dst = (which ? b2 : b1) + (size_t)(m - 1) * nyp;
+                    for (int j = 0; j < nyp; j++)
+                        dst[j] = rr[(size_t)j * nxp] + c->mx.rdm2[m] * sol[(size_t)j * nxp];
+                }
+            }
+            const double dy = d.dy;
+            for (int m = 1; m < nl; m++) {
+                const double* h1 = b1 + (size_t)(m - 1) * nyp;
+                const double* h2 = b2 + (size_t)(m - 1) * nyp;
+                double p1s = -xl * (h1[1] - h1[0]) / dy;
+                double p2s = -xl * (h2[1] - h2[0]) / dy;
+                double p1n = xl * (h1[nyp - 1] - h1[nyp - 2]) / dy;
+                double p2n = xl * (h2[nyp - 1] - h2[nyp - 2]) / dy;
+                p1s = p1s + 0.5 * dy * c->mx.rdm2[m] * h1[0] * xl;
+                p2s = p2s + 0.5 * dy * c->mx.rdm2[m] * h2[0] * xl;
+                p1n = p1n + 0.5 * dy * c->mx.rdm2[m] * h1[nyp - 1] * xl;
+                p2n = p2n + 0.5 * dy * c->mx.rdm2[m] * h2[nyp - 1] * xl;
+                const double det = p1s * p2n - p2s * p1n;
+                c->ch.hc1s[m] = p1s / det;
+                c->ch.hc2s[m] = p2s / det;
+                c->ch.hc1n[m] = p1n / det;
+                c->ch.hc2n[m] = p2n / det;
+            }
+        }
+        c->ch.c0 = d.dx * (s.ftnorm * (double)Nref);
+        HIPCHK(c, hipMemcpyAsync(c->hom, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    }
+    return upload_consts();
+}
+
+// ------------------------------------------------------------------ ABI
+
+extern "C" {
+
+int oqg_create(const oqg_desc* desc, oqg_ctx** out) {
+    g_create_err.clear();
+    if (!out) return fail(nullptr, OQG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!desc) return fail(nullptr, OQG_ERR_ARG, "desc is NULL");
+    const oqg_desc& d = *desc;
+    if (d.nl < 2 || d.nl > OQG_MAX_LAYERS) return fail(nullptr, OQG_ERR_ARG, "nl must be in 2..8");
+    if (d.nxp < 5 || d.nyp < 8) return fail(nullptr, OQG_ERR_ARG, "grid too small (nxp >= 5, nyp >= 8)");
+    if (!(d.dx > 0) || !(d.dy > 0) || d.fnot == 0.0) return fail(nullptr, OQG_ERR_ARG, "dx, dy must be > 0 and fnot != 0");
+    for (int k = 0; k < d.nl; k++)
+        if (!(d.h[k] > 0)) return fail(nullptr, OQG_ERR_ARG, "layer thicknesses must be > 0");
+    for (int k = 0; k < d.nl - 1; k++)
+        if (!(d.gp[k] > 0)) return fail(nullptr, OQG_ERR_ARG, "reduced gravities must be > 0");
+    if (d.k_topo < 0 || d.k_topo > d.nl) return fail(nullptr, OQG_ERR_ARG, "k_topo out of range");
+    if (d.cyclic && d.nxp - 1 > 1024) return fail(nullptr, OQG_ERR_ARG, "cyclic nxp-1 must be <= 1024");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, OQG_ERR_HIP, "no HIP device available");
+    auto c = std::make_unique<oqg_ctx>();
+    c->d = d;
+    c->d.ddyn = nullptr;   // borrowed pointers are not kept
+    c->d.amat = c->d.ctl2m = c->d.ctm2l = c->d.rdm2 = nullptr;
+    c->d.hom_sol_bc = c->d.hom_bt = c->d.hom_bc1 = c->d.hom_bc2 = c->d.hom_coef = nullptr;
+    c->variant = (d.flags & OQG_FLAG_YSOLVE_THOMAS) ? 1 : 0;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess)
+        return fail(nullptr, OQG_ERR_HIP, "hipStreamCreate failed");
+    c->st = c->own;
+    // setup() may read the caller's optional arrays through a temporary copy of desc
+    oqg_desc keep = c->d;
+    c->d = d;
+    int rc = setup(c.get());
+    c->d = keep;
+    if (rc) {
+        g_create_err = c->err;
+        oqg_destroy(c.release());
+        return rc;
+    }
+    *out = c.release();
+    return 0;
+}
+
+void oqg_destroy(oqg_ctx* c) {
+    if (!c) return;
+    if (c->st) hipStreamSynchronize(c->st);
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->own) hipStreamDestroy(c->own);
+    delete c;
+}
+
+const char* oqg_last_error(const oqg_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int oqg_set_state(oqg_ctx* c, const double* q, const double* qm, const double* p, const double* pm, int kind) {
+    if (!c) return OQG_ERR_ARG;
+    if (c->g.cyclic && kind == OQG_HOST) {
+        const double* fs[4] = {q, qm, p, pm};
+        const int nx = c->g.nxp;
+        for (int f = 0; f < 4; f++) {
+            if (!fs[f]) continue;
+            for (size_t r = 0; r < (size_t)c->g.nl * c->g.nyp; r++)
+                if (fs[f][r * nx] != fs[f][r * nx + nx - 1])
+                    return fail(c, OQG_ERR_ARG, "cyclic field: column nxp differs from column 1");
+        }
+    }
+    const int cur = c->cur;
+    if (q) if (int r = upload_field(c, c->qbuf[cur], q, c->g.nl, kind)) return r;
+    if (qm) if (int r = upload_field(c, c->qbuf[1 - cur], qm, c->g.nl, kind)) return r;
+    if (p) if (int r = upload_field(c, c->pbuf[cur], p, c->g.nl, kind)) return r;
+    if (pm) if (int r = upload_field(c, c->pbuf[1 - cur], pm, c->g.nl, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_get_state(oqg_ctx* c, double* q, double* qm, double* p, double* pm, int kind) {
+    if (!c) return OQG_ERR_ARG;
+    const int cur = c->cur;
+    if (q) if (int r = download_field(c, q, c->qbuf[cur], c->g.nl, kind)) return r;
+    if (qm) if (int r = download_field(c, qm, c->qbuf[1 - cur], c->g.nl, kind)) return r;
+    if (p) if (int r = download_field(c, p, c->pbuf[cur], c->g.nl, kind)) return r;
+    if (pm) if (int r = download_field(c, pm, c->pbuf[1 - cur], c->g.nl, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_get_layer(oqg_ctx* c, int which, int k, double* out, int kind) {
+    if (!c || !out) return OQG_ERR_ARG;
+    if (k < 1 || k > c->g.nl) return fail(c, OQG_ERR_ARG, "layer index out of range");
+    const double* src;
+    switch (which) {
+        case OQG_FIELD_Q: src = c->qbuf[c->cur]; break;
+        case OQG_FIELD_QM: src = c->qbuf[1 - c->cur]; break;
+        case OQG_FIELD_P: src = c->pbuf[c->cur]; break;
+        case OQG_FIELD_PM: src = c->pbuf[1 - c->cur]; break;
+        default: return fail(c, OQG_ERR_ARG, "unknown field selector");
+    }
+    if (int r = download_field(c, out, src + (size_t)(k - 1) * c->g.fs, 1, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_constr_len(const oqg_ctx* c) { return c ? 2 * (c->g.nl - 1) + 4 * c->g.nl : 0; }
+
+int oqg_get_constr(oqg_ctx* c, double* v) {
+    if (!c || !v) return OQG_ERR_ARG;
+    double h[C_LEN];
+    HIPCHK(c, hipMemcpyAsync(h, c->cbuf[c->cur], sizeof h, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const int nl = c->g.nl;
+    int t = 0;
+    for (int k = 0; k < nl - 1; k++) v[t++] = h[C_DPI + k];
+    for (int k = 0; k < nl - 1; k++) v[t++] = h[C_DPIP + k];
+    for (int k = 0; k < nl; k++) v[t++] = h[C_CS + k];
+    for (int k = 0; k < nl; k++) v[t++] = h[C_CN + k];
+    for (int k = 0; k < nl; k++) v[t++] = h[C_CSP + k];
+    for (int k = 0; k < nl; k++) v[t++] = h[C_CNP + k];
+    return 0;
+}
+
+int oqg_set_constr(oqg_ctx* c, const double* v) {
+    if (!c || !v) return OQG_ERR_ARG;
+    double h[C_LEN] = {};
+    const int nl = c->g.nl;
+    int t = 0;
+    for (int k = 0; k < nl - 1; k++) h[C_DPI + k] = v[t++];
+    for (int k = 0; k < nl - 1; k++) h[C_DPIP + k] = v[t++];
+    for (int k = 0; k < nl; k++) h[C_CS + k] = v[t++];
+    for (int k = 0; k < nl; k++) h[C_CN + k] = v[t++];
+    for (int k = 0; k < nl; k++) h[C_CSP + k] = v[t++];
+    for (int k = 0; k < nl; k++) h[C_CNP + k] = v[t++];
+    HIPCHK(c, hipMemcpyAsync(c->cbuf[c->cur], h, sizeof h, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_init_state(oqg_ctx* c) {
+    if (!c) return OQG_ERR_ARG;
+    const int cur = c->cur;
+    for (int t = 0; t < 2; t++) {
+        QcompArgs qa{};
+        qa.g = c->g;
+        qa.mx = c->mx;
+        qa.p = t == 0 ? c->pbuf[cur] : c->pbuf[1 - cur];
+        qa.ddyn = c->ddyn;
+        qa.yprel = c->yprel;
+        qa.q = t == 0 ? c->qbuf[cur] : c->qbuf[1 - cur];
+        launch_qcomp(qa, c->st);
+    }
+    HIPCHK(c, hipGetLastError());
+    // constraint integrals on the host (init only)
+    const int nl = c->g.nl, nx = c->g.nxp, ny = c->g.nyp;
+    const size_t nxy = (size_t)nx * ny;
+    std::vector<double> p(nxy * nl), pm(nxy * nl);
+    if (int r = download_field(c, p.data(), c->pbuf[cur], nl, OQG_HOST)) return r;
+    if (int r = download_field(c, pm.data(), c->pbuf[1 - cur], nl, OQG_HOST)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double h[C_LEN] = {};
+    if (c->g.cyclic) {
+        // init_constraint (constraint.F90:114-141)
+        double pins[MAXL], pinn[MAXL], pinsp[MAXL], pinnp[MAXL];
+        std::vector<double> row(nx);
+        const double dy = c->g.dy, f0 = c->g.fnot;
+        for (int k = 0; k < nl; k++) {
+            const double* P = p.data() + k * nxy;
+            const double* PM = pm.data() + k * nxy;
+            pinsp[k] = host_int_dx(c, PM);
+            pins[k] = host_int_dx(c, P);
+            pinnp[k] = host_int_dx(c, PM + (size_t)(ny - 1) * nx);
+            pinn[k] = host_int_dx(c, P + (size_t)(ny - 1) * nx);
+            for (int i = 0; i < nx; i++) row[i] = PM[nx + i] - PM[i];
+            h[C_CSP + k] = host_int_dx(c, row.data()) / dy;
+            for (int i = 0; i < nx; i++) row[i] = PM[(size_t)(ny - 1) * nx + i] - PM[(size_t)(ny - 2) * nx + i];
+            h[C_CNP + k] = host_int_dx(c, row.data()) / dy;
+            for (int i = 0; i < nx; i++) row[i] = P[nx + i] - P[i];
+            h[C_CS + k] = host_int_dx(c, row.data()) / dy;
+            for (int i = 0; i < nx; i++) row[i] = P[(size_t)(ny - 1) * nx + i] - P[(size_t)(ny - 2) * nx + i];
+            h[C_CN + k] = host_int_dx(c, row.data()) / dy;
+        }
+        for (int k = 0; k < nl; k++) {
+            double apsp = 0, apnp = 0, aps = 0, apn = 0;
+            for (int l = 0; l < nl; l++) {
+                const double a = c->mx.amat[k * MAXL + l];
+                apsp += a * pinsp[l];
+                apnp += a * pinnp[l];
+                aps += a * pins[l];
+                apn += a * pinn[l];
+            }
+            const double cf = 0.5 * dy * f0 * f0;
+            h[C_CSP + k] = -h[C_CSP + k] + cf * apsp;
+            h[C_CNP + k] = h[C_CNP + k] + cf * apnp;
+            h[C_CS + k] = -h[C_CS + k] + cf * aps;
+            h[C_CN + k] = h[C_CN + k] + cf * apn;
+        }
+    }
+    // init_mass_constr (constraint.F90:157-163)
+    for (int k = 0; k < nl - 1; k++) {
+        const double ipm0 = host_int_dA(c, pm.data() + k * nxy), ipm1 = host_int_dA(c, pm.data() + (k + 1) * nxy);
+        const double ip0 = host_int_dA(c, p.data() + k * nxy), ip1 = host_int_dA(c, p.data() + (k + 1) * nxy);
+        h[C_DPIP + k] = c->d.dz_sign * (ipm0 - ipm1);
+        h[C_DPI + k] = c->d.dz_sign * (ip0 - ip1);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->cbuf[cur], h, sizeof h, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_set_forcing(oqg_ctx* c, const double* ent, int n_ent, const double* wekp, double txis, double txin, int kind) {
+    if (!c) return OQG_ERR_ARG;
+    const hipMemcpyKind mk = kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const int nx = c->g.nxp, ny = c->g.nyp;
+    if (ent) {
+        if (n_ent < 1) return fail(c, OQG_ERR_ARG, "n_ent must be >= 1");
+        if (int r = upload_field(c, c->ent1, ent, 1, kind)) return r;
+        c->has_ent2 = (n_ent >= 2);
+        if (c->has_ent2) {
+            const double* e2 = ent + (size_t)nx * ny;
+            HIPCHK(c, hipMemcpyAsync(c->ent2s, e2, nx * sizeof(double), mk, c->st));
+            HIPCHK(c, hipMemcpyAsync(c->ent2n, e2 + (size_t)(ny - 1) * nx, nx * sizeof(double), mk, c->st));
+        }
+    }
+    if (wekp)
+        if (int r = upload_field(c, c->wekp, wekp, 1, kind)) return r;
+    c->txis = txis;
+    c->txin = txin;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_step(oqg_ctx* c, double tdt, const double* ent, int n_ent, const double* wekp, double txis, double txin) {
+    if (!c) return OQG_ERR_ARG;
+    if ((ent == nullptr) != (wekp == nullptr))
+        return fail(c, OQG_ERR_ARG, "pass both ent and wekp, or neither to use the resident forcing");
+    if (ent)
+        if (int r = oqg_set_forcing(c, ent, n_ent, wekp, txis, txin, OQG_HOST)) return r;
+    return enqueue_step(c, tdt, nullptr);
+}
+
+int oqg_suppress(oqg_ctx* c) {
+    if (!c) return OQG_ERR_ARG;
+    return enqueue_suppress(c);
+}
+
+int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
+    if (!c || nsteps < 0) return OQG_ERR_ARG;
+    const int P = supp > 0 ? (supp % 2 == 0 ? supp : 2 * supp) : 2;
+    int nt = step0 + 1;
+    const int end = step0 + nsteps;
+    while (nt <= end) {
+        if ((nt - 1) % P == 0 && end - nt + 1 >= P) {
+            GraphKey key{tdt, P, supp, c->cur};
+            auto it = c->graphs.find(key);
+            if (it == c->graphs.end()) {
+                const int cur0 = c->cur;
+                hipGraph_t gr;
+                HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                int rc = 0;
+                for (int t = 0; t < P && !rc; t++) {
+                    rc = enqueue_step(c, tdt, nullptr);
+                    if (!rc && supp > 0 && (t % supp) == 0) rc = enqueue_suppress(c);
+                }
+                hipError_t ce = hipStreamEndCapture(c->st, &gr);
+                if (rc) return rc;
+                if (ce != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+                hipGraphExec_t ex;
+                hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                if (ie != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+                c->cur = cur0;   // capture advanced the parity; the launch below does it for real
+                it = c->graphs.emplace(key, ex).first;
+            }
+            HIPCHK(c, hipGraphLaunch(it->second, c->st));
+            // P is even, so the time-level parity is unchanged
+            nt += P;
+        } else {
+            if (int r = enqueue_step(c, tdt, nullptr)) return r;
+            if (supp > 0 && (nt - 1) % supp == 0)
+                if (int r = enqueue_suppress(c)) return r;
+            nt++;
+        }
+    }
+    return 0;
+}
+
+int oqg_sync(oqg_ctx* c) {
+    if (!c) return OQG_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_solve_inhomog(oqg_ctx* c, int m, const double* rhs, double* out, int kind) {
+    if (!c || !rhs || !out) return OQG_ERR_ARG;
+    if (m < 1 || m > c->g.nl) return fail(c, OQG_ERR_ARG, "mode index out of range");
+    if (int r = upload_field(c, c->raw[0], rhs, 1, kind)) return r;
+    if (int r = enqueue_raw_solve(c, m - 1)) return r;
+    if (int r = download_field(c, out, c->raw[1], 1, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_get_modes(oqg_ctx* c, double* amat, double* ctl2m, double* ctm2l, double* rdm2) {
+    if (!c) return OQG_ERR_ARG;
+    const int nl = c->g.nl;
+    for (int i = 0; i < nl; i++)
+        for (int j = 0; j < nl; j++) {
+            // Fortran column-major (i,j) at [j*nl + i]
+            if (amat) amat[j * nl + i] = c->mx.amat[i * MAXL + j];
+            if (ctl2m) ctl2m[j * nl + i] = c->mx.ctl2m[i * MAXL + j];   // ctl2m(k=i, m=j)
+            if (ctm2l) ctm2l[j * nl + i] = c->mx.ctm2l[i * MAXL + j];   // ctm2l(m=i, k=j)
+        }
+    if (rdm2)
+        for (int m = 0; m < nl; m++) rdm2[m] = c->mx.rdm2[m];
+    return 0;
+}
+
+int oqg_set_stream(oqg_ctx* c, void* stream) {
+    if (!c) return OQG_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->st = stream ? static_cast<hipStream_t>(stream) : c->own;
+    return 0;
+}
+
+void* oqg_get_stream(oqg_ctx* c) { return c ? static_cast<void*>(c->st) : nullptr; }
+
+int oqg_state_ptrs(oqg_ctx* c, double** q, double** qm, double** p, double** pm, size_t* pitch) {
+    if (!c) return OQG_ERR_ARG;
+    if (q) *q = c->qbuf[c->cur];
+    if (qm) *qm = c->qbuf[1 - c->cur];
+    if (p) *p = c->pbuf[c->cur];
+    if (pm) *pm = c->pbuf[1 - c->cur];
+    if (pitch) *pitch = (size_t)c->g.pitch;
+    return 0;
+}
+
+static int time_kernels(oqg_ctx* c, double tdt, int nsteps, hipEvent_t* ev, double* acc) {
+    for (int s = 0; s < nsteps; s++) {
+        if (int r = enqueue_step(c, tdt, ev)) return r;
+        HIPCHK(c, hipEventSynchronize(ev[NK]));
+        for (int k = 0; k < NK; k++) {
+            float t = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            acc[k] += t;
+        }
+    }
+    return 0;
+}
+
+int oqg_kernel_times(oqg_ctx* c, double tdt, int nsteps, double* ms, int max, char* names, int names_len) {
+    if (!c || nsteps < 1 || !ms) return -OQG_ERR_ARG;
+    hipEvent_t ev[NK + 1];
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return -fail(c, OQG_ERR_HIP, "hipEventCreate failed");
+    double acc[NK] = {};
+    const int rc = time_kernels(c, tdt, nsteps, ev, acc);
+    for (auto& e : ev) hipEventDestroy(e);
+    if (rc) return -rc;
+    const int n = std::min(max, (int)NK);
+    std::string nm;
+    for (int k = 0; k < n; k++) {
+        ms[k] = acc[k] / nsteps;
+        nm += kNames[k];
+        nm += '\n';
+    }
+    if (names && names_len > 0) {
+        std::strncpy(names, nm.c_str(), names_len - 1);
+        names[names_len - 1] = '\0';
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// oqg_internal.h -- shared definitions of the HIP QG step library (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "../../include/oqg.h"
+
+namespace oqg {
+
+constexpr int MAXL = OQG_MAX_LAYERS;
+constexpr int WAVE = 64;
+constexpr int MAX_RADIX_STAGES = 16;
+constexpr int MAX_FFT = 4096;        // longest x transform (complex points)
+constexpr int VORT_TX = 64;          // vorticity tile (x)
+constexpr int VORT_TY = 16;          // vorticity tile (y)
+constexpr int MAX_YROWS_PER_LANE = 16;  // y-solver: nyp-2 <= 64*16
+
+// Geometry and physics shared by every kernel (by value in kernel arguments).
+struct Geo {
+    int nxp, nyp, nl, pitch;
+    int cyclic, kt;          // kt: 0-based k_topo
+    long long fs;            // layer stride in doubles = pitch*nyp
+    double dx, dy, fnot, beta;
+    double fx, fy;           // 1/dx^2, 1/dy^2 as in dP2dx2_bc / dP2dy2_bc
+    double cxl, cxr, cyl, cyr;   // mixed-BC coefficients (numerics.F90:439-440,461-462)
+    double jfac;             // Arakawa factor (numerics.F90:361)
+};
+
+// Spectral (x-transformed) layout: spec[m][col][j], j fastest with pitch nypad.
+struct Spec {
+    int nin;        // transform input points per row (box nxp-2, cyclic nxp-1)
+    int L;          // complex FFT length (box 2(nxp-1), cyclic nxp-1)
+    int ncol;       // real tridiagonal columns per mode
+    int nyi;        // interior rows nyp-2
+    int R;          // rows per lane in the wavefront y-solver
+    int nypad;      // 64*R
+    int nstages;
+    int radix[MAX_RADIX_STAGES];
+    double a;       // 1/dy^2 (inhomog.F90:127)
+    double ftnorm;  // 1/N (inhomog.F90:124)
+};
+
+struct Mix {
+    double ctl2m[MAXL * MAXL];   // [k*MAXL + m]
+    double ctm2l[MAXL * MAXL];   // [m*MAXL + k]
+    double amat[MAXL * MAXL];    // [k*MAXL + l]
+    double rdm2[MAXL];
+};
+
+// Per-layer forcing and friction coefficients (vorticity.F90:47-74).
+struct Phys {
+    double c2[MAXL], c4[MAXL];   // ah2/f0, ah4/f0
+    double fohfac[MAXL];         // f0/h
+    double cent1, cwek, cent2, cdrag;
+    double ah2[MAXL], ah4[MAXL];
+    double h[MAXL], gp[MAXL];
+    double cd_bdr;               // f0*delek/(2|f0|)  (vorticity.F90:134)
+    double qbfac;                // bcco*dxm2/(0.5*bcco+1)/f0 (vorsubs.F90:76)
+    int dz_sign, tau_sign;
+};
+
+// Box mass-constraint precompute (homog.F90:214-278).
+struct BoxHom {
+    double cdhoc[MAXL * MAXL];   // [(k-1)*MAXL + (m-1)]  (nl-1)^2
+    double LU[MAXL * MAXL];
+    int ipiv[MAXL];
+};
+
+// Cyclic precompute (homog.F90:128-212).
+struct CycHom {
+    double hc1s[MAXL], hc2s[MAXL], hc1n[MAXL], hc2n[MAXL];
+    double hbsi;
+    double c0;      // int_P_dx of a transformed row per unit U_0: dx * ftnorm * N
+};
+
+// Constraint state block (one per time-level parity).
+//   [0, MAXL)  dpi   [MAXL, 2MAXL) dpip  then cs, cn, csp, cnp
+constexpr int C_DPI = 0, C_DPIP = MAXL, C_CS = 2 * MAXL, C_CN = 3 * MAXL, C_CSP = 4 * MAXL,
+              C_CNP = 5 * MAXL, C_LEN = 6 * MAXL;
+// Cyclic per-step boundary integrals written by the constraint kernel.
+//   ajis, ajin, ap3s, ap3n, ap5s, ap5n, int_Be_s, int_Be_n (per layer), then
+//   bdrins, bdrinn, txis, txin
+constexpr int A_AJIS = 0, A_AJIN = MAXL, A_AP3S = 2 * MAXL, A_AP3N = 3 * MAXL, A_AP5S = 4 * MAXL,
+              A_AP5N = 5 * MAXL, A_BES = 6 * MAXL, A_BEN = 7 * MAXL, A_BDRS = 8 * MAXL,
+              A_BDRN = 8 * MAXL + 1, A_TXIS = 8 * MAXL + 2, A_TXIN = 8 * MAXL + 3, A_LEN = 8 * MAXL + 4;
+
+// Arguments of the vorticity kernel.
+struct VortArgs {
+    Geo g;
+    Phys ph;
+    const double* q;      // current q
+    const double* p;      // current p
+    const double* pm;     // previous p
+    double* qnew;         // in: qm, out: q_new (same buffer)
+    const double* ent1;   // (pitch layout) layer-1 entrainment
+    const double* wekp;
+    double* entpart;      // per-tile partial integrals of ent1 (layer-0 tiles)
+    double tdt;
+    int tiles_x, tiles_y;
+};
+
+struct FwdArgs {
+    Geo g;
+    Spec s;
+    Mix mx;
+    const double* q;      // q_new (step) or rhs field (raw)
+    const double* ddyn;   // may be null
+    const double* yprel;
+    const double* tw;     // twiddles exp(-2 pi i t / L), interleaved
+    double* spec;
+    int raw_mode;         // >=0: raw single field solved for this mode (0-based)
+};
+
+struct YArgs {
+    Spec s;
+    double* spec;
+    const double* bcol;   // [m][col]
+    double* colsum;       // [m][col] or null
+    double* gam;          // Thomas variant scratch, same layout as spec
+    int mode0, nmodes;    // range of modes solved
+};
+
+struct InvArgs {
+    Geo g;
+    Spec s;
+    Phys ph;
+    const Mix* mx;          // device copies of the precompute blocks
+    const BoxHom* bh;
+    const CycHom* ch;
+    const double* spec;
+    const double* tw;
+    const double* colsum;   // box: [m][col]
+    const double* colw;     // box: closed-form x-sum weights per column
+    const double* entpart;
+    int nentpart;
+    const double* hom;      // box: (nl-1) pitch fields; cyclic: bt(nyp), bc1(nl-1,nyp), bc2(nl-1,nyp)
+    const double* cin;      // constraint state in
+    double* cout;           // constraint state out
+    const double* aux;      // cyclic boundary integrals
+    const double* yprel;
+    const double* ddyn;
+    double* pnew;           // p_new (pm slot)
+    double* qnew;           // q_new (boundary ring written)
+    double* out_raw;        // raw mode output field
+    int raw_mode;           // >=0 raw mode
+    double tdt;
+};
+
+struct CycArgs {
+    Geo g;
+    Phys ph;
+    const double* q;
+    const double* p;
+    const double* pm;
+    const double* ent;      // layer 1 (pitch layout)
+    const double* ent2s;    // layer-2 row 1 (nxp) or null
+    const double* ent2n;    // layer-2 row nyp (nxp) or null
+    double* aux;
+    double txis, txin;
+};
+
+struct SuppArgs {
+    Geo g;
+    double* q;
+    const double* qm;
+    double* p;
+    const double* pm;
+    double* c;
+};
+
+struct QcompArgs {
+    Geo g;
+    Mix mx;
+    const double* p;
+    const double* ddyn;
+    const double* yprel;
+    double* q;
+};
+
+// Kernel launchers (oqg_kernels.hip).
+void launch_vort(const VortArgs& a, hipStream_t st);
+void launch_cyc(const CycArgs& a, hipStream_t st);
+void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st);
+void launch_ysolve(const YArgs& a, int variant, hipStream_t st);
+void launch_inv(const InvArgs& a, hipStream_t st);
+void launch_suppress(const SuppArgs& a, hipStream_t st);
+void launch_qcomp(const QcompArgs& a, hipStream_t st);
+int inv_block_count(const Spec& s);
+void set_kernel_attrs(size_t fwd_lds, size_t inv_lds);
+
+// Device block holding the precompute read by k_inv.
+struct DevConsts {
+    Mix mx;
+    BoxHom bh;
+    CycHom ch;
+};
+
+}  // namespace oqg

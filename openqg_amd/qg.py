@@ -1,0 +1,291 @@
+"""Host mirror of OpenQG's QG component over the HIP library.
+
+`Box`, `Topog` and `QG` carry the fields of box_type (src/glam/box.F90:11-41),
+topog_type (src/glam/topog.F90:16-28) and qg_type (src/qg/qg.F90:17-53) that
+the time step reads; `init_qg` mirrors qg.F90:60.  The dynamic state lives on
+the GPU; the `q`, `qm`, `p`, `pm` properties copy it back as numpy arrays of
+shape (nl, nyp, nxp) (the C view of the Fortran (nxp, nyp, nl) arrays).
+Fields may also be passed as torch tensors (CPU or GPU, float64, contiguous).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import OQG_DEVICE, OQG_HOST, OqgDesc, check, load
+
+
+@dataclass
+class Box:
+    """box_type: p-grid size, spacing, layers and Coriolis parameters."""
+
+    nxp: int
+    nyp: int
+    nl: int
+    dx: float
+    dy: float
+    fnot: float
+    beta: float
+    h: Sequence[float]
+    cyclic: bool = False
+    dz_sign: int = -1
+    hm: float = 0.0
+
+    @property
+    def nxt(self):
+        return self.nxp - 1
+
+    @property
+    def nyt(self):
+        return self.nyp - 1
+
+    @property
+    def xl(self):
+        return self.nxt * self.dx          # box.F90:135
+
+    @property
+    def yl(self):
+        return self.nyt * self.dy          # box.F90:136
+
+    @property
+    def yprel(self):
+        j = np.arange(self.nyp, dtype=np.float64)
+        return -0.5 * self.yl + j * self.dy  # box.F90:144
+
+
+@dataclass
+class Topog:
+    """topog_type: dynamic topography ddyn = f0/H(k_topo) * dtop and k_topo (1-based)."""
+
+    ddyn: Optional[np.ndarray] = None
+    k_topo: Optional[int] = None
+
+
+def _as_f64_contig(a):
+    """Return (object keeping memory alive, pointer, kind) for numpy or torch input."""
+    if a is None:
+        return None, None, OQG_HOST
+    try:
+        import torch  # noqa: F401
+
+        if isinstance(a, torch.Tensor):
+            if a.dtype != torch.float64:
+                raise TypeError("fields must be float64")
+            t = a.contiguous()
+            if t.is_cuda:
+                torch.cuda.current_stream(t.device).synchronize()
+                return t, ctypes.c_void_p(t.data_ptr()), OQG_DEVICE
+            return t, ctypes.c_void_p(t.data_ptr()), OQG_HOST
+    except ImportError:
+        pass
+    arr = np.ascontiguousarray(a, dtype=np.float64)
+    return arr, arr.ctypes.data_as(ctypes.c_void_p), OQG_HOST
+
+
+class QG:
+    """One basin resident on the GPU (qg_type + liboqg context)."""
+
+    def __init__(self, b: Box, bcco: float, delek: float, ah2, ah4, gp, rho: float = 1000.0,
+                 tau_sign: int = 1, topo: Optional[Topog] = None, flags: int = 0, precompute=None):
+        lib = load()
+        self.b = b
+        self.bcco, self.delek, self.rho, self.tau_sign = float(bcco), float(delek), float(rho), int(tau_sign)
+        self.ah2 = [float(x) for x in ah2]
+        self.ah4 = [float(x) for x in ah4]
+        self.gp = [float(x) for x in gp][: b.nl - 1]
+        self.topo = topo or Topog()
+        self.flags = int(flags)
+        d = OqgDesc()
+        d.nxp, d.nyp, d.nl, d.cyclic = b.nxp, b.nyp, b.nl, int(bool(b.cyclic))
+        d.dx, d.dy, d.fnot, d.beta = b.dx, b.dy, b.fnot, b.beta
+        for k in range(b.nl):
+            d.h[k], d.ah2[k], d.ah4[k] = b.h[k], self.ah2[k], self.ah4[k]
+        for k in range(b.nl - 1):
+            d.gp[k] = self.gp[k]
+        d.bcco, d.delek = self.bcco, self.delek
+        d.dz_sign, d.tau_sign = int(b.dz_sign), self.tau_sign
+        d.k_topo = int(self.topo.k_topo or 0)
+        d.flags = self.flags
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            arr = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+        if self.topo.ddyn is not None:
+            d.ddyn = ptr(self.topo.ddyn)
+        if precompute:
+            # Fortran column-major (nl,nl) arrays, as held in qg%mod / qg%hom
+            for name in ("amat", "ctl2m", "ctm2l"):
+                if name in precompute:
+                    setattr(d, name, ptr(np.asarray(precompute[name]).T))
+            for name in ("rdm2", "hom_sol_bc", "hom_bt", "hom_bc1", "hom_bc2", "hom_coef"):
+                if name in precompute:
+                    setattr(d, name, ptr(precompute[name]))
+        h = ctypes.c_void_p()
+        rc = lib.oqg_create(ctypes.byref(d), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.OqgError(f"oqg_create failed (code {rc}): {lib.oqg_last_error(None).decode()}")
+        self._h = h
+        self._lib = lib
+
+    # ------------------------------------------------------------ lifecycle
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.oqg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.b.nl, self.b.nyp, self.b.nxp)
+
+    def to_dict(self) -> dict:
+        """Parameters as a plain dict (the form the test oracle takes)."""
+        b = self.b
+        return dict(nxp=b.nxp, nyp=b.nyp, nl=b.nl, cyclic=bool(b.cyclic), dx=b.dx, dy=b.dy, fnot=b.fnot,
+                    beta=b.beta, h=list(b.h), gp=list(self.gp), ah2=list(self.ah2), ah4=list(self.ah4),
+                    bcco=self.bcco, delek=self.delek, dz_sign=b.dz_sign, tau_sign=self.tau_sign,
+                    k_topo=self.topo.k_topo, ddyn=self.topo.ddyn)
+
+    # ------------------------------------------------------------ state
+    def set_state(self, q=None, qm=None, p=None, pm=None):
+        objs = [_as_f64_contig(a) for a in (q, qm, p, pm)]
+        kinds = {o[2] for o in objs if o[0] is not None}
+        if len(kinds) > 1:
+            raise ValueError("pass all fields on the host or all on the device")
+        kind = kinds.pop() if kinds else OQG_HOST
+        for o in objs:
+            if o[0] is not None and tuple(o[0].shape) != self.shape:
+                raise ValueError(f"field shape {tuple(o[0].shape)} != {self.shape}")
+        check(self._lib.oqg_set_state(self._h, *(o[1] for o in objs), kind), self._h, "oqg_set_state")
+
+    def get_state(self):
+        outs = [np.empty(self.shape) for _ in range(4)]
+        ptrs = [o.ctypes.data_as(ctypes.c_void_p) for o in outs]
+        check(self._lib.oqg_get_state(self._h, *ptrs, OQG_HOST), self._h, "oqg_get_state")
+        return tuple(outs)
+
+    def get_state_torch(self, device="cuda"):
+        """Copy q, qm, p, pm into new torch tensors on `device` (device-to-device)."""
+        import torch
+
+        outs = [torch.empty(self.shape, dtype=torch.float64, device=device) for _ in range(4)]
+        torch.cuda.current_stream(outs[0].device).synchronize()
+        ptrs = [ctypes.c_void_p(t.data_ptr()) for t in outs]
+        check(self._lib.oqg_get_state(self._h, *ptrs, OQG_DEVICE), self._h, "oqg_get_state")
+        return tuple(outs)
+
+    def _field(self, which):
+        out = np.empty(self.shape)
+        for k in range(self.b.nl):
+            check(self._lib.oqg_get_layer(self._h, which, k + 1,
+                                          out[k].ctypes.data_as(ctypes.c_void_p), OQG_HOST),
+                  self._h, "oqg_get_layer")
+        return out
+
+    q = property(lambda self: self._field(_lib.FIELD_Q))
+    qm = property(lambda self: self._field(_lib.FIELD_QM))
+    p = property(lambda self: self._field(_lib.FIELD_P))
+    pm = property(lambda self: self._field(_lib.FIELD_PM))
+
+    def get_layer(self, which: int, k: int) -> np.ndarray:
+        out = np.empty((self.b.nyp, self.b.nxp))
+        check(self._lib.oqg_get_layer(self._h, which, k, out.ctypes.data_as(ctypes.c_void_p), OQG_HOST),
+              self._h, "oqg_get_layer")
+        return out
+
+    def get_constr(self) -> np.ndarray:
+        n = self._lib.oqg_constr_len(self._h)
+        out = np.zeros(n)
+        check(self._lib.oqg_get_constr(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              self._h, "oqg_get_constr")
+        return out
+
+    def set_constr(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        check(self._lib.oqg_set_constr(self._h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+              self._h, "oqg_set_constr")
+
+    def init_state(self):
+        """init_pv + init_constraint + init_mass_constr (openqg.F90:239-243)."""
+        check(self._lib.oqg_init_state(self._h), self._h, "oqg_init_state")
+
+    # ------------------------------------------------------------ stepping
+    def set_forcing(self, ent, wekp, txis: float = 0.0, txin: float = 0.0):
+        ent_o = _as_f64_contig(ent)
+        wek_o = _as_f64_contig(wekp)
+        n_ent = 1 if ent_o[0] is None or ent_o[0].ndim == 2 else int(ent_o[0].shape[0])
+        if ent_o[0] is not None and wek_o[0] is not None and ent_o[2] != wek_o[2]:
+            raise ValueError("ent and wekp must both be on the host or both on the device")
+        kind = ent_o[2] if ent_o[0] is not None else wek_o[2]
+        check(self._lib.oqg_set_forcing(self._h, ent_o[1], n_ent, wek_o[1], float(txis), float(txin), kind),
+              self._h, "oqg_set_forcing")
+
+    def step(self, tdt: float, ent=None, wekp=None, txis: float = 0.0, txin: float = 0.0):
+        """step_qg (core.F90:16).  With ent and wekp None the resident forcing is used."""
+        if ent is None and wekp is None:
+            check(self._lib.oqg_step(self._h, float(tdt), None, 0, None, 0.0, 0.0), self._h, "oqg_step")
+            return
+        self.set_forcing(ent, wekp, txis, txin)
+        check(self._lib.oqg_step(self._h, float(tdt), None, 0, None, 0.0, 0.0), self._h, "oqg_step")
+
+    def suppress(self):
+        """suppress_comp_mode (driver.F90:174-189)."""
+        check(self._lib.oqg_suppress(self._h), self._h, "oqg_suppress")
+
+    def run(self, tdt: float, nsteps: int, step0: int = 0, supp_every: int = 25):
+        """Ocean-only time loop on the resident forcing (openqg.F90:483-509)."""
+        check(self._lib.oqg_run(self._h, float(tdt), int(nsteps), int(step0), int(supp_every)), self._h, "oqg_run")
+
+    def sync(self):
+        check(self._lib.oqg_sync(self._h), self._h, "oqg_sync")
+
+    def solve_inhomog(self, m: int, rhs) -> np.ndarray:
+        """solve_inhomog_eqn (inhomog.F90:156) for mode m (1-based)."""
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        out = np.empty_like(rhs)
+        check(self._lib.oqg_solve_inhomog(self._h, int(m), rhs.ctypes.data_as(ctypes.c_void_p),
+                                          out.ctypes.data_as(ctypes.c_void_p), OQG_HOST),
+              self._h, "oqg_solve_inhomog")
+        return out
+
+    def modes(self):
+        """(amat, ctl2m, ctm2l, rdm2) with matrices indexed [row][col] as in Fortran (i,j)."""
+        nl = self.b.nl
+        mats = [np.zeros(nl * nl) for _ in range(3)]
+        rdm2 = np.zeros(nl)
+        P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(self._lib.oqg_get_modes(self._h, P(mats[0]), P(mats[1]), P(mats[2]), P(rdm2)), self._h, "oqg_get_modes")
+        return tuple(m.reshape(nl, nl).T.copy() for m in mats) + (rdm2,)
+
+    def kernel_times(self, tdt: float, nsteps: int):
+        """Per-kernel average duration (ms) over nsteps real steps, from HIP events."""
+        ms = (ctypes.c_double * 8)()
+        names = ctypes.create_string_buffer(256)
+        n = self._lib.oqg_kernel_times(self._h, float(tdt), int(nsteps), ms, 8, names, 256)
+        if n < 0:
+            check(-n, self._h, "oqg_kernel_times")
+        keys = names.value.decode().split("\n")[:n]
+        return {k: ms[i] for i, k in enumerate(keys)}
+
+    def set_stream(self, stream_handle: int | None):
+        check(self._lib.oqg_set_stream(self._h, ctypes.c_void_p(stream_handle) if stream_handle else None),
+              self._h, "oqg_set_stream")
+
+
+def init_qg(go: Box, bcco, delek, ah2, ah4, gp, rho, tau_sign, topo: Optional[Topog] = None, **kw) -> QG:
+    """init_qg (src/qg/qg.F90:60-87): a resting basin with derived modes and Helmholtz tables."""
+    return QG(go, bcco, delek, ah2, ah4, gp, rho, tau_sign, topo, **kw)

@@ -1,0 +1,290 @@
+"""ctypes binding of the CPU oracle (oracle/cpuref.c).
+
+Test infrastructure only: imported by tests/, by the smoke check in
+__graft_entry__.py and by the cpu_baseline leg of bench.py, never by the
+openqg_amd package.  Fields are numpy float64 arrays of shape (nl, nyp, nxp),
+the C view of the reference's Fortran (nxp, nyp, nl) arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libcpuref.so")
+MAXL = 8
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class CrParams(ctypes.Structure):
+    _fields_ = [
+        ("nxp", ctypes.c_int), ("nyp", ctypes.c_int), ("nl", ctypes.c_int), ("cyclic", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("fnot", ctypes.c_double), ("beta", ctypes.c_double),
+        ("h", ctypes.c_double * MAXL), ("gp", ctypes.c_double * MAXL),
+        ("ah2", ctypes.c_double * MAXL), ("ah4", ctypes.c_double * MAXL),
+        ("bcco", ctypes.c_double), ("delek", ctypes.c_double),
+        ("dz_sign", ctypes.c_int), ("tau_sign", ctypes.c_int), ("k_topo", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = _c_double_p
+        i = ctypes.c_int
+        d = ctypes.c_double
+        L.cr_create.argtypes = [ctypes.POINTER(CrParams), dp, ctypes.POINTER(vp)]
+        L.cr_create.restype = i
+        L.cr_destroy.argtypes = [vp]
+        L.cr_set_state.argtypes = [vp, dp, dp, dp, dp]
+        L.cr_get_state.argtypes = [vp, dp, dp, dp, dp]
+        L.cr_constr_len.argtypes = [vp]
+        L.cr_constr_len.restype = i
+        L.cr_get_constr.argtypes = [vp, dp]
+        L.cr_set_constr.argtypes = [vp, dp]
+        L.cr_init_state.argtypes = [vp]
+        L.cr_step.argtypes = [vp, d, dp, i, dp, d, d]
+        L.cr_suppress.argtypes = [vp]
+        L.cr_get_modes.argtypes = [vp, dp, dp, dp, dp]
+        L.cr_get_homog.argtypes = [vp, dp, dp, dp, dp]
+        L.cr_solve_inhomog.argtypes = [vp, i, dp, dp]
+        L.cr_del2_bc.argtypes = [vp, dp, dp, i]
+        L.cr_jacobian.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.cr_int_P_dA.argtypes = [vp, dp]
+        L.cr_int_P_dA.restype = d
+        L.cr_int_P_dx.argtypes = [vp, dp]
+        L.cr_int_P_dx.restype = d
+        L.cr_dst1.argtypes = [i, i, dp, dp]
+        L.cr_rfft.argtypes = [i, i, dp, dp]
+        L.cr_irfft.argtypes = [i, i, dp, dp]
+        L.cr_cfft.argtypes = [i, i, i, dp, dp]
+        L.cr_ekman_wekp.argtypes = [ctypes.POINTER(CrParams), d, dp, dp, dp, dp, dp]
+        L.cr_set_threads.argtypes = [i]
+        L.cr_ocqbdy.argtypes = [vp]
+        L.cr_thomas.argtypes = [i, d, d, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_c_double_p)
+
+
+def _get(cfg, key, default=None):
+    if isinstance(cfg, dict):
+        return cfg.get(key, default)
+    return getattr(cfg, key, default)
+
+
+def make_params(cfg) -> CrParams:
+    nl = int(_get(cfg, "nl"))
+    P = CrParams()
+    P.nxp, P.nyp, P.nl = int(_get(cfg, "nxp")), int(_get(cfg, "nyp")), nl
+    P.cyclic = int(bool(_get(cfg, "cyclic", False)))
+    P.dx, P.dy = float(_get(cfg, "dx")), float(_get(cfg, "dy"))
+    P.fnot, P.beta = float(_get(cfg, "fnot")), float(_get(cfg, "beta"))
+    for name in ("h", "ah2", "ah4"):
+        arr = getattr(P, name)
+        for k, v in enumerate(_get(cfg, name)):
+            arr[k] = float(v)
+    for k, v in enumerate(_get(cfg, "gp")):
+        P.gp[k] = float(v)
+    P.bcco, P.delek = float(_get(cfg, "bcco")), float(_get(cfg, "delek"))
+    P.dz_sign, P.tau_sign = int(_get(cfg, "dz_sign")), int(_get(cfg, "tau_sign"))
+    kt = _get(cfg, "k_topo", None)
+    P.k_topo = int(kt) if kt is not None else (1 if P.dz_sign == 1 else nl)
+    return P
+
+
+def set_threads(n: int):
+    lib().cr_set_threads(int(n))
+
+
+class Basin:
+    """One oracle basin: init_qg + state + step_qg."""
+
+    def __init__(self, cfg):
+        L = lib()
+        self.cfg = cfg
+        self.P = make_params(cfg)
+        self.nxp, self.nyp, self.nl = self.P.nxp, self.P.nyp, self.P.nl
+        ddyn = _get(cfg, "ddyn", None)
+        self._ddyn = None if ddyn is None else np.ascontiguousarray(ddyn, dtype=np.float64)
+        h = ctypes.c_void_p()
+        rc = L.cr_create(ctypes.byref(self.P), _ptr(self._ddyn), ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"cr_create failed rc={rc}")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.cr_destroy(h)
+            self._h = None
+
+    @property
+    def shape(self):
+        return (self.nl, self.nyp, self.nxp)
+
+    def _field(self):
+        return np.zeros(self.shape, dtype=np.float64)
+
+    def set_state(self, q=None, qm=None, p=None, pm=None):
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        q, qm, p, pm = c(q), c(qm), c(p), c(pm)
+        lib().cr_set_state(self._h, _ptr(q), _ptr(qm), _ptr(p), _ptr(pm))
+
+    def get_state(self):
+        q, qm, p, pm = (self._field() for _ in range(4))
+        lib().cr_get_state(self._h, _ptr(q), _ptr(qm), _ptr(p), _ptr(pm))
+        return q, qm, p, pm
+
+    def get_constr(self):
+        n = lib().cr_constr_len(self._h)
+        out = np.zeros(n)
+        lib().cr_get_constr(self._h, _ptr(out))
+        return out
+
+    def set_constr(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        lib().cr_set_constr(self._h, _ptr(v))
+
+    def init_state(self):
+        lib().cr_init_state(self._h)
+
+    def step(self, tdt, ent, wekp, txis=0.0, txin=0.0):
+        ent = np.ascontiguousarray(ent, dtype=np.float64)
+        if ent.ndim == 2:
+            ent = ent[None]
+        wekp = np.ascontiguousarray(wekp, dtype=np.float64)
+        lib().cr_step(self._h, float(tdt), _ptr(ent), int(ent.shape[0]), _ptr(wekp), float(txis), float(txin))
+
+    def suppress(self):
+        lib().cr_suppress(self._h)
+
+    def ocqbdy(self):
+        lib().cr_ocqbdy(self._h)
+
+    def modes(self):
+        nl = self.nl
+        amat, ctl2m, ctm2l = (np.zeros((nl, nl)) for _ in range(3))
+        rdm2 = np.zeros(nl)
+        lib().cr_get_modes(self._h, _ptr(amat), _ptr(ctl2m), _ptr(ctm2l), _ptr(rdm2))
+        # stored Fortran order (column-major) -> transpose to [row][col] = (i, j)
+        return amat.T.copy(), ctl2m.T.copy(), ctm2l.T.copy(), rdm2
+
+    def homog(self):
+        nl = self.nl
+        if not self.P.cyclic:
+            hs = np.zeros((nl - 1, self.nyp, self.nxp))
+            lib().cr_get_homog(self._h, _ptr(hs), None, None, None)
+            return hs
+        bt = np.zeros(self.nyp)
+        b1 = np.zeros((nl - 1, self.nyp))
+        b2 = np.zeros((nl - 1, self.nyp))
+        coef = np.zeros(4 * (nl - 1) + 1)
+        lib().cr_get_homog(self._h, _ptr(bt), _ptr(b1), _ptr(b2), _ptr(coef))
+        return bt, b1, b2, coef
+
+    def solve_inhomog(self, m, rhs):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        out = np.zeros((self.nyp, self.nxp))
+        lib().cr_solve_inhomog(self._h, int(m), _ptr(rhs), _ptr(out))
+        return out
+
+    def del2(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        nl = 1 if f.ndim == 2 else f.shape[0]
+        out = np.zeros_like(f)
+        lib().cr_del2_bc(self._h, _ptr(f), _ptr(out), nl)
+        return out
+
+    def jacobian(self, p, q):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        jac = np.zeros_like(p)
+        ajis = np.zeros(self.nl)
+        ajin = np.zeros(self.nl)
+        lib().cr_jacobian(self._h, _ptr(p), _ptr(q), _ptr(jac), _ptr(ajis), _ptr(ajin))
+        return jac, ajis, ajin
+
+    def int_P_dA(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        return lib().cr_int_P_dA(self._h, _ptr(f))
+
+    def int_P_dx(self, row):
+        row = np.ascontiguousarray(row, dtype=np.float64)
+        return lib().cr_int_P_dx(self._h, _ptr(row))
+
+
+def dst1(x):
+    """FFTW RODFT00 (unnormalised) along the last axis."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[-1]
+    out = np.zeros_like(x)
+    lib().cr_dst1(n, int(x.size // n), _ptr(x), _ptr(out))
+    return out
+
+
+def thomas(a, bb, f):
+    """The reference's Numerical Recipes Thomas solve (inhomog.F90:188-199)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    u = np.zeros_like(f)
+    lib().cr_thomas(int(f.size), float(a), float(bb), _ptr(f), _ptr(u))
+    return u
+
+
+def rfft(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.shape[-1]
+    cnt = int(x.size // n)
+    out = np.zeros(x.shape[:-1] + (n // 2 + 1, 2))
+    lib().cr_rfft(n, cnt, _ptr(x), _ptr(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def irfft(X, n):
+    Xr = np.ascontiguousarray(np.stack([X.real, X.imag], axis=-1), dtype=np.float64)
+    cnt = int(Xr.size // (2 * (n // 2 + 1)))
+    out = np.zeros(X.shape[:-1] + (n,))
+    lib().cr_irfft(n, cnt, _ptr(Xr), _ptr(out))
+    return out
+
+
+def cfft(x, sign=-1):
+    x = np.asarray(x, dtype=np.complex128)
+    n = x.shape[-1]
+    xr = np.ascontiguousarray(np.stack([x.real, x.imag], axis=-1))
+    out = np.zeros_like(xr)
+    lib().cr_cfft(n, int(sign), int(x.size // n), _ptr(xr), _ptr(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def ekman_wekp(cfg, taux, tauy):
+    P = make_params(cfg)
+    taux = np.ascontiguousarray(taux, dtype=np.float64)
+    tauy = np.ascontiguousarray(tauy, dtype=np.float64)
+    wekp = np.zeros_like(taux)
+    txis = ctypes.c_double()
+    txin = ctypes.c_double()
+    lib().cr_ekman_wekp(ctypes.byref(P), 0.0, _ptr(taux), _ptr(tauy), _ptr(wekp),
+                        ctypes.cast(ctypes.byref(txis), _c_double_p), ctypes.cast(ctypes.byref(txin), _c_double_p))
+    return wekp, txis.value, txin.value
