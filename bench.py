@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0, help="oqg_desc.flags (1: per-thread Thomas y-solve)")
+    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying graphs")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 10 device copies of 256 MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); this build has no multi-GPU path",
@@ -102,6 +105,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate_copy:
+        n = 256 * 1024 * 1024 // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 10}), flush=True)
+        return
 
     from openqg_amd import workloads as W
 
@@ -113,11 +126,20 @@ def main():
     ent, wekp, txis, txin = w.forcing()
     qg.set_forcing(ent, wekp, txis, txin)
 
-    qg.run(w.tdt, args.warmup, 0, w.supp_every)
+    def advance(nsteps, step0):
+        if args.eager:   # one launch per kernel, no graph (for counter collection)
+            for nt in range(step0 + 1, step0 + nsteps + 1):
+                qg.step(w.tdt)
+                if (nt - 1) % w.supp_every == 0:
+                    qg.suppress()
+        else:
+            qg.run(w.tdt, nsteps, step0, w.supp_every)
+
+    advance(args.warmup, 0)
     qg.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    qg.run(w.tdt, args.steps, args.warmup, w.supp_every)
+    advance(args.steps, args.warmup)
     qg.sync()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
@@ -126,6 +148,8 @@ def main():
 
     # per-kernel durations from HIP events on the library's stream
     kt = qg.kernel_times(w.tdt, 50)
+    if not w.cyclic:
+        kt.pop("k_cyc", None)   # not launched for a box; its slot only spans two event records
     dom = max((k for k in kt if kt[k] > 0), key=lambda k: kt[k])
     ach = algorithmic_bytes(w, dom) / (kt[dom] * 1e-3) / 1e9
     roofline = {

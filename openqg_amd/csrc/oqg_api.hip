@@ -521,11 +521,9 @@ static int setup(oqg_ctx* c) {
         Nref = s.L;
     }
     if (s.L > MAX_FFT) return fail(c, OQG_ERR_ARG, "x transform longer than 4096 points");
-    {
-        const size_t inv_lds = sizeof(double2) * 2 * (size_t)s.L + sizeof(double) * 2 * (size_t)nl * g.pitch;
-        if (inv_lds > 150 * 1024) return fail(c, OQG_ERR_ARG, "grid too wide for the LDS-staged x transform");
-    }
     s.nyi = nyp - 2;
+    s.nrow = s.nyi + 1;
+    s.kp = (s.ncol + 15) / 16 * 16;
     s.R = (s.nyi + WAVE - 1) / WAVE;
     if (s.R > MAX_YROWS_PER_LANE) return fail(c, OQG_ERR_ARG, "nyp too large for the y solver (nyp-2 > 1024)");
     s.nypad = WAVE * s.R;
@@ -539,7 +537,11 @@ static int setup(oqg_ctx* c) {
         while (m % 3 == 0) { s.radix[s.nstages++] = 3; m /= 3; }
         while (m % 5 == 0) { s.radix[s.nstages++] = 5; m /= 5; }
         if (m != 1) return fail(c, OQG_ERR_ARG, "x transform length must factor into 2, 3 and 5");
+        for (int st = 0; st < s.nstages; st++)   // each thread holds at most 4 butterflies per stage
+            if (s.L / s.radix[st] > 4 * 256) return fail(c, OQG_ERR_ARG, "x transform too long for the LDS FFT");
     }
+    if (inv_lds_bytes(s, g, nl) > 150 * 1024)
+        return fail(c, OQG_ERR_ARG, "grid too wide for the LDS-staged x transforms");
 
     // modes
     if (d.ctl2m && d.ctm2l && d.rdm2 && d.amat) {
@@ -559,7 +561,7 @@ static int setup(oqg_ctx* c) {
 
     // device buffers
     const size_t fl = (size_t)nl * g.fs, f1 = (size_t)g.fs;
-    const size_t nspec = (size_t)nl * s.ncol * s.nypad;
+    const size_t nspec = (size_t)nl * s.nrow * s.kp;
     c->tiles_x = (nxp + VORT_TX - 1) / VORT_TX;
     c->tiles_y = (nyp + VORT_TY - 1) / VORT_TY;
     int rc = 0;
@@ -634,8 +636,7 @@ static int setup(oqg_ctx* c) {
         }
         HIPCHK(c, hipMemcpyAsync(c->colw, colw.data(), colw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
     }
-    set_kernel_attrs(sizeof(double2) * 2 * (size_t)s.L,
-                     sizeof(double2) * 2 * (size_t)s.L + sizeof(double) * 2 * (size_t)nl * g.pitch);
+    set_kernel_attrs(s, g);
 
     // homogeneous solutions (homsol, homog.F90:108-278) from this library's own solver
     const size_t nxy = (size_t)nxp * nyp;
@@ -748,6 +749,7 @@ int oqg_create(const oqg_desc* desc, oqg_ctx** out) {
         if (!(d.gp[k] > 0)) return fail(nullptr, OQG_ERR_ARG, "reduced gravities must be > 0");
     if (d.k_topo < 0 || d.k_topo > d.nl) return fail(nullptr, OQG_ERR_ARG, "k_topo out of range");
     if (d.cyclic && d.nxp - 1 > 1024) return fail(nullptr, OQG_ERR_ARG, "cyclic nxp-1 must be <= 1024");
+    if (!d.cyclic && d.nxp > 1025) return fail(nullptr, OQG_ERR_ARG, "box nxp must be <= 1025");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(nullptr, OQG_ERR_HIP, "no HIP device available");

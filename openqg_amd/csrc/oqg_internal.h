@@ -30,12 +30,15 @@ struct Geo {
     double jfac;             // Arakawa factor (numerics.F90:361)
 };
 
-// Spectral (x-transformed) layout: spec[m][col][j], j fastest with pitch nypad.
+// Spectral (x-transformed) layout: spec[m][j][k], row-major, rows of kp doubles,
+// nrow = nyi+1 rows per mode (the last one only padding).
 struct Spec {
     int nin;        // transform input points per row (box nxp-2, cyclic nxp-1)
     int L;          // complex FFT length (box 2(nxp-1), cyclic nxp-1)
     int ncol;       // real tridiagonal columns per mode
     int nyi;        // interior rows nyp-2
+    int nrow;       // nyi + 1
+    int kp;         // spectral row pitch (doubles), multiple of 16
     int R;          // rows per lane in the wavefront y-solver
     int nypad;      // 64*R
     int nstages;
@@ -190,7 +193,8 @@ void launch_inv(const InvArgs& a, hipStream_t st);
 void launch_suppress(const SuppArgs& a, hipStream_t st);
 void launch_qcomp(const QcompArgs& a, hipStream_t st);
 int inv_block_count(const Spec& s);
-void set_kernel_attrs(size_t fwd_lds, size_t inv_lds);
+size_t inv_lds_bytes(const Spec& s, const Geo& g, int nmodes);
+void set_kernel_attrs(const Spec& s, const Geo& g);
 
 // Device block holding the precompute read by k_inv.
 struct DevConsts {

@@ -1,0 +1,318 @@
+// oqg_stencil.hip -- the grid-point kernels of the QG step:
+//   k_vort     Arakawa Jacobian + Del^2/Del^4/Del^6 friction + forcing + leapfrog
+//              (vorticity.F90:14-103, numerics.F90:340-468) on 64x16 LDS tiles with a
+//              halo of 3; writes q_new over qm;
+//   k_cyc      (channel only) boundary line integrals of the momentum constraints
+//              (vorticity.F90:57-62,84-101, numerics.F90:374-386);
+//   k_suppress suppress_comp_mode (driver.F90:174-189);
+//   k_qcomp    qcomp (vorsubs.F90:27-56), used by init_pv.
+// All arithmetic is FP64; the library is built with -ffp-contract=off.
+
+#include "oqg_device.h"
+
+namespace oqg {
+
+// ------------------------------------------------------------------ k_vort
+
+__global__ __launch_bounds__(256) void k_vort(VortArgs a) {
+    constexpr int TX = VORT_TX, TY = VORT_TY;
+    constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
+    __shared__ double s_pm[H3 * W3];
+    __shared__ double s_d2[H2 * W2];
+    __shared__ double s_d4[H1 * W1];
+    __shared__ double s_q[H1 * W1];
+    __shared__ double s_p[H1 * W1];
+    __shared__ double s_red[16];
+    const Geo& g = a.g;
+    const int tid = threadIdx.x;
+    const int tx = tid & 63, ty = tid >> 6;
+    const int k = blockIdx.z;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const long long lo = (long long)k * g.fs;
+    const double* pm = a.pm + lo;
+    const double* q = a.q + lo;
+    const double* p = a.p + lo;
+
+    // Issue every global load of the block at once (one memory round trip):
+    // the pm tile with halo 3, the q and p tiles with halo 1, and the pointwise
+    // inputs of this thread's four output points.
+    constexpr int NPM = (H3 * W3 + 255) / 256, NQP = (H1 * W1 + 255) / 256, NOUT = TY / 4;
+    double rpm[NPM], rq[NQP], rp[NQP], rqm[NOUT], rent[NOUT], rwek[NOUT];
+#pragma unroll
+    for (int t = 0; t < NPM; t++) {
+        const int idx = tid + 256 * t;
+        const int r = idx / W3, c = idx - r * W3;
+        rpm[t] = (idx < H3 * W3) ? ldg_field(pm, i0 - 3 + c, j0 - 3 + r, g) : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NQP; t++) {
+        const int idx = tid + 256 * t;
+        const int r = idx / W1, c = idx - r * W1;
+        const bool in = idx < H1 * W1;
+        rq[t] = in ? ldg_field(q, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
+        rp[t] = in ? ldg_field(p, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
+    }
+    const int gi = i0 + tx;
+#pragma unroll
+    for (int u = 0; u < NOUT; u++) {
+        const int gj = j0 + ty + 4 * u;
+        const bool in = gi < g.nxp && gj < g.nyp;
+        const long long o2 = (long long)gj * g.pitch + gi;
+        rqm[u] = in ? a.qnew[lo + o2] : 0.0;
+        rent[u] = (in && k <= 1) ? a.ent1[o2] : 0.0;
+        rwek[u] = (in && k == 0) ? a.wekp[o2] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < NPM; t++) {
+        const int idx = tid + 256 * t;
+        if (idx < H3 * W3) s_pm[idx] = rpm[t];
+    }
+#pragma unroll
+    for (int t = 0; t < NQP; t++) {
+        const int idx = tid + 256 * t;
+        if (idx < H1 * W1) {
+            s_q[idx] = rq[t];
+            s_p[idx] = rp[t];
+        }
+    }
+    __syncthreads();
+    // Del^2 of pm on the tile with halo 2
+    for (int idx = tid; idx < H2 * W2; idx += 256) {
+        const int r = idx / W2, c = idx - r * W2;
+        const int gj = j0 - 2 + r, gx = i0 - 2 + c;
+        double v = 0.0;
+        if (gj >= 0 && gj < g.nyp) {
+            if (g.cyclic) v = del2_lds(s_pm, W3, r + 1, c + 1, wrapx(gx, g), gj, g);
+            else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_pm, W3, r + 1, c + 1, gx, gj, g);
+        }
+        s_d2[idx] = v;
+    }
+    __syncthreads();
+    // Del^4 on the tile with halo 1
+    for (int idx = tid; idx < H1 * W1; idx += 256) {
+        const int r = idx / W1, c = idx - r * W1;
+        const int gj = j0 - 1 + r, gx = i0 - 1 + c;
+        double v = 0.0;
+        if (gj >= 0 && gj < g.nyp) {
+            if (g.cyclic) v = del2_lds(s_d2, W2, r + 1, c + 1, wrapx(gx, g), gj, g);
+            else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_d2, W2, r + 1, c + 1, gx, gj, g);
+        }
+        s_d4[idx] = v;
+    }
+    __syncthreads();
+    const double c2 = a.ph.c2[k], c4 = a.ph.c4[k];
+    double epart = 0.0;
+#pragma unroll
+    for (int u = 0; u < NOUT; u++) {
+        const int r = ty + 4 * u;
+        const int gj = j0 + r;
+        if (gi >= g.nxp || gj >= g.nyp) continue;
+        const double d6 = del2_lds(s_d4, W1, r + 1, tx + 1, gi, gj, g);
+        const double d4 = s_d4[(r + 1) * W1 + tx + 1];
+        const double d2 = s_d2[(r + 2) * W2 + tx + 2];
+        double dqdt = jac_lds(s_q, s_p, W1, r + 1, tx + 1, gi, gj, g);
+        dqdt = dqdt + c2 * d4 - c4 * d6;
+        if (k == 0) {
+            const double e = rent[u];
+            dqdt = dqdt + a.ph.cent1 * e + a.ph.cwek * rwek[u];
+            const bool ex = (gi == 0 || gi == g.nxp - 1), ey = (gj == 0 || gj == g.nyp - 1);
+            const double w = (ex && ey) ? 0.25 : ((ex || ey) ? 0.5 : 1.0);
+            epart += w * e;
+        }
+        if (k == 1) dqdt = dqdt - a.ph.cent2 * rent[u];
+        if (k == g.kt) dqdt = dqdt - a.ph.cdrag * d2;
+        a.qnew[lo + (long long)gj * g.pitch + gi] = rqm[u] + a.tdt * dqdt;
+    }
+    if (k == 0) {
+        double v[1] = {epart};
+        block_sum<1>(v, s_red);
+        if (threadIdx.x == 0) a.entpart[blockIdx.y * a.tiles_x + blockIdx.x] = v[0];
+    }
+}
+
+void launch_vort(const VortArgs& a, hipStream_t st) {
+    dim3 grid(a.tiles_x, a.tiles_y, a.g.nl);
+    hipLaunchKernelGGL(k_vort, grid, dim3(256), 0, st, a);
+}
+
+// ------------------------------------------------------------------ k_cyc
+
+// One workgroup per layer.  Row integrals use the int_P_dx weights
+// (numerics.F90:206-212): 1/2 at the two ends, 1 inside, times dx.
+__global__ __launch_bounds__(256) void k_cyc(CycArgs a) {
+    extern __shared__ double smem[];
+    const Geo& g = a.g;
+    const int nx = g.nxp, ny = g.nyp;
+    const int k = blockIdx.x;
+    const long long lo = (long long)k * g.fs;
+    double* d2 = smem;            // Del^2 pm on rows 0,1,2 | ny-3,ny-2,ny-1  (6 x nx)
+    double* d4 = smem + 6 * nx;   // Del^4 pm on rows 0,1 | ny-2,ny-1         (4 x nx)
+    __shared__ double red[10 * 16];
+    const int d2rows[6] = {0, 1, 2, ny - 3, ny - 2, ny - 1};
+    const double* pm = a.pm + lo;
+    for (int idx = threadIdx.x; idx < 6 * nx; idx += 256) {
+        const int rr = idx / nx, i = idx - rr * nx;
+        const int j = d2rows[rr];
+        const int il = (i == 0) ? nx - 2 : i - 1, ir = (i == nx - 1) ? 1 : i + 1;
+        const double C = pm[(long long)j * g.pitch + i];
+        double dx2, dy2;
+        if (j == 0 || j == ny - 1) dx2 = 0.0;
+        else dx2 = g.fx * ((pm[(long long)j * g.pitch + il] + pm[(long long)j * g.pitch + ir]) - 2.0 * C);
+        if (j >= 1 && j <= ny - 2) dy2 = g.fy * ((pm[(long long)(j - 1) * g.pitch + i] + pm[(long long)(j + 1) * g.pitch + i]) - 2.0 * C);
+        else if (j == 0) dy2 = g.cyl * (pm[(long long)1 * g.pitch + i] - C);
+        else dy2 = g.cyr * (C - pm[(long long)(ny - 2) * g.pitch + i]);
+        d2[idx] = dx2 + dy2;
+    }
+    __syncthreads();
+    const int d4rows[4] = {0, 1, ny - 2, ny - 1};
+    const int d4src[4] = {0, 1, 4, 5};   // the same row's index in d2
+    for (int idx = threadIdx.x; idx < 4 * nx; idx += 256) {
+        const int rr = idx / nx, i = idx - rr * nx;
+        const int j = d4rows[rr], sr = d4src[rr];
+        const int il = (i == 0) ? nx - 2 : i - 1, ir = (i == nx - 1) ? 1 : i + 1;
+        const double C = d2[sr * nx + i];
+        double dx2, dy2;
+        if (j == 0 || j == ny - 1) dx2 = 0.0;
+        else dx2 = g.fx * ((d2[sr * nx + il] + d2[sr * nx + ir]) - 2.0 * C);
+        if (j >= 1 && j <= ny - 2) dy2 = g.fy * ((d2[(sr - 1) * nx + i] + d2[(sr + 1) * nx + i]) - 2.0 * C);
+        else if (j == 0) dy2 = g.cyl * (d2[(sr + 1) * nx + i] - C);
+        else dy2 = g.cyr * (C - d2[(sr - 1) * nx + i]);
+        d4[idx] = dx2 + dy2;
+    }
+    __syncthreads();
+    const double* q = a.q + lo;
+    const double* p = a.p + lo;
+    const double* pkt = a.pm + (long long)g.kt * g.fs;
+    double v[10];
+#pragma unroll
+    for (int t = 0; t < 10; t++) v[t] = 0.0;
+    for (int i = threadIdx.x; i < nx; i += 256) {
+        const double w = (i == 0 || i == nx - 1) ? 0.5 : 1.0;
+        const int il = (i == 0) ? nx - 2 : i - 1, ir = (i == nx - 1) ? 1 : i + 1;
+        v[0] += w * (d2[1 * nx + i] - d2[0 * nx + i]);               // ap3s
+        v[1] += w * (d2[5 * nx + i] - d2[4 * nx + i]);               // ap3n
+        v[2] += w * (d4[1 * nx + i] - d4[0 * nx + i]);               // ap5s
+        v[3] += w * (d4[3 * nx + i] - d4[2 * nx + i]);               // ap5n
+        const double dpx2 = p[(long long)1 * g.pitch + ir] - p[(long long)1 * g.pitch + il];
+        const double dpxn = p[(long long)(ny - 2) * g.pitch + ir] - p[(long long)(ny - 2) * g.pitch + il];
+        v[4] += w * (q[i] * dpx2);                                   // aj5sms
+        v[5] += w * (q[(long long)1 * g.pitch + i] * dpx2);          // aj9sms
+        v[6] += w * (q[(long long)(ny - 1) * g.pitch + i] * dpxn);   // aj5smn
+        v[7] += w * (q[(long long)(ny - 2) * g.pitch + i] * dpxn);   // aj9smn
+        v[8] += w * (pkt[(long long)1 * g.pitch + i] - pkt[i]);      // bdsums
+        v[9] += w * (pkt[(long long)(ny - 1) * g.pitch + i] - pkt[(long long)(ny - 2) * g.pitch + i]);  // bdsumn
+    }
+    block_sum<10>(v, red);
+    double e[4] = {0.0, 0.0, 0.0, 0.0};
+    if (k == 0) {
+        for (int i = threadIdx.x; i < nx; i += 256) {
+            const double w = (i == 0 || i == nx - 1) ? 0.5 : 1.0;
+            e[0] += w * a.ent[i];
+            e[1] += w * a.ent[(long long)(ny - 1) * g.pitch + i];
+            if (a.ent2s) e[2] += w * a.ent2s[i];
+            if (a.ent2n) e[3] += w * a.ent2n[i];
+        }
+        block_sum<4>(e, red);
+    }
+    if (threadIdx.x == 0) {
+        const double dx = g.dx, dy = g.dy;
+        double* o = a.aux;
+        o[A_AP3S + k] = a.ph.ah2[k] * (v[0] * dx) / dy;
+        o[A_AP3N + k] = a.ph.ah2[k] * (v[1] * dx) / dy;
+        o[A_AP5S + k] = a.ph.ah4[k] * (v[2] * dx) / dy;
+        o[A_AP5N + k] = a.ph.ah4[k] * (v[3] * dx) / dy;
+        const double aj5s = (v[4] * dx) / (2.0 * dx), aj9s = (v[5] * dx) / (2.0 * dx);
+        const double aj5n = (v[6] * dx) / (2.0 * dx), aj9n = (v[7] * dx) / (2.0 * dx);
+        o[A_AJIS + k] = (aj5s + 2.0 * aj9s) / 6.0;
+        o[A_AJIN + k] = -(aj5n + 2.0 * aj9n) / 6.0;
+        if (k == g.kt) {
+            o[A_BDRS] = a.ph.cd_bdr * ((v[8] * dx) / dy);
+            o[A_BDRN] = a.ph.cd_bdr * ((v[9] * dx) / dy);
+        }
+        if (k == 0) {
+            const double dz = (double)a.ph.dz_sign;
+            for (int l = 0; l < MAXL; l++) { o[A_BES + l] = 0.0; o[A_BEN + l] = 0.0; }
+            o[A_BES + 0] = dz * a.ph.fohfac[0] * (e[0] * dx);
+            o[A_BES + 1] = dz * a.ph.fohfac[0] * (e[2] * dx);
+            o[A_BEN + 0] = -dz * a.ph.fohfac[1] * (e[1] * dx);
+            o[A_BEN + 1] = -dz * a.ph.fohfac[1] * (e[3] * dx);
+            o[A_TXIS] = a.txis;
+            o[A_TXIN] = a.txin;
+        }
+    }
+}
+
+void launch_cyc(const CycArgs& a, hipStream_t st) {
+    const size_t lds = sizeof(double) * 10 * (size_t)a.g.nxp;
+    hipLaunchKernelGGL(k_cyc, dim3(a.g.nl), dim3(256), lds, st, a);
+}
+
+// ------------------------------------------------------------------ k_suppress
+
+__global__ void k_suppress(SuppArgs a) {
+    const Geo& g = a.g;
+    const long long n = (long long)g.nl * g.nyp * g.pitch;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        a.q[t] = 0.5 * (a.q[t] + a.qm[t]);
+        a.p[t] = 0.5 * (a.p[t] + a.pm[t]);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int l = 0; l < g.nl - 1; l++) a.c[C_DPI + l] = 0.5 * (a.c[C_DPI + l] + a.c[C_DPIP + l]);
+        if (g.cyclic)
+            for (int k = 0; k < g.nl; k++) {
+                a.c[C_CS + k] = 0.5 * (a.c[C_CS + k] + a.c[C_CSP + k]);
+                a.c[C_CN + k] = 0.5 * (a.c[C_CN + k] + a.c[C_CNP + k]);
+            }
+    }
+}
+
+void launch_suppress(const SuppArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_suppress, dim3(1024), dim3(256), 0, st, a);
+}
+
+// ------------------------------------------------------------------ k_qcomp
+
+// q = Del^2_bc(p)/f0 (+ddyn at k_topo) + beta*yprel - f0*A p  (vorsubs.F90:46-54)
+__global__ void k_qcomp(QcompArgs a) {
+    const Geo& g = a.g;
+    const long long npts = (long long)g.nyp * g.nxp;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npts; t += (long long)gridDim.x * blockDim.x) {
+        const int gj = (int)(t / g.nxp), gi = (int)(t - (long long)gj * g.nxp);
+        const long long o = (long long)gj * g.pitch + gi;
+        double pv[MAXL];
+#pragma unroll
+        for (int l = 0; l < MAXL; l++) pv[l] = (l < g.nl) ? a.p[l * g.fs + o] : 0.0;
+        const int il = g.cyclic ? wrapx(gi - 1, g) : gi - 1;
+        const int ir = g.cyclic ? wrapx(gi + 1, g) : gi + 1;
+#pragma unroll
+        for (int k = 0; k < MAXL; k++) {
+            if (k >= g.nl) break;
+            const double* f = a.p + k * g.fs;
+            const double C = pv[k];
+            double dx2, dy2;
+            if (gj == 0 || gj == g.nyp - 1) dx2 = 0.0;
+            else if (g.cyclic || (gi >= 1 && gi <= g.nxp - 2)) dx2 = g.fx * ((f[(long long)gj * g.pitch + il] + f[(long long)gj * g.pitch + ir]) - 2.0 * C);
+            else if (gi == 0) dx2 = g.cxl * (f[(long long)gj * g.pitch + 1] - C);
+            else dx2 = g.cxr * (C - f[(long long)gj * g.pitch + g.nxp - 2]);
+            if (!g.cyclic && (gi == 0 || gi == g.nxp - 1)) dy2 = 0.0;
+            else if (gj >= 1 && gj <= g.nyp - 2) dy2 = g.fy * ((f[(long long)(gj - 1) * g.pitch + gi] + f[(long long)(gj + 1) * g.pitch + gi]) - 2.0 * C);
+            else if (gj == 0) dy2 = g.cyl * (f[(long long)g.pitch + gi] - C);
+            else dy2 = g.cyr * (C - f[(long long)(g.nyp - 2) * g.pitch + gi]);
+            double qv = (dx2 + dy2) / g.fnot;
+            if (k == g.kt && a.ddyn) qv = qv + a.ddyn[o];
+            qv = qv + g.beta * a.yprel[gj];
+            double ap = 0.0;
+#pragma unroll
+            for (int l = 0; l < MAXL; l++)
+                if (l < g.nl) ap += a.mx.amat[k * MAXL + l] * pv[l];
+            qv = qv - g.fnot * ap;
+            a.q[k * g.fs + o] = qv;
+        }
+    }
+}
+
+void launch_qcomp(const QcompArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_qcomp, dim3(512), dim3(256), 0, st, a);
+}
+
+}  // namespace oqg

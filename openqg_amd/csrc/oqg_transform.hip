@@ -1,0 +1,803 @@
+// oqg_transform.hip -- PV inversion kernels of the QG step (pressure.F90:21-69).
+//
+//   k_fwd     layer->mode mix (modes.F90:248-266) fused with the x transform of two
+//             interior rows per workgroup: DST-I via the odd-extended complex FFT in
+//             the box, r2c in the channel (inhomog.F90:184-185,228-229).  Spectra are
+//             stored row-major, spec[m][j][k], so both row kernels move whole rows.
+//   k_ysolve  tridiagonal solve along y for each (mode, wavenumber) column
+//             (inhomog.F90:188-199,232-243).  A workgroup stages CT columns through
+//             LDS (transposing them), then one wavefront per column: each lane
+//             eliminates its own block of rows, and the block separators are solved
+//             by parallel cyclic reduction across the 64 lanes.  Box: also the
+//             column sums that give the area integral of each modal solution.
+//   k_ythomas the reference's sequential Thomas recurrence, one thread per column
+//             (selected with OQG_FLAG_YSOLVE_THOMAS).
+//   k_inv     mass/momentum-constraint correction (homog.F90:280-366,
+//             constraint.F90:167-215), inverse x transforms of all modes of two rows
+//             (one 256-thread group per mode), mode->layer mix (modes.F90:268-284)
+//             and the boundary PV (vorsubs.F90:58-114); writes p_new over pm.
+
+#include "oqg_device.h"
+
+namespace oqg {
+
+constexpr int YS_CT = 8;   // columns per y-solver workgroup (one wavefront each)
+
+// ------------------------------------------------------------------ k_fwd
+
+template <int NL>
+__device__ __forceinline__ double rhs_mode(const FwdArgs& a, int m, int gi, int gj) {
+    // p_modes_rhs (modes.F90:259-264): f0 * sum_k (q - beta*y - [k=k_topo] ddyn) ctl2m(k,m)
+    const Geo& g = a.g;
+    const long long o = (long long)gj * g.pitch + gi;
+    const double by = g.beta * a.yprel[gj];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double ql = a.q[k * g.fs + o] - by;
+        if (k == g.kt && a.ddyn) ql = ql - a.ddyn[o];
+        s += ql * a.mx.ctl2m[k * MAXL + m];
+    }
+    return g.fnot * s;
+}
+
+template <int NL, int MB>
+__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
+    extern __shared__ double2 zbuf[];
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L;
+    const int jp = blockIdx.x;
+    const int m = (a.raw_mode >= 0) ? a.raw_mode : (int)blockIdx.y;
+    const int r0 = 2 * jp, r1 = 2 * jp + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    const double2* tw = (const double2*)a.tw;
+    const bool raw = a.raw_mode >= 0;
+    // inputs of this thread's (at most four) points of both rows, loaded together
+    const int first = g.cyclic ? 0 : 1;
+    const int npts = s.nin;   // box: interior points 1..nin; channel: 0..nin-1
+    double xv[4], yv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int t = first + threadIdx.x + 256 * u;
+        xv[u] = 0.0;
+        yv[u] = 0.0;
+        if (t < first + npts) {
+            if (raw) {
+                xv[u] = a.q[(long long)gj0 * g.pitch + t];
+                if (has1) yv[u] = a.q[(long long)gj1 * g.pitch + t];
+            } else {
+                xv[u] = rhs_mode<NL>(a, m, t, gj0);
+                if (has1) yv[u] = rhs_mode<NL>(a, m, t, gj1);
+            }
+        }
+    }
+    if (!g.cyclic) {
+        // odd extension of the two rows: z_t = x_t + i y_t, z_{L-t} = -z_t, z_0 = z_N = 0
+        if (threadIdx.x == 0) {
+            zbuf[0] = make_double2(0.0, 0.0);
+            zbuf[L / 2] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int t = 1 + threadIdx.x + 256 * u;
+            if (t <= npts) {
+                zbuf[t] = make_double2(xv[u], yv[u]);
+                zbuf[L - t] = make_double2(-xv[u], -yv[u]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int t = threadIdx.x + 256 * u;
+            if (t < npts) zbuf[t] = make_double2(xv[u], yv[u]);
+        }
+    }
+    __syncthreads();
+    fft_lds<MB>(zbuf, s, tw, -1, threadIdx.x);
+    double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
+    double* row1 = row0 + s.kp;
+    if (!g.cyclic) {
+        // DST(x)_K = -Im Z_K (row r0), DST(y)_K = Re Z_K (row r1)
+        for (int K = threadIdx.x + 1; K <= s.nin; K += blockDim.x) {
+            const double2 v = zbuf[K];
+            row0[K - 1] = -v.y;
+            if (has1) row1[K - 1] = v.x;
+        }
+    } else {
+        const int N = L;
+        for (int kk = threadIdx.x; kk <= N / 2; kk += blockDim.x) {
+            const double2 A = zbuf[kk], B = zbuf[(N - kk) % N];
+            // split the spectra of the two packed real rows
+            const double xr = 0.5 * (A.x + B.x), xi = 0.5 * (A.y - B.y);
+            const double yr = 0.5 * (A.y + B.y), yi = 0.5 * (B.x - A.x);
+            *(double2*)(row0 + 2 * kk) = make_double2(xr, xi);
+            if (has1) *(double2*)(row1 + 2 * kk) = make_double2(yr, yi);
+        }
+    }
+}
+
+// Instantiations over the layer count (2..8) and butterflies per thread (1, 2, 4).
+#define OQG_NL_MB_SWITCH(NLV, MBV, CALL)                                      \
+    switch ((NLV) * 8 + (MBV)) {                                              \
+        case 2 * 8 + 1: CALL(2, 1); break; case 2 * 8 + 2: CALL(2, 2); break; case 2 * 8 + 4: CALL(2, 4); break; \
+        case 3 * 8 + 1: CALL(3, 1); break; case 3 * 8 + 2: CALL(3, 2); break; case 3 * 8 + 4: CALL(3, 4); break; \
+        case 4 * 8 + 1: CALL(4, 1); break; case 4 * 8 + 2: CALL(4, 2); break; case 4 * 8 + 4: CALL(4, 4); break; \
+        case 5 * 8 + 1: CALL(5, 1); break; case 5 * 8 + 2: CALL(5, 2); break; case 5 * 8 + 4: CALL(5, 4); break; \
+        case 6 * 8 + 1: CALL(6, 1); break; case 6 * 8 + 2: CALL(6, 2); break; case 6 * 8 + 4: CALL(6, 4); break; \
+        case 7 * 8 + 1: CALL(7, 1); break; case 7 * 8 + 2: CALL(7, 2); break; case 7 * 8 + 4: CALL(7, 4); break; \
+        case 8 * 8 + 1: CALL(8, 1); break; case 8 * 8 + 2: CALL(8, 2); break; case 8 * 8 + 4: CALL(8, 4); break; \
+        default: break;                                                       \
+    }
+
+void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
+    const int npairs = (a.s.nyi + 1) / 2;
+    const size_t lds = sizeof(double2) * (size_t)a.s.L;
+#define FWD_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd<NLc, MBc>), dim3(npairs, nmodes), dim3(256), lds, st, a)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), FWD_CALL)
+#undef FWD_CALL
+}
+
+// ------------------------------------------------------------------ k_ysolve
+
+// a*u[j-1] + b*u[j] + a*u[j+1] = f[j], j = 0..nyi-1, u[-1] = u[nyi] = 0.
+// Lane l owns rows [l*R, l*R+R).  The last row of each lane's block is its
+// separator; the other rows are eliminated locally in terms of the two
+// neighbouring separators, and the separators' tridiagonal system is solved
+// by parallel cyclic reduction across the 64 lanes.
+template <int R>
+__device__ void wave_tridiag(double (&f)[R], int L, double aa, double b, int lane) {
+    double v[R], x[R], y[R], gam[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) { v[t] = 0.0; x[t] = 0.0; y[t] = 0.0; gam[t] = 0.0; }
+    if (L >= 2) {
+        double betinv = 1.0 / b;
+        v[0] = f[0] * betinv;
+        x[0] = (-aa) * betinv;
+        y[0] = (L == 2) ? (-aa) * betinv : 0.0;
+#pragma unroll
+        for (int t = 1; t < R - 1; t++) {
+            if (t <= L - 2) {
+                gam[t] = aa * betinv;
+                betinv = 1.0 / (b - aa * gam[t]);
+                v[t] = (f[t] - aa * v[t - 1]) * betinv;
+                x[t] = (0.0 - aa * x[t - 1]) * betinv;
+                y[t] = (((t == L - 2) ? -aa : 0.0) - aa * y[t - 1]) * betinv;
+            }
+        }
+#pragma unroll
+        for (int t = R - 3; t >= 0; t--) {
+            if (t <= L - 3) {
+                v[t] = v[t] - gam[t + 1] * v[t + 1];
+                x[t] = x[t] - gam[t + 1] * x[t + 1];
+                y[t] = y[t] - gam[t + 1] * y[t + 1];
+            }
+        }
+    }
+    // responses of the first / last interior row (v: particular part, x: per unit
+    // left separator, y: per unit own separator)
+    double vf = 0.0, xf = 0.0, yf = 0.0, vl = 0.0, xl = 0.0, yl = 0.0, fs = 0.0;
+    if (L >= 2) {
+        vf = v[0]; xf = x[0]; yf = y[0];
+#pragma unroll
+        for (int t = 0; t < R - 1; t++)
+            if (t == L - 2) { vl = v[t]; xl = x[t]; yl = y[t]; }
+    } else if (L == 1) {
+        yf = 1.0;   // the next block's "first interior row" is its separator
+        xl = 1.0;   // the own "last interior row" is the left separator
+    }
+#pragma unroll
+    for (int t = 0; t < R; t++)
+        if (t == L - 1) fs = f[t];
+    const double vfn = __shfl(vf, (lane + 1) & 63), xfn = __shfl(xf, (lane + 1) & 63), yfn = __shfl(yf, (lane + 1) & 63);
+    const bool nxt = (lane < 63);
+    double A, B, C, D;
+    if (L >= 1) {
+        A = (lane == 0) ? 0.0 : aa * xl;
+        B = b + aa * yl + (nxt ? aa * xfn : 0.0);
+        C = nxt ? aa * yfn : 0.0;
+        D = fs - aa * vl - (nxt ? aa * vfn : 0.0);
+    } else {
+        A = 0.0; B = 1.0; C = 0.0; D = 0.0;
+    }
+#pragma unroll
+    for (int st = 1; st < 64; st <<= 1) {
+        const int lm = lane - st, lp = lane + st;
+        const double Am = __shfl(A, lm & 63), Bm = __shfl(B, lm & 63), Cm = __shfl(C, lm & 63), Dm = __shfl(D, lm & 63);
+        const double Ap = __shfl(A, lp & 63), Bp = __shfl(B, lp & 63), Cp = __shfl(C, lp & 63), Dp = __shfl(D, lp & 63);
+        const bool hm = lm >= 0, hp = lp < 64;
+        const double k1 = hm ? A / Bm : 0.0;
+        const double k2 = hp ? C / Bp : 0.0;
+        const double An = hm ? -Am * k1 : 0.0;
+        const double Cn = hp ? -Cp * k2 : 0.0;
+        const double Bn = B - (hm ? Cm * k1 : 0.0) - (hp ? Ap * k2 : 0.0);
+        const double Dn = D - (hm ? Dm * k1 : 0.0) - (hp ? Dp * k2 : 0.0);
+        A = An; B = Bn; C = Cn; D = Dn;
+    }
+    const double S = D / B;
+    double Sm = __shfl(S, (lane - 1) & 63);
+    if (lane == 0) Sm = 0.0;
+#pragma unroll
+    for (int t = 0; t < R; t++)
+        if (t < L) f[t] = (t == L - 1) ? S : v[t] + x[t] * Sm + y[t] * S;
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * YS_CT) void k_ysolve(YArgs a) {
+    extern __shared__ double tile[];   // [YS_CT][P]
+    const Spec& s = a.s;
+    const int P = s.nypad + 1;
+    const int m = a.mode0 + blockIdx.y;
+    const int c0 = blockIdx.x * YS_CT;
+    double* S = a.spec + (long long)m * s.nrow * s.kp;
+    // nyi <= 64 R, so each of the 64*YS_CT threads loads at most R values: issue them together
+    double tmp[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        const int idx = threadIdx.x + 64 * YS_CT * t;
+        const int j = idx / YS_CT, c = idx - j * YS_CT;
+        const int col = c0 + c;
+        tmp[t] = (j < s.nyi && col < s.ncol) ? S[(long long)j * s.kp + col] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        const int idx = threadIdx.x + 64 * YS_CT * t;
+        const int j = idx / YS_CT, c = idx - j * YS_CT;
+        if (j < s.nyi) tile[c * P + j] = tmp[t];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
+    const int col = c0 + c;
+    if (col < s.ncol) {
+        const double b = a.bcol[m * s.ncol + col];
+        const int rs = lane * R;
+        int L = s.nyi - rs;
+        L = L < 0 ? 0 : (L > R ? R : L);
+        double f[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) f[t] = (t < L) ? tile[c * P + rs + t] : 0.0;
+        wave_tridiag<R>(f, L, s.a, b, lane);
+        double csum = 0.0;
+#pragma unroll
+        for (int t = 0; t < R; t++)
+            if (t < L) {
+                tile[c * P + rs + t] = f[t];
+                csum += f[t];
+            }
+        if (a.colsum) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) csum += __shfl_xor(csum, off);
+            if (lane == 0) a.colsum[m * s.ncol + col] = csum;
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < s.nyi * YS_CT; idx += blockDim.x) {
+        const int j = idx / YS_CT, cc = idx - j * YS_CT;
+        const int cl = c0 + cc;
+        if (cl < s.ncol) S[(long long)j * s.kp + cl] = tile[cc * P + j];
+    }
+}
+
+// Reference-order Thomas (Numerical Recipes form, inhomog.F90:188-199), one
+// thread per column walking down the rows; gam is kept in a scratch array of
+// the spectra's shape.
+__global__ __launch_bounds__(256) void k_ythomas(YArgs a) {
+    const Spec& s = a.s;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nmodes * s.ncol) return;
+    const int m = a.mode0 + c / s.ncol;
+    const int col = c - (c / s.ncol) * s.ncol;
+    double* u = a.spec + (long long)m * s.nrow * s.kp + col;
+    double* gam = a.gam + (long long)m * s.nrow * s.kp + col;
+    const double b = a.bcol[m * s.ncol + col];
+    const double aa = s.a;
+    double betinv = 1.0 / b;
+    double prev = u[0] * betinv;
+    u[0] = prev;
+    for (int j = 1; j < s.nyi; j++) {
+        const double gj = aa * betinv;
+        gam[(long long)j * s.kp] = gj;
+        betinv = 1.0 / (b - aa * gj);
+        prev = (u[(long long)j * s.kp] - aa * prev) * betinv;
+        u[(long long)j * s.kp] = prev;
+    }
+    double csum = prev;
+    for (int j = s.nyi - 2; j >= 0; j--) {
+        prev = u[(long long)j * s.kp] - gam[(long long)(j + 1) * s.kp] * prev;
+        u[(long long)j * s.kp] = prev;
+        csum += prev;
+    }
+    if (a.colsum) a.colsum[m * s.ncol + col] = csum;
+}
+
+void launch_ysolve(const YArgs& a, int variant, hipStream_t st) {
+    const int ncols = a.nmodes * a.s.ncol;
+    if (variant == 1) {
+        hipLaunchKernelGGL(k_ythomas, dim3((ncols + 255) / 256), dim3(256), 0, st, a);
+        return;
+    }
+    const dim3 grid((a.s.ncol + YS_CT - 1) / YS_CT, a.nmodes);
+    const size_t lds = sizeof(double) * YS_CT * (size_t)(a.s.nypad + 1);
+    switch (a.s.R) {
+#define YCASE(RR) case RR: hipLaunchKernelGGL(k_ysolve<RR>, grid, dim3(64 * YS_CT), lds, st, a); break;
+        YCASE(1) YCASE(2) YCASE(3) YCASE(4) YCASE(5) YCASE(6) YCASE(7) YCASE(8)
+        YCASE(9) YCASE(10) YCASE(11) YCASE(12) YCASE(13) YCASE(14) YCASE(15) YCASE(16)
+#undef YCASE
+        default: break;
+    }
+}
+
+// ------------------------------------------------------------------ k_inv
+
+// LU solve of the (nl-1)^2 mass-constraint system with DGERFS-style iterative
+// refinement (homog.F90:356-359, linalg.F90:57-93); N = nl-1 is a compile-time
+// size so the vectors stay in registers.
+template <int N>
+__device__ void small_lu_solve(const BoxHom& h, const double (&rhs)[N], double (&x)[N]) {
+    auto lusolve = [&](double (&v)[N]) {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const int p = h.ipiv[k];
+#pragma unroll
+            for (int q = k + 1; q < N; q++)
+                if (q == p) { const double t = v[k]; v[k] = v[q]; v[q] = t; }
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++)
+#pragma unroll
+            for (int j = 0; j < i; j++) v[i] -= h.LU[i * MAXL + j] * v[j];
+#pragma unroll
+        for (int i = N - 1; i >= 0; i--) {
+#pragma unroll
+            for (int j = i + 1; j < N; j++) v[i] -= h.LU[i * MAXL + j] * v[j];
+            v[i] /= h.LU[i * MAXL + i];
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < N; i++) x[i] = rhs[i];
+    lusolve(x);
+    double lstres = 3.0;
+    const double eps = 1.1102230246251565e-16;
+    for (int it = 0; it < 5; it++) {
+        double r[N], berr = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            double sres = rhs[i], den = fabs(rhs[i]);
+#pragma unroll
+            for (int j = 0; j < N; j++) {
+                sres -= h.cdhoc[i * MAXL + j] * x[j];
+                den += fabs(h.cdhoc[i * MAXL + j]) * fabs(x[j]);
+            }
+            r[i] = sres;
+            if (den > 0.0 && fabs(sres) / den > berr) berr = fabs(sres) / den;
+        }
+        if (!(berr > eps && 2.0 * berr <= lstres)) break;
+        lusolve(r);
+#pragma unroll
+        for (int i = 0; i < N; i++) x[i] += r[i];
+        lstres = berr;
+    }
+}
+
+// Per-step correction coefficients, computed by thread 0 of every k_inv block
+// from the block-reduced integrals v[0..nl-1] (box: spectral column sums) and
+// v[MAXL] (entrainment).  Writes coef[] and, in block 0, the new constraint state.
+template <int NL>
+__device__ void step_coefficients(const InvArgs& a, const double (&v)[MAXL + 1], double* coef, bool write_state) {
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const double* ci = a.cin;
+    double dpi[NL], dpip[NL], cs[NL], cn[NL], csp[NL], cnp[NL];
+#pragma unroll
+    for (int l = 0; l < NL; l++) {
+        dpi[l] = ci[C_DPI + l]; dpip[l] = ci[C_DPIP + l];
+        cs[l] = ci[C_CS + l]; cn[l] = ci[C_CN + l]; csp[l] = ci[C_CSP + l]; cnp[l] = ci[C_CNP + l];
+    }
+    // update_mass_constr (constraint.F90:167-179) with ent_xn(1) = int_P_dA(ent1)
+    const double exn = (v[MAXL] * g.dx) * g.dy;
+#pragma unroll
+    for (int l = 0; l < NL - 1; l++) {
+        const double e = (l == 0) ? exn : 0.0;
+        const double old = dpi[l];
+        dpi[l] = dpip[l] - a.tdt * a.ph.gp[l] * e;
+        dpip[l] = old;
+    }
+    if (!g.cyclic) {
+        // box_homog (homog.F90:345-364); modal area integrals from the spectral column sums
+        double im[NL], ik[NL], rhs[NL - 1], alpha[NL - 1];
+#pragma unroll
+        for (int m = 0; m < NL; m++) im[m] = (v[m] * g.dx) * g.dy;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            double t = 0.0;
+#pragma unroll
+            for (int m = 0; m < NL; m++) t += a.mx->ctm2l[m * MAXL + k] * im[m];
+            ik[k] = t;
+        }
+#pragma unroll
+        for (int l = 0; l < NL - 1; l++) rhs[l] = dpi[l] - (ik[l + 1] - ik[l]);
+        small_lu_solve<NL - 1>(*a.bh, rhs, alpha);
+#pragma unroll
+        for (int l = 0; l < NL - 1; l++) coef[l] = alpha[l];
+    } else {
+        // step_constraints (constraint.F90:182-215)
+        const double* x = a.aux;
+        const double f0 = g.fnot, dy = g.dy;
+        double rhss[NL], rhsn[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            rhss[k] = -0.5 * dy * f0 * x[A_BES + k];
+            rhsn[k] = -0.5 * dy * f0 * x[A_BEN + k];
+        }
+        rhss[0] = rhss[0] + a.ph.tau_sign * (f0 / a.ph.h[0]) * x[A_TXIS];
+        rhsn[0] = rhsn[0] - a.ph.tau_sign * (f0 / a.ph.h[0]) * x[A_TXIN];
+        rhss[NL - 1] = rhss[NL - 1] + (f0 / a.ph.h[NL - 1]) * x[A_BDRS];
+        rhsn[NL - 1] = rhsn[NL - 1] - (f0 / a.ph.h[NL - 1]) * x[A_BDRN];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            rhss[k] = rhss[k] + x[A_AJIS + k] - x[A_AP3S + k] + x[A_AP5S + k];
+            rhsn[k] = rhsn[k] + x[A_AJIN + k] + x[A_AP3N + k] - x[A_AP5N + k];
+        }
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const double csnew = csp[k] + a.tdt * rhss[k];
+            const double cnnew = cnp[k] + a.tdt * rhsn[k];
+            csp[k] = cs[k];
+            cnp[k] = cn[k];
+            cs[k] = csnew;
+            cn[k] = cnnew;
+        }
+        // cyclic_homog (homog.F90:305-320); row integrals from the k = 0 spectra
+        double clhss[NL], clhsn[NL];
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double ts = 0.0, tn = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                ts += a.mx->ctl2m[k * MAXL + m] * cs[k];
+                tn += a.mx->ctl2m[k * MAXL + m] * cn[k];
+            }
+            const double* sp = a.spec + (long long)m * s.nrow * s.kp;
+            clhss[m] = ts + (a.ch->c0 * sp[0]) / dy;
+            clhsn[m] = tn + (a.ch->c0 * sp[(long long)(s.nyi - 1) * s.kp]) / dy;
+        }
+        coef[0] = clhss[0] * a.ch->hbsi;
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            coef[1 + m] = a.ch->hc2n[m] * clhss[m] - a.ch->hc2s[m] * clhsn[m];
+            coef[1 + MAXL + m] = a.ch->hc1s[m] * clhsn[m] - a.ch->hc1n[m] * clhss[m];
+        }
+    }
+    if (write_state) {
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            a.cout[C_DPI + l] = (l < NL - 1) ? dpi[l] : 0.0;
+            a.cout[C_DPIP + l] = (l < NL - 1) ? dpip[l] : 0.0;
+            a.cout[C_CS + l] = cs[l]; a.cout[C_CN + l] = cn[l];
+            a.cout[C_CSP + l] = csp[l]; a.cout[C_CNP + l] = cnp[l];
+        }
+    }
+}
+
+// p(i,gj,:) = sum_m (inhomog + homcor)(i,gj,m) ctm2l(m,:)  (modes.F90:268-284).
+// sel: 0/1 interior row from LDS, 2: boundary row (inhomog = 0).
+// hpre: the box homogeneous solutions at (i,gj) when already loaded (use_pre).
+template <int NL>
+__device__ __forceinline__ void pmix(const InvArgs& a, const double* inh, const double* coef,
+                                     const double* sctm2l, int i, int sel, int gj, double (&pk)[NL],
+                                     const double (&hpre)[NL > 1 ? NL - 1 : 1], bool use_pre) {
+    const Geo& g = a.g;
+    double vm[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) {
+        const double iv = (sel == 2) ? 0.0 : inh[(m * 2 + sel) * g.pitch + i];
+        double h;
+        if (!g.cyclic) {
+            h = (m == 0) ? 0.0
+                         : coef[m - 1] * (use_pre ? hpre[m - 1]
+                                                  : a.hom[(long long)(m - 1) * g.fs + (long long)gj * g.pitch + i]);
+        } else if (m == 0) {
+            h = coef[0] * a.hom[gj];
+        } else {
+            const double* b1 = a.hom + g.nyp;
+            const double* b2 = b1 + (long long)(NL - 1) * g.nyp;
+            h = coef[1 + m] * b1[(m - 1) * g.nyp + gj] + coef[1 + MAXL + m] * b2[(m - 1) * g.nyp + gj];
+        }
+        vm[m] = iv + h;
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < NL; m++) t += vm[m] * sctm2l[m * MAXL + k];
+        pk[k] = t;
+    }
+}
+
+// ocqbdy (vorsubs.F90:58-114) at one boundary point: kind 0 zonal row, 1 box
+// meridional column, 2 box corner (x difference, the row and column terms twice).
+template <int NL>
+__device__ __forceinline__ void qbdy(const InvArgs& a, const double* samat, int i, int gj, int kind,
+                                     const double (&pb)[NL], const double (&pin)[NL]) {
+    const Geo& g = a.g;
+    const long long o = (long long)gj * g.pitch + i;
+    const int reps = (kind == 2) ? 2 : 1;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double qv = a.ph.qbfac * (pin[k] - pb[k]);
+        double ap = 0.0;
+#pragma unroll
+        for (int l = 0; l < NL; l++) ap += samat[k * MAXL + l] * pb[l];
+        for (int r = 0; r < reps; r++) qv = qv - g.fnot * ap;
+        for (int r = 0; r < reps; r++) qv = qv + g.beta * a.yprel[gj];
+        if (k == g.kt && a.ddyn)
+            for (int r = 0; r < reps; r++) qv = qv + a.ddyn[o];
+        a.qnew[k * g.fs + o] = qv;
+    }
+}
+
+// Transforms in flight per k_inv block: all modes for nl <= 3, else two at a time
+// (keeps the register budget of a 1024-thread block out of the way).
+constexpr int inv_group_count(int nm) { return nm <= 3 ? nm : 2; }
+
+template <int NL, int MB>
+__global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs a) {
+    extern __shared__ double smem[];
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L;
+    const bool raw = a.raw_mode >= 0;
+    const int nm = raw ? 1 : NL;
+    const int G = blockDim.x / FFT_GROUP;          // transforms in flight
+    const int grp = threadIdx.x / FFT_GROUP, gt = threadIdx.x - grp * FFT_GROUP;
+    double2* fbuf = (double2*)smem;                 // [G][L]
+    double* inh = (double*)(fbuf + (size_t)G * L);  // [nm][2][pitch]
+    __shared__ double red[(MAXL + 1) * 16];
+    __shared__ double coef[3 * MAXL + 4];
+    __shared__ double s_ctm2l[MAXL * MAXL];
+    __shared__ double s_amat[MAXL * MAXL];
+    for (int t = threadIdx.x; t < MAXL * MAXL; t += blockDim.x) {
+        s_ctm2l[t] = a.mx->ctm2l[t];
+        s_amat[t] = a.mx->amat[t];
+    }
+    const int jp = blockIdx.x;
+    const int r0 = 2 * jp, r1 = 2 * jp + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    const bool first = (jp == 0), last = (jp == (int)gridDim.x - 1);
+    const double2* tw = (const double2*)a.tw;
+    const int nsel = has1 ? 2 : 1;
+
+    // ---- issue the block's global loads together: the integrals behind the
+    // correction coefficients, the spectra of the first round of modes, and the
+    // homogeneous solutions under this thread's first two output points
+    double v[MAXL + 1];
+#pragma unroll
+    for (int t = 0; t <= MAXL; t++) v[t] = 0.0;
+    double entv[2] = {0.0, 0.0}, colv[2][NL];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int t = threadIdx.x + blockDim.x * u;
+        if (!raw && t < a.nentpart) entv[u] = a.entpart[t];
+#pragma unroll
+        for (int m = 0; m < NL; m++)
+            colv[u][m] = (!raw && !g.cyclic && t < s.ncol) ? a.colw[t] * a.colsum[m * s.ncol + t] : 0.0;
+    }
+    double2 px[4], py[4];
+    {
+        const int m = raw ? a.raw_mode : grp;
+        const bool ok = grp < nm;
+        const double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
+        const double* row1 = row0 + s.kp;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            px[u] = make_double2(0.0, 0.0);
+            py[u] = make_double2(0.0, 0.0);
+            const int t = gt + FFT_GROUP * u;
+            if (!ok) continue;
+            if (!g.cyclic) {
+                if (t < s.nin) px[u] = make_double2(row0[t], has1 ? row1[t] : 0.0);
+            } else if (t <= L / 2) {
+                px[u] = *(const double2*)(row0 + 2 * t);
+                if (has1) py[u] = *(const double2*)(row1 + 2 * t);
+            }
+        }
+    }
+    double hv[2][NL > 1 ? NL - 1 : 1];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int idx = threadIdx.x + blockDim.x * u;
+        const int sel = idx / g.nxp, i = idx - sel * g.nxp;
+        const int gj = sel == 0 ? gj0 : gj1;
+#pragma unroll
+        for (int m = 1; m < NL; m++)
+            hv[u][m - 1] = (!raw && !g.cyclic && idx < nsel * g.nxp)
+                               ? a.hom[(long long)(m - 1) * g.fs + (long long)gj * g.pitch + i] : 0.0;
+    }
+
+    // ---- correction coefficients (identical in every block)
+    if (!raw) {
+        v[MAXL] = entv[0] + entv[1];
+        for (int t = threadIdx.x + 2 * blockDim.x; t < a.nentpart; t += blockDim.x) v[MAXL] += a.entpart[t];
+        if (!g.cyclic) {
+#pragma unroll
+            for (int m = 0; m < NL; m++) v[m] = colv[0][m] + colv[1][m];
+            for (int col = threadIdx.x + 2 * blockDim.x; col < s.ncol; col += blockDim.x) {
+                const double w = a.colw[col];
+#pragma unroll
+                for (int m = 0; m < NL; m++) v[m] += w * a.colsum[m * s.ncol + col];
+            }
+        }
+        block_sum<MAXL + 1>(v, red);
+        if (threadIdx.x == 0) step_coefficients<NL>(a, v, coef, blockIdx.x == 0);
+    }
+    __syncthreads();
+
+    // ---- inverse x transforms, G modes at a time
+    const double fn = s.ftnorm;
+    for (int m0 = 0; m0 < nm; m0 += G) {
+        const int mi = m0 + grp;
+        const bool valid = mi < nm;
+        const int m = raw ? a.raw_mode : mi;
+        double2* z = fbuf + (size_t)grp * L;
+        if (valid && m0 > 0) {
+            // later rounds (nl > 3) load their spectra here
+            const double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
+            const double* row1 = row0 + s.kp;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = gt + FFT_GROUP * u;
+                px[u] = make_double2(0.0, 0.0);
+                py[u] = make_double2(0.0, 0.0);
+                if (!g.cyclic) {
+                    if (t < s.nin) px[u] = make_double2(row0[t], has1 ? row1[t] : 0.0);
+                } else if (t <= L / 2) {
+                    px[u] = *(const double2*)(row0 + 2 * t);
+                    if (has1) py[u] = *(const double2*)(row1 + 2 * t);
+                }
+            }
+        }
+        if (valid) {
+            if (!g.cyclic) {
+                // odd extension of ftnorm*u for the two rows (inhomog.F90:202-203)
+                if (gt == 0) {
+                    z[0] = make_double2(0.0, 0.0);
+                    z[L / 2] = make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int t = gt + FFT_GROUP * u;   // spectral index K-1
+                    if (t < s.nin) {
+                        const double2 w = make_double2(fn * px[u].x, fn * px[u].y);
+                        z[t + 1] = w;
+                        z[L - t - 1] = make_double2(-w.x, -w.y);
+                    }
+                }
+            } else {
+                // Hermitian spectra of the two packed rows (inhomog.F90:246-247)
+                const int N = L, h = N / 2;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int kk = gt + FFT_GROUP * u;
+                    if (kk <= h) {
+                        double xr = fn * px[u].x, xi = fn * px[u].y, yr = fn * py[u].x, yi = fn * py[u].y;
+                        if (kk == 0 || (N % 2 == 0 && kk == h)) { xi = 0.0; yi = 0.0; }
+                        z[kk] = make_double2(xr - yi, xi + yr);
+                        if (kk != 0 && !(N % 2 == 0 && kk == h)) z[N - kk] = make_double2(xr + yi, -xi + yr);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        fft_lds<MB>(z, s, tw, g.cyclic ? +1 : -1, gt);
+        if (valid) {
+            double* o0 = inh + (size_t)mi * 2 * g.pitch;
+            double* o1 = o0 + g.pitch;
+            for (int i = gt; i < g.nxp; i += FFT_GROUP) {
+                double x0 = 0.0, x1 = 0.0;
+                if (!g.cyclic) {
+                    if (i >= 1 && i <= s.nin) { const double2 v = z[i]; x0 = -v.y; x1 = v.x; }
+                } else {
+                    const double2 v = z[(i == g.nxp - 1) ? 0 : i];
+                    x0 = v.x;
+                    x1 = v.y;
+                }
+                o0[i] = x0;
+                o1[i] = x1;
+            }
+        }
+        __syncthreads();
+    }
+
+    if (raw) {
+        double* o = a.out_raw;
+        for (int i = threadIdx.x; i < g.nxp; i += blockDim.x) {
+            o[(long long)gj0 * g.pitch + i] = inh[i];
+            if (has1) o[(long long)gj1 * g.pitch + i] = inh[g.pitch + i];
+            if (first) o[i] = 0.0;
+            if (last) o[(long long)(g.nyp - 1) * g.pitch + i] = 0.0;
+        }
+        return;
+    }
+
+    // ---- p_new on the interior rows, boundary q on their meridional ends
+    double hnone[NL > 1 ? NL - 1 : 1];
+#pragma unroll
+    for (int m = 0; m < (NL > 1 ? NL - 1 : 1); m++) hnone[m] = 0.0;
+    auto interior_point = [&](int idx, const double (&hp)[NL > 1 ? NL - 1 : 1], bool use_pre) {
+        const int sel = idx / g.nxp, i = idx - sel * g.nxp;
+        const int gj = sel == 0 ? gj0 : gj1;
+        double pk[NL];
+        pmix<NL>(a, inh, coef, s_ctm2l, i, sel, gj, pk, hp, use_pre);
+        const long long o = (long long)gj * g.pitch + i;
+#pragma unroll
+        for (int k = 0; k < NL; k++) a.pnew[k * g.fs + o] = pk[k];
+        if (!g.cyclic && (i == 0 || i == g.nxp - 1)) {
+            double pin[NL];
+            pmix<NL>(a, inh, coef, s_ctm2l, i == 0 ? 1 : g.nxp - 2, sel, gj, pin, hnone, false);
+            qbdy<NL>(a, s_amat, i, gj, 1, pk, pin);
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int idx = threadIdx.x + blockDim.x * u;
+        if (idx < nsel * g.nxp) interior_point(idx, hv[u], true);
+    }
+    for (int idx = threadIdx.x + 2 * blockDim.x; idx < nsel * g.nxp; idx += blockDim.x)
+        interior_point(idx, hnone, false);
+    // ---- zonal boundary rows j = 0 (first block) and j = nyp-1 (last block)
+    for (int side = 0; side < 2; side++) {
+        if ((side == 0 && !first) || (side == 1 && !last)) continue;
+        const int gj = side == 0 ? 0 : g.nyp - 1;
+        const int gin = side == 0 ? 1 : g.nyp - 2;
+        const int selin = side == 0 ? 0 : ((gin == gj1 && has1) ? 1 : 0);
+        for (int i = threadIdx.x; i < g.nxp; i += blockDim.x) {
+            double pb[NL], pin[NL];
+            pmix<NL>(a, inh, coef, s_ctm2l, i, 2, gj, pb, hnone, false);
+            const long long o = (long long)gj * g.pitch + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++) a.pnew[k * g.fs + o] = pb[k];
+            if (!g.cyclic && (i == 0 || i == g.nxp - 1)) {
+                // corners take the x difference (vorsubs.F90:83-84 overwrite :79-80)
+                pmix<NL>(a, inh, coef, s_ctm2l, i == 0 ? 1 : g.nxp - 2, 2, gj, pin, hnone, false);
+                qbdy<NL>(a, s_amat, i, gj, 2, pb, pin);
+            } else {
+                pmix<NL>(a, inh, coef, s_ctm2l, i, selin, gin, pin, hnone, false);
+                qbdy<NL>(a, s_amat, i, gj, 0, pb, pin);
+            }
+        }
+    }
+}
+
+int inv_block_count(const Spec& s) { return (s.nyi + 1) / 2; }
+
+static int inv_groups(const InvArgs& a) { return inv_group_count((a.raw_mode >= 0) ? 1 : a.g.nl); }
+
+size_t inv_lds_bytes(const Spec& s, const Geo& g, int nm) {
+    return sizeof(double2) * (size_t)inv_group_count(nm) * s.L + sizeof(double) * 2 * (size_t)nm * g.pitch;
+}
+
+void launch_inv(const InvArgs& a, hipStream_t st) {
+    const int nm = (a.raw_mode >= 0) ? 1 : a.g.nl;
+    const int G = inv_groups(a);
+    const dim3 grid(inv_block_count(a.s)), block(FFT_GROUP * G);
+    const size_t lds = inv_lds_bytes(a.s, a.g, nm);
+#define INV_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv<NLc, MBc>), grid, block, lds, st, a)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), INV_CALL)
+#undef INV_CALL
+}
+
+// Allow the transforms of this basin more than the default 64 KB of dynamic LDS.
+void set_kernel_attrs(const Spec& s, const Geo& g) {
+    const int fl = (int)(sizeof(double2) * (size_t)s.L), il = (int)inv_lds_bytes(s, g, g.nl);
+#define ATTR_CALL(NLc, MBc)                                                                                       \
+    do {                                                                                                          \
+        (void)hipFuncSetAttribute((const void*)k_fwd<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, fl); \
+        (void)hipFuncSetAttribute((const void*)k_inv<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, il); \
+    } while (0)
+    OQG_NL_MB_SWITCH(g.nl, fft_mb(s), ATTR_CALL)
+#undef ATTR_CALL
+}
+
+}  // namespace oqg
