@@ -154,12 +154,15 @@ void* oqg_get_stream(oqg_ctx* ctx);
 int oqg_state_ptrs(oqg_ctx* ctx, double** q, double** qm, double** p, double** pm,
                    size_t* pitch);
 
-/* Advances the basin by nsteps steps of step_qg on the resident forcing,
- * recording HIP events on the context's stream around each kernel, and
- * returns in ms[] the average duration of each kernel of the step (names in
- * `names`, '\n' separated; a kernel a basin does not use reports 0).
- * Returns the number of kernels reported (<= max) or a negative error. */
-int oqg_kernel_times(oqg_ctx* ctx, double tdt, int nsteps, double* ms, int max, char* names,
+/* Advances the basin by one step of step_qg on the resident forcing (the same
+ * result as oqg_step) and returns in ms[] the average duration of each kernel
+ * of that step, measured with two HIP events on the context's stream around
+ * `reps` (1..256) back-to-back launches of the kernel on the step's inputs
+ * (names in `names`, '\n' separated; a kernel a basin does not use reports
+ * 0).  Allocates 2*reps temporary copies of a field and of the spectra while
+ * it runs.  Returns the number of kernels reported (<= max) or a negative
+ * error. */
+int oqg_kernel_times(oqg_ctx* ctx, double tdt, int reps, double* ms, int max, char* names,
                      int names_len);
 
 #ifdef __cplusplus

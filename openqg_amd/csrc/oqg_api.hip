@@ -68,7 +68,7 @@ struct oqg_ctx {
     DevConsts* dconst = nullptr;
     std::vector<void*> allocs;
     bool has_ent2 = false;
-    double txis = 0.0, txin = 0.0;
+    double* fcon = nullptr;   // txis, txin, layer-2 entrainment flag (device)
     hipStream_t own = nullptr;
     hipStream_t st = nullptr;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -373,12 +373,19 @@ static int enqueue_raw_solve(oqg_ctx* c, int m) {
 enum { KV = 0, KC, KF, KY, KI, NK };
 static const char* kNames[NK] = {"k_vort", "k_cyc", "k_fwd", "k_ysolve", "k_inv"};
 
-// One step_qg on resident forcing.  ev (optional) brackets each kernel.
-static int enqueue_step(oqg_ctx* c, double tdt, hipEvent_t* ev) {
+// Arguments of the five kernels of one step_qg from the current time level.
+struct StepArgs {
+    VortArgs va;
+    CycArgs ca;
+    FwdArgs fa;
+    YArgs ya;
+    InvArgs ia;
+};
+
+static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     const int cur = c->cur, nxt = 1 - cur;
-    int e = 0;
-    auto mark = [&]() { if (ev) hipEventRecord(ev[e++], c->st); };
-    VortArgs va{};
+    std::memset(&a, 0, sizeof a);
+    VortArgs& va = a.va;
     va.g = c->g;
     va.ph = c->ph;
     va.q = c->qbuf[cur];
@@ -391,26 +398,18 @@ static int enqueue_step(oqg_ctx* c, double tdt, hipEvent_t* ev) {
     va.tdt = tdt;
     va.tiles_x = c->tiles_x;
     va.tiles_y = c->tiles_y;
-    mark();
-    launch_vort(va, c->st);
-    mark();
-    if (c->g.cyclic) {
-        CycArgs ca{};
-        ca.g = c->g;
-        ca.ph = c->ph;
-        ca.q = c->qbuf[cur];
-        ca.p = c->pbuf[cur];
-        ca.pm = c->pbuf[nxt];
-        ca.ent = c->ent1;
-        ca.ent2s = c->has_ent2 ? c->ent2s : nullptr;
-        ca.ent2n = c->has_ent2 ? c->ent2n : nullptr;
-        ca.aux = c->aux;
-        ca.txis = c->txis;
-        ca.txin = c->txin;
-        launch_cyc(ca, c->st);
-    }
-    mark();
-    FwdArgs fa{};
+    CycArgs& ca = a.ca;
+    ca.g = c->g;
+    ca.ph = c->ph;
+    ca.q = c->qbuf[cur];
+    ca.p = c->pbuf[cur];
+    ca.pm = c->pbuf[nxt];
+    ca.ent = c->ent1;
+    ca.ent2s = c->ent2s;
+    ca.ent2n = c->ent2n;
+    ca.aux = c->aux;
+    ca.fcon = c->fcon;
+    FwdArgs& fa = a.fa;
     fa.g = c->g;
     fa.s = c->s;
     fa.mx = c->mx;
@@ -420,9 +419,7 @@ static int enqueue_step(oqg_ctx* c, double tdt, hipEvent_t* ev) {
     fa.tw = c->tw;
     fa.spec = c->spec;
     fa.raw_mode = -1;
-    launch_fwd(fa, c->g.nl, c->st);
-    mark();
-    YArgs ya{};
+    YArgs& ya = a.ya;
     ya.s = c->s;
     ya.spec = c->spec;
     ya.bcol = c->bcol;
@@ -430,19 +427,32 @@ static int enqueue_step(oqg_ctx* c, double tdt, hipEvent_t* ev) {
     ya.gam = c->gamw;
     ya.mode0 = 0;
     ya.nmodes = c->g.nl;
-    launch_ysolve(ya, c->variant, c->st);
-    mark();
-    InvArgs ia;
+    InvArgs& ia = a.ia;
     fill_inv_common(c, ia);
     ia.cin = c->cbuf[cur];
     ia.cout = c->cbuf[nxt];
     ia.pnew = c->pbuf[nxt];
     ia.qnew = c->qbuf[nxt];
     ia.tdt = tdt;
-    launch_inv(ia, c->st);
-    mark();
+}
+
+static void launch_kernel(oqg_ctx* c, const StepArgs& a, int k) {
+    switch (k) {
+        case KV: launch_vort(a.va, c->st); break;
+        case KC: if (c->g.cyclic) launch_cyc(a.ca, c->st); break;
+        case KF: launch_fwd(a.fa, c->g.nl, c->st); break;
+        case KY: launch_ysolve(a.ya, c->variant, c->st); break;
+        default: launch_inv(a.ia, c->st); break;
+    }
+}
+
+// One step_qg on resident forcing.
+static int enqueue_step(oqg_ctx* c, double tdt) {
+    StepArgs a;
+    make_step_args(c, tdt, a);
+    for (int k = 0; k < NK; k++) launch_kernel(c, a, k);
     HIPCHK(c, hipGetLastError());
-    c->cur = nxt;
+    c->cur = 1 - c->cur;
     return 0;
 }
 
@@ -577,6 +587,7 @@ static int setup(oqg_ctx* c) {
     if (!rc) rc = dalloc(c, &c->colw, (size_t)s.ncol);
     if (!rc) rc = dalloc(c, &c->entpart, (size_t)c->tiles_x * c->tiles_y);
     if (!rc) rc = dalloc(c, &c->aux, A_LEN);
+    if (!rc) rc = dalloc(c, &c->fcon, F_LEN);
     if (!rc) rc = dalloc(c, &c->ent1, f1);
     if (!rc) rc = dalloc(c, &c->ent2s, (size_t)nxp);
     if (!rc) rc = dalloc(c, &c->ent2n, (size_t)nxp);
@@ -956,8 +967,11 @@ int oqg_set_forcing(oqg_ctx* c, const double* ent, int n_ent, const double* wekp
     }
     if (wekp)
         if (int r = upload_field(c, c->wekp, wekp, 1, kind)) return r;
-    c->txis = txis;
-    c->txin = txin;
+    double f[F_LEN] = {};
+    f[F_TXIS] = txis;
+    f[F_TXIN] = txin;
+    f[F_ENT2] = c->has_ent2 ? 1.0 : 0.0;
+    HIPCHK(c, hipMemcpyAsync(c->fcon, f, sizeof f, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
@@ -968,7 +982,7 @@ int oqg_step(oqg_ctx* c, double tdt, const double* ent, int n_ent, const double*
         return fail(c, OQG_ERR_ARG, "pass both ent and wekp, or neither to use the resident forcing");
     if (ent)
         if (int r = oqg_set_forcing(c, ent, n_ent, wekp, txis, txin, OQG_HOST)) return r;
-    return enqueue_step(c, tdt, nullptr);
+    return enqueue_step(c, tdt);
 }
 
 int oqg_suppress(oqg_ctx* c) {
@@ -991,7 +1005,7 @@ int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
                 HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
                 int rc = 0;
                 for (int t = 0; t < P && !rc; t++) {
-                    rc = enqueue_step(c, tdt, nullptr);
+                    rc = enqueue_step(c, tdt);
                     if (!rc && supp > 0 && (t % supp) == 0) rc = enqueue_suppress(c);
                 }
                 hipError_t ce = hipStreamEndCapture(c->st, &gr);
@@ -1008,7 +1022,7 @@ int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
             // P is even, so the time-level parity is unchanged
             nt += P;
         } else {
-            if (int r = enqueue_step(c, tdt, nullptr)) return r;
+            if (int r = enqueue_step(c, tdt)) return r;
             if (supp > 0 && (nt - 1) % supp == 0)
                 if (int r = enqueue_suppress(c)) return r;
             nt++;
@@ -1067,32 +1081,64 @@ int oqg_state_ptrs(oqg_ctx* c, double** q, double** qm, double** p, double** pm,
     return 0;
 }
 
-static int time_kernels(oqg_ctx* c, double tdt, int nsteps, hipEvent_t* ev, double* acc) {
-    for (int s = 0; s < nsteps; s++) {
-        if (int r = enqueue_step(c, tdt, ev)) return r;
-        HIPCHK(c, hipEventSynchronize(ev[NK]));
-        for (int k = 0; k < NK; k++) {
-            float t = 0.f;
-            HIPCHK(c, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-            acc[k] += t;
-        }
+// Times each kernel of one step over `reps` back-to-back launches between two
+// events, then launches it once more on the real buffers so the next kernel
+// sees the inputs of a normal step.  The in-place kernels (k_vort writes q_new
+// over qm, the y solve overwrites the spectra) run each timed launch on its own
+// copy of that operand, so every timed launch does the work of a real step.
+static int time_kernels(oqg_ctx* c, double tdt, int reps, hipEvent_t* ev, double* ms,
+                        std::vector<void*>& tmp) {
+    const size_t sbytes = (size_t)c->g.nl * c->s.nrow * c->s.kp * sizeof(double);
+    for (int r = 0; r < 2 * reps; r++) {
+        void* p = nullptr;
+        HIPCHK(c, hipMalloc(&p, r < reps ? c->fbytes : sbytes));
+        tmp.push_back(p);
     }
+    StepArgs a;
+    make_step_args(c, tdt, a);
+    for (int k = 0; k < NK; k++) {
+        ms[k] = 0.0;
+        if (k == KC && !c->g.cyclic) continue;
+        for (int r = 0; r < reps; r++) {
+            if (k == KV) HIPCHK(c, hipMemcpyAsync(tmp[r], a.va.qnew, c->fbytes, hipMemcpyDeviceToDevice, c->st));
+            if (k == KY) HIPCHK(c, hipMemcpyAsync(tmp[reps + r], a.ya.spec, sbytes, hipMemcpyDeviceToDevice, c->st));
+        }
+        HIPCHK(c, hipEventRecord(ev[0], c->st));
+        for (int r = 0; r < reps; r++) {
+            StepArgs b = a;
+            b.va.qnew = static_cast<double*>(tmp[r]);
+            b.ya.spec = static_cast<double*>(tmp[reps + r]);
+            launch_kernel(c, (k == KV || k == KY) ? b : a, k);
+        }
+        HIPCHK(c, hipEventRecord(ev[1], c->st));
+        launch_kernel(c, a, k);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventSynchronize(ev[1]));
+        float t = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[1]));
+        ms[k] = (double)t / reps;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->cur = 1 - c->cur;
     return 0;
 }
 
-int oqg_kernel_times(oqg_ctx* c, double tdt, int nsteps, double* ms, int max, char* names, int names_len) {
-    if (!c || nsteps < 1 || !ms) return -OQG_ERR_ARG;
-    hipEvent_t ev[NK + 1];
+int oqg_kernel_times(oqg_ctx* c, double tdt, int reps, double* ms, int max, char* names, int names_len) {
+    if (!c || reps < 1 || reps > 256 || !ms) return -OQG_ERR_ARG;
+    hipEvent_t ev[2];
     for (auto& e : ev)
         if (hipEventCreate(&e) != hipSuccess) return -fail(c, OQG_ERR_HIP, "hipEventCreate failed");
     double acc[NK] = {};
-    const int rc = time_kernels(c, tdt, nsteps, ev, acc);
+    std::vector<void*> tmp;
+    const int rc = time_kernels(c, tdt, reps, ev, acc, tmp);
+    hipStreamSynchronize(c->st);
+    for (void* p : tmp) hipFree(p);
     for (auto& e : ev) hipEventDestroy(e);
     if (rc) return -rc;
     const int n = std::min(max, (int)NK);
     std::string nm;
     for (int k = 0; k < n; k++) {
-        ms[k] = acc[k] / nsteps;
+        ms[k] = acc[k];
         nm += kNames[k];
         nm += '\n';
     }

@@ -160,11 +160,14 @@ struct CycArgs {
     const double* p;
     const double* pm;
     const double* ent;      // layer 1 (pitch layout)
-    const double* ent2s;    // layer-2 row 1 (nxp) or null
-    const double* ent2n;    // layer-2 row nyp (nxp) or null
+    const double* ent2s;    // layer-2 row 1 (nxp)
+    const double* ent2n;    // layer-2 row nyp (nxp)
     double* aux;
-    double txis, txin;
+    // Resident forcing scalars, read at run time so that a replayed graph sees
+    // the latest forcing: [F_TXIS] txis, [F_TXIN] txin, [F_ENT2] 1 when ent2s/ent2n hold data.
+    const double* fcon;
 };
+constexpr int F_TXIS = 0, F_TXIN = 1, F_ENT2 = 2, F_LEN = 4;
 
 struct SuppArgs {
     Geo g;

@@ -14,15 +14,29 @@ namespace oqg {
 
 // ------------------------------------------------------------------ k_vort
 
-__global__ __launch_bounds__(256) void k_vort(VortArgs a) {
+// The tile with its halo of 3 lies inside the domain and holds no boundary point
+// (every stencil is the plain interior formula, identical in value to del2_lds / jac_lds).
+__device__ __forceinline__ double lap_int(const double* s, int W, int o, const Geo& g) {
+    const double C = s[o];
+    return g.fx * ((s[o - 1] + s[o + 1]) - 2.0 * C) + g.fy * ((s[o - W] + s[o + W]) - 2.0 * C);
+}
+
+__device__ __forceinline__ double jac_int(const double* q, const double* p, int W, int o, const Geo& g) {
+    const double dqx = q[o + 1] - q[o - 1], dqy = q[o + W] - q[o - W];
+    const double dpx = p[o + 1] - p[o - 1], dpy = p[o + W] - p[o - W];
+    double t = dqx * dpy - dqy * dpx;
+    t = t + (q[o + 1] * (p[o + 1 + W] - p[o + 1 - W]) - q[o - 1] * (p[o - 1 + W] - p[o - 1 - W]));
+    t = t - (q[o + W] * (p[o + W + 1] - p[o + W - 1]) - q[o - W] * (p[o - W + 1] - p[o - W - 1]));
+    t = t + (p[o + W] * (q[o + W + 1] - q[o + W - 1]) - p[o - W] * (q[o - W + 1] - q[o - W - 1]));
+    t = t - (p[o + 1] * (q[o + 1 + W] - q[o + 1 - W]) - p[o - 1] * (q[o - 1 + W] - q[o - 1 - W]));
+    return g.jfac * t;
+}
+
+template <bool EDGE>
+__device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, double* s_d2, double* s_d4, double* s_q,
+                                          double* s_p, double* s_red) {
     constexpr int TX = VORT_TX, TY = VORT_TY;
     constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
-    __shared__ double s_pm[H3 * W3];
-    __shared__ double s_d2[H2 * W2];
-    __shared__ double s_d4[H1 * W1];
-    __shared__ double s_q[H1 * W1];
-    __shared__ double s_p[H1 * W1];
-    __shared__ double s_red[16];
     const Geo& g = a.g;
     const int tid = threadIdx.x;
     const int tx = tid & 63, ty = tid >> 6;
@@ -42,21 +56,30 @@ __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
     for (int t = 0; t < NPM; t++) {
         const int idx = tid + 256 * t;
         const int r = idx / W3, c = idx - r * W3;
-        rpm[t] = (idx < H3 * W3) ? ldg_field(pm, i0 - 3 + c, j0 - 3 + r, g) : 0.0;
+        if constexpr (EDGE)
+            rpm[t] = (idx < H3 * W3) ? ldg_field(pm, i0 - 3 + c, j0 - 3 + r, g) : 0.0;
+        else
+            rpm[t] = (idx < H3 * W3) ? pm[(j0 - 3 + r) * g.pitch + (i0 - 3 + c)] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < NQP; t++) {
         const int idx = tid + 256 * t;
         const int r = idx / W1, c = idx - r * W1;
         const bool in = idx < H1 * W1;
-        rq[t] = in ? ldg_field(q, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
-        rp[t] = in ? ldg_field(p, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
+        if constexpr (EDGE) {
+            rq[t] = in ? ldg_field(q, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
+            rp[t] = in ? ldg_field(p, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
+        } else {
+            const int o = (j0 - 1 + r) * g.pitch + (i0 - 1 + c);
+            rq[t] = in ? q[o] : 0.0;
+            rp[t] = in ? p[o] : 0.0;
+        }
     }
     const int gi = i0 + tx;
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
         const int gj = j0 + ty + 4 * u;
-        const bool in = gi < g.nxp && gj < g.nyp;
+        const bool in = !EDGE || (gi < g.nxp && gj < g.nyp);
         const long long o2 = (long long)gj * g.pitch + gi;
         rqm[u] = in ? a.qnew[lo + o2] : 0.0;
         rent[u] = (in && k <= 1) ? a.ent1[o2] : 0.0;
@@ -77,27 +100,47 @@ __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
     }
     __syncthreads();
     // Del^2 of pm on the tile with halo 2
-    for (int idx = tid; idx < H2 * W2; idx += 256) {
-        const int r = idx / W2, c = idx - r * W2;
-        const int gj = j0 - 2 + r, gx = i0 - 2 + c;
-        double v = 0.0;
-        if (gj >= 0 && gj < g.nyp) {
-            if (g.cyclic) v = del2_lds(s_pm, W3, r + 1, c + 1, wrapx(gx, g), gj, g);
-            else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_pm, W3, r + 1, c + 1, gx, gj, g);
+    if constexpr (EDGE) {
+        for (int idx = tid; idx < H2 * W2; idx += 256) {
+            const int r = idx / W2, c = idx - r * W2;
+            const int gj = j0 - 2 + r, gx = i0 - 2 + c;
+            double v = 0.0;
+            if (gj >= 0 && gj < g.nyp) {
+                if (g.cyclic) v = del2_lds(s_pm, W3, r + 1, c + 1, wrapx(gx, g), gj, g);
+                else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_pm, W3, r + 1, c + 1, gx, gj, g);
+            }
+            s_d2[idx] = v;
         }
-        s_d2[idx] = v;
+    } else {
+#pragma unroll
+        for (int u = 0; u < H2 / 4; u++) {
+            const int r = ty + 4 * u;
+            s_d2[r * W2 + tx] = lap_int(s_pm, W3, (r + 1) * W3 + tx + 1, g);
+            if (tx < W2 - 64) s_d2[r * W2 + tx + 64] = lap_int(s_pm, W3, (r + 1) * W3 + tx + 65, g);
+        }
     }
     __syncthreads();
     // Del^4 on the tile with halo 1
-    for (int idx = tid; idx < H1 * W1; idx += 256) {
-        const int r = idx / W1, c = idx - r * W1;
-        const int gj = j0 - 1 + r, gx = i0 - 1 + c;
-        double v = 0.0;
-        if (gj >= 0 && gj < g.nyp) {
-            if (g.cyclic) v = del2_lds(s_d2, W2, r + 1, c + 1, wrapx(gx, g), gj, g);
-            else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_d2, W2, r + 1, c + 1, gx, gj, g);
+    if constexpr (EDGE) {
+        for (int idx = tid; idx < H1 * W1; idx += 256) {
+            const int r = idx / W1, c = idx - r * W1;
+            const int gj = j0 - 1 + r, gx = i0 - 1 + c;
+            double v = 0.0;
+            if (gj >= 0 && gj < g.nyp) {
+                if (g.cyclic) v = del2_lds(s_d2, W2, r + 1, c + 1, wrapx(gx, g), gj, g);
+                else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_d2, W2, r + 1, c + 1, gx, gj, g);
+            }
+            s_d4[idx] = v;
         }
-        s_d4[idx] = v;
+    } else {
+#pragma unroll
+        for (int u = 0; u < (H1 + 3) / 4; u++) {
+            const int r = ty + 4 * u;
+            if (r < H1) {
+                s_d4[r * W1 + tx] = lap_int(s_d2, W2, (r + 1) * W2 + tx + 1, g);
+                if (tx < W1 - 64) s_d4[r * W1 + tx + 64] = lap_int(s_d2, W2, (r + 1) * W2 + tx + 65, g);
+            }
+        }
     }
     __syncthreads();
     const double c2 = a.ph.c2[k], c4 = a.ph.c4[k];
@@ -106,18 +149,29 @@ __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
     for (int u = 0; u < NOUT; u++) {
         const int r = ty + 4 * u;
         const int gj = j0 + r;
-        if (gi >= g.nxp || gj >= g.nyp) continue;
-        const double d6 = del2_lds(s_d4, W1, r + 1, tx + 1, gi, gj, g);
-        const double d4 = s_d4[(r + 1) * W1 + tx + 1];
+        if (EDGE && (gi >= g.nxp || gj >= g.nyp)) continue;
+        const int o1 = (r + 1) * W1 + tx + 1;
+        double d6, dqdt;
+        if constexpr (EDGE) {
+            d6 = del2_lds(s_d4, W1, r + 1, tx + 1, gi, gj, g);
+            dqdt = jac_lds(s_q, s_p, W1, r + 1, tx + 1, gi, gj, g);
+        } else {
+            d6 = lap_int(s_d4, W1, o1, g);
+            dqdt = jac_int(s_q, s_p, W1, o1, g);
+        }
+        const double d4 = s_d4[o1];
         const double d2 = s_d2[(r + 2) * W2 + tx + 2];
-        double dqdt = jac_lds(s_q, s_p, W1, r + 1, tx + 1, gi, gj, g);
         dqdt = dqdt + c2 * d4 - c4 * d6;
         if (k == 0) {
             const double e = rent[u];
             dqdt = dqdt + a.ph.cent1 * e + a.ph.cwek * rwek[u];
-            const bool ex = (gi == 0 || gi == g.nxp - 1), ey = (gj == 0 || gj == g.nyp - 1);
-            const double w = (ex && ey) ? 0.25 : ((ex || ey) ? 0.5 : 1.0);
-            epart += w * e;
+            if constexpr (EDGE) {
+                const bool ex = (gi == 0 || gi == g.nxp - 1), ey = (gj == 0 || gj == g.nyp - 1);
+                const double w = (ex && ey) ? 0.25 : ((ex || ey) ? 0.5 : 1.0);
+                epart += w * e;
+            } else {
+                epart += e;
+            }
         }
         if (k == 1) dqdt = dqdt - a.ph.cent2 * rent[u];
         if (k == g.kt) dqdt = dqdt - a.ph.cdrag * d2;
@@ -128,6 +182,22 @@ __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
         block_sum<1>(v, s_red);
         if (threadIdx.x == 0) a.entpart[blockIdx.y * a.tiles_x + blockIdx.x] = v[0];
     }
+}
+
+// Tiles whose halo of 3 stays inside the domain without touching a boundary
+// point or crossing the channel's periodic seam take the branch-free body.
+__global__ __launch_bounds__(256) void k_vort(VortArgs a) {
+    constexpr int TX = VORT_TX, TY = VORT_TY;
+    __shared__ double s_pm[(TY + 6) * (TX + 6)];
+    __shared__ double s_d2[(TY + 4) * (TX + 4)];
+    __shared__ double s_d4[(TY + 2) * (TX + 2)];
+    __shared__ double s_q[(TY + 2) * (TX + 2)];
+    __shared__ double s_p[(TY + 2) * (TX + 2)];
+    __shared__ double s_red[16];
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const bool inner = i0 >= 3 && i0 + TX + 3 <= a.g.nxp && j0 >= 3 && j0 + TY + 3 <= a.g.nyp;
+    if (inner) vort_tile<false>(a, s_pm, s_d2, s_d4, s_q, s_p, s_red);
+    else vort_tile<true>(a, s_pm, s_d2, s_d4, s_q, s_p, s_red);
 }
 
 void launch_vort(const VortArgs& a, hipStream_t st) {
@@ -205,12 +275,15 @@ __global__ __launch_bounds__(256) void k_cyc(CycArgs a) {
     block_sum<10>(v, red);
     double e[4] = {0.0, 0.0, 0.0, 0.0};
     if (k == 0) {
+        const bool ent2 = a.fcon[F_ENT2] != 0.0;
         for (int i = threadIdx.x; i < nx; i += 256) {
             const double w = (i == 0 || i == nx - 1) ? 0.5 : 1.0;
             e[0] += w * a.ent[i];
             e[1] += w * a.ent[(long long)(ny - 1) * g.pitch + i];
-            if (a.ent2s) e[2] += w * a.ent2s[i];
-            if (a.ent2n) e[3] += w * a.ent2n[i];
+            if (ent2) {
+                e[2] += w * a.ent2s[i];
+                e[3] += w * a.ent2n[i];
+            }
         }
         block_sum<4>(e, red);
     }
@@ -236,8 +309,8 @@ __global__ __launch_bounds__(256) void k_cyc(CycArgs a) {
             o[A_BES + 1] = dz * a.ph.fohfac[0] * (e[2] * dx);
             o[A_BEN + 0] = -dz * a.ph.fohfac[1] * (e[1] * dx);
             o[A_BEN + 1] = -dz * a.ph.fohfac[1] * (e[3] * dx);
-            o[A_TXIS] = a.txis;
-            o[A_TXIN] = a.txin;
+            o[A_TXIS] = a.fcon[F_TXIS];
+            o[A_TXIN] = a.fcon[F_TXIN];
         }
     }
 }

@@ -271,11 +271,12 @@ class QG:
         check(self._lib.oqg_get_modes(self._h, P(mats[0]), P(mats[1]), P(mats[2]), P(rdm2)), self._h, "oqg_get_modes")
         return tuple(m.reshape(nl, nl).T.copy() for m in mats) + (rdm2,)
 
-    def kernel_times(self, tdt: float, nsteps: int):
-        """Per-kernel average duration (ms) over nsteps real steps, from HIP events."""
+    def kernel_times(self, tdt: float, reps: int = 20):
+        """Advance one step and return each kernel's average duration (ms) over
+        `reps` back-to-back launches between two HIP events on the library stream."""
         ms = (ctypes.c_double * 8)()
         names = ctypes.create_string_buffer(256)
-        n = self._lib.oqg_kernel_times(self._h, float(tdt), int(nsteps), ms, 8, names, 256)
+        n = self._lib.oqg_kernel_times(self._h, float(tdt), int(reps), ms, 8, names, 256)
         if n < 0:
             check(-n, self._h, "oqg_kernel_times")
         keys = names.value.decode().split("\n")[:n]

@@ -211,6 +211,68 @@ def test_graph_run_equals_eager_and_is_deterministic(gpu):
         assert np.array_equal(a, b)
 
 
+def test_graph_replay_sees_new_channel_forcing(gpu):
+    """A graph captured under one forcing must use the forcing set later
+    (txis/txin and the layer-2 entrainment switch are read at run time)."""
+    w = W.small_channel(33, 17, 3, seed=4)
+    p = w.initial_p()
+    _, wekp, txis, txin = w.forcing()
+    rng = np.random.default_rng(9)
+    ent_a = 1e-7 * rng.standard_normal((1, w.nyp, w.nxp))
+    ent_a[..., -1] = ent_a[..., 0]
+    ent_b = np.concatenate([ent_a, 1e-7 * rng.standard_normal((1, w.nyp, w.nxp))])
+    ent_b[..., -1] = ent_b[..., 0]
+    outs = []
+    for mode in ("eager", "graph", "unchanged"):
+        qg = w.make_qg()
+        qg.set_state(p=p, pm=p)
+        qg.init_state()
+        qg.set_forcing(ent_a, wekp, txis, txin)
+        if mode == "eager":
+            for nt in range(1, 5):
+                qg.step(w.tdt)
+        else:
+            qg.run(w.tdt, 4, 0, 0)
+        if mode != "unchanged":
+            qg.set_forcing(ent_b, wekp, txis + 0.3, txin - 0.2)
+        if mode == "eager":
+            for nt in range(5, 9):
+                qg.step(w.tdt)
+        else:
+            qg.run(w.tdt, 4, 4, 0)
+        qg.sync()
+        outs.append(qg.get_state())
+        qg.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    assert not np.array_equal(outs[1][2], outs[2][2]), "the forcing change had no effect"
+
+
+def test_kernel_times_advances_one_step(gpu):
+    w = W.small_box(65, 3, seed=5)
+    p = w.initial_p()
+    ent, wekp, txis, txin = w.forcing()
+    outs = []
+    for timed in (False, True):
+        qg = w.make_qg()
+        qg.set_state(p=p, pm=p)
+        qg.init_state()
+        qg.set_forcing(ent, wekp, txis, txin)
+        qg.step(w.tdt)
+        if timed:
+            kt = qg.kernel_times(w.tdt, 4)
+            assert set(kt) == {"k_vort", "k_cyc", "k_fwd", "k_ysolve", "k_inv"}
+            assert kt["k_cyc"] == 0.0 and all(kt[k] > 0 for k in ("k_vort", "k_fwd", "k_ysolve", "k_inv"))
+        else:
+            qg.step(w.tdt)
+        qg.step(w.tdt)
+        outs.append((qg.get_state(), qg.get_constr()))
+        qg.close()
+    for a, b in zip(outs[0][0], outs[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
 def test_set_get_state_roundtrip_and_torch(gpu):
     import torch
 
