@@ -65,5 +65,35 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN, "build")
+FORTRAN_TEST = os.path.join(FORTRAN_BUILD, "test_oqg_c")
+
+
+def flang() -> str | None:
+    for cand in (os.environ.get("FC_OQG"), shutil.which("amdflang"), "/opt/rocm/bin/amdflang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the ISO_C_BINDING interface module and the Fortran test program
+    against liboqg.so (needs amdflang; returns None when it is absent)."""
+    fc = flang()
+    if fc is None:
+        return None
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    cmd = [fc, "-O2", "-J", FORTRAN_BUILD, os.path.join(FORTRAN, "oqg_c.F90"), os.path.join(FORTRAN, "test_oqg_c.F90"),
+           "-L", HERE, "-loqg", "-Wl,-rpath,$ORIGIN/../..", "-o", FORTRAN_TEST]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=FORTRAN_BUILD, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed:\n{r.stdout.decode(errors='replace')}")
+    return FORTRAN_TEST
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

@@ -325,9 +325,7 @@ static void fill_inv_common(oqg_ctx* c, InvArgs& ia) {
     ia.g = c->g;
     ia.s = c->s;
     ia.ph = c->ph;
-    ia.mx = &c->dconst->mx;
-    ia.bh = &c->dconst->bh;
-    ia.ch = &c->dconst->ch;
+    ia.dc = c->dconst;
     ia.spec = c->spec;
     ia.tw = c->tw;
     ia.colsum = c->colsum;
@@ -398,6 +396,7 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     va.tdt = tdt;
     va.tiles_x = c->tiles_x;
     va.tiles_y = c->tiles_y;
+    va.org_x = vort_org_x(c->g);
     CycArgs& ca = a.ca;
     ca.g = c->g;
     ca.ph = c->ph;
@@ -572,8 +571,8 @@ static int setup(oqg_ctx* c) {
     // device buffers
     const size_t fl = (size_t)nl * g.fs, f1 = (size_t)g.fs;
     const size_t nspec = (size_t)nl * s.nrow * s.kp;
-    c->tiles_x = (nxp + VORT_TX - 1) / VORT_TX;
-    c->tiles_y = (nyp + VORT_TY - 1) / VORT_TY;
+    c->tiles_x = (vort_lim_x(g) - vort_org_x(g) + VORT_TX - 1) / VORT_TX;
+    c->tiles_y = (nyp - 2 + VORT_TY - 1) / VORT_TY;
     int rc = 0;
     for (int t = 0; t < 2 && !rc; t++) {
         rc |= dalloc(c, &c->qbuf[t], fl);

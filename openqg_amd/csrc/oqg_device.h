@@ -19,59 +19,101 @@ __device__ __forceinline__ int wrapx(int gi, const Geo& g) {
     return gi;
 }
 
-// Value of a (pitch-layout) layer at global (gi,gj); cyclic wrap in x, 0 outside.
-__device__ __forceinline__ double ldg_field(const double* f, int gi, int gj, const Geo& g) {
-    if (gj < 0 || gj >= g.nyp) return 0.0;
+// Offset of global (gi,gj) in a pitch-layout layer and whether it lies in the
+// basin (cyclic wrap in x).  Out-of-basin points get offset 0, so a load from
+// the returned offset is always legal: callers load unconditionally and zero
+// the value afterwards, which keeps a block's loads free of branches and
+// lets them all issue before any is waited on.
+__device__ __forceinline__ int field_offset(int gi, int gj, const Geo& g, bool& ok) {
     if (g.cyclic) gi = wrapx(gi, g);
-    else if (gi < 0 || gi >= g.nxp) return 0.0;
-    return f[(long long)gj * g.pitch + gi];
+    ok = gj >= 0 && gj < g.nyp && gi >= 0 && gi < g.nxp;
+    return ok ? gj * g.pitch + gi : 0;
 }
 
 // del2_P_bc at one point (numerics.F90:410-468): dP2dx2_bc + dP2dy2_bc with the
 // mixed boundary condition.  s holds the field on an LDS tile of width W with
-// the point at (r,c); (gi,gj) is its global position.  In a channel the tile
-// holds the x-periodic neighbours.
+// the point at (r,c); (gi,gj) is its global position in the basin.  In a
+// channel the tile holds the x-periodic neighbours.  Branch-free: the four
+// neighbours are always in the tile, every case of the reference is evaluated
+// as written there and the applicable one is selected.
 __device__ __forceinline__ double del2_lds(const double* s, int W, int r, int c, int gi, int gj,
                                            const Geo& g) {
-    const double C = s[r * W + c];
-    double dx2, dy2;
-    if (gj == 0 || gj == g.nyp - 1) dx2 = 0.0;
-    else if (g.cyclic || (gi >= 1 && gi <= g.nxp - 2)) dx2 = g.fx * ((s[r * W + c - 1] + s[r * W + c + 1]) - 2.0 * C);
-    else if (gi == 0) dx2 = g.cxl * (s[r * W + c + 1] - C);
-    else dx2 = g.cxr * (C - s[r * W + c - 1]);
-    if (!g.cyclic && (gi == 0 || gi == g.nxp - 1)) dy2 = 0.0;
-    else if (gj >= 1 && gj <= g.nyp - 2) dy2 = g.fy * ((s[(r - 1) * W + c] + s[(r + 1) * W + c]) - 2.0 * C);
-    else if (gj == 0) dy2 = g.cyl * (s[(r + 1) * W + c] - C);
-    else dy2 = g.cyr * (C - s[(r - 1) * W + c]);
+    const int o = r * W + c;
+    const double C = s[o], Wv = s[o - 1], Ev = s[o + 1], Sv = s[o - W], Nv = s[o + W];
+    const bool xin = g.cyclic || (gi >= 1 && gi <= g.nxp - 2);
+    const bool yin = gj >= 1 && gj <= g.nyp - 2;
+    const double dxi = g.fx * ((Wv + Ev) - 2.0 * C);
+    const double dxl = g.cxl * (Ev - C), dxr = g.cxr * (C - Wv);
+    const double dyi = g.fy * ((Sv + Nv) - 2.0 * C);
+    const double dyl = g.cyl * (Nv - C), dyr = g.cyr * (C - Sv);
+    const double dx2 = !yin ? 0.0 : (xin ? dxi : (gi == 0 ? dxl : dxr));   // dx2 = 0 on rows 1, nyp
+    const double dy2 = !xin ? 0.0 : (yin ? dyi : (gj == 0 ? dyl : dyr));   // dy2 = 0 on box columns 1, nxp
     return dx2 + dy2;
 }
 
-// dPdx_2 / dPdy_2 (numerics.F90:311-338) on an LDS tile.
-__device__ __forceinline__ double dX(const double* s, int W, int r, int c, int gi, const Geo& g) {
-    return (g.cyclic || (gi >= 1 && gi <= g.nxp - 2)) ? s[r * W + c + 1] - s[r * W + c - 1] : 0.0;
-}
-__device__ __forceinline__ double dY(const double* s, int W, int r, int c, int gj, const Geo& g) {
-    return (gj >= 1 && gj <= g.nyp - 2) ? s[(r + 1) * W + c] - s[(r - 1) * W + c] : 0.0;
-}
-
-// Arakawa J(q,p) (numerics.F90:361-372) at tile point (r,c), times the factor of :361.
+// Arakawa J(q,p) (numerics.F90:361-372) at tile point (r,c), times the factor
+// of :361, with dPdx_2 / dPdy_2 (numerics.F90:311-338) zero on the boundary
+// columns (box) and rows.  Branch-free like del2_lds: the 3x3 neighbourhoods
+// are read and the zero cases selected, with the reference's operations.
 __device__ __forceinline__ double jac_lds(const double* q, const double* p, int W, int r, int c, int gi,
                                           int gj, const Geo& g) {
     const bool xin = g.cyclic || (gi >= 1 && gi <= g.nxp - 2);
     const bool yin = gj >= 1 && gj <= g.nyp - 2;
-    const double dqx = dX(q, W, r, c, gi, g), dqy = dY(q, W, r, c, gj, g);
-    const double dpx = dX(p, W, r, c, gi, g), dpy = dY(p, W, r, c, gj, g);
+    const int o = r * W + c;
+    const double qW = q[o - 1], qE = q[o + 1], qS = q[o - W], qN = q[o + W];
+    const double pW = p[o - 1], pE = p[o + 1], pS = p[o - W], pN = p[o + W];
+    const double qNE = q[o + W + 1], qNW = q[o + W - 1], qSE = q[o - W + 1], qSW = q[o - W - 1];
+    const double pNE = p[o + W + 1], pNW = p[o + W - 1], pSE = p[o - W + 1], pSW = p[o - W - 1];
+    const double dqx = xin ? qE - qW : 0.0, dqy = yin ? qN - qS : 0.0;
+    const double dpx = xin ? pE - pW : 0.0, dpy = yin ? pN - pS : 0.0;
     double t = dqx * dpy - dqy * dpx;
+    // dY at (r, c+-1), dX at (r+-1, c): the point's own row/column decides
+    const double dpyE = yin ? pNE - pSE : 0.0, dpyW = yin ? pNW - pSW : 0.0;
+    const double dpxN = xin ? pNE - pNW : 0.0, dpxS = xin ? pSE - pSW : 0.0;
+    const double dqxN = xin ? qNE - qNW : 0.0, dqxS = xin ? qSE - qSW : 0.0;
+    const double dqyE = yin ? qNE - qSE : 0.0, dqyW = yin ? qNW - qSW : 0.0;
     double v;
-    v = xin ? q[r * W + c + 1] * dY(p, W, r, c + 1, gj, g) - q[r * W + c - 1] * dY(p, W, r, c - 1, gj, g) : 0.0;
+    v = xin ? qE * dpyE - qW * dpyW : 0.0;
     t = t + v;
-    v = yin ? q[(r + 1) * W + c] * dX(p, W, r + 1, c, gi, g) - q[(r - 1) * W + c] * dX(p, W, r - 1, c, gi, g) : 0.0;
+    v = yin ? qN * dpxN - qS * dpxS : 0.0;
     t = t - v;
-    v = yin ? p[(r + 1) * W + c] * dX(q, W, r + 1, c, gi, g) - p[(r - 1) * W + c] * dX(q, W, r - 1, c, gi, g) : 0.0;
+    v = yin ? pN * dqxN - pS * dqxS : 0.0;
     t = t + v;
-    v = xin ? p[r * W + c + 1] * dY(q, W, r, c + 1, gj, g) - p[r * W + c - 1] * dY(q, W, r, c - 1, gj, g) : 0.0;
+    v = xin ? pE * dqyE - pW * dqyW : 0.0;
     t = t - v;
     return g.jfac * t;
+}
+
+// Global -> LDS copies split in two halves so that a block issues all of its
+// global loads before it waits for any: stage_load reads this thread's first
+// NPT words of `src` into registers (straight-line, predicated), stage_store
+// later writes them to LDS and copies any remainder (n > NPT * blockDim) with a
+// strided loop.  The caller's next barrier publishes the copy.
+template <int NPT>
+struct Staged {
+    unsigned long long w[NPT];
+};
+
+template <int NPT>
+__device__ __forceinline__ void stage_load(Staged<NPT>& r, const void* src, int nwords) {
+    const unsigned long long* s = static_cast<const unsigned long long*>(src);
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        const int idx = threadIdx.x + p * blockDim.x;
+        r.w[p] = (idx < nwords) ? s[idx] : 0ull;
+    }
+}
+
+template <int NPT>
+__device__ __forceinline__ void stage_store(const Staged<NPT>& r, void* dst, const void* src, int nwords) {
+    unsigned long long* d = static_cast<unsigned long long*>(dst);
+#pragma unroll
+    for (int p = 0; p < NPT; p++) {
+        const int idx = threadIdx.x + p * blockDim.x;
+        if (idx < nwords) d[idx] = r.w[p];
+    }
+    const unsigned long long* s = static_cast<const unsigned long long*>(src);
+    for (int idx = threadIdx.x + NPT * blockDim.x; idx < nwords; idx += blockDim.x) d[idx] = s[idx];
 }
 
 // Deterministic block sum of NV values per thread: a fixed butterfly inside
@@ -101,7 +143,29 @@ __device__ void block_sum(double (&v)[NV], double* red) {
 // Stockham autosort (Govindaraju et al. 2008) with mixed radix 2/3/4/5, one
 // complex buffer of length L per transform; each of the FFT_GROUP threads of a
 // group holds up to FFT_MAXB butterflies in registers across the barrier that
-// separates a stage's reads from its writes.  tw[t] = exp(-2 pi i t / L).
+// separates a stage's reads from its writes.
+//
+// Twiddles come from an LDS table: when 4 | L it holds the first quadrant only,
+// tw[r] = exp(-2 pi i r / L) for r < L/4, and exp(-2 pi i (q L/4 + r) / L) is
+// tw[r] times (-i)^q, an exact swap and sign change (the host table is built
+// by the same quadrant reduction, so the values are identical); otherwise it
+// holds all L entries.
+
+// Entries of the LDS twiddle table for a transform of length L.
+__host__ __device__ inline int twiddle_table_len(int L) { return (L % 4 == 0) ? L / 4 : L; }
+
+// Branch-free: when 4 does not divide L the quadrant width is L itself and q = 0.
+__device__ __forceinline__ double2 twid(const double2* tw, int t, int L) {
+    const int L4 = (L & 3) ? L : (L >> 2);
+    const int q = (t >= L4) + (t >= 2 * L4) + (t >= 3 * L4);
+    const double2 w = tw[t - q * L4];
+    // q = 0: (x, y); 1: (y, -x); 2: (-x, -y); 3: (-y, x)
+    const double u = (q & 1) ? w.y : w.x, v = (q & 1) ? w.x : w.y;
+    return make_double2((q >= 2) ? -u : u, (q == 1 || q == 2) ? -v : v);
+}
+
+// 8-byte words of the LDS twiddle table of a length-L transform.
+__host__ __device__ inline int twiddle_words(int L) { return 2 * twiddle_table_len(L); }
 
 template <int R>
 __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, int L, int sign) {
@@ -130,7 +194,7 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
             double sr = in[0].x, si = in[0].y;
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                const double2 w = tw[((r * q) % R) * pstep];
+                const double2 w = twid(tw, ((r * q) % R) * pstep, L);
                 const double c = w.x, s = sg * w.y;
                 sr += in[r].x * c - in[r].y * s;
                 si += in[r].x * s + in[r].y * c;
@@ -141,7 +205,7 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
 }
 
 template <int R, int MB>
-__device__ void fft_stage(double2* buf, int L, int Ns, const double2* __restrict__ tw, int sign, int gt) {
+__device__ __forceinline__ void fft_stage(double2* buf, int L, int Ns, const double2* tw, int sign, int gt) {
     const int LR = L / R;
     const int tstep = L / (Ns * R);
     double2 v[MB][R];
@@ -155,7 +219,7 @@ __device__ void fft_stage(double2* buf, int L, int Ns, const double2* __restrict
             if (k != 0) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    double2 w = tw[r * k * tstep];
+                    double2 w = twid(tw, r * k * tstep, L);
                     if (sign > 0) w.y = -w.y;
                     v[b][r] = cmul(v[b][r], w);
                 }
@@ -177,11 +241,12 @@ __device__ void fft_stage(double2* buf, int L, int Ns, const double2* __restrict
     __syncthreads();
 }
 
-// In-place transform of buf (length s.L); gt = thread index within its group;
-// MB >= the most butterflies a thread owns in any stage.  Every thread of the
-// block must call this (it contains block barriers).
+// In-place transform of buf (length s.L) with the LDS twiddle table tw;
+// gt = thread index within its group; MB >= the most butterflies a thread owns
+// in any stage.  Every thread of the block must call this (it contains block
+// barriers).
 template <int MB>
-__device__ void fft_lds(double2* buf, const Spec& s, const double2* tw, int sign, int gt) {
+__device__ __forceinline__ void fft_lds(double2* buf, const Spec& s, const double2* tw, int sign, int gt) {
     int Ns = 1;
     for (int st = 0; st < s.nstages; st++) {
         const int R = s.radix[st];

@@ -104,7 +104,15 @@ struct VortArgs {
     double* entpart;      // per-tile partial integrals of ent1 (layer-0 tiles)
     double tdt;
     int tiles_x, tiles_y;
+    int org_x;            // first column of the tiles: 1 in a box, 0 in a channel
 };
+
+// k_vort computes q_new on columns [org_x, lim_x) and rows [1, nyp-1): the
+// zonal boundary rows, and in a box the meridional boundary columns, are
+// rewritten by ocqbdy (vorsubs.F90:58-114) in the same step before anything
+// reads them.
+__host__ __device__ inline int vort_org_x(const Geo& g) { return g.cyclic ? 0 : 1; }
+__host__ __device__ inline int vort_lim_x(const Geo& g) { return g.cyclic ? g.nxp : g.nxp - 1; }
 
 struct FwdArgs {
     Geo g;
@@ -127,13 +135,18 @@ struct YArgs {
     int mode0, nmodes;    // range of modes solved
 };
 
+// Device block holding the precompute read by k_inv.
+struct DevConsts {
+    Mix mx;
+    BoxHom bh;
+    CycHom ch;
+};
+
 struct InvArgs {
     Geo g;
     Spec s;
     Phys ph;
-    const Mix* mx;          // device copies of the precompute blocks
-    const BoxHom* bh;
-    const CycHom* ch;
+    const DevConsts* dc;    // device copy of the precompute blocks
     const double* spec;
     const double* tw;
     const double* colsum;   // box: [m][col]
@@ -196,14 +209,8 @@ void launch_inv(const InvArgs& a, hipStream_t st);
 void launch_suppress(const SuppArgs& a, hipStream_t st);
 void launch_qcomp(const QcompArgs& a, hipStream_t st);
 int inv_block_count(const Spec& s);
+size_t fwd_lds_bytes(const Spec& s);
 size_t inv_lds_bytes(const Spec& s, const Geo& g, int nmodes);
 void set_kernel_attrs(const Spec& s, const Geo& g);
-
-// Device block holding the precompute read by k_inv.
-struct DevConsts {
-    Mix mx;
-    BoxHom bh;
-    CycHom ch;
-};
 
 }  // namespace oqg

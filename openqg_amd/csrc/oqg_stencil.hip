@@ -1,7 +1,8 @@
 // oqg_stencil.hip -- the grid-point kernels of the QG step:
 //   k_vort     Arakawa Jacobian + Del^2/Del^4/Del^6 friction + forcing + leapfrog
 //              (vorticity.F90:14-103, numerics.F90:340-468) on 64x16 LDS tiles with a
-//              halo of 3; writes q_new over qm;
+//              halo of 3; writes q_new over qm on the points that ocqbdy does not
+//              rewrite (the tiles start at row 1, and at column 1 in a box);
 //   k_cyc      (channel only) boundary line integrals of the momentum constraints
 //              (vorticity.F90:57-62,84-101, numerics.F90:374-386);
 //   k_suppress suppress_comp_mode (driver.F90:174-189);
@@ -41,7 +42,8 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
     const int tid = threadIdx.x;
     const int tx = tid & 63, ty = tid >> 6;
     const int k = blockIdx.z;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = a.org_x + blockIdx.x * TX, j0 = 1 + blockIdx.y * TY;
+    const int lim_x = vort_lim_x(g), lim_y = g.nyp - 1;
     const long long lo = (long long)k * g.fs;
     const double* pm = a.pm + lo;
     const double* q = a.q + lo;
@@ -50,16 +52,23 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
     // Issue every global load of the block at once (one memory round trip):
     // the pm tile with halo 3, the q and p tiles with halo 1, and the pointwise
     // inputs of this thread's four output points.
+    // Edge tiles load from clamped offsets and zero the out-of-basin values
+    // when they are stored, so their loads carry no branches either.
     constexpr int NPM = (H3 * W3 + 255) / 256, NQP = (H1 * W1 + 255) / 256, NOUT = TY / 4;
     double rpm[NPM], rq[NQP], rp[NQP], rqm[NOUT], rent[NOUT], rwek[NOUT];
+    bool okpm[NPM], okqp[NQP];
 #pragma unroll
     for (int t = 0; t < NPM; t++) {
         const int idx = tid + 256 * t;
         const int r = idx / W3, c = idx - r * W3;
-        if constexpr (EDGE)
-            rpm[t] = (idx < H3 * W3) ? ldg_field(pm, i0 - 3 + c, j0 - 3 + r, g) : 0.0;
-        else
+        if constexpr (EDGE) {
+            bool ok;
+            const int o = field_offset(i0 - 3 + c, j0 - 3 + r, g, ok);
+            okpm[t] = ok && idx < H3 * W3;
+            rpm[t] = pm[okpm[t] ? o : 0];
+        } else {
             rpm[t] = (idx < H3 * W3) ? pm[(j0 - 3 + r) * g.pitch + (i0 - 3 + c)] : 0.0;
+        }
     }
 #pragma unroll
     for (int t = 0; t < NQP; t++) {
@@ -67,8 +76,11 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
         const int r = idx / W1, c = idx - r * W1;
         const bool in = idx < H1 * W1;
         if constexpr (EDGE) {
-            rq[t] = in ? ldg_field(q, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
-            rp[t] = in ? ldg_field(p, i0 - 1 + c, j0 - 1 + r, g) : 0.0;
+            bool ok;
+            const int o = field_offset(i0 - 1 + c, j0 - 1 + r, g, ok);
+            okqp[t] = ok && in;
+            rq[t] = q[okqp[t] ? o : 0];
+            rp[t] = p[okqp[t] ? o : 0];
         } else {
             const int o = (j0 - 1 + r) * g.pitch + (i0 - 1 + c);
             rq[t] = in ? q[o] : 0.0;
@@ -79,37 +91,66 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
         const int gj = j0 + ty + 4 * u;
-        const bool in = !EDGE || (gi < g.nxp && gj < g.nyp);
-        const long long o2 = (long long)gj * g.pitch + gi;
-        rqm[u] = in ? a.qnew[lo + o2] : 0.0;
-        rent[u] = (in && k <= 1) ? a.ent1[o2] : 0.0;
-        rwek[u] = (in && k == 0) ? a.wekp[o2] : 0.0;
+        const bool in = !EDGE || (gi < lim_x && gj < lim_y);
+        const long long o2 = in ? (long long)gj * g.pitch + gi : 0;
+        rqm[u] = a.qnew[lo + o2];
+        rent[u] = (k <= 1) ? a.ent1[o2] : 0.0;
+        rwek[u] = (k == 0) ? a.wekp[o2] : 0.0;
     }
+    // Edge tiles of layer 1 also carry the trapezoid weights of the boundary
+    // ring of ent1 (int_P_dA, numerics.F90:165-204), whose points no tile
+    // computes: the rows j = 0, nyp-1 come with the first / last tile row, and
+    // in a box the columns i = 0, nxp-1 and the corners with the first / last
+    // tile column.  At most one ring point per thread.
+    double ring_w = 0.0;
+    long long ring_o = 0;
+    if (EDGE && k == 0) {
+        const bool frow = blockIdx.y == 0, lrow = blockIdx.y == a.tiles_y - 1;
+        const bool fcol = blockIdx.x == 0, lcol = blockIdx.x == a.tiles_x - 1;
+        if (((ty == 0 && frow) || (ty == 1 && lrow)) && gi < lim_x) {
+            const bool ex = (gi == 0 || gi == g.nxp - 1);
+            ring_w = ex ? 0.25 : 0.5;
+            ring_o = (long long)((ty == 0) ? 0 : g.nyp - 1) * g.pitch + gi;
+        }
+        if (!g.cyclic && ((ty == 2 && fcol) || (ty == 3 && lcol))) {
+            const int gic = (ty == 2) ? 0 : g.nxp - 1;
+            if (tx < TY) {
+                if (j0 + tx < lim_y) {
+                    ring_w = 0.5;
+                    ring_o = (long long)(j0 + tx) * g.pitch + gic;
+                }
+            } else if ((tx == TY && frow) || (tx == TY + 1 && lrow)) {
+                ring_w = 0.25;
+                ring_o = (long long)((tx == TY) ? 0 : g.nyp - 1) * g.pitch + gic;
+            }
+        }
+    }
+    const double ring_e = (EDGE && k == 0) ? a.ent1[ring_o] : 0.0;
 #pragma unroll
     for (int t = 0; t < NPM; t++) {
         const int idx = tid + 256 * t;
-        if (idx < H3 * W3) s_pm[idx] = rpm[t];
+        if (idx < H3 * W3) s_pm[idx] = (!EDGE || okpm[t]) ? rpm[t] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < NQP; t++) {
         const int idx = tid + 256 * t;
         if (idx < H1 * W1) {
-            s_q[idx] = rq[t];
-            s_p[idx] = rp[t];
+            s_q[idx] = (!EDGE || okqp[t]) ? rq[t] : 0.0;
+            s_p[idx] = (!EDGE || okqp[t]) ? rp[t] : 0.0;
         }
     }
     __syncthreads();
     // Del^2 of pm on the tile with halo 2
     if constexpr (EDGE) {
-        for (int idx = tid; idx < H2 * W2; idx += 256) {
+#pragma unroll
+        for (int t = 0; t < (H2 * W2 + 255) / 256; t++) {
+            const int idx = tid + 256 * t;
             const int r = idx / W2, c = idx - r * W2;
             const int gj = j0 - 2 + r, gx = i0 - 2 + c;
-            double v = 0.0;
-            if (gj >= 0 && gj < g.nyp) {
-                if (g.cyclic) v = del2_lds(s_pm, W3, r + 1, c + 1, wrapx(gx, g), gj, g);
-                else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_pm, W3, r + 1, c + 1, gx, gj, g);
-            }
-            s_d2[idx] = v;
+            const int gxw = g.cyclic ? wrapx(gx, g) : gx;
+            const bool in = gj >= 0 && gj < g.nyp && gxw >= 0 && gxw < g.nxp;
+            const double v = del2_lds(s_pm, W3, r + 1, c + 1, gxw, gj, g);
+            if (idx < H2 * W2) s_d2[idx] = in ? v : 0.0;
         }
     } else {
 #pragma unroll
@@ -122,15 +163,15 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
     __syncthreads();
     // Del^4 on the tile with halo 1
     if constexpr (EDGE) {
-        for (int idx = tid; idx < H1 * W1; idx += 256) {
+#pragma unroll
+        for (int t = 0; t < (H1 * W1 + 255) / 256; t++) {
+            const int idx = tid + 256 * t;
             const int r = idx / W1, c = idx - r * W1;
             const int gj = j0 - 1 + r, gx = i0 - 1 + c;
-            double v = 0.0;
-            if (gj >= 0 && gj < g.nyp) {
-                if (g.cyclic) v = del2_lds(s_d2, W2, r + 1, c + 1, wrapx(gx, g), gj, g);
-                else if (gx >= 0 && gx < g.nxp) v = del2_lds(s_d2, W2, r + 1, c + 1, gx, gj, g);
-            }
-            s_d4[idx] = v;
+            const int gxw = g.cyclic ? wrapx(gx, g) : gx;
+            const bool in = gj >= 0 && gj < g.nyp && gxw >= 0 && gxw < g.nxp;
+            const double v = del2_lds(s_d2, W2, r + 1, c + 1, gxw, gj, g);
+            if (idx < H1 * W1) s_d4[idx] = in ? v : 0.0;
         }
     } else {
 #pragma unroll
@@ -149,7 +190,7 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
     for (int u = 0; u < NOUT; u++) {
         const int r = ty + 4 * u;
         const int gj = j0 + r;
-        if (EDGE && (gi >= g.nxp || gj >= g.nyp)) continue;
+        const bool out = !EDGE || (gi < lim_x && gj < lim_y);
         const int o1 = (r + 1) * W1 + tx + 1;
         double d6, dqdt;
         if constexpr (EDGE) {
@@ -166,17 +207,18 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
             const double e = rent[u];
             dqdt = dqdt + a.ph.cent1 * e + a.ph.cwek * rwek[u];
             if constexpr (EDGE) {
-                const bool ex = (gi == 0 || gi == g.nxp - 1), ey = (gj == 0 || gj == g.nyp - 1);
-                const double w = (ex && ey) ? 0.25 : ((ex || ey) ? 0.5 : 1.0);
-                epart += w * e;
+                // interior points weigh 1; in a channel the columns 0 and nxp-1 weigh 1/2
+                const double w = (gi == 0 || gi == g.nxp - 1) ? 0.5 : 1.0;
+                if (out) epart += w * e;
             } else {
                 epart += e;
             }
         }
         if (k == 1) dqdt = dqdt - a.ph.cent2 * rent[u];
         if (k == g.kt) dqdt = dqdt - a.ph.cdrag * d2;
-        a.qnew[lo + (long long)gj * g.pitch + gi] = rqm[u] + a.tdt * dqdt;
+        if (out) a.qnew[lo + (long long)gj * g.pitch + gi] = rqm[u] + a.tdt * dqdt;
     }
+    if (EDGE && k == 0) epart += ring_w * ring_e;
     if (k == 0) {
         double v[1] = {epart};
         block_sum<1>(v, s_red);
@@ -194,7 +236,7 @@ __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
     __shared__ double s_q[(TY + 2) * (TX + 2)];
     __shared__ double s_p[(TY + 2) * (TX + 2)];
     __shared__ double s_red[16];
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const int i0 = a.org_x + blockIdx.x * TX, j0 = 1 + blockIdx.y * TY;
     const bool inner = i0 >= 3 && i0 + TX + 3 <= a.g.nxp && j0 >= 3 && j0 + TY + 3 <= a.g.nyp;
     if (inner) vort_tile<false>(a, s_pm, s_d2, s_d4, s_q, s_p, s_red);
     else vort_tile<true>(a, s_pm, s_d2, s_d4, s_q, s_p, s_red);

@@ -25,17 +25,16 @@ constexpr int YS_CT = 8;   // columns per y-solver workgroup (one wavefront each
 
 // ------------------------------------------------------------------ k_fwd
 
+// p_modes_rhs (modes.F90:259-264) at one point from its loaded inputs:
+// f0 * sum_k (q_k - beta*y - [k=k_topo] ddyn) ctl2m(k,m).
 template <int NL>
-__device__ __forceinline__ double rhs_mode(const FwdArgs& a, int m, int gi, int gj) {
-    // p_modes_rhs (modes.F90:259-264): f0 * sum_k (q - beta*y - [k=k_topo] ddyn) ctl2m(k,m)
+__device__ __forceinline__ double rhs_mode(const FwdArgs& a, int m, const double (&qk)[NL], double by, double dd) {
     const Geo& g = a.g;
-    const long long o = (long long)gj * g.pitch + gi;
-    const double by = g.beta * a.yprel[gj];
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
-        double ql = a.q[k * g.fs + o] - by;
-        if (k == g.kt && a.ddyn) ql = ql - a.ddyn[o];
+        double ql = qk[k] - by;
+        if (k == g.kt) ql = ql - dd;
         s += ql * a.mx.ctl2m[k * MAXL + m];
     }
     return g.fnot * s;
@@ -43,7 +42,7 @@ __device__ __forceinline__ double rhs_mode(const FwdArgs& a, int m, int gi, int 
 
 template <int NL, int MB>
 __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
-    extern __shared__ double2 zbuf[];
+    extern __shared__ double2 zbuf[];   // [L] transform, then the twiddle table
     const Geo& g = a.g;
     const Spec& s = a.s;
     const int L = s.L;
@@ -52,27 +51,49 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
     const int r0 = 2 * jp, r1 = 2 * jp + 1;
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
-    const double2* tw = (const double2*)a.tw;
+    double2* tw = zbuf + L;
+    Staged<2> stw;
+    stage_load(stw, a.tw, twiddle_words(L));
     const bool raw = a.raw_mode >= 0;
     // inputs of this thread's (at most four) points of both rows, loaded together
     const int first = g.cyclic ? 0 : 1;
     const int npts = s.nin;   // box: interior points 1..nin; channel: 0..nin-1
-    double xv[4], yv[4];
+    // every load is issued before any is used: q of all layers (one layer in
+    // raw mode), ddyn and beta*y of both rows
+    constexpr int NQ = 4;
+    const int nlq = raw ? 1 : NL;
+    double q0[NQ][NL], q1[NQ][NL], d0[NQ], d1[NQ];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
+    for (int u = 0; u < NQ; u++) {
         const int t = first + threadIdx.x + 256 * u;
-        xv[u] = 0.0;
-        yv[u] = 0.0;
-        if (t < first + npts) {
-            if (raw) {
-                xv[u] = a.q[(long long)gj0 * g.pitch + t];
-                if (has1) yv[u] = a.q[(long long)gj1 * g.pitch + t];
-            } else {
-                xv[u] = rhs_mode<NL>(a, m, t, gj0);
-                if (has1) yv[u] = rhs_mode<NL>(a, m, t, gj1);
-            }
+        const bool in = t < first + npts;
+        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            q0[u][k] = (in && k < nlq) ? a.q[k * g.fs + o0] : 0.0;
+            q1[u][k] = (in && has1 && k < nlq) ? a.q[k * g.fs + o1] : 0.0;
+        }
+        d0[u] = (in && !raw && a.ddyn) ? a.ddyn[o0] : 0.0;
+        d1[u] = (in && has1 && !raw && a.ddyn) ? a.ddyn[o1] : 0.0;
+    }
+    const double by0 = g.beta * a.yprel[gj0], by1 = has1 ? g.beta * a.yprel[gj1] : 0.0;
+    double xv[NQ], yv[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; u++) {
+        if (raw) {
+            xv[u] = q0[u][0];
+            yv[u] = q1[u][0];
+        } else {
+            xv[u] = rhs_mode<NL>(a, m, q0[u], by0, d0[u]);
+            yv[u] = has1 ? rhs_mode<NL>(a, m, q1[u], by1, d1[u]) : 0.0;
+        }
+        const int t = first + threadIdx.x + 256 * u;
+        if (t >= first + npts) {
+            xv[u] = 0.0;
+            yv[u] = 0.0;
         }
     }
+    stage_store(stw, tw, a.tw, twiddle_words(L));
     if (!g.cyclic) {
         // odd extension of the two rows: z_t = x_t + i y_t, z_{L-t} = -z_t, z_0 = z_N = 0
         if (threadIdx.x == 0) {
@@ -131,9 +152,11 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
         default: break;                                                       \
     }
 
+size_t fwd_lds_bytes(const Spec& s) { return sizeof(double2) * (size_t)(s.L + twiddle_table_len(s.L)); }
+
 void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
     const int npairs = (a.s.nyi + 1) / 2;
-    const size_t lds = sizeof(double2) * (size_t)a.s.L;
+    const size_t lds = fwd_lds_bytes(a.s);
 #define FWD_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd<NLc, MBc>), dim3(npairs, nmodes), dim3(256), lds, st, a)
     OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), FWD_CALL)
 #undef FWD_CALL
@@ -382,12 +405,15 @@ __device__ void small_lu_solve(const BoxHom& h, const double (&rhs)[N], double (
 
 // Per-step correction coefficients, computed by thread 0 of every k_inv block
 // from the block-reduced integrals v[0..nl-1] (box: spectral column sums) and
-// v[MAXL] (entrainment).  Writes coef[] and, in block 0, the new constraint state.
+// v[NL] (entrainment).  Every input sits in LDS: the precompute blocks, the
+// constraint state ci, the channel's boundary integrals x and the k = 0
+// spectra of the first and last interior rows spk[2m], spk[2m+1].  Writes
+// coef[] and, in block 0, the new constraint state.
 template <int NL>
-__device__ void step_coefficients(const InvArgs& a, const double (&v)[MAXL + 1], double* coef, bool write_state) {
+__device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom& bh, const CycHom& ch,
+                                  const double* ci, const double* x, const double* spk, const double (&v)[NL + 1],
+                                  double* coef, bool write_state) {
     const Geo& g = a.g;
-    const Spec& s = a.s;
-    const double* ci = a.cin;
     double dpi[NL], dpip[NL], cs[NL], cn[NL], csp[NL], cnp[NL];
 #pragma unroll
     for (int l = 0; l < NL; l++) {
@@ -395,7 +421,7 @@ __device__ void step_coefficients(const InvArgs& a, const double (&v)[MAXL + 1],
         cs[l] = ci[C_CS + l]; cn[l] = ci[C_CN + l]; csp[l] = ci[C_CSP + l]; cnp[l] = ci[C_CNP + l];
     }
     // update_mass_constr (constraint.F90:167-179) with ent_xn(1) = int_P_dA(ent1)
-    const double exn = (v[MAXL] * g.dx) * g.dy;
+    const double exn = (v[NL] * g.dx) * g.dy;
 #pragma unroll
     for (int l = 0; l < NL - 1; l++) {
         const double e = (l == 0) ? exn : 0.0;
@@ -412,17 +438,16 @@ __device__ void step_coefficients(const InvArgs& a, const double (&v)[MAXL + 1],
         for (int k = 0; k < NL; k++) {
             double t = 0.0;
 #pragma unroll
-            for (int m = 0; m < NL; m++) t += a.mx->ctm2l[m * MAXL + k] * im[m];
+            for (int m = 0; m < NL; m++) t += mx.ctm2l[m * MAXL + k] * im[m];
             ik[k] = t;
         }
 #pragma unroll
         for (int l = 0; l < NL - 1; l++) rhs[l] = dpi[l] - (ik[l + 1] - ik[l]);
-        small_lu_solve<NL - 1>(*a.bh, rhs, alpha);
+        small_lu_solve<NL - 1>(bh, rhs, alpha);
 #pragma unroll
         for (int l = 0; l < NL - 1; l++) coef[l] = alpha[l];
     } else {
         // step_constraints (constraint.F90:182-215)
-        const double* x = a.aux;
         const double f0 = g.fnot, dy = g.dy;
         double rhss[NL], rhsn[NL];
 #pragma unroll
@@ -455,18 +480,17 @@ __device__ void step_coefficients(const InvArgs& a, const double (&v)[MAXL + 1],
             double ts = 0.0, tn = 0.0;
 #pragma unroll
             for (int k = 0; k < NL; k++) {
-                ts += a.mx->ctl2m[k * MAXL + m] * cs[k];
-                tn += a.mx->ctl2m[k * MAXL + m] * cn[k];
+                ts += mx.ctl2m[k * MAXL + m] * cs[k];
+                tn += mx.ctl2m[k * MAXL + m] * cn[k];
             }
-            const double* sp = a.spec + (long long)m * s.nrow * s.kp;
-            clhss[m] = ts + (a.ch->c0 * sp[0]) / dy;
-            clhsn[m] = tn + (a.ch->c0 * sp[(long long)(s.nyi - 1) * s.kp]) / dy;
+            clhss[m] = ts + (ch.c0 * spk[2 * m]) / dy;
+            clhsn[m] = tn + (ch.c0 * spk[2 * m + 1]) / dy;
         }
-        coef[0] = clhss[0] * a.ch->hbsi;
+        coef[0] = clhss[0] * ch.hbsi;
 #pragma unroll
         for (int m = 1; m < NL; m++) {
-            coef[1 + m] = a.ch->hc2n[m] * clhss[m] - a.ch->hc2s[m] * clhsn[m];
-            coef[1 + MAXL + m] = a.ch->hc1s[m] * clhsn[m] - a.ch->hc1n[m] * clhss[m];
+            coef[1 + m] = ch.hc2n[m] * clhss[m] - ch.hc2s[m] * clhsn[m];
+            coef[1 + MAXL + m] = ch.hc1s[m] * clhsn[m] - ch.hc1n[m] * clhss[m];
         }
     }
     if (write_state) {
@@ -541,6 +565,42 @@ __device__ __forceinline__ void qbdy(const InvArgs& a, const double* samat, int 
 // (keeps the register budget of a 1024-thread block out of the way).
 constexpr int inv_group_count(int nm) { return nm <= 3 ? nm : 2; }
 
+// This thread's spectral inputs of the row pair (r0, r0+1) of mode m, for
+// points t = gt + FFT_GROUP*u: a box row holds real values (bx0/bx1 from the
+// two rows), a channel row complex pairs (cx/cy).  Every load is unconditional
+// from a valid address (clamped to the first entry of mode 0 when out of
+// range; row r0+1 exists even when it is not a real row, as the padding row)
+// and nothing is computed from the values here, so that the block's loads all
+// issue before any result is waited on.  Entries outside the row, and the
+// second row when it is padding, hold junk that the caller must not use.
+struct SpecIn {
+    double bx0[4], bx1[4];
+    double2 cx[4], cy[4];
+};
+
+__device__ __forceinline__ void load_spec_pair(const InvArgs& a, int m, bool ok, int r0, int gt, SpecIn& in) {
+    const Spec& s = a.s;
+    const double* row0 = a.spec + ((long long)(ok ? m : 0) * s.nrow + (ok ? r0 : 0)) * s.kp;
+    const double* row1 = row0 + s.kp;
+    if (!a.g.cyclic) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int t = gt + FFT_GROUP * u;
+            const int tc = (t < s.nin) ? t : 0;
+            in.bx0[u] = row0[tc];
+            in.bx1[u] = row1[tc];
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int t = gt + FFT_GROUP * u;
+            const int tc = (t <= s.L / 2) ? t : 0;
+            in.cx[u] = *(const double2*)(row0 + 2 * tc);
+            in.cy[u] = *(const double2*)(row1 + 2 * tc);
+        }
+    }
+}
+
 template <int NL, int MB>
 __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs a) {
     extern __shared__ double smem[];
@@ -553,57 +613,56 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
     const int grp = threadIdx.x / FFT_GROUP, gt = threadIdx.x - grp * FFT_GROUP;
     double2* fbuf = (double2*)smem;                 // [G][L]
     double* inh = (double*)(fbuf + (size_t)G * L);  // [nm][2][pitch]
-    __shared__ double red[(MAXL + 1) * 16];
+    double2* tw = (double2*)(inh + (size_t)nm * 2 * g.pitch);   // twiddle table
+    __shared__ double red[(NL + 1) * 16];
     __shared__ double coef[3 * MAXL + 4];
-    __shared__ double s_ctm2l[MAXL * MAXL];
-    __shared__ double s_amat[MAXL * MAXL];
-    for (int t = threadIdx.x; t < MAXL * MAXL; t += blockDim.x) {
-        s_ctm2l[t] = a.mx->ctm2l[t];
-        s_amat[t] = a.mx->amat[t];
-    }
+    __shared__ DevConsts s_dc;
+    __shared__ double s_cin[C_LEN];
+    __shared__ double s_aux[A_LEN];
+    __shared__ double s_spk[2 * MAXL];
+    // the precompute, the twiddles and the constraint inputs of the correction
+    // coefficients are staged through registers with the block's other global
+    // loads (one memory round trip) and land in LDS, so the serial
+    // coefficient code below reads no global memory
+    const int nw_dc = (int)(sizeof(DevConsts) / 8);
+    const int nw_cin = raw ? 0 : C_LEN, nw_aux = (raw || !g.cyclic) ? 0 : A_LEN;
+    Staged<1> st_dc, st_cin, st_aux;
+    Staged<2> st_tw;
+    stage_load(st_dc, a.dc, nw_dc);
+    stage_load(st_tw, a.tw, twiddle_words(L));
+    stage_load(st_cin, a.cin, nw_cin);
+    stage_load(st_aux, a.aux, nw_aux);
+    const bool want_spk = !raw && g.cyclic && threadIdx.x < 2 * NL;
+    const int m_spk = want_spk ? (int)(threadIdx.x >> 1) : 0;
+    const int r_spk = (want_spk && (threadIdx.x & 1)) ? s.nyi - 1 : 0;
+    const double spk = a.spec[((long long)m_spk * s.nrow + r_spk) * s.kp];   // used only when want_spk
+    const double* s_ctm2l = s_dc.mx.ctm2l;
+    const double* s_amat = s_dc.mx.amat;
     const int jp = blockIdx.x;
     const int r0 = 2 * jp, r1 = 2 * jp + 1;
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
     const bool first = (jp == 0), last = (jp == (int)gridDim.x - 1);
-    const double2* tw = (const double2*)a.tw;
     const int nsel = has1 ? 2 : 1;
 
     // ---- issue the block's global loads together: the integrals behind the
     // correction coefficients, the spectra of the first round of modes, and the
     // homogeneous solutions under this thread's first two output points
-    double v[MAXL + 1];
+    double v[NL + 1];
 #pragma unroll
-    for (int t = 0; t <= MAXL; t++) v[t] = 0.0;
-    double entv[2] = {0.0, 0.0}, colv[2][NL];
+    for (int t = 0; t <= NL; t++) v[t] = 0.0;
+    double entv[2], cw[2], csv[2][NL];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         const int t = threadIdx.x + blockDim.x * u;
-        if (!raw && t < a.nentpart) entv[u] = a.entpart[t];
+        const bool oke = !raw && t < a.nentpart, okc = !raw && !g.cyclic && t < s.ncol;
+        entv[u] = oke ? a.entpart[t] : 0.0;
+        cw[u] = okc ? a.colw[t] : 0.0;
 #pragma unroll
-        for (int m = 0; m < NL; m++)
-            colv[u][m] = (!raw && !g.cyclic && t < s.ncol) ? a.colw[t] * a.colsum[m * s.ncol + t] : 0.0;
+        for (int m = 0; m < NL; m++) csv[u][m] = okc ? a.colsum[m * s.ncol + t] : 0.0;
     }
-    double2 px[4], py[4];
-    {
-        const int m = raw ? a.raw_mode : grp;
-        const bool ok = grp < nm;
-        const double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
-        const double* row1 = row0 + s.kp;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            px[u] = make_double2(0.0, 0.0);
-            py[u] = make_double2(0.0, 0.0);
-            const int t = gt + FFT_GROUP * u;
-            if (!ok) continue;
-            if (!g.cyclic) {
-                if (t < s.nin) px[u] = make_double2(row0[t], has1 ? row1[t] : 0.0);
-            } else if (t <= L / 2) {
-                px[u] = *(const double2*)(row0 + 2 * t);
-                if (has1) py[u] = *(const double2*)(row1 + 2 * t);
-            }
-        }
-    }
+    SpecIn sin;
+    load_spec_pair(a, raw ? a.raw_mode : grp, grp < nm, r0, gt, sin);
     double hv[2][NL > 1 ? NL - 1 : 1];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
@@ -615,22 +674,28 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
             hv[u][m - 1] = (!raw && !g.cyclic && idx < nsel * g.nxp)
                                ? a.hom[(long long)(m - 1) * g.fs + (long long)gj * g.pitch + i] : 0.0;
     }
+    stage_store(st_dc, &s_dc, a.dc, nw_dc);
+    stage_store(st_tw, tw, a.tw, twiddle_words(L));
+    stage_store(st_cin, s_cin, a.cin, nw_cin);
+    stage_store(st_aux, s_aux, a.aux, nw_aux);
+    if (want_spk) s_spk[threadIdx.x] = spk;
 
     // ---- correction coefficients (identical in every block)
     if (!raw) {
-        v[MAXL] = entv[0] + entv[1];
-        for (int t = threadIdx.x + 2 * blockDim.x; t < a.nentpart; t += blockDim.x) v[MAXL] += a.entpart[t];
+        v[NL] = entv[0] + entv[1];
+        for (int t = threadIdx.x + 2 * blockDim.x; t < a.nentpart; t += blockDim.x) v[NL] += a.entpart[t];
         if (!g.cyclic) {
 #pragma unroll
-            for (int m = 0; m < NL; m++) v[m] = colv[0][m] + colv[1][m];
+            for (int m = 0; m < NL; m++) v[m] = cw[0] * csv[0][m] + cw[1] * csv[1][m];
             for (int col = threadIdx.x + 2 * blockDim.x; col < s.ncol; col += blockDim.x) {
                 const double w = a.colw[col];
 #pragma unroll
                 for (int m = 0; m < NL; m++) v[m] += w * a.colsum[m * s.ncol + col];
             }
         }
-        block_sum<MAXL + 1>(v, red);
-        if (threadIdx.x == 0) step_coefficients<NL>(a, v, coef, blockIdx.x == 0);
+        block_sum<NL + 1>(v, red);   // its barriers also publish the LDS copies above
+        if (threadIdx.x == 0)
+            step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, blockIdx.x == 0);
     }
     __syncthreads();
 
@@ -641,23 +706,7 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
         const bool valid = mi < nm;
         const int m = raw ? a.raw_mode : mi;
         double2* z = fbuf + (size_t)grp * L;
-        if (valid && m0 > 0) {
-            // later rounds (nl > 3) load their spectra here
-            const double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
-            const double* row1 = row0 + s.kp;
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int t = gt + FFT_GROUP * u;
-                px[u] = make_double2(0.0, 0.0);
-                py[u] = make_double2(0.0, 0.0);
-                if (!g.cyclic) {
-                    if (t < s.nin) px[u] = make_double2(row0[t], has1 ? row1[t] : 0.0);
-                } else if (t <= L / 2) {
-                    px[u] = *(const double2*)(row0 + 2 * t);
-                    if (has1) py[u] = *(const double2*)(row1 + 2 * t);
-                }
-            }
-        }
+        if (m0 > 0) load_spec_pair(a, m, valid, r0, gt, sin);   // later rounds (nl > 3)
         if (valid) {
             if (!g.cyclic) {
                 // odd extension of ftnorm*u for the two rows (inhomog.F90:202-203)
@@ -669,7 +718,7 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
                 for (int u = 0; u < 4; u++) {
                     const int t = gt + FFT_GROUP * u;   // spectral index K-1
                     if (t < s.nin) {
-                        const double2 w = make_double2(fn * px[u].x, fn * px[u].y);
+                        const double2 w = make_double2(fn * sin.bx0[u], has1 ? fn * sin.bx1[u] : 0.0);
                         z[t + 1] = w;
                         z[L - t - 1] = make_double2(-w.x, -w.y);
                     }
@@ -681,7 +730,8 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
                 for (int u = 0; u < 4; u++) {
                     const int kk = gt + FFT_GROUP * u;
                     if (kk <= h) {
-                        double xr = fn * px[u].x, xi = fn * px[u].y, yr = fn * py[u].x, yi = fn * py[u].y;
+                        double xr = fn * sin.cx[u].x, xi = fn * sin.cx[u].y;
+                        double yr = has1 ? fn * sin.cy[u].x : 0.0, yi = has1 ? fn * sin.cy[u].y : 0.0;
                         if (kk == 0 || (N % 2 == 0 && kk == h)) { xi = 0.0; yi = 0.0; }
                         z[kk] = make_double2(xr - yi, xi + yr);
                         if (kk != 0 && !(N % 2 == 0 && kk == h)) z[N - kk] = make_double2(xr + yi, -xi + yr);
@@ -775,7 +825,8 @@ int inv_block_count(const Spec& s) { return (s.nyi + 1) / 2; }
 static int inv_groups(const InvArgs& a) { return inv_group_count((a.raw_mode >= 0) ? 1 : a.g.nl); }
 
 size_t inv_lds_bytes(const Spec& s, const Geo& g, int nm) {
-    return sizeof(double2) * (size_t)inv_group_count(nm) * s.L + sizeof(double) * 2 * (size_t)nm * g.pitch;
+    return sizeof(double2) * ((size_t)inv_group_count(nm) * s.L + twiddle_table_len(s.L)) +
+           sizeof(double) * 2 * (size_t)nm * g.pitch;
 }
 
 void launch_inv(const InvArgs& a, hipStream_t st) {
@@ -790,7 +841,7 @@ void launch_inv(const InvArgs& a, hipStream_t st) {
 
 // Allow the transforms of this basin more than the default 64 KB of dynamic LDS.
 void set_kernel_attrs(const Spec& s, const Geo& g) {
-    const int fl = (int)(sizeof(double2) * (size_t)s.L), il = (int)inv_lds_bytes(s, g, g.nl);
+    const int fl = (int)fwd_lds_bytes(s), il = (int)inv_lds_bytes(s, g, g.nl);
 #define ATTR_CALL(NLc, MBc)                                                                                       \
     do {                                                                                                          \
         (void)hipFuncSetAttribute((const void*)k_fwd<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, fl); \

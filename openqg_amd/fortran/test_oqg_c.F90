@@ -2,9 +2,10 @@
 !
 ! Builds a 65x65, 3-layer double-gyre box, sets p = pm to a smooth analytic
 ! field, derives q (init_pv), steps 10 times with the leapfrog suppression at
-! step 1, and writes q, qm, p, pm (raw FP64, Fortran order) to the file named
-! on the command line.  tests/test_gpu_fortran.py runs the same case through
-! the Python binding and checks the two agree bitwise.
+! step 1, and writes the inputs p, wekp and then q, qm, p, pm (raw FP64,
+! Fortran order) to the file named on the command line.
+! tests/test_gpu_fortran.py runs the same case through the Python binding and
+! checks the two agree bitwise.
 program test_oqg_c
 
   use, intrinsic :: iso_c_binding
@@ -63,6 +64,9 @@ program test_oqg_c
      enddo
   enddo
 
+  open(unit=10, file=trim(outpath), access='stream', form='unformatted', status='replace')
+  write(10) p, wekp
+
   call ok(oqg_set_state(ctx, c_loc(q), c_loc(qm), c_loc(p), c_loc(pm), OQG_HOST))
   call ok(oqg_init_state(ctx))
   do nt = 1, 10
@@ -72,7 +76,6 @@ program test_oqg_c
   call ok(oqg_get_state(ctx, c_loc(q), c_loc(qm), c_loc(p), c_loc(pm), OQG_HOST))
   call oqg_destroy(ctx)
 
-  open(unit=10, file=trim(outpath), access='stream', form='unformatted', status='replace')
   write(10) q, qm, p, pm
   close(10)
   print '(a,es24.16)', 'fortran oqg: max|p| = ', maxval(abs(p))
