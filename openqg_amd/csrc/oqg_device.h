@@ -86,54 +86,90 @@ __device__ __forceinline__ double jac_lds(const double* q, const double* p, int 
 
 // Global -> LDS copies split in two halves so that a block issues all of its
 // global loads before it waits for any: stage_load reads this thread's first
-// NPT words of `src` into registers (straight-line, predicated), stage_store
-// later writes them to LDS and copies any remainder (n > NPT * blockDim) with a
-// strided loop.  The caller's next barrier publishes the copy.
-template <int NPT>
+// NPT elements (of type T) of `src` into registers (straight-line,
+// predicated), stage_store later writes them to LDS and copies any remainder
+// (n > NPT * blockDim) with a strided loop.  The caller's next barrier
+// publishes the copy.
+template <int NPT, class T = unsigned long long>
 struct Staged {
-    unsigned long long w[NPT];
+    T w[NPT];
 };
 
-template <int NPT>
-__device__ __forceinline__ void stage_load(Staged<NPT>& r, const void* src, int nwords) {
-    const unsigned long long* s = static_cast<const unsigned long long*>(src);
+template <int NPT, class T>
+__device__ __forceinline__ void stage_load(Staged<NPT, T>& r, const void* src, int n) {
+    const T* s = static_cast<const T*>(src);
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
         const int idx = threadIdx.x + p * blockDim.x;
-        r.w[p] = (idx < nwords) ? s[idx] : 0ull;
+        r.w[p] = s[(idx < n) ? idx : 0];
     }
 }
 
-template <int NPT>
-__device__ __forceinline__ void stage_store(const Staged<NPT>& r, void* dst, const void* src, int nwords) {
-    unsigned long long* d = static_cast<unsigned long long*>(dst);
+template <int NPT, class T>
+__device__ __forceinline__ void stage_store(const Staged<NPT, T>& r, void* dst, const void* src, int n) {
+    T* d = static_cast<T*>(dst);
 #pragma unroll
     for (int p = 0; p < NPT; p++) {
         const int idx = threadIdx.x + p * blockDim.x;
-        if (idx < nwords) d[idx] = r.w[p];
+        if (idx < n) d[idx] = r.w[p];
     }
-    const unsigned long long* s = static_cast<const unsigned long long*>(src);
-    for (int idx = threadIdx.x + NPT * blockDim.x; idx < nwords; idx += blockDim.x) d[idx] = s[idx];
+    const T* s = static_cast<const T*>(src);
+    for (int idx = threadIdx.x + NPT * blockDim.x; idx < n; idx += blockDim.x) d[idx] = s[idx];
 }
 
-// Deterministic block sum of NV values per thread: a fixed butterfly inside
-// each wavefront, then the wavefront sums in wavefront order.  Every thread
+// x from another lane through a DPP (data-parallel primitive) move of both
+// 32-bit halves; CTRL is the gfx9 dpp_ctrl code.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double read_lane(double x, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wavefront, the same value in every lane and the
+// same order of additions in every lane (each pairwise step adds the partner's
+// partial sum, and addition commutes): lane pairs, quads (quad_perm), half
+// rows (row_half_mirror), rows of 16 (row_mirror), then the four row sums.
+// Pure VALU cross-lane moves: no LDS traffic.
+__device__ __forceinline__ double wave_sum(double x) {
+    x = x + dpp_move<0xB1>(x);    // quad_perm [1,0,3,2]
+    x = x + dpp_move<0x4E>(x);    // quad_perm [2,3,0,1]
+    x = x + dpp_move<0x141>(x);   // row_half_mirror
+    x = x + dpp_move<0x140>(x);   // row_mirror
+    return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
+}
+
+// Deterministic block sum of NV values per thread: wave_sum inside each
+// wavefront, then the wavefront sums in wavefront order.  Every thread
 // returns the same value.  red holds NV*16 doubles; blockDim.x <= 1024.
 template <int NV>
 __device__ void block_sum(double (&v)[NV], double* red) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
-    for (int i = 0; i < NV; i++)
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
+    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
     if (lane == 0)
 #pragma unroll
         for (int i = 0; i < NV; i++) red[i * 16 + w] = v[i];
     __syncthreads();
+    // all 16 slots are read at once (the ones of absent wavefronts are
+    // replaced by exact zeros), then summed in wavefront order
 #pragma unroll
     for (int i = 0; i < NV; i++) {
-        double s = red[i * 16];
-        for (int k = 1; k < nw; k++) s += red[i * 16 + k];
+        double part[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const double r = red[i * 16 + k];
+            part[k] = (k < nw) ? r : 0.0;
+        }
+        double s = part[0];
+#pragma unroll
+        for (int k = 1; k < 16; k++) s += part[k];
         v[i] = s;
     }
     __syncthreads();
@@ -143,32 +179,20 @@ __device__ void block_sum(double (&v)[NV], double* red) {
 // Stockham autosort (Govindaraju et al. 2008) with mixed radix 2/3/4/5, one
 // complex buffer of length L per transform; each of the FFT_GROUP threads of a
 // group holds up to FFT_MAXB butterflies in registers across the barrier that
-// separates a stage's reads from its writes.
+// separates a stage's reads from its writes.  Twiddles come from an LDS copy of
+// the table tw[t] = exp(-2 pi i t / L), t < L.  SIGN = -1 is the forward
+// transform, +1 the backward one (conjugate twiddles).
 //
-// Twiddles come from an LDS table: when 4 | L it holds the first quadrant only,
-// tw[r] = exp(-2 pi i r / L) for r < L/4, and exp(-2 pi i (q L/4 + r) / L) is
-// tw[r] times (-i)^q, an exact swap and sign change (the host table is built
-// by the same quadrant reduction, so the values are identical); otherwise it
-// holds all L entries.
+// Each butterfly index t stays with its thread through all stages; the stage
+// with Ns = R_0...R_{s-1} needs k = t mod Ns and j = t / Ns, which are carried
+// from stage to stage (k += Ns (j mod R), j /= R with R a compile-time radix)
+// instead of being recomputed with runtime divisions.
 
-// Entries of the LDS twiddle table for a transform of length L.
-__host__ __device__ inline int twiddle_table_len(int L) { return (L % 4 == 0) ? L / 4 : L; }
+// Entries of the LDS twiddle table of a length-L transform.
+__host__ __device__ inline int twiddle_table_len(int L) { return L; }
 
-// Branch-free: when 4 does not divide L the quadrant width is L itself and q = 0.
-__device__ __forceinline__ double2 twid(const double2* tw, int t, int L) {
-    const int L4 = (L & 3) ? L : (L >> 2);
-    const int q = (t >= L4) + (t >= 2 * L4) + (t >= 3 * L4);
-    const double2 w = tw[t - q * L4];
-    // q = 0: (x, y); 1: (y, -x); 2: (-x, -y); 3: (-y, x)
-    const double u = (q & 1) ? w.y : w.x, v = (q & 1) ? w.x : w.y;
-    return make_double2((q >= 2) ? -u : u, (q == 1 || q == 2) ? -v : v);
-}
-
-// 8-byte words of the LDS twiddle table of a length-L transform.
-__host__ __device__ inline int twiddle_words(int L) { return 2 * twiddle_table_len(L); }
-
-template <int R>
-__device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, int L, int sign) {
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, int L) {
     if constexpr (R == 2) {
         const double2 a = v[0], b = v[1];
         v[0] = make_double2(a.x + b.x, a.y + b.y);
@@ -178,7 +202,7 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
         const double2 a1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
         const double2 b0 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
         const double2 b1 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
-        const double2 j = (sign < 0) ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
+        const double2 j = (SIGN < 0) ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
         v[0] = make_double2(a0.x + b0.x, a0.y + b0.y);
         v[1] = make_double2(a1.x + j.x, a1.y + j.y);
         v[2] = make_double2(a0.x - b0.x, a0.y - b0.y);
@@ -188,13 +212,13 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
 #pragma unroll
         for (int r = 0; r < R; r++) in[r] = v[r];
         const int pstep = L / R;
-        const double sg = (sign > 0) ? -1.0 : 1.0;
+        constexpr double sg = (SIGN > 0) ? -1.0 : 1.0;
 #pragma unroll
         for (int q = 0; q < R; q++) {
             double sr = in[0].x, si = in[0].y;
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                const double2 w = twid(tw, ((r * q) % R) * pstep, L);
+                const double2 w = tw[((r * q) % R) * pstep];
                 const double c = w.x, s = sg * w.y;
                 sr += in[r].x * c - in[r].y * s;
                 si += in[r].x * s + in[r].y * c;
@@ -204,39 +228,41 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
     }
 }
 
-template <int R, int MB>
-__device__ __forceinline__ void fft_stage(double2* buf, int L, int Ns, const double2* tw, int sign, int gt) {
-    const int LR = L / R;
-    const int tstep = L / (Ns * R);
+// One radix-R stage.  LR = L/R butterflies, tstep = L/(Ns R); kk/jj carry
+// k = t mod Ns and j = t / Ns of this thread's butterflies t = gt + b*FFT_GROUP
+// and are advanced to the next stage on return.
+template <int R, int MB, int SIGN>
+__device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tstep, const double2* tw, int gt,
+                                          int (&kk)[MB], int (&jj)[MB]) {
     double2 v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; b++) {
         const int t = gt + b * FFT_GROUP;
         if (t < LR) {
-            const int k = t % Ns;
+            const int k = kk[b];
 #pragma unroll
             for (int r = 0; r < R; r++) v[b][r] = buf[t + r * LR];
-            if (k != 0) {
 #pragma unroll
-                for (int r = 1; r < R; r++) {
-                    double2 w = twid(tw, r * k * tstep, L);
-                    if (sign > 0) w.y = -w.y;
-                    v[b][r] = cmul(v[b][r], w);
-                }
+            for (int r = 1; r < R; r++) {
+                double2 w = tw[r * k * tstep];   // k = 0 gives exactly (1, 0)
+                if (SIGN > 0) w.y = -w.y;
+                v[b][r] = cmul(v[b][r], w);
             }
-            dft_small<R>(v[b], tw, L, sign);
+            dft_small<R, SIGN>(v[b], tw, LR * R);
         }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < MB; b++) {
         const int t = gt + b * FFT_GROUP;
+        const int j = jj[b], k = kk[b];
         if (t < LR) {
-            const int k = t % Ns;
-            const int base = (t / Ns) * Ns * R + k;
+            const int base = j * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; r++) buf[base + r * Ns] = v[b][r];
         }
+        kk[b] = k + Ns * (j % R);
+        jj[b] = j / R;
     }
     __syncthreads();
 }
@@ -245,16 +271,23 @@ __device__ __forceinline__ void fft_stage(double2* buf, int L, int Ns, const dou
 // gt = thread index within its group; MB >= the most butterflies a thread owns
 // in any stage.  Every thread of the block must call this (it contains block
 // barriers).
-template <int MB>
-__device__ __forceinline__ void fft_lds(double2* buf, const Spec& s, const double2* tw, int sign, int gt) {
+template <int MB, int SIGN>
+__device__ __forceinline__ void fft_lds(double2* buf, const Spec& s, const double2* tw, int gt) {
+    int kk[MB], jj[MB];
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        kk[b] = 0;
+        jj[b] = gt + b * FFT_GROUP;
+    }
     int Ns = 1;
     for (int st = 0; st < s.nstages; st++) {
         const int R = s.radix[st];
+        const int LR = s.L / R, tstep = s.L / (Ns * R);   // uniform (scalar) arithmetic
         switch (R) {
-            case 4: fft_stage<4, MB>(buf, s.L, Ns, tw, sign, gt); break;
-            case 2: fft_stage<2, MB>(buf, s.L, Ns, tw, sign, gt); break;
-            case 3: fft_stage<3, MB>(buf, s.L, Ns, tw, sign, gt); break;
-            default: fft_stage<5, MB>(buf, s.L, Ns, tw, sign, gt); break;
+            case 4: fft_stage<4, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
+            case 2: fft_stage<2, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
+            case 3: fft_stage<3, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
+            default: fft_stage<5, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
         }
         Ns *= R;
     }

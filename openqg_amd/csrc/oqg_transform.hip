@@ -52,8 +52,8 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
     double2* tw = zbuf + L;
-    Staged<2> stw;
-    stage_load(stw, a.tw, twiddle_words(L));
+    Staged<4, double2> stw;   // the whole table for L <= 1024
+    stage_load(stw, a.tw, L);
     const bool raw = a.raw_mode >= 0;
     // inputs of this thread's (at most four) points of both rows, loaded together
     const int first = g.cyclic ? 0 : 1;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
             yv[u] = 0.0;
         }
     }
-    stage_store(stw, tw, a.tw, twiddle_words(L));
+    stage_store(stw, tw, a.tw, L);
     if (!g.cyclic) {
         // odd extension of the two rows: z_t = x_t + i y_t, z_{L-t} = -z_t, z_0 = z_N = 0
         if (threadIdx.x == 0) {
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
         }
     }
     __syncthreads();
-    fft_lds<MB>(zbuf, s, tw, -1, threadIdx.x);
+    fft_lds<MB, -1>(zbuf, s, tw, threadIdx.x);
     double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
     double* row1 = row0 + s.kp;
     if (!g.cyclic) {
@@ -627,10 +627,10 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
     const int nw_dc = (int)(sizeof(DevConsts) / 8);
     const int nw_cin = raw ? 0 : C_LEN, nw_aux = (raw || !g.cyclic) ? 0 : A_LEN;
     Staged<1> st_dc, st_cin, st_aux;
-    Staged<2> st_tw;
+    Staged<2, double2> st_tw;   // the whole table for L <= 2*blockDim
     stage_load(st_dc, a.dc, nw_dc);
-    stage_load(st_tw, a.tw, twiddle_words(L));
-    stage_load(st_cin, a.cin, nw_cin);
+    stage_load(st_tw, a.tw, L);
+    stage_load(st_cin, nw_cin ? (const void*)a.cin : (const void*)a.dc, nw_cin);   // a.cin is null in raw mode
     stage_load(st_aux, a.aux, nw_aux);
     const bool want_spk = !raw && g.cyclic && threadIdx.x < 2 * NL;
     const int m_spk = want_spk ? (int)(threadIdx.x >> 1) : 0;
@@ -675,12 +675,13 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
                                ? a.hom[(long long)(m - 1) * g.fs + (long long)gj * g.pitch + i] : 0.0;
     }
     stage_store(st_dc, &s_dc, a.dc, nw_dc);
-    stage_store(st_tw, tw, a.tw, twiddle_words(L));
+    stage_store(st_tw, tw, a.tw, L);
     stage_store(st_cin, s_cin, a.cin, nw_cin);
     stage_store(st_aux, s_aux, a.aux, nw_aux);
     if (want_spk) s_spk[threadIdx.x] = spk;
 
-    // ---- correction coefficients (identical in every block)
+    // this thread's share of the integrals behind the correction coefficients;
+    // the block reduction waits until after the transforms, which do not need it
     if (!raw) {
         v[NL] = entv[0] + entv[1];
         for (int t = threadIdx.x + 2 * blockDim.x; t < a.nentpart; t += blockDim.x) v[NL] += a.entpart[t];
@@ -693,13 +694,9 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
                 for (int m = 0; m < NL; m++) v[m] += w * a.colsum[m * s.ncol + col];
             }
         }
-        block_sum<NL + 1>(v, red);   // its barriers also publish the LDS copies above
-        if (threadIdx.x == 0)
-            step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, blockIdx.x == 0);
     }
-    __syncthreads();
 
-    // ---- inverse x transforms, G modes at a time
+    // ---- inverse x transforms, G modes at a time (their barriers publish the LDS copies above)
     const double fn = s.ftnorm;
     for (int m0 = 0; m0 < nm; m0 += G) {
         const int mi = m0 + grp;
@@ -740,7 +737,8 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
             }
         }
         __syncthreads();
-        fft_lds<MB>(z, s, tw, g.cyclic ? +1 : -1, gt);
+        if (g.cyclic) fft_lds<MB, +1>(z, s, tw, gt);   // c2r (inhomog.F90:247)
+        else fft_lds<MB, -1>(z, s, tw, gt);
         if (valid) {
             double* o0 = inh + (size_t)mi * 2 * g.pitch;
             double* o1 = o0 + g.pitch;
@@ -757,6 +755,14 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
                 o1[i] = x1;
             }
         }
+        __syncthreads();
+    }
+
+    // ---- correction coefficients (identical in every block)
+    if (!raw) {
+        block_sum<NL + 1>(v, red);
+        if (threadIdx.x == 0)
+            step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, blockIdx.x == 0);
         __syncthreads();
     }
 

@@ -20,7 +20,7 @@ STEP_TOL = 1e-12
 TRAJ_TOL = 1e-9
 
 
-def _setup(w, flags=0):
+def _setup(w, flags=0, ent_amp=0.0):
     qg = w.make_qg(flags=flags)
     o = cr.Basin(w.params())
     p = w.initial_p()
@@ -28,6 +28,12 @@ def _setup(w, flags=0):
         b.set_state(p=p, pm=p)
         b.init_state()
     ent, wekp, txis, txin = w.forcing()
+    if ent_amp:
+        # seeded entrainment on every layer interface (m/s); a channel's
+        # duplicate column nxp equals column 1
+        ent = ent_amp * np.random.default_rng(w.seed + 100).standard_normal(ent.shape)
+        if w.cyclic:
+            ent[..., -1] = ent[..., 0]
     qg.set_forcing(ent, wekp, txis, txin)
     return qg, o, (ent, wekp, txis, txin)
 
@@ -38,8 +44,8 @@ def _constr_err(a, b):
     return float(np.max(np.abs(a - b) / scale))
 
 
-def _per_step(w, nsteps, flags=0, tol=STEP_TOL):
-    qg, o, (ent, wekp, txis, txin) = _setup(w, flags)
+def _per_step(w, nsteps, flags=0, tol=STEP_TOL, ent_amp=0.0):
+    qg, o, (ent, wekp, txis, txin) = _setup(w, flags, ent_amp)
     print(f"[parity] {w.name} nsteps={nsteps} flags={flags}", flush=True)
     gq, gqm, gp, gpm = qg.get_state()
     oq, oqm, op, opm = o.get_state()
@@ -72,6 +78,15 @@ def _per_step(w, nsteps, flags=0, tol=STEP_TOL):
 @pytest.mark.parametrize("make", [W.small_box, W.small_channel], ids=["box33", "chan33x17"])
 def test_step_parity_small(gpu, make):
     _per_step(make(), 30)
+
+
+@pytest.mark.parametrize("make", [lambda: W.small_box(n=65), W.small_channel, W.atmos_channel],
+                         ids=["box65", "chan33x17", "atmos97x49"])
+def test_step_parity_with_entrainment(gpu, make):
+    """Entrainment forcing of layers 1 and 2 (vorticity.F90:71-72), the mass
+    constraint's ent integral (constraint.F90:167-179) and, in a channel, the
+    layer-2 edge rows in the momentum constraints (vorticity.F90:88-93)."""
+    _per_step(make(), 12, ent_amp=1e-6)
 
 
 @pytest.mark.parametrize("nl", [2, 4])
