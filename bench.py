@@ -10,7 +10,10 @@ Also reported:
   roofline      the dominant kernel's algorithmic bytes per launch / its mean
                 duration (HIP events on the library's stream), against 8 TB/s;
   cpu_baseline  the CPU oracle (a C restatement of the reference step) on the
-                host cores, on a bounded sample of the same workload.
+                host cores, on a bounded sample of the same workload;
+  ensemble      64 uneven basins (65^2..513^2, 2-4 layers) for 10 simulated
+                days each: batched launches against the same members run one
+                after another by the plain path (aggregate simulated days/s).
 """
 from __future__ import annotations
 
@@ -89,6 +92,72 @@ def cpu_baseline(w, seconds: float):
     }
 
 
+def ensemble_bench(members: int, days: float):
+    """The uneven ensemble (grids 65^2..513^2, 2-4 layers, seeded wind, viscosity
+    and PV noise; each member its stable time step) for `days` simulated days
+    per member: batched (oqg_ens_*, launch schedule picked by the tuner) against
+    the same members advanced one after another by the plain graph path.  Both
+    from the same initial states, resident forcing, suppression every 25 steps."""
+    from openqg_amd import workloads as W
+    from openqg_amd.ensemble import Ensemble
+
+    ws = W.uneven_ensemble(members, days=days)
+
+    def make():
+        qgs = []
+        for w in ws:
+            qg = w.make_qg()
+            p = w.initial_p()
+            qg.set_state(p=p, pm=p)
+            qg.init_state()
+            ent, wekp, txis, txin = w.forcing()
+            qg.set_forcing(ent, wekp, txis, txin)
+            qg.sync()
+            qgs.append(qg)
+        return qgs
+
+    qgs = make()
+    tdt = [w.tdt for w in ws]
+    nsteps = [w.nsteps for w in ws]
+    sim_days = sum(n * w.dt for n, w in zip(nsteps, ws)) / 86400.0
+    # plain path, one member after another (each warmed up: graph captured)
+    for w, qg in zip(ws, qgs):
+        qg.run(w.tdt, 2 * w.supp_every, 0, w.supp_every)
+        qg.sync()
+    t0 = time.perf_counter()
+    for w, qg, n in zip(ws, qgs, nsteps):
+        qg.run(w.tdt, n, 2 * w.supp_every, w.supp_every)
+    for qg in qgs:
+        qg.sync()
+    seq = time.perf_counter() - t0
+    # batched: tune the launch schedule, warm up (plans and graphs), time
+    ens = Ensemble(qgs)
+    sched, tune_ms = ens.tune(tdt, trial_steps=2 * 25, supp_every=25)
+    ens.run(tdt, [2 * 25] * len(ws), 0, 25)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.run(tdt, nsteps, 2 * 25, 25)
+    ens.sync()
+    bat = time.perf_counter() - t0
+    ens.close()
+    for qg in qgs:
+        qg.close()
+    return {
+        "workload": f"uneven_ensemble_{members}x{days:g}d",
+        "members": members,
+        "grids": sorted({f"{w.nxp}^2x{w.nl}" for w in ws}),
+        "member_steps": int(sum(nsteps)),
+        "value": round(sim_days / bat, 2),
+        "sequential_value": round(sim_days / seq, 2),
+        "unit": "aggregate simulated days/s",
+        "speedup_vs_sequential": round(seq / bat, 2),
+        "batched_s": round(bat, 4),
+        "sequential_s": round(seq, 4),
+        "schedule": sched,
+        "tune_ms": [round(t, 3) for t in tune_ms],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +168,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--flags", type=int, default=0, help="oqg_desc.flags (1: per-thread Thomas y-solve)")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of replaying graphs")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ens-members", type=int, default=64)
+    ap.add_argument("--ens-days", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 10 device copies of 256 MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -171,6 +243,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_seconds)
+    qg.close()
+    ens = None if args.no_ensemble else ensemble_bench(args.ens_members, args.ens_days)
     out = {
         "metric": METRIC,
         "value": round(days_per_s, 3),
@@ -188,6 +262,7 @@ def main():
                    "supp_every": w.supp_every, "y_solver": "thomas" if args.flags & 1 else "wavefront-pcr"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

@@ -41,6 +41,12 @@ enum {
     OQG_FLAG_YSOLVE_THOMAS = 1   /* y-tridiagonal by per-thread Thomas instead of the
                                     wavefront partition + PCR solver */
 };
+/* Ensemble launch schedules (bits), see oqg_ens_set_schedule. */
+enum {
+    OQG_ENS_ONE_YSOLVE = 1,      /* one y-solve launch for all members (rows per lane of
+                                    the largest) instead of one per rows-per-lane class */
+    OQG_ENS_LARGEST_FIRST = 2    /* members ordered by size, largest first, in every launch */
+};
 
 /*
  * Basin description: the fields of box_type (src/glam/box.F90:11-41),
@@ -164,6 +170,39 @@ int oqg_state_ptrs(oqg_ctx* ctx, double** q, double** qm, double** p, double** p
  * error. */
 int oqg_kernel_times(oqg_ctx* ctx, double tdt, int reps, double* ms, int max, char* names,
                      int names_len);
+
+/*
+ * Batched ensembles.  No counterpart in the reference, whose driver steps one
+ * basin per step_qg call (src/driver.F90:107,143); an ensemble of independent
+ * basins there is a set of separate runs.  Here the members, ordinary contexts
+ * of any sizes and layer counts, advance together: each phase of the step is
+ * one launch over the blocks of every member (per class of template
+ * parameters), on the ensemble's own stream, replayed from HIP graphs.  A
+ * member's results are those of stepping it alone.  While an ensemble runs,
+ * its members must not be used otherwise.
+ */
+typedef struct oqg_ens oqg_ens;
+
+/* Members must stay alive until oqg_ens_destroy. */
+int oqg_ens_create(oqg_ctx* const* members, int n, oqg_ens** out);
+void oqg_ens_destroy(oqg_ens* ens);
+const char* oqg_ens_last_error(const oqg_ens* ens);
+
+/* Advances member i by nsteps[i] steps of step_qg with time step tdt[i] on its
+ * resident forcing, all members in lockstep from global step step0 + 1 (a
+ * member stops after its own count), with suppress_comp_mode after the steps
+ * nt with mod(nt-1, supp_every) = 0 (0: never) as src/openqg.F90:485-487.
+ * Asynchronous on the ensemble stream; see oqg_ens_sync. */
+int oqg_ens_run(oqg_ens* ens, const double* tdt, const int* nsteps, int step0, int supp_every);
+int oqg_ens_sync(oqg_ens* ens);
+
+/* Launch schedule (OQG_ENS_* bits); oqg_ens_tune picks the fastest one by
+ * timing trial_steps steps of each on the members (whose state it restores)
+ * and returns it (negative on error); ms[] (4 entries, may be NULL) receives
+ * each schedule's time. */
+int oqg_ens_set_schedule(oqg_ens* ens, int schedule);
+int oqg_ens_get_schedule(const oqg_ens* ens);
+int oqg_ens_tune(oqg_ens* ens, const double* tdt, int trial_steps, int supp_every, double* ms);
 
 #ifdef __cplusplus
 }

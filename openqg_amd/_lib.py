@@ -16,6 +16,8 @@ MAX_LAYERS = 8
 OQG_HOST, OQG_DEVICE = 0, 1
 FIELD_Q, FIELD_QM, FIELD_P, FIELD_PM = 0, 1, 2, 3
 FLAG_YSOLVE_THOMAS = 1
+ENS_ONE_YSOLVE = 1
+ENS_LARGEST_FIRST = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -65,6 +67,14 @@ SIGNATURES = [
     ("oqg_state_ptrs", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                             ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_size_t)]),
     ("oqg_kernel_times", _i, [_vp, _d, _i, _dp, _i, ctypes.c_char_p, _i]),
+    ("oqg_ens_create", _i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
+    ("oqg_ens_destroy", None, [_vp]),
+    ("oqg_ens_last_error", ctypes.c_char_p, [_vp]),
+    ("oqg_ens_run", _i, [_vp, _dp, ctypes.POINTER(_i), _i, _i]),
+    ("oqg_ens_sync", _i, [_vp]),
+    ("oqg_ens_set_schedule", _i, [_vp, _i]),
+    ("oqg_ens_get_schedule", _i, [_vp]),
+    ("oqg_ens_tune", _i, [_vp, _dp, _i, _i, _dp]),
 ]
 
 _lib = None

@@ -549,8 +549,8 @@ static int setup(oqg_ctx* c) {
         for (int st = 0; st < s.nstages; st++)   // each thread holds at most 4 butterflies per stage
             if (s.L / s.radix[st] > 4 * 256) return fail(c, OQG_ERR_ARG, "x transform too long for the LDS FFT");
     }
-    if (inv_lds_bytes(s, g, nl) > 150 * 1024)
-        return fail(c, OQG_ERR_ARG, "grid too wide for the LDS-staged x transforms");
+    if (inv_lds_bytes(s, g, nl) > MAX_DYN_LDS || fwd_lds_bytes(s) > MAX_DYN_LDS || ysolve_lds_bytes(s) > MAX_DYN_LDS)
+        return fail(c, OQG_ERR_ARG, "grid too large for the LDS-staged transforms");
 
     // modes
     if (d.ctl2m && d.ctm2l && d.rdm2 && d.amat) {
@@ -646,7 +646,7 @@ static int setup(oqg_ctx* c) {
         }
         HIPCHK(c, hipMemcpyAsync(c->colw, colw.data(), colw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
     }
-    set_kernel_attrs(s, g);
+    set_kernel_attrs();
 
     // homogeneous solutions (homsol, homog.F90:108-278) from this library's own solver
     const size_t nxy = (size_t)nxp * nyp;
@@ -1146,6 +1146,425 @@ int oqg_kernel_times(oqg_ctx* c, double tdt, int reps, double* ms, int max, char
         names[names_len - 1] = '\0';
     }
     return n;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ ensembles
+//
+// A batched ensemble advances many basins together.  Each phase of the step is
+// one launch over the blocks of all basins that share its template parameters
+// (layer count and FFT butterflies for the x transforms, rows per lane for the
+// y solve), driven by a block map (member, member's own block).  The members'
+// argument blocks live in device memory, one table per time-level parity.  The
+// set of members with steps left and its launch schedule form a plan; lcm(2,
+// supp) steps of a plan are captured in a HIP graph and replayed.
+
+namespace {
+
+enum { ENS_VORT = 0, ENS_FWD, ENS_YS, ENS_INV };
+
+struct EnsLaunch {
+    int kind = ENS_VORT;
+    int nl = 0, mb = 0, R = 0;
+    size_t lds = 0;
+    int nblocks = 0;
+    int2* map = nullptr;
+};
+
+struct EnsPlan {
+    std::vector<int> members;   // indices into oqg_ens::m, in launch order
+    VortArgs* va[2] = {nullptr, nullptr};
+    FwdArgs* fa[2] = {nullptr, nullptr};
+    YArgs* ya = nullptr;
+    InvArgs* ia[2] = {nullptr, nullptr};
+    SuppArgs* sa[2] = {nullptr, nullptr};
+    std::vector<CycArgs> cyc[2];   // channel members: k_cyc is launched per member
+    std::vector<EnsLaunch> launches;
+    std::vector<void*> allocs;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // (period, supp)
+};
+
+}  // namespace
+
+struct oqg_ens {
+    std::vector<oqg_ctx*> m;
+    hipStream_t st = nullptr;
+    std::string err;
+    int schedule = 0;          // OQG_ENS_* bits
+    std::vector<double> tdt;   // per member, baked into the plans' argument tables
+    std::map<std::vector<int>, EnsPlan*> plans;   // key: members, then their parities at odd steps, then schedule
+};
+
+static int ens_fail(oqg_ens* e, int code, const std::string& msg) {
+    e->err = msg;
+    return code;
+}
+
+#define ENSCHK(e, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return ens_fail(e, OQG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static void ens_free_plan(EnsPlan* p) {
+    for (auto& kv : p->graphs) hipGraphExecDestroy(kv.second);
+    for (void* a : p->allocs) hipFree(a);
+    delete p;
+}
+
+static void ens_clear_plans(oqg_ens* e) {
+    if (e->st) hipStreamSynchronize(e->st);
+    for (auto& kv : e->plans) ens_free_plan(kv.second);
+    e->plans.clear();
+}
+
+template <class T>
+static int ens_upload(oqg_ens* e, EnsPlan* p, T** dst, const std::vector<T>& v) {
+    void* d = nullptr;
+    ENSCHK(e, hipMalloc(&d, std::max<size_t>(1, v.size()) * sizeof(T)));
+    p->allocs.push_back(d);
+    if (!v.empty()) ENSCHK(e, hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, e->st));
+    *dst = static_cast<T*>(d);
+    return 0;
+}
+
+// Plan for the members `act` whose time-level parity before an odd-numbered
+// step (nt - 1 even) is base[i].
+static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::vector<int>& base, EnsPlan** out) {
+    std::unique_ptr<EnsPlan, void (*)(EnsPlan*)> P(new EnsPlan, ens_free_plan);
+    const int n = (int)act.size();
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    if (e->schedule & OQG_ENS_LARGEST_FIRST) {
+        auto work = [&](int i) {
+            const oqg_ctx* c = e->m[act[i]];
+            return (long long)c->g.nxp * c->g.nyp * c->g.nl;
+        };
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    }
+    for (int i = 0; i < n; i++) P->members.push_back(act[order[i]]);
+    std::vector<int> pbase(n);
+    for (int i = 0; i < n; i++) pbase[i] = base[order[i]];
+
+    // argument tables for both parities
+    for (int par = 0; par < 2; par++) {
+        std::vector<VortArgs> va;
+        std::vector<FwdArgs> fa;
+        std::vector<YArgs> ya;
+        std::vector<InvArgs> ia;
+        std::vector<SuppArgs> sa;
+        for (int i = 0; i < n; i++) {
+            oqg_ctx* c = e->m[P->members[i]];
+            const int saved = c->cur;
+            c->cur = pbase[i] ^ par;
+            StepArgs a;
+            make_step_args(c, e->tdt[P->members[i]], a);
+            va.push_back(a.va);
+            fa.push_back(a.fa);
+            ya.push_back(a.ya);
+            ia.push_back(a.ia);
+            if (c->g.cyclic) P->cyc[par].push_back(a.ca);
+            SuppArgs s{};   // suppression follows the step: the new level is 1 - cur
+            const int cn = 1 - c->cur;
+            s.g = c->g;
+            s.q = c->qbuf[cn];
+            s.qm = c->qbuf[1 - cn];
+            s.p = c->pbuf[cn];
+            s.pm = c->pbuf[1 - cn];
+            s.c = c->cbuf[cn];
+            sa.push_back(s);
+            c->cur = saved;
+        }
+        if (int r = ens_upload(e, P.get(), &P->va[par], va)) return r;
+        if (int r = ens_upload(e, P.get(), &P->fa[par], fa)) return r;
+        if (int r = ens_upload(e, P.get(), &P->ia[par], ia)) return r;
+        if (int r = ens_upload(e, P.get(), &P->sa[par], sa)) return r;
+        if (par == 0)
+            if (int r = ens_upload(e, P.get(), &P->ya, ya)) return r;
+    }
+
+    // block maps: one launch for the stencil, one per (layer count, butterflies)
+    // class for each x transform, one per rows-per-lane class (or one for all)
+    // for the y solve
+    auto add_launch = [&](EnsLaunch L, const std::vector<int2>& map) -> int {
+        L.nblocks = (int)map.size();
+        if (!L.nblocks) return 0;
+        if (int r = ens_upload(e, P.get(), &L.map, map)) return r;
+        P->launches.push_back(L);
+        return 0;
+    };
+    {
+        std::vector<int2> map;
+        for (int i = 0; i < n; i++) {
+            const oqg_ctx* c = e->m[P->members[i]];
+            const int nb = c->tiles_x * c->tiles_y * c->g.nl;
+            for (int b = 0; b < nb; b++) map.push_back(make_int2(i, b));
+        }
+        EnsLaunch L;
+        L.kind = ENS_VORT;
+        if (int r = add_launch(L, map)) return r;
+    }
+    std::map<std::pair<int, int>, std::pair<std::vector<int2>, size_t>> fwd, inv;
+    std::map<int, std::pair<std::vector<int2>, size_t>> ys;
+    int rmax = 1;
+    for (int i = 0; i < n; i++) rmax = std::max(rmax, e->m[P->members[i]]->s.R);
+    for (int i = 0; i < n; i++) {
+        const oqg_ctx* c = e->m[P->members[i]];
+        const std::pair<int, int> key(c->g.nl, fft_mb(c->s));
+        auto& f = fwd[key];
+        const int np = inv_block_count(c->s);
+        for (int b = 0; b < np * c->g.nl; b++) f.first.push_back(make_int2(i, b));
+        f.second = std::max(f.second, fwd_lds_bytes(c->s));
+        auto& v = inv[key];
+        for (int b = 0; b < np; b++) v.first.push_back(make_int2(i, b));
+        v.second = std::max(v.second, inv_lds_bytes(c->s, c->g, c->g.nl));
+        auto& y = ys[(e->schedule & OQG_ENS_ONE_YSOLVE) ? rmax : c->s.R];
+        for (int b = 0; b < ysolve_blocks(c->s, c->g.nl); b++) y.first.push_back(make_int2(i, b));
+        y.second = std::max(y.second, ysolve_lds_bytes(c->s));
+    }
+    for (auto& kv : fwd) {
+        EnsLaunch L;
+        L.kind = ENS_FWD;
+        L.nl = kv.first.first;
+        L.mb = kv.first.second;
+        L.lds = kv.second.second;
+        if (int r = add_launch(L, kv.second.first)) return r;
+    }
+    for (auto& kv : ys) {
+        EnsLaunch L;
+        L.kind = ENS_YS;
+        L.R = kv.first;
+        L.lds = std::max(kv.second.second, sizeof(double) * 8 * (size_t)(64 * kv.first + 1));
+        if (int r = add_launch(L, kv.second.first)) return r;
+    }
+    for (auto& kv : inv) {
+        EnsLaunch L;
+        L.kind = ENS_INV;
+        L.nl = kv.first.first;
+        L.mb = kv.first.second;
+        L.lds = kv.second.second;
+        if (int r = add_launch(L, kv.second.first)) return r;
+    }
+    ENSCHK(e, hipStreamSynchronize(e->st));   // the host tables above go out of scope
+    *out = P.release();
+    return 0;
+}
+
+// One step of every member of plan P; par = (nt - 1) & 1.
+static int ens_enqueue_step(oqg_ens* e, EnsPlan* P, int par, bool supp) {
+    for (const EnsLaunch& L : P->launches) {
+        switch (L.kind) {
+            case ENS_VORT:
+                launch_vort_ens(P->va[par], L.map, L.nblocks, e->st);
+                for (const CycArgs& ca : P->cyc[par]) launch_cyc(ca, e->st);
+                break;
+            case ENS_FWD: launch_fwd_ens(P->fa[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;
+            case ENS_YS: launch_ysolve_ens(P->ya, L.map, L.nblocks, L.R, L.lds, e->st); break;
+            default: launch_inv_ens(P->ia[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;
+        }
+    }
+    if (supp) launch_suppress_ens(P->sa[par], (int)P->members.size(), e->st);
+    ENSCHK(e, hipGetLastError());
+    return 0;
+}
+
+static int ens_get_plan(oqg_ens* e, const std::vector<int>& act, const std::vector<int>& base, EnsPlan** out) {
+    std::vector<int> key = act;
+    key.insert(key.end(), base.begin(), base.end());
+    key.push_back(e->schedule);
+    auto it = e->plans.find(key);
+    if (it != e->plans.end()) {
+        *out = it->second;
+        return 0;
+    }
+    EnsPlan* P = nullptr;
+    if (int r = ens_build_plan(e, act, base, &P)) return r;
+    e->plans[key] = P;
+    *out = P;
+    return 0;
+}
+
+static int ens_run(oqg_ens* e, const double* tdt, const int* nsteps, int step0, int supp) {
+    const int n = (int)e->m.size();
+    bool same = (int)e->tdt.size() == n;
+    for (int i = 0; same && i < n; i++) same = e->tdt[i] == tdt[i];
+    if (!same) {
+        ens_clear_plans(e);
+        e->tdt.assign(tdt, tdt + n);
+    }
+    int maxs = 0;
+    std::vector<int> cur0(n);
+    for (int i = 0; i < n; i++) {
+        if (nsteps[i] < 0) return ens_fail(e, OQG_ERR_ARG, "negative step count");
+        maxs = std::max(maxs, nsteps[i]);
+        ENSCHK(e, hipStreamSynchronize(e->m[i]->st));
+        cur0[i] = e->m[i]->cur;
+    }
+    const int Pd = supp > 0 ? (supp % 2 == 0 ? supp : 2 * supp) : 2;
+    int nt = step0 + 1;
+    const int end = step0 + maxs;
+    while (nt <= end) {
+        std::vector<int> act, base;
+        int seg_end = end;
+        for (int i = 0; i < n; i++)
+            if (nt <= step0 + nsteps[i]) {
+                act.push_back(i);
+                base.push_back(cur0[i] ^ (step0 & 1));
+                seg_end = std::min(seg_end, step0 + nsteps[i]);
+            }
+        EnsPlan* P = nullptr;
+        if (int r = ens_get_plan(e, act, base, &P)) return r;
+        while (nt <= seg_end) {
+            if ((nt - 1) % Pd == 0 && seg_end - nt + 1 >= Pd) {
+                const std::pair<int, int> gk(Pd, supp);
+                auto it = P->graphs.find(gk);
+                if (it == P->graphs.end()) {
+                    hipGraph_t gr;
+                    ENSCHK(e, hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+                    int rc = 0;
+                    for (int t = 0; t < Pd && !rc; t++) rc = ens_enqueue_step(e, P, t & 1, supp > 0 && t % supp == 0);
+                    hipError_t ce = hipStreamEndCapture(e->st, &gr);
+                    if (rc) return rc;
+                    if (ce != hipSuccess) return ens_fail(e, OQG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+                    hipGraphExec_t ex;
+                    hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                    hipGraphDestroy(gr);
+                    if (ie != hipSuccess)
+                        return ens_fail(e, OQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+                    it = P->graphs.emplace(gk, ex).first;
+                }
+                ENSCHK(e, hipGraphLaunch(it->second, e->st));
+                nt += Pd;
+            } else {
+                if (int r = ens_enqueue_step(e, P, (nt - 1) & 1, supp > 0 && (nt - 1) % supp == 0)) return r;
+                nt++;
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) e->m[i]->cur = cur0[i] ^ (nsteps[i] & 1);
+    return 0;
+}
+
+extern "C" {
+
+int oqg_ens_create(oqg_ctx* const* members, int n, oqg_ens** out) {
+    if (!members || n < 1 || !out) return fail(nullptr, OQG_ERR_ARG, "oqg_ens_create: members and n >= 1 required");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!members[i]) return fail(nullptr, OQG_ERR_ARG, "oqg_ens_create: null member");
+        for (int j = 0; j < i; j++)
+            if (members[j] == members[i]) return fail(nullptr, OQG_ERR_ARG, "oqg_ens_create: a basin appears twice");
+    }
+    std::unique_ptr<oqg_ens> e(new oqg_ens);
+    e->m.assign(members, members + n);
+    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess)
+        return fail(nullptr, OQG_ERR_HIP, "hipStreamCreate failed");
+    *out = e.release();
+    return 0;
+}
+
+void oqg_ens_destroy(oqg_ens* e) {
+    if (!e) return;
+    ens_clear_plans(e);
+    if (e->st) hipStreamDestroy(e->st);
+    delete e;
+}
+
+const char* oqg_ens_last_error(const oqg_ens* e) { return e ? e->err.c_str() : g_create_err.c_str(); }
+
+int oqg_ens_run(oqg_ens* e, const double* tdt, const int* nsteps, int step0, int supp_every) {
+    if (!e || !tdt || !nsteps || step0 < 0) return OQG_ERR_ARG;
+    return ens_run(e, tdt, nsteps, step0, supp_every);
+}
+
+int oqg_ens_sync(oqg_ens* e) {
+    if (!e) return OQG_ERR_ARG;
+    ENSCHK(e, hipStreamSynchronize(e->st));
+    return 0;
+}
+
+int oqg_ens_set_schedule(oqg_ens* e, int schedule) {
+    if (!e || schedule < 0 || schedule > (OQG_ENS_ONE_YSOLVE | OQG_ENS_LARGEST_FIRST)) return OQG_ERR_ARG;
+    if (schedule != e->schedule) {
+        ens_clear_plans(e);
+        e->schedule = schedule;
+    }
+    return 0;
+}
+
+int oqg_ens_get_schedule(const oqg_ens* e) { return e ? e->schedule : -OQG_ERR_ARG; }
+
+int oqg_ens_tune(oqg_ens* e, const double* tdt, int trial_steps, int supp_every, double* ms) {
+    if (!e || !tdt || trial_steps < 1) return -OQG_ERR_ARG;
+    const int n = (int)e->m.size();
+    // keep the members' state: both time levels of q, p and the constraint blocks
+    std::vector<void*> keep;
+    auto release = [&]() {
+        hipStreamSynchronize(e->st);
+        for (void* p : keep) hipFree(p);
+    };
+    std::vector<int> cur0(n), steps(n, trial_steps);
+    for (int i = 0; i < n; i++) {
+        oqg_ctx* c = e->m[i];
+        if (hipStreamSynchronize(c->st) != hipSuccess) return (release(), -ens_fail(e, OQG_ERR_HIP, "member stream"));
+        cur0[i] = c->cur;
+        for (int t = 0; t < 2; t++) {
+            void* a = nullptr;
+            void* b = nullptr;
+            void* cc = nullptr;
+            if (hipMalloc(&a, c->fbytes) != hipSuccess || (keep.push_back(a), hipMalloc(&b, c->fbytes)) != hipSuccess ||
+                (keep.push_back(b), hipMalloc(&cc, C_LEN * sizeof(double))) != hipSuccess)
+                return (release(), -ens_fail(e, OQG_ERR_HIP, "hipMalloc failed while tuning"));
+            keep.push_back(cc);
+            hipMemcpyAsync(a, c->qbuf[t], c->fbytes, hipMemcpyDeviceToDevice, e->st);
+            hipMemcpyAsync(b, c->pbuf[t], c->fbytes, hipMemcpyDeviceToDevice, e->st);
+            hipMemcpyAsync(cc, c->cbuf[t], C_LEN * sizeof(double), hipMemcpyDeviceToDevice, e->st);
+        }
+    }
+    auto restore = [&]() {
+        for (int i = 0; i < n; i++) {
+            oqg_ctx* c = e->m[i];
+            for (int t = 0; t < 2; t++) {
+                hipMemcpyAsync(c->qbuf[t], keep[(i * 2 + t) * 3 + 0], c->fbytes, hipMemcpyDeviceToDevice, e->st);
+                hipMemcpyAsync(c->pbuf[t], keep[(i * 2 + t) * 3 + 1], c->fbytes, hipMemcpyDeviceToDevice, e->st);
+                hipMemcpyAsync(c->cbuf[t], keep[(i * 2 + t) * 3 + 2], C_LEN * sizeof(double), hipMemcpyDeviceToDevice, e->st);
+            }
+            c->cur = cur0[i];
+        }
+    };
+    hipEvent_t ev[2];
+    hipEventCreate(&ev[0]);
+    hipEventCreate(&ev[1]);
+    int best = 0;
+    double best_ms = 1e300;
+    const int nsched = OQG_ENS_ONE_YSOLVE | OQG_ENS_LARGEST_FIRST;
+    int rc = 0;
+    for (int sched = 0; sched <= nsched && !rc; sched++) {
+        oqg_ens_set_schedule(e, sched);
+        rc = ens_run(e, tdt, steps.data(), 0, supp_every);   // builds the plan and its graphs
+        restore();
+        if (rc) break;
+        hipEventRecord(ev[0], e->st);
+        rc = ens_run(e, tdt, steps.data(), 0, supp_every);
+        hipEventRecord(ev[1], e->st);
+        hipEventSynchronize(ev[1]);
+        restore();
+        float t = 0.f;
+        hipEventElapsedTime(&t, ev[0], ev[1]);
+        if (ms) ms[sched] = t;
+        if (t < best_ms) {
+            best_ms = t;
+            best = sched;
+        }
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    release();
+    if (rc) return -rc;
+    oqg_ens_set_schedule(e, best);
+    return best;
 }
 
 }  // extern "C"

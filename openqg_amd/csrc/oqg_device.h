@@ -293,14 +293,5 @@ __device__ __forceinline__ void fft_lds(double2* buf, const Spec& s, const doubl
     }
 }
 
-// Butterflies per thread needed by the plan of s (1, 2 or 4).
-inline int fft_mb(const Spec& s) {
-    int mx = 1;
-    for (int st = 0; st < s.nstages; st++) {
-        const int b = (s.L / s.radix[st] + FFT_GROUP - 1) / FFT_GROUP;
-        if (b > mx) mx = b;
-    }
-    return mx <= 1 ? 1 : (mx <= 2 ? 2 : 4);
-}
 
 }  // namespace oqg

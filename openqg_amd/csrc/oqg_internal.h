@@ -200,7 +200,11 @@ struct QcompArgs {
     double* q;
 };
 
-// Kernel launchers (oqg_kernels.hip).
+// Most dynamic LDS a launch of the LDS-staged kernels may ask for (the rest of
+// the 160 KB of a CU is left to their static LDS).
+constexpr size_t MAX_DYN_LDS = 150 * 1024;
+
+// Kernel launchers (oqg_stencil.hip, oqg_transform.hip).
 void launch_vort(const VortArgs& a, hipStream_t st);
 void launch_cyc(const CycArgs& a, hipStream_t st);
 void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st);
@@ -209,8 +213,22 @@ void launch_inv(const InvArgs& a, hipStream_t st);
 void launch_suppress(const SuppArgs& a, hipStream_t st);
 void launch_qcomp(const QcompArgs& a, hipStream_t st);
 int inv_block_count(const Spec& s);
+int ysolve_blocks(const Spec& s, int nmodes);
+int fft_mb(const Spec& s);   // butterflies per thread of the x transform plan (1, 2 or 4)
 size_t fwd_lds_bytes(const Spec& s);
+size_t ysolve_lds_bytes(const Spec& s);
 size_t inv_lds_bytes(const Spec& s, const Geo& g, int nmodes);
-void set_kernel_attrs(const Spec& s, const Geo& g);
+void set_kernel_attrs();
+
+// Ensemble launchers: one launch covers blocks of many basins; map[b] =
+// (member index into args, that member's block index).  The caller groups
+// members so that one launch shares the template parameters (layer count,
+// butterflies per thread, y-solver rows per lane); lds is the largest need of
+// the members in the launch.
+void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, hipStream_t st);
+void launch_fwd_ens(const FwdArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st);
+void launch_ysolve_ens(const YArgs* args, const int2* map, int nblocks, int R, size_t lds, hipStream_t st);
+void launch_inv_ens(const InvArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st);
+void launch_suppress_ens(const SuppArgs* args, int nmembers, hipStream_t st);
 
 }  // namespace oqg

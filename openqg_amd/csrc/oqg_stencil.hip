@@ -33,16 +33,16 @@ __device__ __forceinline__ double jac_int(const double* q, const double* p, int 
     return g.jfac * t;
 }
 
+// Tile (bx, by) of layer k.
 template <bool EDGE>
-__device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, double* s_d2, double* s_d4, double* s_q,
-                                          double* s_p, double* s_red) {
+__device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int k, double* s_pm, double* s_d2,
+                                          double* s_d4, double* s_q, double* s_p, double* s_red) {
     constexpr int TX = VORT_TX, TY = VORT_TY;
     constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
     const Geo& g = a.g;
     const int tid = threadIdx.x;
     const int tx = tid & 63, ty = tid >> 6;
-    const int k = blockIdx.z;
-    const int i0 = a.org_x + blockIdx.x * TX, j0 = 1 + blockIdx.y * TY;
+    const int i0 = a.org_x + bx * TX, j0 = 1 + by * TY;
     const int lim_x = vort_lim_x(g), lim_y = g.nyp - 1;
     const long long lo = (long long)k * g.fs;
     const double* pm = a.pm + lo;
@@ -105,8 +105,8 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
     double ring_w = 0.0;
     long long ring_o = 0;
     if (EDGE && k == 0) {
-        const bool frow = blockIdx.y == 0, lrow = blockIdx.y == a.tiles_y - 1;
-        const bool fcol = blockIdx.x == 0, lcol = blockIdx.x == a.tiles_x - 1;
+        const bool frow = by == 0, lrow = by == a.tiles_y - 1;
+        const bool fcol = bx == 0, lcol = bx == a.tiles_x - 1;
         if (((ty == 0 && frow) || (ty == 1 && lrow)) && gi < lim_x) {
             const bool ex = (gi == 0 || gi == g.nxp - 1);
             ring_w = ex ? 0.25 : 0.5;
@@ -222,13 +222,13 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, double* s_pm, doubl
     if (k == 0) {
         double v[1] = {epart};
         block_sum<1>(v, s_red);
-        if (threadIdx.x == 0) a.entpart[blockIdx.y * a.tiles_x + blockIdx.x] = v[0];
+        if (threadIdx.x == 0) a.entpart[by * a.tiles_x + bx] = v[0];
     }
 }
 
 // Tiles whose halo of 3 stays inside the domain without touching a boundary
 // point or crossing the channel's periodic seam take the branch-free body.
-__global__ __launch_bounds__(256) void k_vort(VortArgs a) {
+__device__ __forceinline__ void vort_block(const VortArgs& a, int bx, int by, int k) {
     constexpr int TX = VORT_TX, TY = VORT_TY;
     __shared__ double s_pm[(TY + 6) * (TX + 6)];
     __shared__ double s_d2[(TY + 4) * (TX + 4)];
@@ -236,15 +236,32 @@ __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
     __shared__ double s_q[(TY + 2) * (TX + 2)];
     __shared__ double s_p[(TY + 2) * (TX + 2)];
     __shared__ double s_red[16];
-    const int i0 = a.org_x + blockIdx.x * TX, j0 = 1 + blockIdx.y * TY;
+    const int i0 = a.org_x + bx * TX, j0 = 1 + by * TY;
     const bool inner = i0 >= 3 && i0 + TX + 3 <= a.g.nxp && j0 >= 3 && j0 + TY + 3 <= a.g.nyp;
-    if (inner) vort_tile<false>(a, s_pm, s_d2, s_d4, s_q, s_p, s_red);
-    else vort_tile<true>(a, s_pm, s_d2, s_d4, s_q, s_p, s_red);
+    if (inner) vort_tile<false>(a, bx, by, k, s_pm, s_d2, s_d4, s_q, s_p, s_red);
+    else vort_tile<true>(a, bx, by, k, s_pm, s_d2, s_d4, s_q, s_p, s_red);
+}
+
+__global__ __launch_bounds__(256) void k_vort(VortArgs a) { vort_block(a, blockIdx.x, blockIdx.y, blockIdx.z); }
+
+// Ensemble form: block b works on tile map[b].y of member map[b].x, whose
+// arguments are args[map[b].x] (tiles numbered x fastest, then y, then layer).
+__global__ __launch_bounds__(256) void k_vort_ens(const VortArgs* __restrict__ args, const int2* __restrict__ map) {
+    const int2 e = map[blockIdx.x];
+    const VortArgs& a = args[e.x];
+    const int per_layer = a.tiles_x * a.tiles_y;
+    const int k = e.y / per_layer, r = e.y - k * per_layer;
+    const int by = r / a.tiles_x, bx = r - by * a.tiles_x;
+    vort_block(a, bx, by, k);
 }
 
 void launch_vort(const VortArgs& a, hipStream_t st) {
     dim3 grid(a.tiles_x, a.tiles_y, a.g.nl);
     hipLaunchKernelGGL(k_vort, grid, dim3(256), 0, st, a);
+}
+
+void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_vort_ens, dim3(nblocks), dim3(256), 0, st, args, map);
 }
 
 // ------------------------------------------------------------------ k_cyc
@@ -364,14 +381,15 @@ void launch_cyc(const CycArgs& a, hipStream_t st) {
 
 // ------------------------------------------------------------------ k_suppress
 
-__global__ void k_suppress(SuppArgs a) {
+// Block b of nb blocks sweeping one basin.
+__device__ __forceinline__ void suppress_block(const SuppArgs& a, int b, int nb) {
     const Geo& g = a.g;
     const long long n = (long long)g.nl * g.nyp * g.pitch;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    for (long long t = b * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)nb * blockDim.x) {
         a.q[t] = 0.5 * (a.q[t] + a.qm[t]);
         a.p[t] = 0.5 * (a.p[t] + a.pm[t]);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (b == 0 && threadIdx.x == 0) {
         for (int l = 0; l < g.nl - 1; l++) a.c[C_DPI + l] = 0.5 * (a.c[C_DPI + l] + a.c[C_DPIP + l]);
         if (g.cyclic)
             for (int k = 0; k < g.nl; k++) {
@@ -381,8 +399,19 @@ __global__ void k_suppress(SuppArgs a) {
     }
 }
 
+__global__ void k_suppress(SuppArgs a) { suppress_block(a, blockIdx.x, gridDim.x); }
+
+// Ensemble form: blockIdx.y is the member.
+__global__ void k_suppress_ens(const SuppArgs* __restrict__ args) {
+    suppress_block(args[blockIdx.y], blockIdx.x, gridDim.x);
+}
+
 void launch_suppress(const SuppArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_suppress, dim3(1024), dim3(256), 0, st, a);
+}
+
+void launch_suppress_ens(const SuppArgs* args, int nmembers, hipStream_t st) {
+    hipLaunchKernelGGL(k_suppress_ens, dim3(256, nmembers), dim3(256), 0, st, args);
 }
 
 // ------------------------------------------------------------------ k_qcomp

@@ -17,6 +17,8 @@
 //             (one 256-thread group per mode), mode->layer mix (modes.F90:268-284)
 //             and the boundary PV (vorsubs.F90:58-114); writes p_new over pm.
 
+#include <mutex>
+
 #include "oqg_device.h"
 
 namespace oqg {
@@ -40,14 +42,14 @@ __device__ __forceinline__ double rhs_mode(const FwdArgs& a, int m, const double
     return g.fnot * s;
 }
 
+// Row pair jp of mode mb (or of the raw field).
 template <int NL, int MB>
-__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
+__device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     extern __shared__ double2 zbuf[];   // [L] transform, then the twiddle table
     const Geo& g = a.g;
     const Spec& s = a.s;
     const int L = s.L;
-    const int jp = blockIdx.x;
-    const int m = (a.raw_mode >= 0) ? a.raw_mode : (int)blockIdx.y;
+    const int m = (a.raw_mode >= 0) ? a.raw_mode : mb;
     const int r0 = 2 * jp, r1 = 2 * jp + 1;
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
@@ -139,6 +141,20 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
     }
 }
 
+template <int NL, int MB>
+__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) { fwd_block<NL, MB>(a, blockIdx.x, blockIdx.y); }
+
+// Ensemble form: block b works on block map[b].y (row pairs fastest, then
+// modes) of member map[b].x.
+template <int NL, int MB>
+__global__ __launch_bounds__(256) void k_fwd_ens(const FwdArgs* __restrict__ args, const int2* __restrict__ map) {
+    const int2 e = map[blockIdx.x];
+    const FwdArgs& a = args[e.x];
+    const int npairs = (a.s.nyi + 1) / 2;
+    const int mb = e.y / npairs;
+    fwd_block<NL, MB>(a, e.y - mb * npairs, mb);
+}
+
 // Instantiations over the layer count (2..8) and butterflies per thread (1, 2, 4).
 #define OQG_NL_MB_SWITCH(NLV, MBV, CALL)                                      \
     switch ((NLV) * 8 + (MBV)) {                                              \
@@ -152,6 +168,15 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
         default: break;                                                       \
     }
 
+int fft_mb(const Spec& s) {
+    int mx = 1;
+    for (int st = 0; st < s.nstages; st++) {
+        const int b = (s.L / s.radix[st] + FFT_GROUP - 1) / FFT_GROUP;
+        if (b > mx) mx = b;
+    }
+    return mx <= 1 ? 1 : (mx <= 2 ? 2 : 4);
+}
+
 size_t fwd_lds_bytes(const Spec& s) { return sizeof(double2) * (size_t)(s.L + twiddle_table_len(s.L)); }
 
 void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
@@ -160,6 +185,12 @@ void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
 #define FWD_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd<NLc, MBc>), dim3(npairs, nmodes), dim3(256), lds, st, a)
     OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), FWD_CALL)
 #undef FWD_CALL
+}
+
+void launch_fwd_ens(const FwdArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st) {
+#define FWDE_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd_ens<NLc, MBc>), dim3(nblocks), dim3(256), lds, st, args, map)
+    OQG_NL_MB_SWITCH(nl, mb, FWDE_CALL)
+#undef FWDE_CALL
 }
 
 // ------------------------------------------------------------------ k_ysolve
@@ -246,13 +277,14 @@ __device__ void wave_tridiag(double (&f)[R], int L, double aa, double b, int lan
         if (t < L) f[t] = (t == L - 1) ? S : v[t] + x[t] * Sm + y[t] * S;
 }
 
+// Columns [cb*YS_CT, cb*YS_CT + YS_CT) of mode mode0 + mb.
 template <int R>
-__global__ __launch_bounds__(64 * YS_CT) void k_ysolve(YArgs a) {
+__device__ __forceinline__ void ysolve_block(const YArgs& a, int cb, int mb) {
     extern __shared__ double tile[];   // [YS_CT][P]
     const Spec& s = a.s;
     const int P = s.nypad + 1;
-    const int m = a.mode0 + blockIdx.y;
-    const int c0 = blockIdx.x * YS_CT;
+    const int m = a.mode0 + mb;
+    const int c0 = cb * YS_CT;
     double* S = a.spec + (long long)m * s.nrow * s.kp;
     // nyi <= 64 R, so each of the 64*YS_CT threads loads at most R values: issue them together
     double tmp[R];
@@ -302,6 +334,21 @@ __global__ __launch_bounds__(64 * YS_CT) void k_ysolve(YArgs a) {
     }
 }
 
+template <int R>
+__global__ __launch_bounds__(64 * YS_CT) void k_ysolve(YArgs a) { ysolve_block<R>(a, blockIdx.x, blockIdx.y); }
+
+// Ensemble form: block b works on column block map[b].y (column blocks
+// fastest, then modes) of member map[b].x.
+template <int R>
+__global__ __launch_bounds__(64 * YS_CT) void k_ysolve_ens(const YArgs* __restrict__ args,
+                                                           const int2* __restrict__ map) {
+    const int2 e = map[blockIdx.x];
+    const YArgs& a = args[e.x];
+    const int ncb = (a.s.ncol + YS_CT - 1) / YS_CT;
+    const int mb = e.y / ncb;
+    ysolve_block<R>(a, e.y - mb * ncb, mb);
+}
+
 // Reference-order Thomas (Numerical Recipes form, inhomog.F90:188-199), one
 // thread per column walking down the rows; gam is kept in a scratch array of
 // the spectra's shape.
@@ -341,9 +388,23 @@ void launch_ysolve(const YArgs& a, int variant, hipStream_t st) {
         return;
     }
     const dim3 grid((a.s.ncol + YS_CT - 1) / YS_CT, a.nmodes);
-    const size_t lds = sizeof(double) * YS_CT * (size_t)(a.s.nypad + 1);
+    const size_t lds = ysolve_lds_bytes(a.s);
     switch (a.s.R) {
 #define YCASE(RR) case RR: hipLaunchKernelGGL(k_ysolve<RR>, grid, dim3(64 * YS_CT), lds, st, a); break;
+        YCASE(1) YCASE(2) YCASE(3) YCASE(4) YCASE(5) YCASE(6) YCASE(7) YCASE(8)
+        YCASE(9) YCASE(10) YCASE(11) YCASE(12) YCASE(13) YCASE(14) YCASE(15) YCASE(16)
+#undef YCASE
+        default: break;
+    }
+}
+
+size_t ysolve_lds_bytes(const Spec& s) { return sizeof(double) * YS_CT * (size_t)(s.nypad + 1); }
+
+int ysolve_blocks(const Spec& s, int nmodes) { return ((s.ncol + YS_CT - 1) / YS_CT) * nmodes; }
+
+void launch_ysolve_ens(const YArgs* args, const int2* map, int nblocks, int R, size_t lds, hipStream_t st) {
+    switch (R) {
+#define YCASE(RR) case RR: hipLaunchKernelGGL(k_ysolve_ens<RR>, dim3(nblocks), dim3(64 * YS_CT), lds, st, args, map); break;
         YCASE(1) YCASE(2) YCASE(3) YCASE(4) YCASE(5) YCASE(6) YCASE(7) YCASE(8)
         YCASE(9) YCASE(10) YCASE(11) YCASE(12) YCASE(13) YCASE(14) YCASE(15) YCASE(16)
 #undef YCASE
@@ -602,7 +663,7 @@ __device__ __forceinline__ void load_spec_pair(const InvArgs& a, int m, bool ok,
 }
 
 template <int NL, int MB>
-__global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs a) {
+__device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     extern __shared__ double smem[];
     const Geo& g = a.g;
     const Spec& s = a.s;
@@ -638,11 +699,10 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
     const double spk = a.spec[((long long)m_spk * s.nrow + r_spk) * s.kp];   // used only when want_spk
     const double* s_ctm2l = s_dc.mx.ctm2l;
     const double* s_amat = s_dc.mx.amat;
-    const int jp = blockIdx.x;
     const int r0 = 2 * jp, r1 = 2 * jp + 1;
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
-    const bool first = (jp == 0), last = (jp == (int)gridDim.x - 1);
+    const bool first = (jp == 0), last = (jp == (s.nyi + 1) / 2 - 1);
     const int nsel = has1 ? 2 : 1;
 
     // ---- issue the block's global loads together: the integrals behind the
@@ -762,7 +822,7 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
     if (!raw) {
         block_sum<NL + 1>(v, red);
         if (threadIdx.x == 0)
-            step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, blockIdx.x == 0);
+            step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, jp == 0);
         __syncthreads();
     }
 
@@ -826,6 +886,19 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
     }
 }
 
+template <int NL, int MB>
+__global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs a) {
+    inv_block<NL, MB>(a, blockIdx.x);
+}
+
+// Ensemble form: block b works on row pair map[b].y of member map[b].x.
+template <int NL, int MB>
+__global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv_ens(const InvArgs* __restrict__ args,
+                                                                              const int2* __restrict__ map) {
+    const int2 e = map[blockIdx.x];
+    inv_block<NL, MB>(args[e.x], e.y);
+}
+
 int inv_block_count(const Spec& s) { return (s.nyi + 1) / 2; }
 
 static int inv_groups(const InvArgs& a) { return inv_group_count((a.raw_mode >= 0) ? 1 : a.g.nl); }
@@ -845,16 +918,38 @@ void launch_inv(const InvArgs& a, hipStream_t st) {
 #undef INV_CALL
 }
 
-// Allow the transforms of this basin more than the default 64 KB of dynamic LDS.
-void set_kernel_attrs(const Spec& s, const Geo& g) {
-    const int fl = (int)fwd_lds_bytes(s), il = (int)inv_lds_bytes(s, g, g.nl);
-#define ATTR_CALL(NLc, MBc)                                                                                       \
-    do {                                                                                                          \
-        (void)hipFuncSetAttribute((const void*)k_fwd<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, fl); \
-        (void)hipFuncSetAttribute((const void*)k_inv<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, il); \
+void launch_inv_ens(const InvArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st) {
+    const dim3 block(FFT_GROUP * inv_group_count(nl));
+#define INVE_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv_ens<NLc, MBc>), dim3(nblocks), block, lds, st, args, map)
+    OQG_NL_MB_SWITCH(nl, mb, INVE_CALL)
+#undef INVE_CALL
+}
+
+// Every instantiation of the LDS-staged kernels may use up to MAX_DYN_LDS of
+// dynamic LDS (each launch asks for what its basins need; the setup of a basin
+// checks that this fits).  Set once per process, for all basins alike.
+void set_kernel_attrs() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const int lim = (int)MAX_DYN_LDS;
+#define ATTR_NLMB(NLc, MBc)                                                                                           \
+    do {                                                                                                              \
+        (void)hipFuncSetAttribute((const void*)k_fwd<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);     \
+        (void)hipFuncSetAttribute((const void*)k_fwd_ens<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+        (void)hipFuncSetAttribute((const void*)k_inv<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);     \
+        (void)hipFuncSetAttribute((const void*)k_inv_ens<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
     } while (0)
-    OQG_NL_MB_SWITCH(g.nl, fft_mb(s), ATTR_CALL)
-#undef ATTR_CALL
+#define ATTR_NL(NLc) ATTR_NLMB(NLc, 1); ATTR_NLMB(NLc, 2); ATTR_NLMB(NLc, 4);
+        ATTR_NL(2) ATTR_NL(3) ATTR_NL(4) ATTR_NL(5) ATTR_NL(6) ATTR_NL(7) ATTR_NL(8)
+#undef ATTR_NL
+#undef ATTR_NLMB
+#define ATTR_R(RR)                                                                                                 \
+    (void)hipFuncSetAttribute((const void*)k_ysolve<RR>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);        \
+    (void)hipFuncSetAttribute((const void*)k_ysolve_ens<RR>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+        ATTR_R(1) ATTR_R(2) ATTR_R(3) ATTR_R(4) ATTR_R(5) ATTR_R(6) ATTR_R(7) ATTR_R(8)
+        ATTR_R(9) ATTR_R(10) ATTR_R(11) ATTR_R(12) ATTR_R(13) ATTR_R(14) ATTR_R(15) ATTR_R(16)
+#undef ATTR_R
+    });
 }
 
 }  // namespace oqg

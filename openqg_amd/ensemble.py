@@ -1,0 +1,83 @@
+"""Batched ensembles of basins over liboqg.so (oqg_ens_* in include/oqg.h).
+
+The reference runs an ensemble as separate model runs, one basin per step_qg
+call (src/driver.F90:107).  Here independent basins of any sizes and layer
+counts advance together: every phase of the step is one launch over the
+blocks of all members (per class of kernel template parameters), replayed from
+HIP graphs.  A member's trajectory is the one it would follow stepped alone.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from ._lib import ENS_LARGEST_FIRST, ENS_ONE_YSOLVE, OqgError, load  # noqa: F401
+from .qg import QG
+
+
+class Ensemble:
+    """Members are `QG` basins (already holding their state and forcing); they
+    must outlive the ensemble and must not be stepped on their own while it runs."""
+
+    def __init__(self, members: Sequence[QG]):
+        self._lib = load()
+        self.members = list(members)
+        n = len(self.members)
+        arr = (ctypes.c_void_p * n)(*[m._h for m in self.members])
+        h = ctypes.c_void_p()
+        rc = self._lib.oqg_ens_create(arr, n, ctypes.byref(h))
+        if rc != 0:
+            raise OqgError(f"oqg_ens_create failed (code {rc}): {self._lib.oqg_ens_last_error(None).decode()}")
+        self._h = h
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            raise OqgError(f"{what} failed (code {rc}): {self._lib.oqg_ens_last_error(self._h).decode()}")
+
+    @staticmethod
+    def _arrays(tdt, nsteps, n):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(tdt, dtype=np.float64), (n,)))
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(nsteps, dtype=np.int32), (n,)))
+        return t, s
+
+    def run(self, tdt, nsteps, step0: int = 0, supp_every: int = 25):
+        """Advance member i by nsteps[i] leapfrog steps of 2*dt = tdt[i] (scalars broadcast)."""
+        t, s = self._arrays(tdt, nsteps, len(self.members))
+        self._check(self._lib.oqg_ens_run(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                          s.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(step0),
+                                          int(supp_every)), "oqg_ens_run")
+
+    def sync(self):
+        self._check(self._lib.oqg_ens_sync(self._h), "oqg_ens_sync")
+
+    @property
+    def schedule(self) -> int:
+        return int(self._lib.oqg_ens_get_schedule(self._h))
+
+    @schedule.setter
+    def schedule(self, value: int):
+        self._check(self._lib.oqg_ens_set_schedule(self._h, int(value)), "oqg_ens_set_schedule")
+
+    def tune(self, tdt, trial_steps: int = 8, supp_every: int = 25):
+        """Time every launch schedule on the members (their state is restored),
+        keep the fastest; returns (schedule, [ms per schedule])."""
+        t, _ = self._arrays(tdt, 0, len(self.members))
+        ms = (ctypes.c_double * 4)()
+        best = self._lib.oqg_ens_tune(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(trial_steps),
+                                      int(supp_every), ms)
+        if best < 0:
+            self._check(-best, "oqg_ens_tune")
+        return int(best), list(ms)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.oqg_ens_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

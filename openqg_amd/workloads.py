@@ -182,6 +182,37 @@ def atmos_channel() -> Workload:
                     supp_every=100)
 
 
+ENS_SIZES = (65, 129, 257, 385, 513)
+BASIN_L = 3840.0e3   # square basin side (m)
+
+
+def stable_dt(dx: float, ah4: float, cap: float = 1200.0) -> float:
+    """Largest divisor of one day (s) not above min(cap, 0.5 / (64 ah4 / dx^4)),
+    the biharmonic stability bound of the leapfrog step."""
+    lim = min(cap, 0.5 / (64.0 * ah4 / dx ** 4))
+    return float(max(d for d in range(1, 86401) if 86400 % d == 0 and d <= lim))
+
+
+def ensemble_member(i: int, base_seed: int = 3, days: float = 10.0) -> Workload:
+    """Member i of the uneven double-gyre ensemble (seed base_seed + i): grid
+    drawn from 65^2..513^2 over the same 3840 km basin, 2-4 layers, wind
+    amplitude tau0 ~ U(0.05, 0.15) N/m^2, biharmonic viscosity ~ logU(1e9, 1e11)
+    m^4/s, white PV noise of rms 1e-7 1/s, the stable time step for its grid."""
+    rng = np.random.default_rng(base_seed + i)
+    n = int(rng.choice(ENS_SIZES))
+    nl = int(rng.choice([2, 3, 4]))
+    tau0 = float(rng.uniform(0.05, 0.15))
+    ah4 = float(10.0 ** rng.uniform(9.0, 11.0))
+    dx = BASIN_L / (n - 1)
+    return ocean_box(f"ens{i:02d}_box{n}_{nl}l", n, dx, ah4, stable_dt(dx, ah4), days, nl=nl, init="white",
+                     q_rms=1.0e-7, seed=base_seed + i, tau0=tau0)
+
+
+def uneven_ensemble(n_members: int = 64, base_seed: int = 3, days: float = 10.0) -> list:
+    """The 64-member batched-ensemble workload."""
+    return [ensemble_member(i, base_seed, days) for i in range(n_members)]
+
+
 def small_box(n: int = 33, nl: int = 3, seed: int = 7) -> Workload:
     """A small box for fast parity tests (30 km spacing)."""
     return ocean_box(f"box{n}_{nl}l", n, 30.0e3, 2.0e10, 3600.0, 1.0, nl=nl, init="k3", q_rms=5.0e-6, seed=seed)
