@@ -172,6 +172,42 @@ int oqg_kernel_times(oqg_ctx* ctx, double tdt, int reps, double* ms, int max, ch
                      int names_len);
 
 /*
+ * Adjoint (reverse mode) of step_qg for box basins.  No counterpart in the
+ * reference, which has no derivative code; these serve gradient studies of
+ * the model (e.g. of a kinetic-energy diagnostic, qg_monitor.F90:331-333,
+ * with respect to the wind stress and the initial PV).
+ *
+ * The adjoint state mirrors the forward state: with the forward state
+ * S = (q, qm, p, pm) loaded, oqg_adj_set/get address the adjoints of q, qm,
+ * p, pm (NULL arguments: zero / skip).  oqg_adj_step takes the adjoints of
+ * the state after one step_qg (with its suppress_comp_mode when `suppressed`)
+ * to those of the state before it; the forward state before the step must be
+ * loaded (e.g. from a snapshot).  The adjoint of the resident Ekman pumping
+ * field accumulates across steps.  Asynchronous on the context's stream.
+ */
+int oqg_adj_init(oqg_ctx* ctx);
+int oqg_adj_set(oqg_ctx* ctx, const double* lq, const double* lqm, const double* lp, const double* lpm,
+                int kind);
+int oqg_adj_get(oqg_ctx* ctx, double* lq, double* lqm, double* lp, double* lpm, int kind);
+int oqg_adj_step(oqg_ctx* ctx, double tdt, int suppressed);
+int oqg_adj_get_wekp(oqg_ctx* ctx, double* out, int kind);
+int oqg_adj_zero_wekp(oqg_ctx* ctx);
+
+/* The inversion of one step applied to the current q (interior): p from q
+ * (modes, Helmholtz solves, mass correction with the current dpi), q's
+ * boundary PV by ocqbdy, and qm = q, pm = p.  oqg_adj_invert is its adjoint:
+ * from the adjoints of (q, qm, p, pm) it leaves the adjoint of the interior q
+ * in the adjoint of q (ring and the other three zero). */
+int oqg_invert(oqg_ctx* ctx);
+int oqg_adj_invert(oqg_ctx* ctx);
+
+/* Device-resident snapshots of the whole dynamic state (both time levels of
+ * q and p, the constraint blocks and the parity), for checkpointing. */
+int oqg_snap_alloc(oqg_ctx* ctx, int nslots);
+int oqg_snap_save(oqg_ctx* ctx, int slot);
+int oqg_snap_load(oqg_ctx* ctx, int slot);
+
+/*
  * Batched ensembles.  No counterpart in the reference, whose driver steps one
  * basin per step_qg call (src/driver.F90:107,143); an ensemble of independent
  * basins there is a set of separate runs.  Here the members, ordinary contexts

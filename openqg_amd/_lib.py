@@ -67,6 +67,17 @@ SIGNATURES = [
     ("oqg_state_ptrs", _i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                             ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_size_t)]),
     ("oqg_kernel_times", _i, [_vp, _d, _i, _dp, _i, ctypes.c_char_p, _i]),
+    ("oqg_adj_init", _i, [_vp]),
+    ("oqg_adj_set", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    ("oqg_adj_get", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    ("oqg_adj_step", _i, [_vp, _d, _i]),
+    ("oqg_adj_get_wekp", _i, [_vp, _vp, _i]),
+    ("oqg_adj_zero_wekp", _i, [_vp]),
+    ("oqg_invert", _i, [_vp]),
+    ("oqg_adj_invert", _i, [_vp]),
+    ("oqg_snap_alloc", _i, [_vp, _i]),
+    ("oqg_snap_save", _i, [_vp, _i]),
+    ("oqg_snap_load", _i, [_vp, _i]),
     ("oqg_ens_create", _i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
     ("oqg_ens_destroy", None, [_vp]),
     ("oqg_ens_last_error", ctypes.c_char_p, [_vp]),

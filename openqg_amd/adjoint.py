@@ -1,0 +1,306 @@
+"""Gradients of the QG model through the hand-written adjoint of step_qg (box).
+
+The objective is the layer-1 kinetic energy of qg_monitor (ekei(1),
+src/diagnostics/qg_monitor.F90:297-302,331-333) after N steps:
+    J = 0.5 rho H1 int_P_dA(u^2 + v^2),  u = -dPdy_2bc(p1)/f0, v = dPdx_2bc(p1)/f0
+(numerics.F90:470-513 for the boundary differences, :165-204 for int_P_dA).
+Controls: the kinematic wind stress (taux, tauy) at p points, through
+ekman_from_tau + map_T_to_P, and the interior PV of the initial state, whose
+pressure is one inversion of it (dpi = 0), with qm = q and pm = p.
+
+The forward run keeps device snapshots of the state; the reverse sweep loads
+them and applies oqg_adj_step.  With `snapshots=None` every state is kept
+(HBM holds thousands of them); with a number, binomial checkpointing
+(Griewank's revolve schedule) recomputes the forward from the stored ones.
+"""
+from __future__ import annotations
+
+import ctypes
+from math import comb
+from typing import Optional
+
+import numpy as np
+
+from ._lib import OQG_HOST, check
+from .forcing import ekman_wekp, ekman_wekp_adjoint
+from .qg import QG
+
+
+# ------------------------------------------------------------------ objective
+
+def _trap_weights(nyp: int, nxp: int, dx: float, dy: float) -> np.ndarray:
+    w = np.ones((nyp, nxp))
+    w[0, :] *= 0.5
+    w[-1, :] *= 0.5
+    w[:, 0] *= 0.5
+    w[:, -1] *= 0.5
+    return w * dx * dy
+
+
+def _ddx(p, dx, alpha):
+    d = np.zeros_like(p)
+    d[:, 1:-1] = (p[:, 2:] - p[:, :-2]) / (2.0 * dx)
+    d[:, 0] = (p[:, 1] - p[:, 0]) / (dx * (0.5 * alpha + 1.0))
+    d[:, -1] = (p[:, -1] - p[:, -2]) / (dx * (0.5 * alpha + 1.0))
+    d[0, :] = 0.0
+    d[-1, :] = 0.0
+    return d
+
+
+def _ddy(p, dy, alpha):
+    d = np.zeros_like(p)
+    d[1:-1, :] = (p[2:, :] - p[:-2, :]) / (2.0 * dy)
+    d[0, :] = (p[1, :] - p[0, :]) / (dy * (0.5 * alpha + 1.0))
+    d[-1, :] = (p[-1, :] - p[-2, :]) / (dy * (0.5 * alpha + 1.0))
+    d[:, 0] = 0.0
+    d[:, -1] = 0.0
+    return d
+
+
+def _ddx_T(g, dx, alpha):
+    """Transpose of _ddx (box)."""
+    g = g.copy()
+    g[0, :] = 0.0
+    g[-1, :] = 0.0
+    out = np.zeros_like(g)
+    out[:, 2:] += g[:, 1:-1] / (2.0 * dx)
+    out[:, :-2] -= g[:, 1:-1] / (2.0 * dx)
+    c = dx * (0.5 * alpha + 1.0)
+    out[:, 1] += g[:, 0] / c
+    out[:, 0] -= g[:, 0] / c
+    out[:, -1] += g[:, -1] / c
+    out[:, -2] -= g[:, -1] / c
+    return out
+
+
+def _ddy_T(g, dy, alpha):
+    """Transpose of _ddy (box)."""
+    g = g.copy()
+    g[:, 0] = 0.0
+    g[:, -1] = 0.0
+    out = np.zeros_like(g)
+    out[2:, :] += g[1:-1, :] / (2.0 * dy)
+    out[:-2, :] -= g[1:-1, :] / (2.0 * dy)
+    c = dy * (0.5 * alpha + 1.0)
+    out[1, :] += g[0, :] / c
+    out[0, :] -= g[0, :] / c
+    out[-1, :] += g[-1, :] / c
+    out[-2, :] -= g[-1, :] / c
+    return out
+
+
+def ekei1(p1: np.ndarray, dx: float, dy: float, fnot: float, bcco: float, rho: float, h1: float):
+    """Layer-1 kinetic energy (qg_monitor ekei(1)) of the pressure p1 (nyp, nxp)
+    and its gradient with respect to p1."""
+    w = _trap_weights(*p1.shape, dx, dy)
+    u = -_ddy(p1, dy, bcco) / fnot
+    v = _ddx(p1, dx, bcco) / fnot
+    J = 0.5 * rho * h1 * float(np.sum(w * (u * u + v * v)))
+    gu = rho * h1 * w * u
+    gv = rho * h1 * w * v
+    grad = -_ddy_T(gu, dy, bcco) / fnot + _ddx_T(gv, dx, bcco) / fnot
+    return J, grad
+
+
+# ------------------------------------------------------------------ device adjoint state
+
+class AdjointState:
+    """The adjoint buffers and state snapshots of one box basin."""
+
+    def __init__(self, qg: QG):
+        self.qg = qg
+        self._lib = qg._lib
+        check(self._lib.oqg_adj_init(qg._h), qg._h, "oqg_adj_init")
+        self.nslots = 0
+
+    def _c(self, rc, what):
+        check(rc, self.qg._h, what)
+
+    def alloc(self, nslots: int):
+        if nslots > self.nslots:
+            self._c(self._lib.oqg_snap_alloc(self.qg._h, int(nslots)), "oqg_snap_alloc")
+            self.nslots = nslots
+
+    def save(self, slot: int):
+        self._c(self._lib.oqg_snap_save(self.qg._h, int(slot)), "oqg_snap_save")
+
+    def load(self, slot: int):
+        self._c(self._lib.oqg_snap_load(self.qg._h, int(slot)), "oqg_snap_load")
+
+    def set(self, lq=None, lqm=None, lp=None, lpm=None):
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (lq, lqm, lp, lpm)]
+        ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in keep]
+        self._c(self._lib.oqg_adj_set(self.qg._h, *ptrs, OQG_HOST), "oqg_adj_set")
+
+    def get(self):
+        out = [np.empty(self.qg.shape) for _ in range(4)]
+        self._c(self._lib.oqg_adj_get(self.qg._h, *(o.ctypes.data_as(ctypes.c_void_p) for o in out), OQG_HOST),
+                "oqg_adj_get")
+        return tuple(out)
+
+    def step(self, tdt: float, suppressed: bool):
+        self._c(self._lib.oqg_adj_step(self.qg._h, float(tdt), int(bool(suppressed))), "oqg_adj_step")
+
+    def wekp(self) -> np.ndarray:
+        out = np.empty((self.qg.b.nyp, self.qg.b.nxp))
+        self._c(self._lib.oqg_adj_get_wekp(self.qg._h, out.ctypes.data_as(ctypes.c_void_p), OQG_HOST),
+                "oqg_adj_get_wekp")
+        return out
+
+    def zero_wekp(self):
+        self._c(self._lib.oqg_adj_zero_wekp(self.qg._h), "oqg_adj_zero_wekp")
+
+    def invert(self):
+        self._c(self._lib.oqg_invert(self.qg._h), "oqg_invert")
+
+    def adj_invert(self):
+        self._c(self._lib.oqg_adj_invert(self.qg._h), "oqg_adj_invert")
+
+
+# ------------------------------------------------------------------ checkpointing
+
+def _revolve_split(n: int, s: int) -> int:
+    """Steps to advance before the next checkpoint when reversing n steps with
+    s free snapshot slots: with r the least repetition count for which
+    C(s + r, s) >= n, the left part holds C(s + r - 1, s) steps."""
+    r = 0
+    while comb(s + r, s) < n:
+        r += 1
+    m = comb(s + r - 1, s) if r > 0 else 1
+    return max(1, min(n - 1, m))
+
+
+class Trajectory:
+    """Forward run of `nsteps` from the loaded state with the schedule of
+    step_qg + suppress_comp_mode, and its reverse sweep."""
+
+    def __init__(self, adj: AdjointState, tdt: float, nsteps: int, supp_every: int, snapshots: Optional[int]):
+        self.adj, self.tdt, self.n, self.supp = adj, float(tdt), int(nsteps), int(supp_every)
+        self.snapshots = snapshots
+        self.forward_steps = 0   # counts recomputation too
+
+    def _step(self, nt: int):
+        qg = self.adj.qg
+        check(qg._lib.oqg_step(qg._h, self.tdt, None, 0, None, 0.0, 0.0), qg._h, "oqg_step")
+        if self.supp > 0 and (nt - 1) % self.supp == 0:
+            qg.suppress()
+        self.forward_steps += 1
+
+    def _suppressed(self, nt: int) -> bool:
+        return self.supp > 0 and (nt - 1) % self.supp == 0
+
+    def forward(self):
+        """Run forward; returns after step n with the final state loaded."""
+        a = self.adj
+        if self.snapshots is None:
+            a.alloc(self.n + 1)
+            for nt in range(1, self.n + 1):
+                a.save(nt - 1)
+                self._step(nt)
+        else:
+            a.alloc(self.snapshots + 1)
+            a.save(0)
+            for nt in range(1, self.n + 1):
+                self._step(nt)
+
+    def reverse(self):
+        """With the adjoint of the final state set, sweep back to the initial state."""
+        a = self.adj
+        if self.snapshots is None:
+            for nt in range(self.n, 0, -1):
+                a.load(nt - 1)
+                a.step(self.tdt, self._suppressed(nt))
+            a.load(0)
+            return
+        self._rev(0, self.n, 0, self.snapshots)
+        a.load(0)
+
+    def _advance(self, lo: int, hi: int):
+        for nt in range(lo + 1, hi + 1):
+            self._step(nt)
+
+    def _rev(self, lo: int, hi: int, slot: int, free: int):
+        """State lo is in `slot`; the adjoint holds state hi; leave it at state lo."""
+        a = self.adj
+        n = hi - lo
+        if n == 1:
+            a.load(slot)
+            a.step(self.tdt, self._suppressed(hi))
+            return
+        if free == 0:
+            for k in range(hi - 1, lo - 1, -1):
+                a.load(slot)
+                self._advance(lo, k)
+                a.step(self.tdt, self._suppressed(k + 1))
+            return
+        m = lo + _revolve_split(n, free)
+        a.load(slot)
+        self._advance(lo, m)
+        a.save(slot + 1)
+        self._rev(m, hi, slot + 1, free - 1)
+        self._rev(lo, m, slot, free)
+
+
+def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: float, nsteps: int,
+             supp_every: int = 25, snapshots: Optional[int] = None, rho: float = 1000.0):
+    """J = ekei(1) after nsteps from the state whose interior PV is q0
+    (nl, nyp, nxp; ring ignored) under the kinematic wind stress (taux, tauy);
+    returns (J, dJ/dtaux, dJ/dtauy, dJ/dq0 (interior, ring 0), forward steps run)."""
+    b = qg.b
+    if b.cyclic:
+        raise ValueError("the adjoint is implemented for box basins")
+    adj = AdjointState(qg)
+    wekp, _, _ = ekman_wekp(b.nxp, b.nyp, b.dx, b.dy, b.fnot, False, taux, tauy)
+    ent = np.zeros((max(b.nl - 1, 1), b.nyp, b.nxp))
+    qg.set_forcing(ent, wekp, 0.0, 0.0)
+    qg.set_state(q=q0, qm=q0, p=np.zeros(qg.shape), pm=np.zeros(qg.shape))
+    qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
+    adj.invert()
+    traj = Trajectory(adj, tdt, nsteps, supp_every, snapshots)
+    traj.forward()
+    p = qg.p
+    J, gp1 = ekei1(p[0], b.dx, b.dy, b.fnot, qg.bcco, rho, b.h[0])
+    lp = np.zeros(qg.shape)
+    lp[0] = gp1
+    adj.set(lp=lp)
+    adj.zero_wekp()
+    traj.reverse()
+    lw = adj.wekp()
+    adj.adj_invert()
+    gq0 = adj.get()[0]
+    gtx, gty = ekman_wekp_adjoint(b.nxp, b.nyp, b.dx, b.dy, b.fnot, lw)
+    return J, gtx, gty, gq0, traj.forward_steps
+
+
+def objective(qg: QG, q0, taux, tauy, tdt: float, nsteps: int, supp_every: int = 25, rho: float = 1000.0) -> float:
+    """The forward map of `gradient` alone (for finite-difference checks)."""
+    b = qg.b
+    wekp, _, _ = ekman_wekp(b.nxp, b.nyp, b.dx, b.dy, b.fnot, False, taux, tauy)
+    ent = np.zeros((max(b.nl - 1, 1), b.nyp, b.nxp))
+    qg.set_forcing(ent, wekp, 0.0, 0.0)
+    qg.set_state(q=q0, qm=q0, p=np.zeros(qg.shape), pm=np.zeros(qg.shape))
+    qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
+    check(qg._lib.oqg_invert(qg._h), qg._h, "oqg_invert")
+    qg.run(tdt, nsteps, 0, supp_every)
+    qg.sync()
+    return ekei1(qg.p[0], b.dx, b.dy, b.fnot, qg.bcco, rho, b.h[0])[0]
+
+
+def make_autograd(qg: QG, tdt: float, nsteps: int, supp_every: int = 25, snapshots: Optional[int] = None):
+    """A torch.autograd.Function J(taux, tauy, q0) backed by `gradient`."""
+    import torch
+
+    class _KE(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, taux, tauy, q0):
+            J, gx, gy, gq, _ = gradient(qg, q0.detach().cpu().numpy(), taux.detach().cpu().numpy(),
+                                        tauy.detach().cpu().numpy(), tdt, nsteps, supp_every, snapshots)
+            ctx.save_for_backward(*(torch.from_numpy(a).to(taux.device) for a in (gx, gy, gq)))
+            return torch.tensor(J, dtype=torch.float64, device=taux.device)
+
+        @staticmethod
+        def backward(ctx, gout):
+            gx, gy, gq = ctx.saved_tensors
+            return gout * gx, gout * gy, gout * gq
+
+    return _KE.apply

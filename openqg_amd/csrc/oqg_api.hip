@@ -75,6 +75,21 @@ struct oqg_ctx {
     std::string err;
     // host copies
     std::vector<double> yprel_h, ddyn_h;
+    // adjoint (allocated by oqg_adj_init): adjoints of q/p at both time-level
+    // parities (mirroring qbuf/pbuf), of the Ekman pumping, and scratch
+    bool adj = false;
+    double* lq[2] = {nullptr, nullptr};
+    double* lp[2] = {nullptr, nullptr};
+    double* lwek = nullptr;
+    double* adjW = nullptr;       // nl fields
+    double* adjR = nullptr;       // nl fields
+    double* adjG = nullptr;       // nl fields: H_m^-1 of the interior indicator
+    double* adjPart = nullptr;
+    double* adjS = nullptr;
+    double* adjCinvT = nullptr;
+    // state snapshots (oqg_snap_*): q[0..1], p[0..1], c[0..1] and the parity per slot
+    std::vector<double*> snaps;
+    std::vector<int> snap_cur;
 };
 
 // ------------------------------------------------------------------ error helpers
@@ -339,13 +354,14 @@ static void fill_inv_common(oqg_ctx* c, InvArgs& ia) {
     ia.raw_mode = -1;
 }
 
-// Raw Helmholtz solve of mode m (0-based) from raw[0] into raw[1].
-static int enqueue_raw_solve(oqg_ctx* c, int m) {
+// Raw Helmholtz solve of mode m (0-based): out = H_m^-1 in on the interior,
+// 0 on the boundary (default buffers raw[0] -> raw[1]).
+static int enqueue_raw_solve(oqg_ctx* c, int m, const double* in = nullptr, double* out = nullptr) {
     FwdArgs fa{};
     fa.g = c->g;
     fa.s = c->s;
     fa.mx = c->mx;
-    fa.q = c->raw[0];
+    fa.q = in ? in : c->raw[0];
     fa.yprel = c->yprel;
     fa.tw = c->tw;
     fa.spec = c->spec;
@@ -362,7 +378,7 @@ static int enqueue_raw_solve(oqg_ctx* c, int m) {
     InvArgs ia;
     fill_inv_common(c, ia);
     ia.raw_mode = m;
-    ia.out_raw = c->raw[1];
+    ia.out_raw = out ? out : c->raw[1];
     launch_inv(ia, c->st);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1146,6 +1162,302 @@ int oqg_kernel_times(oqg_ctx* c, double tdt, int reps, double* ms, int max, char
         names[names_len - 1] = '\0';
     }
     return n;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ adjoint (box)
+//
+// Reverse mode of step_qg for box basins.  The adjoint buffers mirror the
+// forward state's time-level parity: with the forward state S_n loaded (q_n at
+// qbuf[cur]), lq[cur]/lq[1-cur] hold the adjoints of q_n/q_{n-1}, and lp
+// likewise.  One adjoint step takes the adjoints of S_{n+1} (current parity
+// 1 - cur) to those of S_n, reading only q_n and p_n of the forward state.
+
+static int adj_require(oqg_ctx* c) {
+    if (!c->adj) return fail(c, OQG_ERR_STATE, "call oqg_adj_init first");
+    return 0;
+}
+
+// p_new = Inv(q) with the current parity's q as input (interior), the
+// inversion of one step: pnew over p, q's boundary ring by ocqbdy, both time
+// levels set equal.  tdt = 0 leaves the mass constraint's dpi as it is.
+static int enqueue_invert(oqg_ctx* c) {
+    const int cur = c->cur;
+    StepArgs a;
+    make_step_args(c, 0.0, a);
+    a.fa.q = c->qbuf[cur];
+    a.ia.cin = c->cbuf[cur];
+    a.ia.cout = c->cbuf[1 - cur];
+    a.ia.pnew = c->pbuf[cur];
+    a.ia.qnew = c->qbuf[cur];
+    a.ia.tdt = 0.0;
+    launch_fwd(a.fa, c->g.nl, c->st);
+    launch_ysolve(a.ya, c->variant, c->st);
+    launch_inv(a.ia, c->st);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->qbuf[1 - cur], c->qbuf[cur], c->fbytes, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->pbuf[1 - cur], c->pbuf[cur], c->fbytes, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->cbuf[cur], c->cbuf[1 - cur], C_LEN * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    return 0;
+}
+
+// lq (interior) += M^T lp, where p = M(q) is the inversion (mix, Helmholtz
+// solves, homogeneous correction, mix), with lp already holding the adjoint of
+// p including what the boundary PV passed on.
+static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp) {
+    const int nl = c->g.nl;
+    AdjInv1Args i1{};
+    i1.g = c->g;
+    i1.mx = c->mx;
+    i1.lp = lp;
+    i1.hom = c->hom;
+    i1.w = c->adjW;
+    i1.partial = c->adjPart;
+    i1.nblocks = adj_inv1_blocks(c->g);
+    launch_adj_inv1(i1, c->st);
+    for (int m = 0; m < nl; m++)
+        if (int r = enqueue_raw_solve(c, m, c->adjW + (size_t)m * c->g.fs, c->adjR + (size_t)m * c->g.fs)) return r;
+    AdjInv2Args i2{};
+    i2.g = c->g;
+    i2.mx = c->mx;
+    i2.cinvT = c->adjCinvT;
+    i2.partial = c->adjPart;
+    i2.nblocks = i1.nblocks;
+    i2.s = c->adjS;
+    launch_adj_inv2(i2, c->st);
+    AdjMixArgs mx{};
+    mx.g = c->g;
+    mx.mx = c->mx;
+    mx.R = c->adjR;
+    mx.G = c->adjG;
+    mx.s = c->adjS;
+    mx.lq = lq;
+    launch_adj_mix(mx, c->st);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+static void adj_bdy(oqg_ctx* c, const double* lq, double* lp) {
+    AdjBdyArgs b{};
+    b.g = c->g;
+    b.ph = c->ph;
+    b.mx = c->mx;
+    b.lq = lq;
+    b.lp = lp;
+    launch_adj_bdy(b, c->st);
+}
+
+static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed) {
+    const int cur = c->cur, cn = 1 - cur;
+    if (suppressed) {
+        AdjSuppArgs s{};
+        s.g = c->g;
+        s.lq = c->lq[cn];
+        s.lqm = c->lq[cur];
+        s.lp = c->lp[cn];
+        s.lpm = c->lp[cur];
+        launch_adj_supp(s, c->st);
+    }
+    adj_bdy(c, c->lq[cn], c->lp[cn]);
+    if (int r = enqueue_adj_inversion(c, c->lq[cn], c->lp[cn])) return r;
+    AdjVortArgs v{};
+    v.g = c->g;
+    v.ph = c->ph;
+    v.q = c->qbuf[cur];
+    v.p = c->pbuf[cur];
+    v.lq_np1 = c->lq[cn];
+    v.lq_n = c->lq[cur];
+    v.lp_n = c->lp[cur];
+    v.lp_nm1 = c->lp[cn];
+    v.lwek = c->lwek;
+    v.tdt = tdt;
+    launch_adj_vort(v, c->st);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+extern "C" {
+
+int oqg_adj_init(oqg_ctx* c) {
+    if (!c) return OQG_ERR_ARG;
+    if (c->g.cyclic) return fail(c, OQG_ERR_ARG, "the adjoint is implemented for box basins");
+    if (c->adj) return 0;
+    const size_t fl = (size_t)c->g.nl * c->g.fs, f1 = (size_t)c->g.fs;
+    int rc = 0;
+    for (int t = 0; t < 2 && !rc; t++) {
+        rc = dalloc(c, &c->lq[t], fl);
+        if (!rc) rc = dalloc(c, &c->lp[t], fl);
+    }
+    if (!rc) rc = dalloc(c, &c->lwek, f1);
+    if (!rc) rc = dalloc(c, &c->adjW, fl);
+    if (!rc) rc = dalloc(c, &c->adjR, fl);
+    if (!rc) rc = dalloc(c, &c->adjG, fl);
+    if (!rc) rc = dalloc(c, &c->adjPart, (size_t)256 * MAXL);
+    if (!rc) rc = dalloc(c, &c->adjS, MAXL);
+    if (!rc) rc = dalloc(c, &c->adjCinvT, MAXL * MAXL);
+    if (rc) return rc;
+    // (cdhoc^-1)^T by Gauss-Jordan in long double
+    const int n = c->g.nl - 1;
+    long double A[MAXL][2 * MAXL] = {};
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) A[i][j] = c->bh.cdhoc[i * MAXL + j];
+        A[i][n + i] = 1.0L;
+    }
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        for (int r = col + 1; r < n; r++)
+            if (fabsl(A[r][col]) > fabsl(A[piv][col])) piv = r;
+        if (A[piv][col] == 0.0L) return fail(c, OQG_ERR_NUMERIC, "singular mass-constraint matrix");
+        if (piv != col)
+            for (int j = 0; j < 2 * n; j++) std::swap(A[piv][j], A[col][j]);
+        const long double d = A[col][col];
+        for (int j = 0; j < 2 * n; j++) A[col][j] /= d;
+        for (int r = 0; r < n; r++)
+            if (r != col) {
+                const long double f = A[r][col];
+                for (int j = 0; j < 2 * n; j++) A[r][j] -= f * A[col][j];
+            }
+    }
+    std::vector<double> cinvT(MAXL * MAXL, 0.0);
+    for (int l = 0; l < n; l++)
+        for (int j = 0; j < n; j++) cinvT[l * MAXL + j] = (double)A[j][n + l];   // (C^-1)^T[l][j] = C^-1[j][l]
+    HIPCHK(c, hipMemcpyAsync(c->adjCinvT, cinvT.data(), cinvT.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    // G_m = H_m^-1 (1 on the interior)
+    std::vector<double> one(f1, 0.0);
+    for (int j = 1; j < c->g.nyp - 1; j++)
+        for (int i = 1; i < c->g.nxp - 1; i++) one[(size_t)j * c->g.pitch + i] = 1.0;
+    HIPCHK(c, hipMemcpyAsync(c->adjW, one.data(), f1 * sizeof(double), hipMemcpyHostToDevice, c->st));
+    for (int m = 0; m < c->g.nl; m++)
+        if (int r = enqueue_raw_solve(c, m, c->adjW, c->adjG + (size_t)m * f1)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->adj = true;
+    return 0;
+}
+
+int oqg_adj_set(oqg_ctx* c, const double* lq, const double* lqm, const double* lp, const double* lpm, int kind) {
+    if (!c) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    double* dst[4] = {c->lq[c->cur], c->lq[1 - c->cur], c->lp[c->cur], c->lp[1 - c->cur]};
+    const double* src[4] = {lq, lqm, lp, lpm};
+    for (int t = 0; t < 4; t++) {
+        if (src[t]) {
+            if (int r = upload_field(c, dst[t], src[t], c->g.nl, kind)) return r;
+        } else {
+            HIPCHK(c, hipMemsetAsync(dst[t], 0, c->fbytes, c->st));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_adj_get(oqg_ctx* c, double* lq, double* lqm, double* lp, double* lpm, int kind) {
+    if (!c) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    const double* src[4] = {c->lq[c->cur], c->lq[1 - c->cur], c->lp[c->cur], c->lp[1 - c->cur]};
+    double* dst[4] = {lq, lqm, lp, lpm};
+    for (int t = 0; t < 4; t++)
+        if (dst[t])
+            if (int r = download_field(c, dst[t], src[t], c->g.nl, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_adj_step(oqg_ctx* c, double tdt, int suppressed) {
+    if (!c) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    return enqueue_adj_step(c, tdt, suppressed != 0);
+}
+
+int oqg_adj_get_wekp(oqg_ctx* c, double* out, int kind) {
+    if (!c || !out) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    if (int r = download_field(c, out, c->lwek, 1, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_adj_zero_wekp(oqg_ctx* c) {
+    if (!c) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    HIPCHK(c, hipMemsetAsync(c->lwek, 0, (size_t)c->g.fs * sizeof(double), c->st));
+    return 0;
+}
+
+int oqg_invert(oqg_ctx* c) {
+    if (!c) return OQG_ERR_ARG;
+    return enqueue_invert(c);
+}
+
+int oqg_adj_invert(oqg_ctx* c) {
+    if (!c) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    const int cur = c->cur;
+    const long long n = (long long)c->g.nl * c->g.fs;
+    launch_adj_add(c->lq[cur], c->lq[1 - cur], n, c->st);
+    launch_adj_add(c->lp[cur], c->lp[1 - cur], n, c->st);
+    adj_bdy(c, c->lq[cur], c->lp[cur]);
+    if (int r = enqueue_adj_inversion(c, c->lq[cur], c->lp[cur])) return r;
+    // only the interior of q is an input of the inversion
+    std::vector<double> zero((size_t)std::max(c->g.pitch, c->g.nyp), 0.0);
+    for (int k = 0; k < c->g.nl; k++) {
+        double* base = c->lq[cur] + (size_t)k * c->g.fs;
+        HIPCHK(c, hipMemcpyAsync(base, zero.data(), c->g.nxp * sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(base + (size_t)(c->g.nyp - 1) * c->g.pitch, zero.data(), c->g.nxp * sizeof(double),
+                                 hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpy2DAsync(base, c->g.pitch * sizeof(double), zero.data(), sizeof(double), sizeof(double),
+                                   c->g.nyp, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpy2DAsync(base + c->g.nxp - 1, c->g.pitch * sizeof(double), zero.data(), sizeof(double),
+                                   sizeof(double), c->g.nyp, hipMemcpyHostToDevice, c->st));
+    }
+    HIPCHK(c, hipMemsetAsync(c->lq[1 - cur], 0, c->fbytes, c->st));
+    HIPCHK(c, hipMemsetAsync(c->lp[cur], 0, c->fbytes, c->st));
+    HIPCHK(c, hipMemsetAsync(c->lp[1 - cur], 0, c->fbytes, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_snap_alloc(oqg_ctx* c, int nslots) {
+    if (!c || nslots < 0) return OQG_ERR_ARG;
+    const size_t words = 4 * (size_t)c->g.nl * c->g.fs + 2 * C_LEN;
+    while ((int)c->snaps.size() < nslots) {
+        double* p = nullptr;
+        if (int r = dalloc(c, &p, words)) return r;
+        c->snaps.push_back(p);
+        c->snap_cur.push_back(0);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_snap_save(oqg_ctx* c, int slot) {
+    if (!c || slot < 0 || slot >= (int)c->snaps.size()) return c ? fail(c, OQG_ERR_ARG, "snapshot slot out of range") : OQG_ERR_ARG;
+    double* s = c->snaps[slot];
+    const size_t fl = (size_t)c->g.nl * c->g.fs;
+    const double* src[6] = {c->qbuf[0], c->qbuf[1], c->pbuf[0], c->pbuf[1], c->cbuf[0], c->cbuf[1]};
+    size_t off = 0;
+    for (int t = 0; t < 6; t++) {
+        const size_t n = t < 4 ? fl : (size_t)C_LEN;
+        HIPCHK(c, hipMemcpyAsync(s + off, src[t], n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        off += n;
+    }
+    c->snap_cur[slot] = c->cur;
+    return 0;
+}
+
+int oqg_snap_load(oqg_ctx* c, int slot) {
+    if (!c || slot < 0 || slot >= (int)c->snaps.size()) return c ? fail(c, OQG_ERR_ARG, "snapshot slot out of range") : OQG_ERR_ARG;
+    const double* s = c->snaps[slot];
+    const size_t fl = (size_t)c->g.nl * c->g.fs;
+    double* dst[6] = {c->qbuf[0], c->qbuf[1], c->pbuf[0], c->pbuf[1], c->cbuf[0], c->cbuf[1]};
+    size_t off = 0;
+    for (int t = 0; t < 6; t++) {
+        const size_t n = t < 4 ? fl : (size_t)C_LEN;
+        HIPCHK(c, hipMemcpyAsync(dst[t], s + off, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        off += n;
+    }
+    c->cur = c->snap_cur[slot];
+    return 0;
 }
 
 }  // extern "C"

@@ -200,6 +200,70 @@ struct QcompArgs {
     double* q;
 };
 
+// Adjoint kernels (oqg_adjoint.hip), box basins.
+struct AdjSuppArgs {
+    Geo g;
+    double *lq, *lqm, *lp, *lpm;
+};
+
+struct AdjBdyArgs {
+    Geo g;
+    Phys ph;
+    Mix mx;
+    const double* lq;   // adjoint of the q whose boundary ring ocqbdy wrote
+    double* lp;         // adjoint of the p it was computed from (+=)
+};
+
+struct AdjInv1Args {
+    Geo g;
+    Mix mx;
+    const double* lp;   // adjoint of p_new
+    const double* hom;  // box homogeneous solutions, (nl-1) fields
+    double* w;          // out: adjoint of each modal field, nl fields
+    double* partial;    // out: per-block sums of hom_m . w_m, [block][MAXL]
+    int nblocks;
+};
+
+struct AdjInv2Args {
+    Geo g;
+    Mix mx;
+    const double* cinvT;     // (cdhoc^-1)^T, [l*MAXL + j]
+    const double* partial;
+    int nblocks;
+    double* s;               // out: per-mode constant added to the interior of w_m
+};
+
+struct AdjMixArgs {
+    Geo g;
+    Mix mx;
+    const double* R;    // Helmholtz solves of w_m, nl fields
+    const double* G;    // Helmholtz solves of the interior indicator, nl fields
+    const double* s;
+    double* lq;         // adjoint of q_new (+= on the interior)
+};
+
+struct AdjVortArgs {
+    Geo g;
+    Phys ph;
+    const double* q;    // forward state before the step: q_n, p_n
+    const double* p;
+    double* lq_np1;     // in: adjoint of q_{n+1}; out: adjoint of q_{n-1} (ring zeroed)
+    double* lq_n;       // adjoint of q_n (+=)
+    double* lp_n;       // adjoint of p_n (+=)
+    double* lp_nm1;     // out: adjoint of p_{n-1}
+    double* lwek;       // adjoint of the Ekman pumping field (+=)
+    double tdt;
+};
+
+void launch_adj_supp(const AdjSuppArgs& a, hipStream_t st);
+void launch_adj_bdy(const AdjBdyArgs& a, hipStream_t st);
+void launch_adj_inv1(const AdjInv1Args& a, hipStream_t st);
+void launch_adj_inv2(const AdjInv2Args& a, hipStream_t st);
+void launch_adj_mix(const AdjMixArgs& a, hipStream_t st);
+void launch_adj_add(double* dst, const double* src, long long n, hipStream_t st);
+void launch_adj_vort(const AdjVortArgs& a, hipStream_t st);
+int adj_inv1_blocks(const Geo& g);
+
 // Most dynamic LDS a launch of the LDS-staged kernels may ask for (the rest of
 // the 160 KB of a CU is left to their static LDS).
 constexpr size_t MAX_DYN_LDS = 150 * 1024;

@@ -49,6 +49,47 @@ def ekman_wekp(nxp: int, nyp: int, dx: float, dy: float, fnot: float, cyclic: bo
     return w, txis, txin
 
 
+def ekman_wekp_adjoint(nxp: int, nyp: int, dx: float, dy: float, fnot: float, lam_w: np.ndarray):
+    """Transpose of (taux, tauy) -> wekp of ekman_wekp for a box: given the
+    adjoint of wekp (nyp, nxp), the adjoints of the kinematic stresses
+    (taux, tauy), each (nyp, nxp).  Every line reverses one line above."""
+    lw = np.asarray(lam_w, dtype=np.float64)
+    lt = np.zeros((nyp - 1, nxp - 1))
+    # map_T_to_P (box)
+    q = 0.25 * lw[1:-1, 1:-1]
+    lt[1:, 1:] += q
+    lt[1:, :-1] += q
+    lt[:-1, 1:] += q
+    lt[:-1, :-1] += q
+    lt[0, :-1] += 0.5 * lw[0, 1:-1]
+    lt[0, 1:] += 0.5 * lw[0, 1:-1]
+    lt[-1, :-1] += 0.5 * lw[-1, 1:-1]
+    lt[-1, 1:] += 0.5 * lw[-1, 1:-1]
+    lt[:-1, 0] += 0.5 * lw[1:-1, 0]
+    lt[1:, 0] += 0.5 * lw[1:-1, 0]
+    lt[:-1, -1] += 0.5 * lw[1:-1, -1]
+    lt[1:, -1] += 0.5 * lw[1:-1, -1]
+    lt[0, 0] += lw[0, 0]
+    lt[0, -1] += lw[0, -1]
+    lt[-1, 0] += lw[-1, 0]
+    lt[-1, -1] += lw[-1, -1]
+    # wekt = (tauy_t[:,1:] - tauy_t[:,:-1])/(dx f0) - (taux_t[1:,:] - taux_t[:-1,:])/(dy f0)
+    ltauy_t = np.zeros((nyp - 1, nxp))
+    ltaux_t = np.zeros((nyp, nxp - 1))
+    ltauy_t[:, 1:] += lt / (dx * fnot)
+    ltauy_t[:, :-1] -= lt / (dx * fnot)
+    ltaux_t[1:, :] -= lt / (dy * fnot)
+    ltaux_t[:-1, :] += lt / (dy * fnot)
+    # face averages
+    ltaux = np.zeros((nyp, nxp))
+    ltauy = np.zeros((nyp, nxp))
+    ltaux[:, 1:] += 0.5 * ltaux_t
+    ltaux[:, :-1] += 0.5 * ltaux_t
+    ltauy[1:, :] += 0.5 * ltauy_t
+    ltauy[:-1, :] += 0.5 * ltauy_t
+    return ltaux, ltauy
+
+
 def double_gyre_tau(nyp: int, nxp: int, dy: float, tau0: float, rho0: float = 1000.0):
     """tau_x = tau0 (1 - cos(2 pi y / L)) / rho0 with y = (j-1) dy, L = (nyp-1) dy; tau_y = 0."""
     y = np.arange(nyp) * dy

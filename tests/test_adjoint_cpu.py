@@ -1,0 +1,107 @@
+"""CPU checks of the host parts of the gradient path: the wind-stress adjoint,
+the kinetic-energy objective's gradient, and the checkpointing schedules."""
+import numpy as np
+
+from openqg_amd.adjoint import Trajectory, _revolve_split, ekei1
+from openqg_amd.forcing import ekman_wekp, ekman_wekp_adjoint
+
+
+def test_ekman_wekp_adjoint_is_the_transpose():
+    rng = np.random.default_rng(0)
+    ny, nx, dx, dy, f0 = 13, 17, 3.0e4, 2.5e4, 1.1e-4
+    tx, ty = rng.standard_normal((ny, nx)), rng.standard_normal((ny, nx))
+    lw = rng.standard_normal((ny, nx))
+    w, _, _ = ekman_wekp(nx, ny, dx, dy, f0, False, tx, ty)
+    ax, ay = ekman_wekp_adjoint(nx, ny, dx, dy, f0, lw)
+    lhs = float(np.sum(lw * w))
+    rhs = float(np.sum(ax * tx) + np.sum(ay * ty))
+    assert abs(lhs - rhs) <= 1e-13 * max(abs(lhs), 1.0)
+
+
+def test_ekei1_gradient_matches_central_differences():
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal((11, 9))
+    args = (3.0e4, 3.0e4, 1.0e-4, 0.5, 1000.0, 300.0)
+    J, g = ekei1(p, *args)
+    for _ in range(4):
+        d = rng.standard_normal(p.shape)
+        eps = 1e-6
+        fd = (ekei1(p + eps * d, *args)[0] - ekei1(p - eps * d, *args)[0]) / (2 * eps)
+        assert abs(fd - np.sum(g * d)) <= 1e-7 * abs(fd)
+    # J is a quadratic form: J(p) = 0.5 <grad(p), p>
+    assert abs(J - 0.5 * np.sum(g * p)) <= 1e-12 * J
+
+
+class _FakeLib:
+    def __init__(self, rec):
+        self.rec = rec
+
+    def oqg_step(self, h, tdt, *a):
+        self.rec["state"] += 1
+        return 0
+
+
+class _FakeQG:
+    def __init__(self, rec):
+        self.rec = rec
+        self._lib = _FakeLib(rec)
+        self._h = None
+
+    def suppress(self):
+        pass
+
+
+class _FakeAdj:
+    """Records the state index of every snapshot and which step each adjoint
+    step reverses (the loaded state must be the one before that step)."""
+
+    def __init__(self):
+        self.rec = {"state": 0}
+        self.qg = _FakeQG(self.rec)
+        self.slots = {}
+        self.reversed = []
+        self.lam_at = None
+
+    def alloc(self, n):
+        self.nslots = n
+
+    def save(self, slot):
+        assert slot < self.nslots
+        self.slots[slot] = self.rec["state"]
+
+    def load(self, slot):
+        self.rec["state"] = self.slots[slot]
+
+    def step(self, tdt, suppressed):
+        nt = self.rec["state"] + 1            # the step whose input is loaded
+        assert nt == self.lam_at, (nt, self.lam_at)
+        self.reversed.append(nt)
+        self.lam_at -= 1
+
+
+def _sweep(n, snapshots):
+    a = _FakeAdj()
+    t = Trajectory(a, 1.0, n, 5, snapshots)
+    t.forward()
+    assert a.rec["state"] == n
+    a.lam_at = n
+    t.reverse()
+    assert a.reversed == list(range(n, 0, -1))
+    assert a.rec["state"] == 0
+    return t.forward_steps
+
+
+def test_checkpointing_reverses_every_step_once_in_order():
+    assert _sweep(17, None) == 17
+    for n, s in ((1, 1), (2, 1), (17, 1), (17, 3), (64, 4), (300, 8), (2160, 32)):
+        steps = _sweep(n, s)
+        assert steps >= n
+        if n == 2160 and s == 32:
+            assert steps <= 4 * n      # binomial: C(35, 3) >= 2160, at most ~3 extra sweeps
+
+
+def test_revolve_split_bounds():
+    for n in range(2, 200):
+        for s in range(1, 6):
+            m = _revolve_split(n, s)
+            assert 1 <= m <= n - 1
