@@ -13,7 +13,10 @@ Also reported:
                 host cores, on a bounded sample of the same workload;
   ensemble      64 uneven basins (65^2..513^2, 2-4 layers) for 10 simulated
                 days each: batched launches against the same members run one
-                after another by the plain path (aggregate simulated days/s).
+                after another by the plain path (aggregate simulated days/s);
+  adjoint       gradient of a kinetic-energy objective after 30 days of the
+                257^2 3-layer box (forward + hand-written adjoint) against the
+                plain forward run, store-all and with 32 revolve snapshots.
 """
 from __future__ import annotations
 
@@ -173,6 +176,45 @@ def ensemble_bench(members: int, days: float):
     }
 
 
+def adjoint_bench(days: float, snapshots: int):
+    """Gradient of the layer-1 kinetic energy after `days` of the 257^2 3-layer
+    box with respect to the wind stress and the initial PV: forward + adjoint
+    (store-all snapshots, and binomial checkpointing with `snapshots` slots)
+    against the plain forward run of the same steps (graph replay)."""
+    from openqg_amd import workloads as W
+    from openqg_amd.adjoint import gradient
+    from openqg_amd.forcing import double_gyre_tau
+
+    w = W.c5_box257()
+    n = int(round(days * 86400.0 / w.dt))
+    qg = w.make_qg()
+    p = w.initial_p()
+    qg.set_state(p=p, pm=p)
+    qg.init_state()
+    q0 = qg.get_state()[0]
+    taux, tauy = double_gyre_tau(w.nyp, w.nxp, w.dx, w.tau0)
+    ent, wekp, txis, txin = w.forcing()
+    qg.set_forcing(ent, wekp, txis, txin)
+    # plain forward (graphs captured in the warm-up)
+    qg.run(w.tdt, 2 * w.supp_every, 0, w.supp_every)
+    qg.sync()
+    t0 = time.perf_counter()
+    qg.run(w.tdt, n, 0, w.supp_every)
+    qg.sync()
+    fwd = time.perf_counter() - t0
+    out = {"workload": f"{w.name}_{days:g}d_ke_gradient", "steps": n, "forward_s": round(fwd, 4)}
+    for name, snaps in (("store_all", None), (f"revolve_{snapshots}", snapshots)):
+        gradient(qg, q0, taux, tauy, w.tdt, n, w.supp_every, snaps)   # allocation, graph capture
+        qg.sync()
+        t0 = time.perf_counter()
+        J, gx, gy, gq, fsteps = gradient(qg, q0, taux, tauy, w.tdt, n, w.supp_every, snaps)
+        el = time.perf_counter() - t0
+        out[name] = {"s": round(el, 4), "ratio_vs_forward": round(el / fwd, 3), "forward_steps": int(fsteps)}
+    out["J"] = J
+    qg.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +228,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-days", type=float, default=10.0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
+    ap.add_argument("--adj-days", type=float, default=30.0)
+    ap.add_argument("--adj-snapshots", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 10 device copies of 256 MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -260,6 +305,7 @@ def main():
         cpu = cpu_baseline(w, args.cpu_seconds)
     qg.close()
     ens = None if args.no_ensemble else ensemble_bench(args.ens_members, args.ens_days)
+    adj = None if args.no_adjoint else adjoint_bench(args.adj_days, args.adj_snapshots)
     out = {
         "metric": METRIC,
         "value": round(days_per_s, 3),
@@ -278,6 +324,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

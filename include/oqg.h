@@ -207,6 +207,15 @@ int oqg_snap_alloc(oqg_ctx* ctx, int nslots);
 int oqg_snap_save(oqg_ctx* ctx, int slot);
 int oqg_snap_load(oqg_ctx* ctx, int slot);
 
+/* Store-all trajectories.  oqg_adj_record runs steps step0+1 .. step0+nsteps
+ * (suppress_comp_mode after those with mod(nt-1, supp) == 0 when supp > 0,
+ * as oqg_run) and saves the state before each step in slots slot0 ..
+ * slot0+nsteps-1.  oqg_adj_sweep, with the adjoint of the state after the
+ * last step set, takes it back through those steps to the adjoint of the
+ * state in slot slot0, which it leaves loaded.  Both are asynchronous. */
+int oqg_adj_record(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp, int slot0);
+int oqg_adj_sweep(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp, int slot0);
+
 /*
  * Batched ensembles.  No counterpart in the reference, whose driver steps one
  * basin per step_qg call (src/driver.F90:107,143); an ensemble of independent

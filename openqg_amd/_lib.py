@@ -78,6 +78,8 @@ SIGNATURES = [
     ("oqg_snap_alloc", _i, [_vp, _i]),
     ("oqg_snap_save", _i, [_vp, _i]),
     ("oqg_snap_load", _i, [_vp, _i]),
+    ("oqg_adj_record", _i, [_vp, _d, _i, _i, _i, _i]),
+    ("oqg_adj_sweep", _i, [_vp, _d, _i, _i, _i, _i]),
     ("oqg_ens_create", _i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
     ("oqg_ens_destroy", None, [_vp]),
     ("oqg_ens_last_error", ctypes.c_char_p, [_vp]),

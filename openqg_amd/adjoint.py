@@ -8,10 +8,11 @@ Controls: the kinematic wind stress (taux, tauy) at p points, through
 ekman_from_tau + map_T_to_P, and the interior PV of the initial state, whose
 pressure is one inversion of it (dpi = 0), with qm = q and pm = p.
 
-The forward run keeps device snapshots of the state; the reverse sweep loads
-them and applies oqg_adj_step.  With `snapshots=None` every state is kept
-(HBM holds thousands of them); with a number, binomial checkpointing
-(Griewank's revolve schedule) recomputes the forward from the stored ones.
+The forward run keeps device snapshots of the state; the reverse sweep applies
+oqg_adj_step from them.  With `snapshots=None` every state is kept (HBM holds
+thousands of them; oqg_adj_record / oqg_adj_sweep run both sweeps from C);
+with a number, binomial checkpointing (Griewank's revolve schedule)
+recomputes the forward from the stored ones with graph-replayed oqg_run.
 """
 from __future__ import annotations
 
@@ -141,6 +142,20 @@ class AdjointState:
     def step(self, tdt: float, suppressed: bool):
         self._c(self._lib.oqg_adj_step(self.qg._h, float(tdt), int(bool(suppressed))), "oqg_adj_step")
 
+    def advance(self, tdt: float, nsteps: int, step0: int, supp_every: int):
+        """Forward steps step0+1 .. step0+nsteps without snapshots (oqg_run)."""
+        self.qg.run(tdt, nsteps, step0, supp_every)
+
+    def record(self, tdt: float, nsteps: int, step0: int, supp_every: int, slot0: int):
+        """Forward steps saving the state before each in slots slot0 + i."""
+        self._c(self._lib.oqg_adj_record(self.qg._h, float(tdt), int(nsteps), int(step0), int(supp_every),
+                                         int(slot0)), "oqg_adj_record")
+
+    def sweep(self, tdt: float, nsteps: int, step0: int, supp_every: int, slot0: int):
+        """Adjoint steps back through a record; leaves slot0's state loaded."""
+        self._c(self._lib.oqg_adj_sweep(self.qg._h, float(tdt), int(nsteps), int(step0), int(supp_every),
+                                        int(slot0)), "oqg_adj_sweep")
+
     def wekp(self) -> np.ndarray:
         out = np.empty((self.qg.b.nyp, self.qg.b.nxp))
         self._c(self._lib.oqg_adj_get_wekp(self.qg._h, out.ctypes.data_as(ctypes.c_void_p), OQG_HOST),
@@ -179,13 +194,6 @@ class Trajectory:
         self.snapshots = snapshots
         self.forward_steps = 0   # counts recomputation too
 
-    def _step(self, nt: int):
-        qg = self.adj.qg
-        check(qg._lib.oqg_step(qg._h, self.tdt, None, 0, None, 0.0, 0.0), qg._h, "oqg_step")
-        if self.supp > 0 and (nt - 1) % self.supp == 0:
-            qg.suppress()
-        self.forward_steps += 1
-
     def _suppressed(self, nt: int) -> bool:
         return self.supp > 0 and (nt - 1) % self.supp == 0
 
@@ -193,31 +201,27 @@ class Trajectory:
         """Run forward; returns after step n with the final state loaded."""
         a = self.adj
         if self.snapshots is None:
-            a.alloc(self.n + 1)
-            for nt in range(1, self.n + 1):
-                a.save(nt - 1)
-                self._step(nt)
+            a.alloc(max(self.n, 1))
+            a.record(self.tdt, self.n, 0, self.supp, 0)
+            self.forward_steps += self.n
         else:
             a.alloc(self.snapshots + 1)
             a.save(0)
-            for nt in range(1, self.n + 1):
-                self._step(nt)
+            self._advance(0, self.n)
 
     def reverse(self):
         """With the adjoint of the final state set, sweep back to the initial state."""
         a = self.adj
         if self.snapshots is None:
-            for nt in range(self.n, 0, -1):
-                a.load(nt - 1)
-                a.step(self.tdt, self._suppressed(nt))
-            a.load(0)
+            a.sweep(self.tdt, self.n, 0, self.supp, 0)
             return
         self._rev(0, self.n, 0, self.snapshots)
         a.load(0)
 
     def _advance(self, lo: int, hi: int):
-        for nt in range(lo + 1, hi + 1):
-            self._step(nt)
+        if hi > lo:
+            self.adj.advance(self.tdt, hi - lo, lo, self.supp)
+            self.forward_steps += hi - lo
 
     def _rev(self, lo: int, hi: int, slot: int, free: int):
         """State lo is in `slot`; the adjoint holds state hi; leave it at state lo."""

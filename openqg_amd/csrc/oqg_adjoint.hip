@@ -5,11 +5,11 @@
 // the adjoint of the Ekman pumping field).  Every operator is the exact
 // transpose of the forward discretisation, written out by hand:
 //   k_adj_supp  suppress_comp_mode (driver.F90:174-189)
-//   k_adj_bdy   ocqbdy (vorsubs.F90:58-114): boundary-ring PV -> p
-//   k_adj_inv1  mode->layer mix (modes.F90:268-284) and the homogeneous
+//   k_adj_inv1  ocqbdy (vorsubs.F90:58-114: boundary-ring PV -> p), the
+//               mode->layer mix (modes.F90:268-284) and the homogeneous
 //               correction's inner products (homog.F90:331-366)
-//   k_adj_inv2  the (nl-1)^2 constraint solve, transposed
-//   k_adj_mix   layer->mode mix (modes.F90:248-266) of the Helmholtz adjoint
+//   k_adj_mix   the (nl-1)^2 constraint solve, and the layer->mode mix
+//               (modes.F90:248-266) of the Helmholtz adjoint
 //   k_adj_vort  vort_step (vorticity.F90:14-103): Arakawa Jacobian transposes
 //               in gather form (the p transpose through the discrete
 //               antisymmetry J(q,p) = -J(p,q)), Del^2/Del^4/Del^6 transposes
@@ -63,134 +63,178 @@ __device__ __forceinline__ bool ring_inner(const Geo& g, int xi, int xj, int& ii
     return true;
 }
 
-__global__ void k_adj_bdy(AdjBdyArgs a) {
-    const Geo& g = a.g;
-    const int nx = g.nxp, ny = g.nyp, nl = g.nl;
-    // points of the two outer bands: rows 0,1,ny-2,ny-1 and columns 0,1,nx-2,nx-1
-    const long long npts = (long long)nx * ny;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npts; t += (long long)gridDim.x * blockDim.x) {
-        const int aj = (int)(t / nx), ai = (int)(t - (long long)aj * nx);
-        const bool band = aj <= 1 || aj >= ny - 2 || ai <= 1 || ai >= nx - 2;
-        if (!band) continue;
-        const long long oa = (long long)aj * g.pitch + ai;
-        double acc[MAXL];
+// What the boundary ring's adjoint passes on to lp at point (ai, aj) of the two
+// outer bands (rows 0, 1, nyp-2, nyp-1 and columns 0, 1, nxp-2, nxp-1), added
+// to lp[0..nl-1].
+template <int NL>
+__device__ __forceinline__ void bdy_gather(const Geo& g, const Phys& ph, const Mix& mx, const double* lq, int ai,
+                                           int aj, double (&lp)[NL]) {
+    const int nx = g.nxp, ny = g.nyp;
+    const long long oa = (long long)aj * g.pitch + ai;
+    int ii, ij;
+    if (ring_inner(g, ai, aj, ii, ij)) {
+        const double r = ((ai == 0 || ai == nx - 1) && (aj == 0 || aj == ny - 1)) ? 2.0 : 1.0;
 #pragma unroll
-        for (int k = 0; k < MAXL; k++) acc[k] = 0.0;
-        int ii, ij;
-        if (ring_inner(g, ai, aj, ii, ij)) {
-            const double r = ((ai == 0 || ai == nx - 1) && (aj == 0 || aj == ny - 1)) ? 2.0 : 1.0;
-            for (int k = 0; k < nl; k++) {
-                const double lqk = a.lq[k * g.fs + oa];
-                for (int l = 0; l < nl; l++) {
-                    double c = -r * g.fnot * a.mx.amat[k * MAXL + l];
-                    if (l == k) c -= a.ph.qbfac;
-                    acc[l] += lqk * c;
-                }
+        for (int k = 0; k < NL; k++) {
+            const double lqk = lq[k * g.fs + oa];
+#pragma unroll
+            for (int l = 0; l < NL; l++) {
+                double c = -r * g.fnot * mx.amat[k * MAXL + l];
+                if (l == k) c -= ph.qbfac;
+                lp[l] += lqk * c;
             }
         }
-        // ring neighbours whose inner point is a
-        const int cand[4][2] = {{ai - 1, aj}, {ai + 1, aj}, {ai, aj - 1}, {ai, aj + 1}};
-        for (int c = 0; c < 4; c++) {
-            const int xi = cand[c][0], xj = cand[c][1];
-            if (xi < 0 || xi >= nx || xj < 0 || xj >= ny) continue;
-            if (ring_inner(g, xi, xj, ii, ij) && ii == ai && ij == aj) {
-                const long long ox = (long long)xj * g.pitch + xi;
-                for (int k = 0; k < nl; k++) acc[k] += a.ph.qbfac * a.lq[k * g.fs + ox];
-            }
+    }
+    // ring neighbours whose inner point is a
+    const int cand[4][2] = {{ai - 1, aj}, {ai + 1, aj}, {ai, aj - 1}, {ai, aj + 1}};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int xi = cand[c][0], xj = cand[c][1];
+        if (xi < 0 || xi >= nx || xj < 0 || xj >= ny) continue;
+        if (ring_inner(g, xi, xj, ii, ij) && ii == ai && ij == aj) {
+            const long long ox = (long long)xj * g.pitch + xi;
+#pragma unroll
+            for (int k = 0; k < NL; k++) lp[k] += ph.qbfac * lq[k * g.fs + ox];
         }
-        for (int k = 0; k < nl; k++) a.lp[k * g.fs + oa] += acc[k];
     }
 }
 
 // ------------------------------------------------------------------ k_adj_inv1
 
-// lv_m = sum_k ctm2l(m,k) lp_k at every point (the adjoint of the modal field
+// With lp completed by the boundary ring's adjoint (bdy_gather), lv_m =
+// sum_k ctm2l(m,k) lp_k at every point (the adjoint of the modal field
 // inhomog_m + homcor_m), written to w[m]; per-block partial sums of
 // hom_m . lv_m (m >= 1) for the adjoint of the correction coefficients.
+template <int NL>
 __global__ __launch_bounds__(256) void k_adj_inv1(AdjInv1Args a) {
     const Geo& g = a.g;
-    const int nl = g.nl;
-    __shared__ double red[MAXL * 16];
-    double part[MAXL];
+    __shared__ double red[NL * 16];
+    double part[NL];
 #pragma unroll
-    for (int m = 0; m < MAXL; m++) part[m] = 0.0;
-    const long long npts = (long long)g.nxp * g.nyp;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npts; t += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(t / g.nxp), i = (int)(t - (long long)j * g.nxp);
+    for (int m = 0; m < NL; m++) part[m] = 0.0;
+    const int npts = g.nxp * g.nyp;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += gridDim.x * blockDim.x) {
+        const int j = t / g.nxp, i = t - j * g.nxp;
         const long long o = (long long)j * g.pitch + i;
-        double lp[MAXL];
-        for (int k = 0; k < nl; k++) lp[k] = a.lp[k * g.fs + o];
-        for (int m = 0; m < nl; m++) {
-            double v = 0.0;
-            for (int k = 0; k < nl; k++) v += a.mx.ctm2l[m * MAXL + k] * lp[k];
-            a.w[m * g.fs + o] = v;
-            if (m >= 1) part[m] += a.hom[(long long)(m - 1) * g.fs + o] * v;
-        }
-    }
-    block_sum<MAXL>(part, red);
-    if (threadIdx.x == 0)
-        for (int m = 0; m < nl; m++) a.partial[(long long)blockIdx.x * MAXL + m] = part[m];
-}
-
-// ------------------------------------------------------------------ k_adj_inv2
-
-// One block.  la_{m-1} = sum of the partials of mode m; the forward solves
-// cdhoc alpha = dpi - (ik_{l+1} - ik_l), ik_k = sum_m ctm2l(m,k) I_m, I_m the
-// area integral of inhomog_m (interior weight dx dy).  Transposed:
-// lr = cdhoc^-T la, lI_m = -sum_l lr_l (ctm2l(m,l+1) - ctm2l(m,l)),
-// s_m = lI_m dx dy (the constant added to the interior of lv_m).
-__global__ __launch_bounds__(256) void k_adj_inv2(AdjInv2Args a) {
-    const Geo& g = a.g;
-    const int nl = g.nl;
-    __shared__ double red[MAXL * 16];
-    double v[MAXL];
+        double lp[NL], hm[NL];
 #pragma unroll
-    for (int m = 0; m < MAXL; m++) v[m] = 0.0;
-    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x)
-        for (int m = 0; m < nl; m++) v[m] += a.partial[(long long)b * MAXL + m];
-    block_sum<MAXL>(v, red);
-    if (threadIdx.x == 0) {
-        double lr[MAXL];
-        for (int l = 0; l < nl - 1; l++) {
-            double s = 0.0;
-            for (int j = 0; j < nl - 1; j++) s += a.cinvT[l * MAXL + j] * v[j + 1];
-            lr[l] = s;
-        }
-        for (int m = 0; m < nl; m++) {
-            double s = 0.0;
-            for (int l = 0; l < nl - 1; l++)
-                s -= lr[l] * (a.mx.ctm2l[m * MAXL + l + 1] - a.mx.ctm2l[m * MAXL + l]);
-            a.s[m] = (s * g.dx) * g.dy;
+        for (int k = 0; k < NL; k++) lp[k] = a.lp[k * g.fs + o];
+#pragma unroll
+        for (int m = 1; m < NL; m++) hm[m] = a.hom[(long long)(m - 1) * g.fs + o];
+        if (j <= 1 || j >= g.nyp - 2 || i <= 1 || i >= g.nxp - 2) bdy_gather<NL>(g, a.ph, a.mx, a.lq, i, j, lp);
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) v += a.mx.ctm2l[m * MAXL + k] * lp[k];
+            a.w[m * g.fs + o] = v;
+            if (m >= 1) part[m] += hm[m] * v;
         }
     }
+    block_sum<NL>(part, red);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < NL; m++) a.partial[(long long)blockIdx.x * MAXL + m] = part[m];
 }
 
 // ------------------------------------------------------------------ k_adj_mix
 
-// lq_k += f0 sum_m ctl2m(k,m) (R_m + s_m G_m) on the interior, where R_m is the
-// Helmholtz solve of lv_m and G_m that of the interior indicator.
-__global__ void k_adj_mix(AdjMixArgs a) {
+// First, in every block alike: la_{m-1} = the sum of the partials of mode m.
+// The forward solves cdhoc alpha = dpi - (ik_{l+1} - ik_l), ik_k = sum_m
+// ctm2l(m,k) I_m, I_m the area integral of inhomog_m (interior weight dx dy).
+// Transposed: lr = cdhoc^-T la, lI_m = -sum_l lr_l (ctm2l(m,l+1) - ctm2l(m,l)),
+// s_m = lI_m dx dy, the constant added to the interior of lv_m.  Then
+// lq_k += f0 sum_m ctl2m(k,m) (R_m + s_m G_m) on the interior, where R_m is
+// the Helmholtz solve of lv_m and G_m that of the interior indicator.
+template <int NL>
+__global__ __launch_bounds__(256) void k_adj_mix(AdjMixArgs a) {
     const Geo& g = a.g;
-    const int nl = g.nl, nxi = g.nxp - 2, nyi = g.nyp - 2;
-    const long long npts = (long long)nxi * nyi;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < npts; t += (long long)gridDim.x * blockDim.x) {
-        const int j = 1 + (int)(t / nxi), i = 1 + (int)(t - (long long)(j - 1) * nxi);
+    const int nxi = g.nxp - 2, nyi = g.nyp - 2;
+    __shared__ double red[NL * 16];
+    __shared__ double s_s[NL];
+    // this thread's first point: its loads go out with those of the partials
+    const int npts = nxi * nyi;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double R[NL], G[NL], L[NL];
+    auto load_point = [&](int tt) {
+        const int tc = tt < npts ? tt : 0;
+        const int j = 1 + tc / nxi, i = 1 + (tc - (j - 1) * nxi);
         const long long o = (long long)j * g.pitch + i;
-        double r[MAXL];
-        for (int m = 0; m < nl; m++) r[m] = a.R[m * g.fs + o] + a.s[m] * a.G[m * g.fs + o];
-        for (int k = 0; k < nl; k++) {
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            R[m] = a.R[m * g.fs + o];
+            G[m] = a.G[m * g.fs + o];
+            L[m] = a.lq[m * g.fs + o];
+        }
+        return o;
+    };
+    long long o = load_point(t);
+    double v[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) v[m] = 0.0;
+    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x)
+#pragma unroll
+        for (int m = 0; m < NL; m++) v[m] += a.partial[(long long)b * MAXL + m];
+    block_sum<NL>(v, red);
+    if (threadIdx.x == 0) {
+        double lr[NL];
+#pragma unroll
+        for (int l = 0; l < NL - 1; l++) {
             double s = 0.0;
-            for (int m = 0; m < nl; m++) s += a.mx.ctl2m[k * MAXL + m] * r[m];
-            a.lq[k * g.fs + o] += g.fnot * s;
+#pragma unroll
+            for (int j = 0; j < NL - 1; j++) s += a.cinvT[l * MAXL + j] * v[j + 1];
+            lr[l] = s;
+        }
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double s = 0.0;
+#pragma unroll
+            for (int l = 0; l < NL - 1; l++)
+                s -= lr[l] * (a.mx.ctm2l[m * MAXL + l + 1] - a.mx.ctm2l[m * MAXL + l]);
+            s_s[m] = (s * g.dx) * g.dy;
+        }
+    }
+    __syncthreads();
+    double ss[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) ss[m] = s_s[m];
+    for (; t < npts; t += gridDim.x * blockDim.x) {
+        if (t != (int)(blockIdx.x * blockDim.x + threadIdx.x)) o = load_point(t);
+        double r[NL];
+#pragma unroll
+        for (int m = 0; m < NL; m++) r[m] = R[m] + ss[m] * G[m];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            double s = 0.0;
+#pragma unroll
+            for (int m = 0; m < NL; m++) s += a.mx.ctl2m[k * MAXL + m] * r[m];
+            a.lq[k * g.fs + o] = L[k] + g.fnot * s;
         }
     }
 }
 
-// ------------------------------------------------------------------ k_adj_add
+// ------------------------------------------------------------------ k_adj_add, k_copy_segs
 
 __global__ void k_adj_add(double* dst, const double* src, long long n) {
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
         dst[t] += src[t];
+}
+
+// Segment blockIdx.y; two 16-byte vectors per thread and pass, loaded together.
+__global__ __launch_bounds__(256) void k_copy_segs(CopySegs a) {
+    const int s = blockIdx.y;
+    const long long stride = 2LL * gridDim.x * blockDim.x;
+    double* slot = a.slots ? a.slots[*a.base + a.t] : nullptr;
+    const double2* src = (const double2*)a.src[s];
+    double2* dst = (double2*)(slot ? slot + a.doff[s] : a.dst[s]);
+    const long long n2 = a.n[s] >> 1;
+    for (long long t = blockIdx.x * 2LL * blockDim.x + threadIdx.x; t < n2; t += stride) {
+        const long long t1 = t + blockDim.x;
+        const double2 v0 = src[t];
+        const double2 v1 = src[t1 < n2 ? t1 : t];
+        dst[t] = v0;
+        if (t1 < n2) dst[t1] = v1;
+    }
 }
 
 // ------------------------------------------------------------------ k_adj_vort
@@ -284,9 +328,45 @@ __device__ __forceinline__ double jacT_lds(const double* mu, int Wm, int rm, int
 //   lq_n += jfac J_1^T(mu; p_n),  lp_n += -jfac J_1^T(mu; q_n)   (J(q,p) = -J(p,q))
 //   lp_{n-1} = c2 (L^T)^2 mu - c4 (L^T)^3 mu - [k=kt] cdrag L^T mu
 //   lwekp += cwek mu_0,  lq_{n-1} = lq_{n+1} on the interior, 0 on the ring.
+// k_adj_vort tiles: 64 x 8 points of one layer per 256-thread block (two rows
+// per thread), 33 KB of LDS, so that four blocks share a CU.
+constexpr int ADJ_TX = 64, ADJ_TY = 8;
+
+// Away from the boundary (2 <= i <= nxp-3, 2 <= j <= nyp-3, "deep" points)
+// every stencil point of the gather forms above is interior, and the
+// transposes reduce to forward stencils: L^T = L (the 5-point Laplacian is
+// symmetric) and J_1(h)^T = -J_1(h) (Arakawa's Jacobian J(., h) is
+// antisymmetric as an operator, which is its energy/enstrophy conservation).
+__device__ __forceinline__ bool deep_point(int gi, int gj, const Geo& g) {
+    return gi >= 2 && gi <= g.nxp - 3 && gj >= 2 && gj <= g.nyp - 3;
+}
+
+// 5-point Laplacian (interior coefficients of del2_P_bc) at (r, c) of an LDS tile of width W.
+__device__ __forceinline__ double lap5_lds(const double* s, int W, int r, int c, const Geo& g) {
+    const double* x = s + r * W + c;
+    return g.fx * (x[-1] + x[1]) + g.fy * (x[-W] + x[W]) - 2.0 * (g.fx + g.fy) * x[0];
+}
+
+// Arakawa sum J(f, h) before the factor jfac (J++ + J+x + Jx+, each times 3),
+// f and h LDS tiles of widths Wf, Wh with the point at (rf, cf) and (rh, ch).
+__device__ __forceinline__ double arakawa_lds(const double* f, int Wf, int rf, int cf, const double* h, int Wh,
+                                              int rh, int ch) {
+    const double* F = f + rf * Wf + cf;
+    const double* H = h + rh * Wh + ch;
+    const double fE = F[1], fW = F[-1], fN = F[Wf], fS = F[-Wf];
+    const double fNE = F[Wf + 1], fNW = F[Wf - 1], fSE = F[-Wf + 1], fSW = F[-Wf - 1];
+    const double hE = H[1], hW = H[-1], hN = H[Wh], hS = H[-Wh];
+    const double hNE = H[Wh + 1], hNW = H[Wh - 1], hSE = H[-Wh + 1], hSW = H[-Wh - 1];
+    const double jpp = (fE - fW) * (hN - hS) - (fN - fS) * (hE - hW);
+    const double jpx = fE * (hNE - hSE) - fW * (hNW - hSW) - fN * (hNE - hNW) + fS * (hSE - hSW);
+    const double jxp = fNE * (hN - hE) - fSW * (hW - hS) - fNW * (hN - hW) + fSE * (hE - hS);
+    return jpp + jpx + jxp;
+}
+
 __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
-    constexpr int TX = VORT_TX, TY = VORT_TY;
+    constexpr int TX = ADJ_TX, TY = ADJ_TY;
     constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
+    constexpr int NMU = (H3 * W3 + 255) / 256, NQP = (H2 * W2 + 255) / 256, NOUT = TY / 4;
     __shared__ double s_mu[H3 * W3];
     __shared__ double s_n1[H2 * W2];
     __shared__ double s_n2[H1 * W1];
@@ -298,57 +378,105 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
     const long long lo = (long long)k * g.fs;
     const double* ln = a.lq_np1 + lo;
-    auto mu_at = [&](int gi, int gj) -> double {   // tdt * lq_{n+1} on the interior only
-        if (gi < 1 || gi > g.nxp - 2 || gj < 1 || gj > g.nyp - 2) return 0.0;
-        return a.tdt * ln[(long long)gj * g.pitch + gi];
-    };
-    for (int idx = tid; idx < H3 * W3; idx += 256) {
-        const int r = idx / W3, c = idx - r * W3;
-        s_mu[idx] = mu_at(i0 - 3 + c, j0 - 3 + r);
+    const double* qn = a.q + lo;
+    const double* pn = a.p + lo;
+    if (a.slots) {
+        const double* sb = a.slots[*a.base + a.t];
+        qn = sb + a.qoff + lo;
+        pn = sb + a.poff + lo;
     }
-    for (int idx = tid; idx < H2 * W2; idx += 256) {
-        const int r = idx / W2, c = idx - r * W2;
+    // every global load of the block is issued before any is used: the tile
+    // of mu = tdt lq_{n+1} (interior only) with halo 3, q_n and p_n with halo
+    // 2 (from clamped, always valid offsets), and the outputs updated in place
+    double vmu[NMU], vq[NQP], vp[NQP];
+#pragma unroll
+    for (int u = 0; u < NMU; u++) {
+        const int idx = tid + 256 * u, r = idx / W3, c = idx - r * W3;
+        const int gi = i0 - 3 + c, gj = j0 - 3 + r;
+        const bool ok = idx < H3 * W3 && gi >= 1 && gi <= g.nxp - 2 && gj >= 1 && gj <= g.nyp - 2;
+        const double v = ln[ok ? (long long)gj * g.pitch + gi : 0];
+        vmu[u] = ok ? a.tdt * v : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NQP; u++) {
+        const int idx = tid + 256 * u, r = idx / W2, c = idx - r * W2;
         const int gi = i0 - 2 + c, gj = j0 - 2 + r;
-        const bool in = gi >= 0 && gi < g.nxp && gj >= 0 && gj < g.nyp;
-        const long long o = lo + (long long)gj * g.pitch + gi;
-        s_q[idx] = in ? a.q[o] : 0.0;
-        s_p[idx] = in ? a.p[o] : 0.0;
+        const bool ok = idx < H2 * W2 && gi >= 0 && gi < g.nxp && gj >= 0 && gj < g.nyp;
+        const long long o = ok ? (long long)gj * g.pitch + gi : 0;
+        const double q = qn[o], p = pn[o];
+        vq[u] = ok ? q : 0.0;
+        vp[u] = ok ? p : 0.0;
+    }
+    const int gi = i0 + tx;
+    double olq[NOUT], olp[NOUT], olw[NOUT];
+#pragma unroll
+    for (int u = 0; u < NOUT; u++) {
+        const int gj = j0 + ty + 4 * u;
+        const bool ok = gi < g.nxp && gj < g.nyp;
+        const long long o = ok ? (long long)gj * g.pitch + gi : 0;
+        olq[u] = a.lq_n[lo + o];
+        olp[u] = a.lp_n[lo + o];
+        olw[u] = a.lwek[o];
+    }
+#pragma unroll
+    for (int u = 0; u < NMU; u++) {
+        const int idx = tid + 256 * u;
+        if (idx < H3 * W3) s_mu[idx] = vmu[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NQP; u++) {
+        const int idx = tid + 256 * u;
+        if (idx < H2 * W2) {
+            s_q[idx] = vq[u];
+            s_p[idx] = vp[u];
+        }
     }
     __syncthreads();
     // nu1 = L^T mu on the tile with halo 2
     for (int idx = tid; idx < H2 * W2; idx += 256) {
         const int r = idx / W2, c = idx - r * W2;
-        const int gi = i0 - 2 + c, gj = j0 - 2 + r;
-        const bool in = gi >= 0 && gi < g.nxp && gj >= 0 && gj < g.nyp;
-        s_n1[idx] = in ? lapT_lds(s_mu, W3, r + 1, c + 1, gi, gj, g) : 0.0;
+        const int gi2 = i0 - 2 + c, gj2 = j0 - 2 + r;
+        const bool in = gi2 >= 0 && gi2 < g.nxp && gj2 >= 0 && gj2 < g.nyp;
+        double v = 0.0;
+        if (in) v = deep_point(gi2, gj2, g) ? lap5_lds(s_mu, W3, r + 1, c + 1, g) : lapT_lds(s_mu, W3, r + 1, c + 1, gi2, gj2, g);
+        s_n1[idx] = v;
     }
     __syncthreads();
     // nu2 = L^T nu1 on the tile with halo 1
     for (int idx = tid; idx < H1 * W1; idx += 256) {
         const int r = idx / W1, c = idx - r * W1;
-        const int gi = i0 - 1 + c, gj = j0 - 1 + r;
-        const bool in = gi >= 0 && gi < g.nxp && gj >= 0 && gj < g.nyp;
-        s_n2[idx] = in ? lapT_lds(s_n1, W2, r + 1, c + 1, gi, gj, g) : 0.0;
+        const int gi1 = i0 - 1 + c, gj1 = j0 - 1 + r;
+        const bool in = gi1 >= 0 && gi1 < g.nxp && gj1 >= 0 && gj1 < g.nyp;
+        double v = 0.0;
+        if (in) v = deep_point(gi1, gj1, g) ? lap5_lds(s_n1, W2, r + 1, c + 1, g) : lapT_lds(s_n1, W2, r + 1, c + 1, gi1, gj1, g);
+        s_n2[idx] = v;
     }
     __syncthreads();
     const double c2 = a.ph.c2[k], c4 = a.ph.c4[k];
-    const int gi = i0 + tx;
-    for (int u = 0; u < TY / 4; u++) {
+#pragma unroll
+    for (int u = 0; u < NOUT; u++) {
         const int r = ty + 4 * u, gj = j0 + r;
         if (gi >= g.nxp || gj >= g.nyp) continue;
         const long long o = lo + (long long)gj * g.pitch + gi;
-        const double nu3 = lapT_lds(s_n2, W1, r + 1, tx + 1, gi, gj, g);
+        double nu3, jq, jp;
+        if (deep_point(gi, gj, g)) {
+            nu3 = lap5_lds(s_n2, W1, r + 1, tx + 1, g);
+            jq = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_p, W2, r + 2, tx + 2);
+            jp = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_q, W2, r + 2, tx + 2);
+        } else {
+            nu3 = lapT_lds(s_n2, W1, r + 1, tx + 1, gi, gj, g);
+            jq = jacT_lds(s_mu, W3, r + 3, tx + 3, s_p, W2, r + 2, tx + 2, gi, gj, g);
+            jp = jacT_lds(s_mu, W3, r + 3, tx + 3, s_q, W2, r + 2, tx + 2, gi, gj, g);
+        }
         const double nu2 = s_n2[(r + 1) * W1 + tx + 1];
         const double nu1 = s_n1[(r + 2) * W2 + tx + 2];
         double lpm = c2 * nu2 - c4 * nu3;
         if (k == g.kt) lpm -= a.ph.cdrag * nu1;
-        const double jq = jacT_lds(s_mu, W3, r + 3, tx + 3, s_p, W2, r + 2, tx + 2, gi, gj, g);
-        const double jp = jacT_lds(s_mu, W3, r + 3, tx + 3, s_q, W2, r + 2, tx + 2, gi, gj, g);
-        a.lq_n[o] += g.jfac * jq;
-        a.lp_n[o] += -(g.jfac * jp);
+        a.lq_n[o] = olq[u] + g.jfac * jq;
+        a.lp_n[o] = olp[u] - g.jfac * jp;
         a.lp_nm1[o] = lpm;
         const double mu = s_mu[(r + 3) * W3 + tx + 3];
-        if (k == 0) a.lwek[(long long)gj * g.pitch + gi] += a.ph.cwek * mu;
+        if (k == 0) a.lwek[(long long)gj * g.pitch + gi] = olw[u] + a.ph.cwek * mu;
         // lq_{n-1}: the interior of lq_{n+1} (unchanged), 0 on the ring
         if (gi == 0 || gi == g.nxp - 1 || gj == 0 || gj == g.nyp - 1) a.lq_np1[o] = 0.0;
     }
@@ -365,30 +493,71 @@ void launch_adj_supp(const AdjSuppArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_adj_supp, dim3(blocks_for((long long)a.g.nl * a.g.fs)), dim3(256), 0, st, a);
 }
 
-void launch_adj_bdy(const AdjBdyArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_bdy, dim3(blocks_for((long long)a.g.nxp * a.g.nyp)), dim3(256), 0, st, a);
-}
-
 int adj_inv1_blocks(const Geo& g) { return blocks_for((long long)g.nxp * g.nyp) > 256 ? 256 : blocks_for((long long)g.nxp * g.nyp); }
 
-void launch_adj_inv1(const AdjInv1Args& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_inv1, dim3(a.nblocks), dim3(256), 0, st, a);
-}
+// Instantiations over the layer count 2..8.
+#define OQG_ADJ_NL_SWITCH(NLV, CALL) \
+    switch (NLV) {                   \
+        case 2: CALL(2); break;      \
+        case 3: CALL(3); break;      \
+        case 4: CALL(4); break;      \
+        case 5: CALL(5); break;      \
+        case 6: CALL(6); break;      \
+        case 7: CALL(7); break;      \
+        case 8: CALL(8); break;      \
+        default: break;              \
+    }
 
-void launch_adj_inv2(const AdjInv2Args& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_inv2, dim3(1), dim3(256), 0, st, a);
+void launch_adj_inv1(const AdjInv1Args& a, hipStream_t st) {
+#define INV1_CALL(N) hipLaunchKernelGGL(k_adj_inv1<N>, dim3(a.nblocks), dim3(256), 0, st, a)
+    OQG_ADJ_NL_SWITCH(a.g.nl, INV1_CALL)
+#undef INV1_CALL
 }
 
 void launch_adj_mix(const AdjMixArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_mix, dim3(blocks_for((long long)(a.g.nxp - 2) * (a.g.nyp - 2))), dim3(256), 0, st, a);
+    const dim3 grid(blocks_for((long long)(a.g.nxp - 2) * (a.g.nyp - 2)));
+#define MIX_CALL(N) hipLaunchKernelGGL(k_adj_mix<N>, grid, dim3(256), 0, st, a)
+    OQG_ADJ_NL_SWITCH(a.g.nl, MIX_CALL)
+#undef MIX_CALL
 }
 
 void launch_adj_add(double* dst, const double* src, long long n, hipStream_t st) {
     hipLaunchKernelGGL(k_adj_add, dim3(blocks_for(n)), dim3(256), 0, st, dst, src, n);
 }
 
+// Zero the boundary ring of nl fields.
+__global__ void k_zero_ring(double* f, Geo g) {
+    const int nx = g.nxp, ny = g.nyp, per = 2 * nx + 2 * (ny - 2);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= per * g.nl) return;
+    const int k = t / per, r = t - k * per;
+    int i, j;
+    if (r < nx) { i = r; j = 0; }
+    else if (r < 2 * nx) { i = r - nx; j = ny - 1; }
+    else if (r < 2 * nx + ny - 2) { i = 0; j = 1 + (r - 2 * nx); }
+    else { i = nx - 1; j = 1 + (r - 2 * nx - (ny - 2)); }
+    f[(long long)k * g.fs + (long long)j * g.pitch + i] = 0.0;
+}
+
+void launch_zero_ring(double* f, const Geo& g, hipStream_t st) {
+    const int n = (2 * g.nxp + 2 * (g.nyp - 2)) * g.nl;
+    hipLaunchKernelGGL(k_zero_ring, dim3((n + 255) / 256), dim3(256), 0, st, f, g);
+}
+
+__global__ void k_set_int(int* p, int v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+void launch_set_int(int* p, int v, hipStream_t st) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, p, v); }
+
+void launch_copy_segs(const CopySegs& a, hipStream_t st) {
+    long long mx = 0;
+    for (int s = 0; s < a.nseg; s++) mx = a.n[s] > mx ? a.n[s] : mx;
+    hipLaunchKernelGGL(k_copy_segs, dim3(blocks_for((mx / 2 + 1) / 2), a.nseg), dim3(256), 0, st, a);
+}
+
 void launch_adj_vort(const AdjVortArgs& a, hipStream_t st) {
-    const dim3 grid((a.g.nxp + VORT_TX - 1) / VORT_TX, (a.g.nyp + VORT_TY - 1) / VORT_TY, a.g.nl);
+    const dim3 grid((a.g.nxp + ADJ_TX - 1) / ADJ_TX, (a.g.nyp + ADJ_TY - 1) / ADJ_TY, a.g.nl);
     hipLaunchKernelGGL(k_adj_vort, grid, dim3(256), 0, st, a);
 }
 

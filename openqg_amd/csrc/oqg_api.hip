@@ -23,11 +23,15 @@ namespace {
 thread_local std::string g_create_err;
 constexpr double TWOPI_REF = 2.0 * 3.14159265358979324;   // constants.F90: TWOPI
 
+// Graph kinds: plain steps (oqg_run), steps with snapshot saves
+// (oqg_adj_record), adjoint steps (oqg_adj_sweep).
+enum { G_RUN = 0, G_RECORD = 1, G_SWEEP = 2 };
+
 struct GraphKey {
     double tdt;
-    int nsteps, supp, cur;
+    int nsteps, supp, cur, kind;
     bool operator<(const GraphKey& o) const {
-        return std::tie(tdt, nsteps, supp, cur) < std::tie(o.tdt, o.nsteps, o.supp, o.cur);
+        return std::tie(tdt, nsteps, supp, cur, kind) < std::tie(o.tdt, o.nsteps, o.supp, o.cur, o.kind);
     }
 };
 
@@ -85,11 +89,14 @@ struct oqg_ctx {
     double* adjR = nullptr;       // nl fields
     double* adjG = nullptr;       // nl fields: H_m^-1 of the interior indicator
     double* adjPart = nullptr;
-    double* adjS = nullptr;
     double* adjCinvT = nullptr;
     // state snapshots (oqg_snap_*): q[0..1], p[0..1], c[0..1] and the parity per slot
     std::vector<double*> snaps;
     std::vector<int> snap_cur;
+    // device copies for graph-replayed record/sweep stretches: the slot
+    // pointers and the first slot of the stretch being replayed
+    double** snap_ptrs_dev = nullptr;
+    int* snap_base_dev = nullptr;
 };
 
 // ------------------------------------------------------------------ error helpers
@@ -379,6 +386,37 @@ static int enqueue_raw_solve(oqg_ctx* c, int m, const double* in = nullptr, doub
     fill_inv_common(c, ia);
     ia.raw_mode = m;
     ia.out_raw = out ? out : c->raw[1];
+    launch_inv(ia, c->st);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// Raw Helmholtz solves of every mode in one pass: field m of out = H_m^-1
+// (field m of in) on the interior, 0 on the boundary.
+static int enqueue_raw_solve_all(oqg_ctx* c, const double* in, double* out) {
+    FwdArgs fa{};
+    fa.g = c->g;
+    fa.s = c->s;
+    fa.mx = c->mx;
+    fa.q = in;
+    fa.yprel = c->yprel;
+    fa.tw = c->tw;
+    fa.spec = c->spec;
+    fa.raw_mode = -1;
+    fa.raw_all = 1;
+    launch_fwd(fa, c->g.nl, c->st);
+    YArgs ya{};
+    ya.s = c->s;
+    ya.spec = c->spec;
+    ya.bcol = c->bcol;
+    ya.gam = c->gamw;
+    ya.mode0 = 0;
+    ya.nmodes = c->g.nl;
+    launch_ysolve(ya, c->variant, c->st);
+    InvArgs ia;
+    fill_inv_common(c, ia);
+    ia.raw_all = 1;
+    ia.out_raw = out;
     launch_inv(ia, c->st);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1005,35 +1043,59 @@ int oqg_suppress(oqg_ctx* c) {
     return enqueue_suppress(c);
 }
 
+}  // extern "C"
+
+// Steps per replayed graph: a whole number of suppression periods, even so
+// that the time-level parity is the same at both ends.
+static int graph_period(int supp) { return supp > 0 ? (supp % 2 == 0 ? supp : 2 * supp) : 2; }
+
+// The graph cached under `key`, captured from what `enqueue` puts on the
+// stream the first time.  Capture leaves the host's parity as it found it.
+template <class F>
+static int cached_graph(oqg_ctx* c, const GraphKey& key, F&& enqueue, hipGraphExec_t* out) {
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        const int cur0 = c->cur;
+        hipGraph_t gr;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue();
+        hipError_t ce = hipStreamEndCapture(c->st, &gr);
+        c->cur = cur0;
+        if (rc) {
+            if (ce == hipSuccess) hipGraphDestroy(gr);
+            return rc;
+        }
+        if (ce != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+        hipGraphExec_t ex;
+        hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (ie != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+        it = c->graphs.emplace(key, ex).first;
+    }
+    *out = it->second;
+    return 0;
+}
+
+extern "C" {
+
 int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
     if (!c || nsteps < 0) return OQG_ERR_ARG;
-    const int P = supp > 0 ? (supp % 2 == 0 ? supp : 2 * supp) : 2;
+    const int P = graph_period(supp);
     int nt = step0 + 1;
     const int end = step0 + nsteps;
     while (nt <= end) {
         if ((nt - 1) % P == 0 && end - nt + 1 >= P) {
-            GraphKey key{tdt, P, supp, c->cur};
-            auto it = c->graphs.find(key);
-            if (it == c->graphs.end()) {
-                const int cur0 = c->cur;
-                hipGraph_t gr;
-                HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-                int rc = 0;
-                for (int t = 0; t < P && !rc; t++) {
-                    rc = enqueue_step(c, tdt);
-                    if (!rc && supp > 0 && (t % supp) == 0) rc = enqueue_suppress(c);
+            hipGraphExec_t ex;
+            const int rc = cached_graph(c, GraphKey{tdt, P, supp, c->cur, G_RUN}, [&]() {
+                for (int t = 0; t < P; t++) {
+                    if (int r = enqueue_step(c, tdt)) return r;
+                    if (supp > 0 && (t % supp) == 0)
+                        if (int r = enqueue_suppress(c)) return r;
                 }
-                hipError_t ce = hipStreamEndCapture(c->st, &gr);
-                if (rc) return rc;
-                if (ce != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-                hipGraphExec_t ex;
-                hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
-                hipGraphDestroy(gr);
-                if (ie != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
-                c->cur = cur0;   // capture advanced the parity; the launch below does it for real
-                it = c->graphs.emplace(key, ex).first;
-            }
-            HIPCHK(c, hipGraphLaunch(it->second, c->st));
+                return 0;
+            }, &ex);
+            if (rc) return rc;
+            HIPCHK(c, hipGraphLaunch(ex, c->st));
             // P is even, so the time-level parity is unchanged
             nt += P;
         } else {
@@ -1205,50 +1267,49 @@ static int enqueue_invert(oqg_ctx* c) {
 // lq (interior) += M^T lp, where p = M(q) is the inversion (mix, Helmholtz
 // solves, homogeneous correction, mix), with lp already holding the adjoint of
 // p including what the boundary PV passed on.
+// The boundary ring of q_new is written from p_new (ocqbdy), so its adjoint
+// in lq's ring is passed on to lp first (inside k_adj_inv1; lp itself is left
+// as it is).
 static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp) {
-    const int nl = c->g.nl;
     AdjInv1Args i1{};
     i1.g = c->g;
+    i1.ph = c->ph;
     i1.mx = c->mx;
+    i1.lq = lq;
     i1.lp = lp;
     i1.hom = c->hom;
     i1.w = c->adjW;
     i1.partial = c->adjPart;
     i1.nblocks = adj_inv1_blocks(c->g);
     launch_adj_inv1(i1, c->st);
-    for (int m = 0; m < nl; m++)
-        if (int r = enqueue_raw_solve(c, m, c->adjW + (size_t)m * c->g.fs, c->adjR + (size_t)m * c->g.fs)) return r;
-    AdjInv2Args i2{};
-    i2.g = c->g;
-    i2.mx = c->mx;
-    i2.cinvT = c->adjCinvT;
-    i2.partial = c->adjPart;
-    i2.nblocks = i1.nblocks;
-    i2.s = c->adjS;
-    launch_adj_inv2(i2, c->st);
+    if (int r = enqueue_raw_solve_all(c, c->adjW, c->adjR)) return r;
     AdjMixArgs mx{};
     mx.g = c->g;
     mx.mx = c->mx;
     mx.R = c->adjR;
     mx.G = c->adjG;
-    mx.s = c->adjS;
+    mx.cinvT = c->adjCinvT;
+    mx.partial = c->adjPart;
+    mx.nblocks = i1.nblocks;
     mx.lq = lq;
     launch_adj_mix(mx, c->st);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
 
-static void adj_bdy(oqg_ctx* c, const double* lq, double* lp) {
-    AdjBdyArgs b{};
-    b.g = c->g;
-    b.ph = c->ph;
-    b.mx = c->mx;
-    b.lq = lq;
-    b.lp = lp;
-    launch_adj_bdy(b, c->st);
-}
+// Where an adjoint step reads q_n and p_n of the forward state before the
+// step: the loaded state's current time level (all null), given pointers, or
+// a snapshot slot found on the device (slots[*base + t], at qoff and poff).
+struct SnapRef {
+    const double* q = nullptr;
+    const double* p = nullptr;
+    const double* const* slots = nullptr;
+    const int* base = nullptr;
+    int t = 0;
+    long long qoff = 0, poff = 0;
+};
 
-static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed) {
+static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed, const SnapRef& src = SnapRef{}) {
     const int cur = c->cur, cn = 1 - cur;
     if (suppressed) {
         AdjSuppArgs s{};
@@ -1259,13 +1320,17 @@ static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed) {
         s.lpm = c->lp[cur];
         launch_adj_supp(s, c->st);
     }
-    adj_bdy(c, c->lq[cn], c->lp[cn]);
     if (int r = enqueue_adj_inversion(c, c->lq[cn], c->lp[cn])) return r;
     AdjVortArgs v{};
     v.g = c->g;
     v.ph = c->ph;
-    v.q = c->qbuf[cur];
-    v.p = c->pbuf[cur];
+    v.q = src.q ? src.q : c->qbuf[cur];
+    v.p = src.p ? src.p : c->pbuf[cur];
+    v.slots = src.slots;
+    v.base = src.base;
+    v.t = src.t;
+    v.qoff = src.qoff;
+    v.poff = src.poff;
     v.lq_np1 = c->lq[cn];
     v.lq_n = c->lq[cur];
     v.lp_n = c->lp[cur];
@@ -1294,7 +1359,6 @@ int oqg_adj_init(oqg_ctx* c) {
     if (!rc) rc = dalloc(c, &c->adjR, fl);
     if (!rc) rc = dalloc(c, &c->adjG, fl);
     if (!rc) rc = dalloc(c, &c->adjPart, (size_t)256 * MAXL);
-    if (!rc) rc = dalloc(c, &c->adjS, MAXL);
     if (!rc) rc = dalloc(c, &c->adjCinvT, MAXL * MAXL);
     if (rc) return rc;
     // (cdhoc^-1)^T by Gauss-Jordan in long double
@@ -1396,20 +1460,9 @@ int oqg_adj_invert(oqg_ctx* c) {
     const long long n = (long long)c->g.nl * c->g.fs;
     launch_adj_add(c->lq[cur], c->lq[1 - cur], n, c->st);
     launch_adj_add(c->lp[cur], c->lp[1 - cur], n, c->st);
-    adj_bdy(c, c->lq[cur], c->lp[cur]);
     if (int r = enqueue_adj_inversion(c, c->lq[cur], c->lp[cur])) return r;
     // only the interior of q is an input of the inversion
-    std::vector<double> zero((size_t)std::max(c->g.pitch, c->g.nyp), 0.0);
-    for (int k = 0; k < c->g.nl; k++) {
-        double* base = c->lq[cur] + (size_t)k * c->g.fs;
-        HIPCHK(c, hipMemcpyAsync(base, zero.data(), c->g.nxp * sizeof(double), hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpyAsync(base + (size_t)(c->g.nyp - 1) * c->g.pitch, zero.data(), c->g.nxp * sizeof(double),
-                                 hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpy2DAsync(base, c->g.pitch * sizeof(double), zero.data(), sizeof(double), sizeof(double),
-                                   c->g.nyp, hipMemcpyHostToDevice, c->st));
-        HIPCHK(c, hipMemcpy2DAsync(base + c->g.nxp - 1, c->g.pitch * sizeof(double), zero.data(), sizeof(double),
-                                   sizeof(double), c->g.nyp, hipMemcpyHostToDevice, c->st));
-    }
+    launch_zero_ring(c->lq[cur], c->g, c->st);
     HIPCHK(c, hipMemsetAsync(c->lq[1 - cur], 0, c->fbytes, c->st));
     HIPCHK(c, hipMemsetAsync(c->lp[cur], 0, c->fbytes, c->st));
     HIPCHK(c, hipMemsetAsync(c->lp[1 - cur], 0, c->fbytes, c->st));
@@ -1417,46 +1470,199 @@ int oqg_adj_invert(oqg_ctx* c) {
     return 0;
 }
 
+}  // extern "C"
+
+// The slot pointers on the device, for graph-replayed record and sweep
+// stretches.  A new array each time slots are added (the old one stays
+// allocated until oqg_destroy); the graphs captured with the old one go.
+static int snap_sync_dev(oqg_ctx* c) {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double** p = nullptr;
+    if (int r = dalloc(c, &p, c->snaps.size())) return r;
+    HIPCHK(c, hipMemcpyAsync(p, c->snaps.data(), c->snaps.size() * sizeof(double*), hipMemcpyHostToDevice, c->st));
+    if (!c->snap_base_dev)
+        if (int r = dalloc(c, &c->snap_base_dev, 1)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->snap_ptrs_dev = p;
+    for (auto it = c->graphs.begin(); it != c->graphs.end();) {
+        if (it->first.kind != G_RUN) {
+            hipGraphExecDestroy(it->second);
+            it = c->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return 0;
+}
+
+extern "C" {
+
 int oqg_snap_alloc(oqg_ctx* c, int nslots) {
     if (!c || nslots < 0) return OQG_ERR_ARG;
     const size_t words = 4 * (size_t)c->g.nl * c->g.fs + 2 * C_LEN;
+    const size_t had = c->snaps.size();
     while ((int)c->snaps.size() < nslots) {
         double* p = nullptr;
         if (int r = dalloc(c, &p, words)) return r;
         c->snaps.push_back(p);
         c->snap_cur.push_back(0);
     }
+    if (c->snaps.size() != had)
+        if (int r = snap_sync_dev(c)) return r;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
 
-int oqg_snap_save(oqg_ctx* c, int slot) {
-    if (!c || slot < 0 || slot >= (int)c->snaps.size()) return c ? fail(c, OQG_ERR_ARG, "snapshot slot out of range") : OQG_ERR_ARG;
+}  // extern "C"
+
+// Snapshot slot layout: q[0], q[1], p[0], p[1] (nl fields each), c[0], c[1].
+static void snap_copy(oqg_ctx* c, int slot, bool save) {
     double* s = c->snaps[slot];
-    const size_t fl = (size_t)c->g.nl * c->g.fs;
-    const double* src[6] = {c->qbuf[0], c->qbuf[1], c->pbuf[0], c->pbuf[1], c->cbuf[0], c->cbuf[1]};
-    size_t off = 0;
+    const long long fl = (long long)c->g.nl * c->g.fs;
+    double* state[6] = {c->qbuf[0], c->qbuf[1], c->pbuf[0], c->pbuf[1], c->cbuf[0], c->cbuf[1]};
+    CopySegs a{};
+    a.nseg = 6;
+    long long off = 0;
     for (int t = 0; t < 6; t++) {
-        const size_t n = t < 4 ? fl : (size_t)C_LEN;
-        HIPCHK(c, hipMemcpyAsync(s + off, src[t], n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        const long long n = t < 4 ? fl : (long long)C_LEN;
+        a.src[t] = save ? state[t] : s + off;
+        a.dst[t] = save ? s + off : state[t];
+        a.n[t] = n;
         off += n;
     }
-    c->snap_cur[slot] = c->cur;
+    launch_copy_segs(a, c->st);
+    if (save) c->snap_cur[slot] = c->cur;
+    else c->cur = c->snap_cur[slot];
+}
+
+// Inside a captured stretch: save the state in slot (*snap_base_dev + t).
+static void snap_save_dev(oqg_ctx* c, int t) {
+    const long long fl = (long long)c->g.nl * c->g.fs;
+    const double* state[6] = {c->qbuf[0], c->qbuf[1], c->pbuf[0], c->pbuf[1], c->cbuf[0], c->cbuf[1]};
+    CopySegs a{};
+    a.nseg = 6;
+    long long off = 0;
+    for (int s = 0; s < 6; s++) {
+        const long long n = s < 4 ? fl : (long long)C_LEN;
+        a.src[s] = state[s];
+        a.doff[s] = off;
+        a.n[s] = n;
+        off += n;
+    }
+    a.slots = c->snap_ptrs_dev;
+    a.base = c->snap_base_dev;
+    a.t = t;
+    launch_copy_segs(a, c->st);
+}
+
+static bool suppressed_at(int nt, int supp) { return supp > 0 && (nt - 1) % supp == 0; }
+
+extern "C" {
+
+int oqg_snap_save(oqg_ctx* c, int slot) {
+    if (!c || slot < 0 || slot >= (int)c->snaps.size()) return c ? fail(c, OQG_ERR_ARG, "snapshot slot out of range") : OQG_ERR_ARG;
+    snap_copy(c, slot, true);
+    HIPCHK(c, hipGetLastError());
     return 0;
 }
 
 int oqg_snap_load(oqg_ctx* c, int slot) {
     if (!c || slot < 0 || slot >= (int)c->snaps.size()) return c ? fail(c, OQG_ERR_ARG, "snapshot slot out of range") : OQG_ERR_ARG;
-    const double* s = c->snaps[slot];
-    const size_t fl = (size_t)c->g.nl * c->g.fs;
-    double* dst[6] = {c->qbuf[0], c->qbuf[1], c->pbuf[0], c->pbuf[1], c->cbuf[0], c->cbuf[1]};
-    size_t off = 0;
-    for (int t = 0; t < 6; t++) {
-        const size_t n = t < 4 ? fl : (size_t)C_LEN;
-        HIPCHK(c, hipMemcpyAsync(dst[t], s + off, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-        off += n;
+    snap_copy(c, slot, false);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int oqg_adj_record(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, int slot0) {
+    if (!c || nsteps < 0 || slot0 < 0) return OQG_ERR_ARG;
+    if ((long long)slot0 + nsteps > (long long)c->snaps.size())
+        return fail(c, OQG_ERR_ARG, "oqg_adj_record: fewer snapshot slots than steps");
+    // aligned stretches of P steps replay one graph (as oqg_run), the rest is eager
+    const int P = graph_period(supp);
+    int nt = step0 + 1;
+    const int end = step0 + nsteps;
+    while (nt <= end) {
+        const int slot = slot0 + (nt - 1 - step0);
+        if ((nt - 1) % P == 0 && end - nt + 1 >= P) {
+            const int cur0 = c->cur;
+            hipGraphExec_t ex;
+            const int rc = cached_graph(c, GraphKey{tdt, P, supp, cur0, G_RECORD}, [&]() {
+                for (int t = 0; t < P; t++) {
+                    snap_save_dev(c, t);
+                    if (int r = enqueue_step(c, tdt)) return r;
+                    if (suppressed_at(t + 1, supp))
+                        if (int r = enqueue_suppress(c)) return r;
+                }
+                return 0;
+            }, &ex);
+            if (rc) return rc;
+            launch_set_int(c->snap_base_dev, slot, c->st);
+            HIPCHK(c, hipGraphLaunch(ex, c->st));
+            for (int t = 0; t < P; t++) c->snap_cur[slot + t] = cur0 ^ (t & 1);
+            nt += P;   // P even: the parity is back to cur0
+        } else {
+            snap_copy(c, slot, true);
+            if (int r = enqueue_step(c, tdt)) return r;
+            if (suppressed_at(nt, supp))
+                if (int r = enqueue_suppress(c)) return r;
+            nt++;
+        }
     }
-    c->cur = c->snap_cur[slot];
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int oqg_adj_sweep(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, int slot0) {
+    if (!c || nsteps < 0 || slot0 < 0) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    if ((long long)slot0 + nsteps > (long long)c->snaps.size())
+        return fail(c, OQG_ERR_ARG, "oqg_adj_sweep: fewer snapshot slots than steps");
+    const long long fl = (long long)c->g.nl * c->g.fs;
+    // the adjoint step reads only q_n and p_n, straight from the snapshot;
+    // aligned stretches of P steps (steps b+1 .. b+P, b a multiple of P)
+    // replay one graph, the rest is eager
+    const int P = graph_period(supp);
+    int nt = step0 + nsteps;   // the next step to take back
+    while (nt > step0) {
+        const int b = nt - P;
+        if (b >= step0 && b % P == 0) {
+            const int base = slot0 + (b - step0), cb = c->snap_cur[base];
+            bool alternating = true;
+            for (int t = 0; t < P; t++) alternating = alternating && c->snap_cur[base + t] == (cb ^ (t & 1));
+            if (alternating) {
+                hipGraphExec_t ex;
+                const int rc = cached_graph(c, GraphKey{tdt, P, supp, cb, G_SWEEP}, [&]() {
+                    for (int t = P - 1; t >= 0; t--) {
+                        c->cur = cb ^ (t & 1);
+                        SnapRef src;
+                        src.slots = c->snap_ptrs_dev;
+                        src.base = c->snap_base_dev;
+                        src.t = t;
+                        src.qoff = c->cur ? fl : 0;
+                        src.poff = 2 * fl + (c->cur ? fl : 0);
+                        if (int r = enqueue_adj_step(c, tdt, suppressed_at(t + 1, supp), src)) return r;
+                    }
+                    return 0;
+                }, &ex);
+                if (rc) return rc;
+                launch_set_int(c->snap_base_dev, base, c->st);
+                HIPCHK(c, hipGraphLaunch(ex, c->st));
+                c->cur = cb;
+                nt = b;
+                continue;
+            }
+        }
+        const int slot = slot0 + (nt - 1 - step0), sc = c->snap_cur[slot];
+        const double* s = c->snaps[slot];
+        c->cur = sc;
+        SnapRef src;
+        src.q = s + (sc ? fl : 0);
+        src.p = s + 2 * fl + (sc ? fl : 0);
+        if (int r = enqueue_adj_step(c, tdt, suppressed_at(nt, supp), src)) return r;
+        nt--;
+    }
+    if (nsteps > 0) snap_copy(c, slot0, false);
+    HIPCHK(c, hipGetLastError());
     return 0;
 }
 

@@ -148,14 +148,23 @@ __device__ __forceinline__ double wave_sum(double x) {
 // Deterministic block sum of NV values per thread: wave_sum inside each
 // wavefront, then the wavefront sums in wavefront order.  Every thread
 // returns the same value.  red holds NV*16 doubles; blockDim.x <= 1024.
+// In two halves: block_sum_post (the wavefront sums into red, no barrier) and
+// block_sum_finish (barrier, then the sum of the slots), so that a kernel can
+// release the per-thread values before other work and finish later.
 template <int NV>
-__device__ void block_sum(double (&v)[NV], double* red) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+__device__ __forceinline__ void block_sum_post(const double (&v)[NV], double* red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double s[NV];
 #pragma unroll
-    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+    for (int i = 0; i < NV; i++) s[i] = wave_sum(v[i]);
     if (lane == 0)
 #pragma unroll
-        for (int i = 0; i < NV; i++) red[i * 16 + w] = v[i];
+        for (int i = 0; i < NV; i++) red[i * 16 + w] = s[i];
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum_finish(double (&v)[NV], double* red) {
+    const int nw = blockDim.x >> 6;
     __syncthreads();
     // all 16 slots are read at once (the ones of absent wavefronts are
     // replaced by exact zeros), then summed in wavefront order
@@ -173,6 +182,12 @@ __device__ void block_sum(double (&v)[NV], double* red) {
         v[i] = s;
     }
     __syncthreads();
+}
+
+template <int NV>
+__device__ void block_sum(double (&v)[NV], double* red) {
+    block_sum_post<NV>(v, red);
+    block_sum_finish<NV>(v, red);
 }
 
 // ---------------------------------------------------------------- FFT in LDS

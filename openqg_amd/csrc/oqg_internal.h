@@ -124,6 +124,7 @@ struct FwdArgs {
     const double* tw;     // twiddles exp(-2 pi i t / L), interleaved
     double* spec;
     int raw_mode;         // >=0: raw single field solved for this mode (0-based)
+    int raw_all;          // 1: raw field m (at q + m*fs) solved for mode m, every mode
 };
 
 struct YArgs {
@@ -161,8 +162,9 @@ struct InvArgs {
     const double* ddyn;
     double* pnew;           // p_new (pm slot)
     double* qnew;           // q_new (boundary ring written)
-    double* out_raw;        // raw mode output field
+    double* out_raw;        // raw mode output field (raw_all: field m at out_raw + m*fs)
     int raw_mode;           // >=0 raw mode
+    int raw_all;            // 1: every mode raw
     double tdt;
 };
 
@@ -206,40 +208,27 @@ struct AdjSuppArgs {
     double *lq, *lqm, *lp, *lpm;
 };
 
-struct AdjBdyArgs {
+struct AdjInv1Args {
     Geo g;
     Phys ph;
     Mix mx;
-    const double* lq;   // adjoint of the q whose boundary ring ocqbdy wrote
-    double* lp;         // adjoint of the p it was computed from (+=)
-};
-
-struct AdjInv1Args {
-    Geo g;
-    Mix mx;
-    const double* lp;   // adjoint of p_new
+    const double* lq;   // adjoint of q_new, whose boundary ring ocqbdy wrote from p_new
+    const double* lp;   // adjoint of p_new, before what the boundary ring passes on
     const double* hom;  // box homogeneous solutions, (nl-1) fields
     double* w;          // out: adjoint of each modal field, nl fields
     double* partial;    // out: per-block sums of hom_m . w_m, [block][MAXL]
     int nblocks;
 };
 
-struct AdjInv2Args {
-    Geo g;
-    Mix mx;
-    const double* cinvT;     // (cdhoc^-1)^T, [l*MAXL + j]
-    const double* partial;
-    int nblocks;
-    double* s;               // out: per-mode constant added to the interior of w_m
-};
-
 struct AdjMixArgs {
     Geo g;
     Mix mx;
-    const double* R;    // Helmholtz solves of w_m, nl fields
-    const double* G;    // Helmholtz solves of the interior indicator, nl fields
-    const double* s;
-    double* lq;         // adjoint of q_new (+= on the interior)
+    const double* R;         // Helmholtz solves of w_m, nl fields
+    const double* G;         // Helmholtz solves of the interior indicator, nl fields
+    const double* cinvT;     // (cdhoc^-1)^T, [l*MAXL + j]
+    const double* partial;   // k_adj_inv1's per-block sums
+    int nblocks;
+    double* lq;              // adjoint of q_new (+= on the interior)
 };
 
 struct AdjVortArgs {
@@ -253,12 +242,35 @@ struct AdjVortArgs {
     double* lp_nm1;     // out: adjoint of p_{n-1}
     double* lwek;       // adjoint of the Ekman pumping field (+=)
     double tdt;
+    // with `slots` set, q_n and p_n are read from snapshot slot
+    // slots[*base + t] at offsets qoff, poff instead of q, p
+    const double* const* slots;
+    const int* base;
+    int t;
+    long long qoff, poff;
 };
 
 void launch_adj_supp(const AdjSuppArgs& a, hipStream_t st);
-void launch_adj_bdy(const AdjBdyArgs& a, hipStream_t st);
+// Up to six contiguous segments copied in one launch (state snapshots); every
+// length is even and every pointer 16-byte aligned.  With `slots` set, the
+// destinations are offsets doff[] into snapshot slot slots[*base + t] (the
+// base slot is read from device memory, so a captured graph can be replayed
+// for any stretch of slots).
+struct CopySegs {
+    const double* src[6];
+    double* dst[6];
+    long long n[6];
+    int nseg;
+    double* const* slots;
+    const int* base;
+    int t;
+    long long doff[6];
+};
+void launch_copy_segs(const CopySegs& a, hipStream_t st);
+void launch_set_int(int* p, int v, hipStream_t st);
+void launch_zero_ring(double* f, const Geo& g, hipStream_t st);
+
 void launch_adj_inv1(const AdjInv1Args& a, hipStream_t st);
-void launch_adj_inv2(const AdjInv2Args& a, hipStream_t st);
 void launch_adj_mix(const AdjMixArgs& a, hipStream_t st);
 void launch_adj_add(double* dst, const double* src, long long n, hipStream_t st);
 void launch_adj_vort(const AdjVortArgs& a, hipStream_t st);

@@ -56,7 +56,8 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     double2* tw = zbuf + L;
     Staged<4, double2> stw;   // the whole table for L <= 1024
     stage_load(stw, a.tw, L);
-    const bool raw = a.raw_mode >= 0;
+    const bool raw = a.raw_mode >= 0 || a.raw_all;
+    const double* qin = a.q + (a.raw_all ? (long long)mb * g.fs : 0);
     // inputs of this thread's (at most four) points of both rows, loaded together
     const int first = g.cyclic ? 0 : 1;
     const int npts = s.nin;   // box: interior points 1..nin; channel: 0..nin-1
@@ -72,8 +73,8 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
 #pragma unroll
         for (int k = 0; k < NL; k++) {
-            q0[u][k] = (in && k < nlq) ? a.q[k * g.fs + o0] : 0.0;
-            q1[u][k] = (in && has1 && k < nlq) ? a.q[k * g.fs + o1] : 0.0;
+            q0[u][k] = (in && k < nlq) ? qin[k * g.fs + o0] : 0.0;
+            q1[u][k] = (in && has1 && k < nlq) ? qin[k * g.fs + o1] : 0.0;
         }
         d0[u] = (in && !raw && a.ddyn) ? a.ddyn[o0] : 0.0;
         d1[u] = (in && has1 && !raw && a.ddyn) ? a.ddyn[o1] : 0.0;
@@ -668,8 +669,8 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     const Geo& g = a.g;
     const Spec& s = a.s;
     const int L = s.L;
-    const bool raw = a.raw_mode >= 0;
-    const int nm = raw ? 1 : NL;
+    const bool raw1 = a.raw_mode >= 0, raw = raw1 || a.raw_all;
+    const int nm = raw1 ? 1 : NL;
     const int G = blockDim.x / FFT_GROUP;          // transforms in flight
     const int grp = threadIdx.x / FFT_GROUP, gt = threadIdx.x - grp * FFT_GROUP;
     double2* fbuf = (double2*)smem;                 // [G][L]
@@ -722,7 +723,7 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
         for (int m = 0; m < NL; m++) csv[u][m] = okc ? a.colsum[m * s.ncol + t] : 0.0;
     }
     SpecIn sin;
-    load_spec_pair(a, raw ? a.raw_mode : grp, grp < nm, r0, gt, sin);
+    load_spec_pair(a, raw1 ? a.raw_mode : grp, grp < nm, r0, gt, sin);
     double hv[2][NL > 1 ? NL - 1 : 1];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
@@ -740,8 +741,9 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     stage_store(st_aux, s_aux, a.aux, nw_aux);
     if (want_spk) s_spk[threadIdx.x] = spk;
 
-    // this thread's share of the integrals behind the correction coefficients;
-    // the block reduction waits until after the transforms, which do not need it
+    // this thread's share of the integrals behind the correction coefficients,
+    // summed over each wavefront now (which frees the registers during the
+    // transforms); the block's sum of the wavefront sums follows them
     if (!raw) {
         v[NL] = entv[0] + entv[1];
         for (int t = threadIdx.x + 2 * blockDim.x; t < a.nentpart; t += blockDim.x) v[NL] += a.entpart[t];
@@ -754,6 +756,7 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
                 for (int m = 0; m < NL; m++) v[m] += w * a.colsum[m * s.ncol + col];
             }
         }
+        block_sum_post<NL + 1>(v, red);
     }
 
     // ---- inverse x transforms, G modes at a time (their barriers publish the LDS copies above)
@@ -761,7 +764,7 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     for (int m0 = 0; m0 < nm; m0 += G) {
         const int mi = m0 + grp;
         const bool valid = mi < nm;
-        const int m = raw ? a.raw_mode : mi;
+        const int m = raw1 ? a.raw_mode : mi;
         double2* z = fbuf + (size_t)grp * L;
         if (m0 > 0) load_spec_pair(a, m, valid, r0, gt, sin);   // later rounds (nl > 3)
         if (valid) {
@@ -820,19 +823,22 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
 
     // ---- correction coefficients (identical in every block)
     if (!raw) {
-        block_sum<NL + 1>(v, red);
+        block_sum_finish<NL + 1>(v, red);
         if (threadIdx.x == 0)
             step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, jp == 0);
         __syncthreads();
     }
 
     if (raw) {
-        double* o = a.out_raw;
-        for (int i = threadIdx.x; i < g.nxp; i += blockDim.x) {
-            o[(long long)gj0 * g.pitch + i] = inh[i];
-            if (has1) o[(long long)gj1 * g.pitch + i] = inh[g.pitch + i];
-            if (first) o[i] = 0.0;
-            if (last) o[(long long)(g.nyp - 1) * g.pitch + i] = 0.0;
+        for (int m = 0; m < nm; m++) {
+            double* o = a.out_raw + (long long)m * g.fs;
+            const double* src = inh + (size_t)m * 2 * g.pitch;
+            for (int i = threadIdx.x; i < g.nxp; i += blockDim.x) {
+                o[(long long)gj0 * g.pitch + i] = src[i];
+                if (has1) o[(long long)gj1 * g.pitch + i] = src[g.pitch + i];
+                if (first) o[i] = 0.0;
+                if (last) o[(long long)(g.nyp - 1) * g.pitch + i] = 0.0;
+            }
         }
         return;
     }

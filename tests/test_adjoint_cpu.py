@@ -32,32 +32,13 @@ def test_ekei1_gradient_matches_central_differences():
     assert abs(J - 0.5 * np.sum(g * p)) <= 1e-12 * J
 
 
-class _FakeLib:
-    def __init__(self, rec):
-        self.rec = rec
-
-    def oqg_step(self, h, tdt, *a):
-        self.rec["state"] += 1
-        return 0
-
-
-class _FakeQG:
-    def __init__(self, rec):
-        self.rec = rec
-        self._lib = _FakeLib(rec)
-        self._h = None
-
-    def suppress(self):
-        pass
-
-
 class _FakeAdj:
-    """Records the state index of every snapshot and which step each adjoint
-    step reverses (the loaded state must be the one before that step)."""
+    """Stands in for AdjointState: the 'state' is the index of the step it
+    follows.  Records every snapshot and which step each adjoint step
+    reverses (the loaded state must be the one before that step)."""
 
     def __init__(self):
         self.rec = {"state": 0}
-        self.qg = _FakeQG(self.rec)
         self.slots = {}
         self.reversed = []
         self.lam_at = None
@@ -72,11 +53,28 @@ class _FakeAdj:
     def load(self, slot):
         self.rec["state"] = self.slots[slot]
 
+    def advance(self, tdt, nsteps, step0, supp):
+        assert self.rec["state"] == step0 and nsteps > 0
+        self.rec["state"] += nsteps
+
+    def record(self, tdt, nsteps, step0, supp, slot0):
+        assert self.rec["state"] == step0
+        for i in range(nsteps):
+            self.save(slot0 + i)
+            self.rec["state"] += 1
+
     def step(self, tdt, suppressed):
         nt = self.rec["state"] + 1            # the step whose input is loaded
         assert nt == self.lam_at, (nt, self.lam_at)
+        assert suppressed == ((nt - 1) % 5 == 0)
         self.reversed.append(nt)
         self.lam_at -= 1
+
+    def sweep(self, tdt, nsteps, step0, supp, slot0):
+        for i in range(nsteps - 1, -1, -1):
+            self.load(slot0 + i)
+            self.step(tdt, (step0 + i) % supp == 0)
+        self.load(slot0)
 
 
 def _sweep(n, snapshots):
