@@ -67,31 +67,39 @@ def cpu_baseline(w, seconds: float):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    cr.set_threads(cores)
-    b = cr.Basin(w.params())
-    p = w.initial_p()
-    b.set_state(p=p, pm=p)
-    b.init_state()
     ent, wekp, txis, txin = w.forcing()
-    b.step(w.tdt, ent, wekp, txis, txin)   # first step (page-in, thread start) untimed
-    b.suppress()
-    # time whole steps until the sample reaches `seconds`
-    n = 0
-    t0 = time.perf_counter()
-    el = 0.0
-    while el < seconds and n < w.nsteps:
-        nt = n + 2
-        b.step(w.tdt, ent, wekp, txis, txin)
-        if (nt - 1) % w.supp_every == 0:
-            b.suppress()
-        n += 1
-        el = time.perf_counter() - t0
+
+    def sample(threads, secs):
+        cr.set_threads(threads)
+        b = cr.Basin(w.params())
+        p = w.initial_p()
+        b.set_state(p=p, pm=p)
+        b.init_state()
+        b.step(w.tdt, ent, wekp, txis, txin)   # first step (page-in, thread start) untimed
+        b.suppress()
+        # time whole steps until the sample reaches `secs`
+        n = 0
+        t0 = time.perf_counter()
+        el = 0.0
+        while el < secs and n < w.nsteps:
+            nt = n + 2
+            b.step(w.tdt, ent, wekp, txis, txin)
+            if (nt - 1) % w.supp_every == 0:
+                b.suppress()
+            n += 1
+            el = time.perf_counter() - t0
+        return n, el
+
+    n, el = sample(cores, seconds)
+    n1, el1 = sample(1, seconds / 2)
     return {
         "value": n * w.dt / 86400.0 / el,
         "unit": "simulated days/s",
         "cores": cores,
         "kind": "port",
         "sample": f"{n} steps of {w.name} (dt={w.dt:g} s) after init, OpenMP {cores} threads, {el:.1f} s",
+        "value_1thread": n1 * w.dt / 86400.0 / el1,
+        "sample_1thread": f"{n1} steps, 1 thread, {el1:.1f} s",
     }
 
 

@@ -67,7 +67,7 @@ __device__ __forceinline__ bool ring_inner(const Geo& g, int xi, int xj, int& ii
 // outer bands (rows 0, 1, nyp-2, nyp-1 and columns 0, 1, nxp-2, nxp-1), added
 // to lp[0..nl-1].
 template <int NL>
-__device__ __forceinline__ void bdy_gather(const Geo& g, const Phys& ph, const Mix& mx, const double* lq, int ai,
+__device__ __forceinline__ void bdy_gather(const Geo& g, double qbfac, const Mix& mx, const double* lq, int ai,
                                            int aj, double (&lp)[NL]) {
     const int nx = g.nxp, ny = g.nyp;
     const long long oa = (long long)aj * g.pitch + ai;
@@ -80,7 +80,7 @@ __device__ __forceinline__ void bdy_gather(const Geo& g, const Phys& ph, const M
 #pragma unroll
             for (int l = 0; l < NL; l++) {
                 double c = -r * g.fnot * mx.amat[k * MAXL + l];
-                if (l == k) c -= ph.qbfac;
+                if (l == k) c -= qbfac;
                 lp[l] += lqk * c;
             }
         }
@@ -94,7 +94,7 @@ __device__ __forceinline__ void bdy_gather(const Geo& g, const Phys& ph, const M
         if (ring_inner(g, xi, xj, ii, ij) && ii == ai && ij == aj) {
             const long long ox = (long long)xj * g.pitch + xi;
 #pragma unroll
-            for (int k = 0; k < NL; k++) lp[k] += ph.qbfac * lq[k * g.fs + ox];
+            for (int k = 0; k < NL; k++) lp[k] += qbfac * lq[k * g.fs + ox];
         }
     }
 }
@@ -121,12 +121,12 @@ __global__ __launch_bounds__(256) void k_adj_inv1(AdjInv1Args a) {
         for (int k = 0; k < NL; k++) lp[k] = a.lp[k * g.fs + o];
 #pragma unroll
         for (int m = 1; m < NL; m++) hm[m] = a.hom[(long long)(m - 1) * g.fs + o];
-        if (j <= 1 || j >= g.nyp - 2 || i <= 1 || i >= g.nxp - 2) bdy_gather<NL>(g, a.ph, a.mx, a.lq, i, j, lp);
+        if (j <= 1 || j >= g.nyp - 2 || i <= 1 || i >= g.nxp - 2) bdy_gather<NL>(g, a.qbfac, a.dc->mx, a.lq, i, j, lp);
 #pragma unroll
         for (int m = 0; m < NL; m++) {
             double v = 0.0;
 #pragma unroll
-            for (int k = 0; k < NL; k++) v += a.mx.ctm2l[m * MAXL + k] * lp[k];
+            for (int k = 0; k < NL; k++) v += a.dc->mx.ctm2l[m * MAXL + k] * lp[k];
             a.w[m * g.fs + o] = v;
             if (m >= 1) part[m] += hm[m] * v;
         }
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_adj_mix(AdjMixArgs a) {
             double s = 0.0;
 #pragma unroll
             for (int l = 0; l < NL - 1; l++)
-                s -= lr[l] * (a.mx.ctm2l[m * MAXL + l + 1] - a.mx.ctm2l[m * MAXL + l]);
+                s -= lr[l] * (a.dc->mx.ctm2l[m * MAXL + l + 1] - a.dc->mx.ctm2l[m * MAXL + l]);
             s_s[m] = (s * g.dx) * g.dy;
         }
     }
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void k_adj_mix(AdjMixArgs a) {
         for (int k = 0; k < NL; k++) {
             double s = 0.0;
 #pragma unroll
-            for (int m = 0; m < NL; m++) s += a.mx.ctl2m[k * MAXL + m] * r[m];
+            for (int m = 0; m < NL; m++) s += a.dc->mx.ctl2m[k * MAXL + m] * r[m];
             a.lq[k * g.fs + o] = L[k] + g.fnot * s;
         }
     }

@@ -367,7 +367,7 @@ static int enqueue_raw_solve(oqg_ctx* c, int m, const double* in = nullptr, doub
     FwdArgs fa{};
     fa.g = c->g;
     fa.s = c->s;
-    fa.mx = c->mx;
+    fa.dc = c->dconst;
     fa.q = in ? in : c->raw[0];
     fa.yprel = c->yprel;
     fa.tw = c->tw;
@@ -397,7 +397,7 @@ static int enqueue_raw_solve_all(oqg_ctx* c, const double* in, double* out) {
     FwdArgs fa{};
     fa.g = c->g;
     fa.s = c->s;
-    fa.mx = c->mx;
+    fa.dc = c->dconst;
     fa.q = in;
     fa.yprel = c->yprel;
     fa.tw = c->tw;
@@ -465,7 +465,7 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     FwdArgs& fa = a.fa;
     fa.g = c->g;
     fa.s = c->s;
-    fa.mx = c->mx;
+    fa.dc = c->dconst;
     fa.q = c->qbuf[nxt];
     fa.ddyn = c->ddyn;
     fa.yprel = c->yprel;
@@ -1180,14 +1180,29 @@ static int time_kernels(oqg_ctx* c, double tdt, int reps, hipEvent_t* ev, double
             if (k == KV) HIPCHK(c, hipMemcpyAsync(tmp[r], a.va.qnew, c->fbytes, hipMemcpyDeviceToDevice, c->st));
             if (k == KY) HIPCHK(c, hipMemcpyAsync(tmp[reps + r], a.ya.spec, sbytes, hipMemcpyDeviceToDevice, c->st));
         }
-        HIPCHK(c, hipEventRecord(ev[0], c->st));
+        // the burst is replayed from a graph, so that no host submission gap
+        // falls inside the timed region (as in oqg_run)
+        hipGraph_t gr;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         for (int r = 0; r < reps; r++) {
             StepArgs b = a;
             b.va.qnew = static_cast<double*>(tmp[r]);
             b.ya.spec = static_cast<double*>(tmp[reps + r]);
             launch_kernel(c, (k == KV || k == KY) ? b : a, k);
         }
-        HIPCHK(c, hipEventRecord(ev[1], c->st));
+        hipError_t ce = hipStreamEndCapture(c->st, &gr);
+        if (ce != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+        hipGraphExec_t ex;
+        hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        hipGraphDestroy(gr);
+        if (ie != hipSuccess) return fail(c, OQG_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+        hipError_t ue = hipGraphUpload(ex, c->st);
+        if (ue == hipSuccess) ue = hipEventRecord(ev[0], c->st);
+        if (ue == hipSuccess) ue = hipGraphLaunch(ex, c->st);
+        if (ue == hipSuccess) ue = hipEventRecord(ev[1], c->st);
+        if (ue == hipSuccess) ue = hipStreamSynchronize(c->st);
+        hipGraphExecDestroy(ex);
+        HIPCHK(c, ue);
         launch_kernel(c, a, k);
         HIPCHK(c, hipGetLastError());
         HIPCHK(c, hipEventSynchronize(ev[1]));
@@ -1273,8 +1288,8 @@ static int enqueue_invert(oqg_ctx* c) {
 static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp) {
     AdjInv1Args i1{};
     i1.g = c->g;
-    i1.ph = c->ph;
-    i1.mx = c->mx;
+    i1.qbfac = c->ph.qbfac;
+    i1.dc = c->dconst;
     i1.lq = lq;
     i1.lp = lp;
     i1.hom = c->hom;
@@ -1285,7 +1300,7 @@ static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp) {
     if (int r = enqueue_raw_solve_all(c, c->adjW, c->adjR)) return r;
     AdjMixArgs mx{};
     mx.g = c->g;
-    mx.mx = c->mx;
+    mx.dc = c->dconst;
     mx.R = c->adjR;
     mx.G = c->adjG;
     mx.cinvT = c->adjCinvT;

@@ -114,10 +114,18 @@ struct VortArgs {
 __host__ __device__ inline int vort_org_x(const Geo& g) { return g.cyclic ? 0 : 1; }
 __host__ __device__ inline int vort_lim_x(const Geo& g) { return g.cyclic ? g.nxp : g.nxp - 1; }
 
+// Device block holding the precompute read by k_fwd, k_inv and the adjoint
+// kernels (kept out of the kernel arguments, which every launch uploads).
+struct DevConsts {
+    Mix mx;
+    BoxHom bh;
+    CycHom ch;
+};
+
 struct FwdArgs {
     Geo g;
     Spec s;
-    Mix mx;
+    const DevConsts* dc;
     const double* q;      // q_new (step) or rhs field (raw)
     const double* ddyn;   // may be null
     const double* yprel;
@@ -134,13 +142,6 @@ struct YArgs {
     double* colsum;       // [m][col] or null
     double* gam;          // Thomas variant scratch, same layout as spec
     int mode0, nmodes;    // range of modes solved
-};
-
-// Device block holding the precompute read by k_inv.
-struct DevConsts {
-    Mix mx;
-    BoxHom bh;
-    CycHom ch;
 };
 
 struct InvArgs {
@@ -210,9 +211,9 @@ struct AdjSuppArgs {
 
 struct AdjInv1Args {
     Geo g;
-    Phys ph;
-    Mix mx;
-    const double* lq;   // adjoint of q_new, whose boundary ring ocqbdy wrote from p_new
+    double qbfac;
+    const DevConsts* dc;
+    const double* lq;  // adjoint of q_new, whose boundary ring ocqbdy wrote from p_new
     const double* lp;   // adjoint of p_new, before what the boundary ring passes on
     const double* hom;  // box homogeneous solutions, (nl-1) fields
     double* w;          // out: adjoint of each modal field, nl fields
@@ -222,7 +223,7 @@ struct AdjInv1Args {
 
 struct AdjMixArgs {
     Geo g;
-    Mix mx;
+    const DevConsts* dc;
     const double* R;         // Helmholtz solves of w_m, nl fields
     const double* G;         // Helmholtz solves of the interior indicator, nl fields
     const double* cinvT;     // (cdhoc^-1)^T, [l*MAXL + j]

@@ -28,16 +28,18 @@ constexpr int YS_CT = 8;   // columns per y-solver workgroup (one wavefront each
 // ------------------------------------------------------------------ k_fwd
 
 // p_modes_rhs (modes.F90:259-264) at one point from its loaded inputs:
-// f0 * sum_k (q_k - beta*y - [k=k_topo] ddyn) ctl2m(k,m).
+// f0 * sum_k (q_k - beta*y - [k=k_topo] ddyn) ctl2m(k,m), with cm the
+// column ctl2m(:,m).
 template <int NL>
-__device__ __forceinline__ double rhs_mode(const FwdArgs& a, int m, const double (&qk)[NL], double by, double dd) {
+__device__ __forceinline__ double rhs_mode(const FwdArgs& a, const double (&cm)[NL], const double (&qk)[NL], double by,
+                                           double dd) {
     const Geo& g = a.g;
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
         double ql = qk[k] - by;
         if (k == g.kt) ql = ql - dd;
-        s += ql * a.mx.ctl2m[k * MAXL + m];
+        s += ql * cm[k];
     }
     return g.fnot * s;
 }
@@ -65,6 +67,9 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     // raw mode), ddyn and beta*y of both rows
     constexpr int NQ = 4;
     const int nlq = raw ? 1 : NL;
+    double cm[NL];   // ctl2m(:, m) from the device constants
+#pragma unroll
+    for (int k = 0; k < NL; k++) cm[k] = a.dc->mx.ctl2m[k * MAXL + m];
     double q0[NQ][NL], q1[NQ][NL], d0[NQ], d1[NQ];
 #pragma unroll
     for (int u = 0; u < NQ; u++) {
@@ -87,8 +92,8 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
             xv[u] = q0[u][0];
             yv[u] = q1[u][0];
         } else {
-            xv[u] = rhs_mode<NL>(a, m, q0[u], by0, d0[u]);
-            yv[u] = has1 ? rhs_mode<NL>(a, m, q1[u], by1, d1[u]) : 0.0;
+            xv[u] = rhs_mode<NL>(a, cm, q0[u], by0, d0[u]);
+            yv[u] = has1 ? rhs_mode<NL>(a, cm, q1[u], by1, d1[u]) : 0.0;
         }
         const int t = first + threadIdx.x + 256 * u;
         if (t >= first + npts) {

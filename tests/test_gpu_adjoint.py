@@ -146,6 +146,33 @@ def test_gradient_matches_finite_differences(gpu):
         assert err <= 1e-6, (name, fd, ad)
 
 
+def test_gradient_sixteen_unit_directions(gpu):
+    """Central differences in 16 seeded unit directions (seed 5) of the joint
+    control (taux, tauy, interior q0), each block scaled to its norm, with
+    eps = 1e-6 ||x||: dJ/de at e = 0 of J(x + e ||x_b|| d_b)."""
+    w = W.small_box(33, 3)
+    nsteps, supp = 10, 4
+    qg = w.make_qg()
+    q0, tx, ty = _controls(w, qg)
+    J, gx, gy, gq, _ = gradient(qg, q0, tx, ty, w.tdt, nsteps, supp)
+    ntau = float(np.sqrt(np.sum(tx * tx) + np.sum(ty * ty)))
+    nq = float(np.sqrt(np.sum(_interior(q0) ** 2)))
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for k in range(16):
+        dx_, dy_ = rng.standard_normal(tx.shape), rng.standard_normal(ty.shape)
+        dq = _interior(rng.standard_normal(q0.shape))
+        nt = np.sqrt(np.sum(dx_ * dx_) + np.sum(dy_ * dy_))
+        dx_, dy_ = dx_ * (ntau / nt), dy_ * (ntau / nt)
+        dq = dq * (nq / np.sqrt(np.sum(dq * dq)))
+        fd = _fd(qg, q0, tx, ty, (dq, dx_, dy_), w, nsteps, supp, rel=1e-6)
+        ad = float(np.sum(gq * dq) + np.sum(gx * dx_) + np.sum(gy * dy_))
+        err = abs(fd - ad) / abs(fd)
+        worst = max(worst, err)
+        assert err <= 1e-5, (k, fd, ad)
+    print(f"[adjoint] 16 unit directions, {w.name}: worst rel err {worst:.2e}", flush=True)
+
+
 def test_checkpointed_gradient_equals_stored(gpu):
     w = W.small_box(33, 3)
     nsteps, supp = 23, 4
