@@ -17,6 +17,7 @@ recomputes the forward from the stored ones with graph-replayed oqg_run.
 from __future__ import annotations
 
 import ctypes
+from functools import lru_cache
 from math import comb
 from typing import Optional
 
@@ -174,15 +175,44 @@ class AdjointState:
 
 # ------------------------------------------------------------------ checkpointing
 
-def _revolve_split(n: int, s: int) -> int:
-    """Steps to advance before the next checkpoint when reversing n steps with
-    s free snapshot slots: with r the least repetition count for which
-    C(s + r, s) >= n, the left part holds C(s + r - 1, s) steps."""
-    r = 0
-    while comb(s + r, s) < n:
+def _beta(c: int, r: int) -> int:
+    return comb(c + r, c) if r >= 0 else 0
+
+
+def _rev_cost(n: int, s: int) -> int:
+    """Forward steps that reversing n steps costs from a stored state with s
+    further free slots (Trajectory._rev): Griewank's t(n, c) = r n - beta(c+1,
+    r-1) with c = s + 1 checkpoints and r the least repetition number with
+    beta(c, r) = C(c + r, c) >= n (Griewank and Walther, ACM TOMS 26, 2000)."""
+    if n <= 1:
+        return 0
+    c, r = s + 1, 0
+    while _beta(c, r) < n:
         r += 1
-    m = comb(s + r - 1, s) if r > 0 else 1
-    return max(1, min(n - 1, m))
+    return r * n - _beta(c + 1, r - 1)
+
+
+def _cost_vec(x: np.ndarray, c: int) -> np.ndarray:
+    """_rev_cost over an array of step counts x (>= 1) for c = s + 1 checkpoints."""
+    nmax = int(x.max())
+    betas = [1]
+    while betas[-1] < nmax:
+        betas.append(_beta(c, len(betas)))
+    r = np.searchsorted(np.array(betas, dtype=np.float64), x, side="left")
+    prev = np.array([0] + [_beta(c + 1, k) for k in range(len(betas) - 1)], dtype=np.float64)
+    return np.where(x <= 1, 0.0, r * x - prev[r])
+
+
+@lru_cache(maxsize=None)
+def _revolve_split(n: int, s: int) -> int:
+    """The advance m (1 <= m < n) before the next checkpoint when reversing n
+    steps with s >= 1 free slots: the m that minimises
+    m + cost(n - m, s - 1) + cost(m, s)."""
+    if n <= 2:
+        return 1
+    m = np.arange(1, n, dtype=np.float64)
+    total = m + _cost_vec(n - m, s) + _cost_vec(m, s + 1)
+    return int(m[int(np.argmin(total))])
 
 
 class Trajectory:
@@ -198,16 +228,27 @@ class Trajectory:
         return self.supp > 0 and (nt - 1) % self.supp == 0
 
     def forward(self):
-        """Run forward; returns after step n with the final state loaded."""
+        """Run forward; returns after step n with the final state loaded.  With
+        checkpointing, the sweep places the checkpoints of the schedule's
+        rightmost path on the way (the first advance of every level)."""
         a = self.adj
         if self.snapshots is None:
             a.alloc(max(self.n, 1))
             a.record(self.tdt, self.n, 0, self.supp, 0)
             self.forward_steps += self.n
-        else:
-            a.alloc(self.snapshots + 1)
-            a.save(0)
-            self._advance(0, self.n)
+            return
+        a.alloc(self.snapshots + 1)
+        a.save(0)
+        self._left = []
+        lo, slot, free = 0, 0, self.snapshots
+        while self.n - lo > 1 and free > 0:
+            m = lo + _revolve_split(self.n - lo, free)
+            self._advance(lo, m)
+            a.save(slot + 1)
+            self._left.append((lo, m, slot, free))
+            lo, slot, free = m, slot + 1, free - 1
+        self._advance(lo, self.n)
+        self._tail = (lo, slot, free)
 
     def reverse(self):
         """With the adjoint of the final state set, sweep back to the initial state."""
@@ -215,7 +256,10 @@ class Trajectory:
         if self.snapshots is None:
             a.sweep(self.tdt, self.n, 0, self.supp, 0)
             return
-        self._rev(0, self.n, 0, self.snapshots)
+        lo, slot, free = self._tail
+        self._rev(lo, self.n, slot, free)
+        for lo, m, slot, free in reversed(self._left):
+            self._rev(lo, m, slot, free)
         a.load(0)
 
     def _advance(self, lo: int, hi: int):

@@ -5,18 +5,19 @@
 // the adjoint of the Ekman pumping field).  Every operator is the exact
 // transpose of the forward discretisation, written out by hand:
 //   k_adj_supp  suppress_comp_mode (driver.F90:174-189)
-//   k_adj_inv1  ocqbdy (vorsubs.F90:58-114: boundary-ring PV -> p), the
-//               mode->layer mix (modes.F90:268-284) and the homogeneous
-//               correction's inner products (homog.F90:331-366)
-//   k_adj_mix   the (nl-1)^2 constraint solve, and the layer->mode mix
-//               (modes.F90:248-266) of the Helmholtz adjoint
 //   k_adj_vort  vort_step (vorticity.F90:14-103): Arakawa Jacobian transposes
 //               in gather form (the p transpose through the discrete
 //               antisymmetry J(q,p) = -J(p,q)), Del^2/Del^4/Del^6 transposes
 //               of del2_P_bc with its mixed boundary condition, forcing
 //   k_adj_add   sums of fields (state combination for the initial-state map)
-// The Helmholtz solve of the box (DST-I, tridiagonal in y, DST-I) is
-// symmetric, so its adjoint is the forward raw solve.
+// The inversion's adjoint runs through the transform kernels in their adjoint
+// modes (oqg_transform.hip): k_fwd_adj forms the modal inputs from the
+// adjoint of p (ocqbdy's transpose, vorsubs.F90:58-114, and the mode->layer
+// mix's, modes.F90:268-284, with the homogeneous correction's inner products,
+// homog.F90:331-366); the box Helmholtz solve (DST-I, tridiagonal in y,
+// DST-I) is symmetric, so the forward k_ysolve is its adjoint; k_inv_adj
+// applies the constraint solve's transpose and the layer->mode mix's
+// (modes.F90:248-266).
 // Layout and parity follow the forward state: the adjoint buffers lq[2],
 // lp[2] mirror qbuf[2], pbuf[2].
 
@@ -35,181 +36,6 @@ __global__ void k_adj_supp(AdjSuppArgs a) {
         a.lq[t] = hq;
         a.lpm[t] += hp;
         a.lp[t] = hp;
-    }
-}
-
-// ------------------------------------------------------------------ k_adj_bdy
-
-// Forward (box), at a ring point x with inner point x_in:
-//   q_k(x) = qbfac (p_k(x_in) - p_k(x)) - r f0 sum_l A_kl p_l(x) + r beta y + ...
-// with r = 2 at the corners (the row and column terms both applied) and 1
-// otherwise.  Transpose gathered at every point a of the two outer bands:
-//   lp_l(a) += sum_k lq_k(a) (-qbfac d_kl - r f0 A_kl)        (a on the ring)
-//   lp_k(a) += qbfac lq_k(x)   for each ring point x whose inner point is a.
-__device__ __forceinline__ bool ring_inner(const Geo& g, int xi, int xj, int& ii, int& ij) {
-    const int nx = g.nxp, ny = g.nyp;
-    const bool ex = (xi == 0 || xi == nx - 1), ey = (xj == 0 || xj == ny - 1);
-    if (!ex && !ey) return false;
-    if (ex && ey) {   // corner: x difference along its row
-        ii = (xi == 0) ? 1 : nx - 2;
-        ij = xj;
-    } else if (ey) {
-        ii = xi;
-        ij = (xj == 0) ? 1 : ny - 2;
-    } else {
-        ii = (xi == 0) ? 1 : nx - 2;
-        ij = xj;
-    }
-    return true;
-}
-
-// What the boundary ring's adjoint passes on to lp at point (ai, aj) of the two
-// outer bands (rows 0, 1, nyp-2, nyp-1 and columns 0, 1, nxp-2, nxp-1), added
-// to lp[0..nl-1].
-template <int NL>
-__device__ __forceinline__ void bdy_gather(const Geo& g, double qbfac, const Mix& mx, const double* lq, int ai,
-                                           int aj, double (&lp)[NL]) {
-    const int nx = g.nxp, ny = g.nyp;
-    const long long oa = (long long)aj * g.pitch + ai;
-    int ii, ij;
-    if (ring_inner(g, ai, aj, ii, ij)) {
-        const double r = ((ai == 0 || ai == nx - 1) && (aj == 0 || aj == ny - 1)) ? 2.0 : 1.0;
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            const double lqk = lq[k * g.fs + oa];
-#pragma unroll
-            for (int l = 0; l < NL; l++) {
-                double c = -r * g.fnot * mx.amat[k * MAXL + l];
-                if (l == k) c -= qbfac;
-                lp[l] += lqk * c;
-            }
-        }
-    }
-    // ring neighbours whose inner point is a
-    const int cand[4][2] = {{ai - 1, aj}, {ai + 1, aj}, {ai, aj - 1}, {ai, aj + 1}};
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int xi = cand[c][0], xj = cand[c][1];
-        if (xi < 0 || xi >= nx || xj < 0 || xj >= ny) continue;
-        if (ring_inner(g, xi, xj, ii, ij) && ii == ai && ij == aj) {
-            const long long ox = (long long)xj * g.pitch + xi;
-#pragma unroll
-            for (int k = 0; k < NL; k++) lp[k] += qbfac * lq[k * g.fs + ox];
-        }
-    }
-}
-
-// ------------------------------------------------------------------ k_adj_inv1
-
-// With lp completed by the boundary ring's adjoint (bdy_gather), lv_m =
-// sum_k ctm2l(m,k) lp_k at every point (the adjoint of the modal field
-// inhomog_m + homcor_m), written to w[m]; per-block partial sums of
-// hom_m . lv_m (m >= 1) for the adjoint of the correction coefficients.
-template <int NL>
-__global__ __launch_bounds__(256) void k_adj_inv1(AdjInv1Args a) {
-    const Geo& g = a.g;
-    __shared__ double red[NL * 16];
-    double part[NL];
-#pragma unroll
-    for (int m = 0; m < NL; m++) part[m] = 0.0;
-    const int npts = g.nxp * g.nyp;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += gridDim.x * blockDim.x) {
-        const int j = t / g.nxp, i = t - j * g.nxp;
-        const long long o = (long long)j * g.pitch + i;
-        double lp[NL], hm[NL];
-#pragma unroll
-        for (int k = 0; k < NL; k++) lp[k] = a.lp[k * g.fs + o];
-#pragma unroll
-        for (int m = 1; m < NL; m++) hm[m] = a.hom[(long long)(m - 1) * g.fs + o];
-        if (j <= 1 || j >= g.nyp - 2 || i <= 1 || i >= g.nxp - 2) bdy_gather<NL>(g, a.qbfac, a.dc->mx, a.lq, i, j, lp);
-#pragma unroll
-        for (int m = 0; m < NL; m++) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = 0; k < NL; k++) v += a.dc->mx.ctm2l[m * MAXL + k] * lp[k];
-            a.w[m * g.fs + o] = v;
-            if (m >= 1) part[m] += hm[m] * v;
-        }
-    }
-    block_sum<NL>(part, red);
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int m = 0; m < NL; m++) a.partial[(long long)blockIdx.x * MAXL + m] = part[m];
-}
-
-// ------------------------------------------------------------------ k_adj_mix
-
-// First, in every block alike: la_{m-1} = the sum of the partials of mode m.
-// The forward solves cdhoc alpha = dpi - (ik_{l+1} - ik_l), ik_k = sum_m
-// ctm2l(m,k) I_m, I_m the area integral of inhomog_m (interior weight dx dy).
-// Transposed: lr = cdhoc^-T la, lI_m = -sum_l lr_l (ctm2l(m,l+1) - ctm2l(m,l)),
-// s_m = lI_m dx dy, the constant added to the interior of lv_m.  Then
-// lq_k += f0 sum_m ctl2m(k,m) (R_m + s_m G_m) on the interior, where R_m is
-// the Helmholtz solve of lv_m and G_m that of the interior indicator.
-template <int NL>
-__global__ __launch_bounds__(256) void k_adj_mix(AdjMixArgs a) {
-    const Geo& g = a.g;
-    const int nxi = g.nxp - 2, nyi = g.nyp - 2;
-    __shared__ double red[NL * 16];
-    __shared__ double s_s[NL];
-    // this thread's first point: its loads go out with those of the partials
-    const int npts = nxi * nyi;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    double R[NL], G[NL], L[NL];
-    auto load_point = [&](int tt) {
-        const int tc = tt < npts ? tt : 0;
-        const int j = 1 + tc / nxi, i = 1 + (tc - (j - 1) * nxi);
-        const long long o = (long long)j * g.pitch + i;
-#pragma unroll
-        for (int m = 0; m < NL; m++) {
-            R[m] = a.R[m * g.fs + o];
-            G[m] = a.G[m * g.fs + o];
-            L[m] = a.lq[m * g.fs + o];
-        }
-        return o;
-    };
-    long long o = load_point(t);
-    double v[NL];
-#pragma unroll
-    for (int m = 0; m < NL; m++) v[m] = 0.0;
-    for (int b = threadIdx.x; b < a.nblocks; b += blockDim.x)
-#pragma unroll
-        for (int m = 0; m < NL; m++) v[m] += a.partial[(long long)b * MAXL + m];
-    block_sum<NL>(v, red);
-    if (threadIdx.x == 0) {
-        double lr[NL];
-#pragma unroll
-        for (int l = 0; l < NL - 1; l++) {
-            double s = 0.0;
-#pragma unroll
-            for (int j = 0; j < NL - 1; j++) s += a.cinvT[l * MAXL + j] * v[j + 1];
-            lr[l] = s;
-        }
-#pragma unroll
-        for (int m = 0; m < NL; m++) {
-            double s = 0.0;
-#pragma unroll
-            for (int l = 0; l < NL - 1; l++)
-                s -= lr[l] * (a.dc->mx.ctm2l[m * MAXL + l + 1] - a.dc->mx.ctm2l[m * MAXL + l]);
-            s_s[m] = (s * g.dx) * g.dy;
-        }
-    }
-    __syncthreads();
-    double ss[NL];
-#pragma unroll
-    for (int m = 0; m < NL; m++) ss[m] = s_s[m];
-    for (; t < npts; t += gridDim.x * blockDim.x) {
-        if (t != (int)(blockIdx.x * blockDim.x + threadIdx.x)) o = load_point(t);
-        double r[NL];
-#pragma unroll
-        for (int m = 0; m < NL; m++) r[m] = R[m] + ss[m] * G[m];
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            double s = 0.0;
-#pragma unroll
-            for (int m = 0; m < NL; m++) s += a.dc->mx.ctl2m[k * MAXL + m] * r[m];
-            a.lq[k * g.fs + o] = L[k] + g.fnot * s;
-        }
     }
 }
 
@@ -491,34 +317,6 @@ static int blocks_for(long long n) {
 
 void launch_adj_supp(const AdjSuppArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_adj_supp, dim3(blocks_for((long long)a.g.nl * a.g.fs)), dim3(256), 0, st, a);
-}
-
-int adj_inv1_blocks(const Geo& g) { return blocks_for((long long)g.nxp * g.nyp) > 256 ? 256 : blocks_for((long long)g.nxp * g.nyp); }
-
-// Instantiations over the layer count 2..8.
-#define OQG_ADJ_NL_SWITCH(NLV, CALL) \
-    switch (NLV) {                   \
-        case 2: CALL(2); break;      \
-        case 3: CALL(3); break;      \
-        case 4: CALL(4); break;      \
-        case 5: CALL(5); break;      \
-        case 6: CALL(6); break;      \
-        case 7: CALL(7); break;      \
-        case 8: CALL(8); break;      \
-        default: break;              \
-    }
-
-void launch_adj_inv1(const AdjInv1Args& a, hipStream_t st) {
-#define INV1_CALL(N) hipLaunchKernelGGL(k_adj_inv1<N>, dim3(a.nblocks), dim3(256), 0, st, a)
-    OQG_ADJ_NL_SWITCH(a.g.nl, INV1_CALL)
-#undef INV1_CALL
-}
-
-void launch_adj_mix(const AdjMixArgs& a, hipStream_t st) {
-    const dim3 grid(blocks_for((long long)(a.g.nxp - 2) * (a.g.nyp - 2)));
-#define MIX_CALL(N) hipLaunchKernelGGL(k_adj_mix<N>, grid, dim3(256), 0, st, a)
-    OQG_ADJ_NL_SWITCH(a.g.nl, MIX_CALL)
-#undef MIX_CALL
 }
 
 void launch_adj_add(double* dst, const double* src, long long n, hipStream_t st) {

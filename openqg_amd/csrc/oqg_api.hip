@@ -85,8 +85,7 @@ struct oqg_ctx {
     double* lq[2] = {nullptr, nullptr};
     double* lp[2] = {nullptr, nullptr};
     double* lwek = nullptr;
-    double* adjW = nullptr;       // nl fields
-    double* adjR = nullptr;       // nl fields
+    double* adjW = nullptr;       // one field: the interior indicator
     double* adjG = nullptr;       // nl fields: H_m^-1 of the interior indicator
     double* adjPart = nullptr;
     double* adjCinvT = nullptr;
@@ -391,36 +390,6 @@ static int enqueue_raw_solve(oqg_ctx* c, int m, const double* in = nullptr, doub
     return 0;
 }
 
-// Raw Helmholtz solves of every mode in one pass: field m of out = H_m^-1
-// (field m of in) on the interior, 0 on the boundary.
-static int enqueue_raw_solve_all(oqg_ctx* c, const double* in, double* out) {
-    FwdArgs fa{};
-    fa.g = c->g;
-    fa.s = c->s;
-    fa.dc = c->dconst;
-    fa.q = in;
-    fa.yprel = c->yprel;
-    fa.tw = c->tw;
-    fa.spec = c->spec;
-    fa.raw_mode = -1;
-    fa.raw_all = 1;
-    launch_fwd(fa, c->g.nl, c->st);
-    YArgs ya{};
-    ya.s = c->s;
-    ya.spec = c->spec;
-    ya.bcol = c->bcol;
-    ya.gam = c->gamw;
-    ya.mode0 = 0;
-    ya.nmodes = c->g.nl;
-    launch_ysolve(ya, c->variant, c->st);
-    InvArgs ia;
-    fill_inv_common(c, ia);
-    ia.raw_all = 1;
-    ia.out_raw = out;
-    launch_inv(ia, c->st);
-    HIPCHK(c, hipGetLastError());
-    return 0;
-}
 
 enum { KV = 0, KC, KF, KY, KI, NK };
 static const char* kNames[NK] = {"k_vort", "k_cyc", "k_fwd", "k_ysolve", "k_inv"};
@@ -1283,31 +1252,40 @@ static int enqueue_invert(oqg_ctx* c) {
 // solves, homogeneous correction, mix), with lp already holding the adjoint of
 // p including what the boundary PV passed on.
 // The boundary ring of q_new is written from p_new (ocqbdy), so its adjoint
-// in lq's ring is passed on to lp first (inside k_adj_inv1; lp itself is left
-// as it is).
+// in lq's ring is passed on to lp first (inside k_fwd_adj; lp itself is left
+// as it is).  Three launches: the transform kernels in their adjoint modes
+// around the forward y solve.
 static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp) {
-    AdjInv1Args i1{};
-    i1.g = c->g;
-    i1.qbfac = c->ph.qbfac;
-    i1.dc = c->dconst;
-    i1.lq = lq;
-    i1.lp = lp;
-    i1.hom = c->hom;
-    i1.w = c->adjW;
-    i1.partial = c->adjPart;
-    i1.nblocks = adj_inv1_blocks(c->g);
-    launch_adj_inv1(i1, c->st);
-    if (int r = enqueue_raw_solve_all(c, c->adjW, c->adjR)) return r;
-    AdjMixArgs mx{};
-    mx.g = c->g;
-    mx.dc = c->dconst;
-    mx.R = c->adjR;
-    mx.G = c->adjG;
-    mx.cinvT = c->adjCinvT;
-    mx.partial = c->adjPart;
-    mx.nblocks = i1.nblocks;
-    mx.lq = lq;
-    launch_adj_mix(mx, c->st);
+    FwdArgs fa{};
+    fa.g = c->g;
+    fa.s = c->s;
+    fa.dc = c->dconst;
+    fa.yprel = c->yprel;
+    fa.tw = c->tw;
+    fa.spec = c->spec;
+    fa.raw_mode = -1;
+    fa.adj_lp = lp;
+    fa.adj_lq = lq;
+    fa.adj_hom = c->hom;
+    fa.adj_partial = c->adjPart;
+    fa.adj_qbfac = c->ph.qbfac;
+    launch_fwd_adj(fa, c->st);
+    YArgs ya{};
+    ya.s = c->s;
+    ya.spec = c->spec;
+    ya.bcol = c->bcol;
+    ya.gam = c->gamw;
+    ya.mode0 = 0;
+    ya.nmodes = c->g.nl;
+    launch_ysolve(ya, c->variant, c->st);
+    InvArgs ia;
+    fill_inv_common(c, ia);
+    ia.adj_G = c->adjG;
+    ia.adj_cinvT = c->adjCinvT;
+    ia.adj_partial = c->adjPart;
+    ia.adj_npart = (c->s.nyi + 1) / 2;
+    ia.adj_lq = lq;
+    launch_inv_adj(ia, c->st);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
@@ -1370,10 +1348,9 @@ int oqg_adj_init(oqg_ctx* c) {
         if (!rc) rc = dalloc(c, &c->lp[t], fl);
     }
     if (!rc) rc = dalloc(c, &c->lwek, f1);
-    if (!rc) rc = dalloc(c, &c->adjW, fl);
-    if (!rc) rc = dalloc(c, &c->adjR, fl);
+    if (!rc) rc = dalloc(c, &c->adjW, f1);
     if (!rc) rc = dalloc(c, &c->adjG, fl);
-    if (!rc) rc = dalloc(c, &c->adjPart, (size_t)256 * MAXL);
+    if (!rc) rc = dalloc(c, &c->adjPart, (size_t)((c->s.nyi + 1) / 2) * MAXL);   // [row pair][mode]
     if (!rc) rc = dalloc(c, &c->adjCinvT, MAXL * MAXL);
     if (rc) return rc;
     // (cdhoc^-1)^T by Gauss-Jordan in long double

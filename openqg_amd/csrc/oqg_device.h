@@ -308,5 +308,66 @@ __device__ __forceinline__ void fft_lds(double2* buf, const Spec& s, const doubl
     }
 }
 
+// ---------------------------------------------------------------- ocqbdy, transposed
+// Forward (box), at a ring point x with inner point x_in (vorsubs.F90:58-114):
+//   q_k(x) = qbfac (p_k(x_in) - p_k(x)) - r f0 sum_l A_kl p_l(x) + r beta y + ...
+// with r = 2 at the corners (the row and column terms both applied) and 1
+// otherwise; a corner's inner point is along its row.
+__device__ __forceinline__ bool ring_inner(const Geo& g, int xi, int xj, int& ii, int& ij) {
+    const int nx = g.nxp, ny = g.nyp;
+    const bool ex = (xi == 0 || xi == nx - 1), ey = (xj == 0 || xj == ny - 1);
+    if (!ex && !ey) return false;
+    if (ex && ey) {
+        ii = (xi == 0) ? 1 : nx - 2;
+        ij = xj;
+    } else if (ey) {
+        ii = xi;
+        ij = (xj == 0) ? 1 : ny - 2;
+    } else {
+        ii = (xi == 0) ? 1 : nx - 2;
+        ij = xj;
+    }
+    return true;
+}
+
+// The transpose gathered at point a of the two outer bands (rows 0, 1,
+// nyp-2, nyp-1 and columns 0, 1, nxp-2, nxp-1), added to lp[0..NL-1]:
+//   lp_l(a) += sum_k lq_k(a) (-qbfac d_kl - r f0 A_kl)        (a on the ring)
+//   lp_k(a) += qbfac lq_k(x)   for each ring point x whose inner point is a.
+template <int NL>
+__device__ __forceinline__ void bdy_gather(const Geo& g, double qbfac, const Mix& mx, const double* lq, int ai,
+                                           int aj, double (&lp)[NL]) {
+    const int nx = g.nxp, ny = g.nyp;
+    const long long oa = (long long)aj * g.pitch + ai;
+    int ii, ij;
+    if (ring_inner(g, ai, aj, ii, ij)) {
+        const double r = ((ai == 0 || ai == nx - 1) && (aj == 0 || aj == ny - 1)) ? 2.0 : 1.0;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const double lqk = lq[k * g.fs + oa];
+#pragma unroll
+            for (int l = 0; l < NL; l++) {
+                double c = -r * g.fnot * mx.amat[k * MAXL + l];
+                if (l == k) c -= qbfac;
+                lp[l] += lqk * c;
+            }
+        }
+    }
+    const int cand[4][2] = {{ai - 1, aj}, {ai + 1, aj}, {ai, aj - 1}, {ai, aj + 1}};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int xi = cand[c][0], xj = cand[c][1];
+        if (xi < 0 || xi >= nx || xj < 0 || xj >= ny) continue;
+        if (ring_inner(g, xi, xj, ii, ij) && ii == ai && ij == aj) {
+            const long long ox = (long long)xj * g.pitch + xi;
+#pragma unroll
+            for (int k = 0; k < NL; k++) lp[k] += qbfac * lq[k * g.fs + ox];
+        }
+    }
+}
+
+__device__ __forceinline__ bool outer_band(const Geo& g, int i, int j) {
+    return j <= 1 || j >= g.nyp - 2 || i <= 1 || i >= g.nxp - 2;
+}
 
 }  // namespace oqg

@@ -132,7 +132,16 @@ struct FwdArgs {
     const double* tw;     // twiddles exp(-2 pi i t / L), interleaved
     double* spec;
     int raw_mode;         // >=0: raw single field solved for this mode (0-based)
-    int raw_all;          // 1: raw field m (at q + m*fs) solved for mode m, every mode
+    // adjoint mode (k_fwd_adj, box): the input of mode m is
+    // w_m = sum_k ctm2l(m,k) lp_k, lp being adj_lp completed by the transpose
+    // of ocqbdy from adj_lq's boundary ring; per-block sums of hom_m . w_m over
+    // the block's rows (the first and last blocks add rows 0 and nyp-1) go to
+    // adj_partial[row pair][m]
+    const double* adj_lp;
+    const double* adj_lq;
+    const double* adj_hom;
+    double* adj_partial;
+    double adj_qbfac;
 };
 
 struct YArgs {
@@ -163,9 +172,17 @@ struct InvArgs {
     const double* ddyn;
     double* pnew;           // p_new (pm slot)
     double* qnew;           // q_new (boundary ring written)
-    double* out_raw;        // raw mode output field (raw_all: field m at out_raw + m*fs)
+    double* out_raw;        // raw mode output field
     int raw_mode;           // >=0 raw mode
-    int raw_all;            // 1: every mode raw
+    // adjoint mode (k_inv_adj, box): every mode's inverse transform u_m, the
+    // constants s_m from k_fwd_adj's partial sums (adj_npart row pairs) and
+    // adj_cinvT, then adj_lq_k += f0 sum_m ctl2m(k,m) (u_m + s_m G_m) on the
+    // block's interior points
+    const double* adj_G;
+    const double* adj_cinvT;
+    const double* adj_partial;
+    int adj_npart;
+    double* adj_lq;
     double tdt;
 };
 
@@ -209,29 +226,6 @@ struct AdjSuppArgs {
     double *lq, *lqm, *lp, *lpm;
 };
 
-struct AdjInv1Args {
-    Geo g;
-    double qbfac;
-    const DevConsts* dc;
-    const double* lq;  // adjoint of q_new, whose boundary ring ocqbdy wrote from p_new
-    const double* lp;   // adjoint of p_new, before what the boundary ring passes on
-    const double* hom;  // box homogeneous solutions, (nl-1) fields
-    double* w;          // out: adjoint of each modal field, nl fields
-    double* partial;    // out: per-block sums of hom_m . w_m, [block][MAXL]
-    int nblocks;
-};
-
-struct AdjMixArgs {
-    Geo g;
-    const DevConsts* dc;
-    const double* R;         // Helmholtz solves of w_m, nl fields
-    const double* G;         // Helmholtz solves of the interior indicator, nl fields
-    const double* cinvT;     // (cdhoc^-1)^T, [l*MAXL + j]
-    const double* partial;   // k_adj_inv1's per-block sums
-    int nblocks;
-    double* lq;              // adjoint of q_new (+= on the interior)
-};
-
 struct AdjVortArgs {
     Geo g;
     Phys ph;
@@ -271,11 +265,8 @@ void launch_copy_segs(const CopySegs& a, hipStream_t st);
 void launch_set_int(int* p, int v, hipStream_t st);
 void launch_zero_ring(double* f, const Geo& g, hipStream_t st);
 
-void launch_adj_inv1(const AdjInv1Args& a, hipStream_t st);
-void launch_adj_mix(const AdjMixArgs& a, hipStream_t st);
 void launch_adj_add(double* dst, const double* src, long long n, hipStream_t st);
 void launch_adj_vort(const AdjVortArgs& a, hipStream_t st);
-int adj_inv1_blocks(const Geo& g);
 
 // Most dynamic LDS a launch of the LDS-staged kernels may ask for (the rest of
 // the 160 KB of a CU is left to their static LDS).
@@ -287,6 +278,8 @@ void launch_cyc(const CycArgs& a, hipStream_t st);
 void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st);
 void launch_ysolve(const YArgs& a, int variant, hipStream_t st);
 void launch_inv(const InvArgs& a, hipStream_t st);
+void launch_fwd_adj(const FwdArgs& a, hipStream_t st);   // adjoint modes, every mode
+void launch_inv_adj(const InvArgs& a, hipStream_t st);
 void launch_suppress(const SuppArgs& a, hipStream_t st);
 void launch_qcomp(const QcompArgs& a, hipStream_t st);
 int inv_block_count(const Spec& s);

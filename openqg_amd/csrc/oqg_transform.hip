@@ -44,32 +44,47 @@ __device__ __forceinline__ double rhs_mode(const FwdArgs& a, const double (&cm)[
     return g.fnot * s;
 }
 
-// Row pair jp of mode mb (or of the raw field).
-template <int NL, int MB>
+// w = sum_k cm[k] lp_k at (i, gj) with lp completed by the transpose of
+// ocqbdy where (i, gj) lies in the two outer bands (adjoint mode).
+template <int NL>
+__device__ __forceinline__ double adj_modal_input(const FwdArgs& a, const double (&cm)[NL], double (&lp)[NL], int i,
+                                                  int gj) {
+    if (outer_band(a.g, i, gj)) bdy_gather<NL>(a.g, a.adj_qbfac, a.dc->mx, a.adj_lq, i, gj, lp);
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < NL; k++) w += cm[k] * lp[k];
+    return w;
+}
+
+// Row pair jp of mode mb (or of the raw field).  ADJ: the adjoint mode of
+// the box inversion (FwdArgs::adj_*), whose input of mode m is
+// w_m = sum_k ctm2l(m,k) lp_k.
+template <int NL, int MB, bool ADJ = false>
 __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     extern __shared__ double2 zbuf[];   // [L] transform, then the twiddle table
     const Geo& g = a.g;
     const Spec& s = a.s;
     const int L = s.L;
-    const int m = (a.raw_mode >= 0) ? a.raw_mode : mb;
+    const int m = (!ADJ && a.raw_mode >= 0) ? a.raw_mode : mb;
     const int r0 = 2 * jp, r1 = 2 * jp + 1;
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
     double2* tw = zbuf + L;
     Staged<4, double2> stw;   // the whole table for L <= 1024
     stage_load(stw, a.tw, L);
-    const bool raw = a.raw_mode >= 0 || a.raw_all;
-    const double* qin = a.q + (a.raw_all ? (long long)mb * g.fs : 0);
+    const bool raw = !ADJ && a.raw_mode >= 0;
+    const double* qin = ADJ ? a.adj_lp : a.q;
     // inputs of this thread's (at most four) points of both rows, loaded together
     const int first = g.cyclic ? 0 : 1;
     const int npts = s.nin;   // box: interior points 1..nin; channel: 0..nin-1
     // every load is issued before any is used: q of all layers (one layer in
-    // raw mode), ddyn and beta*y of both rows
+    // raw mode), ddyn and beta*y of both rows (adjoint mode: lp of all layers
+    // and the homogeneous solution of mode m)
     constexpr int NQ = 4;
     const int nlq = raw ? 1 : NL;
-    double cm[NL];   // ctl2m(:, m) from the device constants
+    double cm[NL];   // ctl2m(:, m) from the device constants; adjoint mode: ctm2l(m, :)
 #pragma unroll
-    for (int k = 0; k < NL; k++) cm[k] = a.dc->mx.ctl2m[k * MAXL + m];
+    for (int k = 0; k < NL; k++) cm[k] = ADJ ? a.dc->mx.ctm2l[m * MAXL + k] : a.dc->mx.ctl2m[k * MAXL + m];
     double q0[NQ][NL], q1[NQ][NL], d0[NQ], d1[NQ];
 #pragma unroll
     for (int u = 0; u < NQ; u++) {
@@ -81,25 +96,65 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
             q0[u][k] = (in && k < nlq) ? qin[k * g.fs + o0] : 0.0;
             q1[u][k] = (in && has1 && k < nlq) ? qin[k * g.fs + o1] : 0.0;
         }
-        d0[u] = (in && !raw && a.ddyn) ? a.ddyn[o0] : 0.0;
-        d1[u] = (in && has1 && !raw && a.ddyn) ? a.ddyn[o1] : 0.0;
+        if (ADJ) {
+            const double* hm = a.adj_hom + (long long)(m > 0 ? m - 1 : 0) * g.fs;
+            d0[u] = (in && m > 0) ? hm[o0] : 0.0;
+            d1[u] = (in && has1 && m > 0) ? hm[o1] : 0.0;
+        } else {
+            d0[u] = (in && !raw && a.ddyn) ? a.ddyn[o0] : 0.0;
+            d1[u] = (in && has1 && !raw && a.ddyn) ? a.ddyn[o1] : 0.0;
+        }
     }
     const double by0 = g.beta * a.yprel[gj0], by1 = has1 ? g.beta * a.yprel[gj1] : 0.0;
     double xv[NQ], yv[NQ];
+    double part = 0.0;   // adjoint mode: this thread's share of hom_m . w_m
 #pragma unroll
     for (int u = 0; u < NQ; u++) {
-        if (raw) {
+        const int t = first + threadIdx.x + 256 * u;
+        if (ADJ) {
+            const bool in = t < first + npts;
+            xv[u] = in ? adj_modal_input<NL>(a, cm, q0[u], t, gj0) : 0.0;
+            yv[u] = (in && has1) ? adj_modal_input<NL>(a, cm, q1[u], t, gj1) : 0.0;
+        } else if (raw) {
             xv[u] = q0[u][0];
             yv[u] = q1[u][0];
         } else {
             xv[u] = rhs_mode<NL>(a, cm, q0[u], by0, d0[u]);
             yv[u] = has1 ? rhs_mode<NL>(a, cm, q1[u], by1, d1[u]) : 0.0;
         }
-        const int t = first + threadIdx.x + 256 * u;
         if (t >= first + npts) {
             xv[u] = 0.0;
             yv[u] = 0.0;
         }
+        if (ADJ) part += d0[u] * xv[u] + d1[u] * yv[u];
+    }
+    if constexpr (ADJ) {
+        // the block's other points of the inner products: columns 0 and nxp-1
+        // of its rows, and rows 0 / nyp-1 in the first / last block
+        __shared__ double s_red[16];
+        const bool lastb = jp == (s.nyi + 1) / 2 - 1;
+        const int nside = has1 ? 4 : 2;
+        const int nextra = nside + ((jp == 0 ? 1 : 0) + (lastb ? 1 : 0)) * g.nxp;
+        for (int e = threadIdx.x; e < nextra; e += blockDim.x) {
+            int i, gj;
+            if (e < nside) {
+                gj = (e >> 1) ? gj1 : gj0;
+                i = (e & 1) ? g.nxp - 1 : 0;
+            } else {
+                const int r = (e - nside) / g.nxp;
+                i = (e - nside) - r * g.nxp;
+                gj = (r == 0 && jp == 0) ? 0 : g.nyp - 1;
+            }
+            const long long o = (long long)gj * g.pitch + i;
+            double lp[NL];
+#pragma unroll
+            for (int k = 0; k < NL; k++) lp[k] = a.adj_lp[k * g.fs + o];
+            const double w = adj_modal_input<NL>(a, cm, lp, i, gj);
+            if (m > 0) part += a.adj_hom[(long long)(m - 1) * g.fs + o] * w;
+        }
+        double pv[1] = {part};
+        block_sum<1>(pv, s_red);
+        if (threadIdx.x == 0) a.adj_partial[(long long)jp * MAXL + m] = pv[0];
     }
     stage_store(stw, tw, a.tw, L);
     if (!g.cyclic) {
@@ -150,6 +205,9 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
 template <int NL, int MB>
 __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) { fwd_block<NL, MB>(a, blockIdx.x, blockIdx.y); }
 
+template <int NL, int MB>
+__global__ __launch_bounds__(256) void k_fwd_adj(FwdArgs a) { fwd_block<NL, MB, true>(a, blockIdx.x, blockIdx.y); }
+
 // Ensemble form: block b works on block map[b].y (row pairs fastest, then
 // modes) of member map[b].x.
 template <int NL, int MB>
@@ -191,6 +249,14 @@ void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
 #define FWD_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd<NLc, MBc>), dim3(npairs, nmodes), dim3(256), lds, st, a)
     OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), FWD_CALL)
 #undef FWD_CALL
+}
+
+void launch_fwd_adj(const FwdArgs& a, hipStream_t st) {
+    const int npairs = (a.s.nyi + 1) / 2;
+    const size_t lds = fwd_lds_bytes(a.s);
+#define FWDA_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd_adj<NLc, MBc>), dim3(npairs, a.g.nl), dim3(256), lds, st, a)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), FWDA_CALL)
+#undef FWDA_CALL
 }
 
 void launch_fwd_ens(const FwdArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st) {
@@ -668,13 +734,14 @@ __device__ __forceinline__ void load_spec_pair(const InvArgs& a, int m, bool ok,
     }
 }
 
-template <int NL, int MB>
+// Row pair jp.  ADJ: the adjoint mode of the box inversion (InvArgs::adj_*).
+template <int NL, int MB, bool ADJ = false>
 __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     extern __shared__ double smem[];
     const Geo& g = a.g;
     const Spec& s = a.s;
     const int L = s.L;
-    const bool raw1 = a.raw_mode >= 0, raw = raw1 || a.raw_all;
+    const bool raw1 = !ADJ && a.raw_mode >= 0, raw = raw1 || ADJ;
     const int nm = raw1 ? 1 : NL;
     const int G = blockDim.x / FFT_GROUP;          // transforms in flight
     const int grp = threadIdx.x / FFT_GROUP, gt = threadIdx.x - grp * FFT_GROUP;
@@ -834,16 +901,72 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
         __syncthreads();
     }
 
-    if (raw) {
-        for (int m = 0; m < nm; m++) {
-            double* o = a.out_raw + (long long)m * g.fs;
-            const double* src = inh + (size_t)m * 2 * g.pitch;
-            for (int i = threadIdx.x; i < g.nxp; i += blockDim.x) {
-                o[(long long)gj0 * g.pitch + i] = src[i];
-                if (has1) o[(long long)gj1 * g.pitch + i] = src[g.pitch + i];
-                if (first) o[i] = 0.0;
-                if (last) o[(long long)(g.nyp - 1) * g.pitch + i] = 0.0;
+    if constexpr (ADJ) {
+        // The forward solves cdhoc alpha = dpi - (ik_{l+1} - ik_l), ik_k =
+        // sum_m ctm2l(m,k) I_m, I_m the area integral of inhomog_m (interior
+        // weight dx dy).  Transposed, with la_{m-1} the sum over row pairs of
+        // the partials hom_m . w_m: lr = cdhoc^-T la and the constant
+        // s_m = -dx dy sum_l lr_l (ctm2l(m,l+1) - ctm2l(m,l)) on the interior
+        // of w_m, whose Helmholtz solve is s_m G_m.  The same sums in the same
+        // order in every block.
+        double la[NL];
+#pragma unroll
+        for (int mm = 0; mm < NL; mm++) la[mm] = 0.0;
+        for (int b = threadIdx.x; b < a.adj_npart; b += blockDim.x)
+#pragma unroll
+            for (int mm = 0; mm < NL; mm++) la[mm] += a.adj_partial[(long long)b * MAXL + mm];
+        block_sum<NL>(la, red);
+        if (threadIdx.x == 0) {
+            double lr[NL];
+#pragma unroll
+            for (int l = 0; l < NL - 1; l++) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int j = 0; j < NL - 1; j++) sacc += a.adj_cinvT[l * MAXL + j] * la[j + 1];
+                lr[l] = sacc;
             }
+#pragma unroll
+            for (int mm = 0; mm < NL; mm++) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NL - 1; l++)
+                    sacc -= lr[l] * (s_ctm2l[mm * MAXL + l + 1] - s_ctm2l[mm * MAXL + l]);
+                coef[mm] = (sacc * g.dx) * g.dy;
+            }
+        }
+        __syncthreads();
+        // lq_k += f0 sum_m ctl2m(k,m) (u_m + s_m G_m) on the block's interior points
+        for (int idx = threadIdx.x; idx < nsel * g.nxp; idx += blockDim.x) {
+            const int sel = idx / g.nxp, i = idx - sel * g.nxp;
+            if (i < 1 || i > g.nxp - 2) continue;
+            const int gj = sel == 0 ? gj0 : gj1;
+            const long long o = (long long)gj * g.pitch + i;
+            double r[NL], lq[NL];
+#pragma unroll
+            for (int mm = 0; mm < NL; mm++) {
+                r[mm] = a.adj_G[mm * g.fs + o];
+                lq[mm] = a.adj_lq[mm * g.fs + o];
+            }
+#pragma unroll
+            for (int mm = 0; mm < NL; mm++) r[mm] = inh[(size_t)mm * 2 * g.pitch + sel * g.pitch + i] + coef[mm] * r[mm];
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int mm = 0; mm < NL; mm++) acc += s_dc.mx.ctl2m[k * MAXL + mm] * r[mm];
+                a.adj_lq[k * g.fs + o] = lq[k] + g.fnot * acc;
+            }
+        }
+        return;
+    }
+
+    if (raw) {
+        double* o = a.out_raw;
+        for (int i = threadIdx.x; i < g.nxp; i += blockDim.x) {
+            o[(long long)gj0 * g.pitch + i] = inh[i];
+            if (has1) o[(long long)gj1 * g.pitch + i] = inh[g.pitch + i];
+            if (first) o[i] = 0.0;
+            if (last) o[(long long)(g.nyp - 1) * g.pitch + i] = 0.0;
         }
         return;
     }
@@ -902,6 +1025,11 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs
     inv_block<NL, MB>(a, blockIdx.x);
 }
 
+template <int NL, int MB>
+__global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv_adj(InvArgs a) {
+    inv_block<NL, MB, true>(a, blockIdx.x);
+}
+
 // Ensemble form: block b works on row pair map[b].y of member map[b].x.
 template <int NL, int MB>
 __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv_ens(const InvArgs* __restrict__ args,
@@ -929,6 +1057,14 @@ void launch_inv(const InvArgs& a, hipStream_t st) {
 #undef INV_CALL
 }
 
+void launch_inv_adj(const InvArgs& a, hipStream_t st) {
+    const dim3 grid(inv_block_count(a.s)), block(FFT_GROUP * inv_group_count(a.g.nl));
+    const size_t lds = inv_lds_bytes(a.s, a.g, a.g.nl);
+#define INVA_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv_adj<NLc, MBc>), grid, block, lds, st, a)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), INVA_CALL)
+#undef INVA_CALL
+}
+
 void launch_inv_ens(const InvArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st) {
     const dim3 block(FFT_GROUP * inv_group_count(nl));
 #define INVE_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv_ens<NLc, MBc>), dim3(nblocks), block, lds, st, args, map)
@@ -949,6 +1085,8 @@ void set_kernel_attrs() {
         (void)hipFuncSetAttribute((const void*)k_fwd_ens<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
         (void)hipFuncSetAttribute((const void*)k_inv<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);     \
         (void)hipFuncSetAttribute((const void*)k_inv_ens<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+        (void)hipFuncSetAttribute((const void*)k_fwd_adj<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+        (void)hipFuncSetAttribute((const void*)k_inv_adj<NLc, MBc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
     } while (0)
 #define ATTR_NL(NLc) ATTR_NLMB(NLc, 1); ATTR_NLMB(NLc, 2); ATTR_NLMB(NLc, 4);
         ATTR_NL(2) ATTR_NL(3) ATTR_NL(4) ATTR_NL(5) ATTR_NL(6) ATTR_NL(7) ATTR_NL(8)

@@ -1,8 +1,10 @@
 """CPU checks of the host parts of the gradient path: the wind-stress adjoint,
 the kinetic-energy objective's gradient, and the checkpointing schedules."""
+from functools import lru_cache
+
 import numpy as np
 
-from openqg_amd.adjoint import Trajectory, _revolve_split, ekei1
+from openqg_amd.adjoint import Trajectory, _rev_cost, _revolve_split, ekei1
 from openqg_amd.forcing import ekman_wekp, ekman_wekp_adjoint
 
 
@@ -94,12 +96,30 @@ def test_checkpointing_reverses_every_step_once_in_order():
     for n, s in ((1, 1), (2, 1), (17, 1), (17, 3), (64, 4), (300, 8), (2160, 32)):
         steps = _sweep(n, s)
         assert steps >= n
-        if n == 2160 and s == 32:
-            assert steps <= 4 * n      # binomial: C(35, 3) >= 2160, at most ~3 extra sweeps
+        # the schedule's cost plus the forward sweep's last stretch to the end
+        assert steps <= _rev_cost(n, s) + n
+    # 32 free slots besides the initial state's (33 checkpoints) for 2160
+    # steps: repetition number 3, Griewank's minimum t = 3n - C(36, 2)
+    assert _rev_cost(2160, 32) == 3 * 2160 - 630
+    assert _sweep(2160, 32) <= 5850 + 8
 
 
-def test_revolve_split_bounds():
-    for n in range(2, 200):
-        for s in range(1, 6):
+@lru_cache(maxsize=None)
+def _dp(n, s):
+    """Brute-force minimum forward steps of Trajectory._rev's recursion."""
+    if n <= 1:
+        return 0
+    if s == 0:
+        return n * (n - 1) // 2
+    return min(m + _dp(n - m, s - 1) + _dp(m, s) for m in range(1, n))
+
+
+def test_revolve_cost_closed_form_and_split_are_optimal():
+    for s in range(0, 5):
+        for n in range(1, 61):
+            assert _rev_cost(n, s) == _dp(n, s), (n, s)
+    for s in range(1, 5):
+        for n in range(2, 61):
             m = _revolve_split(n, s)
             assert 1 <= m <= n - 1
+            assert m + _dp(n - m, s - 1) + _dp(m, s) == _dp(n, s), (n, s, m)
