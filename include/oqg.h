@@ -216,6 +216,21 @@ int oqg_snap_load(oqg_ctx* ctx, int slot);
 int oqg_adj_record(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp, int slot0);
 int oqg_adj_sweep(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp, int slot0);
 
+/* Binomial checkpointing (Griewank's revolve) with free_slots + 1 snapshot
+ * slots (slot 0 keeps the initial state).  oqg_adj_revolve_forward runs steps
+ * 1 .. nsteps from the loaded state (suppression as oqg_run with step0 = 0),
+ * placing the schedule's first checkpoints on the way, and leaves the final
+ * state loaded.  oqg_adj_revolve_reverse, with the adjoint of that state set,
+ * takes it back to the adjoint of the initial state, which it leaves loaded;
+ * forward_steps (may be NULL) receives the forward steps of both calls.
+ * oqg_revolve_cost gives the forward steps that reversing nsteps from a stored
+ * state costs with free_slots further slots, oqg_revolve_split the first
+ * advance of that schedule (host only; -1 on bad arguments). */
+int oqg_adj_revolve_forward(oqg_ctx* ctx, double tdt, int nsteps, int supp, int free_slots);
+int oqg_adj_revolve_reverse(oqg_ctx* ctx, long long* forward_steps);
+long long oqg_revolve_cost(int nsteps, int free_slots);
+int oqg_revolve_split(int nsteps, int free_slots);
+
 /*
  * Batched ensembles.  No counterpart in the reference, whose driver steps one
  * basin per step_qg call (src/driver.F90:107,143); an ensemble of independent

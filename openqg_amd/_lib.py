@@ -80,6 +80,10 @@ SIGNATURES = [
     ("oqg_snap_load", _i, [_vp, _i]),
     ("oqg_adj_record", _i, [_vp, _d, _i, _i, _i, _i]),
     ("oqg_adj_sweep", _i, [_vp, _d, _i, _i, _i, _i]),
+    ("oqg_adj_revolve_forward", _i, [_vp, _d, _i, _i, _i]),
+    ("oqg_adj_revolve_reverse", _i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    ("oqg_revolve_cost", ctypes.c_longlong, [_i, _i]),
+    ("oqg_revolve_split", _i, [_i, _i]),
     ("oqg_ens_create", _i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
     ("oqg_ens_destroy", None, [_vp]),
     ("oqg_ens_last_error", ctypes.c_char_p, [_vp]),

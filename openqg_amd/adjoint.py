@@ -157,6 +157,17 @@ class AdjointState:
         self._c(self._lib.oqg_adj_sweep(self.qg._h, float(tdt), int(nsteps), int(step0), int(supp_every),
                                         int(slot0)), "oqg_adj_sweep")
 
+    def revolve_forward(self, tdt: float, nsteps: int, supp_every: int, free_slots: int):
+        """Forward sweep of the binomial checkpointing schedule, run by the library."""
+        self._c(self._lib.oqg_adj_revolve_forward(self.qg._h, float(tdt), int(nsteps), int(supp_every),
+                                                  int(free_slots)), "oqg_adj_revolve_forward")
+
+    def revolve_reverse(self) -> int:
+        """Its reverse sweep; returns the forward steps of the whole schedule."""
+        n = ctypes.c_longlong(0)
+        self._c(self._lib.oqg_adj_revolve_reverse(self.qg._h, ctypes.byref(n)), "oqg_adj_revolve_reverse")
+        return int(n.value)
+
     def wekp(self) -> np.ndarray:
         out = np.empty((self.qg.b.nyp, self.qg.b.nxp))
         self._c(self._lib.oqg_adj_get_wekp(self.qg._h, out.ctypes.data_as(ctypes.c_void_p), OQG_HOST),
@@ -238,6 +249,10 @@ class Trajectory:
             self.forward_steps += self.n
             return
         a.alloc(self.snapshots + 1)
+        if hasattr(a, "revolve_forward"):
+            # the library runs the same schedule from C, with graph-replayed steps
+            a.revolve_forward(self.tdt, self.n, self.supp, self.snapshots)
+            return
         a.save(0)
         self._left = []
         lo, slot, free = 0, 0, self.snapshots
@@ -255,6 +270,9 @@ class Trajectory:
         a = self.adj
         if self.snapshots is None:
             a.sweep(self.tdt, self.n, 0, self.supp, 0)
+            return
+        if hasattr(a, "revolve_reverse"):
+            self.forward_steps = a.revolve_reverse()
             return
         lo, slot, free = self._tail
         self._rev(lo, self.n, slot, free)

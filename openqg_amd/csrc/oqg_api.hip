@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -96,6 +97,14 @@ struct oqg_ctx {
     // pointers and the first slot of the stretch being replayed
     double** snap_ptrs_dev = nullptr;
     int* snap_base_dev = nullptr;
+    // binomial checkpointing (oqg_adj_revolve_*): the pending left segments
+    // (lo, hi, slot, free) of the forward sweep's rightmost path, its tail
+    std::vector<std::array<int, 4>> rv_left;
+    std::array<int, 3> rv_tail{};
+    int rv_n = -1, rv_supp = 0;
+    double rv_tdt = 0.0;
+    long long rv_steps = 0;
+    std::map<std::pair<int, int>, int> rv_split;   // memo of revolve_split
 };
 
 // ------------------------------------------------------------------ error helpers
@@ -1068,6 +1077,8 @@ int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
             // P is even, so the time-level parity is unchanged
             nt += P;
         } else {
+            // single steps are launched eagerly: replaying a one-step graph
+            // measured slower than its four launches
             if (int r = enqueue_step(c, tdt)) return r;
             if (supp > 0 && (nt - 1) % supp == 0)
                 if (int r = enqueue_suppress(c)) return r;
@@ -1477,7 +1488,8 @@ static int snap_sync_dev(oqg_ctx* c) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->snap_ptrs_dev = p;
     for (auto it = c->graphs.begin(); it != c->graphs.end();) {
-        if (it->first.kind != G_RUN) {
+        const int k = it->first.kind;
+        if (k == G_RECORD || k == G_SWEEP) {
             hipGraphExecDestroy(it->second);
             it = c->graphs.erase(it);
         } else {
@@ -1654,6 +1666,155 @@ int oqg_adj_sweep(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, int s
         nt--;
     }
     if (nsteps > 0) snap_copy(c, slot0, false);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ binomial checkpointing
+//
+// Reversing n steps from a stored state with s further free snapshot slots:
+// advance m steps, store that state, reverse the last n - m steps with s - 1
+// free slots, then the first m with s.  With no free slot, every step is
+// recomputed from the stored state.  The forward steps this costs, at the best
+// m on every level, are Griewank's t(n, c) = r n - C(c + r, r - 1) with c = s + 1
+// checkpoints and r the least repetition number with C(c + r, r) >= n
+// (Griewank and Walther, ACM TOMS 26, 2000).
+
+static long long binom(int a, int b) {   // C(a, b), exact for the sizes used here
+    if (b < 0 || b > a) return 0;
+    b = std::min(b, a - b);
+    long long v = 1;
+    for (int i = 1; i <= b; i++) v = v * (a - b + i) / i;
+    return v;
+}
+
+static long long rev_cost(int n, int s) {
+    if (n <= 1) return 0;
+    if (s == 0) return (long long)n * (n - 1) / 2;
+    const int c = s + 1;
+    int r = 0;
+    while (binom(c + r, c) < n) r++;
+    return (long long)r * n - binom(c + r, r - 1);
+}
+
+static int rev_split(oqg_ctx* c, int n, int s) {
+    if (n <= 2) return 1;
+    if (c) {
+        auto it = c->rv_split.find({n, s});
+        if (it != c->rv_split.end()) return it->second;
+    }
+    int best_m = 1;
+    long long best = -1;
+    for (int m = 1; m < n; m++) {
+        const long long v = m + rev_cost(n - m, s - 1) + rev_cost(m, s);
+        if (best < 0 || v < best) {
+            best = v;
+            best_m = m;
+        }
+    }
+    if (c) c->rv_split[{n, s}] = best_m;
+    return best_m;
+}
+
+// One adjoint step (eager launches): q_n, p_n from snapshot `slot`, or from
+// the loaded state when slot < 0.  c->cur must be the parity of that state.
+static int adj_step_at(oqg_ctx* c, double tdt, bool suppressed, int slot) {
+    SnapRef src;
+    if (slot >= 0) {
+        const long long fl = (long long)c->g.nl * c->g.fs;
+        const double* s = c->snaps[slot];
+        src.q = s + (c->cur ? fl : 0);
+        src.p = s + 2 * fl + (c->cur ? fl : 0);
+    }
+    return enqueue_adj_step(c, tdt, suppressed, src);
+}
+
+static int rv_advance(oqg_ctx* c, int lo, int hi) {
+    if (hi <= lo) return 0;
+    c->rv_steps += hi - lo;
+    return oqg_run(c, c->rv_tdt, hi - lo, lo, c->rv_supp);
+}
+
+// The state lo is in `slot`; the adjoint holds that of state hi; leave it at state lo.
+static int rv_reverse(oqg_ctx* c, int lo, int hi, int slot, int free) {
+    const int n = hi - lo;
+    const double tdt = c->rv_tdt;
+    const int supp = c->rv_supp;
+    if (n == 1) {
+        c->cur = c->snap_cur[slot];
+        return adj_step_at(c, tdt, suppressed_at(hi, supp), slot);
+    }
+    if (free == 0) {
+        for (int k = hi - 1; k >= lo; k--) {
+            if (k == lo) {
+                c->cur = c->snap_cur[slot];
+                if (int r = adj_step_at(c, tdt, suppressed_at(k + 1, supp), slot)) return r;
+                continue;
+            }
+            snap_copy(c, slot, false);
+            if (int r = rv_advance(c, lo, k)) return r;
+            if (int r = adj_step_at(c, tdt, suppressed_at(k + 1, supp), -1)) return r;
+        }
+        return 0;
+    }
+    const int m = lo + rev_split(c, n, free);
+    snap_copy(c, slot, false);
+    if (int r = rv_advance(c, lo, m)) return r;
+    snap_copy(c, slot + 1, true);
+    if (int r = rv_reverse(c, m, hi, slot + 1, free - 1)) return r;
+    return rv_reverse(c, lo, m, slot, free);
+}
+
+extern "C" {
+
+long long oqg_revolve_cost(int nsteps, int free_slots) {
+    if (nsteps < 0 || free_slots < 0) return -1;
+    return rev_cost(nsteps, free_slots);
+}
+
+int oqg_revolve_split(int nsteps, int free_slots) {
+    if (nsteps < 2 || free_slots < 1) return -1;
+    return rev_split(nullptr, nsteps, free_slots);
+}
+
+int oqg_adj_revolve_forward(oqg_ctx* c, double tdt, int nsteps, int supp, int free_slots) {
+    if (!c || nsteps < 1 || free_slots < 0) return OQG_ERR_ARG;
+    if ((int)c->snaps.size() < free_slots + 1)
+        return fail(c, OQG_ERR_ARG, "oqg_adj_revolve_forward: needs free_slots + 1 snapshot slots");
+    c->rv_tdt = tdt;
+    c->rv_supp = supp;
+    c->rv_n = nsteps;
+    c->rv_steps = 0;
+    c->rv_left.clear();
+    snap_copy(c, 0, true);
+    int lo = 0, slot = 0, free = free_slots;
+    while (nsteps - lo > 1 && free > 0) {
+        const int m = lo + rev_split(c, nsteps - lo, free);
+        if (int r = rv_advance(c, lo, m)) return r;
+        snap_copy(c, slot + 1, true);
+        c->rv_left.push_back({lo, m, slot, free});
+        lo = m;
+        slot++;
+        free--;
+    }
+    if (int r = rv_advance(c, lo, nsteps)) return r;
+    c->rv_tail = {lo, slot, free};
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int oqg_adj_revolve_reverse(oqg_ctx* c, long long* forward_steps) {
+    if (!c) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    if (c->rv_n < 1) return fail(c, OQG_ERR_STATE, "oqg_adj_revolve_reverse: no forward sweep recorded");
+    if (int r = rv_reverse(c, c->rv_tail[0], c->rv_n, c->rv_tail[1], c->rv_tail[2])) return r;
+    for (auto it = c->rv_left.rbegin(); it != c->rv_left.rend(); ++it)
+        if (int r = rv_reverse(c, (*it)[0], (*it)[1], (*it)[2], (*it)[3])) return r;
+    snap_copy(c, 0, false);
+    c->rv_n = -1;
+    if (forward_steps) *forward_steps = c->rv_steps;
     HIPCHK(c, hipGetLastError());
     return 0;
 }

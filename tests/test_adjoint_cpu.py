@@ -123,3 +123,22 @@ def test_revolve_cost_closed_form_and_split_are_optimal():
             m = _revolve_split(n, s)
             assert 1 <= m <= n - 1
             assert m + _dp(n - m, s - 1) + _dp(m, s) == _dp(n, s), (n, s, m)
+
+
+def test_library_revolve_schedule_matches():
+    """The library's schedule (oqg_revolve_cost / oqg_revolve_split, host only)
+    against the brute force and the Python schedule."""
+    import openqg_amd
+
+    lib = openqg_amd.load()
+    for s in range(0, 5):
+        for n in range(1, 61):
+            assert lib.oqg_revolve_cost(n, s) == _dp(n, s), (n, s)
+    for s in range(1, 5):
+        for n in range(2, 61):
+            m = lib.oqg_revolve_split(n, s)
+            assert m + _dp(n - m, s - 1) + _dp(m, s) == _dp(n, s), (n, s, m)
+    for n, s in ((2160, 32), (700, 8), (300, 3)):
+        assert lib.oqg_revolve_cost(n, s) == _rev_cost(n, s)
+        assert lib.oqg_revolve_split(n, s) == _revolve_split(n, s)
+    assert lib.oqg_revolve_split(1, 3) == -1 and lib.oqg_revolve_cost(-1, 0) == -1
