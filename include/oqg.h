@@ -193,6 +193,20 @@ int oqg_adj_step(oqg_ctx* ctx, double tdt, int suppressed);
 int oqg_adj_get_wekp(oqg_ctx* ctx, double* out, int kind);
 int oqg_adj_zero_wekp(oqg_ctx* ctx);
 
+/* Wind-stress forcing on the device: ekman_from_tau (src/coupler/ekman.F90:53-83)
+ * with map_P_to_x_face / map_P_to_y_face and map_T_to_P
+ * (src/glam/numerics.F90:64-107,271-289).  oqg_set_wind takes the kinematic
+ * stress (taux, tauy) at p points, each (nxp,nyp), and sets the resident
+ * Ekman pumping wekp (and, for a channel, txis/txin); the resident
+ * entrainment is left as it is.  oqg_adj_wind (box) applies the transpose to
+ * the accumulated adjoint of wekp (oqg_adj_get_wekp) and returns the adjoints
+ * of taux and tauy. */
+int oqg_set_wind(oqg_ctx* ctx, const double* taux, const double* tauy, int kind);
+int oqg_adj_wind(oqg_ctx* ctx, double* ltaux, double* ltauy, int kind);
+/* The resident forcing: wekp (nxp,nyp) and txi = {txis, txin} (host, 2
+ * doubles); either may be NULL. */
+int oqg_get_forcing(oqg_ctx* ctx, double* wekp, double* txi, int kind);
+
 /* The inversion of one step applied to the current q (interior): p from q
  * (modes, Helmholtz solves, mass correction with the current dpi), q's
  * boundary PV by ocqbdy, and qm = q, pm = p.  oqg_adj_invert is its adjoint:

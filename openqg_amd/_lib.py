@@ -73,6 +73,9 @@ SIGNATURES = [
     ("oqg_adj_step", _i, [_vp, _d, _i]),
     ("oqg_adj_get_wekp", _i, [_vp, _vp, _i]),
     ("oqg_adj_zero_wekp", _i, [_vp]),
+    ("oqg_set_wind", _i, [_vp, _vp, _vp, _i]),
+    ("oqg_adj_wind", _i, [_vp, _vp, _vp, _i]),
+    ("oqg_get_forcing", _i, [_vp, _vp, _dp, _i]),
     ("oqg_invert", _i, [_vp]),
     ("oqg_adj_invert", _i, [_vp]),
     ("oqg_snap_alloc", _i, [_vp, _i]),

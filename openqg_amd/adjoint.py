@@ -5,8 +5,9 @@ src/diagnostics/qg_monitor.F90:297-302,331-333) after N steps:
     J = 0.5 rho H1 int_P_dA(u^2 + v^2),  u = -dPdy_2bc(p1)/f0, v = dPdx_2bc(p1)/f0
 (numerics.F90:470-513 for the boundary differences, :165-204 for int_P_dA).
 Controls: the kinematic wind stress (taux, tauy) at p points, through
-ekman_from_tau + map_T_to_P, and the interior PV of the initial state, whose
-pressure is one inversion of it (dpi = 0), with qm = q and pm = p.
+ekman_from_tau + map_T_to_P (on the device, oqg_set_wind / oqg_adj_wind), and
+the interior PV of the initial state, whose pressure is one inversion of it
+(dpi = 0), with qm = q and pm = p.
 
 The forward run keeps device snapshots of the state; the reverse sweep applies
 oqg_adj_step from them.  With `snapshots=None` every state is kept (HBM holds
@@ -24,7 +25,6 @@ from typing import Optional
 import numpy as np
 
 from ._lib import OQG_HOST, check
-from .forcing import ekman_wekp, ekman_wekp_adjoint
 from .qg import QG
 
 
@@ -177,6 +177,14 @@ class AdjointState:
     def zero_wekp(self):
         self._c(self._lib.oqg_adj_zero_wekp(self.qg._h), "oqg_adj_zero_wekp")
 
+    def wind(self):
+        """The adjoints of (taux, tauy) from the accumulated adjoint of wekp,
+        through the transpose of ekman_from_tau on the device."""
+        out = [np.empty((self.qg.b.nyp, self.qg.b.nxp)) for _ in range(2)]
+        self._c(self._lib.oqg_adj_wind(self.qg._h, *(o.ctypes.data_as(ctypes.c_void_p) for o in out), OQG_HOST),
+                "oqg_adj_wind")
+        return out[0], out[1]
+
     def invert(self):
         self._c(self._lib.oqg_invert(self.qg._h), "oqg_invert")
 
@@ -316,9 +324,9 @@ def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: fl
     if b.cyclic:
         raise ValueError("the adjoint is implemented for box basins")
     adj = AdjointState(qg)
-    wekp, _, _ = ekman_wekp(b.nxp, b.nyp, b.dx, b.dy, b.fnot, False, taux, tauy)
     ent = np.zeros((max(b.nl - 1, 1), b.nyp, b.nxp))
-    qg.set_forcing(ent, wekp, 0.0, 0.0)
+    qg.set_forcing(ent, None, 0.0, 0.0)
+    qg.set_wind(taux, tauy)   # wekp on the device
     qg.set_state(q=q0, qm=q0, p=np.zeros(qg.shape), pm=np.zeros(qg.shape))
     qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
     adj.invert()
@@ -331,19 +339,18 @@ def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: fl
     adj.set(lp=lp)
     adj.zero_wekp()
     traj.reverse()
-    lw = adj.wekp()
+    gtx, gty = adj.wind()     # the transpose of the wind forcing on the device
     adj.adj_invert()
     gq0 = adj.get()[0]
-    gtx, gty = ekman_wekp_adjoint(b.nxp, b.nyp, b.dx, b.dy, b.fnot, lw)
     return J, gtx, gty, gq0, traj.forward_steps
 
 
 def objective(qg: QG, q0, taux, tauy, tdt: float, nsteps: int, supp_every: int = 25, rho: float = 1000.0) -> float:
     """The forward map of `gradient` alone (for finite-difference checks)."""
     b = qg.b
-    wekp, _, _ = ekman_wekp(b.nxp, b.nyp, b.dx, b.dy, b.fnot, False, taux, tauy)
     ent = np.zeros((max(b.nl - 1, 1), b.nyp, b.nxp))
-    qg.set_forcing(ent, wekp, 0.0, 0.0)
+    qg.set_forcing(ent, None, 0.0, 0.0)
+    qg.set_wind(taux, tauy)
     qg.set_state(q=q0, qm=q0, p=np.zeros(qg.shape), pm=np.zeros(qg.shape))
     qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
     check(qg._lib.oqg_invert(qg._h), qg._h, "oqg_invert")

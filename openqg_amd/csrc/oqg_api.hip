@@ -74,6 +74,8 @@ struct oqg_ctx {
     std::vector<void*> allocs;
     bool has_ent2 = false;
     double* fcon = nullptr;   // txis, txin, layer-2 entrainment flag (device)
+    double* taux_d = nullptr;   // kinematic wind stress at p points (oqg_set_wind / oqg_adj_wind)
+    double* tauy_d = nullptr;
     hipStream_t own = nullptr;
     hipStream_t st = nullptr;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -1003,6 +1005,63 @@ int oqg_set_forcing(oqg_ctx* c, const double* ent, int n_ent, const double* wekp
     f[F_TXIN] = txin;
     f[F_ENT2] = c->has_ent2 ? 1.0 : 0.0;
     HIPCHK(c, hipMemcpyAsync(c->fcon, f, sizeof f, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+static int wind_buffers(oqg_ctx* c) {
+    if (!c->taux_d)
+        if (int r = dalloc(c, &c->taux_d, (size_t)c->g.fs)) return r;
+    if (!c->tauy_d)
+        if (int r = dalloc(c, &c->tauy_d, (size_t)c->g.fs)) return r;
+    return 0;
+}
+
+int oqg_set_wind(oqg_ctx* c, const double* taux, const double* tauy, int kind) {
+    if (!c || !taux || !tauy) return c ? fail(c, OQG_ERR_ARG, "oqg_set_wind: null stress") : OQG_ERR_ARG;
+    if (int r = wind_buffers(c)) return r;
+    if (int r = upload_field(c, c->taux_d, taux, 1, kind)) return r;
+    if (int r = upload_field(c, c->tauy_d, tauy, 1, kind)) return r;
+    EkmanArgs a{};
+    a.g = c->g;
+    a.taux = c->taux_d;
+    a.tauy = c->tauy_d;
+    a.wekp = c->wekp;
+    a.fcon = c->fcon;
+    launch_ekman(a, c->st);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_get_forcing(oqg_ctx* c, double* wekp, double* txi, int kind) {
+    if (!c) return OQG_ERR_ARG;
+    if (wekp)
+        if (int r = download_field(c, wekp, c->wekp, 1, kind)) return r;
+    if (txi) {
+        double f[F_LEN];
+        HIPCHK(c, hipMemcpyAsync(f, c->fcon, sizeof f, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        txi[0] = f[F_TXIS];
+        txi[1] = f[F_TXIN];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_adj_wind(oqg_ctx* c, double* ltaux, double* ltauy, int kind) {
+    if (!c || !ltaux || !ltauy) return c ? fail(c, OQG_ERR_ARG, "oqg_adj_wind: null output") : OQG_ERR_ARG;
+    if (!c->adj) return fail(c, OQG_ERR_STATE, "call oqg_adj_init first");
+    if (int r = wind_buffers(c)) return r;
+    EkmanArgs a{};
+    a.g = c->g;
+    a.taux = c->taux_d;
+    a.tauy = c->tauy_d;
+    a.lwek = c->lwek;
+    launch_ekman_adj(a, c->st);
+    HIPCHK(c, hipGetLastError());
+    if (int r = download_field(c, ltaux, c->taux_d, 1, kind)) return r;
+    if (int r = download_field(c, ltauy, c->tauy_d, 1, kind)) return r;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }

@@ -246,6 +246,18 @@ struct AdjVortArgs {
 };
 
 void launch_adj_supp(const AdjSuppArgs& a, hipStream_t st);
+
+// Wind-stress forcing producer (oqg_forcing.hip).
+struct EkmanArgs {
+    Geo g;
+    double* taux;         // kinematic stress at p points (pitch layout); the adjoint writes its result here
+    double* tauy;
+    double* wekp;         // out (k_ekman)
+    double* fcon;         // out, channel: txis, txin (k_ekman_txi)
+    const double* lwek;   // in (k_ekman_adj): adjoint of wekp
+};
+void launch_ekman(const EkmanArgs& a, hipStream_t st);
+void launch_ekman_adj(const EkmanArgs& a, hipStream_t st);
 // Up to six contiguous segments copied in one launch (state snapshots); every
 // length is even and every pointer 16-byte aligned.  With `slots` set, the
 // destinations are offsets doff[] into snapshot slot slots[*base + t] (the

@@ -234,6 +234,26 @@ class QG:
         check(self._lib.oqg_set_forcing(self._h, ent_o[1], n_ent, wek_o[1], float(txis), float(txin), kind),
               self._h, "oqg_set_forcing")
 
+    def set_wind(self, taux, tauy):
+        """ekman_from_tau on the device (ekman.F90:53-83): the resident wekp (and a
+        channel's txis/txin) from the kinematic stress (nyp, nxp) at p points."""
+        tx_o = _as_f64_contig(taux)
+        ty_o = _as_f64_contig(tauy)
+        if tx_o[2] != ty_o[2]:
+            raise ValueError("taux and tauy must both be on the host or both on the device")
+        for o in (tx_o, ty_o):
+            if tuple(o[0].shape) != (self.b.nyp, self.b.nxp):
+                raise ValueError(f"stress shape {tuple(o[0].shape)} != {(self.b.nyp, self.b.nxp)}")
+        check(self._lib.oqg_set_wind(self._h, tx_o[1], ty_o[1], tx_o[2]), self._h, "oqg_set_wind")
+
+    def get_forcing(self):
+        """(wekp (nyp, nxp), txis, txin) as resident on the device."""
+        w = np.empty((self.b.nyp, self.b.nxp))
+        txi = (ctypes.c_double * 2)()
+        check(self._lib.oqg_get_forcing(self._h, w.ctypes.data_as(ctypes.c_void_p), txi, OQG_HOST),
+              self._h, "oqg_get_forcing")
+        return w, float(txi[0]), float(txi[1])
+
     def step(self, tdt: float, ent=None, wekp=None, txis: float = 0.0, txin: float = 0.0):
         """step_qg (core.F90:16).  With ent and wekp None the resident forcing is used."""
         if ent is None and wekp is None:
