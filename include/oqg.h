@@ -207,6 +207,32 @@ int oqg_adj_wind(oqg_ctx* ctx, double* ltaux, double* ltauy, int kind);
  * doubles); either may be NULL. */
 int oqg_get_forcing(oqg_ctx* ctx, double* wekp, double* txi, int kind);
 
+/*
+ * Diagnostics of the resident state, computed on the device: the energetics
+ * and averages of qg_diagno (src/diagnostics/qg_monitor.F90:219-383) that
+ * depend on the state alone, scan_cfl's velocity maxima (:507-569) and a
+ * count of non-finite values of p and q (the check of valids,
+ * src/diagnostics/valsubs.F90:46-139, extended to NaN and Inf).  Per layer k
+ * (nl used) and interface (nl-1 used), in the units of qg_monitor.
+ */
+typedef struct oqg_diag {
+    double ekei[OQG_MAX_LAYERS];    /* 0.5 rho H_k int (u^2 + v^2) dA, u = -dPdy_2bc(p)/f0, v = dPdx_2bc(p)/f0 */
+    double pavg[OQG_MAX_LAYERS];    /* avg_P(p_k) */
+    double qavg[OQG_MAX_LAYERS];    /* avg_P(q_k) */
+    double pmin[OQG_MAX_LAYERS], pmax[OQG_MAX_LAYERS];
+    double qmin[OQG_MAX_LAYERS], qmax[OQG_MAX_LAYERS];
+    double sfmin[OQG_MAX_LAYERS], sfmax[OQG_MAX_LAYERS];   /* transport stream function extrema (Sv) */
+    double circ[OQG_MAX_LAYERS];    /* (p(1,1,k) - p(1,nyp,k))/f0 in Sv */
+    double umax[OQG_MAX_LAYERS], vmax[OQG_MAX_LAYERS];     /* max |u|, |v| on the staggered faces */
+    double cflu[OQG_MAX_LAYERS], cflv[OQG_MAX_LAYERS];     /* umax dt/dx, vmax dt/dx */
+    double etam[OQG_MAX_LAYERS];    /* avg_P(eta_k), eta_k = dz_sign (p_k - p_{k+1})/gp_k */
+    double et2m[OQG_MAX_LAYERS];    /* avg_P(eta_k^2) */
+    double ctot;                    /* sum of circ */
+    double nonfinite;               /* number of non-finite values of p and q */
+} oqg_diag;
+
+int oqg_diagnose(oqg_ctx* ctx, double dt, double rho, oqg_diag* out);
+
 /* The inversion of one step applied to the current q (interior): p from q
  * (modes, Helmholtz solves, mass correction with the current dpi), q's
  * boundary PV by ocqbdy, and qm = q, pm = p.  oqg_adj_invert is its adjoint:

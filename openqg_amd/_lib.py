@@ -40,6 +40,17 @@ class OqgDesc(ctypes.Structure):
     ]
 
 
+_DIAG_LAYER_FIELDS = ("ekei", "pavg", "qavg", "pmin", "pmax", "qmin", "qmax", "sfmin", "sfmax", "circ",
+                      "umax", "vmax", "cflu", "cflv", "etam", "et2m")
+
+
+class OqgDiag(ctypes.Structure):
+    """Mirror of `oqg_diag` (include/oqg.h)."""
+
+    _fields_ = [(n, ctypes.c_double * MAX_LAYERS) for n in _DIAG_LAYER_FIELDS] + \
+               [("ctot", ctypes.c_double), ("nonfinite", ctypes.c_double)]
+
+
 # (name, restype, argtypes) for every entry point of include/oqg.h
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -76,6 +87,7 @@ SIGNATURES = [
     ("oqg_set_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_adj_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_get_forcing", _i, [_vp, _vp, _dp, _i]),
+    ("oqg_diagnose", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
     ("oqg_invert", _i, [_vp]),
     ("oqg_adj_invert", _i, [_vp]),
     ("oqg_snap_alloc", _i, [_vp, _i]),

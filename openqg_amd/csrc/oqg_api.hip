@@ -76,6 +76,7 @@ struct oqg_ctx {
     double* fcon = nullptr;   // txis, txin, layer-2 entrainment flag (device)
     double* taux_d = nullptr;   // kinematic wind stress at p points (oqg_set_wind / oqg_adj_wind)
     double* tauy_d = nullptr;
+    double* diag_part = nullptr;   // per-row diagnostics partials (oqg_diagnose)
     hipStream_t own = nullptr;
     hipStream_t st = nullptr;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -1046,6 +1047,76 @@ int oqg_get_forcing(oqg_ctx* c, double* wekp, double* txi, int kind) {
         txi[1] = f[F_TXIN];
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_diagnose(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
+    if (!c || !out) return OQG_ERR_ARG;
+    const Geo& g = c->g;
+    const int nl = g.nl, ny = g.nyp;
+    if (!c->diag_part)
+        if (int r = dalloc(c, &c->diag_part, (size_t)ny * DG_LEN)) return r;
+    DiagArgs a{};
+    a.g = g;
+    for (int k = 0; k < nl - 1; k++) a.gp[k] = c->d.gp[k];
+    a.dz_sign = c->d.dz_sign;
+    a.bcco = c->d.bcco;
+    a.q = c->qbuf[c->cur];
+    a.p = c->pbuf[c->cur];
+    a.partial = c->diag_part;
+    launch_diag(a, c->st);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h((size_t)ny * DG_LEN);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->diag_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    // the rows in order (deterministic)
+    double sum[5][MAXL] = {}, mn[6][MAXL], bad = 0.0;
+    const int kinds[6] = {DG_PMIN, DG_PMAX, DG_QMIN, DG_QMAX, DG_UMAX, DG_VMAX};
+    for (int t = 0; t < 6; t++)
+        for (int k = 0; k < nl; k++) mn[t][k] = h[kinds[t] * MAXL + k];
+    for (int j = 0; j < ny; j++) {
+        const double* r = h.data() + (size_t)j * DG_LEN;
+        for (int k = 0; k < nl; k++) {
+            sum[0][k] += r[DG_KE * MAXL + k];
+            sum[1][k] += r[DG_P * MAXL + k];
+            sum[2][k] += r[DG_Q * MAXL + k];
+            sum[3][k] += r[DG_ETA * MAXL + k];
+            sum[4][k] += r[DG_ETA2 * MAXL + k];
+            for (int t = 0; t < 6; t++) {
+                const double v = r[kinds[t] * MAXL + k];
+                mn[t][k] = (t == 0 || t == 2) ? std::fmin(mn[t][k], v) : std::fmax(mn[t][k], v);
+            }
+        }
+        bad += r[DG_BAD];
+    }
+    const double dA = g.dx * g.dy, area = ((g.nxp - 1) * g.dx) * ((ny - 1) * g.dy), f0 = g.fnot;
+    std::memset(out, 0, sizeof *out);
+    const double* p_first = h.data() + DG_P0 * MAXL;                      // row 0, column 1
+    const double* p_last = h.data() + (size_t)(ny - 1) * DG_LEN + DG_P0 * MAXL;   // row nyp, column 1
+    for (int k = 0; k < nl; k++) {
+        out->ekei[k] = 0.5 * rho * c->d.h[k] * (sum[0][k] * dA);
+        out->pavg[k] = (sum[1][k] * dA) / area;
+        out->qavg[k] = (sum[2][k] * dA) / area;
+        out->pmin[k] = mn[0][k];
+        out->pmax[k] = mn[1][k];
+        out->qmin[k] = mn[2][k];
+        out->qmax[k] = mn[3][k];
+        out->umax[k] = mn[4][k];
+        out->vmax[k] = mn[5][k];
+        out->cflu[k] = mn[4][k] * dt / g.dx;
+        out->cflv[k] = mn[5][k] * dt / g.dx;
+        out->circ[k] = 1.0e-6 * ((p_first[k] - p_last[k]) / f0);
+        out->ctot += out->circ[k];
+        const double poref = f0 > 0.0 ? p_first[k] : p_last[k];
+        const double lo = std::fmin(mn[0][k] / f0, mn[1][k] / f0), hi = std::fmax(mn[0][k] / f0, mn[1][k] / f0);
+        out->sfmin[k] = 1.0e-6 * (c->d.h[k] * (lo - poref / f0));
+        out->sfmax[k] = 1.0e-6 * (c->d.h[k] * (hi - poref / f0));
+        if (k < nl - 1) {
+            out->etam[k] = (sum[3][k] * dA) / area;
+            out->et2m[k] = (sum[4][k] * dA) / area;
+        }
+    }
+    out->nonfinite = bad;
     return 0;
 }
 

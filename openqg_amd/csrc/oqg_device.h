@@ -145,6 +145,24 @@ __device__ __forceinline__ double wave_sum(double x) {
     return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
 }
 
+// Maximum and minimum over the 64 lanes, by the same moves (fmax/fmin drop a
+// NaN operand; callers count non-finite values separately).
+__device__ __forceinline__ double wave_max(double x) {
+    x = fmax(x, dpp_move<0xB1>(x));
+    x = fmax(x, dpp_move<0x4E>(x));
+    x = fmax(x, dpp_move<0x141>(x));
+    x = fmax(x, dpp_move<0x140>(x));
+    return fmax(fmax(read_lane(x, 0), read_lane(x, 16)), fmax(read_lane(x, 32), read_lane(x, 48)));
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+    x = fmin(x, dpp_move<0xB1>(x));
+    x = fmin(x, dpp_move<0x4E>(x));
+    x = fmin(x, dpp_move<0x141>(x));
+    x = fmin(x, dpp_move<0x140>(x));
+    return fmin(fmin(read_lane(x, 0), read_lane(x, 16)), fmin(read_lane(x, 32), read_lane(x, 48)));
+}
+
 // Deterministic block sum of NV values per thread: wave_sum inside each
 // wavefront, then the wavefront sums in wavefront order.  Every thread
 // returns the same value.  red holds NV*16 doubles; blockDim.x <= 1024.

@@ -258,6 +258,21 @@ struct EkmanArgs {
 };
 void launch_ekman(const EkmanArgs& a, hipStream_t st);
 void launch_ekman_adj(const EkmanArgs& a, hipStream_t st);
+
+// Diagnostics (oqg_diag.hip): per-row partials, MAXL slots per quantity.
+enum { DG_KE = 0, DG_P, DG_Q, DG_ETA, DG_ETA2, DG_PMIN, DG_PMAX, DG_QMIN, DG_QMAX, DG_UMAX, DG_VMAX, DG_P0, DG_NQ };
+constexpr int DG_BAD = DG_NQ * MAXL;    // count of non-finite p and q values
+constexpr int DG_LEN = DG_NQ * MAXL + 1;
+struct DiagArgs {
+    Geo g;
+    double gp[MAXL];
+    int dz_sign;
+    double bcco;
+    const double* q;   // current time level
+    const double* p;
+    double* partial;   // [nyp][DG_LEN]
+};
+void launch_diag(const DiagArgs& a, hipStream_t st);
 // Up to six contiguous segments copied in one launch (state snapshots); every
 // length is even and every pointer 16-byte aligned.  With `slots` set, the
 // destinations are offsets doff[] into snapshot slot slots[*base + t] (the

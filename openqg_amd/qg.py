@@ -246,6 +246,21 @@ class QG:
                 raise ValueError(f"stress shape {tuple(o[0].shape)} != {(self.b.nyp, self.b.nxp)}")
         check(self._lib.oqg_set_wind(self._h, tx_o[1], ty_o[1], tx_o[2]), self._h, "oqg_set_wind")
 
+    def diagnose(self, dt: float) -> dict:
+        """qg_diagno's state energetics and averages, scan_cfl's maxima and the
+        count of non-finite values, computed on the device (oqg_diagnose)."""
+        d = _lib.OqgDiag()
+        check(self._lib.oqg_diagnose(self._h, float(dt), self.rho, ctypes.byref(d)), self._h, "oqg_diagnose")
+        nl = self.b.nl
+        out = {}
+        for name, _ in _lib.OqgDiag._fields_:
+            v = getattr(d, name)
+            if isinstance(v, float):
+                out[name] = v
+            else:
+                out[name] = np.array(v[: nl - 1] if name in ("etam", "et2m") else v[:nl])
+        return out
+
     def get_forcing(self):
         """(wekp (nyp, nxp), txis, txin) as resident on the device."""
         w = np.empty((self.b.nyp, self.b.nxp))

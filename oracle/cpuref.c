@@ -2,7 +2,8 @@
  * cpuref.c -- CPU oracle for OpenQG's QG timestep (step_qg, src/qg/core.F90:16-33).
  *
  * TEST INFRASTRUCTURE ONLY: a plain-C restatement of the reference Fortran,
- * used as the checker for the HIP library and as the timed CPU baseline.
+ * the reference the HIP library's results are compared with, and the timed
+ * CPU baseline.
  * Nothing in the product links or calls this file.
  *
  * Each routine cites the reference file:line it restates.  Whole-array

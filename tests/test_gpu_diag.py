@@ -1,0 +1,58 @@
+"""Device diagnostics (oqg_diagnose) against the numpy restatement of
+qg_diagno / scan_cfl / valids (oracle/monitor.py) on evolved states of a box
+and a channel and on the 513^2 initial state."""
+import numpy as np
+import pytest
+
+from oracle import monitor as M
+from openqg_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SUM_TOL = 1e-12   # sums in a different order than numpy's
+
+
+def _check(qg, w, dt):
+    q, _, p, _ = qg.get_state()
+    d = qg.diagnose(dt)
+    o = M.diagnose(q, p, w.dx, w.dx, w.fnot, qg.bcco, w.cyclic, w.h, w.gp, w.params()["dz_sign"], qg.rho, dt)
+    for name in ("ekei", "pavg", "qavg", "etam", "et2m", "circ", "sfmin", "sfmax"):
+        a, b = np.asarray(d[name]), np.asarray(o[name])
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= SUM_TOL * scale, (name, a, b)
+    for name in ("pmin", "pmax", "qmin", "qmax", "umax", "vmax", "cflu", "cflv"):
+        np.testing.assert_allclose(d[name], o[name], rtol=1e-15, atol=0, err_msg=name)
+    assert abs(d["ctot"] - o["ctot"]) <= 1e-13 * max(abs(o["ctot"]), 1e-300)
+    assert d["nonfinite"] == 0.0
+    return d
+
+
+@pytest.mark.parametrize("make,steps", [(W.small_box, 12), (W.small_channel, 12), (W.c2_box513, 0)],
+                         ids=["box33", "chan33x17", "box513"])
+def test_diagnostics_match_oracle(gpu, make, steps):
+    w = make()
+    qg = w.make_qg()
+    p = w.initial_p()
+    qg.set_state(p=p, pm=p)
+    qg.init_state()
+    ent, wekp, txis, txin = w.forcing()
+    qg.set_forcing(ent, wekp, txis, txin)
+    print(f"[diag] {w.name} after {steps} steps", flush=True)
+    if steps:
+        qg.run(w.tdt, steps, 0, w.supp_every)
+        qg.sync()
+    d = _check(qg, w, w.dt)
+    assert np.all(d["ekei"] > 0)
+
+
+def test_nonfinite_values_are_counted(gpu):
+    w = W.small_box(33, 3)
+    qg = w.make_qg()
+    qg.set_state(p=w.initial_p(), pm=w.initial_p())
+    qg.init_state()
+    q, qm, p, pm = qg.get_state()
+    q[1, 5, 7] = np.nan
+    p[2, 3, 3] = np.inf
+    p[0, 0, 0] = -np.inf
+    qg.set_state(q, qm, p, pm)
+    assert qg.diagnose(w.dt)["nonfinite"] == 3.0
