@@ -107,8 +107,12 @@ def ensemble_bench(members: int, days: float):
     """The uneven ensemble (grids 65^2..513^2, 2-4 layers, seeded wind, viscosity
     and PV noise; each member its stable time step) for `days` simulated days
     per member: batched (oqg_ens_*, launch schedule picked by the tuner) against
-    the same members advanced one after another by the plain graph path.  Both
-    from the same initial states, resident forcing, suppression every 25 steps."""
+    the same members advanced one after another by the plain graph path, each
+    finished before the next starts; also against the plain runs all enqueued
+    at once (each member on its own stream, overlapping on the GPU) and
+    against one eager step_qg call per member step (a 25-step sample of each
+    member, scaled to its step count).  All from the same initial states,
+    resident forcing, suppression every 25 steps."""
     from openqg_amd import workloads as W
     from openqg_amd.ensemble import Ensemble
 
@@ -131,28 +135,36 @@ def ensemble_bench(members: int, days: float):
     tdt = [w.tdt for w in ws]
     nsteps = [w.nsteps for w in ws]
     sim_days = sum(n * w.dt for n, w in zip(nsteps, ws)) / 86400.0
-    # plain path, one member after another (each warmed up: graph captured)
+    # plain path, one member after another, each finished before the next
+    # starts (each warmed up: graph captured)
     for w, qg in zip(ws, qgs):
         qg.run(w.tdt, 2 * w.supp_every, 0, w.supp_every)
         qg.sync()
     t0 = time.perf_counter()
     for w, qg, n in zip(ws, qgs, nsteps):
         qg.run(w.tdt, n, 2 * w.supp_every, w.supp_every)
-    for qg in qgs:
         qg.sync()
     seq = time.perf_counter() - t0
+    # the same runs enqueued without waiting: every member on its own stream,
+    # so the GPU overlaps them as it can
+    t0 = time.perf_counter()
+    for w, qg, n in zip(ws, qgs, nsteps):
+        qg.run(w.tdt, n, 2 * w.supp_every + n, w.supp_every)
+    for qg in qgs:
+        qg.sync()
+    conc = time.perf_counter() - t0
     # the drop-in call pattern: one step_qg call per member step (eager
     # launches), timed on a 25-step sample of every member and scaled
     sample = 25
-    t0 = time.perf_counter()
-    for w, qg in zip(ws, qgs):
+    eager = 0.0
+    for w, qg, n in zip(ws, qgs, nsteps):
+        t0 = time.perf_counter()
         for nt in range(1, sample + 1):
             qg.step(w.tdt)
             if (nt - 1) % w.supp_every == 0:
                 qg.suppress()
-    for qg in qgs:
         qg.sync()
-    eager = (time.perf_counter() - t0) * sum(nsteps) / (sample * len(ws))
+        eager += (time.perf_counter() - t0) * n / sample
     # batched: tune the launch schedule, warm up (plans and graphs), time
     ens = Ensemble(qgs)
     sched, tune_ms = ens.tune(tdt, trial_steps=2 * 25, supp_every=25)
@@ -172,12 +184,15 @@ def ensemble_bench(members: int, days: float):
         "member_steps": int(sum(nsteps)),
         "value": round(sim_days / bat, 2),
         "sequential_value": round(sim_days / seq, 2),
+        "concurrent_streams_value": round(sim_days / conc, 2),
         "sequential_per_call_value": round(sim_days / eager, 2),
         "unit": "aggregate simulated days/s",
         "speedup_vs_sequential": round(seq / bat, 2),
+        "speedup_vs_concurrent_streams": round(conc / bat, 2),
         "speedup_vs_sequential_per_call": round(eager / bat, 2),
         "batched_s": round(bat, 4),
         "sequential_s": round(seq, 4),
+        "concurrent_streams_s": round(conc, 4),
         "sequential_per_call_s": round(eager, 4),
         "schedule": sched,
         "tune_ms": [round(t, 3) for t in tune_ms],
