@@ -574,15 +574,20 @@ static int setup(oqg_ctx* c) {
     s.a = 1.0 / (d.dy * d.dy);
     s.ftnorm = 1.0 / Nref;
     {
-        int m = s.L;
-        s.nstages = 0;
-        while (m % 4 == 0) { s.radix[s.nstages++] = 4; m /= 4; }
-        while (m % 2 == 0) { s.radix[s.nstages++] = 2; m /= 2; }
-        while (m % 3 == 0) { s.radix[s.nstages++] = 3; m /= 3; }
-        while (m % 5 == 0) { s.radix[s.nstages++] = 5; m /= 5; }
-        if (m != 1) return fail(c, OQG_ERR_ARG, "x transform length must factor into 2, 3 and 5");
-        for (int st = 0; st < s.nstages; st++)   // each thread holds at most 4 butterflies per stage
-            if (s.L / s.radix[st] > 4 * 256) return fail(c, OQG_ERR_ARG, "x transform too long for the LDS FFT");
+        // the fewest LDS passes within each kernel's largest butterfly
+        auto plan = [&](int maxr, int& nst, int* radix) {
+            int m = s.L;
+            nst = 0;
+            for (int r : {16, 8, 4, 2, 3, 5})
+                if (r <= maxr || r == 2 || r == 3 || r == 5)
+                    while (m % r == 0) { radix[nst++] = r; m /= r; }
+            if (m != 1) return fail(c, OQG_ERR_ARG, "x transform length must factor into 2, 3 and 5");
+            for (int st = 0; st < nst; st++)   // each thread holds at most 4 butterflies per stage
+                if (s.L / radix[st] > 4 * 256) return fail(c, OQG_ERR_ARG, "x transform too long for the LDS FFT");
+            return 0;
+        };
+        if (int r = plan(FWD_MAX_RADIX, s.nstages, s.radix)) return r;
+        if (int r = plan(INV_MAX_RADIX, s.instages, s.iradix)) return r;
     }
     if (inv_lds_bytes(s, g, nl) > MAX_DYN_LDS || fwd_lds_bytes(s) > MAX_DYN_LDS || ysolve_lds_bytes(s) > MAX_DYN_LDS)
         return fail(c, OQG_ERR_ARG, "grid too large for the LDS-staged transforms");
@@ -2113,12 +2118,12 @@ static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::ve
     for (int i = 0; i < n; i++) rmax = std::max(rmax, e->m[P->members[i]]->s.R);
     for (int i = 0; i < n; i++) {
         const oqg_ctx* c = e->m[P->members[i]];
-        const std::pair<int, int> key(c->g.nl, fft_mb(c->s));
+        const std::pair<int, int> key(c->g.nl, fft_mb(c->s, false)), ikey(c->g.nl, fft_mb(c->s, true));
         auto& f = fwd[key];
         const int np = inv_block_count(c->s);
         for (int b = 0; b < np * c->g.nl; b++) f.first.push_back(make_int2(i, b));
         f.second = std::max(f.second, fwd_lds_bytes(c->s));
-        auto& v = inv[key];
+        auto& v = inv[ikey];
         for (int b = 0; b < np; b++) v.first.push_back(make_int2(i, b));
         v.second = std::max(v.second, inv_lds_bytes(c->s, c->g, c->g.nl));
         auto& y = ys[(e->schedule & OQG_ENS_ONE_YSOLVE) ? rmax : c->s.R];

@@ -224,9 +224,66 @@ __device__ void block_sum(double (&v)[NV], double* red) {
 // Entries of the LDS twiddle table of a length-L transform.
 __host__ __device__ inline int twiddle_table_len(int L) { return L; }
 
+// exp(-2 pi i m / 16) for the forward transform (SIGN < 0), its conjugate for
+// the backward one, m = 0..9: the twiddles inside a radix-16 butterfly.
+__device__ __forceinline__ double2 w16(int m, int sign) {
+    constexpr double C[10] = {1.0, 0.92387953251128674, 0.70710678118654752, 0.38268343236508977, 0.0,
+                              -0.38268343236508977, -0.70710678118654752, -0.92387953251128674, -1.0,
+                              -0.92387953251128674};
+    constexpr double S[10] = {0.0, 0.38268343236508977, 0.70710678118654752, 0.92387953251128674, 1.0,
+                              0.92387953251128674, 0.70710678118654752, 0.38268343236508977, 0.0,
+                              -0.38268343236508977};
+    return make_double2(C[m], sign < 0 ? -S[m] : S[m]);
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, int L);
+
+// Radix 8 as two radix-4 DFTs of the even and odd inputs and one twiddle
+// layer: X[k] = E[k] + W8^k O[k], X[k+4] = E[k] - W8^k O[k].
+template <int SIGN>
+__device__ __forceinline__ void dft8(double2 (&v)[8]) {
+    double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<4, SIGN>(e, nullptr, 4);
+    dft_small<4, SIGN>(o, nullptr, 4);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double2 t = (k == 0) ? o[0] : cmul(o[k], w16(2 * k, SIGN));
+        v[k] = make_double2(e[k].x + t.x, e[k].y + t.y);
+        v[k + 4] = make_double2(e[k].x - t.x, e[k].y - t.y);
+    }
+}
+
+// Radix 16 as 4 x 4: Y_n2 = DFT4 of x[4 n1 + n2] over n1, Z_n2[k1] =
+// W16^(n2 k1) Y_n2[k1], X[k1 + 4 k2] = DFT4 of Z_n2[k1] over n2.
+template <int SIGN>
+__device__ __forceinline__ void dft16(double2 (&v)[16]) {
+    double2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+#pragma unroll
+        for (int n1 = 0; n1 < 4; n1++) y[n2][n1] = v[4 * n1 + n2];
+        dft_small<4, SIGN>(y[n2], nullptr, 4);
+#pragma unroll
+        for (int k1 = 1; k1 < 4; k1++)
+            if (n2 > 0) y[n2][k1] = cmul(y[n2][k1], w16(n2 * k1, SIGN));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        double2 z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+        dft_small<4, SIGN>(z, nullptr, 4);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = z[k2];
+    }
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, int L) {
-    if constexpr (R == 2) {
+    if constexpr (R == 8) {
+        dft8<SIGN>(v);
+    } else if constexpr (R == 16) {
+        dft16<SIGN>(v);
+    } else if constexpr (R == 2) {
         const double2 a = v[0], b = v[1];
         v[0] = make_double2(a.x + b.x, a.y + b.y);
         v[1] = make_double2(a.x - b.x, a.y - b.y);
@@ -300,12 +357,13 @@ __device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tste
     __syncthreads();
 }
 
-// In-place transform of buf (length s.L) with the LDS twiddle table tw;
-// gt = thread index within its group; MB >= the most butterflies a thread owns
-// in any stage.  Every thread of the block must call this (it contains block
-// barriers).
-template <int MB, int SIGN>
-__device__ __forceinline__ void fft_lds(double2* buf, const Spec& s, const double2* tw, int gt) {
+// In-place transform of buf (length L) with the LDS twiddle table tw, by the
+// plan radix[0..nst) whose radices are all <= MAXR (a larger one is never
+// instantiated, so it costs the kernel no registers); gt = thread index within
+// its group; MB >= the most butterflies a thread owns in any stage.  Every
+// thread of the block must call this (it contains block barriers).
+template <int MB, int SIGN, int MAXR>
+__device__ __forceinline__ void fft_lds(double2* buf, int L, int nst, const int* radix, const double2* tw, int gt) {
     int kk[MB], jj[MB];
 #pragma unroll
     for (int b = 0; b < MB; b++) {
@@ -313,9 +371,15 @@ __device__ __forceinline__ void fft_lds(double2* buf, const Spec& s, const doubl
         jj[b] = gt + b * FFT_GROUP;
     }
     int Ns = 1;
-    for (int st = 0; st < s.nstages; st++) {
-        const int R = s.radix[st];
-        const int LR = s.L / R, tstep = s.L / (Ns * R);   // uniform (scalar) arithmetic
+    for (int st = 0; st < nst; st++) {
+        const int R = radix[st];
+        const int LR = L / R, tstep = L / (Ns * R);   // uniform (scalar) arithmetic
+        if constexpr (MAXR >= 16) {
+            if (R == 16) { fft_stage<16, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); Ns *= R; continue; }
+        }
+        if constexpr (MAXR >= 8) {
+            if (R == 8) { fft_stage<8, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); Ns *= R; continue; }
+        }
         switch (R) {
             case 4: fft_stage<4, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
             case 2: fft_stage<2, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;

@@ -14,6 +14,8 @@ namespace oqg {
 constexpr int MAXL = OQG_MAX_LAYERS;
 constexpr int WAVE = 64;
 constexpr int MAX_RADIX_STAGES = 16;
+constexpr int FWD_MAX_RADIX = 16;    // radix-16/8 butterflies: fewest LDS passes
+constexpr int INV_MAX_RADIX = 4;     // the inverse kernel's register budget keeps radix 4
 constexpr int MAX_FFT = 4096;        // longest x transform (complex points)
 constexpr int VORT_TX = 64;          // vorticity tile (x)
 constexpr int VORT_TY = 16;          // vorticity tile (y)
@@ -41,8 +43,10 @@ struct Spec {
     int kp;         // spectral row pitch (doubles), multiple of 16
     int R;          // rows per lane in the wavefront y-solver
     int nypad;      // 64*R
-    int nstages;
+    int nstages;    // x transform plan of the forward (spectral) kernels
     int radix[MAX_RADIX_STAGES];
+    int instages;   // plan of the inverse kernels, radix <= INV_MAX_RADIX
+    int iradix[MAX_RADIX_STAGES];
     double a;       // 1/dy^2 (inhomog.F90:127)
     double ftnorm;  // 1/N (inhomog.F90:124)
 };
@@ -311,7 +315,7 @@ void launch_suppress(const SuppArgs& a, hipStream_t st);
 void launch_qcomp(const QcompArgs& a, hipStream_t st);
 int inv_block_count(const Spec& s);
 int ysolve_blocks(const Spec& s, int nmodes);
-int fft_mb(const Spec& s);   // butterflies per thread of the x transform plan (1, 2 or 4)
+int fft_mb(const Spec& s, bool inverse);   // butterflies per thread of a plan (1, 2 or 4)
 size_t fwd_lds_bytes(const Spec& s);
 size_t ysolve_lds_bytes(const Spec& s);
 size_t inv_lds_bytes(const Spec& s, const Geo& g, int nmodes);

@@ -179,7 +179,7 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         }
     }
     __syncthreads();
-    fft_lds<MB, -1>(zbuf, s, tw, threadIdx.x);
+    fft_lds<MB, -1, FWD_MAX_RADIX>(zbuf, s.L, s.nstages, s.radix, tw, threadIdx.x);
     double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
     double* row1 = row0 + s.kp;
     if (!g.cyclic) {
@@ -232,10 +232,12 @@ __global__ __launch_bounds__(256) void k_fwd_ens(const FwdArgs* __restrict__ arg
         default: break;                                                       \
     }
 
-int fft_mb(const Spec& s) {
+int fft_mb(const Spec& s, bool inverse) {
+    const int nst = inverse ? s.instages : s.nstages;
+    const int* radix = inverse ? s.iradix : s.radix;
     int mx = 1;
-    for (int st = 0; st < s.nstages; st++) {
-        const int b = (s.L / s.radix[st] + FFT_GROUP - 1) / FFT_GROUP;
+    for (int st = 0; st < nst; st++) {
+        const int b = (s.L / radix[st] + FFT_GROUP - 1) / FFT_GROUP;
         if (b > mx) mx = b;
     }
     return mx <= 1 ? 1 : (mx <= 2 ? 2 : 4);
@@ -247,7 +249,7 @@ void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
     const int npairs = (a.s.nyi + 1) / 2;
     const size_t lds = fwd_lds_bytes(a.s);
 #define FWD_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd<NLc, MBc>), dim3(npairs, nmodes), dim3(256), lds, st, a)
-    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), FWD_CALL)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s, false), FWD_CALL)
 #undef FWD_CALL
 }
 
@@ -255,7 +257,7 @@ void launch_fwd_adj(const FwdArgs& a, hipStream_t st) {
     const int npairs = (a.s.nyi + 1) / 2;
     const size_t lds = fwd_lds_bytes(a.s);
 #define FWDA_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd_adj<NLc, MBc>), dim3(npairs, a.g.nl), dim3(256), lds, st, a)
-    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), FWDA_CALL)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s, false), FWDA_CALL)
 #undef FWDA_CALL
 }
 
@@ -872,8 +874,8 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
             }
         }
         __syncthreads();
-        if (g.cyclic) fft_lds<MB, +1>(z, s, tw, gt);   // c2r (inhomog.F90:247)
-        else fft_lds<MB, -1>(z, s, tw, gt);
+        if (g.cyclic) fft_lds<MB, +1, INV_MAX_RADIX>(z, s.L, s.instages, s.iradix, tw, gt);   // c2r (inhomog.F90:247)
+        else fft_lds<MB, -1, INV_MAX_RADIX>(z, s.L, s.instages, s.iradix, tw, gt);
         if (valid) {
             double* o0 = inh + (size_t)mi * 2 * g.pitch;
             double* o1 = o0 + g.pitch;
@@ -1053,7 +1055,7 @@ void launch_inv(const InvArgs& a, hipStream_t st) {
     const dim3 grid(inv_block_count(a.s)), block(FFT_GROUP * G);
     const size_t lds = inv_lds_bytes(a.s, a.g, nm);
 #define INV_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv<NLc, MBc>), grid, block, lds, st, a)
-    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), INV_CALL)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s, true), INV_CALL)
 #undef INV_CALL
 }
 
@@ -1061,7 +1063,7 @@ void launch_inv_adj(const InvArgs& a, hipStream_t st) {
     const dim3 grid(inv_block_count(a.s)), block(FFT_GROUP * inv_group_count(a.g.nl));
     const size_t lds = inv_lds_bytes(a.s, a.g, a.g.nl);
 #define INVA_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv_adj<NLc, MBc>), grid, block, lds, st, a)
-    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s), INVA_CALL)
+    OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s, true), INVA_CALL)
 #undef INVA_CALL
 }
 
