@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
         s_n2[idx] = v;
     }
     __syncthreads();
-    const double c2 = a.ph.c2[k], c4 = a.ph.c4[k];
+    const double c2 = a.ph->c2[k], c4 = a.ph->c4[k];
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
         const int r = ty + 4 * u, gj = j0 + r;
@@ -297,12 +297,12 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
         const double nu2 = s_n2[(r + 1) * W1 + tx + 1];
         const double nu1 = s_n1[(r + 2) * W2 + tx + 2];
         double lpm = c2 * nu2 - c4 * nu3;
-        if (k == g.kt) lpm -= a.ph.cdrag * nu1;
+        if (k == g.kt) lpm -= a.ph->cdrag * nu1;
         a.lq_n[o] = olq[u] + g.jfac * jq;
         a.lp_n[o] = olp[u] - g.jfac * jp;
         a.lp_nm1[o] = lpm;
         const double mu = s_mu[(r + 3) * W3 + tx + 3];
-        if (k == 0) a.lwek[(long long)gj * g.pitch + gi] = olw[u] + a.ph.cwek * mu;
+        if (k == 0) a.lwek[(long long)gj * g.pitch + gi] = olw[u] + a.ph->cwek * mu;
         // lq_{n-1}: the interior of lq_{n+1} (unchanged), 0 on the ring
         if (gi == 0 || gi == g.nxp - 1 || gj == 0 || gj == g.nyp - 1) a.lq_np1[o] = 0.0;
     }

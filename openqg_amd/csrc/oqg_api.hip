@@ -48,6 +48,8 @@ struct oqg_ctx {
     CycHom ch{};
     int variant = 0;       // y-solver: 0 wavefront partition+PCR, 1 per-thread Thomas
     int cur = 0;           // time-level parity
+    double* hst = nullptr; // pinned staging of host field transfers (hst_words doubles)
+    size_t hst_words = 0;
     int tiles_x = 0, tiles_y = 0;
     size_t fbytes = 0;     // bytes of one nl-layer field (pitch layout)
     // device memory
@@ -337,17 +339,45 @@ static int host_lu(oqg_ctx* c, int n) {
 
 // ------------------------------------------------------------------ copies
 
+// Host fields move through a pinned staging buffer: a pitched copy between
+// pageable memory and the device goes row by row (milliseconds per field),
+// one between pinned memory and the device is a single DMA.  Host transfers
+// are synchronous (the staging buffer is free again on return).
+static int host_stage(oqg_ctx* c, size_t words) {
+    if (c->hst_words >= words) return 0;
+    if (c->hst) HIPCHK(c, hipHostFree(c->hst));
+    c->hst = nullptr;
+    c->hst_words = 0;
+    HIPCHK(c, hipHostMalloc((void**)&c->hst, words * sizeof(double), hipHostMallocDefault));
+    c->hst_words = words;
+    return 0;
+}
+
 static int upload_field(oqg_ctx* c, double* dst, const double* src, int nlayers, int kind) {
-    const size_t w = (size_t)c->g.nxp * sizeof(double);
-    HIPCHK(c, hipMemcpy2DAsync(dst, (size_t)c->g.pitch * sizeof(double), src, w, w, (size_t)nlayers * c->g.nyp,
-                               kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+    const size_t w = (size_t)c->g.nxp * sizeof(double), rows = (size_t)nlayers * c->g.nyp;
+    const size_t dp = (size_t)c->g.pitch * sizeof(double);
+    if (kind == OQG_DEVICE) {
+        HIPCHK(c, hipMemcpy2DAsync(dst, dp, src, w, w, rows, hipMemcpyDeviceToDevice, c->st));
+        return 0;
+    }
+    if (int r = host_stage(c, rows * c->g.nxp)) return r;
+    std::memcpy(c->hst, src, rows * w);
+    HIPCHK(c, hipMemcpy2DAsync(dst, dp, c->hst, w, w, rows, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
 
 static int download_field(oqg_ctx* c, double* dst, const double* src, int nlayers, int kind) {
-    const size_t w = (size_t)c->g.nxp * sizeof(double);
-    HIPCHK(c, hipMemcpy2DAsync(dst, w, src, (size_t)c->g.pitch * sizeof(double), w, (size_t)nlayers * c->g.nyp,
-                               kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->st));
+    const size_t w = (size_t)c->g.nxp * sizeof(double), rows = (size_t)nlayers * c->g.nyp;
+    const size_t sp = (size_t)c->g.pitch * sizeof(double);
+    if (kind == OQG_DEVICE) {
+        HIPCHK(c, hipMemcpy2DAsync(dst, w, src, sp, w, rows, hipMemcpyDeviceToDevice, c->st));
+        return 0;
+    }
+    if (int r = host_stage(c, rows * c->g.nxp)) return r;
+    HIPCHK(c, hipMemcpy2DAsync(c->hst, w, src, sp, w, rows, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::memcpy(dst, c->hst, rows * w);
     return 0;
 }
 
@@ -357,7 +387,7 @@ static void fill_inv_common(oqg_ctx* c, InvArgs& ia) {
     std::memset(&ia, 0, sizeof ia);
     ia.g = c->g;
     ia.s = c->s;
-    ia.ph = c->ph;
+    ia.ph = &c->dconst->ph;
     ia.dc = c->dconst;
     ia.spec = c->spec;
     ia.tw = c->tw;
@@ -420,7 +450,7 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     std::memset(&a, 0, sizeof a);
     VortArgs& va = a.va;
     va.g = c->g;
-    va.ph = c->ph;
+    va.ph = &c->dconst->ph;
     va.q = c->qbuf[cur];
     va.p = c->pbuf[cur];
     va.pm = c->pbuf[nxt];
@@ -434,7 +464,7 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     va.org_x = vort_org_x(c->g);
     CycArgs& ca = a.ca;
     ca.g = c->g;
-    ca.ph = c->ph;
+    ca.ph = &c->dconst->ph;
     ca.q = c->qbuf[cur];
     ca.p = c->pbuf[cur];
     ca.pm = c->pbuf[nxt];
@@ -643,6 +673,7 @@ static int setup(oqg_ctx* c) {
         h.mx = c->mx;
         h.bh = c->bh;
         h.ch = c->ch;
+        h.ph = c->ph;
         HIPCHK(c, hipMemcpyAsync(c->dconst, &h, sizeof h, hipMemcpyHostToDevice, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         return 0;
@@ -831,6 +862,7 @@ void oqg_destroy(oqg_ctx* c) {
     if (c->st) hipStreamSynchronize(c->st);
     for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (void* p : c->allocs) hipFree(p);
+    if (c->hst) hipHostFree(c->hst);
     if (c->own) hipStreamDestroy(c->own);
     delete c;
 }
@@ -1462,7 +1494,7 @@ static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed, const SnapR
     if (int r = enqueue_adj_inversion(c, c->lq[cn], c->lp[cn])) return r;
     AdjVortArgs v{};
     v.g = c->g;
-    v.ph = c->ph;
+    v.ph = &c->dconst->ph;
     v.q = src.q ? src.q : c->qbuf[cur];
     v.p = src.p ? src.p : c->pbuf[cur];
     v.slots = src.slots;

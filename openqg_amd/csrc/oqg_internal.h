@@ -98,7 +98,7 @@ constexpr int A_AJIS = 0, A_AJIN = MAXL, A_AP3S = 2 * MAXL, A_AP3N = 3 * MAXL, A
 // Arguments of the vorticity kernel.
 struct VortArgs {
     Geo g;
-    Phys ph;
+    const Phys* ph;   // DevConsts::ph (kept out of the kernel arguments)
     const double* q;      // current q
     const double* p;      // current p
     const double* pm;     // previous p
@@ -124,6 +124,7 @@ struct DevConsts {
     Mix mx;
     BoxHom bh;
     CycHom ch;
+    Phys ph;
 };
 
 struct FwdArgs {
@@ -160,7 +161,7 @@ struct YArgs {
 struct InvArgs {
     Geo g;
     Spec s;
-    Phys ph;
+    const Phys* ph;   // DevConsts::ph (kept out of the kernel arguments)
     const DevConsts* dc;    // device copy of the precompute blocks
     const double* spec;
     const double* tw;
@@ -192,7 +193,7 @@ struct InvArgs {
 
 struct CycArgs {
     Geo g;
-    Phys ph;
+    const Phys* ph;   // DevConsts::ph (kept out of the kernel arguments)
     const double* q;
     const double* p;
     const double* pm;
@@ -232,7 +233,7 @@ struct AdjSuppArgs {
 
 struct AdjVortArgs {
     Geo g;
-    Phys ph;
+    const Phys* ph;   // DevConsts::ph (kept out of the kernel arguments)
     const double* q;    // forward state before the step: q_n, p_n
     const double* p;
     double* lq_np1;     // in: adjoint of q_{n+1}; out: adjoint of q_{n-1} (ring zeroed)

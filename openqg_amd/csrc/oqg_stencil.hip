@@ -184,7 +184,7 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
         }
     }
     __syncthreads();
-    const double c2 = a.ph.c2[k], c4 = a.ph.c4[k];
+    const double c2 = a.ph->c2[k], c4 = a.ph->c4[k];
     double epart = 0.0;
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
@@ -205,7 +205,7 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
         dqdt = dqdt + c2 * d4 - c4 * d6;
         if (k == 0) {
             const double e = rent[u];
-            dqdt = dqdt + a.ph.cent1 * e + a.ph.cwek * rwek[u];
+            dqdt = dqdt + a.ph->cent1 * e + a.ph->cwek * rwek[u];
             if constexpr (EDGE) {
                 // interior points weigh 1; in a channel the columns 0 and nxp-1 weigh 1/2
                 const double w = (gi == 0 || gi == g.nxp - 1) ? 0.5 : 1.0;
@@ -214,8 +214,8 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
                 epart += e;
             }
         }
-        if (k == 1) dqdt = dqdt - a.ph.cent2 * rent[u];
-        if (k == g.kt) dqdt = dqdt - a.ph.cdrag * d2;
+        if (k == 1) dqdt = dqdt - a.ph->cent2 * rent[u];
+        if (k == g.kt) dqdt = dqdt - a.ph->cdrag * d2;
         if (out) a.qnew[lo + (long long)gj * g.pitch + gi] = rqm[u] + a.tdt * dqdt;
     }
     if (EDGE && k == 0) epart += ring_w * ring_e;
@@ -349,25 +349,25 @@ __global__ __launch_bounds__(256) void k_cyc(CycArgs a) {
     if (threadIdx.x == 0) {
         const double dx = g.dx, dy = g.dy;
         double* o = a.aux;
-        o[A_AP3S + k] = a.ph.ah2[k] * (v[0] * dx) / dy;
-        o[A_AP3N + k] = a.ph.ah2[k] * (v[1] * dx) / dy;
-        o[A_AP5S + k] = a.ph.ah4[k] * (v[2] * dx) / dy;
-        o[A_AP5N + k] = a.ph.ah4[k] * (v[3] * dx) / dy;
+        o[A_AP3S + k] = a.ph->ah2[k] * (v[0] * dx) / dy;
+        o[A_AP3N + k] = a.ph->ah2[k] * (v[1] * dx) / dy;
+        o[A_AP5S + k] = a.ph->ah4[k] * (v[2] * dx) / dy;
+        o[A_AP5N + k] = a.ph->ah4[k] * (v[3] * dx) / dy;
         const double aj5s = (v[4] * dx) / (2.0 * dx), aj9s = (v[5] * dx) / (2.0 * dx);
         const double aj5n = (v[6] * dx) / (2.0 * dx), aj9n = (v[7] * dx) / (2.0 * dx);
         o[A_AJIS + k] = (aj5s + 2.0 * aj9s) / 6.0;
         o[A_AJIN + k] = -(aj5n + 2.0 * aj9n) / 6.0;
         if (k == g.kt) {
-            o[A_BDRS] = a.ph.cd_bdr * ((v[8] * dx) / dy);
-            o[A_BDRN] = a.ph.cd_bdr * ((v[9] * dx) / dy);
+            o[A_BDRS] = a.ph->cd_bdr * ((v[8] * dx) / dy);
+            o[A_BDRN] = a.ph->cd_bdr * ((v[9] * dx) / dy);
         }
         if (k == 0) {
-            const double dz = (double)a.ph.dz_sign;
+            const double dz = (double)a.ph->dz_sign;
             for (int l = 0; l < MAXL; l++) { o[A_BES + l] = 0.0; o[A_BEN + l] = 0.0; }
-            o[A_BES + 0] = dz * a.ph.fohfac[0] * (e[0] * dx);
-            o[A_BES + 1] = dz * a.ph.fohfac[0] * (e[2] * dx);
-            o[A_BEN + 0] = -dz * a.ph.fohfac[1] * (e[1] * dx);
-            o[A_BEN + 1] = -dz * a.ph.fohfac[1] * (e[3] * dx);
+            o[A_BES + 0] = dz * a.ph->fohfac[0] * (e[0] * dx);
+            o[A_BES + 1] = dz * a.ph->fohfac[0] * (e[2] * dx);
+            o[A_BEN + 0] = -dz * a.ph->fohfac[1] * (e[1] * dx);
+            o[A_BEN + 1] = -dz * a.ph->fohfac[1] * (e[3] * dx);
             o[A_TXIS] = a.fcon[F_TXIS];
             o[A_TXIN] = a.fcon[F_TXIN];
         }

@@ -561,7 +561,7 @@ __device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom&
     for (int l = 0; l < NL - 1; l++) {
         const double e = (l == 0) ? exn : 0.0;
         const double old = dpi[l];
-        dpi[l] = dpip[l] - a.tdt * a.ph.gp[l] * e;
+        dpi[l] = dpip[l] - a.tdt * a.ph->gp[l] * e;
         dpip[l] = old;
     }
     if (!g.cyclic) {
@@ -590,10 +590,10 @@ __device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom&
             rhss[k] = -0.5 * dy * f0 * x[A_BES + k];
             rhsn[k] = -0.5 * dy * f0 * x[A_BEN + k];
         }
-        rhss[0] = rhss[0] + a.ph.tau_sign * (f0 / a.ph.h[0]) * x[A_TXIS];
-        rhsn[0] = rhsn[0] - a.ph.tau_sign * (f0 / a.ph.h[0]) * x[A_TXIN];
-        rhss[NL - 1] = rhss[NL - 1] + (f0 / a.ph.h[NL - 1]) * x[A_BDRS];
-        rhsn[NL - 1] = rhsn[NL - 1] - (f0 / a.ph.h[NL - 1]) * x[A_BDRN];
+        rhss[0] = rhss[0] + a.ph->tau_sign * (f0 / a.ph->h[0]) * x[A_TXIS];
+        rhsn[0] = rhsn[0] - a.ph->tau_sign * (f0 / a.ph->h[0]) * x[A_TXIN];
+        rhss[NL - 1] = rhss[NL - 1] + (f0 / a.ph->h[NL - 1]) * x[A_BDRS];
+        rhsn[NL - 1] = rhsn[NL - 1] - (f0 / a.ph->h[NL - 1]) * x[A_BDRN];
 #pragma unroll
         for (int k = 0; k < NL; k++) {
             rhss[k] = rhss[k] + x[A_AJIS + k] - x[A_AP3S + k] + x[A_AP5S + k];
@@ -684,7 +684,7 @@ __device__ __forceinline__ void qbdy(const InvArgs& a, const double* samat, int 
     const int reps = (kind == 2) ? 2 : 1;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
-        double qv = a.ph.qbfac * (pin[k] - pb[k]);
+        double qv = a.ph->qbfac * (pin[k] - pb[k]);
         double ap = 0.0;
 #pragma unroll
         for (int l = 0; l < NL; l++) ap += samat[k * MAXL + l] * pb[l];
