@@ -107,6 +107,7 @@ struct oqg_ctx {
     std::vector<std::array<int, 4>> rv_left;
     std::array<int, 3> rv_tail{};
     int rv_n = -1, rv_supp = 0;
+    int rv_loaded = -1;   // reverse sweep: the slot the loaded state equals, or -1
     double rv_tdt = 0.0;
     long long rv_steps = 0;
     std::map<std::pair<int, int>, int> rv_split;   // memo of revolve_split
@@ -1901,7 +1902,25 @@ static int adj_step_at(oqg_ctx* c, double tdt, bool suppressed, int slot) {
 static int rv_advance(oqg_ctx* c, int lo, int hi) {
     if (hi <= lo) return 0;
     c->rv_steps += hi - lo;
+    c->rv_loaded = -1;
     return oqg_run(c, c->rv_tdt, hi - lo, lo, c->rv_supp);
+}
+
+// Checkpoint copies of the reverse sweep.  rv_loaded is the slot whose content
+// the loaded state still equals (adjoint steps leave the forward state alone),
+// so restoring the checkpoint just taken costs nothing.
+static void rv_save(oqg_ctx* c, int slot) {
+    snap_copy(c, slot, true);
+    c->rv_loaded = slot;
+}
+
+static void rv_restore(oqg_ctx* c, int slot) {
+    if (c->rv_loaded == slot) {
+        c->cur = c->snap_cur[slot];
+        return;
+    }
+    snap_copy(c, slot, false);
+    c->rv_loaded = slot;
 }
 
 // The state lo is in `slot`; the adjoint holds that of state hi; leave it at state lo.
@@ -1920,16 +1939,16 @@ static int rv_reverse(oqg_ctx* c, int lo, int hi, int slot, int free) {
                 if (int r = adj_step_at(c, tdt, suppressed_at(k + 1, supp), slot)) return r;
                 continue;
             }
-            snap_copy(c, slot, false);
+            rv_restore(c, slot);
             if (int r = rv_advance(c, lo, k)) return r;
             if (int r = adj_step_at(c, tdt, suppressed_at(k + 1, supp), -1)) return r;
         }
         return 0;
     }
     const int m = lo + rev_split(c, n, free);
-    snap_copy(c, slot, false);
+    rv_restore(c, slot);
     if (int r = rv_advance(c, lo, m)) return r;
-    snap_copy(c, slot + 1, true);
+    rv_save(c, slot + 1);
     if (int r = rv_reverse(c, m, hi, slot + 1, free - 1)) return r;
     return rv_reverse(c, lo, m, slot, free);
 }
@@ -1976,6 +1995,7 @@ int oqg_adj_revolve_reverse(oqg_ctx* c, long long* forward_steps) {
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     if (c->rv_n < 1) return fail(c, OQG_ERR_STATE, "oqg_adj_revolve_reverse: no forward sweep recorded");
+    c->rv_loaded = -1;   // the caller may have changed the state since the forward sweep
     if (int r = rv_reverse(c, c->rv_tail[0], c->rv_n, c->rv_tail[1], c->rv_tail[2])) return r;
     for (auto it = c->rv_left.rbegin(); it != c->rv_left.rend(); ++it)
         if (int r = rv_reverse(c, (*it)[0], (*it)[1], (*it)[2], (*it)[3])) return r;
