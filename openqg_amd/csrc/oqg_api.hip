@@ -1948,6 +1948,12 @@ static int rv_reverse(oqg_ctx* c, int lo, int hi, int slot, int free) {
     const int m = lo + rev_split(c, n, free);
     rv_restore(c, slot);
     if (int r = rv_advance(c, lo, m)) return r;
+    if (hi - m == 1) {
+        // the checkpoint at m would serve this one adjoint step only: it
+        // reads the loaded state instead
+        if (int r = adj_step_at(c, tdt, suppressed_at(hi, supp), -1)) return r;
+        return rv_reverse(c, lo, m, slot, free);
+    }
     rv_save(c, slot + 1);
     if (int r = rv_reverse(c, m, hi, slot + 1, free - 1)) return r;
     return rv_reverse(c, lo, m, slot, free);
@@ -2194,7 +2200,7 @@ static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::ve
         EnsLaunch L;
         L.kind = ENS_YS;
         L.R = kv.first;
-        L.lds = std::max(kv.second.second, sizeof(double) * 8 * (size_t)(64 * kv.first + 1));
+        L.lds = std::max(kv.second.second, sizeof(double) * 8 * (size_t)ys_pitch(kv.first));
         if (int r = add_launch(L, kv.second.first)) return r;
     }
     for (auto& kv : inv) {

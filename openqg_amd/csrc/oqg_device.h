@@ -318,6 +318,13 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
     }
 }
 
+// LDS slot of element x of a transform buffer: one 16-byte slot of padding
+// after every 16, so that the strided writes of a first stage (stride R slots,
+// R <= 16) and the contiguous reads of every stage fall in distinct banks.
+// A buffer of length L takes zpad_len(L) slots.
+__host__ __device__ constexpr int zpad(int x) { return x + (x >> 4); }
+__host__ __device__ constexpr int zpad_len(int L) { return L + (L >> 4); }
+
 // One radix-R stage.  LR = L/R butterflies, tstep = L/(Ns R); kk/jj carry
 // k = t mod Ns and j = t / Ns of this thread's butterflies t = gt + b*FFT_GROUP
 // and are advanced to the next stage on return.
@@ -331,7 +338,7 @@ __device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tste
         if (t < LR) {
             const int k = kk[b];
 #pragma unroll
-            for (int r = 0; r < R; r++) v[b][r] = buf[t + r * LR];
+            for (int r = 0; r < R; r++) v[b][r] = buf[zpad(t + r * LR)];
 #pragma unroll
             for (int r = 1; r < R; r++) {
                 double2 w = tw[r * k * tstep];   // k = 0 gives exactly (1, 0)
@@ -349,7 +356,7 @@ __device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tste
         if (t < LR) {
             const int base = j * Ns * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) buf[base + r * Ns] = v[b][r];
+            for (int r = 0; r < R; r++) buf[zpad(base + r * Ns)] = v[b][r];
         }
         kk[b] = k + Ns * (j % R);
         jj[b] = j / R;
@@ -357,7 +364,8 @@ __device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tste
     __syncthreads();
 }
 
-// In-place transform of buf (length L) with the LDS twiddle table tw, by the
+// In-place transform of buf (length L, stored through zpad) with the LDS
+// twiddle table tw, by the
 // plan radix[0..nst) whose radices are all <= MAXR (a larger one is never
 // instantiated, so it costs the kernel no registers); gt = thread index within
 // its group; MB >= the most butterflies a thread owns in any stage.  Every

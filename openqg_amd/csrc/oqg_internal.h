@@ -115,6 +115,10 @@ struct VortArgs {
 // zonal boundary rows, and in a box the meridional boundary columns, are
 // rewritten by ocqbdy (vorsubs.F90:58-114) in the same step before anything
 // reads them.
+// Column pitch (doubles) of the y solver's LDS tile for R rows per lane: room
+// for R row groups of stride 65, rounded up to 2 mod 16, which puts the 8
+// columns of a transposing access 4 banks apart.
+__host__ __device__ constexpr int ys_pitch(int R) { return 65 * R + ((2 - 65 * R) % 16 + 16) % 16; }
 __host__ __device__ inline int vort_org_x(const Geo& g) { return g.cyclic ? 0 : 1; }
 __host__ __device__ inline int vort_lim_x(const Geo& g) { return g.cyclic ? g.nxp : g.nxp - 1; }
 

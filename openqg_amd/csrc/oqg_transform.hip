@@ -61,7 +61,7 @@ __device__ __forceinline__ double adj_modal_input(const FwdArgs& a, const double
 // w_m = sum_k ctm2l(m,k) lp_k.
 template <int NL, int MB, bool ADJ = false>
 __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
-    extern __shared__ double2 zbuf[];   // [L] transform, then the twiddle table
+    extern __shared__ double2 zbuf[];   // [zpad_len(L)] transform, then the twiddle table
     const Geo& g = a.g;
     const Spec& s = a.s;
     const int L = s.L;
@@ -69,7 +69,7 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     const int r0 = 2 * jp, r1 = 2 * jp + 1;
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
-    double2* tw = zbuf + L;
+    double2* tw = zbuf + zpad_len(L);
     Staged<4, double2> stw;   // the whole table for L <= 1024
     stage_load(stw, a.tw, L);
     const bool raw = !ADJ && a.raw_mode >= 0;
@@ -160,22 +160,22 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     if (!g.cyclic) {
         // odd extension of the two rows: z_t = x_t + i y_t, z_{L-t} = -z_t, z_0 = z_N = 0
         if (threadIdx.x == 0) {
-            zbuf[0] = make_double2(0.0, 0.0);
-            zbuf[L / 2] = make_double2(0.0, 0.0);
+            zbuf[zpad(0)] = make_double2(0.0, 0.0);
+            zbuf[zpad(L / 2)] = make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int t = 1 + threadIdx.x + 256 * u;
             if (t <= npts) {
-                zbuf[t] = make_double2(xv[u], yv[u]);
-                zbuf[L - t] = make_double2(-xv[u], -yv[u]);
+                zbuf[zpad(t)] = make_double2(xv[u], yv[u]);
+                zbuf[zpad(L - t)] = make_double2(-xv[u], -yv[u]);
             }
         }
     } else {
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int t = threadIdx.x + 256 * u;
-            if (t < npts) zbuf[t] = make_double2(xv[u], yv[u]);
+            if (t < npts) zbuf[zpad(t)] = make_double2(xv[u], yv[u]);
         }
     }
     __syncthreads();
@@ -185,14 +185,14 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     if (!g.cyclic) {
         // DST(x)_K = -Im Z_K (row r0), DST(y)_K = Re Z_K (row r1)
         for (int K = threadIdx.x + 1; K <= s.nin; K += blockDim.x) {
-            const double2 v = zbuf[K];
+            const double2 v = zbuf[zpad(K)];
             row0[K - 1] = -v.y;
             if (has1) row1[K - 1] = v.x;
         }
     } else {
         const int N = L;
         for (int kk = threadIdx.x; kk <= N / 2; kk += blockDim.x) {
-            const double2 A = zbuf[kk], B = zbuf[(N - kk) % N];
+            const double2 A = zbuf[zpad(kk)], B = zbuf[zpad((N - kk) % N)];
             // split the spectra of the two packed real rows
             const double xr = 0.5 * (A.x + B.x), xi = 0.5 * (A.y - B.y);
             const double yr = 0.5 * (A.y + B.y), yi = 0.5 * (B.x - A.x);
@@ -243,7 +243,7 @@ int fft_mb(const Spec& s, bool inverse) {
     return mx <= 1 ? 1 : (mx <= 2 ? 2 : 4);
 }
 
-size_t fwd_lds_bytes(const Spec& s) { return sizeof(double2) * (size_t)(s.L + twiddle_table_len(s.L)); }
+size_t fwd_lds_bytes(const Spec& s) { return sizeof(double2) * (size_t)(zpad_len(s.L) + twiddle_table_len(s.L)); }
 
 void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
     const int npairs = (a.s.nyi + 1) / 2;
@@ -351,12 +351,19 @@ __device__ void wave_tridiag(double (&f)[R], int L, double aa, double b, int lan
         if (t < L) f[t] = (t == L - 1) ? S : v[t] + x[t] * Sm + y[t] * S;
 }
 
+// LDS slot of row j of a y-solver column: row j = lane*R + t sits at
+// t*65 + lane, so a wave's accesses to its rows t are contiguous, and the
+// transposing loads and stores (8 columns by consecutive rows) spread over
+// the banks through the odd row stride and the column pitch ys_pitch(R).
+template <int R>
+__device__ __forceinline__ int ys_slot(int j) { return (j % R) * 65 + j / R; }
+
 // Columns [cb*YS_CT, cb*YS_CT + YS_CT) of mode mode0 + mb.
 template <int R>
 __device__ __forceinline__ void ysolve_block(const YArgs& a, int cb, int mb) {
-    extern __shared__ double tile[];   // [YS_CT][P]
+    extern __shared__ double tile[];   // [YS_CT][ys_pitch(R)]
     const Spec& s = a.s;
-    const int P = s.nypad + 1;
+    constexpr int P = ys_pitch(R);
     const int m = a.mode0 + mb;
     const int c0 = cb * YS_CT;
     double* S = a.spec + (long long)m * s.nrow * s.kp;
@@ -373,7 +380,7 @@ __device__ __forceinline__ void ysolve_block(const YArgs& a, int cb, int mb) {
     for (int t = 0; t < R; t++) {
         const int idx = threadIdx.x + 64 * YS_CT * t;
         const int j = idx / YS_CT, c = idx - j * YS_CT;
-        if (j < s.nyi) tile[c * P + j] = tmp[t];
+        if (j < s.nyi) tile[c * P + ys_slot<R>(j)] = tmp[t];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
@@ -385,13 +392,13 @@ __device__ __forceinline__ void ysolve_block(const YArgs& a, int cb, int mb) {
         L = L < 0 ? 0 : (L > R ? R : L);
         double f[R];
 #pragma unroll
-        for (int t = 0; t < R; t++) f[t] = (t < L) ? tile[c * P + rs + t] : 0.0;
+        for (int t = 0; t < R; t++) f[t] = (t < L) ? tile[c * P + t * 65 + lane] : 0.0;
         wave_tridiag<R>(f, L, s.a, b, lane);
         double csum = 0.0;
 #pragma unroll
         for (int t = 0; t < R; t++)
             if (t < L) {
-                tile[c * P + rs + t] = f[t];
+                tile[c * P + t * 65 + lane] = f[t];
                 csum += f[t];
             }
         if (a.colsum) {
@@ -404,7 +411,7 @@ __device__ __forceinline__ void ysolve_block(const YArgs& a, int cb, int mb) {
     for (int idx = threadIdx.x; idx < s.nyi * YS_CT; idx += blockDim.x) {
         const int j = idx / YS_CT, cc = idx - j * YS_CT;
         const int cl = c0 + cc;
-        if (cl < s.ncol) S[(long long)j * s.kp + cl] = tile[cc * P + j];
+        if (cl < s.ncol) S[(long long)j * s.kp + cl] = tile[cc * P + ys_slot<R>(j)];
     }
 }
 
@@ -472,7 +479,7 @@ void launch_ysolve(const YArgs& a, int variant, hipStream_t st) {
     }
 }
 
-size_t ysolve_lds_bytes(const Spec& s) { return sizeof(double) * YS_CT * (size_t)(s.nypad + 1); }
+size_t ysolve_lds_bytes(const Spec& s) { return sizeof(double) * YS_CT * (size_t)ys_pitch(s.R); }
 
 int ysolve_blocks(const Spec& s, int nmodes) { return ((s.ncol + YS_CT - 1) / YS_CT) * nmodes; }
 
@@ -747,8 +754,8 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     const int nm = raw1 ? 1 : NL;
     const int G = blockDim.x / FFT_GROUP;          // transforms in flight
     const int grp = threadIdx.x / FFT_GROUP, gt = threadIdx.x - grp * FFT_GROUP;
-    double2* fbuf = (double2*)smem;                 // [G][L]
-    double* inh = (double*)(fbuf + (size_t)G * L);  // [nm][2][pitch]
+    double2* fbuf = (double2*)smem;                           // [G][zpad_len(L)]
+    double* inh = (double*)(fbuf + (size_t)G * zpad_len(L));  // [nm][2][pitch]
     double2* tw = (double2*)(inh + (size_t)nm * 2 * g.pitch);   // twiddle table
     __shared__ double red[(NL + 1) * 16];
     __shared__ double coef[3 * MAXL + 4];
@@ -839,22 +846,22 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
         const int mi = m0 + grp;
         const bool valid = mi < nm;
         const int m = raw1 ? a.raw_mode : mi;
-        double2* z = fbuf + (size_t)grp * L;
+        double2* z = fbuf + (size_t)grp * zpad_len(L);
         if (m0 > 0) load_spec_pair(a, m, valid, r0, gt, sin);   // later rounds (nl > 3)
         if (valid) {
             if (!g.cyclic) {
                 // odd extension of ftnorm*u for the two rows (inhomog.F90:202-203)
                 if (gt == 0) {
-                    z[0] = make_double2(0.0, 0.0);
-                    z[L / 2] = make_double2(0.0, 0.0);
+                    z[zpad(0)] = make_double2(0.0, 0.0);
+                    z[zpad(L / 2)] = make_double2(0.0, 0.0);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int t = gt + FFT_GROUP * u;   // spectral index K-1
                     if (t < s.nin) {
                         const double2 w = make_double2(fn * sin.bx0[u], has1 ? fn * sin.bx1[u] : 0.0);
-                        z[t + 1] = w;
-                        z[L - t - 1] = make_double2(-w.x, -w.y);
+                        z[zpad(t + 1)] = w;
+                        z[zpad(L - t - 1)] = make_double2(-w.x, -w.y);
                     }
                 }
             } else {
@@ -867,8 +874,8 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
                         double xr = fn * sin.cx[u].x, xi = fn * sin.cx[u].y;
                         double yr = has1 ? fn * sin.cy[u].x : 0.0, yi = has1 ? fn * sin.cy[u].y : 0.0;
                         if (kk == 0 || (N % 2 == 0 && kk == h)) { xi = 0.0; yi = 0.0; }
-                        z[kk] = make_double2(xr - yi, xi + yr);
-                        if (kk != 0 && !(N % 2 == 0 && kk == h)) z[N - kk] = make_double2(xr + yi, -xi + yr);
+                        z[zpad(kk)] = make_double2(xr - yi, xi + yr);
+                        if (kk != 0 && !(N % 2 == 0 && kk == h)) z[zpad(N - kk)] = make_double2(xr + yi, -xi + yr);
                     }
                 }
             }
@@ -882,9 +889,9 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
             for (int i = gt; i < g.nxp; i += FFT_GROUP) {
                 double x0 = 0.0, x1 = 0.0;
                 if (!g.cyclic) {
-                    if (i >= 1 && i <= s.nin) { const double2 v = z[i]; x0 = -v.y; x1 = v.x; }
+                    if (i >= 1 && i <= s.nin) { const double2 v = z[zpad(i)]; x0 = -v.y; x1 = v.x; }
                 } else {
-                    const double2 v = z[(i == g.nxp - 1) ? 0 : i];
+                    const double2 v = z[zpad((i == g.nxp - 1) ? 0 : i)];
                     x0 = v.x;
                     x1 = v.y;
                 }
@@ -1045,7 +1052,7 @@ int inv_block_count(const Spec& s) { return (s.nyi + 1) / 2; }
 static int inv_groups(const InvArgs& a) { return inv_group_count((a.raw_mode >= 0) ? 1 : a.g.nl); }
 
 size_t inv_lds_bytes(const Spec& s, const Geo& g, int nm) {
-    return sizeof(double2) * ((size_t)inv_group_count(nm) * s.L + twiddle_table_len(s.L)) +
+    return sizeof(double2) * ((size_t)inv_group_count(nm) * zpad_len(s.L) + twiddle_table_len(s.L)) +
            sizeof(double) * 2 * (size_t)nm * g.pitch;
 }
 
