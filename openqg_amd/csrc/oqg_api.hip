@@ -70,7 +70,8 @@ struct oqg_ctx {
     double* yprel = nullptr;
     double* hom = nullptr;
     double* bcol = nullptr;
-    double* tw = nullptr;
+    double* tw = nullptr;    // twiddles of the forward transform plan (k_fwd)
+    double* itw = nullptr;   // twiddles of the inverse transform plan (k_inv)
     double* raw[2] = {nullptr, nullptr};
     DevConsts* dconst = nullptr;
     std::vector<void*> allocs;
@@ -391,7 +392,7 @@ static void fill_inv_common(oqg_ctx* c, InvArgs& ia) {
     ia.ph = &c->dconst->ph;
     ia.dc = c->dconst;
     ia.spec = c->spec;
-    ia.tw = c->tw;
+    ia.tw = c->itw;
     ia.colsum = c->colsum;
     ia.colw = c->colw;
     ia.entpart = c->entpart;
@@ -664,7 +665,8 @@ static int setup(oqg_ctx* c) {
     if (!rc) rc = dalloc(c, &c->wekp, f1);
     if (!rc) rc = dalloc(c, &c->yprel, (size_t)nyp);
     if (!rc) rc = dalloc(c, &c->bcol, (size_t)nl * s.ncol);
-    if (!rc) rc = dalloc(c, &c->tw, 2 * (size_t)s.L);
+    if (!rc) rc = dalloc(c, &c->tw, 2 * (size_t)twiddle_table_len(s.L));
+    if (!rc) rc = dalloc(c, &c->itw, 2 * (size_t)twiddle_table_len(s.L));
     if (!rc && d.ddyn) rc = dalloc(c, &c->ddyn, f1);
     if (!rc) rc = dalloc(c, &c->hom, g.cyclic ? (size_t)nyp * (1 + 2 * (nl - 1)) : (size_t)(nl - 1) * f1);
     if (!rc) rc = dalloc(c, &c->dconst, 1);
@@ -690,9 +692,29 @@ static int setup(oqg_ctx* c) {
         c->ddyn_h.assign(d.ddyn, d.ddyn + (size_t)nxp * nyp);
         if (int r = upload_field(c, c->ddyn, d.ddyn, 1, OQG_HOST)) return r;
     }
-    std::vector<double> tw(2 * (size_t)s.L);
-    for (int t = 0; t < s.L; t++) twiddle_ld(t, s.L, &tw[2 * t], &tw[2 * t + 1]);
-    HIPCHK(c, hipMemcpyAsync(c->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    for (int inv = 0; inv < 2; inv++) {
+        // the plan's twiddle table in the stage-by-stage layout the LDS FFT reads
+        const int L = s.L, nst = inv ? s.instages : s.nstages;
+        const int* radix = inv ? s.iradix : s.radix;
+        std::vector<double> tw(2 * (size_t)twiddle_table_len(L), 0.0);
+        size_t e = 0;
+        int Ns = 1;
+        for (int st = 0; st < nst; st++) {
+            const int R = radix[st], tstep = L / (Ns * R);
+            for (int r = 1; r < R; r++)
+                for (int k = 0; k < Ns; k++, e++) twiddle_ld((long)r * k * tstep, L, &tw[2 * e], &tw[2 * e + 1]);
+            Ns *= R;
+        }
+        for (int m = 1; m < 3 && L % 3 == 0; m++) {
+            const size_t o = twiddle_roots3(L) + m - 1;
+            twiddle_ld((long)m * (L / 3), L, &tw[2 * o], &tw[2 * o + 1]);
+        }
+        for (int m = 1; m < 5 && L % 5 == 0; m++) {
+            const size_t o = twiddle_roots5(L) + m - 1;
+            twiddle_ld((long)m * (L / 5), L, &tw[2 * o], &tw[2 * o + 1]);
+        }
+        HIPCHK(c, hipMemcpy(inv ? c->itw : c->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     // tridiagonal diagonals bb(k,m) (inhomog.F90:127-141)
     std::vector<double> bcol((size_t)nl * s.ncol);
     const double dxm2 = 1.0 / (d.dx * d.dx);

@@ -213,16 +213,19 @@ __device__ void block_sum(double (&v)[NV], double* red) {
 // complex buffer of length L per transform; each of the FFT_GROUP threads of a
 // group holds up to FFT_MAXB butterflies in registers across the barrier that
 // separates a stage's reads from its writes.  Twiddles come from an LDS copy of
-// the table tw[t] = exp(-2 pi i t / L), t < L.  SIGN = -1 is the forward
-// transform, +1 the backward one (conjugate twiddles).
+// the plan's table: with W = exp(-2 pi i / L), stage s (Ns = R_0...R_{s-1},
+// tstep = L/(Ns R_s)) holds W^(r k tstep) at off_s + (r-1) Ns + k for
+// r = 1..R_s-1, k < Ns (off_s = sum of (R_i - 1) N_i over earlier stages;
+// L - 1 entries in all), so a wave reads consecutive entries; then W^(m L/3)
+// at L-1+(m-1), m = 1, 2, and W^(m L/5) at L+1+(m-1), m = 1..4, for the
+// radix-3 and radix-5 butterflies.  SIGN = -1 is the forward transform, +1 the
+// backward one (conjugate twiddles).
 //
 // Each butterfly index t stays with its thread through all stages; the stage
 // with Ns = R_0...R_{s-1} needs k = t mod Ns and j = t / Ns, which are carried
 // from stage to stage (k += Ns (j mod R), j /= R with R a compile-time radix)
 // instead of being recomputed with runtime divisions.
 
-// Entries of the LDS twiddle table of a length-L transform.
-__host__ __device__ inline int twiddle_table_len(int L) { return L; }
 
 // exp(-2 pi i m / 16) for the forward transform (SIGN < 0), its conjugate for
 // the backward one, m = 0..9: the twiddles inside a radix-16 butterfly.
@@ -298,17 +301,18 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
         v[2] = make_double2(a0.x - b0.x, a0.y - b0.y);
         v[3] = make_double2(a1.x - j.x, a1.y - j.y);
     } else {
+        // radix 3 and 5: tw holds the roots W^(m L/R) at m-1, m = 1..R-1
         double2 in[R];
 #pragma unroll
         for (int r = 0; r < R; r++) in[r] = v[r];
-        const int pstep = L / R;
         constexpr double sg = (SIGN > 0) ? -1.0 : 1.0;
 #pragma unroll
         for (int q = 0; q < R; q++) {
             double sr = in[0].x, si = in[0].y;
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                const double2 w = tw[((r * q) % R) * pstep];
+                const int mq = (r * q) % R;
+                const double2 w = (mq == 0) ? make_double2(1.0, 0.0) : tw[mq - 1];
                 const double c = w.x, s = sg * w.y;
                 sr += in[r].x * c - in[r].y * s;
                 si += in[r].x * s + in[r].y * c;
@@ -325,12 +329,13 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], const double2* tw, in
 __host__ __device__ constexpr int zpad(int x) { return x + (x >> 4); }
 __host__ __device__ constexpr int zpad_len(int L) { return L + (L >> 4); }
 
-// One radix-R stage.  LR = L/R butterflies, tstep = L/(Ns R); kk/jj carry
+// One radix-R stage.  LR = L/R butterflies; tws = the stage's twiddles
+// (W^(r k tstep) at (r-1) Ns + k), roots = the radix-3/5 roots; kk/jj carry
 // k = t mod Ns and j = t / Ns of this thread's butterflies t = gt + b*FFT_GROUP
 // and are advanced to the next stage on return.
 template <int R, int MB, int SIGN>
-__device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tstep, const double2* tw, int gt,
-                                          int (&kk)[MB], int (&jj)[MB]) {
+__device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, const double2* tws, const double2* roots,
+                                          int gt, int (&kk)[MB], int (&jj)[MB]) {
     double2 v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; b++) {
@@ -341,11 +346,11 @@ __device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tste
             for (int r = 0; r < R; r++) v[b][r] = buf[zpad(t + r * LR)];
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                double2 w = tw[r * k * tstep];   // k = 0 gives exactly (1, 0)
+                double2 w = tws[(r - 1) * Ns + k];   // k = 0 gives exactly (1, 0)
                 if (SIGN > 0) w.y = -w.y;
                 v[b][r] = cmul(v[b][r], w);
             }
-            dft_small<R, SIGN>(v[b], tw, LR * R);
+            dft_small<R, SIGN>(v[b], roots, R);
         }
     }
     __syncthreads();
@@ -364,9 +369,9 @@ __device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, int tste
     __syncthreads();
 }
 
-// In-place transform of buf (length L, stored through zpad) with the LDS
-// twiddle table tw, by the
-// plan radix[0..nst) whose radices are all <= MAXR (a larger one is never
+// In-place transform of buf (length L, stored through zpad) with the LDS copy
+// tw of the twiddle table of the plan radix[0..nst), whose radices are all
+// <= MAXR (a larger one is never
 // instantiated, so it costs the kernel no registers); gt = thread index within
 // its group; MB >= the most butterflies a thread owns in any stage.  Every
 // thread of the block must call this (it contains block barriers).
@@ -379,21 +384,33 @@ __device__ __forceinline__ void fft_lds(double2* buf, int L, int nst, const int*
         jj[b] = gt + b * FFT_GROUP;
     }
     int Ns = 1;
+    const double2* tws = tw;   // this stage's twiddles
     for (int st = 0; st < nst; st++) {
         const int R = radix[st];
-        const int LR = L / R, tstep = L / (Ns * R);   // uniform (scalar) arithmetic
+        const int LR = L / R;   // uniform (scalar) arithmetic
         if constexpr (MAXR >= 16) {
-            if (R == 16) { fft_stage<16, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); Ns *= R; continue; }
+            if (R == 16) {
+                fft_stage<16, MB, SIGN>(buf, LR, Ns, tws, nullptr, gt, kk, jj);
+                tws += 15 * Ns;
+                Ns *= R;
+                continue;
+            }
         }
         if constexpr (MAXR >= 8) {
-            if (R == 8) { fft_stage<8, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); Ns *= R; continue; }
+            if (R == 8) {
+                fft_stage<8, MB, SIGN>(buf, LR, Ns, tws, nullptr, gt, kk, jj);
+                tws += 7 * Ns;
+                Ns *= R;
+                continue;
+            }
         }
         switch (R) {
-            case 4: fft_stage<4, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
-            case 2: fft_stage<2, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
-            case 3: fft_stage<3, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
-            default: fft_stage<5, MB, SIGN>(buf, LR, Ns, tstep, tw, gt, kk, jj); break;
+            case 4: fft_stage<4, MB, SIGN>(buf, LR, Ns, tws, nullptr, gt, kk, jj); break;
+            case 2: fft_stage<2, MB, SIGN>(buf, LR, Ns, tws, nullptr, gt, kk, jj); break;
+            case 3: fft_stage<3, MB, SIGN>(buf, LR, Ns, tws, tw + twiddle_roots3(L), gt, kk, jj); break;
+            default: fft_stage<5, MB, SIGN>(buf, LR, Ns, tws, tw + twiddle_roots5(L), gt, kk, jj); break;
         }
+        tws += (R - 1) * Ns;
         Ns *= R;
     }
 }

@@ -115,6 +115,12 @@ struct VortArgs {
 // zonal boundary rows, and in a box the meridional boundary columns, are
 // rewritten by ocqbdy (vorsubs.F90:58-114) in the same step before anything
 // reads them.
+// Entries of the twiddle table of a length-L transform plan (the stages'
+// twiddles, then the radix-3 and radix-5 roots at the two offsets below).
+__host__ __device__ inline int twiddle_table_len(int L) { return L + 8; }
+__host__ __device__ inline int twiddle_roots3(int L) { return L - 1; }
+__host__ __device__ inline int twiddle_roots5(int L) { return L + 1; }
+
 // Column pitch (doubles) of the y solver's LDS tile for R rows per lane: room
 // for R row groups of stride 65, rounded up to 2 mod 16, which puts the 8
 // columns of a transposing access 4 banks apart.

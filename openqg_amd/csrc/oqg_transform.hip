@@ -71,7 +71,7 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     const int gj0 = r0 + 1, gj1 = r1 + 1;
     double2* tw = zbuf + zpad_len(L);
     Staged<4, double2> stw;   // the whole table for L <= 1024
-    stage_load(stw, a.tw, L);
+    stage_load(stw, a.tw, twiddle_table_len(L));
     const bool raw = !ADJ && a.raw_mode >= 0;
     const double* qin = ADJ ? a.adj_lp : a.q;
     // inputs of this thread's (at most four) points of both rows, loaded together
@@ -156,7 +156,7 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         block_sum<1>(pv, s_red);
         if (threadIdx.x == 0) a.adj_partial[(long long)jp * MAXL + m] = pv[0];
     }
-    stage_store(stw, tw, a.tw, L);
+    stage_store(stw, tw, a.tw, twiddle_table_len(L));
     if (!g.cyclic) {
         // odd extension of the two rows: z_t = x_t + i y_t, z_{L-t} = -z_t, z_0 = z_N = 0
         if (threadIdx.x == 0) {
@@ -772,7 +772,7 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     Staged<1> st_dc, st_cin, st_aux;
     Staged<2, double2> st_tw;   // the whole table for L <= 2*blockDim
     stage_load(st_dc, a.dc, nw_dc);
-    stage_load(st_tw, a.tw, L);
+    stage_load(st_tw, a.tw, twiddle_table_len(L));
     stage_load(st_cin, nw_cin ? (const void*)a.cin : (const void*)a.dc, nw_cin);   // a.cin is null in raw mode
     stage_load(st_aux, a.aux, nw_aux);
     const bool want_spk = !raw && g.cyclic && threadIdx.x < 2 * NL;
@@ -817,7 +817,7 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
                                ? a.hom[(long long)(m - 1) * g.fs + (long long)gj * g.pitch + i] : 0.0;
     }
     stage_store(st_dc, &s_dc, a.dc, nw_dc);
-    stage_store(st_tw, tw, a.tw, L);
+    stage_store(st_tw, tw, a.tw, twiddle_table_len(L));
     stage_store(st_cin, s_cin, a.cin, nw_cin);
     stage_store(st_aux, s_aux, a.aux, nw_aux);
     if (want_spk) s_spk[threadIdx.x] = spk;

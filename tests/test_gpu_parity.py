@@ -95,6 +95,13 @@ def test_step_parity_layers(gpu, nl):
     _per_step(W.small_channel(nx=49, ny=21, nl=nl), 8)
 
 
+def test_step_parity_mixed_radix(gpu):
+    """Transform lengths with both 3 and 5 as factors (120 = 8*3*5 in the box,
+    60 = 4*3*5 in the channel): both root blocks of a plan's twiddle table."""
+    _per_step(W.small_box(n=61), 6)
+    _per_step(W.small_channel(nx=61, ny=31), 6)
+
+
 def test_step_parity_rectangular_box(gpu):
     w = W.small_box(n=33)
     w.nxp, w.nyp, w.name = 65, 97, "box65x97"
