@@ -111,10 +111,6 @@ struct VortArgs {
     int org_x;            // first column of the tiles: 1 in a box, 0 in a channel
 };
 
-// k_vort computes q_new on columns [org_x, lim_x) and rows [1, nyp-1): the
-// zonal boundary rows, and in a box the meridional boundary columns, are
-// rewritten by ocqbdy (vorsubs.F90:58-114) in the same step before anything
-// reads them.
 // Entries of the twiddle table of a length-L transform plan (the stages'
 // twiddles, then the radix-3 and radix-5 roots at the two offsets below).
 __host__ __device__ inline int twiddle_table_len(int L) { return L + 8; }
@@ -125,6 +121,11 @@ __host__ __device__ inline int twiddle_roots5(int L) { return L + 1; }
 // for R row groups of stride 65, rounded up to 2 mod 16, which puts the 8
 // columns of a transposing access 4 banks apart.
 __host__ __device__ constexpr int ys_pitch(int R) { return 65 * R + ((2 - 65 * R) % 16 + 16) % 16; }
+
+// k_vort computes q_new on columns [org_x, lim_x) and rows [1, nyp-1): the
+// zonal boundary rows, and in a box the meridional boundary columns, are
+// rewritten by ocqbdy (vorsubs.F90:58-114) in the same step before anything
+// reads them.
 __host__ __device__ inline int vort_org_x(const Geo& g) { return g.cyclic ? 0 : 1; }
 __host__ __device__ inline int vort_lim_x(const Geo& g) { return g.cyclic ? g.nxp : g.nxp - 1; }
 
@@ -144,7 +145,7 @@ struct FwdArgs {
     const double* q;      // q_new (step) or rhs field (raw)
     const double* ddyn;   // may be null
     const double* yprel;
-    const double* tw;     // twiddles exp(-2 pi i t / L), interleaved
+    const double* tw;     // twiddle table of the forward plan (stage order, interleaved)
     double* spec;
     int raw_mode;         // >=0: raw single field solved for this mode (0-based)
     // adjoint mode (k_fwd_adj, box): the input of mode m is
@@ -174,7 +175,7 @@ struct InvArgs {
     const Phys* ph;   // DevConsts::ph (kept out of the kernel arguments)
     const DevConsts* dc;    // device copy of the precompute blocks
     const double* spec;
-    const double* tw;
+    const double* tw;       // twiddle table of the inverse plan
     const double* colsum;   // box: [m][col]
     const double* colw;     // box: closed-form x-sum weights per column
     const double* entpart;
