@@ -13,6 +13,7 @@ namespace oqg {
 
 constexpr int MAXL = OQG_MAX_LAYERS;
 constexpr int WAVE = 64;
+constexpr int NXCD = 8;              // accelerator dies (XCDs) of an MI355X, one L2 each
 constexpr int MAX_RADIX_STAGES = 16;
 constexpr int FWD_MAX_RADIX = 16;    // radix-16/8 butterflies: fewest LDS passes
 constexpr int INV_MAX_RADIX = 4;     // the inverse kernel's register budget keeps radix 4
