@@ -200,8 +200,9 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     __shared__ double s_p[H2 * W2];
     const Geo& g = a.g;
     const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-    const int k = blockIdx.z;
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    int bx, by, k;
+    xcd_block(bx, by, k);   // neighbouring tiles, which share halo lines, on one XCD
+    const int i0 = bx * TX, j0 = by * TY;
     const long long lo = (long long)k * g.fs;
     const double* ln = a.lq_np1 + lo;
     const double* qn = a.q + lo;

@@ -242,19 +242,12 @@ __device__ __forceinline__ void vort_block(const VortArgs& a, int bx, int by, in
     else vort_tile<true>(a, bx, by, k, s_pm, s_d2, s_d4, s_q, s_p, s_red);
 }
 
-// Workgroups go to the XCDs round-robin by linear id, and each XCD has its own
-// L2: each XCD takes a contiguous run of tiles (whole rows of tiles), so the
-// halo lines that neighbouring tiles share are fetched from HBM once.
+// Each XCD takes a contiguous run of tiles (whole rows of tiles), so the halo
+// lines that neighbouring tiles share are fetched from HBM into one L2.
 __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
-    int bx = blockIdx.x, by = blockIdx.y;
-    const int T = gridDim.x * gridDim.y;
-    if (T % NXCD == 0) {
-        const int w = blockIdx.x + gridDim.x * blockIdx.y;
-        const int t = (w % NXCD) * (T / NXCD) + w / NXCD;
-        by = t / gridDim.x;
-        bx = t - by * gridDim.x;
-    }
-    vort_block(a, bx, by, blockIdx.z);
+    int bx, by, bz;
+    xcd_block(bx, by, bz);
+    vort_block(a, bx, by, bz);
 }
 
 // Ensemble form: block b works on tile map[b].y of member map[b].x, whose

@@ -418,11 +418,11 @@ __device__ __forceinline__ void ysolve_block(const YArgs& a, int cb, int mb) {
 template <int R>
 __global__ __launch_bounds__(64 * YS_CT) void k_ysolve(YArgs a) {
     // neighbouring column blocks share 128-byte lines of the spectral rows:
-    // each XCD (workgroups are dealt to them round-robin) takes a contiguous
-    // run of column blocks, so a shared line is fetched into one L2
-    int cb = blockIdx.x;
-    if (gridDim.x % NXCD == 0) cb = (blockIdx.x % NXCD) * (gridDim.x / NXCD) + blockIdx.x / NXCD;
-    ysolve_block<R>(a, cb, blockIdx.y);
+    // each XCD takes a contiguous run of column blocks, so a shared line is
+    // fetched into one L2
+    int cb, mb, bz;
+    xcd_block(cb, mb, bz);
+    ysolve_block<R>(a, cb, mb);
 }
 
 // Ensemble form: block b works on column block map[b].y (column blocks
