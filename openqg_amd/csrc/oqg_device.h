@@ -332,22 +332,20 @@ __device__ __forceinline__ int xcd_contig(int w, int n) {
 }
 
 // This workgroup's (x, y, z) block, remapped so that neighbouring blocks,
-// which share halo and row lines, mostly run on one die.  When a z slice
+// which share halo and row lines, mostly run on one die: when a z slice
 // (layer or mode) holds a multiple of NXCD blocks, every die takes one
-// contiguous run of each slice (the same band of every layer: measured 1 us
-// faster for k_vort at 513^2 than runs over the whole grid); otherwise the
-// runs are taken over the whole grid, x fastest.
+// contiguous run of each slice (the same band of every layer).  Other grids
+// keep the hardware order: runs over the whole grid measured slower for
+// k_vort at 513^2 (1 us) and for k_adj_vort at 257^2 (0.4 us).
 __device__ __forceinline__ void xcd_block(int& bx, int& by, int& bz) {
     const int gx = gridDim.x, gy = gridDim.y, slice = gx * gy;
-    int t;
-    if (slice % NXCD == 0) {
-        t = xcd_contig(blockIdx.x + gx * blockIdx.y, slice);
-        bz = blockIdx.z;
-    } else {
-        const int g = xcd_contig(blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z), slice * gridDim.z);
-        bz = g / slice;
-        t = g - bz * slice;
+    bz = blockIdx.z;
+    if (slice % NXCD != 0) {
+        bx = blockIdx.x;
+        by = blockIdx.y;
+        return;
     }
+    const int t = xcd_contig(blockIdx.x + gx * blockIdx.y, slice);
     by = t / gx;
     bx = t - by * gx;
 }

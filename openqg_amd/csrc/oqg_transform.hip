@@ -129,21 +129,25 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         if (ADJ) part += d0[u] * xv[u] + d1[u] * yv[u];
     }
     if constexpr (ADJ) {
-        // the block's other points of the inner products: columns 0 and nxp-1
-        // of its rows, and rows 0 / nyp-1 in the first / last block
+        // the other points of the inner products: columns 0 and nxp-1 of the
+        // block's rows, and a share of the ring rows 0 and nyp-1 (2 nxp points
+        // dealt out evenly over the row pairs, so that no block's chain of
+        // loads is longer than the others')
         __shared__ double s_red[16];
-        const bool lastb = jp == (s.nyi + 1) / 2 - 1;
+        const int npairs = (s.nyi + 1) / 2, nring = 2 * g.nxp;
+        const int per = (nring + npairs - 1) / npairs;
+        const int rlo = min(nring, jp * per), rhi = min(nring, rlo + per);
         const int nside = has1 ? 4 : 2;
-        const int nextra = nside + ((jp == 0 ? 1 : 0) + (lastb ? 1 : 0)) * g.nxp;
+        const int nextra = nside + (rhi - rlo);
         for (int e = threadIdx.x; e < nextra; e += blockDim.x) {
             int i, gj;
             if (e < nside) {
                 gj = (e >> 1) ? gj1 : gj0;
                 i = (e & 1) ? g.nxp - 1 : 0;
             } else {
-                const int r = (e - nside) / g.nxp;
-                i = (e - nside) - r * g.nxp;
-                gj = (r == 0 && jp == 0) ? 0 : g.nyp - 1;
+                const int idx = rlo + (e - nside);
+                gj = idx < g.nxp ? 0 : g.nyp - 1;
+                i = idx < g.nxp ? idx : idx - g.nxp;
             }
             const long long o = (long long)gj * g.pitch + i;
             double lp[NL];
