@@ -6,9 +6,10 @@
 // transpose of the forward discretisation, written out by hand:
 //   k_adj_supp  suppress_comp_mode (driver.F90:174-189)
 //   k_adj_vort  vort_step (vorticity.F90:14-103): Arakawa Jacobian transposes
-//               in gather form (the p transpose through the discrete
-//               antisymmetry J(q,p) = -J(p,q)), Del^2/Del^4/Del^6 transposes
-//               of del2_P_bc with its mixed boundary condition, forcing
+//               (J_1(h)^T mu = -J(mu, h) for mu vanishing on the ring; the p
+//               transpose through the discrete antisymmetry J(q,p) = -J(p,q)),
+//               Del^2/Del^4/Del^6 transposes of del2_P_bc with its mixed
+//               boundary condition, forcing
 //   k_adj_add   sums of fields (state combination for the initial-state map)
 // The inversion's adjoint runs through the transform kernels in their adjoint
 // modes (oqg_transform.hip): k_fwd_adj forms the modal inputs from the
@@ -98,55 +99,6 @@ __device__ __forceinline__ double lapT_lds(const double* s, int W, int r, int c,
     return acc;
 }
 
-// Derivative of the Arakawa sum at x (before the factor jfac) with respect to
-// its first argument f at x + o, o != 0, for second argument h around x (the
-// nine values hv[(dj+1)*3 + di+1]) and x's flags.
-__device__ __forceinline__ double jac_coef(int oi, int oj, const double (&h)[9], bool xin, bool yin) {
-    const double hN = h[7], hS = h[1], hE = h[5], hW = h[3];
-    const double hNE = h[8], hNW = h[6], hSE = h[2], hSW = h[0];
-    const double dhy = yin ? hN - hS : 0.0, dhx = xin ? hE - hW : 0.0;
-    const double dhyE = yin ? hNE - hSE : 0.0, dhyW = yin ? hNW - hSW : 0.0;
-    const double dhxN = xin ? hNE - hNW : 0.0, dhxS = xin ? hSE - hSW : 0.0;
-    const double b = (xin && yin) ? 1.0 : 0.0;
-    if (oj == 0) {
-        if (oi == 1) return xin ? dhy + dhyE : 0.0;
-        return xin ? -dhy - dhyW : 0.0;
-    }
-    if (oi == 0) {
-        if (oj == 1) return yin ? -dhx - dhxN : 0.0;
-        return yin ? dhx + dhxS : 0.0;
-    }
-    if (oi == 1 && oj == 1) return b * (hN - hE);
-    if (oi == -1 && oj == 1) return b * (hW - hN);
-    if (oi == 1 && oj == -1) return b * (hE - hS);
-    return b * (hS - hW);
-}
-
-// Transpose of f -> J(f, h) gathered at a: sum over x = a - o of
-// mu(x) * jac_coef(o; h around x).  mu is an LDS tile of width Wm with a at
-// (rm, cm); h an LDS tile of width Wh with a at (rh, ch).
-__device__ __forceinline__ double jacT_lds(const double* mu, int Wm, int rm, int cm, const double* h, int Wh, int rh,
-                                           int ch, int ai, int aj, const Geo& g) {
-    double acc = 0.0;
-#pragma unroll
-    for (int oj = -1; oj <= 1; oj++)
-#pragma unroll
-        for (int oi = -1; oi <= 1; oi++) {
-            if (oi == 0 && oj == 0) continue;
-            const int xi = ai - oi, xj = aj - oj;
-            if (xi < 0 || xi >= g.nxp || xj < 0 || xj >= g.nyp) continue;
-            const double m = mu[(rm - oj) * Wm + cm - oi];
-            double hv[9];
-#pragma unroll
-            for (int dj = -1; dj <= 1; dj++)
-#pragma unroll
-                for (int di = -1; di <= 1; di++) hv[(dj + 1) * 3 + di + 1] = h[(rh - oj + dj) * Wh + ch - oi + di];
-            const bool xin = xi >= 1 && xi <= g.nxp - 2, yin = xj >= 1 && xj <= g.nyp - 2;
-            acc += m * jac_coef(oi, oj, hv, xin, yin);
-        }
-    return acc;
-}
-
 // Forward (box interior points x): q_{n+1} = q_{n-1} + tdt F with
 //   F = jfac J(q_n, p_n) + c2 L^2 p_{n-1} - c4 L^3 p_{n-1} - [k=kt] cdrag L p_{n-1}
 //       + [k=0] (cent1 ent + cwek wekp) - [k=1] cent2 ent.
@@ -159,10 +111,15 @@ __device__ __forceinline__ double jacT_lds(const double* mu, int Wm, int rm, int
 constexpr int ADJ_TX = 64, ADJ_TY = 8;
 
 // Away from the boundary (2 <= i <= nxp-3, 2 <= j <= nyp-3, "deep" points)
-// every stencil point of the gather forms above is interior, and the
-// transposes reduce to forward stencils: L^T = L (the 5-point Laplacian is
-// symmetric) and J_1(h)^T = -J_1(h) (Arakawa's Jacobian J(., h) is
-// antisymmetric as an operator, which is its energy/enstrophy conservation).
+// every stencil point of lapT_lds is interior and L^T = L (the 5-point
+// Laplacian is symmetric); nearer the ring the boundary coefficients of
+// del2_P_bc make lapT_lds necessary.  The Jacobian needs no such case: mu
+// vanishes on the ring, so every nonzero term of the gather comes from an
+// interior point x, whose Arakawa coefficients are the full ones, and the
+// antisymmetry of the stencil (the coefficient of f(a) in J(f, h)(x) is minus
+// that of f(x) in J(f, h)(a), both built from the h values the two 3x3
+// neighbourhoods share) gives J_1(h)^T mu = -J(mu, h) at every point, the
+// ring included, whatever h holds outside the domain.
 __device__ __forceinline__ bool deep_point(int gi, int gj, const Geo& g) {
     return gi >= 2 && gi <= g.nxp - 3 && gj >= 2 && gj <= g.nyp - 3;
 }
@@ -285,16 +242,12 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
         const int r = ty + 4 * u, gj = j0 + r;
         if (gi >= g.nxp || gj >= g.nyp) continue;
         const long long o = lo + (long long)gj * g.pitch + gi;
-        double nu3, jq, jp;
-        if (deep_point(gi, gj, g)) {
-            nu3 = lap5_lds(s_n2, W1, r + 1, tx + 1, g);
-            jq = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_p, W2, r + 2, tx + 2);
-            jp = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_q, W2, r + 2, tx + 2);
-        } else {
-            nu3 = lapT_lds(s_n2, W1, r + 1, tx + 1, gi, gj, g);
-            jq = jacT_lds(s_mu, W3, r + 3, tx + 3, s_p, W2, r + 2, tx + 2, gi, gj, g);
-            jp = jacT_lds(s_mu, W3, r + 3, tx + 3, s_q, W2, r + 2, tx + 2, gi, gj, g);
-        }
+        const double nu3 = deep_point(gi, gj, g) ? lap5_lds(s_n2, W1, r + 1, tx + 1, g)
+                                                 : lapT_lds(s_n2, W1, r + 1, tx + 1, gi, gj, g);
+        // J_1(h)^T mu = -J(mu, h) at every point, the ring included (mu is 0
+        // there; see the note above deep_point)
+        const double jq = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_p, W2, r + 2, tx + 2);
+        const double jp = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_q, W2, r + 2, tx + 2);
         const double nu2 = s_n2[(r + 1) * W1 + tx + 1];
         const double nu1 = s_n1[(r + 2) * W2 + tx + 2];
         double lpm = c2 * nu2 - c4 * nu3;
