@@ -106,6 +106,44 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         }
     }
     const double by0 = g.beta * a.yprel[gj0], by1 = has1 ? g.beta * a.yprel[gj1] : 0.0;
+    // adjoint mode: the other points of the inner products, columns 0 and
+    // nxp-1 of the block's rows and a share of the ring rows 0 and nyp-1 (2
+    // nxp points dealt out evenly over the row pairs).  All lie on the ring,
+    // where the transpose of ocqbdy needs lq at the point and, for the points
+    // next to a corner in rows 0 and nyp-1, at that corner; the first such
+    // point of each thread is loaded here with the rest.
+    const int npairs = (s.nyi + 1) / 2, nring = 2 * g.nxp;
+    const int rper = (nring + npairs - 1) / npairs;
+    const int rlo = min(nring, jp * rper), rhi = min(nring, rlo + rper);
+    const int nside = has1 ? 4 : 2;
+    const int nextra = ADJ ? nside + (rhi - rlo) : 0;
+    auto extra_point = [&](int e, int& i, int& gj) {
+        if (e < nside) {
+            gj = (e >> 1) ? gj1 : gj0;
+            i = (e & 1) ? g.nxp - 1 : 0;
+        } else {
+            const int idx = rlo + (e - nside);
+            gj = idx < g.nxp ? 0 : g.nyp - 1;
+            i = idx < g.nxp ? idx : idx - g.nxp;
+        }
+    };
+    int ei = 0, ej = 0;
+    const bool ehas = (int)threadIdx.x < nextra;
+    if (ehas) extra_point(threadIdx.x, ei, ej);
+    const bool erow = ej == 0 || ej == g.nyp - 1;
+    const bool ewc = ehas && erow && ei == 1, eec = ehas && erow && ei == g.nxp - 2;
+    double elp[NL], elq[NL], elw[NL], ele[NL], ehm = 0.0;
+    if constexpr (ADJ) {
+        const long long o = (long long)ej * g.pitch + ei;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            elp[k] = ehas ? a.adj_lp[k * g.fs + o] : 0.0;
+            elq[k] = ehas ? a.adj_lq[k * g.fs + o] : 0.0;
+            elw[k] = ewc ? a.adj_lq[k * g.fs + o - 1] : 0.0;
+            ele[k] = eec ? a.adj_lq[k * g.fs + o + 1] : 0.0;
+        }
+        ehm = (ehas && m > 0) ? a.adj_hom[(long long)(m - 1) * g.fs + o] : 0.0;
+    }
     double xv[NQ], yv[NQ];
     double part = 0.0;   // adjoint mode: this thread's share of hom_m . w_m
 #pragma unroll
@@ -129,26 +167,37 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         if (ADJ) part += d0[u] * xv[u] + d1[u] * yv[u];
     }
     if constexpr (ADJ) {
-        // the other points of the inner products: columns 0 and nxp-1 of the
-        // block's rows, and a share of the ring rows 0 and nyp-1 (2 nxp points
-        // dealt out evenly over the row pairs, so that no block's chain of
-        // loads is longer than the others')
         __shared__ double s_red[16];
-        const int npairs = (s.nyi + 1) / 2, nring = 2 * g.nxp;
-        const int per = (nring + npairs - 1) / npairs;
-        const int rlo = min(nring, jp * per), rhi = min(nring, rlo + per);
-        const int nside = has1 ? 4 : 2;
-        const int nextra = nside + (rhi - rlo);
-        for (int e = threadIdx.x; e < nextra; e += blockDim.x) {
-            int i, gj;
-            if (e < nside) {
-                gj = (e >> 1) ? gj1 : gj0;
-                i = (e & 1) ? g.nxp - 1 : 0;
-            } else {
-                const int idx = rlo + (e - nside);
-                gj = idx < g.nxp ? 0 : g.nyp - 1;
-                i = idx < g.nxp ? idx : idx - g.nxp;
+        if (ehas) {
+            // bdy_gather at a ring point, from the loaded values: the point's
+            // own term, then its west and east neighbours when they are the
+            // corner whose inner point it is
+            const Mix& mx = a.dc->mx;
+            const double r = ((ei == 0 || ei == g.nxp - 1) && erow) ? 2.0 : 1.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+#pragma unroll
+                for (int l = 0; l < NL; l++) {
+                    double c = -r * g.fnot * mx.amat[k * MAXL + l];
+                    if (l == k) c -= a.adj_qbfac;
+                    elp[l] += elq[k] * c;
+                }
             }
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+                if (ewc) elp[k] += a.adj_qbfac * elw[k];
+#pragma unroll
+            for (int k = 0; k < NL; k++)
+                if (eec) elp[k] += a.adj_qbfac * ele[k];
+            double w = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) w += cm[k] * elp[k];
+            if (m > 0) part += ehm * w;
+        }
+        // points beyond one per thread (very wide, short basins)
+        for (int e = threadIdx.x + blockDim.x; e < nextra; e += blockDim.x) {
+            int i, gj;
+            extra_point(e, i, gj);
             const long long o = (long long)gj * g.pitch + i;
             double lp[NL];
 #pragma unroll
