@@ -98,7 +98,7 @@ struct Staged {
 template <int NPT, class T>
 __device__ __forceinline__ void stage_load(Staged<NPT, T>& r, const void* src, int n) {
     const T* s = static_cast<const T*>(src);
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int p = 0; p < NPT; p++) {
         const int idx = threadIdx.x + p * blockDim.x;
         r.w[p] = s[(idx < n) ? idx : 0];
@@ -108,7 +108,7 @@ __device__ __forceinline__ void stage_load(Staged<NPT, T>& r, const void* src, i
 template <int NPT, class T>
 __device__ __forceinline__ void stage_store(const Staged<NPT, T>& r, void* dst, const void* src, int n) {
     T* d = static_cast<T*>(dst);
-#pragma unroll
+#pragma clang loop unroll(full)   // constant indices keep r in registers
     for (int p = 0; p < NPT; p++) {
         const int idx = threadIdx.x + p * blockDim.x;
         if (idx < n) d[idx] = r.w[p];

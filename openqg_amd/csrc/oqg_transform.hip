@@ -44,16 +44,52 @@ __device__ __forceinline__ double rhs_mode(const FwdArgs& a, const double (&cm)[
     return g.fnot * s;
 }
 
-// w = sum_k cm[k] lp_k at (i, gj) with lp completed by the transpose of
-// ocqbdy where (i, gj) lies in the two outer bands (adjoint mode).
+// w = sum_k cm[k] lp_k at the interior point (i, gj) with lp completed by the
+// transpose of ocqbdy (adjoint mode).  The ring points whose inner point it is
+// are its west (i = 1), east (i = nxp-2), south (gj = 1) and north
+// (gj = nyp-2) neighbours; they are added in bdy_gather's order, without its
+// general case analysis (which kept the kernel's registers spilling).
 template <int NL>
 __device__ __forceinline__ double adj_modal_input(const FwdArgs& a, const double (&cm)[NL], double (&lp)[NL], int i,
                                                   int gj) {
-    if (outer_band(a.g, i, gj)) bdy_gather<NL>(a.g, a.adj_qbfac, a.dc->mx, a.adj_lq, i, gj, lp);
+    const Geo& g = a.g;
+    const double* lq = a.adj_lq;
+    const double qb = a.adj_qbfac;
+    const long long o = (long long)gj * g.pitch + i;
+    if (i == 1) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o - 1];
+    }
+    if (i == g.nxp - 2) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o + 1];
+    }
+    if (gj == 1) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o - g.pitch];
+    }
+    if (gj == g.nyp - 2) {
+#pragma unroll
+        for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o + g.pitch];
+    }
     double w = 0.0;
 #pragma unroll
     for (int k = 0; k < NL; k++) w += cm[k] * lp[k];
     return w;
+}
+
+// Extra point e of a row pair's adjoint inner products: the first nside are
+// columns 0 and nxp-1 of its rows gj0 (and gj1), the rest its share of the
+// ring rows 0 and nyp-1 from ring index rlo on.
+__device__ __forceinline__ void extra_point(const Geo& g, int e, int nside, int rlo, int gj0, int gj1, int& i, int& gj) {
+    if (e < nside) {
+        gj = (e >> 1) ? gj1 : gj0;
+        i = (e & 1) ? g.nxp - 1 : 0;
+    } else {
+        const int idx = rlo + (e - nside);
+        gj = idx < g.nxp ? 0 : g.nyp - 1;
+        i = idx < g.nxp ? idx : idx - g.nxp;
+    }
 }
 
 // Row pair jp of mode mb (or of the raw field).  ADJ: the adjoint mode of
@@ -70,8 +106,13 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     const bool has1 = r1 < s.nyi;
     const int gj0 = r0 + 1, gj1 = r1 + 1;
     double2* tw = zbuf + zpad_len(L);
-    Staged<4, double2> stw;   // the whole table for L <= 1024
-    stage_load(stw, a.tw, twiddle_table_len(L));
+    // the twiddle table, staged in double2 (the forward kernel) or as doubles
+    // (the adjoint one, where the compiler kept a staged double2 array in
+    // scratch memory)
+    Staged<4, double2> stw2;
+    Staged<8, double> stw1;
+    if constexpr (ADJ) stage_load(stw1, a.tw, 2 * twiddle_table_len(L));
+    else stage_load(stw2, a.tw, twiddle_table_len(L));
     const bool raw = !ADJ && a.raw_mode >= 0;
     const double* qin = ADJ ? a.adj_lp : a.q;
     // inputs of this thread's (at most four) points of both rows, loaded together
@@ -117,19 +158,9 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     const int rlo = min(nring, jp * rper), rhi = min(nring, rlo + rper);
     const int nside = has1 ? 4 : 2;
     const int nextra = ADJ ? nside + (rhi - rlo) : 0;
-    auto extra_point = [&](int e, int& i, int& gj) {
-        if (e < nside) {
-            gj = (e >> 1) ? gj1 : gj0;
-            i = (e & 1) ? g.nxp - 1 : 0;
-        } else {
-            const int idx = rlo + (e - nside);
-            gj = idx < g.nxp ? 0 : g.nyp - 1;
-            i = idx < g.nxp ? idx : idx - g.nxp;
-        }
-    };
     int ei = 0, ej = 0;
     const bool ehas = (int)threadIdx.x < nextra;
-    if (ehas) extra_point(threadIdx.x, ei, ej);
+    if (ehas) extra_point(g, threadIdx.x, nside, rlo, gj0, gj1, ei, ej);
     const bool erow = ej == 0 || ej == g.nyp - 1;
     const bool ewc = ehas && erow && ei == 1, eec = ehas && erow && ei == g.nxp - 2;
     double elp[NL], elq[NL], elw[NL], ele[NL], ehm = 0.0;
@@ -197,19 +228,23 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         // points beyond one per thread (very wide, short basins)
         for (int e = threadIdx.x + blockDim.x; e < nextra; e += blockDim.x) {
             int i, gj;
-            extra_point(e, i, gj);
+            extra_point(g, e, nside, rlo, gj0, gj1, i, gj);
             const long long o = (long long)gj * g.pitch + i;
             double lp[NL];
 #pragma unroll
             for (int k = 0; k < NL; k++) lp[k] = a.adj_lp[k * g.fs + o];
-            const double w = adj_modal_input<NL>(a, cm, lp, i, gj);
+            bdy_gather<NL>(g, a.adj_qbfac, a.dc->mx, a.adj_lq, i, gj, lp);   // a ring point
+            double w = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) w += cm[k] * lp[k];
             if (m > 0) part += a.adj_hom[(long long)(m - 1) * g.fs + o] * w;
         }
         double pv[1] = {part};
         block_sum<1>(pv, s_red);
         if (threadIdx.x == 0) a.adj_partial[(long long)jp * MAXL + m] = pv[0];
     }
-    stage_store(stw, tw, a.tw, twiddle_table_len(L));
+    if constexpr (ADJ) stage_store(stw1, (double*)tw, a.tw, 2 * twiddle_table_len(L));
+    else stage_store(stw2, tw, a.tw, twiddle_table_len(L));
     if (!g.cyclic) {
         // odd extension of the two rows: z_t = x_t + i y_t, z_{L-t} = -z_t, z_0 = z_N = 0
         if (threadIdx.x == 0) {
