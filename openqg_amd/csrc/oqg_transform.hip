@@ -95,7 +95,7 @@ __device__ __forceinline__ void extra_point(const Geo& g, int e, int nside, int 
 // Row pair jp of mode mb (or of the raw field).  ADJ: the adjoint mode of
 // the box inversion (FwdArgs::adj_*), whose input of mode m is
 // w_m = sum_k ctm2l(m,k) lp_k.
-template <int NL, int MB, bool ADJ = false>
+template <int NL, int MB, bool ADJ = false, bool STAGE_DOUBLES = ADJ>
 __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     extern __shared__ double2 zbuf[];   // [zpad_len(L)] transform, then the twiddle table
     const Geo& g = a.g;
@@ -107,11 +107,11 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     const int gj0 = r0 + 1, gj1 = r1 + 1;
     double2* tw = zbuf + zpad_len(L);
     // the twiddle table, staged in double2 (the forward kernel) or as doubles
-    // (the adjoint one, where the compiler kept a staged double2 array in
-    // scratch memory)
+    // (the adjoint and ensemble forms, where the compiler kept a staged double2
+    // array in scratch memory)
     Staged<4, double2> stw2;
     Staged<8, double> stw1;
-    if constexpr (ADJ) stage_load(stw1, a.tw, 2 * twiddle_table_len(L));
+    if constexpr (STAGE_DOUBLES) stage_load(stw1, a.tw, 2 * twiddle_table_len(L));
     else stage_load(stw2, a.tw, twiddle_table_len(L));
     const bool raw = !ADJ && a.raw_mode >= 0;
     const double* qin = ADJ ? a.adj_lp : a.q;
@@ -243,7 +243,7 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         block_sum<1>(pv, s_red);
         if (threadIdx.x == 0) a.adj_partial[(long long)jp * MAXL + m] = pv[0];
     }
-    if constexpr (ADJ) stage_store(stw1, (double*)tw, a.tw, 2 * twiddle_table_len(L));
+    if constexpr (STAGE_DOUBLES) stage_store(stw1, (double*)tw, a.tw, 2 * twiddle_table_len(L));
     else stage_store(stw2, tw, a.tw, twiddle_table_len(L));
     if (!g.cyclic) {
         // odd extension of the two rows: z_t = x_t + i y_t, z_{L-t} = -z_t, z_0 = z_N = 0
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_fwd_ens(const FwdArgs* __restrict__ arg
     const FwdArgs& a = args[e.x];
     const int npairs = (a.s.nyi + 1) / 2;
     const int mb = e.y / npairs;
-    fwd_block<NL, MB>(a, e.y - mb * npairs, mb);
+    fwd_block<NL, MB, false, true>(a, e.y - mb * npairs, mb);
 }
 
 // Instantiations over the layer count (2..8) and butterflies per thread (1, 2, 4).
