@@ -290,8 +290,16 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     }
 }
 
+// Row pair of workgroup b of n in the step kernels: with n a multiple of 8,
+// XCD x takes row pairs [x n/8, (x+1) n/8), the rows that k_vort's tiles on
+// the same XCD cover, so q_new (read by k_fwd) and p_new (read by the next
+// k_vort) mostly stay in the L2 that wrote them.
+__device__ __forceinline__ int xcd_row_pair(int b, int n) { return (n % NXCD == 0) ? xcd_contig(b, n) : b; }
+
 template <int NL, int MB>
-__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) { fwd_block<NL, MB>(a, blockIdx.x, blockIdx.y); }
+__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
+    fwd_block<NL, MB>(a, xcd_row_pair(blockIdx.x, gridDim.x), blockIdx.y);
+}
 
 template <int NL, int MB>
 __global__ __launch_bounds__(256) void k_fwd_adj(FwdArgs a) { fwd_block<NL, MB, true>(a, blockIdx.x, blockIdx.y); }
@@ -1126,7 +1134,7 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
 
 template <int NL, int MB>
 __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv(InvArgs a) {
-    inv_block<NL, MB>(a, blockIdx.x);
+    inv_block<NL, MB>(a, xcd_row_pair(blockIdx.x, gridDim.x));
 }
 
 template <int NL, int MB>
