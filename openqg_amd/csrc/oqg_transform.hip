@@ -1005,11 +1005,43 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
         __syncthreads();
     }
 
+    // The first and last row pairs also write the zonal boundary rows.  The
+    // homogeneous-solution values these need (boundary row, and the row or
+    // corner column that ocqbdy pairs with it) are fetched into the transform
+    // buffers, now free, by waves 1.. while thread 0 computes the correction
+    // coefficients.  Box, nl <= 3, where the buffers hold both sides:
+    // zh[side][0: boundary, 1: inner][m][i].
+    const bool zone = !raw && !g.cyclic && NL <= 3 && (first || last);
+    double* zh = (double*)fbuf;
+    auto zone_fetch = [&]() {
+        for (int t = (int)threadIdx.x - WAVE; t >= 0 && t < 2 * g.nxp; t += (int)blockDim.x - WAVE) {
+            const int side = t / g.nxp, i = t - side * g.nxp;
+            if ((side == 0 && !first) || (side == 1 && !last)) continue;
+            const int gj = side == 0 ? 0 : g.nyp - 1, gin = side == 0 ? 1 : g.nyp - 2;
+            const bool corner = i == 0 || i == g.nxp - 1;
+            const long long ob = (long long)gj * g.pitch + i;
+            const long long oi = corner ? (long long)gj * g.pitch + (i == 0 ? 1 : g.nxp - 2) : (long long)gin * g.pitch + i;
+            double hb[NL > 1 ? NL - 1 : 1], hi[NL > 1 ? NL - 1 : 1];
+#pragma unroll
+            for (int m = 0; m < NL - 1; m++) {
+                hb[m] = a.hom[(long long)m * g.fs + ob];
+                hi[m] = a.hom[(long long)m * g.fs + oi];
+            }
+#pragma unroll
+            for (int m = 0; m < NL - 1; m++) {
+                zh[((side * 2 + 0) * (NL - 1) + m) * g.nxp + i] = hb[m];
+                zh[((side * 2 + 1) * (NL - 1) + m) * g.nxp + i] = hi[m];
+            }
+        }
+    };
+
     // ---- correction coefficients (identical in every block)
     if (!raw) {
         block_sum_finish<NL + 1>(v, red);
         if (threadIdx.x == 0)
             step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, jp == 0);
+        else if (zone)
+            zone_fetch();
         __syncthreads();
     }
 
@@ -1116,16 +1148,22 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
         const int selin = side == 0 ? 0 : ((gin == gj1 && has1) ? 1 : 0);
         for (int i = threadIdx.x; i < g.nxp; i += blockDim.x) {
             double pb[NL], pin[NL];
-            pmix<NL>(a, inh, coef, s_ctm2l, i, 2, gj, pb, hnone, false);
+            double hb[NL > 1 ? NL - 1 : 1], hi[NL > 1 ? NL - 1 : 1];   // fetched above (zone)
+#pragma unroll
+            for (int m = 0; m < (NL > 1 ? NL - 1 : 1); m++) {
+                hb[m] = zone ? zh[((side * 2 + 0) * (NL - 1) + m) * g.nxp + i] : 0.0;
+                hi[m] = zone ? zh[((side * 2 + 1) * (NL - 1) + m) * g.nxp + i] : 0.0;
+            }
+            pmix<NL>(a, inh, coef, s_ctm2l, i, 2, gj, pb, hb, zone);
             const long long o = (long long)gj * g.pitch + i;
 #pragma unroll
             for (int k = 0; k < NL; k++) a.pnew[k * g.fs + o] = pb[k];
             if (!g.cyclic && (i == 0 || i == g.nxp - 1)) {
                 // corners take the x difference (vorsubs.F90:83-84 overwrite :79-80)
-                pmix<NL>(a, inh, coef, s_ctm2l, i == 0 ? 1 : g.nxp - 2, 2, gj, pin, hnone, false);
+                pmix<NL>(a, inh, coef, s_ctm2l, i == 0 ? 1 : g.nxp - 2, 2, gj, pin, hi, zone);
                 qbdy<NL>(a, s_amat, i, gj, 2, pb, pin);
             } else {
-                pmix<NL>(a, inh, coef, s_ctm2l, i, selin, gin, pin, hnone, false);
+                pmix<NL>(a, inh, coef, s_ctm2l, i, selin, gin, pin, hi, zone);
                 qbdy<NL>(a, s_amat, i, gj, 0, pb, pin);
             }
         }
