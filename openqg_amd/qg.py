@@ -34,6 +34,8 @@ class Box:
     cyclic: bool = False
     dz_sign: int = -1
     hm: float = 0.0
+    x0: float = 0.0   # position of p point (1,1) in its mesh (box.F90:62-63)
+    y0: float = 0.0
 
     @property
     def nxt(self):
@@ -55,6 +57,14 @@ class Box:
     def yprel(self):
         j = np.arange(self.nyp, dtype=np.float64)
         return -0.5 * self.yl + j * self.dy  # box.F90:144
+
+    @property
+    def xp(self):
+        return self.x0 + np.arange(self.nxp, dtype=np.float64) * self.dx   # box.F90:154
+
+    @property
+    def yp(self):
+        return self.y0 + np.arange(self.nyp, dtype=np.float64) * self.dy   # box.F90:143
 
 
 @dataclass
