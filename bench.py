@@ -16,7 +16,9 @@ Also reported:
                 after another by the plain path (aggregate simulated days/s);
   adjoint       gradient of a kinetic-energy objective after 30 days of the
                 257^2 3-layer box (forward + hand-written adjoint) against the
-                plain forward run, store-all and with 32 revolve snapshots.
+                plain forward run, store-all and with 32 revolve snapshots;
+                and a batch of 16 perturbed members' gradients, one after
+                another and concurrently (one context and stream each).
 """
 from __future__ import annotations
 
@@ -199,7 +201,7 @@ def ensemble_bench(members: int, days: float):
     }
 
 
-def adjoint_bench(days: float, snapshots: int):
+def adjoint_bench(days: float, snapshots: int, batch: int = 16):
     """Gradient of the layer-1 kinetic energy after `days` of the 257^2 3-layer
     box with respect to the wind stress and the initial PV: forward + adjoint
     (store-all snapshots, and binomial checkpointing with `snapshots` slots)
@@ -235,7 +237,46 @@ def adjoint_bench(days: float, snapshots: int):
         out[name] = {"s": round(el, 4), "ratio_vs_forward": round(el / fwd, 3), "forward_steps": int(fsteps)}
     out["J"] = J
     qg.close()
+    if batch > 1:
+        out["batch"] = adjoint_batch_bench(w, n, snapshots, batch)
     return out
+
+
+def adjoint_batch_bench(w, n: int, snapshots: int, members: int):
+    """The gradient of `members` perturbed members (initial PV seed and wind
+    amplitude vary), each on its own context and stream with binomial
+    checkpointing: one after another, then concurrently from host threads."""
+    import dataclasses
+
+    from openqg_amd.adjoint import gradient_batch
+    from openqg_amd.forcing import double_gyre_tau
+
+    qgs, q0s, txs, tys = [], [], [], []
+    for i in range(members):
+        wi = dataclasses.replace(w, seed=w.seed + i, tau0=w.tau0 * (0.9 + 0.2 * i / max(members - 1, 1)))
+        qg = wi.make_qg()
+        p = wi.initial_p()
+        qg.set_state(p=p, pm=p)
+        qg.init_state()
+        q0s.append(qg.get_state()[0])
+        tx, ty = double_gyre_tau(wi.nyp, wi.nxp, wi.dx, wi.tau0)
+        txs.append(tx)
+        tys.append(ty)
+        qgs.append(qg)
+    gradient_batch(qgs, q0s, txs, tys, w.tdt, n, w.supp_every, snapshots)   # allocation, graph capture
+    t0 = time.perf_counter()
+    seq = gradient_batch(qgs, q0s, txs, tys, w.tdt, n, w.supp_every, snapshots, concurrent=False)
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    con = gradient_batch(qgs, q0s, txs, tys, w.tdt, n, w.supp_every, snapshots)
+    t_con = time.perf_counter() - t0
+    import numpy as np
+
+    same = all(a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[3], b[3]) for a, b in zip(seq, con))
+    for qg in qgs:
+        qg.close()
+    return {"members": members, "snapshots": snapshots, "s": round(t_con, 4), "sequential_s": round(t_seq, 4),
+            "speedup_vs_sequential": round(t_seq / t_con, 3), "identical_to_sequential": bool(same)}
 
 
 def main():
@@ -254,6 +295,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
     ap.add_argument("--adj-days", type=float, default=30.0)
     ap.add_argument("--adj-snapshots", type=int, default=32)
+    ap.add_argument("--adj-batch", type=int, default=16, help="members of the batched gradient (<= 1: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 10 device copies of 256 MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -328,7 +370,7 @@ def main():
         cpu = cpu_baseline(w, args.cpu_seconds)
     qg.close()
     ens = None if args.no_ensemble else ensemble_bench(args.ens_members, args.ens_days)
-    adj = None if args.no_adjoint else adjoint_bench(args.adj_days, args.adj_snapshots)
+    adj = None if args.no_adjoint else adjoint_bench(args.adj_days, args.adj_snapshots, args.adj_batch)
     out = {
         "metric": METRIC,
         "value": round(days_per_s, 3),

@@ -345,6 +345,30 @@ def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: fl
     return J, gtx, gty, gq0, traj.forward_steps
 
 
+def gradient_batch(qgs, q0s, tauxs, tauys, tdt: float, nsteps: int, supp_every: int = 25,
+                   snapshots: Optional[int] = None, rho: float = 1000.0, concurrent: bool = True):
+    """`gradient` for a batch of members, member i on its own context qgs[i]
+    (each with its own stream) from (q0s[i], tauxs[i], tauys[i]).
+
+    With `concurrent` every member runs in its own host thread: the library
+    releases the interpreter lock inside each call and a context touches only
+    its own stream and memory, so the members' kernels overlap on the device.
+    The results equal those of running the members one after another.
+    Returns the list of `gradient`'s tuples in member order."""
+    n = len(qgs)
+    if not (len(q0s) == len(tauxs) == len(tauys) == n):
+        raise ValueError("one q0, taux and tauy per member")
+    if len({id(q) for q in qgs}) != n:
+        raise ValueError("every member needs its own context")
+    run1 = lambda i: gradient(qgs[i], q0s[i], tauxs[i], tauys[i], tdt, nsteps, supp_every, snapshots, rho)
+    if not concurrent or n == 1:
+        return [run1(i) for i in range(n)]
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        return list(ex.map(run1, range(n)))
+
+
 def objective(qg: QG, q0, taux, tauy, tdt: float, nsteps: int, supp_every: int = 25, rho: float = 1000.0) -> float:
     """The forward map of `gradient` alone (for finite-difference checks)."""
     b = qg.b
