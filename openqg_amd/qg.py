@@ -258,7 +258,8 @@ class QG:
 
     def diagnose(self, dt: float) -> dict:
         """qg_diagno's state energetics and averages, scan_cfl's maxima and the
-        count of non-finite values, computed on the device (oqg_diagnose)."""
+        count of non-finite values, computed on the device (oqg_diagnose); in a
+        cyclic basin also check_continuity's ermas and emfr (tdt = 2 dt)."""
         d = _lib.OqgDiag()
         check(self._lib.oqg_diagnose(self._h, float(dt), self.rho, ctypes.byref(d)), self._h, "oqg_diagnose")
         nl = self.b.nl
@@ -269,7 +270,27 @@ class QG:
                 out[name] = v
             else:
                 out[name] = np.array(v[: nl - 1] if name in ("etam", "et2m") else v[:nl])
+        if self.b.cyclic:
+            out["ermas"], out["emfr"] = self._continuity(out["pavg"], 2.0 * float(dt))
         return out
+
+    def _continuity(self, pavg, tdt):
+        """check_continuity (qg_monitor.F90:571-604, cyclic basins only): the
+        interface volume from the layer integrals of p (avg_P times the area)
+        against the constraint's dpi.  Returns (ermas, emfr) per interface."""
+        b = self.b
+        nl = b.nl
+        area = b.xl * b.yl
+        int_p = np.asarray(pavg) * area
+        dpi = self.get_constr()[: nl - 1]
+        ermas, emfr = np.zeros(nl - 1), np.zeros(nl - 1)
+        for k in range(nl - 1):
+            est1 = b.dz_sign * (int_p[k] - int_p[k + 1])
+            est2 = dpi[k]
+            esum = abs(est1) + abs(est2)
+            ermas[k] = est1 - est2
+            emfr[k] = 2.0 * ermas[k] / esum if esum > 1.0e-13 * area * tdt * self.gp[k] else 0.0
+        return ermas, emfr
 
     def get_forcing(self):
         """(wekp (nyp, nxp), txis, txin) as resident on the device."""

@@ -90,3 +90,20 @@ def diagnose(q, p, dx, dy, fnot, bcco, cyclic, h, gp, dz_sign, rho, dt):
     out["ctot"] = float(out["circ"].sum())
     out["nonfinite"] = float(np.count_nonzero(~np.isfinite(p)) + np.count_nonzero(~np.isfinite(q)))
     return out
+
+
+def check_continuity(int_p, dpi, gp, dz_sign, xl, yl, tdt):
+    """check_continuity (qg_monitor.F90:571-604), called for cyclic basins
+    (:378-381): per interface k, ermas = dz_sign (int_p_k - int_p_k+1) - dpi_k
+    and emfr = 2 ermas / (|est1| + |est2|) when that sum exceeds
+    1e-13 xl yl tdt gp_k, else 0.  Returns (ermas, emfr)."""
+    nl = len(int_p)
+    ermas, emfr = np.zeros(nl - 1), np.zeros(nl - 1)
+    for k in range(nl - 1):
+        est1 = dz_sign * (int_p[k] - int_p[k + 1])
+        est2 = dpi[k]
+        edif = est1 - est2
+        esum = abs(est1) + abs(est2)
+        ermas[k] = edif
+        emfr[k] = 2.0 * edif / esum if esum > 1.0e-13 * xl * yl * tdt * gp[k] else 0.0
+    return ermas, emfr

@@ -56,3 +56,34 @@ def test_nonfinite_values_are_counted(gpu):
     p[0, 0, 0] = -np.inf
     qg.set_state(q, qm, p, pm)
     assert qg.diagnose(w.dt)["nonfinite"] == 3.0
+
+
+def test_continuity_check_matches_oracle(gpu):
+    """check_continuity (qg_monitor.F90:571-604) on a channel evolved with
+    entrainment: ermas and emfr from the device's layer integrals and the
+    constraint's dpi against the oracle's int_P_dA of the same p."""
+    w = W.small_channel()
+    qg = w.make_qg()
+    p = w.initial_p()
+    qg.set_state(p=p, pm=p)
+    qg.init_state()
+    ent, wekp, txis, txin = w.forcing()
+    ent = 1e-6 * np.random.default_rng(w.seed + 100).standard_normal(ent.shape)
+    ent[..., -1] = ent[..., 0]
+    qg.set_forcing(ent, wekp, txis, txin)
+    print(f"[diag] continuity {w.name}", flush=True)
+    qg.run(w.tdt, 30, 0, w.supp_every)
+    qg.sync()
+    d = qg.diagnose(w.dt)
+    _, _, p, _ = qg.get_state()
+    nl = w.nl
+    int_p = [M.int_P_dA(p[k], w.dx, w.dx) for k in range(nl)]
+    dpi = qg.get_constr()[: nl - 1]
+    assert np.any(dpi != 0.0)   # the entrainment moved the interfaces
+    xl, yl = (w.nxp - 1) * w.dx, (w.nyp - 1) * w.dx
+    ermas, emfr = M.check_continuity(int_p, dpi, w.gp, w.params()["dz_sign"], xl, yl, w.tdt)
+    for k in range(nl - 1):
+        esum = abs(w.params()["dz_sign"] * (int_p[k] - int_p[k + 1])) + abs(dpi[k])
+        assert abs(d["ermas"][k] - ermas[k]) <= 1e-12 * esum, (k, d["ermas"], ermas)
+        assert abs(d["emfr"][k] - emfr[k]) <= 4e-12 + 1e-9 * abs(emfr[k]), (k, d["emfr"], emfr)
+    print(f"[diag] ermas {d['ermas']} emfr {d['emfr']}", flush=True)

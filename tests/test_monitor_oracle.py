@@ -62,3 +62,18 @@ def test_diagnostics_of_linear_layers():
     q2 = q.copy()
     q2[0, 0, 0] = np.inf
     assert M.diagnose(q2, p2, DX, DY, F0, ALPHA, False, h, [0.02], -1, rho, dt)["nonfinite"] == 2.0
+
+
+def test_check_continuity_known_answers():
+    # continuity holds: dz_sign (int_p_k - int_p_k+1) == dpi_k
+    e, f = M.check_continuity([3.0, 1.0, -2.0], [2.0, 3.0], [0.02, 0.01], 1, 1e5, 1e5, 2400.0)
+    assert list(e) == [0.0, 0.0] and list(f) == [0.0, 0.0]
+    # the other sign convention: est1 = -(2, 3) against dpi = (2, 3)
+    e, f = M.check_continuity([3.0, 1.0, -2.0], [2.0, 3.0], [0.02, 0.01], -1, 1e5, 1e5, 2400.0)
+    assert list(e) == [-4.0, -6.0] and list(f) == [-2.0, -2.0]
+    # below 1e-13 xl yl tdt gp the fraction is reported as 0, the error is not
+    # (threshold 1e-13 * 1e3 * 1e3 * 100 * 0.02 = 2e-7)
+    e, f = M.check_continuity([0.0, 0.0], [1e-8], [0.02], 1, 1e3, 1e3, 100.0)
+    assert e[0] == -1e-8 and f[0] == 0.0
+    e, f = M.check_continuity([0.0, 0.0], [1e-6], [0.02], 1, 1e3, 1e3, 100.0)
+    assert e[0] == -1e-6 and f[0] == -2.0
