@@ -81,3 +81,47 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+class EnsembleGroups:
+    """Members split into `groups` batched ensembles of about equal work per
+    step (largest members first, each to the group with the least work so far),
+    each ensemble on its own stream.  `run` enqueues every group before any
+    finishes, so one group's long launches (its largest members) overlap the
+    other groups' phases instead of leaving the device partly idle at the end
+    of each phase.  Each member's trajectory is unchanged."""
+
+    def __init__(self, members: Sequence[QG], groups: int = 2):
+        self.members = list(members)
+        n = len(self.members)
+        groups = max(1, min(int(groups), n))
+        work = [m.b.nxp * m.b.nyp * m.b.nl for m in self.members]
+        load_ = [0] * groups
+        idx = [[] for _ in range(groups)]
+        for i in sorted(range(n), key=lambda i: -work[i]):
+            g = int(np.argmin(load_))
+            idx[g].append(i)
+            load_[g] += work[i]
+        self.index = [sorted(g) for g in idx if g]
+        self.parts = [Ensemble([self.members[i] for i in g]) for g in self.index]
+
+    def _split(self, v):
+        v = np.broadcast_to(np.asarray(v), (len(self.members),))
+        return [v[g] for g in self.index]
+
+    def run(self, tdt, nsteps, step0: int = 0, supp_every: int = 25):
+        """As Ensemble.run; the groups run concurrently."""
+        for e, t, s in zip(self.parts, self._split(tdt), self._split(nsteps)):
+            e.run(t, s, step0, supp_every)
+
+    def sync(self):
+        for e in self.parts:
+            e.sync()
+
+    def tune(self, tdt, trial_steps: int = 8, supp_every: int = 25):
+        """Ensemble.tune for every group; returns [(schedule, [ms])] per group."""
+        return [e.tune(t, trial_steps, supp_every) for e, t in zip(self.parts, self._split(tdt))]
+
+    def close(self):
+        for e in self.parts:
+            e.close()

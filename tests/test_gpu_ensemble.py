@@ -90,3 +90,22 @@ def test_ensemble_runs_continue_and_tune_restores_state(gpu):
         for a, b in zip(qg.get_state(), st):
             assert np.array_equal(a, b)
     ens.close()
+
+
+def test_ensemble_groups_equal_members_stepped_alone(gpu):
+    from openqg_amd.ensemble import EnsembleGroups
+
+    ws = _members()
+    nsteps = [7, 12, 12, 5, 9, 12]
+    ref = _alone(ws, nsteps, 4)
+    qgs = _make(ws)
+    grp = EnsembleGroups(qgs, 2)
+    assert sorted(i for g in grp.index for i in g) == list(range(len(ws))) and len(grp.index) == 2
+    print(f"[ensemble] two groups {grp.index}", flush=True)
+    grp.run([w.tdt for w in ws], nsteps, 0, 4)
+    grp.sync()
+    for i, (qg, (st, con)) in enumerate(zip(qgs, ref)):
+        for name, a, b in zip(("q", "qm", "p", "pm"), qg.get_state(), st):
+            assert np.array_equal(a, b), f"member {i} {name}"
+        assert np.allclose(qg.get_constr(), con, rtol=1e-10, atol=0.0)
+    grp.close()
