@@ -9,7 +9,7 @@ HIP graphs.  A member's trajectory is the one it would follow stepped alone.
 from __future__ import annotations
 
 import ctypes
-from typing import Sequence
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -89,16 +89,23 @@ class EnsembleGroups:
     each ensemble on its own stream.  `run` enqueues every group before any
     finishes, so one group's long launches (its largest members) overlap the
     other groups' phases instead of leaving the device partly idle at the end
-    of each phase.  Each member's trajectory is unchanged."""
+    of each phase.  Members of at least `solo_points` grid points (nxp nyp)
+    are not batched: each replays its own graphs (`QG.run`) on its own stream,
+    beside the groups; such a member fills the device on its own, and batching
+    it only makes the small members' blocks wait behind it.  Each member's
+    trajectory is unchanged."""
 
-    def __init__(self, members: Sequence[QG], groups: int = 2):
+    def __init__(self, members: Sequence[QG], groups: int = 2, solo_points: Optional[int] = None):
         self.members = list(members)
         n = len(self.members)
-        groups = max(1, min(int(groups), n))
+        pts = [m.b.nxp * m.b.nyp for m in self.members]
+        self.solo = [i for i in range(n) if solo_points is not None and pts[i] >= solo_points]
+        rest = [i for i in range(n) if i not in self.solo]
+        groups = max(1, min(int(groups), max(len(rest), 1)))
         work = [m.b.nxp * m.b.nyp * m.b.nl for m in self.members]
         load_ = [0] * groups
         idx = [[] for _ in range(groups)]
-        for i in sorted(range(n), key=lambda i: -work[i]):
+        for i in sorted(rest, key=lambda i: -work[i]):
             g = int(np.argmin(load_))
             idx[g].append(i)
             load_[g] += work[i]
@@ -110,11 +117,18 @@ class EnsembleGroups:
         return [v[g] for g in self.index]
 
     def run(self, tdt, nsteps, step0: int = 0, supp_every: int = 25):
-        """As Ensemble.run; the groups run concurrently."""
+        """As Ensemble.run; the solo members and the groups run concurrently."""
+        t_all = np.broadcast_to(np.asarray(tdt, dtype=np.float64), (len(self.members),))
+        s_all = np.broadcast_to(np.asarray(nsteps), (len(self.members),))
+        for i in self.solo:
+            if s_all[i] > 0:
+                self.members[i].run(float(t_all[i]), int(s_all[i]), step0, supp_every)
         for e, t, s in zip(self.parts, self._split(tdt), self._split(nsteps)):
             e.run(t, s, step0, supp_every)
 
     def sync(self):
+        for i in self.solo:
+            self.members[i].sync()
         for e in self.parts:
             e.sync()
 

@@ -92,16 +92,19 @@ def test_ensemble_runs_continue_and_tune_restores_state(gpu):
     ens.close()
 
 
-def test_ensemble_groups_equal_members_stepped_alone(gpu):
+@pytest.mark.parametrize("solo", [False, True], ids=["groups", "groups_and_solo"])
+def test_ensemble_groups_equal_members_stepped_alone(gpu, solo):
     from openqg_amd.ensemble import EnsembleGroups
 
     ws = _members()
     nsteps = [7, 12, 12, 5, 9, 12]
     ref = _alone(ws, nsteps, 4)
     qgs = _make(ws)
-    grp = EnsembleGroups(qgs, 2)
-    assert sorted(i for g in grp.index for i in g) == list(range(len(ws))) and len(grp.index) == 2
-    print(f"[ensemble] two groups {grp.index}", flush=True)
+    big = max(w.nxp * w.nyp for w in ws)
+    grp = EnsembleGroups(qgs, 2, solo_points=big if solo else None)
+    assert sorted(grp.solo + [i for g in grp.index for i in g]) == list(range(len(ws)))
+    assert len(grp.index) == 2 and (len(grp.solo) > 0) == solo
+    print(f"[ensemble] groups {grp.index} solo {grp.solo}", flush=True)
     grp.run([w.tdt for w in ws], nsteps, 0, 4)
     grp.sync()
     for i, (qg, (st, con)) in enumerate(zip(qgs, ref)):
