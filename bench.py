@@ -18,7 +18,9 @@ Also reported:
                 257^2 3-layer box (forward + hand-written adjoint) against the
                 plain forward run, store-all and with 32 revolve snapshots;
                 and a batch of 16 perturbed members' gradients, one after
-                another and concurrently (one context and stream each).
+                another and concurrently (one context and stream each);
+  coupled       the coupled configuration's 385^2 ocean and 97 x 49 atmosphere
+                (QG only) through the run loop for 30 days.
 """
 from __future__ import annotations
 
@@ -264,6 +266,55 @@ def adjoint_bench(days: float, snapshots: int, batch: int = 16):
     return out
 
 
+def coupled_bench(days: float):
+    """The coupled configuration's two basins, QG only: the 385^2 ocean box
+    (dt = 1200 s) and the 97 x 49 atmosphere channel (dt = 200 s, nstr = 6)
+    through the run loop (openqg_amd.driver: the reference's step cadence,
+    daily validity checks, the lastday dump) for `days` simulated days, each
+    basin on its own stream on its resident forcing; against the two basins'
+    plain runs one after the other."""
+    import tempfile
+
+    from openqg_amd import workloads as W
+    from openqg_amd.driver import Basin, Clock, Valids, run
+
+    wo, wa = W.ocean385(), W.atmos_channel()
+    nstr = int(round(wo.dt / wa.dt))
+
+    def make(w):
+        qg = w.make_qg()
+        p = w.initial_p()
+        qg.set_state(p=p, pm=p)
+        qg.init_state()
+        qg.set_forcing(*w.forcing())
+        return qg
+
+    o, a = make(wo), make(wa)
+    with tempfile.TemporaryDirectory() as out:
+        val = lambda: Valids(int(round(86400.0 / wa.dt)), 1.0e12, 1.0e3)
+        warm = Clock(dta=wa.dt, nstr=nstr, tini=0.0, trun=1.0 / 365.0)
+        run(warm, out, ocn=Basin("ocean", o, val()), atm=Basin("atmos", a, val()))   # graph capture
+        clk = Clock(dta=wa.dt, nstr=nstr, tini=1.0 / 365.0, trun=days / 365.0)
+        t0 = time.perf_counter()
+        run(clk, out, ocn=Basin("ocean", o, val()), atm=Basin("atmos", a, val()))
+        t_cpl = time.perf_counter() - t0
+    no, na = clk.ntsrun // nstr, clk.ntsrun
+    t0 = time.perf_counter()
+    o.run(clk.tdto, no, 0, 25)
+    o.sync()
+    t_o = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    a.run(clk.tdta, na, 0, 100)
+    a.sync()
+    t_a = time.perf_counter() - t0
+    o.close()
+    a.close()
+    return {"workload": f"coupled_ocean385_atmos97x49_qg_only_{days:g}d", "ocean_steps": no, "atmos_steps": na,
+            "value": round(days / t_cpl, 2), "unit": "simulated days/s", "s": round(t_cpl, 4),
+            "ocean_alone_s": round(t_o, 4), "atmos_alone_s": round(t_a, 4),
+            "vs_basins_one_after_other": round((t_o + t_a) / t_cpl, 3)}
+
+
 def adjoint_batch_bench(w, n: int, snapshots: int, members: int):
     """The gradient of `members` perturbed members (initial PV seed and wind
     amplitude vary), each on its own context and stream with binomial
@@ -318,6 +369,7 @@ def main():
     ap.add_argument("--adj-days", type=float, default=30.0)
     ap.add_argument("--adj-snapshots", type=int, default=32)
     ap.add_argument("--adj-batch", type=int, default=16, help="members of the batched gradient (<= 1: skip)")
+    ap.add_argument("--no-coupled", action="store_true", help="skip the coupled two-basin run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 10 device copies of 256 MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -393,6 +445,7 @@ def main():
     qg.close()
     ens = None if args.no_ensemble else ensemble_bench(args.ens_members, args.ens_days)
     adj = None if args.no_adjoint else adjoint_bench(args.adj_days, args.adj_snapshots, args.adj_batch)
+    cpl = None if args.no_coupled else coupled_bench(30.0)
     out = {
         "metric": METRIC,
         "value": round(days_per_s, 3),
@@ -412,6 +465,7 @@ def main():
         "cpu_baseline": cpu,
         "ensemble": ens,
         "adjoint": adj,
+        "coupled": cpl,
     }
     print(json.dumps(out), flush=True)
 
