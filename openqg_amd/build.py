@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -38,11 +39,22 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every source and link liboqg.so; returns the library path."""
+    """Compile every source and link liboqg.so; returns the library path.
+
+    Objects go to a private temporary directory, so an interrupted build or two
+    builds at once leave nothing behind in the source tree."""
+    objdir = tempfile.mkdtemp(prefix="oqg_build_")
+    try:
+        return _build_in(objdir, verbose)
+    finally:
+        shutil.rmtree(objdir, ignore_errors=True)
+
+
+def _build_in(objdir: str, verbose: bool) -> str:
     objs = []
     procs = []
     for src in SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
         cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -55,14 +67,14 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed ({' '.join(cmd)}):\n{text}")
         if verbose and text.strip():
             print(text)
-    tmp = LIB + ".tmp"
+    tmp = os.path.join(objdir, "liboqg.so")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout.decode(errors='replace')}")
-    os.replace(tmp, LIB)
-    for o in objs:
-        os.remove(o)
+    shutil.copyfile(tmp, LIB + ".tmp")
+    os.chmod(LIB + ".tmp", 0o755)
+    os.replace(LIB + ".tmp", LIB)
     return LIB
 
 

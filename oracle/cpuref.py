@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libcpuref.so")
+# OQG_CPUREF_LIB points the tests at another build of the oracle (the mutation
+# tests load copies with planted mistakes this way).
+_LIB_PATH = os.environ.get("OQG_CPUREF_LIB") or os.path.join(_HERE, "libcpuref.so")
 MAXL = 8
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 

@@ -1,0 +1,82 @@
+"""The oracle's pins are sharp: plausible slips planted in a copy of oracle/cpuref.c fail them.
+
+Each case rewrites one line of the oracle (a sign, an index, a transposed
+operand, a dropped factor in the routines of src/qg/vorsubs.F90,
+constraint.F90, homog.F90, vorticity.F90 and glam/numerics.F90 that
+tests/test_oracle_pins.py fixes), compiles the copy into a temporary
+directory and runs the pin tests against it (OQG_CPUREF_LIB); they must fail.
+The oracle in the tree is never modified.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "oracle", "cpuref.c")
+CFLAGS = ["-O1", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-std=c11", "-shared"]
+
+# (case, line in the oracle, planted replacement)
+M = [
+ ("qcomp beta sign", "E3(q, i, j, k) = E3(q, i, j, k) + b->P.beta * b->yprel[j - 1];\n        for (int i = 1; i <= NXP; i++)\n            for (int k = 1; k <= nl; k++) {",
+                     "E3(q, i, j, k) = E3(q, i, j, k) - b->P.beta * b->yprel[j - 1];\n        for (int i = 1; i <= NXP; i++)\n            for (int k = 1; k <= nl; k++) {"),
+ ("qcomp A transposed", "for (int l = 1; l <= nl; l++) s += M2(b->amat, k, l) * E3(p, i, j, l);\n                E3(q, i, j, k) = E3(q, i, j, k) - b->P.fnot * s;\n            }\n    }\n}",
+                        "for (int l = 1; l <= nl; l++) s += M2(b->amat, l, k) * E3(p, i, j, l);\n                E3(q, i, j, k) = E3(q, i, j, k) - b->P.fnot * s;\n            }\n    }\n}"),
+ ("qcomp ddyn layer", "for (int i = 1; i <= NXP; i++) E3(q, i, j, kt) = E3(q, i, j, kt) + E2(b->ddyn, i, j);",
+                      "for (int i = 1; i <= NXP; i++) E3(q, i, j, 1) = E3(q, i, j, 1) + E2(b->ddyn, i, j);"),
+ ("ocqbdy north diff sign", "E3(q, i, NYP, k) = fac * (E3(p, i, NYP - 1, k) - E3(p, i, NYP, k));",
+                            "E3(q, i, NYP, k) = fac * (E3(p, i, NYP, k) - E3(p, i, NYP - 1, k));"),
+ ("ocqbdy west index", "E3(q, 1, j, k) = fac * (E3(p, 2, j, k) - E3(p, 1, j, k));",
+                       "E3(q, 1, j, k) = fac * (E3(p, 3, j, k) - E3(p, 1, j, k));"),
+ ("ocqbdy corner beta once", "E3(q, 1, j, k) = E3(q, 1, j, k) + b->P.beta * b->yprel[j - 1];",
+                             "if (j > 1 && j < NYP) E3(q, 1, j, k) = E3(q, 1, j, k) + b->P.beta * b->yprel[j - 1];"),
+ ("dpi from dpi", "b->dpi[l] = b->dpip[l] - tdt * b->P.gp[l] * exn;", "b->dpi[l] = b->dpi[l] - tdt * b->P.gp[l] * exn;"),
+ ("dpi gp index", "b->dpi[l] = b->dpip[l] - tdt * b->P.gp[l] * exn;", "b->dpi[l] = b->dpip[l] - tdt * b->P.gp[l + 1] * exn;"),
+ ("init_constraint cs sign", "b->cs[k - 1] = -b->cs[k - 1] + c * aps;", "b->cs[k - 1] = b->cs[k - 1] + c * aps;"),
+ ("init_constraint A transposed", "apsp += M2(b->amat, k, l) * pinsp[l - 1];", "apsp += M2(b->amat, l, k) * pinsp[l - 1];"),
+ ("init_constraint cn row", "row[i - 1] = E3(b->p, i, NYP, k) - E3(b->p, i, NYP - 1, k);\n            b->cn[k - 1]",
+                            "row[i - 1] = E3(b->p, i, NYP - 1, k) - E3(b->p, i, NYP - 2, k);\n            b->cn[k - 1]"),
+ ("ajis weight", "*ajis = (aj5sms + 2.0 * aj9sms) / 6.0;", "*ajis = (2.0 * aj5sms + aj9sms) / 6.0;"),
+ ("ajin sign", "*ajin = -(aj5smn + 2.0 * aj9smn) / 6.0;", "*ajin = (aj5smn + 2.0 * aj9smn) / 6.0;"),
+ ("ajis dp row", "row[i - 1] = E2(q, i, 1) * E2(dpx, i, 2);", "row[i - 1] = E2(q, i, 1) * E2(dpx, i, 1);"),
+ ("step_constraints from cs", "double csnew = b->csp[k] + tdt * rhss[k];", "double csnew = b->cs[k] + tdt * rhss[k];"),
+ ("int_Be_n fohfac(1)", "b->int_Be_n[0] = -b->P.dz_sign * fohfac[1]", "b->int_Be_n[0] = -b->P.dz_sign * fohfac[0]"),
+ ("txin sign", "rhsn[0] = rhsn[0] - b->P.tau_sign * (f0 / b->P.h[0]) * b->txin;", "rhsn[0] = rhsn[0] + b->P.tau_sign * (f0 / b->P.h[0]) * b->txin;"),
+ ("ap3n sign", "rhsn[k] = rhsn[k] + b->ajin[k] + b->ap3n[k] - b->ap5n[k];", "rhsn[k] = rhsn[k] + b->ajin[k] - b->ap3n[k] - b->ap5n[k];"),
+ ("ap5s from del2p", "row[i - 1] = E3(d4p, i, 2, k) - E3(d4p, i, 1, k);", "row[i - 1] = E3(del2p, i, 2, k) - E3(del2p, i, 1, k);"),
+ ("bdrins layer", "row[i - 1] = E3(b->pm, i, 2, kt) - E3(b->pm, i, 1, kt);", "row[i - 1] = E3(b->pm, i, 2, 1) - E3(b->pm, i, 1, 1);"),
+ ("bottom drag to layer kt", "rhss[nl - 1] = rhss[nl - 1] + (f0 / b->P.h[nl - 1]) * b->bdrins;", "rhss[0] = rhss[0] + (f0 / b->P.h[nl - 1]) * b->bdrins;"),
+ ("cyclic_homog c_bc2 sign", "c2[m - 1] = b->hc1s[m - 1] * clhsn[m - 1] - b->hc1n[m - 1] * clhss[m - 1];",
+                             "c2[m - 1] = b->hc1n[m - 1] * clhsn[m - 1] - b->hc1s[m - 1] * clhss[m - 1];"),
+ ("cyclic_homog row nyp-1", "clhsn[m - 1] = n + cr_int_P_dx(b, &E3(inh, 1, NYP - 1, m)) / b->P.dy;", "clhsn[m - 1] = n + cr_int_P_dx(b, &E3(inh, 1, NYP - 2, m)) / b->P.dy;"),
+ ("homsol pch rdm2 term", "pch1ys = pch1ys + 0.5 * dy * b->rdm2[m - 1] * h1[0] * xl;", "pch1ys = pch1ys - 0.5 * dy * b->rdm2[m - 1] * h1[0] * xl;"),
+ ("hbsi inverted", "b->hbsi = b->yl / b->xl;", "b->hbsi = b->xl / b->yl;"),
+]
+
+
+def _caught(case, tmp):
+    name, old, new = case
+    text = open(SRC).read()
+    assert text.count(old) == 1, f"{name}: the oracle line to mutate is not unique"
+    d = os.path.join(tmp, name.replace(" ", "_"))
+    os.makedirs(d)
+    c = os.path.join(d, "cpuref.c")
+    open(c, "w").write(text.replace(old, new))
+    with open(os.path.join(ROOT, "oracle", "cpuref.h")) as h:
+        open(os.path.join(d, "cpuref.h"), "w").write(h.read())
+    so = os.path.join(d, "libcpuref.so")
+    subprocess.run(["gcc", *CFLAGS, "-o", so, c, "-lm"], check=True, capture_output=True)
+    env = dict(os.environ, OQG_CPUREF_LIB=so, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
+                        os.path.join(ROOT, "tests", "test_oracle_pins.py"), os.path.join(ROOT, "tests", "test_oracle.py")],
+                       cwd=ROOT, env=env, capture_output=True, text=True)
+    return name, r.returncode != 0
+
+
+def test_every_planted_mistake_fails_the_pins(tmp_path):
+    with ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
+        res = list(ex.map(lambda c: _caught(c, str(tmp_path)), M))
+    missed = [n for n, ok in res if not ok]
+    assert not missed, f"planted mistakes not caught: {missed}"
