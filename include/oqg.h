@@ -233,6 +233,19 @@ typedef struct oqg_diag {
 
 int oqg_diagnose(oqg_ctx* ctx, double dt, double rho, oqg_diag* out);
 
+/* oqg_diagnose without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (serving loops, harnesses
+ * that forbid hipMalloc).  oqg_diagnose allocates its per-row partials on its
+ * first call; this variant keeps them, and their combination, in a
+ * library-wide device array of 2 MiB (262144 doubles; one per GPU, held until
+ * the process ends), so it serves nyp * 97 + 105 <= 262144, i.e. nyp <= 2701
+ * (every basin the library accepts), and returns OQG_ERR_ARG otherwise.  It
+ * calls no hipMalloc, hipFree or other allocating routine and gives the same
+ * result as oqg_diagnose, bit for bit.  All calls on one GPU share the array:
+ * calls that overlap on one GPU (from several threads or streams) race, so
+ * serialise them. */
+int oqg_diagnose_noalloc(oqg_ctx* ctx, double dt, double rho, oqg_diag* out);
+
 /* The inversion of one step applied to the current q (interior): p from q
  * (modes, Helmholtz solves, mass correction with the current dpi), q's
  * boundary PV by ocqbdy, and qm = q, pm = p.  oqg_adj_invert is its adjoint:

@@ -88,6 +88,7 @@ SIGNATURES = [
     ("oqg_adj_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_get_forcing", _i, [_vp, _vp, _dp, _i]),
     ("oqg_diagnose", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
+    ("oqg_diagnose_noalloc", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
     ("oqg_invert", _i, [_vp]),
     ("oqg_adj_invert", _i, [_vp]),
     ("oqg_snap_alloc", _i, [_vp, _i]),

@@ -80,6 +80,8 @@ struct oqg_ctx {
     double* taux_d = nullptr;   // kinematic wind stress at p points (oqg_set_wind / oqg_adj_wind)
     double* tauy_d = nullptr;
     double* diag_part = nullptr;   // per-row diagnostics partials (oqg_diagnose)
+    int dev = 0;                   // the HIP device the context was created on
+    double* scratch = nullptr;     // this device's address of the no-alloc scratch array
     hipStream_t own = nullptr;
     hipStream_t st = nullptr;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -858,6 +860,10 @@ int oqg_create(const oqg_desc* desc, oqg_ctx** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(nullptr, OQG_ERR_HIP, "no HIP device available");
     auto c = std::make_unique<oqg_ctx>();
+    if (hipGetDevice(&c->dev) != hipSuccess) return fail(nullptr, OQG_ERR_HIP, "hipGetDevice failed");
+    // the device address of oqg_diagnose_noalloc's scratch array, once per device
+    if (hipError_t e = noalloc_scratch(c->dev, &c->scratch); e != hipSuccess)
+        return fail(nullptr, OQG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
     c->d = d;
     c->d.ddyn = nullptr;   // borrowed pointers are not kept
     c->d.amat = c->d.ctl2m = c->d.ctm2l = c->d.rdm2 = nullptr;
@@ -1110,32 +1116,28 @@ int oqg_get_forcing(oqg_ctx* c, double* wekp, double* txi, int kind) {
     return 0;
 }
 
-int oqg_diagnose(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
-    if (!c || !out) return OQG_ERR_ARG;
-    const Geo& g = c->g;
-    const int nl = g.nl, ny = g.nyp;
-    if (!c->diag_part)
-        if (int r = dalloc(c, &c->diag_part, (size_t)ny * DG_LEN)) return r;
-    DiagArgs a{};
-    a.g = g;
-    for (int k = 0; k < nl - 1; k++) a.gp[k] = c->d.gp[k];
+}  // extern "C"
+
+static void diag_args(oqg_ctx* c, double* partial, DiagArgs& a) {
+    a = DiagArgs{};
+    a.g = c->g;
+    for (int k = 0; k < c->g.nl - 1; k++) a.gp[k] = c->d.gp[k];
     a.dz_sign = c->d.dz_sign;
     a.bcco = c->d.bcco;
     a.q = c->qbuf[c->cur];
     a.p = c->pbuf[c->cur];
-    a.partial = c->diag_part;
-    launch_diag(a, c->st);
-    HIPCHK(c, hipGetLastError());
-    std::vector<double> h((size_t)ny * DG_LEN);
-    HIPCHK(c, hipMemcpyAsync(h.data(), c->diag_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    // the rows in order (deterministic)
-    double sum[5][MAXL] = {}, mn[6][MAXL], bad = 0.0;
+    a.partial = partial;
+}
+
+// The rows' partials [nyp][DG_LEN] combined in row order into comb[DG_COMB_LEN]
+// (the same operations, in the same order, as k_diag_combine).
+static void diag_combine_host(const double* h, int ny, int nl, double* comb) {
     const int kinds[6] = {DG_PMIN, DG_PMAX, DG_QMIN, DG_QMAX, DG_UMAX, DG_VMAX};
+    double sum[5][MAXL] = {}, mn[6][MAXL] = {}, bad = 0.0;
     for (int t = 0; t < 6; t++)
         for (int k = 0; k < nl; k++) mn[t][k] = h[kinds[t] * MAXL + k];
     for (int j = 0; j < ny; j++) {
-        const double* r = h.data() + (size_t)j * DG_LEN;
+        const double* r = h + (size_t)j * DG_LEN;
         for (int k = 0; k < nl; k++) {
             sum[0][k] += r[DG_KE * MAXL + k];
             sum[1][k] += r[DG_P * MAXL + k];
@@ -1149,34 +1151,96 @@ int oqg_diagnose(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
         }
         bad += r[DG_BAD];
     }
+    for (int k = 0; k < MAXL; k++) {
+        for (int t = 0; t < 5; t++) comb[DC_SUM + t * MAXL + k] = sum[t][k];
+        for (int t = 0; t < 6; t++) comb[DC_EXT + t * MAXL + k] = mn[t][k];
+        comb[DC_P0FIRST + k] = h[DG_P0 * MAXL + k];
+        comb[DC_P0LAST + k] = h[(size_t)(ny - 1) * DG_LEN + DG_P0 * MAXL + k];
+    }
+    comb[DC_BAD] = bad;
+}
+
+// qg_diagno's state energetics and averages and scan_cfl's maxima from the
+// combined rows.
+static void diag_finish(const oqg_ctx* c, const double* comb, double dt, double rho, oqg_diag* out) {
+    const Geo& g = c->g;
+    const int nl = g.nl, ny = g.nyp;
     const double dA = g.dx * g.dy, area = ((g.nxp - 1) * g.dx) * ((ny - 1) * g.dy), f0 = g.fnot;
     std::memset(out, 0, sizeof *out);
-    const double* p_first = h.data() + DG_P0 * MAXL;                      // row 0, column 1
-    const double* p_last = h.data() + (size_t)(ny - 1) * DG_LEN + DG_P0 * MAXL;   // row nyp, column 1
+    const double* sum0 = comb + DC_SUM;
+    auto sum = [&](int t, int k) { return sum0[t * MAXL + k]; };
+    auto ext = [&](int t, int k) { return comb[DC_EXT + t * MAXL + k]; };
+    const double* p_first = comb + DC_P0FIRST;   // row 0, column 1
+    const double* p_last = comb + DC_P0LAST;     // row nyp, column 1
     for (int k = 0; k < nl; k++) {
-        out->ekei[k] = 0.5 * rho * c->d.h[k] * (sum[0][k] * dA);
-        out->pavg[k] = (sum[1][k] * dA) / area;
-        out->qavg[k] = (sum[2][k] * dA) / area;
-        out->pmin[k] = mn[0][k];
-        out->pmax[k] = mn[1][k];
-        out->qmin[k] = mn[2][k];
-        out->qmax[k] = mn[3][k];
-        out->umax[k] = mn[4][k];
-        out->vmax[k] = mn[5][k];
-        out->cflu[k] = mn[4][k] * dt / g.dx;
-        out->cflv[k] = mn[5][k] * dt / g.dx;
+        out->ekei[k] = 0.5 * rho * c->d.h[k] * (sum(0, k) * dA);
+        out->pavg[k] = (sum(1, k) * dA) / area;
+        out->qavg[k] = (sum(2, k) * dA) / area;
+        out->pmin[k] = ext(0, k);
+        out->pmax[k] = ext(1, k);
+        out->qmin[k] = ext(2, k);
+        out->qmax[k] = ext(3, k);
+        out->umax[k] = ext(4, k);
+        out->vmax[k] = ext(5, k);
+        out->cflu[k] = ext(4, k) * dt / g.dx;
+        out->cflv[k] = ext(5, k) * dt / g.dx;
         out->circ[k] = 1.0e-6 * ((p_first[k] - p_last[k]) / f0);
         out->ctot += out->circ[k];
         const double poref = f0 > 0.0 ? p_first[k] : p_last[k];
-        const double lo = std::fmin(mn[0][k] / f0, mn[1][k] / f0), hi = std::fmax(mn[0][k] / f0, mn[1][k] / f0);
+        const double lo = std::fmin(ext(0, k) / f0, ext(1, k) / f0), hi = std::fmax(ext(0, k) / f0, ext(1, k) / f0);
         out->sfmin[k] = 1.0e-6 * (c->d.h[k] * (lo - poref / f0));
         out->sfmax[k] = 1.0e-6 * (c->d.h[k] * (hi - poref / f0));
         if (k < nl - 1) {
-            out->etam[k] = (sum[3][k] * dA) / area;
-            out->et2m[k] = (sum[4][k] * dA) / area;
+            out->etam[k] = (sum(3, k) * dA) / area;
+            out->et2m[k] = (sum(4, k) * dA) / area;
         }
     }
-    out->nonfinite = bad;
+    out->nonfinite = comb[DC_BAD];
+}
+
+extern "C" {
+
+int oqg_diagnose(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
+    if (!c || !out) return OQG_ERR_ARG;
+    const int nl = c->g.nl, ny = c->g.nyp;
+    if (!c->diag_part)
+        if (int r = dalloc(c, &c->diag_part, (size_t)ny * DG_LEN)) return r;
+    DiagArgs a;
+    diag_args(c, c->diag_part, a);
+    launch_diag(a, c->st);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h((size_t)ny * DG_LEN);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->diag_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    double comb[DG_COMB_LEN];
+    diag_combine_host(h.data(), ny, nl, comb);   // the rows in order (deterministic)
+    diag_finish(c, comb, dt, rho, out);
+    return 0;
+}
+
+int oqg_diagnose_noalloc(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
+    if (!c || !out) return OQG_ERR_ARG;
+    const int nl = c->g.nl, ny = c->g.nyp;
+    const long long need = (long long)ny * DG_LEN + DG_COMB_LEN;
+    if (need > NOALLOC_SCRATCH_DOUBLES)
+        return fail(c, OQG_ERR_ARG, "oqg_diagnose_noalloc: nyp = " + std::to_string(ny) + " needs " +
+                                        std::to_string(need) + " doubles of scratch, more than its " +
+                                        std::to_string(NOALLOC_SCRATCH_DOUBLES));
+    double* scr = c->scratch;
+    if (!scr) {
+        HIPCHK(c, noalloc_scratch(c->dev, &scr));
+        c->scratch = scr;
+    }
+    double* comb_d = scr + (long long)ny * DG_LEN;
+    DiagArgs a;
+    diag_args(c, scr, a);
+    launch_diag(a, c->st);   // writes every slot that the combination reads
+    launch_diag_combine(scr, ny, nl, comb_d, c->st);
+    HIPCHK(c, hipGetLastError());
+    double comb[DG_COMB_LEN];
+    HIPCHK(c, hipMemcpyAsync(comb, comb_d, sizeof comb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    diag_finish(c, comb, dt, rho, out);
     return 0;
 }
 

@@ -11,43 +11,116 @@
 // values of p and q (the check valids makes, valsubs.F90:46-139, extended to
 // NaN and Inf).  One block per row; the host combines the rows in order.
 
+#include <mutex>
+
 #include "oqg_device.h"
 
 namespace oqg {
 
-template <int NL>
+// Scratch memory of oqg_diagnose_noalloc: the per-row partials of k_diag
+// ([nyp][DG_LEN]) followed by their combination (DG_COMB_LEN doubles).  One
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to the device: it uses the address hipGetSymbolAddress returns
+// for each device (noalloc_scratch).
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int dev, double** out) {
+    *out = nullptr;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int prev = -1;
+        hipError_t e = hipGetDevice(&prev);
+        if (e != hipSuccess) return e;
+        if (prev != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (prev != dev) {
+            const hipError_t e2 = hipSetDevice(prev);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+// The rows' partials combined in row order, one thread per accumulator (each
+// accumulator sees the rows in order, with the operations of the host
+// combination in oqg_diagnose, so the two agree bit for bit): sums of the five
+// summed quantities, the six extrema, p at column 1 of the first and last rows
+// and the non-finite count.  Layers k >= nl are written as zeros.
+__global__ __launch_bounds__(128) void k_diag_combine(const double* partial, int ny, int nl, double* comb) {
+    const int t = threadIdx.x;
+    if (t >= DG_COMB_LEN) return;
+    if (t == DC_BAD) {
+        double bad = 0.0;
+        for (int j = 0; j < ny; j++) bad += partial[(long long)j * DG_LEN + DG_BAD];
+        comb[DC_BAD] = bad;
+        return;
+    }
+    const int k = t % MAXL;
+    double v = 0.0;
+    if (k < nl) {
+        if (t < DC_EXT) {               // sums: ke, p, q, eta, eta^2
+            const int slot = (t / MAXL) * MAXL + k;   // DG_KE..DG_ETA2 are 0..4
+            for (int j = 0; j < ny; j++) v += partial[(long long)j * DG_LEN + slot];
+        } else if (t < DC_P0FIRST) {    // extrema, seeded with row 0
+            const int e = t / MAXL - 5;
+            const int slot = (DG_PMIN + e) * MAXL + k;   // DG_PMIN..DG_VMAX are consecutive
+            const bool is_min = (e == 0 || e == 2);
+            v = partial[slot];
+            for (int j = 0; j < ny; j++) {
+                const double x = partial[(long long)j * DG_LEN + slot];
+                v = is_min ? fmin(v, x) : fmax(v, x);
+            }
+        } else if (t < DC_P0LAST) {
+            v = partial[DG_P0 * MAXL + k];
+        } else {
+            v = partial[(long long)(ny - 1) * DG_LEN + DG_P0 * MAXL + k];
+        }
+    }
+    comb[t] = v;
+}
+
+void launch_diag_combine(const double* partial, int ny, int nl, double* comb, hipStream_t st) {
+    hipLaunchKernelGGL(k_diag_combine, dim3(1), dim3(128), 0, st, partial, ny, nl, comb);
+}
+
+// One block per row, the layers one after another: per thread a few
+// accumulators in registers (no per-thread arrays, so no private memory), then
+// block reductions per layer.
 __global__ __launch_bounds__(256) void k_diag(DiagArgs a) {
     const Geo& g = a.g;
-    const int nx = g.nxp, ny = g.nyp, j = blockIdx.x;
+    const int nx = g.nxp, ny = g.nyp, nl = g.nl, j = blockIdx.x;
     const long long pitch = g.pitch;
-    constexpr int NS = 5 * NL + 1;   // ke, p, q (NL each), eta, eta^2 (NL-1 used each), non-finite count
-    constexpr int NM = 6 * NL;       // min p, max p, min q, max q, max |u|, max |v|
-    __shared__ double red_s[NS * 16];
-    __shared__ double red_m[NM * 16];
+    __shared__ double red_s[6 * 16];
+    __shared__ double red_m[6 * 16];
     const double wj = (j == 0 || j == ny - 1) ? 0.5 : 1.0;
     const double ca = 0.5 * a.bcco + 1.0;
     const double inf = __builtin_huge_val();
-    double s[NS], mm[NM];
-#pragma unroll
-    for (int t = 0; t < NS; t++) s[t] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NL; k++) {
-        mm[k] = inf;            // min p
-        mm[NL + k] = -inf;      // max p
-        mm[2 * NL + k] = inf;   // min q
-        mm[3 * NL + k] = -inf;  // max q
-        mm[4 * NL + k] = 0.0;   // max |u|
-        mm[5 * NL + k] = 0.0;   // max |v|
-    }
-    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-        const double w = wj * ((i == 0 || i == nx - 1) ? 0.5 : 1.0);
-        const long long o = (long long)j * pitch + i;
-        double pk[NL];
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            const double* P = a.p + k * g.fs;
-            const double pc = P[o], qc = a.q[k * g.fs + o];
-            pk[k] = pc;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    double* out = a.partial + (long long)j * DG_LEN;
+    double bad_total = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < nl; k++) {
+        const double* P = a.p + k * g.fs;
+        const double* Q = a.q + k * g.fs;
+        const double* P2 = (k < nl - 1) ? P + g.fs : P;
+        const double gpk = (k < nl - 1) ? a.gp[k] : 1.0;
+        double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // ke, p, q, eta, eta^2, non-finite (registers)
+        double pmin = inf, pmax = -inf, qmin = inf, qmax = -inf, umax = 0.0, vmax = 0.0;
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const double w = wj * ((i == 0 || i == nx - 1) ? 0.5 : 1.0);
+            const long long o = (long long)j * pitch + i;
+            const double pc = P[o], qc = Q[o];
             double dpdx = 0.0, dpdy = 0.0;
             if (j != 0 && j != ny - 1) {
                 if (i >= 1 && i <= nx - 2) dpdx = (P[o + 1] - P[o - 1]) / (2.0 * g.dx);
@@ -59,69 +132,58 @@ __global__ __launch_bounds__(256) void k_diag(DiagArgs a) {
                 else if (j == 0) dpdy = (P[o + pitch] - P[o]) / (g.dy * ca);
                 else dpdy = (P[o] - P[o - pitch]) / (g.dy * ca);
             }
-            const double u = -dpdy / g.fnot, v = dpdx / g.fnot;
-            s[k] += w * (u * u + v * v);
-            s[NL + k] += w * pc;
-            s[2 * NL + k] += w * qc;
-            mm[k] = fmin(mm[k], pc);
-            mm[NL + k] = fmax(mm[NL + k], pc);
-            mm[2 * NL + k] = fmin(mm[2 * NL + k], qc);
-            mm[3 * NL + k] = fmax(mm[3 * NL + k], qc);
-            if (j <= ny - 2) mm[4 * NL + k] = fmax(mm[4 * NL + k], fabs(-((1.0 / g.dy) * (P[o + pitch] - pc)) / g.fnot));
-            if (i <= nx - 2) mm[5 * NL + k] = fmax(mm[5 * NL + k], fabs(((1.0 / g.dx) * (P[o + 1] - pc)) / g.fnot));
-            s[5 * NL] += (isfinite(pc) ? 0.0 : 1.0) + (isfinite(qc) ? 0.0 : 1.0);
+            const double u = -dpdy / g.fnot, vv = dpdx / g.fnot;
+            v[0] += w * (u * u + vv * vv);
+            v[1] += w * pc;
+            v[2] += w * qc;
+            pmin = fmin(pmin, pc);
+            pmax = fmax(pmax, pc);
+            qmin = fmin(qmin, qc);
+            qmax = fmax(qmax, qc);
+            if (j <= ny - 2) umax = fmax(umax, fabs(-((1.0 / g.dy) * (P[o + pitch] - pc)) / g.fnot));
+            if (i <= nx - 2) vmax = fmax(vmax, fabs(((1.0 / g.dx) * (P[o + 1] - pc)) / g.fnot));
+            v[5] += (isfinite(pc) ? 0.0 : 1.0) + (isfinite(qc) ? 0.0 : 1.0);
+            if (k < nl - 1) {
+                const double eta = a.dz_sign * (pc - P2[o]) / gpk;
+                v[3] += w * eta;
+                v[4] += w * (eta * eta);
+            }
         }
+        block_sum<6>(v, red_s);
+        const double m0 = wave_min(pmin), m1 = wave_max(pmax), m2 = wave_min(qmin), m3 = wave_max(qmax),
+                     m4 = wave_max(umax), m5 = wave_max(vmax);
+        if (lane == 0) {
+            red_m[0 * 16 + wv] = m0;
+            red_m[1 * 16 + wv] = m1;
+            red_m[2 * 16 + wv] = m2;
+            red_m[3 * 16 + wv] = m3;
+            red_m[4 * 16 + wv] = m4;
+            red_m[5 * 16 + wv] = m5;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            out[DG_KE * MAXL + k] = v[0];
+            out[DG_P * MAXL + k] = v[1];
+            out[DG_Q * MAXL + k] = v[2];
+            out[DG_ETA * MAXL + k] = (k < nl - 1) ? v[3] : 0.0;
+            out[DG_ETA2 * MAXL + k] = (k < nl - 1) ? v[4] : 0.0;
+            out[DG_P0 * MAXL + k] = P[(long long)j * pitch];
 #pragma unroll
-        for (int k = 0; k < NL - 1; k++) {
-            const double eta = a.dz_sign * (pk[k] - pk[k + 1]) / a.gp[k];
-            s[3 * NL + k] += w * eta;
-            s[4 * NL + k] += w * (eta * eta);
+            for (int e = 0; e < 6; e++) {
+                const bool is_min = (e == 0 || e == 2);
+                double r = red_m[e * 16];
+                for (int q = 1; q < nw; q++) r = is_min ? fmin(r, red_m[e * 16 + q]) : fmax(r, red_m[e * 16 + q]);
+                out[(DG_PMIN + e) * MAXL + k] = r;
+            }
+            bad_total += v[5];
         }
+        __syncthreads();
     }
-    block_sum<NS>(s, red_s);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-    for (int t = 0; t < NM; t++) {
-        const bool is_min = (t < NL) || (t >= 2 * NL && t < 3 * NL);
-        const double r = is_min ? wave_min(mm[t]) : wave_max(mm[t]);
-        if (lane == 0) red_m[t * 16 + wv] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double* out = a.partial + (long long)j * DG_LEN;
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            out[DG_KE * MAXL + k] = s[k];
-            out[DG_P * MAXL + k] = s[NL + k];
-            out[DG_Q * MAXL + k] = s[2 * NL + k];
-            out[DG_ETA * MAXL + k] = (k < NL - 1) ? s[3 * NL + k] : 0.0;
-            out[DG_ETA2 * MAXL + k] = (k < NL - 1) ? s[4 * NL + k] : 0.0;
-            out[DG_P0 * MAXL + k] = a.p[k * g.fs + (long long)j * pitch];
-        }
-        const int kinds[6] = {DG_PMIN, DG_PMAX, DG_QMIN, DG_QMAX, DG_UMAX, DG_VMAX};
-#pragma unroll
-        for (int t = 0; t < NM; t++) {
-            const bool is_min = (t < NL) || (t >= 2 * NL && t < 3 * NL);
-            double r = red_m[t * 16];
-            for (int q = 1; q < nw; q++) r = is_min ? fmin(r, red_m[t * 16 + q]) : fmax(r, red_m[t * 16 + q]);
-            out[kinds[t / NL] * MAXL + (t % NL)] = r;
-        }
-        out[DG_BAD] = s[5 * NL];
-    }
+    if (threadIdx.x == 0) out[DG_BAD] = bad_total;
 }
 
 void launch_diag(const DiagArgs& a, hipStream_t st) {
-    const dim3 grid(a.g.nyp), block(256);
-    switch (a.g.nl) {
-        case 2: hipLaunchKernelGGL(k_diag<2>, grid, block, 0, st, a); break;
-        case 3: hipLaunchKernelGGL(k_diag<3>, grid, block, 0, st, a); break;
-        case 4: hipLaunchKernelGGL(k_diag<4>, grid, block, 0, st, a); break;
-        case 5: hipLaunchKernelGGL(k_diag<5>, grid, block, 0, st, a); break;
-        case 6: hipLaunchKernelGGL(k_diag<6>, grid, block, 0, st, a); break;
-        case 7: hipLaunchKernelGGL(k_diag<7>, grid, block, 0, st, a); break;
-        case 8: hipLaunchKernelGGL(k_diag<8>, grid, block, 0, st, a); break;
-        default: break;
-    }
+    hipLaunchKernelGGL(k_diag, dim3(a.g.nyp), dim3(256), 0, st, a);
 }
 
 }  // namespace oqg

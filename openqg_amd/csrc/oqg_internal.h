@@ -290,6 +290,16 @@ struct DiagArgs {
     double* partial;   // [nyp][DG_LEN]
 };
 void launch_diag(const DiagArgs& a, hipStream_t st);
+// The rows combined (row order): sums [5][MAXL] (ke, p, q, eta, eta^2), extrema
+// [6][MAXL] (min p, max p, min q, max q, max |u|, max |v|), p at column 1 of
+// the first and last rows, the non-finite count.
+constexpr int DC_SUM = 0, DC_EXT = 5 * MAXL, DC_P0FIRST = 11 * MAXL, DC_P0LAST = 12 * MAXL, DC_BAD = 13 * MAXL,
+              DG_COMB_LEN = 13 * MAXL + 1;
+void launch_diag_combine(const double* partial, int ny, int nl, double* comb, hipStream_t st);
+// Scratch array of oqg_diagnose_noalloc (a file-scope device array, one per
+// GPU): its device address on device `dev`, looked up once per device.
+constexpr long long NOALLOC_SCRATCH_DOUBLES = 262144;   // 2 MiB
+hipError_t noalloc_scratch(int dev, double** out);
 // Up to six contiguous segments copied in one launch (state snapshots); every
 // length is even and every pointer 16-byte aligned.  With `slots` set, the
 // destinations are offsets doff[] into snapshot slot slots[*base + t] (the
