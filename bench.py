@@ -47,7 +47,6 @@ def algorithmic_bytes(w, kernel: str) -> float:
         "k_fwd": 2 * nl * F,             # read q_new; write x spectra
         "k_ysolve": 2 * nl * F,          # read + write spectra
         "k_inv": (3 * nl - 1) * F,       # read spectra + hom solutions; write p_new
-        "k_cyc": 0.0,
         "step": (6 * nl + 2) * F,        # whole step: q, qm, p, pm, wekp, ent1 in; q, p out
     }[kernel]
 
@@ -424,8 +423,6 @@ def main():
     # step from the timed region's final state, each kernel timed over
     # KERNEL_REPS back-to-back launches on that step's inputs
     kt = qg.kernel_times(w.tdt, KERNEL_REPS)
-    if not w.cyclic:
-        kt.pop("k_cyc", None)   # not launched for a box
     dom = max((k for k in kt if kt[k] > 0), key=lambda k: kt[k])
     ach = algorithmic_bytes(w, dom) / (kt[dom] * 1e-3) / 1e9
     roofline = {

@@ -61,7 +61,7 @@ struct oqg_ctx {
     double* colsum = nullptr;
     double* colw = nullptr;
     double* entpart = nullptr;
-    double* aux = nullptr;
+    double* cpart = nullptr;   // channel: per-tile partial boundary integrals (k_vort -> k_inv)
     double* ent1 = nullptr;
     double* ent2s = nullptr;
     double* ent2n = nullptr;
@@ -400,7 +400,9 @@ static void fill_inv_common(oqg_ctx* c, InvArgs& ia) {
     ia.entpart = c->entpart;
     ia.nentpart = c->tiles_x * c->tiles_y;
     ia.hom = c->hom;
-    ia.aux = c->aux;
+    ia.cpart = c->cpart;
+    ia.ctiles = c->tiles_x;
+    ia.fcon = c->fcon;
     ia.yprel = c->yprel;
     ia.ddyn = c->ddyn;
     ia.raw_mode = -1;
@@ -437,13 +439,12 @@ static int enqueue_raw_solve(oqg_ctx* c, int m, const double* in = nullptr, doub
 }
 
 
-enum { KV = 0, KC, KF, KY, KI, NK };
-static const char* kNames[NK] = {"k_vort", "k_cyc", "k_fwd", "k_ysolve", "k_inv"};
+enum { KV = 0, KF, KY, KI, NK };
+static const char* kNames[NK] = {"k_vort", "k_fwd", "k_ysolve", "k_inv"};
 
 // Arguments of the five kernels of one step_qg from the current time level.
 struct StepArgs {
     VortArgs va;
-    CycArgs ca;
     FwdArgs fa;
     YArgs ya;
     InvArgs ia;
@@ -466,17 +467,10 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     va.tiles_x = c->tiles_x;
     va.tiles_y = c->tiles_y;
     va.org_x = vort_org_x(c->g);
-    CycArgs& ca = a.ca;
-    ca.g = c->g;
-    ca.ph = &c->dconst->ph;
-    ca.q = c->qbuf[cur];
-    ca.p = c->pbuf[cur];
-    ca.pm = c->pbuf[nxt];
-    ca.ent = c->ent1;
-    ca.ent2s = c->ent2s;
-    ca.ent2n = c->ent2n;
-    ca.aux = c->aux;
-    ca.fcon = c->fcon;
+    va.cpart = c->g.cyclic ? c->cpart : nullptr;
+    va.ent2s = c->ent2s;
+    va.ent2n = c->ent2n;
+    va.fcon = c->fcon;
     FwdArgs& fa = a.fa;
     fa.g = c->g;
     fa.s = c->s;
@@ -507,7 +501,6 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
 static void launch_kernel(oqg_ctx* c, const StepArgs& a, int k) {
     switch (k) {
         case KV: launch_vort(a.va, c->st); break;
-        case KC: if (c->g.cyclic) launch_cyc(a.ca, c->st); break;
         case KF: launch_fwd(a.fa, c->g.nl, c->st); break;
         case KY: launch_ysolve(a.ya, c->variant, c->st); break;
         default: launch_inv(a.ia, c->st); break;
@@ -646,6 +639,7 @@ static int setup(oqg_ctx* c) {
     const size_t fl = (size_t)nl * g.fs, f1 = (size_t)g.fs;
     const size_t nspec = (size_t)nl * s.nrow * s.kp;
     c->tiles_x = (vort_lim_x(g) - vort_org_x(g) + VORT_TX - 1) / VORT_TX;
+    if (g.cyclic && c->tiles_x > CYC_MAX_TX) return fail(c, OQG_ERR_ARG, "channel too wide for the boundary-integral partials");
     c->tiles_y = (nyp - 2 + VORT_TY - 1) / VORT_TY;
     int rc = 0;
     for (int t = 0; t < 2 && !rc; t++) {
@@ -659,7 +653,7 @@ static int setup(oqg_ctx* c) {
     if (!rc) rc = dalloc(c, &c->colsum, (size_t)nl * s.ncol);
     if (!rc) rc = dalloc(c, &c->colw, (size_t)s.ncol);
     if (!rc) rc = dalloc(c, &c->entpart, (size_t)c->tiles_x * c->tiles_y);
-    if (!rc) rc = dalloc(c, &c->aux, A_LEN);
+    if (!rc && g.cyclic) rc = dalloc(c, &c->cpart, (size_t)2 * nl * c->tiles_x * CYC_SLOTS);
     if (!rc) rc = dalloc(c, &c->fcon, F_LEN);
     if (!rc) rc = dalloc(c, &c->ent1, f1);
     if (!rc) rc = dalloc(c, &c->ent2s, (size_t)nxp);
@@ -1409,7 +1403,6 @@ static int time_kernels(oqg_ctx* c, double tdt, int reps, hipEvent_t* ev, double
     make_step_args(c, tdt, a);
     for (int k = 0; k < NK; k++) {
         ms[k] = 0.0;
-        if (k == KC && !c->g.cyclic) continue;
         for (int r = 0; r < reps; r++) {
             if (k == KV) HIPCHK(c, hipMemcpyAsync(tmp[r], a.va.qnew, c->fbytes, hipMemcpyDeviceToDevice, c->st));
             if (k == KY) HIPCHK(c, hipMemcpyAsync(tmp[reps + r], a.ya.spec, sbytes, hipMemcpyDeviceToDevice, c->st));
@@ -2129,7 +2122,6 @@ struct EnsPlan {
     YArgs* ya = nullptr;
     InvArgs* ia[2] = {nullptr, nullptr};
     SuppArgs* sa[2] = {nullptr, nullptr};
-    std::vector<CycArgs> cyc[2];   // channel members: k_cyc is launched per member
     std::vector<EnsLaunch> launches;
     std::vector<void*> allocs;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;   // (period, supp)
@@ -2215,7 +2207,6 @@ static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::ve
             fa.push_back(a.fa);
             ya.push_back(a.ya);
             ia.push_back(a.ia);
-            if (c->g.cyclic) P->cyc[par].push_back(a.ca);
             SuppArgs s{};   // suppression follows the step: the new level is 1 - cur
             const int cn = 1 - c->cur;
             s.g = c->g;
@@ -2308,7 +2299,6 @@ static int ens_enqueue_step(oqg_ens* e, EnsPlan* P, int par, bool supp) {
         switch (L.kind) {
             case ENS_VORT:
                 launch_vort_ens(P->va[par], L.map, L.nblocks, e->st);
-                for (const CycArgs& ca : P->cyc[par]) launch_cyc(ca, e->st);
                 break;
             case ENS_FWD: launch_fwd_ens(P->fa[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;
             case ENS_YS: launch_ysolve_ens(P->ya, L.map, L.nblocks, L.R, L.lds, e->st); break;

@@ -89,12 +89,26 @@ struct CycHom {
 //   [0, MAXL)  dpi   [MAXL, 2MAXL) dpip  then cs, cn, csp, cnp
 constexpr int C_DPI = 0, C_DPIP = MAXL, C_CS = 2 * MAXL, C_CN = 3 * MAXL, C_CSP = 4 * MAXL,
               C_CNP = 5 * MAXL, C_LEN = 6 * MAXL;
-// Cyclic per-step boundary integrals written by the constraint kernel.
+// Cyclic per-step boundary integrals of the momentum constraints, as the
+// correction code reads them (built in LDS by k_inv from the tiles' partials):
 //   ajis, ajin, ap3s, ap3n, ap5s, ap5n, int_Be_s, int_Be_n (per layer), then
 //   bdrins, bdrinn, txis, txin
 constexpr int A_AJIS = 0, A_AJIN = MAXL, A_AP3S = 2 * MAXL, A_AP3N = 3 * MAXL, A_AP5S = 4 * MAXL,
               A_AP5N = 5 * MAXL, A_BES = 6 * MAXL, A_BEN = 7 * MAXL, A_BDRS = 8 * MAXL,
               A_BDRN = 8 * MAXL + 1, A_TXIS = 8 * MAXL + 2, A_TXIN = 8 * MAXL + 3, A_LEN = 8 * MAXL + 4;
+// Channel: k_vort's first and last tile rows write per-tile partial row sums
+// (int_P_dx weights, before the factor dx) of the constraint integrals
+// (vorticity.F90:57-62,84-101, numerics.F90:374-386) for the zonal boundary
+// at side 0 (south, rows 1-2) and side 1 (north, rows nyp-1, nyp), at
+// cpart[((k * 2 + side) * tiles_x + tile) * CYC_SLOTS + quantity]:
+//   CQ_D2: Del^2 pm difference across the boundary row pair (ap3)
+//   CQ_D4: Del^4 pm difference (ap5)
+//   CQ_AJ5, CQ_AJ9: q of the boundary / inner row times dPdx_2 p of the inner row
+//   CQ_BD: pm difference (layer k_topo; bottom drag)
+//   CQ_E1, CQ_E2: layer-1 entrainment on the boundary row, layer-2 edge row (layer 1)
+enum { CQ_D2 = 0, CQ_D4, CQ_AJ5, CQ_AJ9, CQ_BD, CQ_E1, CQ_E2, CQ_N };
+constexpr int CYC_SLOTS = 8;
+constexpr int CYC_MAX_TX = 17;   // tiles per row of a channel of nxp <= 1025
 
 // Arguments of the vorticity kernel.
 struct VortArgs {
@@ -110,6 +124,12 @@ struct VortArgs {
     double tdt;
     int tiles_x, tiles_y;
     int org_x;            // first column of the tiles: 1 in a box, 0 in a channel
+    // channel: the constraint integrals' partials (cpart, see CQ_*), the
+    // layer-2 edge rows of ent and the resident forcing scalars (F_*)
+    double* cpart;
+    const double* ent2s;
+    const double* ent2n;
+    const double* fcon;
 };
 
 // Entries of the twiddle table of a length-L transform plan (the stages'
@@ -184,7 +204,9 @@ struct InvArgs {
     const double* hom;      // box: (nl-1) pitch fields; cyclic: bt(nyp), bc1(nl-1,nyp), bc2(nl-1,nyp)
     const double* cin;      // constraint state in
     double* cout;           // constraint state out
-    const double* aux;      // cyclic boundary integrals
+    const double* cpart;    // cyclic: k_vort's partial boundary integrals (CQ_*)
+    int ctiles;             // cyclic: tiles per row in cpart
+    const double* fcon;     // cyclic: resident forcing scalars (F_TXIS, F_TXIN)
     const double* yprel;
     const double* ddyn;
     double* pnew;           // p_new (pm slot)
@@ -203,20 +225,9 @@ struct InvArgs {
     double tdt;
 };
 
-struct CycArgs {
-    Geo g;
-    const Phys* ph;   // DevConsts::ph (kept out of the kernel arguments)
-    const double* q;
-    const double* p;
-    const double* pm;
-    const double* ent;      // layer 1 (pitch layout)
-    const double* ent2s;    // layer-2 row 1 (nxp)
-    const double* ent2n;    // layer-2 row nyp (nxp)
-    double* aux;
-    // Resident forcing scalars, read at run time so that a replayed graph sees
-    // the latest forcing: [F_TXIS] txis, [F_TXIN] txin, [F_ENT2] 1 when ent2s/ent2n hold data.
-    const double* fcon;
-};
+// Resident forcing scalars of a channel, read at run time so that a replayed
+// graph sees the latest forcing: [F_TXIS] txis, [F_TXIN] txin, [F_ENT2] 1 when
+// the layer-2 edge rows of ent hold data.
 constexpr int F_TXIS = 0, F_TXIN = 1, F_ENT2 = 2, F_LEN = 4;
 
 struct SuppArgs {
@@ -328,7 +339,6 @@ constexpr size_t MAX_DYN_LDS = 150 * 1024;
 
 // Kernel launchers (oqg_stencil.hip, oqg_transform.hip).
 void launch_vort(const VortArgs& a, hipStream_t st);
-void launch_cyc(const CycArgs& a, hipStream_t st);
 void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st);
 void launch_ysolve(const YArgs& a, int variant, hipStream_t st);
 void launch_inv(const InvArgs& a, hipStream_t st);

@@ -3,7 +3,8 @@
 //              (vorticity.F90:14-103, numerics.F90:340-468) on 64x16 LDS tiles with a
 //              halo of 3; writes q_new over qm on the points that ocqbdy does not
 //              rewrite (the tiles start at row 1, and at column 1 in a box);
-//   k_cyc      (channel only) boundary line integrals of the momentum constraints
+//              in a channel its first and last tile rows also write the per-tile
+//              partials of the momentum constraints' boundary line integrals
 //              (vorticity.F90:57-62,84-101, numerics.F90:374-386);
 //   k_suppress suppress_comp_mode (driver.F90:174-189);
 //   k_qcomp    qcomp (vorsubs.F90:27-56), used by init_pv.
@@ -126,6 +127,17 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
         }
     }
     const double ring_e = (EDGE && k == 0) ? a.ent1[ring_o] : 0.0;
+    // channel: wave 0 of the first tile row and wave 1 of the last one sum the
+    // south and north boundary integrals over the tile's columns; layer 1 also
+    // needs the layer-2 edge rows of ent there (ring_e is ent1 on the boundary row)
+    const bool cyc_s = EDGE && g.cyclic && a.cpart && ty == 0 && by == 0;
+    const bool cyc_n = EDGE && g.cyclic && a.cpart && ty == 1 && by == a.tiles_y - 1;
+    double e2v = 0.0, e2on = 0.0;
+    if (EDGE && k == 0 && (cyc_s || cyc_n)) {
+        const int gic = gi < g.nxp ? gi : 0;
+        e2v = cyc_s ? a.ent2s[gic] : a.ent2n[gic];
+        e2on = a.fcon[F_ENT2];
+    }
 #pragma unroll
     for (int t = 0; t < NPM; t++) {
         const int idx = tid + 256 * t;
@@ -184,6 +196,32 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
         }
     }
     __syncthreads();
+    if (EDGE && (cyc_s || cyc_n)) {
+        // rows gb0 (boundary) and gb1 (inner) of this side in the LDS tiles
+        const int gb0 = cyc_s ? 0 : g.nyp - 1, gb1 = cyc_s ? 1 : g.nyp - 2;
+        const int r2a = gb0 - (j0 - 2), r2b = gb1 - (j0 - 2);   // s_d2 rows
+        const int r1a = gb0 - (j0 - 1), r1b = gb1 - (j0 - 1);   // s_d4, s_q, s_p rows
+        const int r3a = gb0 - (j0 - 3), r3b = gb1 - (j0 - 3);   // s_pm rows
+        const double w = gi < g.nxp ? ((gi == 0 || gi == g.nxp - 1) ? 0.5 : 1.0) : 0.0;
+        const double d2a = s_d2[r2a * W2 + tx + 2], d2b = s_d2[r2b * W2 + tx + 2];
+        const double d4a = s_d4[r1a * W1 + tx + 1], d4b = s_d4[r1b * W1 + tx + 1];
+        const double dpx = s_p[r1b * W1 + tx + 2] - s_p[r1b * W1 + tx];
+        const double pma = s_pm[r3a * W3 + tx + 3], pmb = s_pm[r3b * W3 + tx + 3];
+        double v[CQ_N];
+        v[CQ_D2] = w * (cyc_s ? d2b - d2a : d2a - d2b);
+        v[CQ_D4] = w * (cyc_s ? d4b - d4a : d4a - d4b);
+        v[CQ_AJ5] = w * (s_q[r1a * W1 + tx + 1] * dpx);
+        v[CQ_AJ9] = w * (s_q[r1b * W1 + tx + 1] * dpx);
+        v[CQ_BD] = (k == g.kt) ? w * (cyc_s ? pmb - pma : pma - pmb) : 0.0;
+        v[CQ_E1] = (k == 0) ? w * ring_e : 0.0;
+        v[CQ_E2] = (k == 0 && e2on != 0.0) ? w * e2v : 0.0;
+        double* o = a.cpart + ((long long)(k * 2 + (cyc_s ? 0 : 1)) * a.tiles_x + bx) * CYC_SLOTS;
+#pragma unroll
+        for (int t = 0; t < CQ_N; t++) {
+            const double r = wave_sum(v[t]);
+            if (tx == 0) o[t] = r;
+        }
+    }
     const double c2 = a.ph->c2[k], c4 = a.ph->c4[k];
     double epart = 0.0;
 #pragma unroll
@@ -268,121 +306,6 @@ void launch_vort(const VortArgs& a, hipStream_t st) {
 
 void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, hipStream_t st) {
     hipLaunchKernelGGL(k_vort_ens, dim3(nblocks), dim3(256), 0, st, args, map);
-}
-
-// ------------------------------------------------------------------ k_cyc
-
-// One workgroup per layer.  Row integrals use the int_P_dx weights
-// (numerics.F90:206-212): 1/2 at the two ends, 1 inside, times dx.
-__global__ __launch_bounds__(256) void k_cyc(CycArgs a) {
-    extern __shared__ double smem[];
-    const Geo& g = a.g;
-    const int nx = g.nxp, ny = g.nyp;
-    const int k = blockIdx.x;
-    const long long lo = (long long)k * g.fs;
-    double* d2 = smem;            // Del^2 pm on rows 0,1,2 | ny-3,ny-2,ny-1  (6 x nx)
-    double* d4 = smem + 6 * nx;   // Del^4 pm on rows 0,1 | ny-2,ny-1         (4 x nx)
-    __shared__ double red[10 * 16];
-    const int d2rows[6] = {0, 1, 2, ny - 3, ny - 2, ny - 1};
-    const double* pm = a.pm + lo;
-    for (int idx = threadIdx.x; idx < 6 * nx; idx += 256) {
-        const int rr = idx / nx, i = idx - rr * nx;
-        const int j = d2rows[rr];
-        const int il = (i == 0) ? nx - 2 : i - 1, ir = (i == nx - 1) ? 1 : i + 1;
-        const double C = pm[(long long)j * g.pitch + i];
-        double dx2, dy2;
-        if (j == 0 || j == ny - 1) dx2 = 0.0;
-        else dx2 = g.fx * ((pm[(long long)j * g.pitch + il] + pm[(long long)j * g.pitch + ir]) - 2.0 * C);
-        if (j >= 1 && j <= ny - 2) dy2 = g.fy * ((pm[(long long)(j - 1) * g.pitch + i] + pm[(long long)(j + 1) * g.pitch + i]) - 2.0 * C);
-        else if (j == 0) dy2 = g.cyl * (pm[(long long)1 * g.pitch + i] - C);
-        else dy2 = g.cyr * (C - pm[(long long)(ny - 2) * g.pitch + i]);
-        d2[idx] = dx2 + dy2;
-    }
-    __syncthreads();
-    const int d4rows[4] = {0, 1, ny - 2, ny - 1};
-    const int d4src[4] = {0, 1, 4, 5};   // the same row's index in d2
-    for (int idx = threadIdx.x; idx < 4 * nx; idx += 256) {
-        const int rr = idx / nx, i = idx - rr * nx;
-        const int j = d4rows[rr], sr = d4src[rr];
-        const int il = (i == 0) ? nx - 2 : i - 1, ir = (i == nx - 1) ? 1 : i + 1;
-        const double C = d2[sr * nx + i];
-        double dx2, dy2;
-        if (j == 0 || j == ny - 1) dx2 = 0.0;
-        else dx2 = g.fx * ((d2[sr * nx + il] + d2[sr * nx + ir]) - 2.0 * C);
-        if (j >= 1 && j <= ny - 2) dy2 = g.fy * ((d2[(sr - 1) * nx + i] + d2[(sr + 1) * nx + i]) - 2.0 * C);
-        else if (j == 0) dy2 = g.cyl * (d2[(sr + 1) * nx + i] - C);
-        else dy2 = g.cyr * (C - d2[(sr - 1) * nx + i]);
-        d4[idx] = dx2 + dy2;
-    }
-    __syncthreads();
-    const double* q = a.q + lo;
-    const double* p = a.p + lo;
-    const double* pkt = a.pm + (long long)g.kt * g.fs;
-    double v[10];
-#pragma unroll
-    for (int t = 0; t < 10; t++) v[t] = 0.0;
-    for (int i = threadIdx.x; i < nx; i += 256) {
-        const double w = (i == 0 || i == nx - 1) ? 0.5 : 1.0;
-        const int il = (i == 0) ? nx - 2 : i - 1, ir = (i == nx - 1) ? 1 : i + 1;
-        v[0] += w * (d2[1 * nx + i] - d2[0 * nx + i]);               // ap3s
-        v[1] += w * (d2[5 * nx + i] - d2[4 * nx + i]);               // ap3n
-        v[2] += w * (d4[1 * nx + i] - d4[0 * nx + i]);               // ap5s
-        v[3] += w * (d4[3 * nx + i] - d4[2 * nx + i]);               // ap5n
-        const double dpx2 = p[(long long)1 * g.pitch + ir] - p[(long long)1 * g.pitch + il];
-        const double dpxn = p[(long long)(ny - 2) * g.pitch + ir] - p[(long long)(ny - 2) * g.pitch + il];
-        v[4] += w * (q[i] * dpx2);                                   // aj5sms
-        v[5] += w * (q[(long long)1 * g.pitch + i] * dpx2);          // aj9sms
-        v[6] += w * (q[(long long)(ny - 1) * g.pitch + i] * dpxn);   // aj5smn
-        v[7] += w * (q[(long long)(ny - 2) * g.pitch + i] * dpxn);   // aj9smn
-        v[8] += w * (pkt[(long long)1 * g.pitch + i] - pkt[i]);      // bdsums
-        v[9] += w * (pkt[(long long)(ny - 1) * g.pitch + i] - pkt[(long long)(ny - 2) * g.pitch + i]);  // bdsumn
-    }
-    block_sum<10>(v, red);
-    double e[4] = {0.0, 0.0, 0.0, 0.0};
-    if (k == 0) {
-        const bool ent2 = a.fcon[F_ENT2] != 0.0;
-        for (int i = threadIdx.x; i < nx; i += 256) {
-            const double w = (i == 0 || i == nx - 1) ? 0.5 : 1.0;
-            e[0] += w * a.ent[i];
-            e[1] += w * a.ent[(long long)(ny - 1) * g.pitch + i];
-            if (ent2) {
-                e[2] += w * a.ent2s[i];
-                e[3] += w * a.ent2n[i];
-            }
-        }
-        block_sum<4>(e, red);
-    }
-    if (threadIdx.x == 0) {
-        const double dx = g.dx, dy = g.dy;
-        double* o = a.aux;
-        o[A_AP3S + k] = a.ph->ah2[k] * (v[0] * dx) / dy;
-        o[A_AP3N + k] = a.ph->ah2[k] * (v[1] * dx) / dy;
-        o[A_AP5S + k] = a.ph->ah4[k] * (v[2] * dx) / dy;
-        o[A_AP5N + k] = a.ph->ah4[k] * (v[3] * dx) / dy;
-        const double aj5s = (v[4] * dx) / (2.0 * dx), aj9s = (v[5] * dx) / (2.0 * dx);
-        const double aj5n = (v[6] * dx) / (2.0 * dx), aj9n = (v[7] * dx) / (2.0 * dx);
-        o[A_AJIS + k] = (aj5s + 2.0 * aj9s) / 6.0;
-        o[A_AJIN + k] = -(aj5n + 2.0 * aj9n) / 6.0;
-        if (k == g.kt) {
-            o[A_BDRS] = a.ph->cd_bdr * ((v[8] * dx) / dy);
-            o[A_BDRN] = a.ph->cd_bdr * ((v[9] * dx) / dy);
-        }
-        if (k == 0) {
-            const double dz = (double)a.ph->dz_sign;
-            for (int l = 0; l < MAXL; l++) { o[A_BES + l] = 0.0; o[A_BEN + l] = 0.0; }
-            o[A_BES + 0] = dz * a.ph->fohfac[0] * (e[0] * dx);
-            o[A_BES + 1] = dz * a.ph->fohfac[0] * (e[2] * dx);
-            o[A_BEN + 0] = -dz * a.ph->fohfac[1] * (e[1] * dx);
-            o[A_BEN + 1] = -dz * a.ph->fohfac[1] * (e[3] * dx);
-            o[A_TXIS] = a.fcon[F_TXIS];
-            o[A_TXIN] = a.fcon[F_TXIN];
-        }
-    }
-}
-
-void launch_cyc(const CycArgs& a, hipStream_t st) {
-    const size_t lds = sizeof(double) * 10 * (size_t)a.g.nxp;
-    hipLaunchKernelGGL(k_cyc, dim3(a.g.nl), dim3(256), lds, st, a);
 }
 
 // ------------------------------------------------------------------ k_suppress

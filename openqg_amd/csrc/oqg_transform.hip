@@ -871,13 +871,27 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     // loads (one memory round trip) and land in LDS, so the serial
     // coefficient code below reads no global memory
     const int nw_dc = (int)(sizeof(DevConsts) / 8);
-    const int nw_cin = raw ? 0 : C_LEN, nw_aux = (raw || !g.cyclic) ? 0 : A_LEN;
-    Staged<1> st_dc, st_cin, st_aux;
+    const int nw_cin = raw ? 0 : C_LEN;
+    Staged<1> st_dc, st_cin;
     Staged<2, double2> st_tw;   // the whole table for L <= 2*blockDim
     stage_load(st_dc, a.dc, nw_dc);
     stage_load(st_tw, a.tw, twiddle_table_len(L));
     stage_load(st_cin, nw_cin ? (const void*)a.cin : (const void*)a.dc, nw_cin);   // a.cin is null in raw mode
-    stage_load(st_aux, a.aux, nw_aux);
+    // channel: thread t < 2 NL CQ_N sums quantity t of k_vort's per-tile partial
+    // boundary integrals over the tiles of a row, in tile order
+    __shared__ double s_csum[2 * MAXL * CQ_N];
+    const int ncsum = (!raw && g.cyclic) ? 2 * NL * CQ_N : 0;
+    const bool do_csum = (int)threadIdx.x < ncsum;
+    constexpr int CSTAGE = 4;   // tiles loaded with the first batch; wider channels loop for the rest
+    double cpv[CSTAGE];
+    const double* cp = a.cpart;
+    if (do_csum) {
+        const int ks = (int)threadIdx.x / CQ_N, qy = (int)threadIdx.x - ks * CQ_N;
+        cp = a.cpart + (long long)ks * a.ctiles * CYC_SLOTS + qy;
+    }
+#pragma unroll
+    for (int t = 0; t < CSTAGE; t++) cpv[t] = (do_csum && t < a.ctiles) ? cp[t * CYC_SLOTS] : 0.0;
+    const double txi_s = (!raw && g.cyclic) ? a.fcon[F_TXIS] : 0.0, txi_n = (!raw && g.cyclic) ? a.fcon[F_TXIN] : 0.0;
     const bool want_spk = !raw && g.cyclic && threadIdx.x < 2 * NL;
     const int m_spk = want_spk ? (int)(threadIdx.x >> 1) : 0;
     const int r_spk = (want_spk && (threadIdx.x & 1)) ? s.nyi - 1 : 0;
@@ -922,7 +936,14 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     stage_store(st_dc, &s_dc, a.dc, nw_dc);
     stage_store(st_tw, tw, a.tw, twiddle_table_len(L));
     stage_store(st_cin, s_cin, a.cin, nw_cin);
-    stage_store(st_aux, s_aux, a.aux, nw_aux);
+    if (do_csum) {
+        double t = cpv[0];
+#pragma unroll
+        for (int u = 1; u < CSTAGE; u++)
+            if (u < a.ctiles) t += cpv[u];
+        for (int u = CSTAGE; u < a.ctiles; u++) t += cp[u * CYC_SLOTS];
+        s_csum[threadIdx.x] = t;
+    }
     if (want_spk) s_spk[threadIdx.x] = spk;
 
     // this thread's share of the integrals behind the correction coefficients,
@@ -1036,6 +1057,38 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     };
 
     // ---- correction coefficients (identical in every block)
+    if (!raw && g.cyclic && (int)threadIdx.x < NL) {
+        // the boundary integrals of layer k from the summed partials (k_vort)
+        const int k = threadIdx.x;
+        const double dx = g.dx, dy = g.dy;
+        auto S = [&](int side, int q) { return s_csum[(k * 2 + side) * CQ_N + q]; };
+        s_aux[A_AP3S + k] = a.ph->ah2[k] * (S(0, CQ_D2) * dx) / dy;
+        s_aux[A_AP3N + k] = a.ph->ah2[k] * (S(1, CQ_D2) * dx) / dy;
+        s_aux[A_AP5S + k] = a.ph->ah4[k] * (S(0, CQ_D4) * dx) / dy;
+        s_aux[A_AP5N + k] = a.ph->ah4[k] * (S(1, CQ_D4) * dx) / dy;
+        const double aj5s = (S(0, CQ_AJ5) * dx) / (2.0 * dx), aj9s = (S(0, CQ_AJ9) * dx) / (2.0 * dx);
+        const double aj5n = (S(1, CQ_AJ5) * dx) / (2.0 * dx), aj9n = (S(1, CQ_AJ9) * dx) / (2.0 * dx);
+        s_aux[A_AJIS + k] = (aj5s + 2.0 * aj9s) / 6.0;
+        s_aux[A_AJIN + k] = -(aj5n + 2.0 * aj9n) / 6.0;
+        if (k >= 2) {   // layers 1 and 2 are written by thread 0 below
+            s_aux[A_BES + k] = 0.0;
+            s_aux[A_BEN + k] = 0.0;
+        }
+        if (k == g.kt) {
+            s_aux[A_BDRS] = a.ph->cd_bdr * ((S(0, CQ_BD) * dx) / dy);
+            s_aux[A_BDRN] = a.ph->cd_bdr * ((S(1, CQ_BD) * dx) / dy);
+        }
+        if (k == 0) {
+            // int_Be_s/n (vorticity.F90:86-93): fohfac(2) in both northern terms
+            const double dz = (double)a.ph->dz_sign;
+            s_aux[A_BES + 0] = dz * a.ph->fohfac[0] * (S(0, CQ_E1) * dx);
+            s_aux[A_BES + 1] = dz * a.ph->fohfac[0] * (S(0, CQ_E2) * dx);
+            s_aux[A_BEN + 0] = -dz * a.ph->fohfac[1] * (S(1, CQ_E1) * dx);
+            s_aux[A_BEN + 1] = -dz * a.ph->fohfac[1] * (S(1, CQ_E2) * dx);
+            s_aux[A_TXIS] = txi_s;
+            s_aux[A_TXIN] = txi_n;
+        }
+    }
     if (!raw) {
         block_sum_finish<NL + 1>(v, red);
         if (threadIdx.x == 0)
