@@ -139,6 +139,14 @@ int oqg_suppress(oqg_ctx* ctx);
  * Replays cached HIP graphs; asynchronous. */
 int oqg_run(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp_every);
 
+/* Planning for oqg_run (no counterpart in the reference): captures, without
+ * running anything, the HIP graphs that oqg_run with the same arguments, from
+ * the current time level, replays.  oqg_run splits a run into whole
+ * suppression periods (replayed from one cached graph each) and a shorter
+ * head and tail; those are replayed from graphs only when planned, and
+ * launched kernel by kernel otherwise.  Plan once, run many times. */
+int oqg_plan_run(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp_every);
+
 /* Wait for all work queued on the context's stream. */
 int oqg_sync(oqg_ctx* ctx);
 

@@ -70,6 +70,7 @@ SIGNATURES = [
     ("oqg_step", _i, [_vp, _d, _vp, _i, _vp, _d, _d]),
     ("oqg_suppress", _i, [_vp]),
     ("oqg_run", _i, [_vp, _d, _i, _i, _i]),
+    ("oqg_plan_run", _i, [_vp, _d, _i, _i, _i]),
     ("oqg_sync", _i, [_vp]),
     ("oqg_solve_inhomog", _i, [_vp, _i, _vp, _vp, _i]),
     ("oqg_get_modes", _i, [_vp, _dp, _dp, _dp, _dp]),

@@ -26,13 +26,16 @@ constexpr double TWOPI_REF = 2.0 * 3.14159265358979324;   // constants.F90: TWOP
 
 // Graph kinds: plain steps (oqg_run), steps with snapshot saves
 // (oqg_adj_record), adjoint steps (oqg_adj_sweep).
-enum { G_RUN = 0, G_RECORD = 1, G_SWEEP = 2 };
+// G_SEG: a stretch of oqg_run shorter than a whole period, captured ahead by
+// oqg_plan_run (phase = the stretch's first nt-1 modulo the period).
+enum { G_RUN = 0, G_RECORD = 1, G_SWEEP = 2, G_SEG = 3 };
 
 struct GraphKey {
     double tdt;
     int nsteps, supp, cur, kind;
+    int phase = 0;
     bool operator<(const GraphKey& o) const {
-        return std::tie(tdt, nsteps, supp, cur, kind) < std::tie(o.tdt, o.nsteps, o.supp, o.cur, o.kind);
+        return std::tie(tdt, nsteps, supp, cur, kind, phase) < std::tie(o.tdt, o.nsteps, o.supp, o.cur, o.kind, o.phase);
     }
 };
 
@@ -574,6 +577,8 @@ static int setup(oqg_ctx* c) {
     ph.cdrag = (ph.fohfac[g.kt] * d.delek / (2.0 * std::fabs(d.fnot)));
     ph.cd_bdr = d.fnot * d.delek / (2.0 * std::fabs(d.fnot));
     ph.qbfac = d.bcco * (1.0 / (d.dx * d.dx)) / (0.5 * d.bcco + 1.0) / d.fnot;
+    ph.rdy = 1.0 / d.dy;
+    ph.rdx2 = 1.0 / (2.0 * d.dx);
     ph.dz_sign = d.dz_sign;
     ph.tau_sign = d.tau_sign;
 
@@ -1304,36 +1309,88 @@ static int cached_graph(oqg_ctx* c, const GraphKey& key, F&& enqueue, hipGraphEx
 
 extern "C" {
 
-int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
-    if (!c || nsteps < 0) return OQG_ERR_ARG;
-    const int P = graph_period(supp);
-    int nt = step0 + 1;
-    const int end = step0 + nsteps;
-    while (nt <= end) {
-        if ((nt - 1) % P == 0 && end - nt + 1 >= P) {
-            hipGraphExec_t ex;
-            const int rc = cached_graph(c, GraphKey{tdt, P, supp, c->cur, G_RUN}, [&]() {
-                for (int t = 0; t < P; t++) {
-                    if (int r = enqueue_step(c, tdt)) return r;
-                    if (supp > 0 && (t % supp) == 0)
-                        if (int r = enqueue_suppress(c)) return r;
-                }
-                return 0;
-            }, &ex);
-            if (rc) return rc;
-            HIPCHK(c, hipGraphLaunch(ex, c->st));
-            // P is even, so the time-level parity is unchanged
-            nt += P;
-        } else {
-            // single steps are launched eagerly: replaying a one-step graph
-            // measured slower than its four launches
-            if (int r = enqueue_step(c, tdt)) return r;
-            if (supp > 0 && (nt - 1) % supp == 0)
-                if (int r = enqueue_suppress(c)) return r;
-            nt++;
-        }
+}  // extern "C"
+
+// Steps nt0 .. nt0+n-1 of a run, from the current time level, eagerly.
+static int run_eager(oqg_ctx* c, double tdt, int nt0, int n, int supp) {
+    for (int nt = nt0; nt < nt0 + n; nt++) {
+        if (int r = enqueue_step(c, tdt)) return r;
+        if (supp > 0 && (nt - 1) % supp == 0)
+            if (int r = enqueue_suppress(c)) return r;
     }
     return 0;
+}
+
+// The stretches oqg_run splits steps step0+1 .. step0+nsteps into: a head up
+// to the first step nt with (nt-1) % P == 0, whole periods, and a tail.
+struct RunSplit {
+    int head, periods, tail;
+};
+static RunSplit run_split(int nsteps, int step0, int P) {
+    RunSplit r{};
+    const int first = step0 + 1;
+    int head = (P - (first - 1) % P) % P;
+    if (head > nsteps) head = nsteps;
+    r.head = head;
+    r.periods = (nsteps - head) / P;
+    r.tail = nsteps - head - r.periods * P;
+    return r;
+}
+
+// A head or tail stretch: its planned graph when oqg_plan_run captured one,
+// otherwise eager launches (a one-step graph measured slower than its four
+// launches, and arbitrary stretches, as in checkpointed sweeps, are not worth
+// a capture each).  Leaves c->cur advanced by n.
+static int run_stretch(oqg_ctx* c, double tdt, int nt0, int n, int supp, int P, bool capture) {
+    if (n <= 0) return 0;
+    const GraphKey key{tdt, n, supp, c->cur, G_SEG, (nt0 - 1) % P};
+    if (capture || c->graphs.count(key)) {
+        hipGraphExec_t ex;
+        if (int r = cached_graph(c, key, [&]() { return run_eager(c, tdt, nt0, n, supp); }, &ex)) return r;
+        if (!capture) HIPCHK(c, hipGraphLaunch(ex, c->st));
+        if (n & 1) c->cur = 1 - c->cur;
+        return 0;
+    }
+    return run_eager(c, tdt, nt0, n, supp);
+}
+
+static int run_steps(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, bool plan) {
+    if (!c || nsteps < 0) return OQG_ERR_ARG;
+    const int P = graph_period(supp);
+    const int cur0 = c->cur;
+    const RunSplit sp = run_split(nsteps, step0, P);
+    int nt = step0 + 1;
+    if (int r = run_stretch(c, tdt, nt, sp.head, supp, P, plan)) return r;
+    nt += sp.head;
+    for (int b = 0; b < sp.periods; b++, nt += P) {
+        hipGraphExec_t ex;
+        const int rc = cached_graph(c, GraphKey{tdt, P, supp, c->cur, G_RUN}, [&]() {
+            for (int t = 0; t < P; t++) {
+                if (int r = enqueue_step(c, tdt)) return r;
+                if (supp > 0 && (t % supp) == 0)
+                    if (int r = enqueue_suppress(c)) return r;
+            }
+            return 0;
+        }, &ex);
+        if (rc) return rc;
+        if (plan) break;   // one period's graph serves every period
+        HIPCHK(c, hipGraphLaunch(ex, c->st));
+        // P is even, so the time-level parity is unchanged
+    }
+    if (plan) nt = step0 + 1 + sp.head + sp.periods * P;
+    if (int r = run_stretch(c, tdt, nt, sp.tail, supp, P, plan)) return r;
+    if (plan) c->cur = cur0;   // planning runs nothing
+    return 0;
+}
+
+extern "C" {
+
+int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
+    return run_steps(c, tdt, nsteps, step0, supp, false);
+}
+
+int oqg_plan_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
+    return run_steps(c, tdt, nsteps, step0, supp, true);
 }
 
 int oqg_sync(oqg_ctx* c) {

@@ -68,6 +68,7 @@ struct Phys {
     double h[MAXL], gp[MAXL];
     double cd_bdr;               // f0*delek/(2|f0|)  (vorticity.F90:134)
     double qbfac;                // bcco*dxm2/(0.5*bcco+1)/f0 (vorsubs.F90:76)
+    double rdy, rdx2;            // 1/dy, 1/(2 dx): the per-step correction code multiplies
     int dz_sign, tau_sign;
 };
 

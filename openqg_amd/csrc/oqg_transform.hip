@@ -655,7 +655,7 @@ __device__ void small_lu_solve(const BoxHom& h, const double (&rhs)[N], double (
 // spectra of the first and last interior rows spk[2m], spk[2m+1].  Writes
 // coef[] and, in block 0, the new constraint state.
 template <int NL>
-__device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom& bh, const CycHom& ch,
+__device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom& bh, const CycHom& ch, const Phys& ph,
                                   const double* ci, const double* x, const double* spk, const double (&v)[NL + 1],
                                   double* coef, bool write_state) {
     const Geo& g = a.g;
@@ -671,7 +671,7 @@ __device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom&
     for (int l = 0; l < NL - 1; l++) {
         const double e = (l == 0) ? exn : 0.0;
         const double old = dpi[l];
-        dpi[l] = dpip[l] - a.tdt * a.ph->gp[l] * e;
+        dpi[l] = dpip[l] - a.tdt * ph.gp[l] * e;
         dpip[l] = old;
     }
     if (!g.cyclic) {
@@ -700,10 +700,10 @@ __device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom&
             rhss[k] = -0.5 * dy * f0 * x[A_BES + k];
             rhsn[k] = -0.5 * dy * f0 * x[A_BEN + k];
         }
-        rhss[0] = rhss[0] + a.ph->tau_sign * (f0 / a.ph->h[0]) * x[A_TXIS];
-        rhsn[0] = rhsn[0] - a.ph->tau_sign * (f0 / a.ph->h[0]) * x[A_TXIN];
-        rhss[NL - 1] = rhss[NL - 1] + (f0 / a.ph->h[NL - 1]) * x[A_BDRS];
-        rhsn[NL - 1] = rhsn[NL - 1] - (f0 / a.ph->h[NL - 1]) * x[A_BDRN];
+        rhss[0] = rhss[0] + ph.tau_sign * ph.fohfac[0] * x[A_TXIS];
+        rhsn[0] = rhsn[0] - ph.tau_sign * ph.fohfac[0] * x[A_TXIN];
+        rhss[NL - 1] = rhss[NL - 1] + ph.fohfac[NL - 1] * x[A_BDRS];
+        rhsn[NL - 1] = rhsn[NL - 1] - ph.fohfac[NL - 1] * x[A_BDRN];
 #pragma unroll
         for (int k = 0; k < NL; k++) {
             rhss[k] = rhss[k] + x[A_AJIS + k] - x[A_AP3S + k] + x[A_AP5S + k];
@@ -728,8 +728,8 @@ __device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom&
                 ts += mx.ctl2m[k * MAXL + m] * cs[k];
                 tn += mx.ctl2m[k * MAXL + m] * cn[k];
             }
-            clhss[m] = ts + (ch.c0 * spk[2 * m]) / dy;
-            clhsn[m] = tn + (ch.c0 * spk[2 * m + 1]) / dy;
+            clhss[m] = ts + (ch.c0 * spk[2 * m]) * ph.rdy;
+            clhsn[m] = tn + (ch.c0 * spk[2 * m + 1]) * ph.rdy;
         }
         coef[0] = clhss[0] * ch.hbsi;
 #pragma unroll
@@ -749,18 +749,35 @@ __device__ void step_coefficients(const InvArgs& a, const Mix& mx, const BoxHom&
     }
 }
 
-// p(i,gj,:) = sum_m (inhomog + homcor)(i,gj,m) ctm2l(m,:)  (modes.F90:268-284).
-// sel: 0/1 interior row from LDS, 2: boundary row (inhomog = 0).
-// hpre: the box homogeneous solutions at (i,gj) when already loaded (use_pre).
+// p(i,gj,:) = sum_m (inhomog + homcor)(i,gj,m) ctm2l(m,:)  (modes.F90:268-284),
+// evaluated as sum_m inhomog_m ctm2l(m,:) + sum_m homcor_m ctm2l(m,:): the
+// first part needs no correction coefficient, so it can be formed while they
+// are computed.  sel: 0/1 interior row from LDS, 2: boundary row (inhomog = 0).
 template <int NL>
-__device__ __forceinline__ void pmix(const InvArgs& a, const double* inh, const double* coef,
-                                     const double* sctm2l, int i, int sel, int gj, double (&pk)[NL],
-                                     const double (&hpre)[NL > 1 ? NL - 1 : 1], bool use_pre) {
+__device__ __forceinline__ void pmix_inh(const InvArgs& a, const double* inh, const double* sctm2l, int i, int sel,
+                                         double (&pk)[NL]) {
+    const Geo& g = a.g;
+    double vm[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) vm[m] = (sel == 2) ? 0.0 : inh[(m * 2 + sel) * g.pitch + i];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < NL; m++) t += vm[m] * sctm2l[m * MAXL + k];
+        pk[k] = t;
+    }
+}
+
+// pk += sum_m homcor_m(i,gj) ctm2l(m,:).  hpre: the box homogeneous solutions
+// at (i,gj) when already loaded (use_pre).
+template <int NL>
+__device__ __forceinline__ void pmix_hom(const InvArgs& a, const double* coef, const double* sctm2l, int i, int gj,
+                                         double (&pk)[NL], const double (&hpre)[NL > 1 ? NL - 1 : 1], bool use_pre) {
     const Geo& g = a.g;
     double vm[NL];
 #pragma unroll
     for (int m = 0; m < NL; m++) {
-        const double iv = (sel == 2) ? 0.0 : inh[(m * 2 + sel) * g.pitch + i];
         double h;
         if (!g.cyclic) {
             h = (m == 0) ? 0.0
@@ -773,28 +790,36 @@ __device__ __forceinline__ void pmix(const InvArgs& a, const double* inh, const 
             const double* b2 = b1 + (long long)(NL - 1) * g.nyp;
             h = coef[1 + m] * b1[(m - 1) * g.nyp + gj] + coef[1 + MAXL + m] * b2[(m - 1) * g.nyp + gj];
         }
-        vm[m] = iv + h;
+        vm[m] = h;
     }
 #pragma unroll
     for (int k = 0; k < NL; k++) {
         double t = 0.0;
 #pragma unroll
         for (int m = 0; m < NL; m++) t += vm[m] * sctm2l[m * MAXL + k];
-        pk[k] = t;
+        pk[k] = pk[k] + t;
     }
+}
+
+template <int NL>
+__device__ __forceinline__ void pmix(const InvArgs& a, const double* inh, const double* coef,
+                                     const double* sctm2l, int i, int sel, int gj, double (&pk)[NL],
+                                     const double (&hpre)[NL > 1 ? NL - 1 : 1], bool use_pre) {
+    pmix_inh<NL>(a, inh, sctm2l, i, sel, pk);
+    pmix_hom<NL>(a, coef, sctm2l, i, gj, pk, hpre, use_pre);
 }
 
 // ocqbdy (vorsubs.F90:58-114) at one boundary point: kind 0 zonal row, 1 box
 // meridional column, 2 box corner (x difference, the row and column terms twice).
 template <int NL>
-__device__ __forceinline__ void qbdy(const InvArgs& a, const double* samat, int i, int gj, int kind,
+__device__ __forceinline__ void qbdy(const InvArgs& a, const double* samat, double qbfac, int i, int gj, int kind,
                                      const double (&pb)[NL], const double (&pin)[NL]) {
     const Geo& g = a.g;
     const long long o = (long long)gj * g.pitch + i;
     const int reps = (kind == 2) ? 2 : 1;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
-        double qv = a.ph->qbfac * (pin[k] - pb[k]);
+        double qv = qbfac * (pin[k] - pb[k]);
         double ap = 0.0;
 #pragma unroll
         for (int l = 0; l < NL; l++) ap += samat[k * MAXL + l] * pb[l];
@@ -880,6 +905,7 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     // channel: thread t < 2 NL CQ_N sums quantity t of k_vort's per-tile partial
     // boundary integrals over the tiles of a row, in tile order
     __shared__ double s_csum[2 * MAXL * CQ_N];
+    __shared__ double s_hin[4 * (NL > 1 ? NL - 1 : 1)];   // [row][side][m], see the correction step
     const int ncsum = (!raw && g.cyclic) ? 2 * NL * CQ_N : 0;
     const bool do_csum = (int)threadIdx.x < ncsum;
     constexpr int CSTAGE = 4;   // tiles loaded with the first batch; wider channels loop for the rest
@@ -1060,41 +1086,61 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     if (!raw && g.cyclic && (int)threadIdx.x < NL) {
         // the boundary integrals of layer k from the summed partials (k_vort)
         const int k = threadIdx.x;
-        const double dx = g.dx, dy = g.dy;
+        const double dx = g.dx;
         auto S = [&](int side, int q) { return s_csum[(k * 2 + side) * CQ_N + q]; };
-        s_aux[A_AP3S + k] = a.ph->ah2[k] * (S(0, CQ_D2) * dx) / dy;
-        s_aux[A_AP3N + k] = a.ph->ah2[k] * (S(1, CQ_D2) * dx) / dy;
-        s_aux[A_AP5S + k] = a.ph->ah4[k] * (S(0, CQ_D4) * dx) / dy;
-        s_aux[A_AP5N + k] = a.ph->ah4[k] * (S(1, CQ_D4) * dx) / dy;
-        const double aj5s = (S(0, CQ_AJ5) * dx) / (2.0 * dx), aj9s = (S(0, CQ_AJ9) * dx) / (2.0 * dx);
-        const double aj5n = (S(1, CQ_AJ5) * dx) / (2.0 * dx), aj9n = (S(1, CQ_AJ9) * dx) / (2.0 * dx);
-        s_aux[A_AJIS + k] = (aj5s + 2.0 * aj9s) / 6.0;
-        s_aux[A_AJIN + k] = -(aj5n + 2.0 * aj9n) / 6.0;
+        s_aux[A_AP3S + k] = s_dc.ph.ah2[k] * (S(0, CQ_D2) * dx) * s_dc.ph.rdy;
+        s_aux[A_AP3N + k] = s_dc.ph.ah2[k] * (S(1, CQ_D2) * dx) * s_dc.ph.rdy;
+        s_aux[A_AP5S + k] = s_dc.ph.ah4[k] * (S(0, CQ_D4) * dx) * s_dc.ph.rdy;
+        s_aux[A_AP5N + k] = s_dc.ph.ah4[k] * (S(1, CQ_D4) * dx) * s_dc.ph.rdy;
+        const double aj5s = (S(0, CQ_AJ5) * dx) * s_dc.ph.rdx2, aj9s = (S(0, CQ_AJ9) * dx) * s_dc.ph.rdx2;
+        const double aj5n = (S(1, CQ_AJ5) * dx) * s_dc.ph.rdx2, aj9n = (S(1, CQ_AJ9) * dx) * s_dc.ph.rdx2;
+        s_aux[A_AJIS + k] = (aj5s + 2.0 * aj9s) * (1.0 / 6.0);
+        s_aux[A_AJIN + k] = -((aj5n + 2.0 * aj9n) * (1.0 / 6.0));
         if (k >= 2) {   // layers 1 and 2 are written by thread 0 below
             s_aux[A_BES + k] = 0.0;
             s_aux[A_BEN + k] = 0.0;
         }
         if (k == g.kt) {
-            s_aux[A_BDRS] = a.ph->cd_bdr * ((S(0, CQ_BD) * dx) / dy);
-            s_aux[A_BDRN] = a.ph->cd_bdr * ((S(1, CQ_BD) * dx) / dy);
+            s_aux[A_BDRS] = s_dc.ph.cd_bdr * ((S(0, CQ_BD) * dx) * s_dc.ph.rdy);
+            s_aux[A_BDRN] = s_dc.ph.cd_bdr * ((S(1, CQ_BD) * dx) * s_dc.ph.rdy);
         }
         if (k == 0) {
             // int_Be_s/n (vorticity.F90:86-93): fohfac(2) in both northern terms
-            const double dz = (double)a.ph->dz_sign;
-            s_aux[A_BES + 0] = dz * a.ph->fohfac[0] * (S(0, CQ_E1) * dx);
-            s_aux[A_BES + 1] = dz * a.ph->fohfac[0] * (S(0, CQ_E2) * dx);
-            s_aux[A_BEN + 0] = -dz * a.ph->fohfac[1] * (S(1, CQ_E1) * dx);
-            s_aux[A_BEN + 1] = -dz * a.ph->fohfac[1] * (S(1, CQ_E2) * dx);
+            const double dz = (double)s_dc.ph.dz_sign;
+            s_aux[A_BES + 0] = dz * s_dc.ph.fohfac[0] * (S(0, CQ_E1) * dx);
+            s_aux[A_BES + 1] = dz * s_dc.ph.fohfac[0] * (S(0, CQ_E2) * dx);
+            s_aux[A_BEN + 0] = -dz * s_dc.ph.fohfac[1] * (S(1, CQ_E1) * dx);
+            s_aux[A_BEN + 1] = -dz * s_dc.ph.fohfac[1] * (S(1, CQ_E2) * dx);
             s_aux[A_TXIS] = txi_s;
             s_aux[A_TXIN] = txi_n;
         }
     }
+    // thread 0 computes them while the other threads form the coefficient-free
+    // part of the mode->layer mix at their first two points
+    double pinh[2][NL];
     if (!raw) {
         block_sum_finish<NL + 1>(v, red);
-        if (threadIdx.x == 0)
-            step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_cin, s_aux, s_spk, v, coef, jp == 0);
-        else if (zone)
-            zone_fetch();
+        if (threadIdx.x == 0) {
+            step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_dc.ph, s_cin, s_aux, s_spk, v, coef, jp == 0);
+        } else {
+            // box: the homogeneous solutions at the inner points (columns 1 and
+            // nxp-2) of the block's rows, which ocqbdy pairs with the side walls
+            const int t = (int)threadIdx.x - WAVE;
+            if (!g.cyclic && t >= 0 && t < 4 * (NL - 1)) {
+                const int sel = t / (2 * (NL - 1)), side = (t / (NL - 1)) & 1, m = t % (NL - 1);
+                const int gj = sel == 0 ? gj0 : gj1, i = side == 0 ? 1 : g.nxp - 2;
+                if (sel == 0 || has1) s_hin[t] = a.hom[(long long)m * g.fs + (long long)gj * g.pitch + i];
+            }
+            if (zone) zone_fetch();
+        }
+        if (!ADJ) {
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int idx = threadIdx.x + blockDim.x * u;
+                const int sel = idx / g.nxp, i = idx - sel * g.nxp;
+                if (idx < nsel * g.nxp) pmix_inh<NL>(a, inh, s_ctm2l, i, sel, pinh[u]);
+            }
+        }
         __syncthreads();
     }
 
@@ -1183,13 +1229,31 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
         if (!g.cyclic && (i == 0 || i == g.nxp - 1)) {
             double pin[NL];
             pmix<NL>(a, inh, coef, s_ctm2l, i == 0 ? 1 : g.nxp - 2, sel, gj, pin, hnone, false);
-            qbdy<NL>(a, s_amat, i, gj, 1, pk, pin);
+            qbdy<NL>(a, s_amat, s_dc.ph.qbfac, i, gj, 1, pk, pin);
         }
     };
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         const int idx = threadIdx.x + blockDim.x * u;
-        if (idx < nsel * g.nxp) interior_point(idx, hv[u], true);
+        if (idx < nsel * g.nxp) {
+            const int sel = idx / g.nxp, i = idx - sel * g.nxp;
+            const int gj = sel == 0 ? gj0 : gj1;
+            double pk[NL];
+#pragma unroll
+            for (int k = 0; k < NL; k++) pk[k] = pinh[u][k];
+            pmix_hom<NL>(a, coef, s_ctm2l, i, gj, pk, hv[u], true);
+            const long long o = (long long)gj * g.pitch + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++) a.pnew[k * g.fs + o] = pk[k];
+            if (!g.cyclic && (i == 0 || i == g.nxp - 1)) {
+                double pin[NL], hin[NL > 1 ? NL - 1 : 1];
+                const int side = i == 0 ? 0 : 1;
+#pragma unroll
+                for (int m = 0; m < NL - 1; m++) hin[m] = s_hin[(sel * 2 + side) * (NL - 1) + m];
+                pmix<NL>(a, inh, coef, s_ctm2l, i == 0 ? 1 : g.nxp - 2, sel, gj, pin, hin, true);
+                qbdy<NL>(a, s_amat, s_dc.ph.qbfac, i, gj, 1, pk, pin);
+            }
+        }
     }
     for (int idx = threadIdx.x + 2 * blockDim.x; idx < nsel * g.nxp; idx += blockDim.x)
         interior_point(idx, hnone, false);
@@ -1214,10 +1278,10 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
             if (!g.cyclic && (i == 0 || i == g.nxp - 1)) {
                 // corners take the x difference (vorsubs.F90:83-84 overwrite :79-80)
                 pmix<NL>(a, inh, coef, s_ctm2l, i == 0 ? 1 : g.nxp - 2, 2, gj, pin, hi, zone);
-                qbdy<NL>(a, s_amat, i, gj, 2, pb, pin);
+                qbdy<NL>(a, s_amat, s_dc.ph.qbfac, i, gj, 2, pb, pin);
             } else {
                 pmix<NL>(a, inh, coef, s_ctm2l, i, selin, gin, pin, hi, zone);
-                qbdy<NL>(a, s_amat, i, gj, 0, pb, pin);
+                qbdy<NL>(a, s_amat, s_dc.ph.qbfac, i, gj, 0, pb, pin);
             }
         }
     }

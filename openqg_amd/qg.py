@@ -316,6 +316,12 @@ class QG:
         """Ocean-only time loop on the resident forcing (openqg.F90:483-509)."""
         check(self._lib.oqg_run(self._h, float(tdt), int(nsteps), int(step0), int(supp_every)), self._h, "oqg_run")
 
+    def plan_run(self, tdt: float, nsteps: int, step0: int = 0, supp_every: int = 25):
+        """Capture, without running, the graphs that `run` with the same arguments
+        replays from the current time level (its short head and tail included)."""
+        check(self._lib.oqg_plan_run(self._h, float(tdt), int(nsteps), int(step0), int(supp_every)), self._h,
+              "oqg_plan_run")
+
     def sync(self):
         check(self._lib.oqg_sync(self._h), self._h, "oqg_sync")
 

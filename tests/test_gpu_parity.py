@@ -322,3 +322,41 @@ def test_cyclic_rejects_inconsistent_columns(gpu):
     bad[0, 3, 0] = 1.0
     with pytest.raises(OqgError):
         qg.set_state(p=bad)
+
+
+def test_planned_run_equals_eager(gpu):
+    """oqg_plan_run captures the head and tail stretches of a run; the planned run
+    replays them and must equal stepping one step at a time, bitwise.  Planning
+    itself runs nothing."""
+    w = W.small_box(65, 3, seed=6)
+    p = w.initial_p()
+    ent, wekp, txis, txin = w.forcing()
+    for n, s0, supp in ((20, 5, 25), (7, 0, 4), (33, 3, 4), (1, 0, 25), (61, 10, 25), (9, 2, 0)):
+        outs = []
+        for mode in ("eager", "planned"):
+            qg = w.make_qg()
+            qg.set_state(p=p, pm=p)
+            qg.init_state()
+            qg.set_forcing(ent, wekp, txis, txin)
+            for nt in range(1, s0 + 1):      # the steps before the run
+                qg.step(w.tdt)
+                if supp and (nt - 1) % supp == 0:
+                    qg.suppress()
+            if mode == "eager":
+                for nt in range(s0 + 1, s0 + n + 1):
+                    qg.step(w.tdt)
+                    if supp and (nt - 1) % supp == 0:
+                        qg.suppress()
+            else:
+                before = qg.get_state()
+                qg.plan_run(w.tdt, n, s0, supp)
+                qg.sync()
+                for a, b in zip(qg.get_state(), before):
+                    assert np.array_equal(a, b), "planning changed the state"
+                qg.run(w.tdt, n, s0, supp)
+            qg.sync()
+            outs.append((qg.get_state(), qg.get_constr()))
+            qg.close()
+        for a, b in zip(outs[0][0], outs[1][0]):
+            assert np.array_equal(a, b), (n, s0, supp)
+        assert np.array_equal(outs[0][1], outs[1][1])
