@@ -62,47 +62,93 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
-def cpu_baseline(w, seconds: float):
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_seconds(w, threads: int, nsteps: int, reps: int = 3) -> float:
+    """Median over `reps` samples of the oracle's time for `nsteps` whole steps of
+    workload w (each sample from the same initial state, after one untimed step)."""
     from oracle import cpuref as cr
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cr.set_threads(threads)
     ent, wekp, txis, txin = w.forcing()
-
-    def sample(threads, secs):
-        cr.set_threads(threads)
+    p = w.initial_p()
+    times = []
+    for _ in range(reps):
         b = cr.Basin(w.params())
-        p = w.initial_p()
         b.set_state(p=p, pm=p)
         b.init_state()
         b.step(w.tdt, ent, wekp, txis, txin)   # first step (page-in, thread start) untimed
         b.suppress()
-        # time whole steps until the sample reaches `secs`
-        n = 0
         t0 = time.perf_counter()
-        el = 0.0
-        while el < secs and n < w.nsteps:
+        for n in range(nsteps):
             nt = n + 2
             b.step(w.tdt, ent, wekp, txis, txin)
             if (nt - 1) % w.supp_every == 0:
                 b.suppress()
-            n += 1
-            el = time.perf_counter() - t0
-        return n, el
+        times.append(time.perf_counter() - t0)
+    return float(sorted(times)[len(times) // 2])
 
-    n, el = sample(cores, seconds)
-    n1, el1 = sample(1, seconds / 2)
+
+def cpu_baseline(w, seconds: float):
+    """The CPU oracle (a restatement of the reference step, not the reference
+    binary) timed on this host: median of 3 samples of a fixed step count sized
+    to about `seconds`/10 each, with the OpenMP threads of this process's CPU
+    share (OMP_NUM_THREADS, which the GPU hosts set to the cores allotted per
+    GPU; otherwise every visible core) and with 1 thread; plus the 129^2 box
+    and the coupled 385^2 + 97x49 pair with the same threads."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except AttributeError:
+        visible = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    cores = int(env) if env.isdigit() and int(env) > 0 else visible
+    cores = max(1, min(cores, visible))
+
+    def days_per_s(wk, threads, nsteps):
+        el = _oracle_seconds(wk, threads, nsteps)
+        return nsteps * wk.dt / 86400.0 / el, el
+
+    # size the samples from one short probe
+    probe = _oracle_seconds(w, cores, 10, reps=1) / 10
+    n = max(20, int(seconds / 10 / probe))
+    v, el = days_per_s(w, cores, n)
+    probe1 = _oracle_seconds(w, 1, 3, reps=1) / 3
+    n1 = max(5, int(seconds / 20 / probe1))
+    v1, el1 = days_per_s(w, 1, n1)
+    from openqg_amd import workloads as W
+
+    w1 = W.c1_box129()
+    v129, el129 = days_per_s(w1, cores, w1.nsteps)
+    # coupled pair, one simulated day: 432 atmosphere steps of 200 s, 72 ocean steps of 1200 s
+    wo, wa = W.ocean385(), W.atmos_channel()
+    _, elo = days_per_s(wo, cores, 72)
+    _, ela = days_per_s(wa, cores, 432)
     return {
-        "value": n * w.dt / 86400.0 / el,
+        "value": v,
         "unit": "simulated days/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{n} steps of {w.name} (dt={w.dt:g} s) after init, OpenMP {cores} threads, {el:.1f} s",
-        "value_1thread": n1 * w.dt / 86400.0 / el1,
-        "sample_1thread": f"{n1} steps, 1 thread, {el1:.1f} s",
+        "sample": f"median of 3 runs of {n} steps of {w.name} (dt={w.dt:g} s) after init, OpenMP {cores} threads, "
+                  f"{el:.2f} s each",
+        "cpu_model": _cpu_model(),
+        "visible_cores": visible,
+        "threads_reason": "OMP_NUM_THREADS (the host's CPU share per GPU)" if env.isdigit() else "all visible cores",
+        "value_1thread": v1,
+        "sample_1thread": f"median of 3 runs of {n1} steps, 1 thread, {el1:.2f} s each",
+        "box129": {"value": v129, "unit": "simulated days/s", "sample": f"median of 3 runs of the whole {w1.nsteps}-step "
+                   f"10-day run (dt={w1.dt:g} s), {cores} threads, {el129:.2f} s each"},
+        "coupled_385_97x49": {"value": 1.0 / (elo + ela), "unit": "simulated days/s",
+                              "sample": f"one day: 72 ocean steps of 385^2 ({elo:.2f} s) + 432 atmosphere steps of "
+                                        f"97x49 ({ela:.2f} s), medians of 3, {cores} threads, QG only"},
     }
 
 

@@ -1386,3 +1386,175 @@ void cr_ekman_wekp(const cr_params* prm, double hm, const double* taux, const do
 #undef P2
 #undef T2
 }
+
+/* ------------------------------------------------------------------ */
+/* Ocean mixed layer: step_oml (omlsubs.F90:69-119) with adv_diff     */
+/* (mixed_layer.F90:16-95) and the Ekman velocities of ekman_from_tau */
+/* (ekman.F90:53-83).  T-point fields are (nxt,nyt), p-point fields   */
+/* (nxp,nyp), both x fastest.                                         */
+/* ------------------------------------------------------------------ */
+
+#define TT(a, i, j) (a)[(size_t)((j) - 1) * nxt + ((i) - 1)]
+#define PP(a, i, j) (a)[(size_t)((j) - 1) * nxp + ((i) - 1)]
+
+/* del2_T (numerics.F90:107-147): Del^2 on T points with zero-gradient or
+ * flux (fixed outside value) conditions in y, cyclic or no-flux in x. */
+void cr_del2_T(const cr_params* P, const double* t, int north_flux, double nval, int south_flux, double sval,
+               double* out) {
+    const int nxp = P->nxp, nyp = P->nyp, nxt = nxp - 1, nyt = nyp - 1;
+    (void)nxp;
+    const double f = 1.0 / (P->dx * P->dx);
+    for (int j = 1; j <= nyt; j++)
+        for (int i = 1; i <= nxt; i++) {
+            double dxl, dxr, dyl, dyr;
+            /* dt_x(i) = t(i) - t(i-1), dt_x(i+1) = t(i+1) - t(i) */
+            if (i >= 2) dxl = TT(t, i, j) - TT(t, i - 1, j);
+            else dxl = P->cyclic ? TT(t, 1, j) - TT(t, nxt, j) : 0.0;
+            if (i <= nxt - 1) dxr = TT(t, i + 1, j) - TT(t, i, j);
+            else dxr = P->cyclic ? TT(t, 1, j) - TT(t, nxt, j) : 0.0;
+            if (j >= 2) dyl = TT(t, i, j) - TT(t, i, j - 1);
+            else dyl = south_flux ? TT(t, i, 1) - sval : 0.0;
+            if (j <= nyt - 1) dyr = TT(t, i, j + 1) - TT(t, i, j);
+            else dyr = north_flux ? nval - TT(t, i, nyt) : 0.0;
+            TT(out, i, j) = f * (dxr - dxl + dyr - dyl);
+        }
+}
+
+/* Ekman velocities (ekman.F90:73-80): uek on x faces (nxp,nyt), vek on y
+ * faces (nxt,nyp), wekt on T points (nxt,nyt), from the kinematic stress
+ * at p points. */
+void cr_ekman_uvw(const cr_params* P, double hm, const double* taux, const double* tauy, double* uek, double* vek,
+                  double* wekt) {
+    const int nxp = P->nxp, nyp = P->nyp, nxt = nxp - 1, nyt = nyp - 1;
+    const double cu = P->tau_sign / (P->fnot * hm), cv = -P->tau_sign / (P->fnot * hm);
+    for (int j = 1; j <= nyt; j++)
+        for (int i = 1; i <= nxp; i++)
+            uek[(size_t)(j - 1) * nxp + i - 1] = cu * (0.5 * (PP(tauy, i, j + 1) + PP(tauy, i, j)));
+    for (int j = 1; j <= nyp; j++)
+        for (int i = 1; i <= nxt; i++)
+            vek[(size_t)(j - 1) * nxt + i - 1] = cv * (0.5 * (PP(taux, i + 1, j) + PP(taux, i, j)));
+    for (int j = 1; j <= nyt; j++)
+        for (int i = 1; i <= nxt; i++) {
+            const double tyr = 0.5 * (PP(tauy, i + 1, j + 1) + PP(tauy, i + 1, j));
+            const double tyl = 0.5 * (PP(tauy, i, j + 1) + PP(tauy, i, j));
+            const double txt = 0.5 * (PP(taux, i + 1, j + 1) + PP(taux, i, j + 1));
+            const double txb = 0.5 * (PP(taux, i + 1, j) + PP(taux, i, j));
+            TT(wekt, i, j) = (1.0 / (P->dx * P->fnot)) * (tyr - tyl) - (1.0 / (P->dy * P->fnot)) * (txt - txb);
+        }
+}
+
+/* adv_diff (mixed_layer.F90:16-57) for the mixed-layer field (data, datam)
+ * advected by the geostrophic velocity of p (layer 1) plus the Ekman
+ * velocity, with Del^2/Del^4 diffusion of datam. */
+void cr_adv_diff(const cr_params* P, const cr_oml_params* o, const double* p, const double* uek, const double* vek,
+                 const double* data, const double* datam, double* out) {
+    const int nxp = P->nxp, nyp = P->nyp, nxt = nxp - 1, nyt = nyp - 1;
+    double* d2 = (double*)calloc((size_t)nxt * nyt, sizeof(double));
+    double* d4 = (double*)calloc((size_t)nxt * nyt, sizeof(double));
+    cr_del2_T(P, datam, o->fluxn, o->nval, o->fluxs, o->sval, d2);
+    cr_del2_T(P, d2, 0, 0.0, 0, 0.0, d4);
+    for (int j = 1; j <= nyt; j++)
+        for (int i = 1; i <= nxt; i++) {
+            double ut[2], vt[2];
+            for (int s = 0; s < 2; s++) {
+                /* x face I = i + s (adv_vel, mixed_layer.F90:68-74; map_T_to_x_face, numerics.F90:250-266) */
+                const int I = i + s;
+                const double dpdy = (1.0 / P->dy) * (PP(p, I, j + 1) - PP(p, I, j));
+                double vu = (-o->ycexp / P->fnot) * dpdy + uek[(size_t)(j - 1) * nxp + I - 1];
+                if (!P->cyclic && (I == 1 || I == nxp)) vu = 0.0;
+                double xf;
+                if (I == 1) xf = P->cyclic ? 0.5 * (TT(data, 1, j) + TT(data, nxt, j)) : TT(data, 1, j);
+                else if (I == nxt + 1) xf = P->cyclic ? 0.5 * (TT(data, 1, j) + TT(data, nxt, j)) : TT(data, nxt, j);
+                else xf = 0.5 * (TT(data, I - 1, j) + TT(data, I, j));
+                ut[s] = vu * xf;
+                /* y face J = j + s (adv_vel :76-92; map_T_to_y_face, numerics.F90:222-248) */
+                const int J = j + s;
+                const double dpdx = (1.0 / P->dx) * (PP(p, i + 1, J) - PP(p, i, J));
+                const double vk = vek[(size_t)(J - 1) * nxt + i - 1];
+                double vv = (o->ycexp / P->fnot) * dpdx + vk;
+                if (J == 1) vv = o->fluxs ? vk : 0.0;
+                if (J == nyp) vv = o->fluxn ? vk : 0.0;
+                double yf;
+                if (J == 1) yf = o->fluxs ? TT(data, i, 1) : 0.5 * (TT(data, i, 1) + o->sval);
+                else if (J == nyt + 1) yf = o->fluxn ? TT(data, i, nyt) : 0.5 * (TT(data, i, nyt) + o->nval);
+                else yf = 0.5 * (TT(data, i, J - 1) + TT(data, i, J));
+                vt[s] = vv * yf;
+            }
+            /* c_grid_advection (numerics.F90:392-408) */
+            double a = (-1.0 / P->dx) * (ut[1] - ut[0]) + (-1.0 / P->dy) * (vt[1] - vt[0]);
+            TT(out, i, j) = a + o->d2 * TT(d2, i, j) - o->d4 * TT(d4, i, j);
+        }
+    free(d2);
+    free(d4);
+}
+
+/* map_T_to_P (numerics.F90:64-107). */
+void cr_map_T_to_P(const cr_params* P, const double* t, double* out) {
+    const int nxp = P->nxp, nyp = P->nyp, nxt = nxp - 1;
+    for (int j = 2; j <= nyp - 1; j++)
+        for (int i = 2; i <= nxp - 1; i++)
+            PP(out, i, j) = 0.25 * (TT(t, i, j) + TT(t, i - 1, j) + TT(t, i, j - 1) + TT(t, i - 1, j - 1));
+    for (int i = 2; i <= nxp - 1; i++) {
+        PP(out, i, 1) = 0.5 * (TT(t, i - 1, 1) + TT(t, i, 1));
+        PP(out, i, nyp) = 0.5 * (TT(t, i - 1, nyp - 1) + TT(t, i, nyp - 1));
+    }
+    for (int j = 2; j <= nyp - 1; j++) {
+        if (P->cyclic) {
+            PP(out, 1, j) = 0.25 * (TT(t, 1, j) + TT(t, nxp - 1, j) + TT(t, 1, j - 1) + TT(t, nxp - 1, j - 1));
+            PP(out, nxp, j) = PP(out, 1, j);
+        } else {
+            PP(out, 1, j) = 0.5 * (TT(t, 1, j - 1) + TT(t, 1, j));
+            PP(out, nxp, j) = 0.5 * (TT(t, nxp - 1, j - 1) + TT(t, nxp - 1, j));
+        }
+    }
+    if (P->cyclic) {
+        PP(out, 1, 1) = 0.5 * (TT(t, 1, 1) + TT(t, nxp - 1, 1));
+        PP(out, 1, nyp) = 0.5 * (TT(t, 1, nyp - 1) + TT(t, nxp - 1, nyp - 1));
+        PP(out, nxp, 1) = PP(out, 1, 1);
+        PP(out, nxp, nyp) = PP(out, 1, nyp);
+    } else {
+        PP(out, 1, 1) = TT(t, 1, 1);
+        PP(out, nxp, 1) = TT(t, nxp - 1, 1);
+        PP(out, 1, nyp) = TT(t, 1, nyp - 1);
+        PP(out, nxp, nyp) = TT(t, nxp - 1, nyp - 1);
+    }
+}
+
+/* step_oml (omlsubs.F90:69-119): new sst, the convective correction and, with
+ * compute_ent, the layer 1/2 entrainment ent1 (nxp,nyp) with zero mean. */
+void cr_oml_step(const cr_params* P, const cr_oml_params* o, const double* p1, const double* taux, const double* tauy,
+                 const double* fnet, double tdto, int compute_ent, double* sst, double* sstm, double* ent1) {
+    const int nxp = P->nxp, nyp = P->nyp, nxt = nxp - 1, nyt = nyp - 1;
+    const size_t nt = (size_t)nxt * nyt;
+    double* uek = (double*)calloc((size_t)nxp * nyt, sizeof(double));
+    double* vek = (double*)calloc((size_t)nxt * nyp, sizeof(double));
+    double* wekt = (double*)calloc(nt, sizeof(double));
+    double* ad = (double*)calloc(nt, sizeof(double));
+    double* snew = (double*)calloc(nt, sizeof(double));
+    double* dtc = (double*)calloc(nt, sizeof(double));
+    double* xfo = (double*)calloc(nt, sizeof(double));
+    cr_ekman_uvw(P, o->hm, taux, tauy, uek, vek, wekt);
+    cr_adv_diff(P, o, p1, uek, vek, sst, sstm, ad);
+    for (size_t t = 0; t < nt; t++) {
+        const double half = 0.5 * wekt[t] * (sstm[t] - o->T1_rel);
+        snew[t] = sstm[t] + tdto * (ad[t] + (1.0 / (o->rho_cp * o->hm)) * fnet[t] +
+                                    (1.0 / o->hm) * (wekt[t] * sstm[t] - half));
+        dtc[t] = fmax(0.0, o->T1_rel - snew[t]);
+        xfo[t] = (-1.0 / o->dT_12) * half - (o->hm / (tdto * o->dT_12)) * dtc[t];
+    }
+    if (compute_ent) {
+        /* avg_T (numerics.F90:155-162): the sum in Fortran's column order */
+        double s = 0.0;
+        for (size_t t = 0; t < nt; t++) s += xfo[t];
+        const double avg = s / (double)(nxt * nyt);
+        for (size_t t = 0; t < nt; t++) xfo[t] = xfo[t] - avg;
+        cr_map_T_to_P(P, xfo, ent1);
+    }
+    for (size_t t = 0; t < nt; t++) {
+        sstm[t] = sst[t];
+        sst[t] = snew[t] + dtc[t];
+    }
+    free(uek); free(vek); free(wekt); free(ad); free(snew); free(dtc); free(xfo);
+}
+#undef TT
+#undef PP

@@ -101,6 +101,29 @@ void cr_cfft(int n, int sign, int count, const double* in, double* out); /* comp
 void cr_ekman_wekp(const cr_params* prm, double hm, const double* taux, const double* tauy,
                    double* wekp, double* txis, double* txin);
 
+/* Ocean mixed layer (omlsubs.F90, mixed_layer.F90, mixed.F90).  T fields are
+ * (nxt,nyt) = (nxp-1,nyp-1), x fastest. */
+typedef struct {
+    double hm;            /* mixed layer depth (m) */
+    double T1_rel, dT_12; /* temp_type (mixed.F90:34-37) */
+    double rho_cp;
+    double d2, d4;        /* st2d, st4d */
+    double ycexp;
+    double nval, sval;    /* boundary temperatures (zonal_temp at nyt and 1) */
+    int fluxn, fluxs;
+} cr_oml_params;
+
+void cr_del2_T(const cr_params* P, const double* t, int north_flux, double nval, int south_flux, double sval,
+               double* out);                                                  /* numerics.F90:107 */
+void cr_ekman_uvw(const cr_params* P, double hm, const double* taux, const double* tauy, double* uek,
+                  double* vek, double* wekt);                                 /* ekman.F90:73-80 */
+void cr_adv_diff(const cr_params* P, const cr_oml_params* o, const double* p, const double* uek,
+                 const double* vek, const double* data, const double* datam, double* out); /* mixed_layer.F90:16 */
+void cr_map_T_to_P(const cr_params* P, const double* t, double* out);        /* numerics.F90:64 */
+void cr_oml_step(const cr_params* P, const cr_oml_params* o, const double* p1, const double* taux,
+                 const double* tauy, const double* fnet, double tdto, int compute_ent, double* sst,
+                 double* sstm, double* ent1);                                 /* omlsubs.F90:69 */
+
 /* OpenMP thread count used by the oracle (0 = runtime default). */
 void cr_set_threads(int n);
 

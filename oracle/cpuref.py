@@ -33,6 +33,13 @@ class CrParams(ctypes.Structure):
     ]
 
 
+class CrOmlParams(ctypes.Structure):
+    _fields_ = [("hm", ctypes.c_double), ("T1_rel", ctypes.c_double), ("dT_12", ctypes.c_double),
+                ("rho_cp", ctypes.c_double), ("d2", ctypes.c_double), ("d4", ctypes.c_double),
+                ("ycexp", ctypes.c_double), ("nval", ctypes.c_double), ("sval", ctypes.c_double),
+                ("fluxn", ctypes.c_int), ("fluxs", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -79,6 +86,13 @@ def lib():
         L.cr_set_threads.argtypes = [i]
         L.cr_ocqbdy.argtypes = [vp]
         L.cr_thomas.argtypes = [i, d, d, dp, dp]
+        PP = ctypes.POINTER(CrParams)
+        OP = ctypes.POINTER(CrOmlParams)
+        L.cr_del2_T.argtypes = [PP, dp, i, d, i, d, dp]
+        L.cr_ekman_uvw.argtypes = [PP, d, dp, dp, dp, dp, dp]
+        L.cr_adv_diff.argtypes = [PP, OP, dp, dp, dp, dp, dp, dp]
+        L.cr_map_T_to_P.argtypes = [PP, dp, dp]
+        L.cr_oml_step.argtypes = [PP, OP, dp, dp, dp, dp, d, i, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -290,3 +304,61 @@ def ekman_wekp(cfg, taux, tauy):
     lib().cr_ekman_wekp(ctypes.byref(P), 0.0, _ptr(taux), _ptr(tauy), _ptr(wekp),
                         ctypes.cast(ctypes.byref(txis), _c_double_p), ctypes.cast(ctypes.byref(txin), _c_double_p))
     return wekp, txis.value, txin.value
+
+
+# ------------------------------------------------------------------ ocean mixed layer
+
+
+def oml_params(o) -> CrOmlParams:
+    """CrOmlParams from a dict with hm, T1_rel, dT_12, rho_cp, d2, d4, ycexp, nval, sval, fluxn, fluxs."""
+    P = CrOmlParams()
+    for k in ("hm", "T1_rel", "dT_12", "rho_cp", "d2", "d4", "ycexp", "nval", "sval"):
+        setattr(P, k, float(o[k]))
+    P.fluxn, P.fluxs = int(bool(o["fluxn"])), int(bool(o["fluxs"]))
+    return P
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def del2_T(cfg, t, north_flux=False, nval=0.0, south_flux=False, sval=0.0):
+    P = make_params(cfg)
+    t = _c(t)
+    out = np.zeros_like(t)
+    lib().cr_del2_T(ctypes.byref(P), _ptr(t), int(north_flux), float(nval), int(south_flux), float(sval), _ptr(out))
+    return out
+
+
+def ekman_uvw(cfg, hm, taux, tauy):
+    """(uek (nyt,nxp), vek (nyp,nxt), wekt (nyt,nxt)) from the kinematic stress at p points."""
+    P = make_params(cfg)
+    nxp, nyp = P.nxp, P.nyp
+    uek, vek, wekt = np.zeros((nyp - 1, nxp)), np.zeros((nyp, nxp - 1)), np.zeros((nyp - 1, nxp - 1))
+    lib().cr_ekman_uvw(ctypes.byref(P), float(hm), _ptr(_c(taux)), _ptr(_c(tauy)), _ptr(uek), _ptr(vek), _ptr(wekt))
+    return uek, vek, wekt
+
+
+def adv_diff(cfg, o, p, uek, vek, data, datam):
+    P, O = make_params(cfg), oml_params(o)
+    out = np.zeros((P.nyp - 1, P.nxp - 1))
+    lib().cr_adv_diff(ctypes.byref(P), ctypes.byref(O), _ptr(_c(p)), _ptr(_c(uek)), _ptr(_c(vek)), _ptr(_c(data)),
+                      _ptr(_c(datam)), _ptr(out))
+    return out
+
+
+def map_T_to_P(cfg, t):
+    P = make_params(cfg)
+    out = np.zeros((P.nyp, P.nxp))
+    lib().cr_map_T_to_P(ctypes.byref(P), _ptr(_c(t)), _ptr(out))
+    return out
+
+
+def oml_step(cfg, o, p1, taux, tauy, fnet, tdto, sst, sstm, compute_ent=True):
+    """step_oml: returns (sst, sstm, ent1) after one step (inputs are not modified)."""
+    P, O = make_params(cfg), oml_params(o)
+    sst, sstm = _c(sst).copy(), _c(sstm).copy()
+    ent1 = np.zeros((P.nyp, P.nxp))
+    lib().cr_oml_step(ctypes.byref(P), ctypes.byref(O), _ptr(_c(p1)), _ptr(_c(taux)), _ptr(_c(tauy)), _ptr(_c(fnet)),
+                      float(tdto), int(compute_ent), _ptr(sst), _ptr(sstm), _ptr(ent1))
+    return sst, sstm, ent1

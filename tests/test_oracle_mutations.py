@@ -2,8 +2,8 @@
 
 Each case rewrites one line of the oracle (a sign, an index, a transposed
 operand, a dropped factor in the routines of src/qg/vorsubs.F90,
-constraint.F90, homog.F90, vorticity.F90 and glam/numerics.F90 that
-tests/test_oracle_pins.py fixes), compiles the copy into a temporary
+constraint.F90, homog.F90, vorticity.F90, glam/numerics.F90 and the ocean
+mixed layer that tests/test_oracle_pins.py and tests/test_oml_oracle.py fix), compiles the copy into a temporary
 directory and runs the pin tests against it (OQG_CPUREF_LIB); they must fail.
 The oracle in the tree is never modified.
 """
@@ -53,6 +53,16 @@ M = [
  ("cyclic_homog row nyp-1", "clhsn[m - 1] = n + cr_int_P_dx(b, &E3(inh, 1, NYP - 1, m)) / b->P.dy;", "clhsn[m - 1] = n + cr_int_P_dx(b, &E3(inh, 1, NYP - 2, m)) / b->P.dy;"),
  ("homsol pch rdm2 term", "pch1ys = pch1ys + 0.5 * dy * b->rdm2[m - 1] * h1[0] * xl;", "pch1ys = pch1ys - 0.5 * dy * b->rdm2[m - 1] * h1[0] * xl;"),
  ("hbsi inverted", "b->hbsi = b->yl / b->xl;", "b->hbsi = b->xl / b->yl;"),
+ ("del2_T south flux sign", "else dyl = south_flux ? TT(t, i, 1) - sval : 0.0;", "else dyl = south_flux ? sval - TT(t, i, 1) : 0.0;"),
+ ("del2_T x difference", "if (i >= 2) dxl = TT(t, i, j) - TT(t, i - 1, j);", "if (i >= 2) dxl = TT(t, i - 1, j) - TT(t, i, j);"),
+ ("vek sign", "cv = -P->tau_sign / (P->fnot * hm);", "cv = P->tau_sign / (P->fnot * hm);"),
+ ("wekt x term", "(1.0 / (P->dx * P->fnot)) * (tyr - tyl)", "(1.0 / (P->dx * P->fnot)) * (tyl - tyr)"),
+ ("wall velocity kept", "if (!P->cyclic && (I == 1 || I == nxp)) vu = 0.0;", "if (0) vu = 0.0;"),
+ ("meridional flux sign", "(-1.0 / P->dy) * (vt[1] - vt[0]);", "(1.0 / P->dy) * (vt[1] - vt[0]);"),
+ ("heat flux without hm", "(1.0 / (o->rho_cp * o->hm)) * fnet[t]", "(1.0 / o->rho_cp) * fnet[t]"),
+ ("convection sign", "dtc[t] = fmax(0.0, o->T1_rel - snew[t]);", "dtc[t] = fmin(0.0, o->T1_rel - snew[t]);"),
+ ("entrainment mean kept", "for (size_t t = 0; t < nt; t++) xfo[t] = xfo[t] - avg;", "for (size_t t = 0; t < nt; t++) xfo[t] = xfo[t];"),
+ ("map_T_to_P corner", "PP(out, 1, 1) = TT(t, 1, 1);", "PP(out, 1, 1) = TT(t, 2, 1);"),
 ]
 
 
@@ -70,7 +80,8 @@ def _caught(case, tmp):
     subprocess.run(["gcc", *CFLAGS, "-o", so, c, "-lm"], check=True, capture_output=True)
     env = dict(os.environ, OQG_CPUREF_LIB=so, OMP_NUM_THREADS="1")
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
-                        os.path.join(ROOT, "tests", "test_oracle_pins.py"), os.path.join(ROOT, "tests", "test_oracle.py")],
+                        os.path.join(ROOT, "tests", "test_oracle_pins.py"), os.path.join(ROOT, "tests", "test_oracle.py"),
+                        os.path.join(ROOT, "tests", "test_oml_oracle.py")],
                        cwd=ROOT, env=env, capture_output=True, text=True)
     return name, r.returncode != 0
 
