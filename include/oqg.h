@@ -214,6 +214,9 @@ int oqg_adj_wind(oqg_ctx* ctx, double* ltaux, double* ltauy, int kind);
 /* The resident forcing: wekp (nxp,nyp) and txi = {txis, txin} (host, 2
  * doubles); either may be NULL. */
 int oqg_get_forcing(oqg_ctx* ctx, double* wekp, double* txi, int kind);
+/* The resident layer 1/2 entrainment (nxp,nyp), as the coupler holds it in
+ * ocn_cpl%ent(:,:,1) between step_oml and step_qg (driver.F90:103-107). */
+int oqg_get_ent(oqg_ctx* ctx, double* ent1, int kind);
 
 /*
  * Diagnostics of the resident state, computed on the device: the energetics
@@ -324,6 +327,49 @@ int oqg_ens_sync(oqg_ens* ens);
 int oqg_ens_set_schedule(oqg_ens* ens, int schedule);
 int oqg_ens_get_schedule(const oqg_ens* ens);
 int oqg_ens_tune(oqg_ens* ens, const double* tdt, int trial_steps, int supp_every, double* ms);
+
+/*
+ * Ocean mixed layer (ocean_mixed_type, src/mixed/omlsubs.F90:13-20) of an
+ * ocean basin, on the device.  Its temperature lives at T points: (nxt,nyt) =
+ * (nxp-1,nyp-1) arrays, x fastest.  It reads the basin's current layer-1
+ * pressure and the resident wind stress of oqg_set_wind (its Ekman
+ * velocities and pumping, src/coupler/ekman.F90:73-80), and writes the
+ * basin's resident layer-1 entrainment, which the next step_qg uses.
+ */
+typedef struct oqg_oml_desc {
+    double hm;            /* mixed layer depth (box_type%hm, m) */
+    double T1_rel;        /* temp_type%T1_rel: T_bar - T(1) (K) */
+    double dT_12;         /* temp_type%dT_12: T(1) - T(2) (K) */
+    double rho_cp;        /* temp_type%rho_cp (J/(m^3 K)) */
+    double st2d, st4d;    /* Del^2 / Del^4 diffusivities (mixed_type%d2, %d4) */
+    double ycexp;         /* geostrophic advection coefficient (mixed_type%ycexp) */
+    double nval, sval;    /* northern / southern boundary temperatures (zonal_temp at nyt, 1) */
+    int fluxn, fluxs;     /* boundary heat-flux switches (basin flux_n, flux_s) */
+} oqg_oml_desc;
+
+typedef struct oqg_oml oqg_oml;
+
+/* init_ocean_ml (omlsubs.F90:27-62) for basin ctx; temperature and net heat
+ * flux start at zero.  The mixed layer must not outlive its basin. */
+int oqg_oml_create(oqg_ctx* ctx, const oqg_oml_desc* desc, oqg_oml** out);
+void oqg_oml_destroy(oqg_oml* oml);
+const char* oqg_oml_last_error(const oqg_oml* oml);
+/* The two time levels sst%data, sst%datam (NULL skips), (nxt,nyt). */
+int oqg_oml_set_state(oqg_oml* oml, const double* sst, const double* sstm, int kind);
+int oqg_oml_get_state(oqg_oml* oml, double* sst, double* sstm, int kind);
+/* The resident net heat flux into the layer, fnetoc (nxt,nyt) (W/m^2). */
+int oqg_oml_set_fnet(oqg_oml* oml, const double* fnet, int kind);
+/* step_oml (omlsubs.F90:69-119): the new temperature with its convective
+ * correction and, when compute_ent, the layer 1/2 entrainment written to the
+ * basin's resident forcing.  Asynchronous on the basin's stream. */
+int oqg_oml_step(oqg_oml* oml, double tdto, int compute_ent);
+/* The mixed-layer part of suppress_ocean (driver.F90:165-170). */
+int oqg_oml_suppress(oqg_oml* oml);
+/* The ocean's run loop with its mixed layer: for nt = step0+1 .. step0+nsteps,
+ * step_ocn (driver.F90:89-110: step_oml, then step_qg on the entrainment it
+ * wrote) and, when mod(nt-1, supp_every) == 0, suppress_ocean (both the basin
+ * and the mixed layer).  Replays cached HIP graphs; asynchronous. */
+int oqg_run_ocean(oqg_oml* oml, double tdto, int nsteps, int step0, int supp_every, int compute_ent);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,13 @@ class OqgDiag(ctypes.Structure):
                [("ctot", ctypes.c_double), ("nonfinite", ctypes.c_double)]
 
 
+class OqgOmlDesc(ctypes.Structure):
+    """Mirror of `oqg_oml_desc` (include/oqg.h)."""
+
+    _fields_ = [(n, ctypes.c_double) for n in ("hm", "T1_rel", "dT_12", "rho_cp", "st2d", "st4d", "ycexp", "nval",
+                                                "sval")] + [("fluxn", ctypes.c_int), ("fluxs", ctypes.c_int)]
+
+
 # (name, restype, argtypes) for every entry point of include/oqg.h
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -88,6 +95,7 @@ SIGNATURES = [
     ("oqg_set_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_adj_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_get_forcing", _i, [_vp, _vp, _dp, _i]),
+    ("oqg_get_ent", _i, [_vp, _vp, _i]),
     ("oqg_diagnose", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
     ("oqg_diagnose_noalloc", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
     ("oqg_invert", _i, [_vp]),
@@ -109,6 +117,15 @@ SIGNATURES = [
     ("oqg_ens_set_schedule", _i, [_vp, _i]),
     ("oqg_ens_get_schedule", _i, [_vp]),
     ("oqg_ens_tune", _i, [_vp, _dp, _i, _i, _dp]),
+    ("oqg_oml_create", _i, [_vp, ctypes.POINTER(OqgOmlDesc), ctypes.POINTER(_vp)]),
+    ("oqg_oml_destroy", None, [_vp]),
+    ("oqg_oml_last_error", ctypes.c_char_p, [_vp]),
+    ("oqg_oml_set_state", _i, [_vp, _vp, _vp, _i]),
+    ("oqg_oml_get_state", _i, [_vp, _vp, _vp, _i]),
+    ("oqg_oml_set_fnet", _i, [_vp, _vp, _i]),
+    ("oqg_oml_step", _i, [_vp, _d, _i]),
+    ("oqg_oml_suppress", _i, [_vp]),
+    ("oqg_run_ocean", _i, [_vp, _d, _i, _i, _i, _i]),
 ]
 
 _lib = None

@@ -1115,6 +1115,13 @@ int oqg_get_forcing(oqg_ctx* c, double* wekp, double* txi, int kind) {
     return 0;
 }
 
+int oqg_get_ent(oqg_ctx* c, double* ent1, int kind) {
+    if (!c || !ent1) return c ? fail(c, OQG_ERR_ARG, "oqg_get_ent: null field") : OQG_ERR_ARG;
+    if (int r = download_field(c, ent1, c->ent1, 1, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
 }  // extern "C"
 
 static void diag_args(oqg_ctx* c, double* partial, DiagArgs& a) {
@@ -2562,6 +2569,248 @@ int oqg_ens_tune(oqg_ens* e, const double* tdt, int trial_steps, int supp_every,
     if (rc) return -rc;
     oqg_ens_set_schedule(e, best);
     return best;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ ocean mixed layer
+
+// The ocean mixed layer of a basin (ocean_mixed_type, mixed/omlsubs.F90:13-20):
+// its temperature at T points (two time levels and the scratch of a step),
+// the resident net heat flux, and the parameters of step_oml.
+struct oqg_oml {
+    oqg_ctx* c = nullptr;
+    oqg_oml_desc d{};
+    double* sst = nullptr;    // current level (T layout: the basin's pitch, nyp-1 rows)
+    double* sstm = nullptr;   // previous level
+    double* snew = nullptr;
+    double* xfo = nullptr;
+    double* fnet = nullptr;
+    double* part = nullptr;
+    int npart = 0;
+    std::vector<void*> allocs;
+    std::map<GraphKey, hipGraphExec_t> graphs;   // oqg_run_ocean
+    std::string err;
+};
+
+static int oml_fail(oqg_oml* m, int code, const std::string& msg) {
+    if (m) m->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define OMLCHK(m, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return oml_fail(m, OQG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static int oml_alloc(oqg_oml* m, double** p, size_t n) {
+    void* v = nullptr;
+    OMLCHK(m, hipMalloc(&v, n * sizeof(double)));
+    OMLCHK(m, hipMemsetAsync(v, 0, n * sizeof(double), m->c->st));
+    m->allocs.push_back(v);
+    *p = static_cast<double*>(v);
+    return 0;
+}
+
+static OmlArgs oml_args(oqg_oml* m, double tdt, int compute_ent) {
+    oqg_ctx* c = m->c;
+    OmlArgs a{};
+    a.g = c->g;
+    a.hm = m->d.hm;
+    a.T1_rel = m->d.T1_rel;
+    a.dT_12 = m->d.dT_12;
+    a.rho_cp = m->d.rho_cp;
+    a.d2 = m->d.st2d;
+    a.d4 = m->d.st4d;
+    a.ycexp = m->d.ycexp;
+    a.nval = m->d.nval;
+    a.sval = m->d.sval;
+    a.fluxn = m->d.fluxn ? 1 : 0;
+    a.fluxs = m->d.fluxs ? 1 : 0;
+    a.tau_sign = c->d.tau_sign;
+    a.p1 = c->pbuf[c->cur];   // layer 1 of the current p (driver.F90:98-101)
+    a.taux = c->taux_d;
+    a.tauy = c->tauy_d;
+    a.fnet = m->fnet;
+    a.sst = m->sst;
+    a.sstm = m->sstm;
+    a.snew = m->snew;
+    a.xfo = m->xfo;
+    a.part = m->part;
+    a.npart = m->npart;
+    a.ent1 = c->ent1;
+    a.tdt = tdt;
+    a.compute_ent = compute_ent;
+    return a;
+}
+
+static int enqueue_oml_step(oqg_oml* m, double tdt, int compute_ent) {
+    const OmlArgs a = oml_args(m, tdt, compute_ent);
+    launch_oml_step(a, m->c->st);
+    launch_oml_finish(a, m->c->st);
+    OMLCHK(m, hipGetLastError());
+    return 0;
+}
+
+static int enqueue_oml_suppress(oqg_oml* m) {
+    launch_oml_suppress(oml_args(m, 0.0, 0), m->c->st);
+    OMLCHK(m, hipGetLastError());
+    return 0;
+}
+
+// step_oml sets the whole entrainment field (omlsubs.F90:100-104): layer 1 from
+// the mixed layer, the deeper interfaces zero.  The basin's step then reads the
+// layer 1 field the mixed layer writes and no deeper entrainment.
+static int oml_own_ent(oqg_oml* m) {
+    oqg_ctx* c = m->c;
+    if (!c->has_ent2) return 0;
+    c->has_ent2 = false;
+    static const double zero = 0.0;
+    OMLCHK(m, hipMemcpyAsync(c->fcon + F_ENT2, &zero, sizeof zero, hipMemcpyHostToDevice, c->st));
+    OMLCHK(m, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+// T fields between (nxt, nyt) arrays and the pitched layout.
+static int oml_copy(oqg_oml* m, double* dst, size_t dpitch, const double* src, size_t spitch, hipMemcpyKind kind) {
+    const Geo& g = m->c->g;
+    OMLCHK(m, hipMemcpy2DAsync(dst, dpitch, src, spitch, (size_t)(g.nxp - 1) * sizeof(double), (size_t)(g.nyp - 1),
+                               kind, m->c->st));
+    OMLCHK(m, hipStreamSynchronize(m->c->st));
+    return 0;
+}
+
+extern "C" {
+
+int oqg_oml_create(oqg_ctx* c, const oqg_oml_desc* d, oqg_oml** out) {
+    if (!out) return oml_fail(nullptr, OQG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!c || !d) return oml_fail(nullptr, OQG_ERR_ARG, "oqg_oml_create: null basin or description");
+    if (!(d->hm > 0) || !(d->rho_cp > 0) || d->dT_12 == 0.0)
+        return oml_fail(nullptr, OQG_ERR_ARG, "oqg_oml_create: need hm > 0, rho_cp > 0, dT_12 != 0");
+    auto m = std::make_unique<oqg_oml>();
+    m->c = c;
+    m->d = *d;
+    const size_t ft = (size_t)c->g.pitch * (c->g.nyp - 1);
+    const dim3 grid = oml_step_grid(c->g);
+    m->npart = (int)(grid.x * grid.y);
+    int rc = 0;
+    for (double** p : {&m->sst, &m->sstm, &m->snew, &m->xfo, &m->fnet})
+        if (!rc) rc = oml_alloc(m.get(), p, ft);
+    if (!rc) rc = oml_alloc(m.get(), &m->part, (size_t)m->npart);
+    if (!rc) rc = wind_buffers(c) ? OQG_ERR_HIP : 0;   // the stress the Ekman velocities come from
+    if (!rc && hipStreamSynchronize(c->st) != hipSuccess) rc = OQG_ERR_HIP;
+    if (rc) {
+        g_create_err = m->err.empty() ? c->err : m->err;
+        oqg_oml_destroy(m.release());
+        return rc;
+    }
+    *out = m.release();
+    return 0;
+}
+
+void oqg_oml_destroy(oqg_oml* m) {
+    if (!m) return;
+    if (m->c && m->c->st) hipStreamSynchronize(m->c->st);
+    for (auto& kv : m->graphs) hipGraphExecDestroy(kv.second);
+    for (void* p : m->allocs) hipFree(p);
+    delete m;
+}
+
+const char* oqg_oml_last_error(const oqg_oml* m) { return m ? m->err.c_str() : g_create_err.c_str(); }
+
+int oqg_oml_set_state(oqg_oml* m, const double* sst, const double* sstm, int kind) {
+    if (!m) return OQG_ERR_ARG;
+    const size_t pb = (size_t)m->c->g.pitch * sizeof(double), w = (size_t)(m->c->g.nxp - 1) * sizeof(double);
+    const hipMemcpyKind k = kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (sst)
+        if (int r = oml_copy(m, m->sst, pb, sst, w, k)) return r;
+    if (sstm)
+        if (int r = oml_copy(m, m->sstm, pb, sstm, w, k)) return r;
+    return 0;
+}
+
+int oqg_oml_get_state(oqg_oml* m, double* sst, double* sstm, int kind) {
+    if (!m) return OQG_ERR_ARG;
+    const size_t pb = (size_t)m->c->g.pitch * sizeof(double), w = (size_t)(m->c->g.nxp - 1) * sizeof(double);
+    const hipMemcpyKind k = kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (sst)
+        if (int r = oml_copy(m, sst, w, m->sst, pb, k)) return r;
+    if (sstm)
+        if (int r = oml_copy(m, sstm, w, m->sstm, pb, k)) return r;
+    return 0;
+}
+
+int oqg_oml_set_fnet(oqg_oml* m, const double* fnet, int kind) {
+    if (!m || !fnet) return m ? oml_fail(m, OQG_ERR_ARG, "oqg_oml_set_fnet: null field") : OQG_ERR_ARG;
+    const size_t pb = (size_t)m->c->g.pitch * sizeof(double), w = (size_t)(m->c->g.nxp - 1) * sizeof(double);
+    return oml_copy(m, m->fnet, pb, fnet, w, kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+}
+
+int oqg_oml_step(oqg_oml* m, double tdto, int compute_ent) {
+    if (!m) return OQG_ERR_ARG;
+    if (compute_ent)
+        if (int r = oml_own_ent(m)) return r;
+    return enqueue_oml_step(m, tdto, compute_ent);
+}
+
+int oqg_oml_suppress(oqg_oml* m) {
+    if (!m) return OQG_ERR_ARG;
+    return enqueue_oml_suppress(m);
+}
+
+int oqg_run_ocean(oqg_oml* m, double tdto, int nsteps, int step0, int supp, int compute_ent) {
+    if (!m || nsteps < 0) return OQG_ERR_ARG;
+    oqg_ctx* c = m->c;
+    if (compute_ent)
+        if (int r = oml_own_ent(m)) return r;
+    const int P = graph_period(supp);
+    auto one = [&](int nt) -> int {
+        // step_ocn (driver.F90:89-110), then suppress_ocean (driver.F90:160-171)
+        if (int r = enqueue_oml_step(m, tdto, compute_ent)) return r;
+        if (int r = enqueue_step(c, tdto)) return oml_fail(m, r, c->err);
+        if (supp > 0 && (nt - 1) % supp == 0) {
+            if (int r = enqueue_suppress(c)) return oml_fail(m, r, c->err);
+            if (int r = enqueue_oml_suppress(m)) return r;
+        }
+        return 0;
+    };
+    int nt = step0 + 1;
+    const int end = step0 + nsteps;
+    while (nt <= end) {
+        if ((nt - 1) % P == 0 && end - nt + 1 >= P) {
+            const GraphKey key{tdto, P, supp, c->cur, G_RUN, compute_ent};
+            auto it = m->graphs.find(key);
+            if (it == m->graphs.end()) {
+                const int cur0 = c->cur;
+                hipGraph_t gr;
+                OMLCHK(m, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+                int rc = 0;
+                for (int t = 0; t < P && !rc; t++) rc = one(nt + t);
+                const hipError_t ce = hipStreamEndCapture(c->st, &gr);
+                c->cur = cur0;
+                if (rc) {
+                    if (ce == hipSuccess) hipGraphDestroy(gr);
+                    return rc;
+                }
+                OMLCHK(m, ce);
+                hipGraphExec_t ex;
+                const hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                OMLCHK(m, ie);
+                it = m->graphs.emplace(key, ex).first;
+            }
+            OMLCHK(m, hipGraphLaunch(it->second, c->st));
+            nt += P;   // P is even: the time-level parity is unchanged
+        } else {
+            if (int r = one(nt)) return r;
+            nt++;
+        }
+    }
+    return 0;
 }
 
 }  // extern "C"

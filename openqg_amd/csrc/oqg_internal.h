@@ -302,6 +302,31 @@ struct DiagArgs {
     double* partial;   // [nyp][DG_LEN]
 };
 void launch_diag(const DiagArgs& a, hipStream_t st);
+
+// Ocean mixed layer (oqg_mixed.hip).  T fields are stored with the basin's
+// pitch, nyp-1 rows.
+struct OmlArgs {
+    Geo g;
+    double hm, T1_rel, dT_12, rho_cp, d2, d4, ycexp, nval, sval;
+    int fluxn, fluxs, tau_sign;
+    const double* p1;     // layer 1 of the basin's current p
+    const double* taux;   // kinematic wind stress at p points
+    const double* tauy;
+    const double* fnet;   // net heat flux into the layer at T points
+    double* sst;          // current level (in); the new level after k_oml_finish
+    double* sstm;         // previous level (in); the current one after k_oml_finish
+    double* snew;         // new level (scratch between the two launches)
+    double* xfo;          // entrainment at T points (scratch)
+    double* part;         // per-block partial sums of xfo (k_oml_step's blocks)
+    int npart;
+    double* ent1;         // the basin's resident layer-1 entrainment (p points)
+    double tdt;
+    int compute_ent;
+};
+dim3 oml_step_grid(const Geo& g);
+void launch_oml_step(const OmlArgs& a, hipStream_t st);
+void launch_oml_finish(const OmlArgs& a, hipStream_t st);
+void launch_oml_suppress(const OmlArgs& a, hipStream_t st);
 // The rows combined (row order): sums [5][MAXL] (ke, p, q, eta, eta^2), extrema
 // [6][MAXL] (min p, max p, min q, max q, max |u|, max |v|), p at column 1 of
 // the first and last rows, the non-finite count.

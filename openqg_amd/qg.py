@@ -147,7 +147,18 @@ class QG:
         self._lib = lib
 
     # ------------------------------------------------------------ lifecycle
+    def _attach(self, obj):
+        """Register an object bound to this basin's context (a mixed layer); it is
+        closed before the context is destroyed."""
+        import weakref
+
+        self.__dict__.setdefault("_bound", []).append(weakref.ref(obj))
+
     def close(self):
+        for r in self.__dict__.pop("_bound", []):
+            o = r()
+            if o is not None:
+                o.close()
         if getattr(self, "_h", None) is not None:
             self._lib.oqg_destroy(self._h)
             self._h = None
@@ -299,6 +310,12 @@ class QG:
         check(self._lib.oqg_get_forcing(self._h, w.ctypes.data_as(ctypes.c_void_p), txi, OQG_HOST),
               self._h, "oqg_get_forcing")
         return w, float(txi[0]), float(txi[1])
+
+    def get_ent(self):
+        """The resident layer 1/2 entrainment (nyp, nxp)."""
+        e = np.empty((self.b.nyp, self.b.nxp))
+        check(self._lib.oqg_get_ent(self._h, e.ctypes.data_as(ctypes.c_void_p), OQG_HOST), self._h, "oqg_get_ent")
+        return e
 
     def step(self, tdt: float, ent=None, wekp=None, txis: float = 0.0, txin: float = 0.0):
         """step_qg (core.F90:16).  With ent and wekp None the resident forcing is used."""
