@@ -81,6 +81,10 @@ def _build_in(objdir: str, verbose: bool) -> str:
 FORTRAN = os.path.join(HERE, "fortran")
 FORTRAN_BUILD = os.path.join(FORTRAN, "build")
 FORTRAN_TEST = os.path.join(FORTRAN_BUILD, "test_oqg_c")
+# the drop-in module core, compiled against a test-only stand-in for OpenQG's
+# module qg (tests/fortran) into a program that drives it like driver.F90 does
+FORTRAN_CORE_TEST = os.path.join(FORTRAN_BUILD, "test_core")
+TESTS_FORTRAN = os.path.join(os.path.dirname(HERE), "tests", "fortran")
 
 
 def flang() -> str | None:
@@ -101,10 +105,23 @@ def build_fortran(verbose: bool = False) -> str | None:
            "-L", HERE, "-loqg", "-Wl,-rpath,$ORIGIN/../..", "-o", FORTRAN_TEST]
     if verbose:
         print(" ".join(cmd), flush=True)
+    _run_fc(cmd)
+    stub = os.path.join(TESTS_FORTRAN, "qg_types.F90")
+    if os.path.exists(stub):
+        # oqg_c, module qg, then core.F90 (which uses both), then the program
+        cmd = [fc, "-O2", "-J", FORTRAN_BUILD, os.path.join(FORTRAN, "oqg_c.F90"), stub,
+               os.path.join(FORTRAN, "core.F90"), os.path.join(TESTS_FORTRAN, "test_core.F90"),
+               "-L", HERE, "-loqg", "-Wl,-rpath,$ORIGIN/../..", "-o", FORTRAN_CORE_TEST]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _run_fc(cmd)
+    return FORTRAN_TEST
+
+
+def _run_fc(cmd):
     r = subprocess.run(cmd, cwd=FORTRAN_BUILD, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError(f"amdflang failed:\n{r.stdout.decode(errors='replace')}")
-    return FORTRAN_TEST
 
 
 if __name__ == "__main__":
