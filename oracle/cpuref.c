@@ -1558,3 +1558,421 @@ void cr_oml_step(const cr_params* P, const cr_oml_params* o, const double* p1, c
 }
 #undef TT
 #undef PP
+
+/* ------------------------------------------------------------------ */
+/* Coupling: grid, coupled stress, heat-flux forcing, atmospheric mixed layer */
+
+#define TT(a, i, j) (a)[(size_t)((j) - 1) * nxt + ((i) - 1)]
+#define PP(a, i, j) (a)[(size_t)((j) - 1) * nxp + ((i) - 1)]
+#define CR_PI 3.14159265358979324
+
+/* load_grid (grid.F90:37-58): ndxr = nint(dxa/dxo), the ocean centred in y. */
+void cr_load_grid(const cr_params* Po, const cr_params* Pa, cr_cpl* g) {
+    const int nxto = Po->nxp - 1, nyto = Po->nyp - 1, nyta = Pa->nyp - 1;
+    g->ndxr = (int)lround(Pa->dx / Po->dx);
+    g->nxtoar = nxto / g->ndxr;
+    g->nytoar = nyto / g->ndxr;
+    g->nxpoar = g->nxtoar + 1;
+    g->nypoar = g->nytoar + 1;
+    g->nxaooc = nxto / g->ndxr;
+    g->nyaooc = nyto / g->ndxr;
+    g->nx1 = 1;
+    g->ny1 = 1 + (nyta - g->nyaooc) / 2;
+}
+
+/* bcuini (bicubic.F90:97-146).  stinv maps the 16 corner data of a cell (values,
+ * then x, y and cross derivatives, each at corners (0,0),(1,0),(0,1),(1,1)) to
+ * the coefficients of s^i t^j (k = 1 + i + 4j).  It is the tensor product of the
+ * cubic Hermite basis on [0,1]: h00 = 1 - 3s^2 + 2s^3, h01 = 3s^2 - 2s^3,
+ * h10 = s - 2s^2 + s^3, h11 = -s^2 + s^3 (value at 0, value at 1, slope at 0,
+ * slope at 1); st = stfn * stinv with stfn(r, k) = ss^i tt^j at fine point r. */
+void cr_bcuini(int ndxr, double* st) {
+    static const double herm[4][4] = {{1, 0, -3, 2}, {0, 0, 3, -2}, {0, 1, -2, 1}, {0, 0, -1, 1}};
+    double stinv[16][16]; /* [k][c] */
+    for (int c = 0; c < 16; c++) {
+        const int kind = c / 4, corner = c % 4;          /* kind: value, d/ds, d/dt, d2/dsdt */
+        const int cx = corner & 1, cy = corner >> 1;
+        const int dx_ = (kind == 1 || kind == 3), dy_ = (kind == 2 || kind == 3);
+        const double* bx = herm[2 * dx_ + cx];
+        const double* by = herm[2 * dy_ + cy];
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) stinv[i + 4 * j][c] = bx[i] * by[j];
+    }
+    const int nr = (ndxr + 1) * (ndxr + 1);
+    for (int jp = 0; jp <= ndxr; jp++) {
+        const double tt = (double)jp / (double)ndxr;
+        for (int ip = 0; ip <= ndxr; ip++) {
+            const double ss = (double)ip / (double)ndxr;
+            double fn[16];
+            double sp[4] = {1.0, ss, ss * ss, ss * ss * ss}, tp[4] = {1.0, tt, tt * tt, tt * tt * tt};
+            for (int j = 0; j < 4; j++)
+                for (int i = 0; i < 4; i++) fn[i + 4 * j] = sp[i] * tp[j];
+            const int r = ip + jp * (ndxr + 1);
+            for (int c = 0; c < 16; c++) {
+                double s = 0.0;
+                for (int k = 0; k < 16; k++) s += fn[k] * stinv[k][c];
+                st[(size_t)c * nr + r] = s;
+            }
+        }
+    }
+}
+
+/* dPdx_2bc (numerics.F90:470-492): centred, one-sided with the mixed
+ * condition at solid walls, periodic in a channel, zero on the y walls. */
+void cr_dPdx_2bc(const cr_params* P, double alpha, const double* p, double* out) {
+    const int nxp = P->nxp, nyp = P->nyp;
+    for (int j = 1; j <= nyp; j++) {
+        for (int i = 2; i <= nxp - 1; i++) PP(out, i, j) = (PP(p, i + 1, j) - PP(p, i - 1, j)) / (2.0 * P->dx);
+        if (!P->cyclic) {
+            PP(out, 1, j) = (PP(p, 2, j) - PP(p, 1, j)) / (P->dx * (0.5 * alpha + 1.0));
+            PP(out, nxp, j) = (PP(p, nxp, j) - PP(p, nxp - 1, j)) / (P->dx * (0.5 * alpha + 1.0));
+        } else {
+            PP(out, 1, j) = (PP(p, 2, j) - PP(p, nxp - 1, j)) / (2.0 * P->dx);
+            PP(out, nxp, j) = PP(out, 1, j);
+        }
+    }
+    for (int i = 1; i <= nxp; i++) PP(out, i, 1) = PP(out, i, nyp) = 0.0;
+}
+
+/* dPdy_2bc (numerics.F90:494-513). */
+void cr_dPdy_2bc(const cr_params* P, double alpha, const double* p, double* out) {
+    const int nxp = P->nxp, nyp = P->nyp;
+    for (int i = 1; i <= nxp; i++) {
+        for (int j = 2; j <= nyp - 1; j++) PP(out, i, j) = (PP(p, i, j + 1) - PP(p, i, j - 1)) / (2.0 * P->dy);
+        PP(out, i, 1) = (PP(p, i, 2) - PP(p, i, 1)) / (P->dy * (0.5 * alpha + 1.0));
+        PP(out, i, nyp) = (PP(p, i, nyp) - PP(p, i, nyp - 1)) / (P->dy * (0.5 * alpha + 1.0));
+    }
+    if (!P->cyclic)
+        for (int j = 1; j <= nyp; j++) PP(out, 1, j) = PP(out, nxp, j) = 0.0;
+}
+
+/* dP2dx2_bc (numerics.F90:427-448). */
+void cr_dP2dx2_bc(const cr_params* P, double alpha, const double* p, double* out) {
+    const int nxp = P->nxp, nyp = P->nyp;
+    const double fac = 1.0 / (P->dx * P->dx);
+    for (int j = 1; j <= nyp; j++) {
+        for (int i = 2; i <= nxp - 1; i++)
+            PP(out, i, j) = fac * (PP(p, i - 1, j) + PP(p, i + 1, j) - 2.0 * PP(p, i, j));
+        if (!P->cyclic) {
+            PP(out, 1, j) = fac * alpha / (0.5 * alpha + 1.0) * (PP(p, 2, j) - PP(p, 1, j));
+            PP(out, nxp, j) = -fac * alpha / (0.5 * alpha + 1.0) * (PP(p, nxp, j) - PP(p, nxp - 1, j));
+        } else {
+            PP(out, 1, j) = fac * (PP(p, nxp - 1, j) + PP(p, 2, j) - 2.0 * PP(p, 1, j));
+            PP(out, nxp, j) = PP(out, 1, j);
+        }
+    }
+    for (int i = 1; i <= nxp; i++) PP(out, i, 1) = PP(out, i, nyp) = 0.0;
+}
+
+/* dP2dy2_bc (numerics.F90:450-468). */
+void cr_dP2dy2_bc(const cr_params* P, double alpha, const double* p, double* out) {
+    const int nxp = P->nxp, nyp = P->nyp;
+    const double fac = 1.0 / (P->dy * P->dy);
+    for (int i = 1; i <= nxp; i++) {
+        for (int j = 2; j <= nyp - 1; j++)
+            PP(out, i, j) = fac * (PP(p, i, j - 1) + PP(p, i, j + 1) - 2.0 * PP(p, i, j));
+        PP(out, i, 1) = fac * alpha / (0.5 * alpha + 1.0) * (PP(p, i, 2) - PP(p, i, 1));
+        PP(out, i, nyp) = -fac * alpha / (0.5 * alpha + 1.0) * (PP(p, i, nyp) - PP(p, i, nyp - 1));
+    }
+    if (!P->cyclic)
+        for (int j = 1; j <= nyp; j++) PP(out, 1, j) = PP(out, nxp, j) = 0.0;
+}
+
+/* new_bicubic (bicubic.F90:26-95): u, ux, uy, uxy on the (nxpoar,nypoar)
+ * coarse p points over the ocean (derivatives scaled by the coarse spacing);
+ * out (nxpo,nypo) on the fine grid, a point on a cell edge taking the value of
+ * the last cell written (bc3's loop order: j outer, i inner). */
+void cr_new_bicubic(const cr_cpl* g, int nxpo, int nypo, const double* st, const double* u, const double* ux,
+                    const double* uy, const double* uxy, double* out) {
+    const int n = g->ndxr, nr = (n + 1) * (n + 1), nxa = g->nxpoar;
+    const double* f[4] = {u, ux, uy, uxy};
+    for (int j = 1; j <= g->nytoar; j++)
+        for (int i = 1; i <= g->nxtoar; i++) {
+            double d[16];
+            for (int t = 0; t < 4; t++)   /* bc1 (bicubic.F90:79-95) */
+                for (int jq = 0; jq <= 1; jq++)
+                    for (int ip = 0; ip <= 1; ip++)
+                        d[4 * t + ip + 2 * jq] = f[t][(size_t)(j + jq - 1) * nxa + (i + ip - 1)];
+            const int iout = 1 + (i - 1) * n, jout = 1 + (j - 1) * n;
+            for (int jp = 0; jp <= n; jp++)
+                for (int ip = 0; ip <= n; ip++) {
+                    const int r = ip + jp * (n + 1);
+                    double s = 0.0;
+                    for (int k = 0; k < 16; k++) s += st[(size_t)k * nr + r] * d[k];   /* bc2 */
+                    const int io = iout + ip, jo = jout + jp;
+                    if (io <= nxpo && jo <= nypo) out[(size_t)(jo - 1) * nxpo + (io - 1)] = s;   /* bc3 */
+                }
+        }
+}
+
+/* xforc + compute_windstress + atmos_geos_vel + compute_udiff +
+ * atmos_windstress (windstress.F90:102-316).  pmo1, pma1: layer 1 of the
+ * basins' previous pressure.  Outputs the kinematic stresses at p points. */
+void cr_xforc(const cr_params* Po, const cr_params* Pa, const cr_cpl* g, const double* st, const double* pmo1,
+              const double* pma1, double* tauxo, double* tauyo, double* tauxa, double* tauya) {
+    const int nxpa = Pa->nxp, nypa = Pa->nyp, nxpo = Po->nxp, nypo = Po->nyp;
+    const size_t na = (size_t)nxpa * nypa, no = (size_t)nxpo * nypo;
+    const double fa = Pa->fnot, fo = Po->fnot;
+    double* dpdx = calloc(na, sizeof(double));
+    double* dpdy = calloc(na, sizeof(double));
+    double* dxx = calloc(na, sizeof(double));
+    double* dyy = calloc(na, sizeof(double));
+    double* dxy = calloc(na, sizeof(double));
+    double* dxxy = calloc(na, sizeof(double));
+    double* dxyy = calloc(na, sizeof(double));
+    cr_dPdx_2bc(Pa, Pa->bcco, pma1, dpdx);
+    cr_dPdy_2bc(Pa, Pa->bcco, pma1, dpdy);
+    /* the atmosphere's own stress (windstress.F90:129-145) */
+    {
+        const double cdhfaa = (g->cdat / fa) / g->hma;
+        const double qpreaa = 1.0 / (fabs(cdhfaa) * sqrt(2.0));
+        for (size_t t = 0; t < na; t++) {
+            const double u1 = -dpdy[t] / fa, v1 = dpdx[t] / fa;
+            const double u2 = 4.0 * cdhfaa * cdhfaa * (u1 * u1 + v1 * v1);
+            const double shr = qpreaa * sqrt(-1.0 + sqrt(1.0 + u2));
+            const double cuohf = cdhfaa * shr;
+            const double cdochi = g->cdat * shr / (1.0 + cuohf * cuohf);
+            tauxa[t] = cdochi * (u1 - cuohf * v1);
+            tauya[t] = cdochi * (v1 + cuohf * u1);
+        }
+    }
+    /* atmos_geos_vel (windstress.F90:201-257) */
+    cr_dP2dx2_bc(Pa, Pa->bcco, pma1, dxx);
+    cr_dP2dy2_bc(Pa, Pa->bcco, pma1, dyy);
+    cr_dPdx_2bc(Pa, Pa->bcco, dpdy, dxy);
+    cr_dPdx_2bc(Pa, Pa->bcco, dxy, dxxy);
+    cr_dPdx_2bc(Pa, Pa->bcco, dyy, dxyy);
+    const int nxa = g->nxpoar, nya = g->nypoar;
+    const size_t nc = (size_t)nxa * nya;
+    double* c8 = calloc(8 * nc, sizeof(double));
+    double *u = c8, *ux = c8 + nc, *uy = c8 + 2 * nc, *uxy = c8 + 3 * nc;
+    double *v = c8 + 4 * nc, *vx = c8 + 5 * nc, *vy = c8 + 6 * nc, *vxy = c8 + 7 * nc;
+    const double dxa = Pa->dx, dya = Pa->dy;
+    for (int j = 1; j <= nya; j++)
+        for (int i = 1; i <= nxa; i++) {
+            const size_t a = (size_t)(g->ny1 + j - 2) * nxpa + (g->nx1 + i - 2), c = (size_t)(j - 1) * nxa + (i - 1);
+            u[c] = -dpdy[a] / fa;
+            v[c] = dpdx[a] / fa;
+            ux[c] = -(dxa / fa) * dxy[a];
+            uy[c] = -(dya / fa) * dyy[a];
+            uxy[c] = -(dxa * dya / fa) * dxyy[a];
+            vx[c] = (dxa / fa) * dxx[a];
+            vy[c] = (dya / fa) * dxy[a];
+            vxy[c] = (dxa * dya / fa) * dxxy[a];
+        }
+    double* u1o = calloc(no, sizeof(double));
+    double* v1o = calloc(no, sizeof(double));
+    cr_new_bicubic(g, nxpo, nypo, st, u, ux, uy, uxy, u1o);
+    cr_new_bicubic(g, nxpo, nypo, st, v, vx, vy, vxy, v1o);
+    if (g->tau_udiff) {   /* compute_udiff (windstress.F90:259-275) */
+        double* ox = calloc(no, sizeof(double));
+        double* oy = calloc(no, sizeof(double));
+        cr_dPdx_2bc(Po, Po->bcco, pmo1, ox);
+        cr_dPdy_2bc(Po, Po->bcco, pmo1, oy);
+        for (size_t t = 0; t < no; t++) {
+            u1o[t] = u1o[t] + oy[t] / fo;
+            v1o[t] = v1o[t] - ox[t] / fo;
+        }
+        free(ox);
+        free(oy);
+    }
+    /* atmos_windstress (windstress.F90:277-314) */
+    const double raoro = g->rho_a / g->rho_o;
+    const double cdhfaa = g->tau_udiff ? (g->cdat / fa) * (1.0 / g->hma + raoro / g->hmo) : (g->cdat / fa) / g->hma;
+    const double qpreaa = 1.0 / (fabs(cdhfaa) * sqrt(2.0));
+    for (size_t t = 0; t < no; t++) {
+        const double shr = sqrt(-1.0 + sqrt(1.0 + 4.0 * cdhfaa * cdhfaa * (u1o[t] * u1o[t] + v1o[t] * v1o[t])));
+        const double cuohf = (qpreaa * cdhfaa) * shr;
+        const double cdchi = (qpreaa * g->cdat) * shr / (1.0 + cuohf * cuohf);
+        const double tx = cdchi * (u1o[t] - cuohf * v1o[t]), ty = cdchi * (v1o[t] + cuohf * u1o[t]);
+        u1o[t] = tx;   /* tauxaor */
+        v1o[t] = ty;   /* tauyaor */
+    }
+    /* compute_windstress (windstress.F90:167-199) */
+    if (g->tau_udiff)
+        for (int ja = g->ny1; ja <= g->ny1 + g->nypoar - 1; ja++)
+            for (int ia = g->nx1; ia <= g->nx1 + g->nxpoar - 1; ia++) {
+                const size_t o = (size_t)((ja - g->ny1) * g->ndxr) * nxpo + (size_t)((ia - g->nx1) * g->ndxr);
+                tauxa[(size_t)(ja - 1) * nxpa + ia - 1] = u1o[o];
+                tauya[(size_t)(ja - 1) * nxpa + ia - 1] = v1o[o];
+            }
+    for (size_t t = 0; t < no; t++) {
+        tauxo[t] = raoro * u1o[t];
+        tauyo[t] = raoro * v1o[t];
+    }
+    free(dpdx); free(dpdy); free(dxx); free(dyy); free(dxy); free(dxxy); free(dxyy);
+    free(c8); free(u1o); free(v1o);
+}
+
+/* bilint (numerics.F90:528-589): atmos T field onto the ocean T grid. */
+void cr_bilint(const cr_params* Pa, const cr_params* Po, const cr_cpl* g, const double* atmos, double* ocean) {
+    const int nxta = Pa->nxp - 1, nyta = Pa->nyp - 1, nxto = Po->nxp - 1, nyto = Po->nyp - 1;
+    const double dxainv = 1.0 / Pa->dx, dyainv = 1.0 / Pa->dy;
+    const double xta1 = (g->xa0 + 0.0 * Pa->dx) + 0.5 * Pa->dx, yta1 = (g->ya0 + 0.0 * Pa->dy) + 0.5 * Pa->dy;
+    int* iam = malloc(sizeof(int) * nxto);
+    int* iap = malloc(sizeof(int) * nxto);
+    double* wpx = malloc(sizeof(double) * nxto);
+    double* wmx = malloc(sizeof(double) * nxto);
+    for (int io = 1; io <= nxto; io++) {
+        const double xto = (g->xo0 + (io - 1) * Po->dx) + 0.5 * Po->dx;
+        int m = (int)(1.0 + dxainv * (xto - xta1));
+        int pi_ = m + 1;
+        double xam;
+        if (m >= 1) xam = (g->xa0 + (m - 1) * Pa->dx) + 0.5 * Pa->dx;
+        else xam = xta1 - Pa->dx;
+        wpx[io - 1] = dxainv * (xto - xam);
+        wmx[io - 1] = 1.0 - wpx[io - 1];
+        m = 1 + (m + nxta - 1) % nxta;
+        pi_ = 1 + (pi_ + nxta - 1) % nxta;
+        iam[io - 1] = m;
+        iap[io - 1] = pi_;
+    }
+    for (int jo = 1; jo <= nyto; jo++) {
+        const double yto = (g->yo0 + (jo - 1) * Po->dy) + 0.5 * Po->dy;
+        int jam = (int)(1.0 + dyainv * (yto - yta1));
+        int jap = jam + 1;
+        jam = jam > 1 ? jam : 1;
+        jap = jap < nyta ? jap : nyta;
+        const double yam = (g->ya0 + (jam - 1) * Pa->dy) + 0.5 * Pa->dy;
+        const double wpy = dyainv * (yto - yam), wmy = 1.0 - wpy;
+        for (int io = 1; io <= nxto; io++) {
+            const int a = iam[io - 1], b = iap[io - 1];
+            const double wm = wmx[io - 1], wp = wpx[io - 1];
+            ocean[(size_t)(jo - 1) * nxto + io - 1] =
+                wm * wmy * atmos[(size_t)(jam - 1) * nxta + a - 1] + wp * wmy * atmos[(size_t)(jam - 1) * nxta + b - 1] +
+                wm * wpy * atmos[(size_t)(jap - 1) * nxta + a - 1] + wp * wpy * atmos[(size_t)(jap - 1) * nxta + b - 1];
+        }
+    }
+    free(iam); free(iap); free(wpx); free(wmx);
+}
+
+/* fsprim (radsubs.F90:97-114). */
+static double fsprim(const cr_params* Pa, double fspco, double yrel) {
+    const double yl = (Pa->nyp - 1) * Pa->dy;
+    return fspco * 0.5 * sin(CR_PI * yrel / yl);
+}
+
+/* ytrel(j) (box.F90:144-150). */
+static double ytrel(const cr_params* P, int j) {
+    const double yl = (P->nyp - 1) * P->dy;
+    return (-0.5 * yl + (j - 1) * P->dy) + 0.5 * P->dy;
+}
+
+/* compute_forcing (forcing.F90:18-99).  sstm (ocean T), astm and hmixam
+ * (atmos T), eta1 and dtop (atmos p). */
+void cr_compute_forcing(const cr_params* Po, const cr_params* Pa, const cr_cpl* g, const cr_aml_params* a,
+                        const double* sstm, const double* astm, const double* hmixam, const double* eta1,
+                        const double* dtop, double* fnetoc, double* fnetat) {
+    const int nxto = Po->nxp - 1, nyto = Po->nyp - 1, nxt = Pa->nxp - 1, nyt = Pa->nyp - 1, nxp = Pa->nxp;
+    const size_t no = (size_t)nxto * nyto;
+    double* asto = calloc(no, sizeof(double));
+    double* up = calloc(no, sizeof(double));
+    double* sl = calloc(no, sizeof(double));
+    cr_bilint(Pa, Po, g, astm, asto);
+    for (size_t t = 0; t < no; t++) {
+        up[t] = a->D0up * sstm[t];
+        sl[t] = a->xlamda * (sstm[t] - asto[t]);
+    }
+    for (int j = 1; j <= nyt; j++) {
+        const double fs = fsprim(Pa, a->fspco, ytrel(Pa, j));
+        for (int i = 1; i <= nxt; i++) TT(fnetat, i, j) = -a->Dup0 * TT(astm, i, j) - fs;
+    }
+    for (int ja = g->ny1; ja <= g->ny1 + g->nyaooc - 1; ja++)
+        for (int ia = g->nx1; ia <= g->nx1 + g->nxaooc - 1; ia++)
+            TT(fnetat, ia, ja) = TT(fnetat, ia, ja) + (a->Dmdown - a->Dup0) * TT(astm, ia, ja);
+    for (int jo = 1; jo <= nyto; jo++) {
+        const double fsp = fsprim(Pa, a->fspco, ytrel(Po, jo));
+        for (int io = 1; io <= nxto; io++) {
+            const size_t t = (size_t)(jo - 1) * nxto + io - 1;
+            fnetoc[t] = -fsp - a->Dmdown * asto[t] - up[t] - sl[t];
+        }
+    }
+    const double ocfrac = Po->dx * Po->dy / (Pa->dx * Pa->dy);
+    for (int jo = 1; jo <= nyto; jo++) {
+        const int ja = g->ny1 + (jo - 1) / g->ndxr;
+        for (int io = 1; io <= nxto; io++) {
+            const int ia = g->nx1 + (io - 1) / g->ndxr;
+            const size_t t = (size_t)(jo - 1) * nxto + io - 1;
+            TT(fnetat, ia, ja) = TT(fnetat, ia, ja) + ocfrac * (up[t] + sl[t]);
+        }
+    }
+    const double fmafac = a->Adown11, fmatop = a->Aup00 + a->Adown10, hmafac = -a->hmadmp - a->Aup00 - a->Adown10;
+    for (int j = 1; j <= nyt; j++)
+        for (int i = 1; i <= nxt; i++) {
+            /* map_P_to_T (numerics.F90:52-61) */
+            const double me = (fmafac * 0.25) * (PP(eta1, i, j) + PP(eta1, i + 1, j) + PP(eta1, i, j + 1) + PP(eta1, i + 1, j + 1));
+            const double mt = (fmatop * 0.25) * (PP(dtop, i, j) + PP(dtop, i + 1, j) + PP(dtop, i, j + 1) + PP(dtop, i + 1, j + 1));
+            TT(fnetat, i, j) = TT(fnetat, i, j) - me - mt + hmafac * (TT(hmixam, i, j) - g->hma);
+        }
+    free(asto); free(up); free(sl);
+}
+
+/* step_aml (amlsubs.F90:184-229) with compute_new_aml (:231-287) and
+ * compute_atmos_entrainment (:289-326).  eta (nxp,nyp,nl-1) from pm, p1 the
+ * current layer-1 pressure, taux/tauy the atmosphere's stress (uek, vek, wekt
+ * from ekman.F90:73-80 with the atmosphere's hm), fnetat and xc1ast (T,
+ * set_xc1ast amlsubs.F90:131-182).  ent1 (p) is written when compute_ent is set. */
+void cr_aml_step(const cr_params* Pa, const cr_cpl* g, const cr_aml_params* a, const double* p1,
+                 const double* taux, const double* tauy, const double* eta, const double* dtop,
+                 const double* fnetat, const double* xc1ast, double tdta, int compute_ent, double* ast, double* astm,
+                 double* hmixa, double* hmixam, double* ent1) {
+    const int nxp = Pa->nxp, nyp = Pa->nyp, nxt = nxp - 1, nyt = nyp - 1, nl = Pa->nl;
+    const size_t nt = (size_t)nxt * nyt;
+    const double hm = g->hma;
+    double* uek = calloc((size_t)nxp * nyt, sizeof(double));
+    double* vek = calloc((size_t)nxt * nyp, sizeof(double));
+    double* wekt = calloc(nt, sizeof(double));
+    double* tm = calloc(nt, sizeof(double));
+    double* dh = calloc(nt, sizeof(double));
+    double* anew = calloc(nt, sizeof(double));
+    double* hnew = calloc(nt, sizeof(double));
+    double* dtc = calloc(nt, sizeof(double));
+    double* xfa = calloc(nt, sizeof(double));
+    cr_ekman_uvw(Pa, hm, taux, tauy, uek, vek, wekt);
+    /* init_atmos_ml (amlsubs.F90:84-87): ast with no flux walls, hmixa with
+     * flux walls holding hm */
+    cr_oml_params fa = {hm, a->T1_rel, a->dT_12, a->rho_cp, a->at2d, a->at4d, 1.0, 0.0, 0.0, 0, 0};
+    cr_oml_params fh = {hm, a->T1_rel, a->dT_12, a->rho_cp, a->ahmd, 0.0, 1.0, hm, hm, 1, 1};
+    cr_adv_diff(Pa, &fa, p1, uek, vek, ast, astm, tm);
+    cr_adv_diff(Pa, &fh, p1, uek, vek, hmixa, hmixam, dh);
+    const double hdrcdt = a->hmadmp / a->rho_cp;
+    const double hmainv = 1.0 / hm, rrcpat = 1.0 / a->rho_cp;
+    for (size_t t = 0; t < nt; t++) {
+        const double dT = a->T1_rel - astm[t];
+        double dT_fix;
+        if (dT >= 2.0 * hdrcdt * tdta) {
+            const double diab = hdrcdt * (hmixam[t] - hm) / dT;
+            double h = hmixam[t] + tdta * (dh[t] - diab);
+            const double fix = fmax(a->hmamin - h, 0.0);
+            h = h + fix;
+            hnew[t] = h;
+            dT_fix = fix * dT / hmixam[t];
+        } else {
+            hnew[t] = hm;
+            dT_fix = 0.0;
+        }
+        const double r = tm[t] + rrcpat * fnetat[t] / hmixam[t] - hmainv * wekt[t] * astm[t];
+        double an = astm[t] + tdta * r + dT_fix;
+        dtc[t] = fmin(0.0, a->T1_rel - an);
+        anew[t] = an + dtc[t];
+    }
+    if (compute_ent) {
+        for (size_t t = 0; t < nt; t++)
+            xfa[t] = a->xcexp * a->bface * (hmixam[t] - hm) + a->dface * (a->xcexp * astm[t] + (1.0 - a->xcexp) * xc1ast[t]) +
+                     (a->xcexp / (tdta * a->dT_12)) * hmixa[t] * dtc[t];
+        cr_map_T_to_P(Pa, xfa, ent1);
+        for (int k = 1; k <= nl - 1; k++)
+            for (size_t t = 0; t < (size_t)nxp * nyp; t++)
+                ent1[t] = ent1[t] + a->aface[k - 1] * eta[(size_t)(k - 1) * nxp * nyp + t];
+        for (size_t t = 0; t < (size_t)nxp * nyp; t++) ent1[t] = ent1[t] + a->bface * dtop[t];
+    }
+    for (size_t t = 0; t < nt; t++) {
+        astm[t] = ast[t];
+        ast[t] = anew[t];
+        hmixam[t] = hmixa[t];
+        hmixa[t] = hnew[t];
+    }
+    free(uek); free(vek); free(wekt); free(tm); free(dh); free(anew); free(hnew); free(dtc); free(xfa);
+}
+#undef TT
+#undef PP

@@ -124,6 +124,49 @@ void cr_oml_step(const cr_params* P, const cr_oml_params* o, const double* p1, c
                  const double* tauy, const double* fnet, double tdto, int compute_ent, double* sst,
                  double* sstm, double* ent1);                                 /* omlsubs.F90:69 */
 
+/* Coupling of the ocean and the atmosphere: the coupling grid, the coupled
+ * wind stress (windstress.F90:102-316 with bicubic.F90), the heat-flux forcing
+ * (forcing.F90:18-99) and the atmospheric mixed layer (amlsubs.F90:184-326). */
+typedef struct {
+    /* grid_type (grid.F90:11-27), filled by cr_load_grid (grid.F90:37-58) */
+    int ndxr, nxtoar, nytoar, nxpoar, nypoar, nxaooc, nyaooc, nx1, ny1;
+    /* box origins x0, y0 (box.F90:18, 62-63) and mixed-layer depths hm */
+    double xo0, yo0, xa0, ya0, hmo, hma;
+    double rho_o, rho_a;  /* qgo%rho, qga%rho */
+    double cdat;          /* windstress_type drag coefficient */
+    int tau_udiff;        /* stress from the atmosphere-ocean velocity difference */
+} cr_cpl;
+
+typedef struct {
+    /* the radiate_type coefficients a step reads (radsubs.F90:14-53) */
+    double D0up, Dmdown, Dup0, xlamda, fspco, Aup00, Adown10, Adown11;
+    /* atmos_mixed_type (amlsubs.F90:30-53) and its temp_type */
+    double hmamin, hmadmp, xcexp, T1_rel, dT_12, rho_cp;
+    double at2d, at4d, ahmd;
+    double aface[CR_MAXL];  /* aface(l), l = 1..nl-1, at [l-1] */
+    double bface, dface;
+} cr_aml_params;
+
+void cr_load_grid(const cr_params* Po, const cr_params* Pa, cr_cpl* g);
+void cr_bcuini(int ndxr, double* st);          /* bicubic.F90:97-146: st((ndxr+1)^2, 16), column-major */
+void cr_dPdx_2bc(const cr_params* P, double alpha, const double* p, double* out);   /* numerics.F90:470 */
+void cr_dPdy_2bc(const cr_params* P, double alpha, const double* p, double* out);   /* numerics.F90:494 */
+void cr_dP2dx2_bc(const cr_params* P, double alpha, const double* p, double* out);  /* numerics.F90:427 */
+void cr_dP2dy2_bc(const cr_params* P, double alpha, const double* p, double* out);  /* numerics.F90:450 */
+void cr_new_bicubic(const cr_cpl* g, int nxpo, int nypo, const double* st, const double* u, const double* ux,
+                    const double* uy, const double* uxy, double* out);               /* bicubic.F90:26-46 */
+void cr_xforc(const cr_params* Po, const cr_params* Pa, const cr_cpl* g, const double* st, const double* pmo1,
+              const double* pma1, double* tauxo, double* tauyo, double* tauxa, double* tauya); /* windstress.F90:102 */
+void cr_bilint(const cr_params* Pa, const cr_params* Po, const cr_cpl* g, const double* atmos,
+               double* ocean);                                                  /* numerics.F90:528-589 */
+void cr_compute_forcing(const cr_params* Po, const cr_params* Pa, const cr_cpl* g, const cr_aml_params* a,
+                        const double* sstm, const double* astm, const double* hmixam, const double* eta1,
+                        const double* dtop, double* fnetoc, double* fnetat);     /* forcing.F90:18-99 */
+void cr_aml_step(const cr_params* Pa, const cr_cpl* g, const cr_aml_params* a, const double* p1,
+                 const double* taux, const double* tauy, const double* eta, const double* dtop,
+                 const double* fnetat, const double* xc1ast, double tdta, int compute_ent, double* ast,
+                 double* astm, double* hmixa, double* hmixam, double* ent1);                                 /* amlsubs.F90:184-326 */
+
 /* OpenMP thread count used by the oracle (0 = runtime default). */
 void cr_set_threads(int n);
 

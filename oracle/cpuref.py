@@ -40,6 +40,22 @@ class CrOmlParams(ctypes.Structure):
                 ("fluxn", ctypes.c_int), ("fluxs", ctypes.c_int)]
 
 
+class CrCpl(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("ndxr", "nxtoar", "nytoar", "nxpoar", "nypoar", "nxaooc", "nyaooc", "nx1",
+                                             "ny1")] + \
+               [(n, ctypes.c_double) for n in ("xo0", "yo0", "xa0", "ya0", "hmo", "hma", "rho_o", "rho_a", "cdat")] + \
+               [("tau_udiff", ctypes.c_int)]
+
+
+_AML_D = ("D0up", "Dmdown", "Dup0", "xlamda", "fspco", "Aup00", "Adown10", "Adown11", "hmamin", "hmadmp", "xcexp",
+          "T1_rel", "dT_12", "rho_cp", "at2d", "at4d", "ahmd")
+
+
+class CrAmlParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in _AML_D] + [("aface", ctypes.c_double * MAXL), ("bface", ctypes.c_double),
+                                                         ("dface", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -93,6 +109,17 @@ def lib():
         L.cr_adv_diff.argtypes = [PP, OP, dp, dp, dp, dp, dp, dp]
         L.cr_map_T_to_P.argtypes = [PP, dp, dp]
         L.cr_oml_step.argtypes = [PP, OP, dp, dp, dp, dp, d, i, dp, dp, dp]
+        GP = ctypes.POINTER(CrCpl)
+        AP = ctypes.POINTER(CrAmlParams)
+        L.cr_load_grid.argtypes = [PP, PP, GP]
+        L.cr_bcuini.argtypes = [i, dp]
+        for n in ("cr_dPdx_2bc", "cr_dPdy_2bc", "cr_dP2dx2_bc", "cr_dP2dy2_bc"):
+            getattr(L, n).argtypes = [PP, d, dp, dp]
+        L.cr_new_bicubic.argtypes = [GP, i, i, dp, dp, dp, dp, dp, dp]
+        L.cr_xforc.argtypes = [PP, PP, GP, dp, dp, dp, dp, dp, dp, dp]
+        L.cr_bilint.argtypes = [PP, PP, GP, dp, dp]
+        L.cr_compute_forcing.argtypes = [PP, PP, GP, AP, dp, dp, dp, dp, dp, dp, dp]
+        L.cr_aml_step.argtypes = [PP, GP, AP, dp, dp, dp, dp, dp, dp, dp, d, i, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -362,3 +389,88 @@ def oml_step(cfg, o, p1, taux, tauy, fnet, tdto, sst, sstm, compute_ent=True):
     lib().cr_oml_step(ctypes.byref(P), ctypes.byref(O), _ptr(_c(p1)), _ptr(_c(taux)), _ptr(_c(tauy)), _ptr(_c(fnet)),
                       float(tdto), int(compute_ent), _ptr(sst), _ptr(sstm), _ptr(ent1))
     return sst, sstm, ent1
+
+
+# ------------------------------------------------------------------ coupling
+
+
+def cpl(cfg_o, cfg_a, xo0=0.0, yo0=0.0, xa0=0.0, ya0=0.0, hmo=100.0, hma=1000.0, rho_o=1000.0, rho_a=1.0,
+        cdat=1.3e-3, tau_udiff=False) -> CrCpl:
+    """The coupling description: load_grid (grid.F90:37-58) plus origins, depths, densities, drag."""
+    g = CrCpl()
+    lib().cr_load_grid(ctypes.byref(make_params(cfg_o)), ctypes.byref(make_params(cfg_a)), ctypes.byref(g))
+    g.xo0, g.yo0, g.xa0, g.ya0, g.hmo, g.hma = xo0, yo0, xa0, ya0, hmo, hma
+    g.rho_o, g.rho_a, g.cdat, g.tau_udiff = rho_o, rho_a, cdat, int(bool(tau_udiff))
+    return g
+
+
+def aml_params(a) -> CrAmlParams:
+    A = CrAmlParams()
+    for n in _AML_D:
+        setattr(A, n, float(a[n]))
+    for k, v in enumerate(a["aface"]):
+        A.aface[k] = float(v)
+    A.bface, A.dface = float(a["bface"]), float(a["dface"])
+    return A
+
+
+def bcuini(ndxr):
+    """st((ndxr+1)^2, 16) as a numpy array [c][r] (column-major storage)."""
+    st = np.zeros((16, (ndxr + 1) ** 2))
+    lib().cr_bcuini(int(ndxr), _ptr(st))
+    return st
+
+
+def deriv_2bc(name, cfg, alpha, p):
+    """dPdx_2bc, dPdy_2bc, dP2dx2_bc or dP2dy2_bc (numerics.F90:427-513) of a (nyp, nxp) field."""
+    P = make_params(cfg)
+    out = np.zeros((P.nyp, P.nxp))
+    getattr(lib(), "cr_" + name)(ctypes.byref(P), float(alpha), _ptr(_c(p)), _ptr(out))
+    return out
+
+
+def new_bicubic(g, nxpo, nypo, st, u, ux, uy, uxy):
+    out = np.zeros((nypo, nxpo))
+    lib().cr_new_bicubic(ctypes.byref(g), int(nxpo), int(nypo), _ptr(_c(st)), _ptr(_c(u)), _ptr(_c(ux)), _ptr(_c(uy)),
+                         _ptr(_c(uxy)), _ptr(out))
+    return out
+
+
+def xforc(cfg_o, cfg_a, g, pmo1, pma1):
+    """(tauxo, tauyo) on the ocean p grid and (tauxa, tauya) on the atmosphere's (windstress.F90:102-316)."""
+    Po, Pa = make_params(cfg_o), make_params(cfg_a)
+    st = bcuini(g.ndxr)
+    out = [np.zeros((Po.nyp, Po.nxp)) for _ in range(2)] + [np.zeros((Pa.nyp, Pa.nxp)) for _ in range(2)]
+    lib().cr_xforc(ctypes.byref(Po), ctypes.byref(Pa), ctypes.byref(g), _ptr(st), _ptr(_c(pmo1)), _ptr(_c(pma1)),
+                   *[_ptr(o) for o in out])
+    return tuple(out)
+
+
+def bilint(cfg_a, cfg_o, g, atmos):
+    Pa, Po = make_params(cfg_a), make_params(cfg_o)
+    out = np.zeros((Po.nyp - 1, Po.nxp - 1))
+    lib().cr_bilint(ctypes.byref(Pa), ctypes.byref(Po), ctypes.byref(g), _ptr(_c(atmos)), _ptr(out))
+    return out
+
+
+def compute_forcing(cfg_o, cfg_a, g, a, sstm, astm, hmixam, eta1, dtop):
+    """(fnetoc (ocean T), fnetat (atmos T)) of forcing.F90:18-99."""
+    Po, Pa = make_params(cfg_o), make_params(cfg_a)
+    fo = np.zeros((Po.nyp - 1, Po.nxp - 1))
+    fa = np.zeros((Pa.nyp - 1, Pa.nxp - 1))
+    lib().cr_compute_forcing(ctypes.byref(Po), ctypes.byref(Pa), ctypes.byref(g), ctypes.byref(aml_params(a)),
+                             _ptr(_c(sstm)), _ptr(_c(astm)), _ptr(_c(hmixam)), _ptr(_c(eta1)), _ptr(_c(dtop)),
+                             _ptr(fo), _ptr(fa))
+    return fo, fa
+
+
+def aml_step(cfg_a, g, a, p1, taux, tauy, eta, dtop, fnetat, xc1ast, tdta, ast, astm, hmixa, hmixam,
+             compute_ent=True):
+    """step_aml (amlsubs.F90:184-326): returns (ast, astm, hmixa, hmixam, ent1); inputs are not modified."""
+    Pa = make_params(cfg_a)
+    ast, astm, hmixa, hmixam = (_c(x).copy() for x in (ast, astm, hmixa, hmixam))
+    ent1 = np.zeros((Pa.nyp, Pa.nxp))
+    lib().cr_aml_step(ctypes.byref(Pa), ctypes.byref(g), ctypes.byref(aml_params(a)), _ptr(_c(p1)), _ptr(_c(taux)),
+                      _ptr(_c(tauy)), _ptr(_c(eta)), _ptr(_c(dtop)), _ptr(_c(fnetat)), _ptr(_c(xc1ast)), float(tdta),
+                      int(compute_ent), _ptr(ast), _ptr(astm), _ptr(hmixa), _ptr(hmixam), _ptr(ent1))
+    return ast, astm, hmixa, hmixam, ent1

@@ -3,7 +3,8 @@
 Each case rewrites one line of the oracle (a sign, an index, a transposed
 operand, a dropped factor in the routines of src/qg/vorsubs.F90,
 constraint.F90, homog.F90, vorticity.F90, glam/numerics.F90 and the ocean
-mixed layer that tests/test_oracle_pins.py and tests/test_oml_oracle.py fix), compiles the copy into a temporary
+mixed layers and the coupling that tests/test_oracle_pins.py, tests/test_oml_oracle.py and
+tests/test_cpl_oracle.py fix), compiles the copy into a temporary
 directory and runs the pin tests against it (OQG_CPUREF_LIB); they must fail.
 The oracle in the tree is never modified.
 """
@@ -63,6 +64,31 @@ M = [
  ("convection sign", "dtc[t] = fmax(0.0, o->T1_rel - snew[t]);", "dtc[t] = fmin(0.0, o->T1_rel - snew[t]);"),
  ("entrainment mean kept", "for (size_t t = 0; t < nt; t++) xfo[t] = xfo[t] - avg;", "for (size_t t = 0; t < nt; t++) xfo[t] = xfo[t];"),
  ("map_T_to_P corner", "PP(out, 1, 1) = TT(t, 1, 1);", "PP(out, 1, 1) = TT(t, 2, 1);"),
+ ("load_grid ny1 offset", "g->ny1 = 1 + (nyta - g->nyaooc) / 2;", "g->ny1 = 1 + (nyta - g->nyaooc + 1) / 2;"),
+ ("Hermite slope basis", "{0, 1, -2, 1}, {0, 0, -1, 1}};", "{0, 1, -2, 1}, {0, 0, 1, -1}};"),
+ ("bicubic corner order", "d[4 * t + ip + 2 * jq] =", "d[4 * t + 2 * ip + jq] ="),
+ ("dPdx_2bc wall alpha", "PP(out, 1, j) = (PP(p, 2, j) - PP(p, 1, j)) / (P->dx * (0.5 * alpha + 1.0));",
+                         "PP(out, 1, j) = (PP(p, 2, j) - PP(p, 1, j)) / (P->dx * (alpha + 1.0));"),
+ ("dP2dy2_bc south sign", "PP(out, i, 1) = fac * alpha / (0.5 * alpha + 1.0) * (PP(p, i, 2) - PP(p, i, 1));",
+                          "PP(out, i, 1) = fac * alpha / (0.5 * alpha + 1.0) * (PP(p, i, 1) - PP(p, i, 2));"),
+ ("stress turning sign", "tauya[t] = cdochi * (v1 + cuohf * u1);", "tauya[t] = cdochi * (v1 - cuohf * u1);"),
+ ("udiff ocean sign", "u1o[t] = u1o[t] + oy[t] / fo;", "u1o[t] = u1o[t] - oy[t] / fo;"),
+ ("udiff depth ratio", "(g->cdat / fa) * (1.0 / g->hma + raoro / g->hmo)", "(g->cdat / fa) * (1.0 / g->hma + 1.0 / g->hmo)"),
+ ("ux scale", "ux[c] = -(dxa / fa) * dxy[a];", "ux[c] = -(dya / fa) * dyy[a];"),
+ ("bilint y weight", "const double wpy = dyainv * (yto - yam), wmy = 1.0 - wpy;",
+                     "const double wmy = dyainv * (yto - yam), wpy = 1.0 - wmy;"),
+ ("ocean flux not gathered", "TT(fnetat, ia, ja) = TT(fnetat, ia, ja) + ocfrac * (up[t] + sl[t]);",
+                             "TT(fnetat, ia, ja) = TT(fnetat, ia, ja) + ocfrac * (up[t] - sl[t]);"),
+ ("fsprim ocean rows", "const double fsp = fsprim(Pa, a->fspco, ytrel(Po, jo));",
+                       "const double fsp = fsprim(Pa, a->fspco, ytrel(Pa, jo));"),
+ ("eta term factor", "const double me = (fmafac * 0.25)", "const double me = (fmatop * 0.25)"),
+ ("aml diabatic sign", "double h = hmixam[t] + tdta * (dh[t] - diab);", "double h = hmixam[t] + tdta * (dh[t] + diab);"),
+ ("aml convection sign", "dtc[t] = fmin(0.0, a->T1_rel - an);", "dtc[t] = fmax(0.0, a->T1_rel - an);"),
+ ("aml hmixa walls", "cr_oml_params fh = {hm, a->T1_rel, a->dT_12, a->rho_cp, a->ahmd, 0.0, 1.0, hm, hm, 1, 1};",
+                     "cr_oml_params fh = {hm, a->T1_rel, a->dT_12, a->rho_cp, a->ahmd, 0.0, 1.0, hm, hm, 0, 0};"),
+ ("aml entrainment current h", "(a->xcexp / (tdta * a->dT_12)) * hmixa[t] * dtc[t];",
+                               "(a->xcexp / (tdta * a->dT_12)) * hmixam[t] * dtc[t];"),
+ ("aml topography term", "ent1[t] = ent1[t] + a->bface * dtop[t];", "ent1[t] = ent1[t] + a->dface * dtop[t];"),
 ]
 
 
@@ -81,7 +107,7 @@ def _caught(case, tmp):
     env = dict(os.environ, OQG_CPUREF_LIB=so, OMP_NUM_THREADS="1")
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
                         os.path.join(ROOT, "tests", "test_oracle_pins.py"), os.path.join(ROOT, "tests", "test_oracle.py"),
-                        os.path.join(ROOT, "tests", "test_oml_oracle.py")],
+                        os.path.join(ROOT, "tests", "test_oml_oracle.py"), os.path.join(ROOT, "tests", "test_cpl_oracle.py")],
                        cwd=ROOT, env=env, capture_output=True, text=True)
     return name, r.returncode != 0
 
