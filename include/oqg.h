@@ -217,6 +217,9 @@ int oqg_get_forcing(oqg_ctx* ctx, double* wekp, double* txi, int kind);
 /* The resident layer 1/2 entrainment (nxp,nyp), as the coupler holds it in
  * ocn_cpl%ent(:,:,1) between step_oml and step_qg (driver.F90:103-107). */
 int oqg_get_ent(oqg_ctx* ctx, double* ent1, int kind);
+/* The resident kinematic wind stress (nxp,nyp) each; either may be NULL.
+ * The basin's eko/eka%taux, %tauy (coupler/ekman.F90:16-30). */
+int oqg_get_wind(oqg_ctx* ctx, double* taux, double* tauy, int kind);
 
 /*
  * Diagnostics of the resident state, computed on the device: the energetics
@@ -370,6 +373,65 @@ int oqg_oml_suppress(oqg_oml* oml);
  * wrote) and, when mod(nt-1, supp_every) == 0, suppress_ocean (both the basin
  * and the mixed layer).  Replays cached HIP graphs; asynchronous. */
 int oqg_run_ocean(oqg_oml* oml, double tdto, int nsteps, int step0, int supp_every, int compute_ent);
+
+/*
+ * Ocean-atmosphere coupling: the exchange of dynamic_step (driver.F90:36-87)
+ * between an ocean box with its mixed layer (oqg_oml) and an atmosphere
+ * channel, with the atmospheric mixed layer held by the coupler.  The coupler
+ * computes on the device the coupled wind stress (xforc, windstress.F90:102-316,
+ * with the bicubic interpolation of bicubic.F90), the heat-flux forcing
+ * (compute_forcing, forcing.F90:18-99) and step_aml (amlsubs.F90:184-326), and
+ * writes the stress, the ocean's heat flux and the atmosphere's entrainment
+ * into the basins' resident buffers.  The coupling grid is load_grid's
+ * (grid.F90:37-58): ndxr = nint(dxa/dxo), the ocean starting at atmosphere p
+ * point (1, 1 + (nyta - nyto/ndxr)/2); the ocean's nxt and nyt must be
+ * multiples of ndxr.  The radiation coefficients (radiate_type,
+ * radsubs.F90:14-53) and the entrainment coefficients (init_entrain_coeff,
+ * amlsubs.F90:93-129) are inputs.
+ */
+typedef struct oqg_cpl oqg_cpl;
+
+typedef struct {
+    double xo0, yo0, xa0, ya0;   /* box origins x0, y0 (box.F90:18, 62-63) (m) */
+    double hmo, hma;             /* ocean and atmosphere mixed-layer depths go%hm, ga%hm (m) */
+    double rho_o, rho_a;         /* qgo%rho, qga%rho */
+    double cdat;                 /* drag coefficient (windstress_type) */
+    int tau_udiff;               /* stress from the atmosphere-ocean velocity difference */
+    double D0up, Dmdown, Dup0, xlamda, fspco, Aup00, Adown10, Adown11;   /* rad%D0up, %Dmdown, %Dup(0), %xlamda,
+                                    %fspco, %Aup(0,0), %Adown(1,0), %Adown(1,1) */
+    double hmamin, hmadmp, xcexp;          /* atmos_mixed_type */
+    double T1_rel, dT_12, rho_cp;          /* the atmosphere's temp_type */
+    double at2d, at4d, ahmd;               /* ast Del^2, Del^4 and hmixa Del^2 diffusivities */
+    double aface[OQG_MAX_LAYERS];          /* aface(l), l = 1..nl-1 at [l-1] */
+    double bface, dface;
+    const double* xc1ast;        /* (nxta, nyta) host field (set_xc1ast); NULL: zero */
+    const double* dtop;          /* (nxpa, nypa) atmosphere topography, host; NULL: flat */
+} oqg_cpl_desc;
+
+#define OQG_CPL_OCEAN_ENT 1      /* step_oml computes the ocean's entrainment (ent_coupled) */
+#define OQG_CPL_ATMOS_ENT 2      /* step_aml computes the atmosphere's entrainment */
+
+int oqg_cpl_create(oqg_oml* ocean_ml, oqg_ctx* atmos, const oqg_cpl_desc* d, oqg_cpl** out);
+void oqg_cpl_destroy(oqg_cpl* cpl);
+const char* oqg_cpl_last_error(const oqg_cpl* cpl);
+/* The atmospheric mixed layer: aml%ast%data, %datam, aml%hmixa%data, %datam (nxta, nyta). */
+int oqg_cpl_set_aml_state(oqg_cpl* cpl, const double* ast, const double* astm, const double* hmixa,
+                          const double* hmixam, int kind);
+int oqg_cpl_get_aml_state(oqg_cpl* cpl, double* ast, double* astm, double* hmixa, double* hmixam, int kind);
+/* The last heat fluxes: fnetoc (ocean T, = the ocean mixed layer's fnet) and fnetat (atmosphere T). */
+int oqg_cpl_get_fnet(oqg_cpl* cpl, double* fnetoc, double* fnetat, int kind);
+/* xforc + ekman_from_tau of both basins (driver.F90:47-53) from their previous pressure. */
+int oqg_cpl_xforc(oqg_cpl* cpl);
+/* compute_forcing (forcing.F90:18-99): the ocean's fnet and the atmosphere's fnetat. */
+int oqg_cpl_forcing(oqg_cpl* cpl);
+/* step_aml (amlsubs.F90:184-229) on the atmosphere's current p and resident stress. */
+int oqg_cpl_aml_step(oqg_cpl* cpl, double tdta, int compute_ent);
+/* nsteps atmospheric steps of dynamic_step from step0 (openqg.F90:483-488):
+ * step nt is an ocean step when mod(nt-1, nstr) == 0, suppresses the ocean when
+ * mod(nt-1, osupp) == 0 and the atmosphere when mod(nt-1, asupp) == 0 (0: never).
+ * Replays cached HIP graphs; asynchronous. */
+int oqg_cpl_run(oqg_cpl* cpl, int nsteps, int step0, double tdto, double tdta, int nstr, int osupp, int asupp,
+                int flags);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,20 @@ class OqgOmlDesc(ctypes.Structure):
                                                 "sval")] + [("fluxn", ctypes.c_int), ("fluxs", ctypes.c_int)]
 
 
+_CPL_D = ("D0up", "Dmdown", "Dup0", "xlamda", "fspco", "Aup00", "Adown10", "Adown11", "hmamin", "hmadmp", "xcexp",
+          "T1_rel", "dT_12", "rho_cp", "at2d", "at4d", "ahmd")
+OQG_CPL_OCEAN_ENT, OQG_CPL_ATMOS_ENT = 1, 2
+
+
+class OqgCplDesc(ctypes.Structure):
+    """Mirror of `oqg_cpl_desc` (include/oqg.h)."""
+
+    _fields_ = [(n, ctypes.c_double) for n in ("xo0", "yo0", "xa0", "ya0", "hmo", "hma", "rho_o", "rho_a", "cdat")] + \
+               [("tau_udiff", ctypes.c_int)] + [(n, ctypes.c_double) for n in _CPL_D] + \
+               [("aface", ctypes.c_double * MAX_LAYERS), ("bface", ctypes.c_double), ("dface", ctypes.c_double),
+                ("xc1ast", _dp), ("dtop", _dp)]
+
+
 # (name, restype, argtypes) for every entry point of include/oqg.h
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -96,6 +110,7 @@ SIGNATURES = [
     ("oqg_adj_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_get_forcing", _i, [_vp, _vp, _dp, _i]),
     ("oqg_get_ent", _i, [_vp, _vp, _i]),
+    ("oqg_get_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_diagnose", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
     ("oqg_diagnose_noalloc", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
     ("oqg_invert", _i, [_vp]),
@@ -126,6 +141,16 @@ SIGNATURES = [
     ("oqg_oml_step", _i, [_vp, _d, _i]),
     ("oqg_oml_suppress", _i, [_vp]),
     ("oqg_run_ocean", _i, [_vp, _d, _i, _i, _i, _i]),
+    ("oqg_cpl_create", _i, [_vp, _vp, ctypes.POINTER(OqgCplDesc), ctypes.POINTER(_vp)]),
+    ("oqg_cpl_destroy", None, [_vp]),
+    ("oqg_cpl_last_error", ctypes.c_char_p, [_vp]),
+    ("oqg_cpl_set_aml_state", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    ("oqg_cpl_get_aml_state", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    ("oqg_cpl_get_fnet", _i, [_vp, _vp, _vp, _i]),
+    ("oqg_cpl_xforc", _i, [_vp]),
+    ("oqg_cpl_forcing", _i, [_vp]),
+    ("oqg_cpl_aml_step", _i, [_vp, _d, _i]),
+    ("oqg_cpl_run", _i, [_vp, _i, _i, _d, _d, _i, _i, _i, _i]),
 ]
 
 _lib = None

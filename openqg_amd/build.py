@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "liboqg.so")
 SOURCES = ["oqg_stencil.hip", "oqg_transform.hip", "oqg_adjoint.hip", "oqg_forcing.hip", "oqg_diag.hip",
-           "oqg_mixed.hip", "oqg_api.hip"]
+           "oqg_mixed.hip", "oqg_couple.hip", "oqg_api.hip"]
 ARCH = os.environ.get("OQG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

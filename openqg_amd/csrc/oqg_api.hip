@@ -1115,6 +1115,17 @@ int oqg_get_forcing(oqg_ctx* c, double* wekp, double* txi, int kind) {
     return 0;
 }
 
+int oqg_get_wind(oqg_ctx* c, double* taux, double* tauy, int kind) {
+    if (!c) return OQG_ERR_ARG;
+    if (!c->taux_d) return fail(c, OQG_ERR_ARG, "oqg_get_wind: the basin has no resident stress");
+    if (taux)
+        if (int r = download_field(c, taux, c->taux_d, 1, kind)) return r;
+    if (tauy)
+        if (int r = download_field(c, tauy, c->tauy_d, 1, kind)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
 int oqg_get_ent(oqg_ctx* c, double* ent1, int kind) {
     if (!c || !ent1) return c ? fail(c, OQG_ERR_ARG, "oqg_get_ent: null field") : OQG_ERR_ARG;
     if (int r = download_field(c, ent1, c->ent1, 1, kind)) return r;
@@ -2814,3 +2825,469 @@ int oqg_run_ocean(oqg_oml* m, double tdto, int nsteps, int step0, int supp, int 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ coupling
+// The ocean-atmosphere exchange of dynamic_step (driver.F90:36-87) between an
+// ocean basin with its mixed layer and an atmosphere basin, whose atmospheric
+// mixed layer the coupler holds.  Everything is enqueued on the ocean basin's
+// stream (the atmosphere's work joins it through events).
+
+struct oqg_cpl {
+    oqg_oml* om = nullptr;   // the ocean mixed layer; its basin is om->c
+    oqg_ctx* ca = nullptr;   // the atmosphere basin
+    oqg_cpl_desc d{};
+    int ndxr = 0, nxtoar = 0, nytoar = 0, nxpoar = 0, nypoar = 0, nxaooc = 0, nyaooc = 0, nx1 = 0, ny1 = 0;
+    double *ast = nullptr, *astm = nullptr, *hmixa = nullptr, *hmixam = nullptr, *anew = nullptr, *hnew = nullptr;
+    double *xfa = nullptr, *fnetat = nullptr, *xc1ast = nullptr, *dtop = nullptr;
+    double *scr = nullptr, *coarse = nullptr, *st = nullptr;
+    std::vector<void*> allocs;
+    std::map<std::array<double, 10>, hipGraphExec_t> graphs;   // oqg_cpl_run
+    hipEvent_t ev = nullptr;
+    std::string err;
+};
+
+static int cpl_fail(oqg_cpl* p, int code, const std::string& msg) {
+    if (p) p->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define CPLCHK(p, expr)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return cpl_fail(p, OQG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static int cpl_alloc(oqg_cpl* p, double** out, size_t n) {
+    void* v = nullptr;
+    CPLCHK(p, hipMalloc(&v, n * sizeof(double)));
+    CPLCHK(p, hipMemsetAsync(v, 0, n * sizeof(double), p->om->c->st));
+    p->allocs.push_back(v);
+    *out = static_cast<double*>(v);
+    return 0;
+}
+
+// The bicubic weights of bcuini (bicubic.F90:97-146): for fine point r = ip +
+// jp (ndxr+1) of a cell and corner datum c (values, then x, y and cross
+// derivatives, each at corners (0,0), (1,0), (0,1), (1,1)), st[c][r] is the sum
+// over the monomials s^i t^j of their value at (ip, jp)/ndxr times the
+// datum's coefficient, which is a product of cubic Hermite basis coefficients.
+static void bicubic_weights(int n, std::vector<double>& st) {
+    static const double herm[4][4] = {{1, 0, -3, 2}, {0, 0, 3, -2}, {0, 1, -2, 1}, {0, 0, -1, 1}};
+    const int nr = (n + 1) * (n + 1);
+    double coef[16][16];   // [monomial][datum]
+    for (int c = 0; c < 16; c++) {
+        const int kind = c / 4, corner = c % 4;
+        const int sx = (kind == 1 || kind == 3) ? 2 : 0, sy = (kind == 2 || kind == 3) ? 2 : 0;
+        const double* bx = herm[sx + (corner & 1)];
+        const double* by = herm[sy + (corner >> 1)];
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) coef[i + 4 * j][c] = bx[i] * by[j];
+    }
+    st.assign((size_t)16 * nr, 0.0);
+    for (int jp = 0; jp <= n; jp++) {
+        const double tt = (double)jp / (double)n;
+        const double tp[4] = {1.0, tt, tt * tt, tt * tt * tt};
+        for (int ip = 0; ip <= n; ip++) {
+            const double ss = (double)ip / (double)n;
+            const double sp[4] = {1.0, ss, ss * ss, ss * ss * ss};
+            const int r = ip + jp * (n + 1);
+            for (int c = 0; c < 16; c++) {
+                double s = 0.0;
+                for (int j = 0; j < 4; j++)
+                    for (int i = 0; i < 4; i++) s += (sp[i] * tp[j]) * coef[i + 4 * j][c];
+                st[(size_t)c * nr + r] = s;
+            }
+        }
+    }
+}
+
+static CplArgs cpl_args(oqg_cpl* p, double tdta, int compute_ent) {
+    oqg_ctx *co = p->om->c, *ca = p->ca;
+    const oqg_cpl_desc& d = p->d;
+    CplArgs a{};
+    a.go = co->g;
+    a.ga = ca->g;
+    a.ndxr = p->ndxr;
+    a.nxtoar = p->nxtoar;
+    a.nytoar = p->nytoar;
+    a.nxpoar = p->nxpoar;
+    a.nypoar = p->nypoar;
+    a.nxaooc = p->nxaooc;
+    a.nyaooc = p->nyaooc;
+    a.nx1 = p->nx1;
+    a.ny1 = p->ny1;
+    a.xo0 = d.xo0;
+    a.yo0 = d.yo0;
+    a.xa0 = d.xa0;
+    a.ya0 = d.ya0;
+    a.hmo = d.hmo;
+    a.hma = d.hma;
+    a.raoro = d.rho_a / d.rho_o;
+    a.cdat = d.cdat;
+    a.bcco_o = co->d.bcco;
+    a.bcco_a = ca->d.bcco;
+    a.tau_udiff = d.tau_udiff ? 1 : 0;
+    a.tau_sign_a = ca->d.tau_sign;
+    a.D0up = d.D0up;
+    a.Dmdown = d.Dmdown;
+    a.Dup0 = d.Dup0;
+    a.xlamda = d.xlamda;
+    a.fspco = d.fspco;
+    a.Aup00 = d.Aup00;
+    a.Adown10 = d.Adown10;
+    a.Adown11 = d.Adown11;
+    a.hmamin = d.hmamin;
+    a.hmadmp = d.hmadmp;
+    a.xcexp = d.xcexp;
+    a.T1_rel = d.T1_rel;
+    a.dT_12 = d.dT_12;
+    a.rho_cp = d.rho_cp;
+    a.at2d = d.at2d;
+    a.at4d = d.at4d;
+    a.ahmd = d.ahmd;
+    for (int k = 0; k < MAXL; k++) {
+        a.aface[k] = d.aface[k];
+        a.gpa[k] = ca->d.gp[k];
+    }
+    a.bface = d.bface;
+    a.dface = d.dface;
+    a.pmo = co->pbuf[1 - co->cur];
+    a.pma = ca->pbuf[1 - ca->cur];
+    a.pa1 = ca->pbuf[ca->cur];
+    a.tauxo = co->taux_d;
+    a.tauyo = co->tauy_d;
+    a.tauxa = ca->taux_d;
+    a.tauya = ca->tauy_d;
+    a.scr = p->scr;
+    a.coarse = p->coarse;
+    a.st = p->st;
+    a.sstm = p->om->sstm;
+    a.fnetoc = p->om->fnet;
+    a.ast = p->ast;
+    a.astm = p->astm;
+    a.hmixa = p->hmixa;
+    a.hmixam = p->hmixam;
+    a.anew = p->anew;
+    a.hnew = p->hnew;
+    a.xfa = p->xfa;
+    a.fnetat = p->fnetat;
+    a.xc1ast = p->xc1ast;
+    a.dtop = p->dtop;
+    a.enta = ca->ent1;
+    a.tdta = tdta;
+    a.compute_ent = compute_ent;
+    return a;
+}
+
+// ekman_from_tau (ekman.F90:53-83) of a basin's resident stress.
+static int enqueue_ekman(oqg_ctx* c) {
+    EkmanArgs a{};
+    a.g = c->g;
+    a.taux = c->taux_d;
+    a.tauy = c->tauy_d;
+    a.wekp = c->wekp;
+    a.fcon = c->fcon;
+    launch_ekman(a, c->st);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// The atmosphere's work goes on the ocean's stream for the duration of a call.
+struct OnStream {
+    oqg_ctx* c;
+    hipStream_t saved;
+    OnStream(oqg_ctx* c_, hipStream_t s) : c(c_), saved(c_->st) { c->st = s; }
+    ~OnStream() { c->st = saved; }
+};
+
+// Order the ocean stream after the atmosphere's pending work (begin) and the
+// atmosphere's stream after what the call enqueued (end).
+static int cpl_join(oqg_cpl* p, bool begin) {
+    hipStream_t so = p->om->c->st, sa = p->ca->st;
+    if (so == sa) return 0;
+    CPLCHK(p, hipEventRecord(p->ev, begin ? sa : so));
+    CPLCHK(p, hipStreamWaitEvent(begin ? so : sa, p->ev, 0));
+    return 0;
+}
+
+static int cpl_own_ent(oqg_cpl* p, int ent_o, int ent_a) {
+    if (ent_o)
+        if (int r = oml_own_ent(p->om)) return cpl_fail(p, r, p->om->err);
+    oqg_ctx* ca = p->ca;
+    if (ent_a && ca->has_ent2) {
+        ca->has_ent2 = false;
+        static const double zero = 0.0;
+        CPLCHK(p, hipMemcpyAsync(ca->fcon + F_ENT2, &zero, sizeof zero, hipMemcpyHostToDevice, ca->st));
+        CPLCHK(p, hipStreamSynchronize(ca->st));
+    }
+    return 0;
+}
+
+static int enqueue_xforc(oqg_cpl* p) {
+    launch_xforc(cpl_args(p, 0.0, 0), p->om->c->st);
+    CPLCHK(p, hipGetLastError());
+    if (int r = enqueue_ekman(p->om->c)) return cpl_fail(p, r, p->om->c->err);
+    if (int r = enqueue_ekman(p->ca)) return cpl_fail(p, r, p->ca->err);
+    return 0;
+}
+
+// One atmospheric step nt of dynamic_step (driver.F90:36-87).
+static int enqueue_dynamic(oqg_cpl* p, int nt, double tdto, double tdta, int nstr, int osupp, int asupp, int ent_o,
+                           int ent_a) {
+    oqg_ctx *co = p->om->c, *ca = p->ca;
+    const bool ocean_step = (nt - 1) % nstr == 0;
+    if (ocean_step)
+        if (int r = enqueue_xforc(p)) return r;
+    // step_atm (driver.F90:112-147)
+    if (ocean_step) launch_forcing(cpl_args(p, tdta, ent_a), co->st);
+    launch_aml_step(cpl_args(p, tdta, ent_a), co->st);
+    CPLCHK(p, hipGetLastError());
+    if (int r = enqueue_step(ca, tdta)) return cpl_fail(p, r, ca->err);
+    // step_ocn (driver.F90:89-110)
+    if (ocean_step) {
+        if (int r = enqueue_oml_step(p->om, tdto, ent_o)) return cpl_fail(p, r, p->om->err);
+        if (int r = enqueue_step(co, tdto)) return cpl_fail(p, r, co->err);
+    }
+    // suppress_ocean / suppress_atmos (driver.F90:149-171)
+    if (osupp > 0 && (nt - 1) % osupp == 0) {
+        if (int r = enqueue_suppress(co)) return cpl_fail(p, r, co->err);
+        if (int r = enqueue_oml_suppress(p->om)) return cpl_fail(p, r, p->om->err);
+    }
+    if (asupp > 0 && (nt - 1) % asupp == 0) {
+        if (int r = enqueue_suppress(ca)) return cpl_fail(p, r, ca->err);
+        launch_aml_suppress(cpl_args(p, tdta, 0), co->st);
+        CPLCHK(p, hipGetLastError());
+    }
+    return 0;
+}
+
+static int lcm(int a, int b) {
+    int x = a, y = b;
+    while (y) {
+        const int t = x % y;
+        x = y;
+        y = t;
+    }
+    return a / x * b;
+}
+
+static int cpl_copy(oqg_cpl* p, double* dst, size_t dpitch, const double* src, size_t spitch, size_t w, size_t rows,
+                    hipMemcpyKind kind) {
+    hipStream_t s = p->om->c->st;
+    CPLCHK(p, hipMemcpy2DAsync(dst, dpitch, src, spitch, w, rows, kind, s));
+    CPLCHK(p, hipStreamSynchronize(s));
+    return 0;
+}
+
+static int cpl_run_steps(oqg_cpl* p, int nsteps, int step0, double tdto, double tdta, int nstr, int osupp, int asupp,
+                         int flags, int ent_o, int ent_a);
+
+extern "C" {
+
+int oqg_cpl_create(oqg_oml* om, oqg_ctx* ca, const oqg_cpl_desc* d, oqg_cpl** out) {
+    if (!out) return cpl_fail(nullptr, OQG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!om || !ca || !d) return cpl_fail(nullptr, OQG_ERR_ARG, "oqg_cpl_create: null ocean mixed layer, basin or description");
+    oqg_ctx* co = om->c;
+    if (co->g.cyclic || !ca->g.cyclic)
+        return cpl_fail(nullptr, OQG_ERR_ARG, "oqg_cpl_create: the ocean must be a box and the atmosphere a channel");
+    if (ca->g.nl < 2) return cpl_fail(nullptr, OQG_ERR_ARG, "oqg_cpl_create: the atmosphere needs >= 2 layers");
+    if (!(d->hma > 0) || !(d->hmo > 0) || !(d->rho_cp > 0) || !(d->rho_o > 0) || !(d->rho_a > 0) || d->dT_12 == 0.0)
+        return cpl_fail(nullptr, OQG_ERR_ARG, "oqg_cpl_create: need hmo, hma, rho_o, rho_a, rho_cp > 0 and dT_12 != 0");
+    auto p = std::make_unique<oqg_cpl>();
+    p->om = om;
+    p->ca = ca;
+    p->d = *d;
+    p->d.xc1ast = nullptr;
+    p->d.dtop = nullptr;
+    // load_grid (grid.F90:37-58)
+    const int nxto = co->g.nxp - 1, nyto = co->g.nyp - 1, nxta = ca->g.nxp - 1, nyta = ca->g.nyp - 1;
+    p->ndxr = (int)std::lround(ca->g.dx / co->g.dx);
+    if (p->ndxr < 1 || nxto % p->ndxr || nyto % p->ndxr)
+        return cpl_fail(nullptr, OQG_ERR_ARG, "oqg_cpl_create: the ocean's cells must tile whole atmosphere cells "
+                                              "(nxt and nyt of the ocean multiples of dxa/dxo)");
+    p->nxtoar = nxto / p->ndxr;
+    p->nytoar = nyto / p->ndxr;
+    p->nxpoar = p->nxtoar + 1;
+    p->nypoar = p->nytoar + 1;
+    p->nxaooc = nxto / p->ndxr;
+    p->nyaooc = nyto / p->ndxr;
+    p->nx1 = 1;
+    p->ny1 = 1 + (nyta - p->nyaooc) / 2;
+    if (p->nxaooc > nxta || p->ny1 < 1 || p->nx1 + p->nxpoar - 1 > ca->g.nxp || p->ny1 + p->nypoar - 1 > ca->g.nyp)
+        return cpl_fail(nullptr, OQG_ERR_ARG, "oqg_cpl_create: the ocean does not fit inside the atmosphere");
+    int rc = 0;
+    const size_t ta = (size_t)ca->g.pitch * (ca->g.nyp - 1);
+    for (double** f : {&p->ast, &p->astm, &p->hmixa, &p->hmixam, &p->anew, &p->hnew, &p->xfa, &p->fnetat, &p->xc1ast})
+        if (!rc) rc = cpl_alloc(p.get(), f, ta);
+    if (!rc) rc = cpl_alloc(p.get(), &p->dtop, (size_t)ca->g.fs);
+    if (!rc) rc = cpl_alloc(p.get(), &p->scr, 6 * (size_t)ca->g.fs);
+    if (!rc) rc = cpl_alloc(p.get(), &p->coarse, 8 * (size_t)p->nxpoar * p->nypoar);
+    std::vector<double> st;
+    bicubic_weights(p->ndxr, st);
+    if (!rc) rc = cpl_alloc(p.get(), &p->st, st.size());
+    if (!rc) rc = wind_buffers(co) ? OQG_ERR_HIP : 0;
+    if (!rc) rc = wind_buffers(ca) ? OQG_ERR_HIP : 0;
+    if (!rc && hipEventCreateWithFlags(&p->ev, hipEventDisableTiming) != hipSuccess) rc = OQG_ERR_HIP;
+    hipStream_t s = co->st;
+    if (!rc) rc = hipMemcpyAsync(p->st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, s) ? OQG_ERR_HIP : 0;
+    const size_t wa = (size_t)nxta * sizeof(double), pa = (size_t)ca->g.pitch * sizeof(double);
+    if (!rc && d->xc1ast)
+        rc = hipMemcpy2DAsync(p->xc1ast, pa, d->xc1ast, wa, wa, nyta, hipMemcpyHostToDevice, s) ? OQG_ERR_HIP : 0;
+    if (!rc && d->dtop)
+        rc = hipMemcpy2DAsync(p->dtop, pa, d->dtop, wa + sizeof(double), wa + sizeof(double), ca->g.nyp,
+                              hipMemcpyHostToDevice, s) ? OQG_ERR_HIP : 0;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = OQG_ERR_HIP;
+    if (rc) {
+        g_create_err = p->err.empty() ? std::string("oqg_cpl_create: device allocation or copy failed") : p->err;
+        oqg_cpl_destroy(p.release());
+        return rc;
+    }
+    *out = p.release();
+    return 0;
+}
+
+void oqg_cpl_destroy(oqg_cpl* p) {
+    if (!p) return;
+    if (p->om && p->om->c && p->om->c->st) hipStreamSynchronize(p->om->c->st);
+    for (auto& kv : p->graphs) hipGraphExecDestroy(kv.second);
+    for (void* v : p->allocs) hipFree(v);
+    if (p->ev) hipEventDestroy(p->ev);
+    delete p;
+}
+
+const char* oqg_cpl_last_error(const oqg_cpl* p) { return p ? p->err.c_str() : g_create_err.c_str(); }
+
+int oqg_cpl_set_aml_state(oqg_cpl* p, const double* ast, const double* astm, const double* hmixa,
+                          const double* hmixam, int kind) {
+    if (!p) return OQG_ERR_ARG;
+    const Geo& g = p->ca->g;
+    const size_t w = (size_t)(g.nxp - 1) * sizeof(double), pb = (size_t)g.pitch * sizeof(double);
+    const hipMemcpyKind k = kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const double* src[4] = {ast, astm, hmixa, hmixam};
+    double* dst[4] = {p->ast, p->astm, p->hmixa, p->hmixam};
+    for (int f = 0; f < 4; f++)
+        if (src[f])
+            if (int r = cpl_copy(p, dst[f], pb, src[f], w, w, g.nyp - 1, k)) return r;
+    return 0;
+}
+
+int oqg_cpl_get_aml_state(oqg_cpl* p, double* ast, double* astm, double* hmixa, double* hmixam, int kind) {
+    if (!p) return OQG_ERR_ARG;
+    const Geo& g = p->ca->g;
+    const size_t w = (size_t)(g.nxp - 1) * sizeof(double), pb = (size_t)g.pitch * sizeof(double);
+    const hipMemcpyKind k = kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (int r = cpl_join(p, true)) return r;
+    double* dst[4] = {ast, astm, hmixa, hmixam};
+    const double* src[4] = {p->ast, p->astm, p->hmixa, p->hmixam};
+    for (int f = 0; f < 4; f++)
+        if (dst[f])
+            if (int r = cpl_copy(p, dst[f], w, src[f], pb, w, g.nyp - 1, k)) return r;
+    return 0;
+}
+
+int oqg_cpl_get_fnet(oqg_cpl* p, double* fnetoc, double* fnetat, int kind) {
+    if (!p) return OQG_ERR_ARG;
+    const hipMemcpyKind k = kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (int r = cpl_join(p, true)) return r;
+    if (fnetoc) {
+        const Geo& g = p->om->c->g;
+        const size_t w = (size_t)(g.nxp - 1) * sizeof(double), pb = (size_t)g.pitch * sizeof(double);
+        if (int r = cpl_copy(p, fnetoc, w, p->om->fnet, pb, w, g.nyp - 1, k)) return r;
+    }
+    if (fnetat) {
+        const Geo& g = p->ca->g;
+        const size_t w = (size_t)(g.nxp - 1) * sizeof(double), pb = (size_t)g.pitch * sizeof(double);
+        if (int r = cpl_copy(p, fnetat, w, p->fnetat, pb, w, g.nyp - 1, k)) return r;
+    }
+    return 0;
+}
+
+int oqg_cpl_xforc(oqg_cpl* p) {
+    if (!p) return OQG_ERR_ARG;
+    if (int r = cpl_join(p, true)) return r;
+    {
+        OnStream on(p->ca, p->om->c->st);
+        if (int r = enqueue_xforc(p)) return r;
+    }
+    return cpl_join(p, false);
+}
+
+int oqg_cpl_forcing(oqg_cpl* p) {
+    if (!p) return OQG_ERR_ARG;
+    if (int r = cpl_join(p, true)) return r;
+    launch_forcing(cpl_args(p, 0.0, 0), p->om->c->st);
+    CPLCHK(p, hipGetLastError());
+    return cpl_join(p, false);
+}
+
+int oqg_cpl_aml_step(oqg_cpl* p, double tdta, int compute_ent) {
+    if (!p) return OQG_ERR_ARG;
+    if (int r = cpl_own_ent(p, 0, compute_ent)) return r;
+    if (int r = cpl_join(p, true)) return r;
+    launch_aml_step(cpl_args(p, tdta, compute_ent), p->om->c->st);
+    CPLCHK(p, hipGetLastError());
+    return cpl_join(p, false);
+}
+
+int oqg_cpl_run(oqg_cpl* p, int nsteps, int step0, double tdto, double tdta, int nstr, int osupp, int asupp,
+                int flags) {
+    if (!p || nsteps < 0 || nstr < 1 || osupp < 0 || asupp < 0) return p ? cpl_fail(p, OQG_ERR_ARG, "oqg_cpl_run: bad arguments") : OQG_ERR_ARG;
+    const int ent_o = (flags & OQG_CPL_OCEAN_ENT) ? 1 : 0, ent_a = (flags & OQG_CPL_ATMOS_ENT) ? 1 : 0;
+    if (int r = cpl_own_ent(p, ent_o, ent_a)) return r;
+    if (int r = cpl_join(p, true)) return r;
+    if (int r = cpl_run_steps(p, nsteps, step0, tdto, tdta, nstr, osupp, asupp, flags, ent_o, ent_a)) return r;
+    return cpl_join(p, false);
+}
+
+}  // extern "C"
+
+static int cpl_run_steps(oqg_cpl* p, int nsteps, int step0, double tdto, double tdta, int nstr, int osupp, int asupp,
+                         int flags, int ent_o, int ent_a) {
+    oqg_ctx *co = p->om->c, *ca = p->ca;
+    OnStream on(ca, co->st);
+    // the cadence repeats every lcm(nstr, osupp, asupp) steps; a graph also
+    // needs both basins back at their starting time-level parity
+    int P = nstr;
+    if (osupp > 0) P = lcm(P, osupp);
+    if (asupp > 0) P = lcm(P, asupp);
+    if (P % 2) P *= 2;
+    if ((P / nstr) % 2) P *= 2;
+    int nt = step0 + 1;
+    const int end = step0 + nsteps;
+    while (nt <= end) {
+        if ((nt - 1) % P == 0 && end - nt + 1 >= P && P <= 4096) {
+            const std::array<double, 10> key{tdto, tdta, (double)P, (double)nstr, (double)osupp, (double)asupp,
+                                             (double)co->cur, (double)ca->cur, (double)flags, 0.0};
+            auto it = p->graphs.find(key);
+            if (it == p->graphs.end()) {
+                const int c0 = co->cur, a0 = ca->cur;
+                hipGraph_t gr;
+                CPLCHK(p, hipStreamBeginCapture(co->st, hipStreamCaptureModeThreadLocal));
+                int rc = 0;
+                for (int t = 0; t < P && !rc; t++) rc = enqueue_dynamic(p, nt + t, tdto, tdta, nstr, osupp, asupp, ent_o, ent_a);
+                const hipError_t ce = hipStreamEndCapture(co->st, &gr);
+                co->cur = c0;
+                ca->cur = a0;
+                if (rc) {
+                    if (ce == hipSuccess) hipGraphDestroy(gr);
+                    return rc;
+                }
+                CPLCHK(p, ce);
+                hipGraphExec_t ex;
+                const hipError_t ie = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                hipGraphDestroy(gr);
+                CPLCHK(p, ie);
+                it = p->graphs.emplace(key, ex).first;
+            }
+            CPLCHK(p, hipGraphLaunch(it->second, co->st));
+            nt += P;
+        } else {
+            if (int r = enqueue_dynamic(p, nt, tdto, tdta, nstr, osupp, asupp, ent_o, ent_a)) return r;
+            nt++;
+        }
+    }
+    return 0;
+}

@@ -1,0 +1,375 @@
+// oqg_couple.hip -- the ocean-atmosphere exchange of dynamic_step
+// (driver.F90:36-87) on the device:
+//   k_xforc_atm   one block over the atmosphere's p grid: the atmosphere's
+//                 geostrophic velocity and its derivatives from layer 1 of
+//                 its previous pressure, its own stress (windstress.F90:129-145)
+//                 and the bicubic corner data over the ocean
+//                 (atmos_geos_vel, windstress.F90:201-257);
+//   k_xforc_ocn   one thread per ocean p point: the bicubic velocity
+//                 (bicubic.F90:26-95), the ocean velocity subtracted when the
+//                 stress depends on the velocity difference (:259-275), the
+//                 stress (atmos_windstress :277-314) and its ocean and
+//                 atmosphere copies (compute_windstress :167-199);
+//   k_force_ocn   one thread per ocean T point: bilinear atmospheric
+//                 temperature (numerics.F90:528-589) and the ocean's heat flux
+//                 (forcing.F90:43-72);
+//   k_force_atm   one thread per atmosphere T point: its heat flux
+//                 (forcing.F90:52-97), the flux from the ocean cells under it
+//                 gathered in the reference's loop order;
+//   k_aml_step    one thread per atmosphere T point: step_aml's two adv_diff
+//                 terms and compute_new_aml (amlsubs.F90:207-287);
+//   k_aml_finish  one thread per atmosphere p point: the entrainment
+//                 (amlsubs.F90:289-326) into the basin's resident forcing and
+//                 the time levels shifted by copies.
+// The atmosphere grid is small (about 5000 p points), so its derivative chain
+// runs in one block with the intermediate fields in a global scratch area,
+// its phases separated by block barriers.
+
+#include "oqg_mixed_dev.h"
+
+namespace oqg {
+
+namespace {
+
+constexpr double CPL_PI = 3.14159265358979324;
+
+// dPdx_2bc (numerics.F90:470-492) at p point (i, j), 0-based, of field F.
+template <class F>
+__device__ __forceinline__ double ddx_2bc(const F& p, const Geo& g, double alpha, int i, int j) {
+    const int nxp = g.nxp, nyp = g.nyp;
+    if (j == 0 || j == nyp - 1) return 0.0;
+    if (i >= 1 && i <= nxp - 2) return (p(i + 1, j) - p(i - 1, j)) / (2.0 * g.dx);
+    if (!g.cyclic) {
+        if (i == 0) return (p(1, j) - p(0, j)) / (g.dx * (0.5 * alpha + 1.0));
+        return (p(nxp - 1, j) - p(nxp - 2, j)) / (g.dx * (0.5 * alpha + 1.0));
+    }
+    return (p(1, j) - p(nxp - 2, j)) / (2.0 * g.dx);
+}
+
+// dPdy_2bc (numerics.F90:494-513).
+template <class F>
+__device__ __forceinline__ double ddy_2bc(const F& p, const Geo& g, double alpha, int i, int j) {
+    const int nxp = g.nxp, nyp = g.nyp;
+    if (!g.cyclic && (i == 0 || i == nxp - 1)) return 0.0;
+    if (j >= 1 && j <= nyp - 2) return (p(i, j + 1) - p(i, j - 1)) / (2.0 * g.dy);
+    if (j == 0) return (p(i, 1) - p(i, 0)) / (g.dy * (0.5 * alpha + 1.0));
+    return (p(i, nyp - 1) - p(i, nyp - 2)) / (g.dy * (0.5 * alpha + 1.0));
+}
+
+// dP2dx2_bc (numerics.F90:427-448).
+template <class F>
+__device__ __forceinline__ double dxx_2bc(const F& p, const Geo& g, double alpha, int i, int j) {
+    const int nxp = g.nxp, nyp = g.nyp;
+    if (j == 0 || j == nyp - 1) return 0.0;
+    const double fac = 1.0 / (g.dx * g.dx);
+    if (i >= 1 && i <= nxp - 2) return fac * (p(i - 1, j) + p(i + 1, j) - 2.0 * p(i, j));
+    if (!g.cyclic) {
+        if (i == 0) return fac * alpha / (0.5 * alpha + 1.0) * (p(1, j) - p(0, j));
+        return -fac * alpha / (0.5 * alpha + 1.0) * (p(nxp - 1, j) - p(nxp - 2, j));
+    }
+    return fac * (p(nxp - 2, j) + p(1, j) - 2.0 * p(0, j));
+}
+
+// dP2dy2_bc (numerics.F90:450-468).
+template <class F>
+__device__ __forceinline__ double dyy_2bc(const F& p, const Geo& g, double alpha, int i, int j) {
+    const int nxp = g.nxp, nyp = g.nyp;
+    if (!g.cyclic && (i == 0 || i == nxp - 1)) return 0.0;
+    const double fac = 1.0 / (g.dy * g.dy);
+    if (j >= 1 && j <= nyp - 2) return fac * (p(i, j - 1) + p(i, j + 1) - 2.0 * p(i, j));
+    if (j == 0) return fac * alpha / (0.5 * alpha + 1.0) * (p(i, 1) - p(i, 0));
+    return -fac * alpha / (0.5 * alpha + 1.0) * (p(i, nyp - 1) - p(i, nyp - 2));
+}
+
+struct PF {   // a pitched p field
+    const double* f;
+    long long pitch;
+    __device__ __forceinline__ double operator()(int i, int j) const { return f[(long long)j * pitch + i]; }
+};
+
+// bilint (numerics.F90:528-589) of the atmosphere T field `at` at ocean T point (io, jo), 0-based.
+__device__ __forceinline__ double bilint_at(const CplArgs& a, const double* at, int io, int jo) {
+    const Geo &ga = a.ga, &go = a.go;
+    const int nxta = ga.nxp - 1, nyta = ga.nyp - 1;
+    const double dxainv = 1.0 / ga.dx, dyainv = 1.0 / ga.dy;
+    const double xta1 = (a.xa0 + 0.0 * ga.dx) + 0.5 * ga.dx, yta1 = (a.ya0 + 0.0 * ga.dy) + 0.5 * ga.dy;
+    const double xto = (a.xo0 + io * go.dx) + 0.5 * go.dx;
+    int m = (int)(1.0 + dxainv * (xto - xta1));
+    int mp = m + 1;
+    const double xam = (m >= 1) ? (a.xa0 + (m - 1) * ga.dx) + 0.5 * ga.dx : xta1 - ga.dx;
+    const double wp = dxainv * (xto - xam), wm = 1.0 - wp;
+    m = 1 + (m + nxta - 1) % nxta;
+    mp = 1 + (mp + nxta - 1) % nxta;
+    const double yto = (a.yo0 + jo * go.dy) + 0.5 * go.dy;
+    int jam = (int)(1.0 + dyainv * (yto - yta1));
+    int jap = jam + 1;
+    jam = jam > 1 ? jam : 1;
+    jap = jap < nyta ? jap : nyta;
+    const double yam = (a.ya0 + (jam - 1) * ga.dy) + 0.5 * ga.dy;
+    const double wpy = dyainv * (yto - yam), wmy = 1.0 - wpy;
+    const long long p = ga.pitch;
+    const double* r0 = at + (long long)(jam - 1) * p;
+    const double* r1 = at + (long long)(jap - 1) * p;
+    return wm * wmy * r0[m - 1] + wp * wmy * r0[mp - 1] + wm * wpy * r1[m - 1] + wp * wpy * r1[mp - 1];
+}
+
+// fsprim (radsubs.F90:97-114) at T row j (0-based) of box g, with the atmosphere's yl.
+__device__ __forceinline__ double fsprim_row(const CplArgs& a, const Geo& g, int j) {
+    const double yl = (g.nyp - 1) * g.dy, yla = (a.ga.nyp - 1) * a.ga.dy;
+    const double yrel = (-0.5 * yl + j * g.dy) + 0.5 * g.dy;
+    return a.fspco * 0.5 * sin(CPL_PI * yrel / yla);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(1024) void k_xforc_atm(CplArgs a) {
+    const Geo& g = a.ga;
+    const int nxp = g.nxp, nyp = g.nyp, n = nxp * nyp;
+    const long long P = g.pitch, fs = g.fs;
+    double *dpdx = a.scr, *dpdy = a.scr + fs, *dxx = a.scr + 2 * fs, *dyy = a.scr + 3 * fs, *dxy = a.scr + 4 * fs,
+           *dxyy = a.scr + 5 * fs;
+    const PF pm{a.pma, P};
+    const double fa = g.fnot, al = a.bcco_a;
+    const double cdhfaa = (a.cdat / fa) / a.hma;
+    const double qpreaa = 1.0 / (fabs(cdhfaa) * sqrt(2.0));
+    // phase 1: first derivatives and the atmosphere's own stress (windstress.F90:129-145)
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int j = t / nxp, i = t - j * nxp;
+        const long long o = (long long)j * P + i;
+        const double px = ddx_2bc(pm, g, al, i, j), py = ddy_2bc(pm, g, al, i, j);
+        dpdx[o] = px;
+        dpdy[o] = py;
+        dxx[o] = dxx_2bc(pm, g, al, i, j);
+        dyy[o] = dyy_2bc(pm, g, al, i, j);
+        const double u1 = -py / fa, v1 = px / fa;
+        const double u2 = 4.0 * cdhfaa * cdhfaa * (u1 * u1 + v1 * v1);
+        const double shr = qpreaa * sqrt(-1.0 + sqrt(1.0 + u2));
+        const double cuohf = cdhfaa * shr;
+        const double cdochi = a.cdat * shr / (1.0 + cuohf * cuohf);
+        a.tauxa[o] = cdochi * (u1 - cuohf * v1);
+        a.tauya[o] = cdochi * (v1 + cuohf * u1);
+    }
+    __syncthreads();
+    // phase 2: the x derivatives of dpdy and dyy
+    const PF fy{dpdy, P}, fyy{dyy, P};
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int j = t / nxp, i = t - j * nxp;
+        const long long o = (long long)j * P + i;
+        dxy[o] = ddx_2bc(fy, g, al, i, j);
+        dxyy[o] = ddx_2bc(fyy, g, al, i, j);
+    }
+    __syncthreads();
+    // phase 3: the bicubic corner data on the (nxpoar, nypoar) points over the ocean
+    const PF fxy{dxy, P};
+    const int nc = a.nxpoar * a.nypoar;
+    const double dxa = g.dx, dya = g.dy;
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+        const int jc = t / a.nxpoar, ic = t - jc * a.nxpoar;
+        const int i = a.nx1 - 1 + ic, j = a.ny1 - 1 + jc;
+        const long long o = (long long)j * P + i;
+        const double dxxy = ddx_2bc(fxy, g, al, i, j);
+        double* c = a.coarse + t;
+        c[0 * nc] = -dpdy[o] / fa;
+        c[1 * nc] = -(dxa / fa) * dxy[o];
+        c[2 * nc] = -(dya / fa) * dyy[o];
+        c[3 * nc] = -(dxa * dya / fa) * dxyy[o];
+        c[4 * nc] = dpdx[o] / fa;
+        c[5 * nc] = (dxa / fa) * dxx[o];
+        c[6 * nc] = (dya / fa) * dxy[o];
+        c[7 * nc] = (dxa * dya / fa) * dxxy;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_xforc_ocn(CplArgs a) {
+    const Geo& g = a.go;
+    const int io = blockIdx.x * 64 + (threadIdx.x & 63), jo = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (io >= g.nxp || jo >= g.nyp) return;
+    const int n = a.ndxr, nr = (n + 1) * (n + 1), nc = a.nxpoar * a.nypoar;
+    // the cell whose block covers this point last (bc3's loop order)
+    const int ic = min(io / n, a.nxtoar - 1), jc = min(jo / n, a.nytoar - 1);
+    const int r = (io - ic * n) + (jo - jc * n) * (n + 1);
+    double vel[2];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        const double* base = a.coarse + 4 * f * nc;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int t = k >> 2, corner = k & 3;
+            const int ip = corner & 1, jq = corner >> 1;
+            const double d = base[t * nc + (jc + jq) * a.nxpoar + (ic + ip)];
+            s += a.st[(long long)k * nr + r] * d;
+        }
+        vel[f] = s;
+    }
+    double u = vel[0], v = vel[1];
+    if (a.tau_udiff) {   // compute_udiff (windstress.F90:259-275)
+        const PF pm{a.pmo, g.pitch};
+        const double ox = ddx_2bc(pm, g, a.bcco_o, io, jo), oy = ddy_2bc(pm, g, a.bcco_o, io, jo);
+        u = u + oy / g.fnot;
+        v = v - ox / g.fnot;
+    }
+    const double fa = a.ga.fnot;
+    const double cdhfaa = a.tau_udiff ? (a.cdat / fa) * (1.0 / a.hma + a.raoro / a.hmo) : (a.cdat / fa) / a.hma;
+    const double qpreaa = 1.0 / (fabs(cdhfaa) * sqrt(2.0));
+    const double shr = sqrt(-1.0 + sqrt(1.0 + 4.0 * cdhfaa * cdhfaa * (u * u + v * v)));
+    const double cuohf = (qpreaa * cdhfaa) * shr;
+    const double cdchi = (qpreaa * a.cdat) * shr / (1.0 + cuohf * cuohf);
+    const double tx = cdchi * (u - cuohf * v), ty = cdchi * (v + cuohf * u);
+    const long long o = (long long)jo * g.pitch + io;
+    a.tauxo[o] = a.raoro * tx;
+    a.tauyo[o] = a.raoro * ty;
+    if (a.tau_udiff && io % n == 0 && jo % n == 0) {
+        const long long oa = (long long)(a.ny1 - 1 + jo / n) * a.ga.pitch + (a.nx1 - 1 + io / n);
+        a.tauxa[oa] = tx;
+        a.tauya[oa] = ty;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_force_ocn(CplArgs a) {
+    const Geo& g = a.go;
+    const int io = blockIdx.x * 64 + (threadIdx.x & 63), jo = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (io >= g.nxp - 1 || jo >= g.nyp - 1) return;
+    const long long o = (long long)jo * g.pitch + io;
+    const double asto = bilint_at(a, a.astm, io, jo);
+    const double s = a.sstm[o];
+    const double up = a.D0up * s, sl = a.xlamda * (s - asto);
+    const double fsp = fsprim_row(a, g, jo);
+    a.fnetoc[o] = -fsp - a.Dmdown * asto - up - sl;
+}
+
+__global__ __launch_bounds__(256) void k_force_atm(CplArgs a) {
+    const Geo &g = a.ga, &go = a.go;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxt = g.nxp - 1, nyt = g.nyp - 1;
+    const int ja = t / nxt, ia = t - ja * nxt;   // 0-based
+    if (ja >= nyt) return;
+    const long long P = g.pitch, o = (long long)ja * P + ia;
+    const double am = a.astm[o];
+    double f = -a.Dup0 * am - fsprim_row(a, g, ja);
+    const int ia1 = ia - (a.nx1 - 1), ja1 = ja - (a.ny1 - 1);   // position in the cells over the ocean
+    if (ia1 >= 0 && ia1 < a.nxaooc && ja1 >= 0 && ja1 < a.nyaooc) f = f + (a.Dmdown - a.Dup0) * am;
+    // the ocean cells under this one, in the reference's loop order (jo outer, io inner)
+    const double ocfrac = go.dx * go.dy / (g.dx * g.dy);
+    const int n = a.ndxr, nxto = go.nxp - 1, nyto = go.nyp - 1;
+    if (ia1 >= 0 && ja1 >= 0) {
+        const int jo0 = ja1 * n, io0 = ia1 * n;
+        for (int jo = jo0; jo < min(jo0 + n, nyto); jo++)
+            for (int io = io0; io < min(io0 + n, nxto); io++) {
+                const double s = a.sstm[(long long)jo * go.pitch + io];
+                const double asto = bilint_at(a, a.astm, io, jo);
+                f = f + ocfrac * (a.D0up * s + a.xlamda * (s - asto));
+            }
+    }
+    // the thickness, topography and mixed-layer depth terms (forcing.F90:87-97)
+    const double fmafac = a.Adown11, fmatop = a.Aup00 + a.Adown10, hmafac = -a.hmadmp - a.Aup00 - a.Adown10;
+    auto eta1 = [&](int i, int j) {
+        const long long q = (long long)j * P + i;
+        return (a.pma[q] - a.pma[q + g.fs]) / a.gpa[0];
+    };
+    auto top = [&](int i, int j) { return a.dtop[(long long)j * P + i]; };
+    const double me = (fmafac * 0.25) * (eta1(ia, ja) + eta1(ia + 1, ja) + eta1(ia, ja + 1) + eta1(ia + 1, ja + 1));
+    const double mt = (fmatop * 0.25) * (top(ia, ja) + top(ia + 1, ja) + top(ia, ja + 1) + top(ia + 1, ja + 1));
+    a.fnetat[o] = f - me - mt + hmafac * (a.hmixam[o] - a.hma);
+}
+
+__global__ __launch_bounds__(256) void k_aml_step(CplArgs a) {
+    const Geo& g = a.ga;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= g.nxp - 1 || j >= g.nyp - 1) return;
+    const long long o = (long long)j * g.pitch + i;
+    const double hm = a.hma;
+    // init_atmos_ml (amlsubs.F90:84-87): ast with no flux walls, hmixa with flux walls holding hm
+    const AdvField fa{a.ast, a.astm, a.at2d, a.at4d, 1.0, 0.0, 0.0, 0, 0};
+    const AdvField fh{a.hmixa, a.hmixam, a.ahmd, 0.0, 1.0, hm, hm, 1, 1};
+    const double tm = adv_diff_at(g, fa, a.pa1, a.tauxa, a.tauya, hm, a.tau_sign_a, i, j);
+    const double dh = adv_diff_at(g, fh, a.pa1, a.tauxa, a.tauya, hm, a.tau_sign_a, i, j);
+    const double wekt = wekt_at(g, a.tauxa, a.tauya, i, j);
+    // compute_new_aml (amlsubs.F90:231-287)
+    const double tdta = a.tdta, astm = a.astm[o], hmixam = a.hmixam[o];
+    const double dT = a.T1_rel - astm;
+    const double hdrcdt = a.hmadmp / a.rho_cp;
+    double hnew, dT_fix;
+    if (dT >= 2.0 * hdrcdt * tdta) {
+        const double diab = hdrcdt * (hmixam - hm) / dT;
+        double h = hmixam + tdta * (dh - diab);
+        const double fix = fmax(a.hmamin - h, 0.0);
+        h = h + fix;
+        hnew = h;
+        dT_fix = fix * dT / hmixam;
+    } else {
+        hnew = hm;
+        dT_fix = 0.0;
+    }
+    const double hmainv = 1.0 / hm, rrcpat = 1.0 / a.rho_cp;
+    const double r = tm + rrcpat * a.fnetat[o] / hmixam - hmainv * wekt * astm;
+    const double an = astm + tdta * r + dT_fix;
+    const double dtc = fmin(0.0, a.T1_rel - an);
+    a.anew[o] = an + dtc;
+    a.hnew[o] = hnew;
+    // compute_atmos_entrainment's T-point terms (amlsubs.F90:303-309)
+    a.xfa[o] = a.xcexp * a.bface * (hmixam - hm) + a.dface * (a.xcexp * astm + (1.0 - a.xcexp) * a.xc1ast[o]) +
+               (a.xcexp / (tdta * a.dT_12)) * a.hmixa[o] * dtc;
+}
+
+__global__ __launch_bounds__(256) void k_aml_finish(CplArgs a) {
+    const Geo& g = a.ga;
+    const int nxp = g.nxp, nyp = g.nyp;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = t / nxp, i = t - j * nxp;
+    if (j >= nyp) return;
+    const long long P = g.pitch, o = (long long)j * P + i;
+    if (a.compute_ent) {
+        auto X = [&](int ii, int jj) { return a.xfa[(long long)jj * P + ii]; };
+        double e = map_T_to_P_at(X, i, j, nxp, nyp, g.cyclic);
+        for (int k = 0; k < g.nl - 1; k++) {
+            const double eta = (a.pma[o + k * g.fs] - a.pma[o + (k + 1) * g.fs]) / a.gpa[k];
+            e = e + a.aface[k] * eta;
+        }
+        a.enta[o] = e + a.bface * a.dtop[o];
+    }
+    if (i < nxp - 1 && j < nyp - 1) {
+        const double as = a.ast[o], hs = a.hmixa[o];
+        a.astm[o] = as;
+        a.ast[o] = a.anew[o];
+        a.hmixam[o] = hs;
+        a.hmixa[o] = a.hnew[o];
+    }
+}
+
+// suppress_atmos's mixed-layer part (driver.F90:150-157)
+__global__ __launch_bounds__(256) void k_aml_suppress(CplArgs a) {
+    const int nxt = a.ga.nxp - 1, nyt = a.ga.nyp - 1;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = t / nxt, i = t - j * nxt;
+    if (j >= nyt) return;
+    const long long o = (long long)j * a.ga.pitch + i;
+    a.ast[o] = 0.5 * (a.ast[o] + a.astm[o]);
+    a.hmixa[o] = 0.5 * (a.hmixa[o] + a.hmixam[o]);
+}
+
+static dim3 tgrid(int nx, int ny) { return dim3((nx + 63) / 64, (ny + 3) / 4); }
+
+void launch_xforc(const CplArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_xforc_atm, dim3(1), dim3(1024), 0, st, a);
+    hipLaunchKernelGGL(k_xforc_ocn, tgrid(a.go.nxp, a.go.nyp), dim3(256), 0, st, a);
+}
+
+void launch_forcing(const CplArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_force_ocn, tgrid(a.go.nxp - 1, a.go.nyp - 1), dim3(256), 0, st, a);
+    const int n = (a.ga.nxp - 1) * (a.ga.nyp - 1);
+    hipLaunchKernelGGL(k_force_atm, dim3((n + 255) / 256), dim3(256), 0, st, a);
+}
+
+void launch_aml_step(const CplArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_aml_step, tgrid(a.ga.nxp - 1, a.ga.nyp - 1), dim3(256), 0, st, a);
+    const int n = a.ga.nxp * a.ga.nyp;
+    hipLaunchKernelGGL(k_aml_finish, dim3((n + 255) / 256), dim3(256), 0, st, a);
+}
+
+void launch_aml_suppress(const CplArgs& a, hipStream_t st) {
+    const int n = (a.ga.nxp - 1) * (a.ga.nyp - 1);
+    hipLaunchKernelGGL(k_aml_suppress, dim3((n + 255) / 256), dim3(256), 0, st, a);
+}
+
+}  // namespace oqg

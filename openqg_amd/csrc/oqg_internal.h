@@ -324,6 +324,48 @@ struct OmlArgs {
     int compute_ent;
 };
 dim3 oml_step_grid(const Geo& g);
+// adv_diff (mixed_layer.F90:16-95) of one mixed-layer field at T point (i, j),
+// 0-based: its C-grid advection by the layer-1 geostrophic plus Ekman velocity
+// and its Del^2/Del^4 diffusion.  The Ekman velocity comes from the stress at
+// p points with depth hm (ekman.F90:73-77).
+struct AdvField {
+    const double* data;    // current level (advected)
+    const double* datam;   // previous level (diffused)
+    double d2, d4, ycexp, nval, sval;
+    int fluxn, fluxs;
+};
+// Ocean-atmosphere coupling (windstress.F90, forcing.F90, amlsubs.F90): the
+// coupling grid, the fields either side and the coefficients one step reads.
+struct CplArgs {
+    Geo go, ga;                       // ocean and atmosphere geometry
+    int ndxr, nxtoar, nytoar, nxpoar, nypoar, nxaooc, nyaooc, nx1, ny1;   // grid_type, nx1/ny1 1-based
+    double xo0, yo0, xa0, ya0;        // box origins
+    double hmo, hma, raoro, cdat, bcco_o, bcco_a;
+    int tau_udiff, tau_sign_a;
+    double D0up, Dmdown, Dup0, xlamda, fspco, Aup00, Adown10, Adown11;
+    double hmamin, hmadmp, xcexp, T1_rel, dT_12, rho_cp, at2d, at4d, ahmd;
+    double aface[MAXL], gpa[MAXL];
+    double bface, dface;
+    const double* pmo;                // ocean previous p (layer 1 at offset 0)
+    const double* pma;                // atmosphere previous p, all layers
+    const double* pa1;                // atmosphere current p, layer 1
+    double *tauxo, *tauyo, *tauxa, *tauya;   // the basins' resident stress
+    double* scr;                      // atmosphere p scratch: dpdx, dpdy, dxx, dyy, dxy, dxyy
+    double* coarse;                   // u, ux, uy, uxy, v, vx, vy, vxy on the (nxpoar, nypoar) points
+    const double* st;                 // bicubic weights st[c][r], ((ndxr+1)^2, 16)
+    const double* sstm;               // ocean mixed layer, previous level (T)
+    double* fnetoc;                   // the ocean mixed layer's heat flux (T)
+    double *ast, *astm, *hmixa, *hmixam, *anew, *hnew, *xfa, *fnetat;   // atmosphere T fields
+    const double* xc1ast;             // T
+    const double* dtop;               // atmosphere topography (p)
+    double* enta;                     // the atmosphere's resident layer-1 entrainment (p)
+    double tdta;
+    int compute_ent;
+};
+void launch_xforc(const CplArgs& a, hipStream_t st);
+void launch_forcing(const CplArgs& a, hipStream_t st);
+void launch_aml_step(const CplArgs& a, hipStream_t st);
+void launch_aml_suppress(const CplArgs& a, hipStream_t st);
 void launch_oml_step(const OmlArgs& a, hipStream_t st);
 void launch_oml_finish(const OmlArgs& a, hipStream_t st);
 void launch_oml_suppress(const OmlArgs& a, hipStream_t st);

@@ -155,7 +155,7 @@ class QG:
         self.__dict__.setdefault("_bound", []).append(weakref.ref(obj))
 
     def close(self):
-        for r in self.__dict__.pop("_bound", []):
+        for r in reversed(self.__dict__.pop("_bound", [])):   # the last bound first (it may use earlier ones)
             o = r()
             if o is not None:
                 o.close()
@@ -310,6 +310,13 @@ class QG:
         check(self._lib.oqg_get_forcing(self._h, w.ctypes.data_as(ctypes.c_void_p), txi, OQG_HOST),
               self._h, "oqg_get_forcing")
         return w, float(txi[0]), float(txi[1])
+
+    def get_wind(self):
+        """The resident kinematic wind stress (taux, tauy), each (nyp, nxp)."""
+        tx, ty = np.empty((self.b.nyp, self.b.nxp)), np.empty((self.b.nyp, self.b.nxp))
+        check(self._lib.oqg_get_wind(self._h, tx.ctypes.data_as(ctypes.c_void_p), ty.ctypes.data_as(ctypes.c_void_p),
+                                     OQG_HOST), self._h, "oqg_get_wind")
+        return tx, ty
 
     def get_ent(self):
         """The resident layer 1/2 entrainment (nyp, nxp)."""
