@@ -19,8 +19,11 @@ Also reported:
                 plain forward run, store-all and with 32 revolve snapshots;
                 and a batch of 16 perturbed members' gradients, one after
                 another and concurrently (one context and stream each);
-  coupled       the coupled configuration's 385^2 ocean and 97 x 49 atmosphere
-                (QG only) through the run loop for 30 days.
+  coupled       the coupled configuration for 30 days: the 385^2 ocean with its
+                100 m mixed layer and the 97 x 49 atmosphere with its 1000 m
+                mixed layer, exchanging stress, heat flux and entrainment every
+                ocean step (dynamic_step in one captured graph per cadence
+                period), against the two basins' QG-only runs.
 """
 from __future__ import annotations
 
@@ -312,16 +315,22 @@ def adjoint_bench(days: float, snapshots: int, batch: int = 16):
 
 
 def coupled_bench(days: float):
-    """The coupled configuration's two basins, QG only: the 385^2 ocean box
-    (dt = 1200 s) and the 97 x 49 atmosphere channel (dt = 200 s, nstr = 6)
-    through the run loop (openqg_amd.driver: the reference's step cadence,
-    daily validity checks, the lastday dump) for `days` simulated days, each
-    basin on its own stream on its resident forcing; against the two basins'
-    plain runs one after the other."""
-    import tempfile
+    """The coupled configuration: the 385^2 ocean box (dt = 1200 s) with a
+    100 m mixed layer and the 97 x 49 atmosphere channel (dt = 200 s, nstr = 6)
+    with a 1000 m mixed layer, coupled through the device coupler
+    (openqg_amd.coupled: stress, heat flux, both mixed layers, suppression every
+    25 ocean and 100 atmosphere steps), radiation coefficients from the
+    double-gyre radiative balance, mixed layers started at radiative balance;
+    `days` simulated days after a one-day warm-up (graph capture).  Against the
+    two basins' QG-only runs one after the other."""
+    from dataclasses import replace
+
+    import numpy as np
 
     from openqg_amd import workloads as W
-    from openqg_amd.driver import Basin, Clock, Valids, run
+    from openqg_amd.coupled import Coupler, load_grid
+    from openqg_amd.mixed import OceanMixedLayer, zonal_temp
+    from openqg_amd.radiation import coupled_setup
 
     wo, wa = W.ocean385(), W.atmos_channel()
     nstr = int(round(wo.dt / wa.dt))
@@ -335,29 +344,51 @@ def coupled_bench(days: float):
         return qg
 
     o, a = make(wo), make(wa)
-    with tempfile.TemporaryDirectory() as out:
-        val = lambda: Valids(int(round(86400.0 / wa.dt)), 1.0e12, 1.0e3)
-        warm = Clock(dta=wa.dt, nstr=nstr, tini=0.0, trun=1.0 / 365.0)
-        run(warm, out, ocn=Basin("ocean", o, val()), atm=Basin("atmos", a, val()))   # graph capture
-        clk = Clock(dta=wa.dt, nstr=nstr, tini=1.0 / 365.0, trun=days / 365.0)
-        t0 = time.perf_counter()
-        run(clk, out, ocn=Basin("ocean", o, val()), atm=Basin("atmos", a, val()))
-        t_cpl = time.perf_counter() - t0
-    no, na = clk.ntsrun // nstr, clk.ntsrun
+    go, ga = replace(o.b, hm=100.0), replace(a.b, hm=1000.0)
+    g = load_grid(go, ga)
+    prm, octemp, attemp, rad, xc1, _ = coupled_setup(go, ga, yo0=(g.ny1 - 1) * ga.dx)
+    oml = OceanMixedLayer(o, octemp, fluxn=False, fluxs=True, st2d=380.0, st4d=4.0e10, ycexp=1.0, hm=go.hm)
+    sst = np.array([zonal_temp(go, octemp, j) for j in range(1, go.nyt + 1)])[:, None] + np.zeros((1, go.nxt))
+    oml.set_state(sst, sst)
+    cp = Coupler(oml, a, prm, xc1ast=xc1)
+    ast = np.array([zonal_temp(ga, attemp, j) for j in range(1, ga.nyt + 1)])[:, None] + np.zeros((1, ga.nxt))
+    h = np.full(ast.shape, ga.hm)
+    cp.set_aml_state(ast, ast, h, h)
+    cp.xforc()     # the initial exchange (openqg.F90:456-478)
+    cp.forcing()
+    nday = int(round(86400.0 / wa.dt))
+    cp.run(nday, 0, wo.tdt, wa.tdt, nstr)   # one day: captures the cadence graph
+    o.sync()
+    na = int(round(days * nday))
     t0 = time.perf_counter()
-    o.run(clk.tdto, no, 0, 25)
+    cp.run(na, nday, wo.tdt, wa.tdt, nstr)
+    o.sync()
+    t_cpl = time.perf_counter() - t0
+    bad = o.diagnose(wo.dt)["nonfinite"] + a.diagnose(wa.dt)["nonfinite"]
+    s_o, s_m = oml.get_state()
+    ast1, _, hm1, _ = cp.get_aml_state()
+    bad += float(np.count_nonzero(~np.isfinite(s_o)) + np.count_nonzero(~np.isfinite(ast1)))
+    cp.close()
+    oml.close()
+    no = na // nstr
+    # the same basins QG-only, one after the other
+    t0 = time.perf_counter()
+    o.run(wo.tdt, no, 0, 25)
     o.sync()
     t_o = time.perf_counter() - t0
     t0 = time.perf_counter()
-    a.run(clk.tdta, na, 0, 100)
+    a.run(wa.tdt, na, 0, 100)
     a.sync()
     t_a = time.perf_counter() - t0
     o.close()
     a.close()
-    return {"workload": f"coupled_ocean385_atmos97x49_qg_only_{days:g}d", "ocean_steps": no, "atmos_steps": na,
-            "value": round(days / t_cpl, 2), "unit": "simulated days/s", "s": round(t_cpl, 4),
-            "ocean_alone_s": round(t_o, 4), "atmos_alone_s": round(t_a, 4),
-            "vs_basins_one_after_other": round((t_o + t_a) / t_cpl, 3)}
+    return {"workload": f"coupled_ocean385_oml100m_atmos97x49_aml1000m_{days:g}d", "ocean_steps": no,
+            "atmos_steps": na, "value": round(days / t_cpl, 2), "unit": "simulated days/s", "s": round(t_cpl, 4),
+            "nonfinite_after": bad, "sst_range": [round(float(s_o.min()), 3), round(float(s_o.max()), 3)],
+            "ast_range": [round(float(ast1.min()), 3), round(float(ast1.max()), 3)],
+            "hmixa_range": [round(float(hm1.min()), 1), round(float(hm1.max()), 1)],
+            "qg_only_ocean_s": round(t_o, 4), "qg_only_atmos_s": round(t_a, 4),
+            "qg_only_one_after_other_value": round(days / (t_o + t_a), 2)}
 
 
 def adjoint_batch_bench(w, n: int, snapshots: int, members: int):

@@ -2839,10 +2839,15 @@ struct oqg_cpl {
     int ndxr = 0, nxtoar = 0, nytoar = 0, nxpoar = 0, nypoar = 0, nxaooc = 0, nyaooc = 0, nx1 = 0, ny1 = 0;
     double *ast = nullptr, *astm = nullptr, *hmixa = nullptr, *hmixam = nullptr, *anew = nullptr, *hnew = nullptr;
     double *xfa = nullptr, *fnetat = nullptr, *xc1ast = nullptr, *dtop = nullptr;
-    double *scr = nullptr, *coarse = nullptr, *st = nullptr;
+    double *scr = nullptr, *coarse = nullptr, *st = nullptr, *ocflux = nullptr;
     std::vector<void*> allocs;
     std::map<std::array<double, 10>, hipGraphExec_t> graphs;   // oqg_cpl_run
     hipEvent_t ev = nullptr;
+    // the ocean's step runs on a side stream beside the atmosphere's steps, from
+    // the heat-flux forcing of an ocean step to the next ocean step's stress
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool pending = false;   // ocean work forked and not yet joined
     std::string err;
 };
 
@@ -2965,6 +2970,7 @@ static CplArgs cpl_args(oqg_cpl* p, double tdta, int compute_ent) {
     a.st = p->st;
     a.sstm = p->om->sstm;
     a.fnetoc = p->om->fnet;
+    a.ocflux = p->ocflux;
     a.ast = p->ast;
     a.astm = p->astm;
     a.hmixa = p->hmixa;
@@ -3033,31 +3039,52 @@ static int enqueue_xforc(oqg_cpl* p) {
     return 0;
 }
 
-// One atmospheric step nt of dynamic_step (driver.F90:36-87).
-static int enqueue_dynamic(oqg_cpl* p, int nt, double tdto, double tdta, int nstr, int osupp, int asupp, int ent_o,
-                           int ent_a) {
+// Wait, on the main (ocean basin's) stream, for the forked ocean work.
+static int cpl_join_side(oqg_cpl* p, hipStream_t main) {
+    if (!p->pending) return 0;
+    CPLCHK(p, hipStreamWaitEvent(main, p->ev_join, 0));
+    p->pending = false;
+    return 0;
+}
+
+// One atmospheric step nt of dynamic_step (driver.F90:36-87) on stream `main`.
+// On an ocean step the exchange (stress, heat flux) runs on `main`, then the
+// ocean's step (mixed layer, QG, its suppression) forks onto the side stream:
+// nothing the atmosphere does before the next ocean step reads or writes what
+// the ocean's step does, and the next ocean step's exchange joins it first.
+static int enqueue_dynamic(oqg_cpl* p, hipStream_t main, int nt, double tdto, double tdta, int nstr, int osupp,
+                           int asupp, int ent_o, int ent_a) {
     oqg_ctx *co = p->om->c, *ca = p->ca;
     const bool ocean_step = (nt - 1) % nstr == 0;
-    if (ocean_step)
+    if (ocean_step) {
+        if (int r = cpl_join_side(p, main)) return r;
         if (int r = enqueue_xforc(p)) return r;
-    // step_atm (driver.F90:112-147)
-    if (ocean_step) launch_forcing(cpl_args(p, tdta, ent_a), co->st);
-    launch_aml_step(cpl_args(p, tdta, ent_a), co->st);
+        // step_atm's compute_forcing (driver.F90:130-133)
+        launch_forcing(cpl_args(p, tdta, ent_a), main);
+        CPLCHK(p, hipGetLastError());
+        // step_ocn (driver.F90:89-110) and suppress_ocean (:160-171), forked
+        CPLCHK(p, hipEventRecord(p->ev_fork, main));
+        CPLCHK(p, hipStreamWaitEvent(p->side, p->ev_fork, 0));
+        {
+            OnStream on(co, p->side);
+            if (int r = enqueue_oml_step(p->om, tdto, ent_o)) return cpl_fail(p, r, p->om->err);
+            if (int r = enqueue_step(co, tdto)) return cpl_fail(p, r, co->err);
+            if (osupp > 0 && (nt - 1) % osupp == 0) {
+                if (int r = enqueue_suppress(co)) return cpl_fail(p, r, co->err);
+                if (int r = enqueue_oml_suppress(p->om)) return cpl_fail(p, r, p->om->err);
+            }
+        }
+        CPLCHK(p, hipEventRecord(p->ev_join, p->side));
+        p->pending = true;
+    }
+    // the rest of step_atm (driver.F90:134-146)
+    launch_aml_step(cpl_args(p, tdta, ent_a), main);
     CPLCHK(p, hipGetLastError());
     if (int r = enqueue_step(ca, tdta)) return cpl_fail(p, r, ca->err);
-    // step_ocn (driver.F90:89-110)
-    if (ocean_step) {
-        if (int r = enqueue_oml_step(p->om, tdto, ent_o)) return cpl_fail(p, r, p->om->err);
-        if (int r = enqueue_step(co, tdto)) return cpl_fail(p, r, co->err);
-    }
-    // suppress_ocean / suppress_atmos (driver.F90:149-171)
-    if (osupp > 0 && (nt - 1) % osupp == 0) {
-        if (int r = enqueue_suppress(co)) return cpl_fail(p, r, co->err);
-        if (int r = enqueue_oml_suppress(p->om)) return cpl_fail(p, r, p->om->err);
-    }
+    // suppress_atmos (driver.F90:149-158)
     if (asupp > 0 && (nt - 1) % asupp == 0) {
         if (int r = enqueue_suppress(ca)) return cpl_fail(p, r, ca->err);
-        launch_aml_suppress(cpl_args(p, tdta, 0), co->st);
+        launch_aml_suppress(cpl_args(p, tdta, 0), main);
         CPLCHK(p, hipGetLastError());
     }
     return 0;
@@ -3123,7 +3150,8 @@ int oqg_cpl_create(oqg_oml* om, oqg_ctx* ca, const oqg_cpl_desc* d, oqg_cpl** ou
     for (double** f : {&p->ast, &p->astm, &p->hmixa, &p->hmixam, &p->anew, &p->hnew, &p->xfa, &p->fnetat, &p->xc1ast})
         if (!rc) rc = cpl_alloc(p.get(), f, ta);
     if (!rc) rc = cpl_alloc(p.get(), &p->dtop, (size_t)ca->g.fs);
-    if (!rc) rc = cpl_alloc(p.get(), &p->scr, 6 * (size_t)ca->g.fs);
+    if (!rc) rc = cpl_alloc(p.get(), &p->scr, 4 * (size_t)ca->g.fs);
+    if (!rc) rc = cpl_alloc(p.get(), &p->ocflux, (size_t)co->g.pitch * (co->g.nyp - 1));
     if (!rc) rc = cpl_alloc(p.get(), &p->coarse, 8 * (size_t)p->nxpoar * p->nypoar);
     std::vector<double> st;
     bicubic_weights(p->ndxr, st);
@@ -3131,6 +3159,9 @@ int oqg_cpl_create(oqg_oml* om, oqg_ctx* ca, const oqg_cpl_desc* d, oqg_cpl** ou
     if (!rc) rc = wind_buffers(co) ? OQG_ERR_HIP : 0;
     if (!rc) rc = wind_buffers(ca) ? OQG_ERR_HIP : 0;
     if (!rc && hipEventCreateWithFlags(&p->ev, hipEventDisableTiming) != hipSuccess) rc = OQG_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) rc = OQG_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess) rc = OQG_ERR_HIP;
+    if (!rc && hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess) rc = OQG_ERR_HIP;
     hipStream_t s = co->st;
     if (!rc) rc = hipMemcpyAsync(p->st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, s) ? OQG_ERR_HIP : 0;
     const size_t wa = (size_t)nxta * sizeof(double), pa = (size_t)ca->g.pitch * sizeof(double);
@@ -3153,8 +3184,12 @@ void oqg_cpl_destroy(oqg_cpl* p) {
     if (!p) return;
     if (p->om && p->om->c && p->om->c->st) hipStreamSynchronize(p->om->c->st);
     for (auto& kv : p->graphs) hipGraphExecDestroy(kv.second);
+    if (p->side) hipStreamSynchronize(p->side);
     for (void* v : p->allocs) hipFree(v);
     if (p->ev) hipEventDestroy(p->ev);
+    if (p->ev_fork) hipEventDestroy(p->ev_fork);
+    if (p->ev_join) hipEventDestroy(p->ev_join);
+    if (p->side) hipStreamDestroy(p->side);
     delete p;
 }
 
@@ -3247,7 +3282,8 @@ int oqg_cpl_run(oqg_cpl* p, int nsteps, int step0, double tdto, double tdta, int
 static int cpl_run_steps(oqg_cpl* p, int nsteps, int step0, double tdto, double tdta, int nstr, int osupp, int asupp,
                          int flags, int ent_o, int ent_a) {
     oqg_ctx *co = p->om->c, *ca = p->ca;
-    OnStream on(ca, co->st);
+    hipStream_t main = co->st;
+    OnStream on(ca, main);
     // the cadence repeats every lcm(nstr, osupp, asupp) steps; a graph also
     // needs both basins back at their starting time-level parity
     int P = nstr;
@@ -3265,10 +3301,15 @@ static int cpl_run_steps(oqg_cpl* p, int nsteps, int step0, double tdto, double 
             if (it == p->graphs.end()) {
                 const int c0 = co->cur, a0 = ca->cur;
                 hipGraph_t gr;
-                CPLCHK(p, hipStreamBeginCapture(co->st, hipStreamCaptureModeThreadLocal));
+                // work forked before the capture is joined outside it
+                if (int r = cpl_join_side(p, main)) return r;
+                CPLCHK(p, hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
                 int rc = 0;
-                for (int t = 0; t < P && !rc; t++) rc = enqueue_dynamic(p, nt + t, tdto, tdta, nstr, osupp, asupp, ent_o, ent_a);
-                const hipError_t ce = hipStreamEndCapture(co->st, &gr);
+                for (int t = 0; t < P && !rc; t++)
+                    rc = enqueue_dynamic(p, main, nt + t, tdto, tdta, nstr, osupp, asupp, ent_o, ent_a);
+                if (!rc) rc = cpl_join_side(p, main);   // every forked stream rejoins before the capture ends
+                p->pending = false;
+                const hipError_t ce = hipStreamEndCapture(main, &gr);
                 co->cur = c0;
                 ca->cur = a0;
                 if (rc) {
@@ -3282,12 +3323,13 @@ static int cpl_run_steps(oqg_cpl* p, int nsteps, int step0, double tdto, double 
                 CPLCHK(p, ie);
                 it = p->graphs.emplace(key, ex).first;
             }
-            CPLCHK(p, hipGraphLaunch(it->second, co->st));
+            if (int r = cpl_join_side(p, main)) return r;
+            CPLCHK(p, hipGraphLaunch(it->second, main));
             nt += P;
         } else {
-            if (int r = enqueue_dynamic(p, nt, tdto, tdta, nstr, osupp, asupp, ent_o, ent_a)) return r;
+            if (int r = enqueue_dynamic(p, main, nt, tdto, tdta, nstr, osupp, asupp, ent_o, ent_a)) return r;
             nt++;
         }
     }
-    return 0;
+    return cpl_join_side(p, main);
 }

@@ -1,10 +1,10 @@
 // oqg_couple.hip -- the ocean-atmosphere exchange of dynamic_step
 // (driver.F90:36-87) on the device:
-//   k_xforc_atm   one block over the atmosphere's p grid: the atmosphere's
-//                 geostrophic velocity and its derivatives from layer 1 of
-//                 its previous pressure, its own stress (windstress.F90:129-145)
-//                 and the bicubic corner data over the ocean
-//                 (atmos_geos_vel, windstress.F90:201-257);
+//   k_xforc_atm   one thread per atmosphere p point: the first and second
+//                 derivatives of layer 1 of its previous pressure and its own
+//                 stress (windstress.F90:129-145);
+//   k_xforc_corner one thread per p point over the ocean: the bicubic corner
+//                 data (atmos_geos_vel, windstress.F90:201-257);
 //   k_xforc_ocn   one thread per ocean p point: the bicubic velocity
 //                 (bicubic.F90:26-95), the ocean velocity subtracted when the
 //                 stress depends on the velocity difference (:259-275), the
@@ -14,16 +14,15 @@
 //                 temperature (numerics.F90:528-589) and the ocean's heat flux
 //                 (forcing.F90:43-72);
 //   k_force_atm   one thread per atmosphere T point: its heat flux
-//                 (forcing.F90:52-97), the flux from the ocean cells under it
-//                 gathered in the reference's loop order;
+//                 (forcing.F90:52-97), the flux k_force_ocn left for each
+//                 ocean cell under it gathered in the reference's loop order;
 //   k_aml_step    one thread per atmosphere T point: step_aml's two adv_diff
 //                 terms and compute_new_aml (amlsubs.F90:207-287);
 //   k_aml_finish  one thread per atmosphere p point: the entrainment
 //                 (amlsubs.F90:289-326) into the basin's resident forcing and
 //                 the time levels shifted by copies.
-// The atmosphere grid is small (about 5000 p points), so its derivative chain
-// runs in one block with the intermediate fields in a global scratch area,
-// its phases separated by block barriers.
+// k_aml_step stages its tile's fields in LDS: each T point reads about forty
+// neighbours across its two adv_diff terms.
 
 #include "oqg_mixed_dev.h"
 
@@ -122,62 +121,59 @@ __device__ __forceinline__ double fsprim_row(const CplArgs& a, const Geo& g, int
 
 }  // namespace
 
-__global__ __launch_bounds__(1024) void k_xforc_atm(CplArgs a) {
+// The atmosphere's first derivatives and its own stress (windstress.F90:129-145).
+__global__ __launch_bounds__(256) void k_xforc_atm(CplArgs a) {
     const Geo& g = a.ga;
-    const int nxp = g.nxp, nyp = g.nyp, n = nxp * nyp;
+    const int nxp = g.nxp, nyp = g.nyp;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nxp * nyp) return;
     const long long P = g.pitch, fs = g.fs;
-    double *dpdx = a.scr, *dpdy = a.scr + fs, *dxx = a.scr + 2 * fs, *dyy = a.scr + 3 * fs, *dxy = a.scr + 4 * fs,
-           *dxyy = a.scr + 5 * fs;
+    const int j = t / nxp, i = t - j * nxp;
+    const long long o = (long long)j * P + i;
     const PF pm{a.pma, P};
     const double fa = g.fnot, al = a.bcco_a;
     const double cdhfaa = (a.cdat / fa) / a.hma;
     const double qpreaa = 1.0 / (fabs(cdhfaa) * sqrt(2.0));
-    // phase 1: first derivatives and the atmosphere's own stress (windstress.F90:129-145)
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const int j = t / nxp, i = t - j * nxp;
-        const long long o = (long long)j * P + i;
-        const double px = ddx_2bc(pm, g, al, i, j), py = ddy_2bc(pm, g, al, i, j);
-        dpdx[o] = px;
-        dpdy[o] = py;
-        dxx[o] = dxx_2bc(pm, g, al, i, j);
-        dyy[o] = dyy_2bc(pm, g, al, i, j);
-        const double u1 = -py / fa, v1 = px / fa;
-        const double u2 = 4.0 * cdhfaa * cdhfaa * (u1 * u1 + v1 * v1);
-        const double shr = qpreaa * sqrt(-1.0 + sqrt(1.0 + u2));
-        const double cuohf = cdhfaa * shr;
-        const double cdochi = a.cdat * shr / (1.0 + cuohf * cuohf);
-        a.tauxa[o] = cdochi * (u1 - cuohf * v1);
-        a.tauya[o] = cdochi * (v1 + cuohf * u1);
-    }
-    __syncthreads();
-    // phase 2: the x derivatives of dpdy and dyy
-    const PF fy{dpdy, P}, fyy{dyy, P};
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const int j = t / nxp, i = t - j * nxp;
-        const long long o = (long long)j * P + i;
-        dxy[o] = ddx_2bc(fy, g, al, i, j);
-        dxyy[o] = ddx_2bc(fyy, g, al, i, j);
-    }
-    __syncthreads();
-    // phase 3: the bicubic corner data on the (nxpoar, nypoar) points over the ocean
-    const PF fxy{dxy, P};
+    const double px = ddx_2bc(pm, g, al, i, j), py = ddy_2bc(pm, g, al, i, j);
+    a.scr[o] = px;
+    a.scr[fs + o] = py;
+    a.scr[2 * fs + o] = dxx_2bc(pm, g, al, i, j);
+    a.scr[3 * fs + o] = dyy_2bc(pm, g, al, i, j);
+    const double u1 = -py / fa, v1 = px / fa;
+    const double u2 = 4.0 * cdhfaa * cdhfaa * (u1 * u1 + v1 * v1);
+    const double shr = qpreaa * sqrt(-1.0 + sqrt(1.0 + u2));
+    const double cuohf = cdhfaa * shr;
+    const double cdochi = a.cdat * shr / (1.0 + cuohf * cuohf);
+    a.tauxa[o] = cdochi * (u1 - cuohf * v1);
+    a.tauya[o] = cdochi * (v1 + cuohf * u1);
+}
+
+// The bicubic corner data on the (nxpoar, nypoar) p points over the ocean
+// (atmos_geos_vel, windstress.F90:236-246): the mixed and third derivatives are
+// x derivatives of dpdy and dP2dy2_bc taken here from the first pass's fields.
+__global__ __launch_bounds__(256) void k_xforc_corner(CplArgs a) {
+    const Geo& g = a.ga;
     const int nc = a.nxpoar * a.nypoar;
-    const double dxa = g.dx, dya = g.dy;
-    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
-        const int jc = t / a.nxpoar, ic = t - jc * a.nxpoar;
-        const int i = a.nx1 - 1 + ic, j = a.ny1 - 1 + jc;
-        const long long o = (long long)j * P + i;
-        const double dxxy = ddx_2bc(fxy, g, al, i, j);
-        double* c = a.coarse + t;
-        c[0 * nc] = -dpdy[o] / fa;
-        c[1 * nc] = -(dxa / fa) * dxy[o];
-        c[2 * nc] = -(dya / fa) * dyy[o];
-        c[3 * nc] = -(dxa * dya / fa) * dxyy[o];
-        c[4 * nc] = dpdx[o] / fa;
-        c[5 * nc] = (dxa / fa) * dxx[o];
-        c[6 * nc] = (dya / fa) * dxy[o];
-        c[7 * nc] = (dxa * dya / fa) * dxxy;
-    }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc) return;
+    const long long P = g.pitch, fs = g.fs;
+    const PF dpdx{a.scr, P}, dpdy{a.scr + fs, P}, dxx{a.scr + 2 * fs, P}, dyy{a.scr + 3 * fs, P};
+    const double fa = g.fnot, al = a.bcco_a, dxa = g.dx, dya = g.dy;
+    const int jc = t / a.nxpoar, ic = t - jc * a.nxpoar;
+    const int i = a.nx1 - 1 + ic, j = a.ny1 - 1 + jc;
+    auto dxy_at = [&](int ii, int jj) { return ddx_2bc(dpdy, g, al, ii, jj); };   // dp2_dxy
+    const double dxy = dxy_at(i, j);
+    const double dxxy = ddx_2bc(dxy_at, g, al, i, j);                             // dp3_dxxy
+    const double dxyy = ddx_2bc(dyy, g, al, i, j);                                // dp3_dxyy
+    double* c = a.coarse + t;
+    c[0 * nc] = -dpdy(i, j) / fa;
+    c[1 * nc] = -(dxa / fa) * dxy;
+    c[2 * nc] = -(dya / fa) * dyy(i, j);
+    c[3 * nc] = -(dxa * dya / fa) * dxyy;
+    c[4 * nc] = dpdx(i, j) / fa;
+    c[5 * nc] = (dxa / fa) * dxx(i, j);
+    c[6 * nc] = (dya / fa) * dxy;
+    c[7 * nc] = (dxa * dya / fa) * dxxy;
 }
 
 __global__ __launch_bounds__(256) void k_xforc_ocn(CplArgs a) {
@@ -236,6 +232,7 @@ __global__ __launch_bounds__(256) void k_force_ocn(CplArgs a) {
     const double up = a.D0up * s, sl = a.xlamda * (s - asto);
     const double fsp = fsprim_row(a, g, jo);
     a.fnetoc[o] = -fsp - a.Dmdown * asto - up - sl;
+    a.ocflux[o] = up + sl;
 }
 
 __global__ __launch_bounds__(256) void k_force_atm(CplArgs a) {
@@ -252,14 +249,12 @@ __global__ __launch_bounds__(256) void k_force_atm(CplArgs a) {
     // the ocean cells under this one, in the reference's loop order (jo outer, io inner)
     const double ocfrac = go.dx * go.dy / (g.dx * g.dy);
     const int n = a.ndxr, nxto = go.nxp - 1, nyto = go.nyp - 1;
-    if (ia1 >= 0 && ja1 >= 0) {
-        const int jo0 = ja1 * n, io0 = ia1 * n;
-        for (int jo = jo0; jo < min(jo0 + n, nyto); jo++)
-            for (int io = io0; io < min(io0 + n, nxto); io++) {
-                const double s = a.sstm[(long long)jo * go.pitch + io];
-                const double asto = bilint_at(a, a.astm, io, jo);
-                f = f + ocfrac * (a.D0up * s + a.xlamda * (s - asto));
-            }
+    if (ia1 >= 0 && ja1 >= 0) {   // k_force_ocn wrote each cell's flux
+        const int jo0 = ja1 * n, io0 = ia1 * n, jo1 = min(jo0 + n, nyto), io1 = min(io0 + n, nxto);
+        for (int jo = jo0; jo < jo1; jo++) {
+            const double* row = a.ocflux + (long long)jo * go.pitch;
+            for (int io = io0; io < io1; io++) f = f + ocfrac * row[io];
+        }
     }
     // the thickness, topography and mixed-layer depth terms (forcing.F90:87-97)
     const double fmafac = a.Adown11, fmatop = a.Aup00 + a.Adown10, hmafac = -a.hmadmp - a.Aup00 - a.Adown10;
@@ -273,21 +268,98 @@ __global__ __launch_bounds__(256) void k_force_atm(CplArgs a) {
     a.fnetat[o] = f - me - mt + hmafac * (a.hmixam[o] - a.hma);
 }
 
+// k_aml_step's tile: 64 x 4 T points per 256-thread block, its T fields with a
+// halo of 2 (Del^4 of the previous level) and its p fields (layer-1 p, stress)
+// on the tile's p points, staged in LDS.
+constexpr int AML_TW = 64, AML_TH = 4, AML_H = 2, AML_LW = AML_TW + 2 * AML_H, AML_LH = AML_TH + 2 * AML_H;
+constexpr int AML_PW = AML_TW + 1, AML_PH = AML_TH + 1;
+
+struct TLds {   // a T field in the LDS tile, absolute 0-based indices (wrapped in a channel)
+    const double* s;
+    int i0, j0, nxt, cyclic;
+    __device__ __forceinline__ double operator()(int i, int j) const {
+        int lx = i - i0 + AML_H;
+        if (cyclic) lx = (lx < 0) ? lx + nxt : (lx >= AML_LW ? lx - nxt : lx);
+        return s[(j - j0 + AML_H) * AML_LW + lx];
+    }
+};
+
+constexpr int AML_DW = AML_TW + 2, AML_DH = AML_TH + 2;   // the Del^2 tile: halo 1
+
+struct DLds {   // Del^2 of a previous level in the LDS tile (halo 1), absolute indices
+    const double* s;
+    int i0, j0, nxt, cyclic;
+    __device__ __forceinline__ double operator()(int i, int j) const {
+        int lx = i - i0 + 1;
+        if (cyclic) lx = (lx < 0) ? lx + nxt : (lx >= AML_DW ? lx - nxt : lx);
+        return s[(j - j0 + 1) * AML_DW + lx];
+    }
+};
+
+struct PLds {   // a p field in the LDS tile
+    const double* s;
+    int i0, j0;
+    __device__ __forceinline__ double operator()(int i, int j) const { return s[(j - j0) * AML_PW + (i - i0)]; }
+};
+
 __global__ __launch_bounds__(256) void k_aml_step(CplArgs a) {
     const Geo& g = a.ga;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= g.nxp - 1 || j >= g.nyp - 1) return;
-    const long long o = (long long)j * g.pitch + i;
+    const int nxt = g.nxp - 1, nyt = g.nyp - 1;
+    const int i0 = blockIdx.x * AML_TW, j0 = blockIdx.y * AML_TH;
+    const long long P = g.pitch;
+    __shared__ double tl[4][AML_LH * AML_LW];   // ast, astm, hmixa, hmixam
+    __shared__ double pl[3][AML_PH * AML_PW];   // p1, taux, tauy
+    const double* tf[4] = {a.ast, a.astm, a.hmixa, a.hmixam};
+    for (int t = threadIdx.x; t < AML_LH * AML_LW; t += blockDim.x) {
+        const int ly = t / AML_LW, lx = t - ly * AML_LW;
+        int i = i0 - AML_H + lx;
+        const int j = j0 - AML_H + ly;
+        if (g.cyclic) i = (i % nxt + nxt) % nxt;
+        const bool in = i >= 0 && i < nxt && j >= 0 && j < nyt;
+        const long long o = (long long)j * P + i;
+#pragma unroll
+        for (int f = 0; f < 4; f++) tl[f][t] = in ? tf[f][o] : 0.0;
+    }
+    const double* pf[3] = {a.pa1, a.tauxa, a.tauya};
+    for (int t = threadIdx.x; t < AML_PH * AML_PW; t += blockDim.x) {
+        const int ly = t / AML_PW, lx = t - ly * AML_PW;
+        const int i = i0 + lx, j = j0 + ly;
+        const bool in = i < g.nxp && j < g.nyp;
+        const long long o = (long long)j * P + i;
+#pragma unroll
+        for (int f = 0; f < 3; f++) pl[f][t] = in ? pf[f][o] : 0.0;
+    }
+    __syncthreads();
     const double hm = a.hma;
+    const TLds ast{tl[0], i0, j0, nxt, g.cyclic}, astm{tl[1], i0, j0, nxt, g.cyclic};
+    const TLds hmx{tl[2], i0, j0, nxt, g.cyclic}, hmxm{tl[3], i0, j0, nxt, g.cyclic};
     // init_atmos_ml (amlsubs.F90:84-87): ast with no flux walls, hmixa with flux walls holding hm
-    const AdvField fa{a.ast, a.astm, a.at2d, a.at4d, 1.0, 0.0, 0.0, 0, 0};
-    const AdvField fh{a.hmixa, a.hmixam, a.ahmd, 0.0, 1.0, hm, hm, 1, 1};
-    const double tm = adv_diff_at(g, fa, a.pa1, a.tauxa, a.tauya, hm, a.tau_sign_a, i, j);
-    const double dh = adv_diff_at(g, fh, a.pa1, a.tauxa, a.tauya, hm, a.tau_sign_a, i, j);
-    const double wekt = wekt_at(g, a.tauxa, a.tauya, i, j);
+    const AdvField fa{nullptr, nullptr, a.at2d, a.at4d, 1.0, 0.0, 0.0, 0, 0};
+    const AdvField fh{nullptr, nullptr, a.ahmd, 0.0, 1.0, hm, hm, 1, 1};
+    // Del^2 of both previous levels on the tile and a halo of 1 (del2_T, numerics.F90:107-147)
+    __shared__ double dl[2][AML_DH * AML_DW];
+    const double fx = 1.0 / (g.dx * g.dx);
+    for (int t = threadIdx.x; t < AML_DH * AML_DW; t += blockDim.x) {
+        const int ly = t / AML_DW, lx = t - ly * AML_DW;
+        int i = i0 - 1 + lx;
+        const int j = j0 - 1 + ly;
+        if (g.cyclic) i = (i % nxt + nxt) % nxt;
+        const bool in = i >= 0 && i < nxt && j >= 0 && j < nyt;
+        dl[0][t] = in ? del2T_at(astm, i, j, nxt, nyt, g.cyclic, fx, fa.fluxn, fa.nval, fa.fluxs, fa.sval) : 0.0;
+        dl[1][t] = in ? del2T_at(hmxm, i, j, nxt, nyt, g.cyclic, fx, fh.fluxn, fh.nval, fh.fluxs, fh.sval) : 0.0;
+    }
+    __syncthreads();
+    const int i = i0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
+    if (i >= nxt || j >= nyt) return;
+    const long long o = (long long)j * P + i;
+    const PLds p1{pl[0], i0, j0}, tx{pl[1], i0, j0}, ty{pl[2], i0, j0};
+    const DLds d2a{dl[0], i0, j0, nxt, g.cyclic}, d2h{dl[1], i0, j0, nxt, g.cyclic};
+    const double tm = adv_diff_d2(g, fa, ast, d2a, p1, tx, ty, hm, a.tau_sign_a, i, j);
+    const double dh = adv_diff_d2(g, fh, hmx, d2h, p1, tx, ty, hm, a.tau_sign_a, i, j);
+    const double wekt = wekt_acc(g, tx, ty, i, j);
     // compute_new_aml (amlsubs.F90:231-287)
-    const double tdta = a.tdta, astm = a.astm[o], hmixam = a.hmixam[o];
-    const double dT = a.T1_rel - astm;
+    const double tdta = a.tdta, astm0 = astm(i, j), hmixam = hmxm(i, j);
+    const double dT = a.T1_rel - astm0;
     const double hdrcdt = a.hmadmp / a.rho_cp;
     double hnew, dT_fix;
     if (dT >= 2.0 * hdrcdt * tdta) {
@@ -302,14 +374,14 @@ __global__ __launch_bounds__(256) void k_aml_step(CplArgs a) {
         dT_fix = 0.0;
     }
     const double hmainv = 1.0 / hm, rrcpat = 1.0 / a.rho_cp;
-    const double r = tm + rrcpat * a.fnetat[o] / hmixam - hmainv * wekt * astm;
-    const double an = astm + tdta * r + dT_fix;
+    const double r = tm + rrcpat * a.fnetat[o] / hmixam - hmainv * wekt * astm0;
+    const double an = astm0 + tdta * r + dT_fix;
     const double dtc = fmin(0.0, a.T1_rel - an);
     a.anew[o] = an + dtc;
     a.hnew[o] = hnew;
     // compute_atmos_entrainment's T-point terms (amlsubs.F90:303-309)
-    a.xfa[o] = a.xcexp * a.bface * (hmixam - hm) + a.dface * (a.xcexp * astm + (1.0 - a.xcexp) * a.xc1ast[o]) +
-               (a.xcexp / (tdta * a.dT_12)) * a.hmixa[o] * dtc;
+    a.xfa[o] = a.xcexp * a.bface * (hmixam - hm) + a.dface * (a.xcexp * astm0 + (1.0 - a.xcexp) * a.xc1ast[o]) +
+               (a.xcexp / (tdta * a.dT_12)) * hmx(i, j) * dtc;
 }
 
 __global__ __launch_bounds__(256) void k_aml_finish(CplArgs a) {
@@ -351,7 +423,9 @@ __global__ __launch_bounds__(256) void k_aml_suppress(CplArgs a) {
 static dim3 tgrid(int nx, int ny) { return dim3((nx + 63) / 64, (ny + 3) / 4); }
 
 void launch_xforc(const CplArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_xforc_atm, dim3(1), dim3(1024), 0, st, a);
+    const int na = a.ga.nxp * a.ga.nyp, nc = a.nxpoar * a.nypoar;
+    hipLaunchKernelGGL(k_xforc_atm, dim3((na + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_xforc_corner, dim3((nc + 255) / 256), dim3(256), 0, st, a);
     hipLaunchKernelGGL(k_xforc_ocn, tgrid(a.go.nxp, a.go.nyp), dim3(256), 0, st, a);
 }
 
@@ -362,7 +436,8 @@ void launch_forcing(const CplArgs& a, hipStream_t st) {
 }
 
 void launch_aml_step(const CplArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_aml_step, tgrid(a.ga.nxp - 1, a.ga.nyp - 1), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_aml_step, dim3((a.ga.nxp - 1 + AML_TW - 1) / AML_TW, (a.ga.nyp - 1 + AML_TH - 1) / AML_TH),
+                       dim3(256), 0, st, a);
     const int n = a.ga.nxp * a.ga.nyp;
     hipLaunchKernelGGL(k_aml_finish, dim3((n + 255) / 256), dim3(256), 0, st, a);
 }

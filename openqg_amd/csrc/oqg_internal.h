@@ -350,11 +350,12 @@ struct CplArgs {
     const double* pma;                // atmosphere previous p, all layers
     const double* pa1;                // atmosphere current p, layer 1
     double *tauxo, *tauyo, *tauxa, *tauya;   // the basins' resident stress
-    double* scr;                      // atmosphere p scratch: dpdx, dpdy, dxx, dyy, dxy, dxyy
+    double* scr;                      // atmosphere p scratch: dpdx, dpdy, dxx, dyy
     double* coarse;                   // u, ux, uy, uxy, v, vx, vy, vxy on the (nxpoar, nypoar) points
     const double* st;                 // bicubic weights st[c][r], ((ndxr+1)^2, 16)
     const double* sstm;               // ocean mixed layer, previous level (T)
     double* fnetoc;                   // the ocean mixed layer's heat flux (T)
+    double* ocflux;                   // ocean T scratch: the flux each ocean cell gives the atmosphere
     double *ast, *astm, *hmixa, *hmixam, *anew, *hnew, *xfa, *fnetat;   // atmosphere T fields
     const double* xc1ast;             // T
     const double* dtop;               // atmosphere topography (p)
