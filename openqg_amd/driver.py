@@ -26,7 +26,7 @@ from typing import Callable, Optional
 
 import numpy as np
 
-from .state import save_qg
+from .state import save_aml, save_oml, save_qg
 
 SECDAY = 86400.0            # src/util/constants.F90:17-19
 DAYSYR = 365.0
@@ -113,7 +113,7 @@ class Basin:
 
 
 def run(clk: Clock, outdir: str, ocn: Optional[Basin] = None, atm: Optional[Basin] = None,
-        monitor: Optional[Callable[[str, int, dict], None]] = None) -> int:
+        monitor: Optional[Callable[[str, int, dict], None]] = None, coupler=None) -> int:
     """The run loop of src/openqg.F90:479-512 for QG-only basins.
 
     Writes `<name>_qg_restart.nc` every clk.noutre steps and
@@ -121,8 +121,22 @@ def run(clk: Clock, outdir: str, ocn: Optional[Basin] = None, atm: Optional[Basi
     validity check fails, writes `<name>_qg_invalid.nc` for each basin and
     raises InvalidSolution.  `monitor(name, nt, diag)` receives `QG.diagnose`
     every basin.nocmon steps (qg_monitor_step's cadence).  Returns the last
-    step number."""
+    step number.
+
+    With `coupler` (an openqg_amd.coupled.Coupler of ocn and atm) the two basins
+    advance together through dynamic_step (src/driver.F90:36-87) with both
+    mixed layers, and the restart files include the mixed layers'
+    `<basin>_ml_*.nc` (save_all, src/openqg.F90:518-543)."""
     basins = [b for b in (atm, ocn) if b is not None]
+    if coupler is not None and (ocn is None or atm is None):
+        raise ValueError("a coupled run needs both basins")
+
+    def save_all(suffix):
+        for b in basins:
+            save_qg(outdir, f"{b.name}_qg_{suffix}", b.qg)
+        if coupler is not None:
+            save_oml(outdir, f"{ocn.name}_ml_{suffix}", coupler.oml)
+            save_aml(outdir, f"{atm.name}_ml_{suffix}", coupler)
     dts = {}
     if atm is not None:
         dts[id(atm)] = clk.dta
@@ -150,8 +164,7 @@ def run(clk: Clock, outdir: str, ocn: Optional[Basin] = None, atm: Optional[Basi
                 if want_mon:
                     monitor(b.name, nt, d)
         if not solnok:
-            for b in basins:
-                save_qg(outdir, f"{b.name}_qg_invalid.nc", b.qg)
+            save_all("invalid.nc")
             raise InvalidSolution(f"invalid values at nt = {nt}, day {nt * clk.dta / SECDAY:.2f}")
 
     steps = sorted(set(c for c in cadences() if c > 0))
@@ -161,17 +174,18 @@ def run(clk: Clock, outdir: str, ocn: Optional[Basin] = None, atm: Optional[Basi
         ntdone = nt - clk.nsteps0
         nxt = min([clk.ntsrun] + [(ntdone // c + 1) * c for c in steps])
         nt1 = clk.nsteps0 + nxt
-        if atm is not None:
-            atm.qg.run(clk.tdta, nt1 - nt, nt, ATM_SUPP)
-        if ocn is not None:
-            first, count = ocean_steps(nt, nt1, clk.nstr)
-            if count:
-                ocn.qg.run(clk.tdto, count, first, OCN_SUPP)
+        if coupler is not None:
+            coupler.run(nt1 - nt, nt, clk.tdto, clk.tdta, clk.nstr, OCN_SUPP * clk.nstr, ATM_SUPP)
+        else:
+            if atm is not None:
+                atm.qg.run(clk.tdta, nt1 - nt, nt, ATM_SUPP)
+            if ocn is not None:
+                first, count = ocean_steps(nt, nt1, clk.nstr)
+                if count:
+                    ocn.qg.run(clk.tdto, count, first, OCN_SUPP)
         nt = nt1
         diagnostic_step(nt)
         if (nt - clk.nsteps0) % clk.noutre == 0:
-            for b in basins:
-                save_qg(outdir, f"{b.name}_qg_restart.nc", b.qg)
-    for b in basins:
-        save_qg(outdir, f"{b.name}_qg_lastday.nc", b.qg)
+            save_all("restart.nc")
+    save_all("lastday.nc")
     return nt

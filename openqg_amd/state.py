@@ -94,3 +94,74 @@ def restart_qg(indir: str, filename: str, qg, init: bool = True) -> None:
     qg.set_state(p=p, pm=pm)
     if init:
         qg.init_state()
+
+
+# ---------------------------------------------------------------- mixed layers
+# save_oml / save_aml (src/state.F90:114-176) and restart_oml / restart_aml
+# (:46-80): dimensions nxt, nyt, x(nxt) and y(nyt) at T points in km, and the
+# fields (nyt, nxt) in the file's C order.
+_ML_UNITS = {"sst": "Current mixed layer temp", "sstm": "Previous mixed layer temp",
+             "ast": "Current mixed layer temp", "astm": "Previous mixed layer temp",
+             "hmixa": "Current mixed layer height", "hmixam": "Previous mixed layer height"}
+
+
+def write_ml_file(path: str, box, fields: dict) -> None:
+    shape = (box.nyt, box.nxt)
+    with netcdf_file(path, "w", version=1) as f:
+        f.createDimension("nxt", box.nxt)
+        f.createDimension("nyt", box.nyt)
+        xt = box.xp[:-1] + 0.5 * box.dx
+        yt = box.yp[:-1] + 0.5 * box.dy
+        for name, dim, vals in (("x", "nxt", xt), ("y", "nyt", yt)):
+            v = f.createVariable(name, "d", (dim,))
+            v.units = "km"
+            v[:] = 1.0e-3 * vals
+        for name, a in fields.items():
+            a = np.asarray(a, dtype=np.float64)
+            if a.shape != shape:
+                raise ValueError(f"{name}: shape {a.shape} != {shape}")
+            v = f.createVariable(name, "d", ("nyt", "nxt"))
+            v.units = _ML_UNITS[name]
+            v[:] = a
+
+
+def read_ml_file(path: str, box, names):
+    shape = (box.nyt, box.nxt)
+    out = []
+    with netcdf_file(path, "r", mmap=False) as f:
+        for name in names:
+            if name not in f.variables:
+                raise ValueError(f"{path}: no variable '{name}'")
+            v = f.variables[name]
+            if tuple(v.shape) != shape:
+                raise ValueError(f"{path}: '{name}' has shape {tuple(v.shape)}, the mixed layer needs {shape}")
+            out.append(np.array(v[:], dtype=np.float64))
+    return out
+
+
+def save_oml(outdir: str, filename: str, oml) -> str:
+    """save_oml (src/state.F90:114-143) of an OceanMixedLayer."""
+    sst, sstm = oml.get_state()
+    path = os.path.join(outdir, filename)
+    write_ml_file(path, oml.qg.b, {"sst": sst, "sstm": sstm})
+    return path
+
+
+def restart_oml(indir: str, filename: str, oml) -> None:
+    """restart_oml (src/state.F90:46-61)."""
+    sst, sstm = read_ml_file(os.path.join(indir, filename), oml.qg.b, ("sst", "sstm"))
+    oml.set_state(sst, sstm)
+
+
+def save_aml(outdir: str, filename: str, cpl) -> str:
+    """save_aml (src/state.F90:145-176) of the atmospheric mixed layer a Coupler holds."""
+    ast, astm, hmixa, hmixam = cpl.get_aml_state()
+    path = os.path.join(outdir, filename)
+    write_ml_file(path, cpl.atm.b, {"ast": ast, "astm": astm, "hmixa": hmixa, "hmixam": hmixam})
+    return path
+
+
+def restart_aml(indir: str, filename: str, cpl) -> None:
+    """restart_aml (src/state.F90:63-80)."""
+    f = read_ml_file(os.path.join(indir, filename), cpl.atm.b, ("ast", "astm", "hmixa", "hmixam"))
+    cpl.set_aml_state(*f)
