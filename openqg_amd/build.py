@@ -16,6 +16,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "liboqg.so")
 SOURCES = ["oqg_stencil.hip", "oqg_transform.hip", "oqg_adjoint.hip", "oqg_forcing.hip", "oqg_diag.hip",
            "oqg_mixed.hip", "oqg_couple.hip", "oqg_api.hip"]
+WAVE_SOURCE = "oqg_wave.hip"
+WAVE_LAYERS = range(2, 9)   # OQG_MAX_LAYERS
 ARCH = os.environ.get("OQG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -53,9 +55,12 @@ def build(verbose: bool = False) -> str:
 def _build_in(objdir: str, verbose: bool) -> str:
     objs = []
     procs = []
-    for src in SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", "-c", os.path.join(CSRC, src), "-o", obj]
+    units = [(src, src.replace(".hip", ".o"), []) for src in SOURCES]
+    # the wavefront-transform kernels, one object per layer count (compiled in parallel)
+    units += [(WAVE_SOURCE, f"oqg_wave_nl{nl}.o", [f"-DOQG_WAVE_NL={nl}"]) for nl in WAVE_LAYERS]
+    for src, oname, extra in units:
+        obj = os.path.join(objdir, oname)
+        cmd = [hipcc(), *HIPCC_FLAGS, *extra, f"-I{INCLUDE}", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -22,6 +23,39 @@ using namespace oqg;
 namespace {
 
 thread_local std::string g_create_err;
+
+// Streams are created once per process and reused: a basin, an ensemble or a
+// coupler takes one from its device's pool and gives it back, synchronised,
+// when it is destroyed.  No stream is destroyed while the process runs, so the
+// hardware queues behind the library's streams keep their lifetimes (two runs
+// under rocprofv3's kernel trace that aborted with a malformed dispatch packet
+// had both destroyed a stream of the same process earlier).
+std::mutex g_pool_mu;
+std::map<int, std::vector<hipStream_t>> g_stream_pool;
+
+hipError_t pool_take(hipStream_t* out) {
+    int dev = 0;
+    if (hipError_t e = hipGetDevice(&dev); e != hipSuccess) return e;
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto& v = g_stream_pool[dev];
+        if (!v.empty()) {
+            *out = v.back();
+            v.pop_back();
+            return hipSuccess;
+        }
+    }
+    return hipStreamCreateWithFlags(out, hipStreamNonBlocking);
+}
+
+// dev: the device the stream was taken on.
+void pool_give(int dev, hipStream_t s) {
+    if (!s) return;
+    (void)hipStreamSynchronize(s);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_stream_pool[dev].push_back(s);
+}
+
 constexpr double TWOPI_REF = 2.0 * 3.14159265358979324;   // constants.F90: TWOPI
 
 // Graph kinds: plain steps (oqg_run), steps with snapshot saves
@@ -398,6 +432,7 @@ static void fill_inv_common(oqg_ctx* c, InvArgs& ia) {
     ia.dc = c->dconst;
     ia.spec = c->spec;
     ia.tw = c->itw;
+    ia.twf = c->tw;
     ia.colsum = c->colsum;
     ia.colw = c->colw;
     ia.entpart = c->entpart;
@@ -868,8 +903,7 @@ int oqg_create(const oqg_desc* desc, oqg_ctx** out) {
     c->d.amat = c->d.ctl2m = c->d.ctm2l = c->d.rdm2 = nullptr;
     c->d.hom_sol_bc = c->d.hom_bt = c->d.hom_bc1 = c->d.hom_bc2 = c->d.hom_coef = nullptr;
     c->variant = (d.flags & OQG_FLAG_YSOLVE_THOMAS) ? 1 : 0;
-    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess)
-        return fail(nullptr, OQG_ERR_HIP, "hipStreamCreate failed");
+    if (pool_take(&c->own) != hipSuccess) return fail(nullptr, OQG_ERR_HIP, "hipStreamCreate failed");
     c->st = c->own;
     // setup() may read the caller's optional arrays through a temporary copy of desc
     oqg_desc keep = c->d;
@@ -891,7 +925,7 @@ void oqg_destroy(oqg_ctx* c) {
     for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (void* p : c->allocs) hipFree(p);
     if (c->hst) hipHostFree(c->hst);
-    if (c->own) hipStreamDestroy(c->own);
+    pool_give(c->dev, c->own);
     delete c;
 }
 
@@ -2180,11 +2214,12 @@ int oqg_adj_revolve_reverse(oqg_ctx* c, long long* forward_steps) {
 
 namespace {
 
-enum { ENS_VORT = 0, ENS_FWD, ENS_YS, ENS_INV };
+enum { ENS_VORT = 0, ENS_FWD, ENS_YS, ENS_INV, ENS_FWDW, ENS_INVW };
 
 struct EnsLaunch {
     int kind = ENS_VORT;
-    int nl = 0, mb = 0, R = 0;
+    int nl = 0, mb = 0, R = 0;   // mb: butterflies per thread, or points per lane (wavefront forms)
+    bool cyc = false;            // wavefront inverse form: channel members
     size_t lds = 0;
     int nblocks = 0;
     int2* map = nullptr;
@@ -2207,6 +2242,7 @@ struct EnsPlan {
 struct oqg_ens {
     std::vector<oqg_ctx*> m;
     hipStream_t st = nullptr;
+    int dev = 0;
     std::string err;
     int schedule = 0;          // OQG_ENS_* bits
     std::vector<double> tdt;   // per member, baked into the plans' argument tables
@@ -2322,29 +2358,33 @@ static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::ve
         L.kind = ENS_VORT;
         if (int r = add_launch(L, map)) return r;
     }
-    std::map<std::pair<int, int>, std::pair<std::vector<int2>, size_t>> fwd, inv;
+    // x-transform classes: (wavefront form?, layer count, points per lane or butterflies per thread)
+    std::map<std::array<int, 4>, std::pair<std::vector<int2>, size_t>> fwd, inv;
     std::map<int, std::pair<std::vector<int2>, size_t>> ys;
     int rmax = 1;
     for (int i = 0; i < n; i++) rmax = std::max(rmax, e->m[P->members[i]]->s.R);
     for (int i = 0; i < n; i++) {
         const oqg_ctx* c = e->m[P->members[i]];
-        const std::pair<int, int> key(c->g.nl, fft_mb(c->s, false)), ikey(c->g.nl, fft_mb(c->s, true));
+        const bool w = fftw_fits(c->s, c->g.nl);
+        const std::array<int, 4> key{w ? 1 : 0, c->g.nl, w ? fftw_points(c->s) : fft_mb(c->s, false), 0};
+        const std::array<int, 4> ikey{w ? 1 : 0, c->g.nl, w ? fftw_points(c->s) : fft_mb(c->s, true),
+                                      w ? c->g.cyclic : 0};
         auto& f = fwd[key];
         const int np = inv_block_count(c->s);
-        for (int b = 0; b < np * c->g.nl; b++) f.first.push_back(make_int2(i, b));
-        f.second = std::max(f.second, fwd_lds_bytes(c->s));
+        for (int b = 0; b < np * (w ? 1 : c->g.nl); b++) f.first.push_back(make_int2(i, b));
+        f.second = std::max(f.second, w ? fftw_lds_bytes(c->s, c->g.nl) : fwd_lds_bytes(c->s));
         auto& v = inv[ikey];
         for (int b = 0; b < np; b++) v.first.push_back(make_int2(i, b));
-        v.second = std::max(v.second, inv_lds_bytes(c->s, c->g, c->g.nl));
+        v.second = std::max(v.second, w ? fftw_lds_bytes(c->s, c->g.nl) : inv_lds_bytes(c->s, c->g, c->g.nl));
         auto& y = ys[(e->schedule & OQG_ENS_ONE_YSOLVE) ? rmax : c->s.R];
         for (int b = 0; b < ysolve_blocks(c->s, c->g.nl); b++) y.first.push_back(make_int2(i, b));
         y.second = std::max(y.second, ysolve_lds_bytes(c->s));
     }
     for (auto& kv : fwd) {
         EnsLaunch L;
-        L.kind = ENS_FWD;
-        L.nl = kv.first.first;
-        L.mb = kv.first.second;
+        L.kind = kv.first[0] ? ENS_FWDW : ENS_FWD;
+        L.nl = kv.first[1];
+        L.mb = kv.first[2];
         L.lds = kv.second.second;
         if (int r = add_launch(L, kv.second.first)) return r;
     }
@@ -2357,9 +2397,10 @@ static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::ve
     }
     for (auto& kv : inv) {
         EnsLaunch L;
-        L.kind = ENS_INV;
-        L.nl = kv.first.first;
-        L.mb = kv.first.second;
+        L.kind = kv.first[0] ? ENS_INVW : ENS_INV;
+        L.nl = kv.first[1];
+        L.mb = kv.first[2];
+        L.cyc = kv.first[3] != 0;
         L.lds = kv.second.second;
         if (int r = add_launch(L, kv.second.first)) return r;
     }
@@ -2376,6 +2417,8 @@ static int ens_enqueue_step(oqg_ens* e, EnsPlan* P, int par, bool supp) {
                 launch_vort_ens(P->va[par], L.map, L.nblocks, e->st);
                 break;
             case ENS_FWD: launch_fwd_ens(P->fa[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;
+            case ENS_FWDW: launch_fwdw_ens(P->fa[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;
+            case ENS_INVW: launch_invw_ens(P->ia[par], L.map, L.nblocks, L.nl, L.mb, L.cyc, L.lds, e->st); break;
             case ENS_YS: launch_ysolve_ens(P->ya, L.map, L.nblocks, L.R, L.lds, e->st); break;
             default: launch_inv_ens(P->ia[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;
         }
@@ -2474,8 +2517,8 @@ int oqg_ens_create(oqg_ctx* const* members, int n, oqg_ens** out) {
     }
     std::unique_ptr<oqg_ens> e(new oqg_ens);
     e->m.assign(members, members + n);
-    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess)
-        return fail(nullptr, OQG_ERR_HIP, "hipStreamCreate failed");
+    e->dev = members[0]->dev;
+    if (pool_take(&e->st) != hipSuccess) return fail(nullptr, OQG_ERR_HIP, "hipStreamCreate failed");
     *out = e.release();
     return 0;
 }
@@ -2483,7 +2526,7 @@ int oqg_ens_create(oqg_ctx* const* members, int n, oqg_ens** out) {
 void oqg_ens_destroy(oqg_ens* e) {
     if (!e) return;
     ens_clear_plans(e);
-    if (e->st) hipStreamDestroy(e->st);
+    pool_give(e->dev, e->st);
     delete e;
 }
 
@@ -2846,6 +2889,7 @@ struct oqg_cpl {
     // the ocean's step runs on a side stream beside the atmosphere's steps, from
     // the heat-flux forcing of an ocean step to the next ocean step's stress
     hipStream_t side = nullptr;
+    int dev = 0;   // the device of the side stream
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool pending = false;   // ocean work forked and not yet joined
     std::string err;
@@ -3161,7 +3205,8 @@ int oqg_cpl_create(oqg_oml* om, oqg_ctx* ca, const oqg_cpl_desc* d, oqg_cpl** ou
     if (!rc && hipEventCreateWithFlags(&p->ev, hipEventDisableTiming) != hipSuccess) rc = OQG_ERR_HIP;
     if (!rc && hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess) rc = OQG_ERR_HIP;
     if (!rc && hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess) rc = OQG_ERR_HIP;
-    if (!rc && hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess) rc = OQG_ERR_HIP;
+    p->dev = co->dev;
+    if (!rc && pool_take(&p->side) != hipSuccess) rc = OQG_ERR_HIP;
     hipStream_t s = co->st;
     if (!rc) rc = hipMemcpyAsync(p->st, st.data(), st.size() * sizeof(double), hipMemcpyHostToDevice, s) ? OQG_ERR_HIP : 0;
     const size_t wa = (size_t)nxta * sizeof(double), pa = (size_t)ca->g.pitch * sizeof(double);
@@ -3189,7 +3234,7 @@ void oqg_cpl_destroy(oqg_cpl* p) {
     if (p->ev) hipEventDestroy(p->ev);
     if (p->ev_fork) hipEventDestroy(p->ev_fork);
     if (p->ev_join) hipEventDestroy(p->ev_join);
-    if (p->side) hipStreamDestroy(p->side);
+    pool_give(p->dev, p->side);
     delete p;
 }
 

@@ -443,6 +443,198 @@ __device__ __forceinline__ void fft_lds(double2* buf, int L, int nst, const int*
     }
 }
 
+// ---------------------------------------------------------------- FFT per wavefront
+// The same Stockham stages, twiddle layout and buffer padding as fft_lds, with
+// one wavefront per transform: lane l owns the butterflies t = l + 64 b of
+// every stage, b < ceil(P / R) for a plan whose transform has at most 64 P
+// points.  A wavefront's LDS reads and writes are performed in the order it
+// issues them, so a compiler-level fence separates a stage's reads from its
+// writes and its writes from the next stage's reads: the transforms of the
+// wavefronts of a block need no block barrier, and a block may give other
+// wavefronts other work meanwhile.
+
+// a*b with fused multiply-adds (two roundings per component): the wavefront
+// transforms' twiddle products, where it saves a third of the instructions.
+__device__ __forceinline__ double2 cmul_fma(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int R, int P, int SIGN>
+__device__ __forceinline__ void fftw_stage(double2* buf, int L, int Ns, const double2* tws, const double2* roots,
+                                           int lane) {
+    constexpr int MB = (P + R - 1) / R;
+    // The stage's addresses are formed here from an opaque copy of L: hoisted
+    // out of the stage loop for every radix the plan might use, they held
+    // about a hundred registers and the stages' LDS reads were then issued one
+    // at a time.
+    int Lx = L;
+    __asm__ volatile("" : "+s"(Lx));
+    const int LR = Lx / R;
+    const double rNs = 1.0 / (double)Ns;   // t < 2^13: (t + 1/2)/Ns rounds to the right floor
+    double2 v[MB][R], w[MB][R > 1 ? R - 1 : 1];
+    int kk[MB], jj[MB];
+    const bool twiddled = Ns > 1;   // the first stage's twiddles are all exactly 1
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int t = lane + 64 * b;
+        jj[b] = (int)(((double)t + 0.5) * rNs);
+        kk[b] = t - jj[b] * Ns;
+        if (t < LR) {
+            if (twiddled)
+#pragma unroll
+                for (int r = 1; r < R; r++) w[b][r - 1] = tws[(r - 1) * Ns + kk[b]];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = buf[zpad(t + r * LR)];
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int t = lane + 64 * b;
+        if (t < LR) {
+            if (twiddled)
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2 wr = w[b][r - 1];
+                    if (SIGN > 0) wr.y = -wr.y;
+                    v[b][r] = cmul_fma(v[b][r], wr);
+                }
+            dft_small<R, SIGN>(v[b], roots, R);
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int t = lane + 64 * b;
+        if (t < LR) {
+            const int base = jj[b] * Ns * R + kk[b];
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[zpad(base + r * Ns)] = v[b][r];
+        }
+    }
+    wave_lds_sync();
+}
+
+// In-place transform of buf (length L <= 64 P, stored through zpad) by the
+// calling wavefront, with the LDS copy tw of the plan's twiddle table.
+template <int P, int SIGN>
+__device__ __forceinline__ void fft_wave(double2* buf, int L, int nst, const int* radix, const double2* tw, int lane) {
+    int Ns = 1;
+    const double2* tws = tw;
+    for (int st = 0; st < nst; st++) {
+        const int R = radix[st];
+        switch (R) {
+            case 16: fftw_stage<16, P, SIGN>(buf, L, Ns, tws, nullptr, lane); break;
+            case 8: fftw_stage<8, P, SIGN>(buf, L, Ns, tws, nullptr, lane); break;
+            case 4: fftw_stage<4, P, SIGN>(buf, L, Ns, tws, nullptr, lane); break;
+            case 2: fftw_stage<2, P, SIGN>(buf, L, Ns, tws, nullptr, lane); break;
+            case 3: fftw_stage<3, P, SIGN>(buf, L, Ns, tws, tw + twiddle_roots3(L), lane); break;
+            default: fftw_stage<5, P, SIGN>(buf, L, Ns, tws, tw + twiddle_roots5(L), lane); break;
+        }
+        tws += (R - 1) * Ns;
+        Ns *= R;
+    }
+}
+
+// The same transform with the plan fixed at compile time, for the lengths of
+// the usual basins: every stride, twiddle offset and butterfly count is a
+// constant (LDS offsets fold into the instructions), so a stage costs about
+// two thirds of the run-time plan's.  The plan is the one the library builds
+// on the host (radices 16, 8, 4, 2, 3, 5 taken greedily), so the stage-ordered
+// twiddle table is the same.
+struct FixedPlan {
+    int n;
+    int r[16];
+};
+constexpr FixedPlan fixed_plan(int L) {
+    FixedPlan p{0, {}};
+    int m = L;
+    const int rs[6] = {16, 8, 4, 2, 3, 5};
+    for (int q = 0; q < 6; q++)
+        while (m % rs[q] == 0) {
+            p.r[p.n++] = rs[q];
+            m /= rs[q];
+        }
+    return p;
+}
+
+template <int R, int L, int NS, int SIGN>
+__device__ __forceinline__ void fftf_stage(double2* buf, const double2* tws, const double2* roots, int lane) {
+    constexpr int LR = L / R, MB = (LR + 63) / 64;
+    double2 v[MB][R], w[MB][R > 1 ? R - 1 : 1];
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int t = lane + 64 * b;
+        if (LR % 64 == 0 || t < LR) {
+            const int k = t % NS;
+            if constexpr (NS > 1)
+#pragma unroll
+                for (int r = 1; r < R; r++) w[b][r - 1] = tws[(r - 1) * NS + k];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = buf[zpad(t + r * LR)];
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int t = lane + 64 * b;
+        if (LR % 64 == 0 || t < LR) {
+            if constexpr (NS > 1)
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2 wr = w[b][r - 1];
+                    if (SIGN > 0) wr.y = -wr.y;
+                    v[b][r] = cmul_fma(v[b][r], wr);
+                }
+            dft_small<R, SIGN>(v[b], roots, R);
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int b = 0; b < MB; b++) {
+        const int t = lane + 64 * b;
+        if (LR % 64 == 0 || t < LR) {
+            const int base = (t / NS) * NS * R + t % NS;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[zpad(base + r * NS)] = v[b][r];
+        }
+    }
+    wave_lds_sync();
+}
+
+template <int L, int SIGN, int ST = 0, int NS = 1, int OFF = 0>
+__device__ __forceinline__ void fft_wave_fixed(double2* buf, const double2* tw, int lane) {
+    constexpr FixedPlan pl = fixed_plan(L);
+    if constexpr (ST < pl.n) {
+        constexpr int R = pl.r[ST];
+        const double2* roots = (R == 3) ? tw + twiddle_roots3(L) : (R == 5 ? tw + twiddle_roots5(L) : nullptr);
+        fftf_stage<R, L, NS, SIGN>(buf, tw + OFF, roots, lane);
+        fft_wave_fixed<L, SIGN, ST + 1, NS * R, OFF + (R - 1) * NS>(buf, tw, lane);
+    }
+}
+
+// fft_wave with the fixed plan when L is one of the lengths compiled for a
+// kernel's points per lane P (L <= 64 P), else the run-time plan.
+template <int P, int SIGN>
+__device__ __forceinline__ void fft_wave_any(double2* buf, int L, int nst, const int* radix, const double2* tw,
+                                             int lane) {
+    if constexpr (P == 16) {
+        if (L == 1024) return fft_wave_fixed<1024, SIGN>(buf, tw, lane);
+        if (L == 768) return fft_wave_fixed<768, SIGN>(buf, tw, lane);
+        if (L == 512) return fft_wave_fixed<512, SIGN>(buf, tw, lane);
+    } else if constexpr (P == 4) {
+        if (L == 256) return fft_wave_fixed<256, SIGN>(buf, tw, lane);
+        if (L == 128) return fft_wave_fixed<128, SIGN>(buf, tw, lane);
+        if (L == 96) return fft_wave_fixed<96, SIGN>(buf, tw, lane);
+        if (L == 64) return fft_wave_fixed<64, SIGN>(buf, tw, lane);
+    }
+    fft_wave<P, SIGN>(buf, L, nst, radix, tw, lane);
+}
+
 // ---------------------------------------------------------------- ocqbdy, transposed
 // Forward (box), at a ring point x with inner point x_in (vorsubs.F90:58-114):
 //   q_k(x) = qbfac (p_k(x_in) - p_k(x)) - r f0 sum_l A_kl p_l(x) + r beta y + ...

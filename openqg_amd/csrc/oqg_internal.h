@@ -198,6 +198,7 @@ struct InvArgs {
     const DevConsts* dc;    // device copy of the precompute blocks
     const double* spec;
     const double* tw;       // twiddle table of the inverse plan
+    const double* twf;      // twiddle table of the forward plan (k_invw)
     const double* colsum;   // box: [m][col]
     const double* colw;     // box: closed-form x-sum weights per column
     const double* entpart;
@@ -432,6 +433,31 @@ void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, hipStre
 void launch_fwd_ens(const FwdArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st);
 void launch_ysolve_ens(const YArgs* args, const int2* map, int nblocks, int R, size_t lds, hipStream_t st);
 void launch_inv_ens(const InvArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st);
+// The wavefront-transform forms (one wavefront per mode and row pair): used for
+// the step whenever their LDS fits (fftw_fits); p = fftw_points of the basin.
+int fftw_points(const Spec& s);
+size_t fftw_lds_bytes(const Spec& s, int nl);
+bool fftw_fits(const Spec& s, int nl);
+void launch_fwdw(const FwdArgs& a, hipStream_t st);
+void launch_invw(const InvArgs& a, hipStream_t st);
+void launch_fwdw_ens(const FwdArgs* args, const int2* map, int nblocks, int nl, int p, size_t lds, hipStream_t st);
+void launch_invw_ens(const InvArgs* args, const int2* map, int nblocks, int nl, int p, bool cyc, size_t lds,
+                     hipStream_t st);
+// Their per-layer-count objects (oqg_wave.hip built with -DOQG_WAVE_NL=N).
+#define OQG_WAVE_DECL(N)                                                                                         \
+    void launch_fwdw_nl##N(const FwdArgs& a, hipStream_t st);                                                    \
+    void launch_invw_nl##N(const InvArgs& a, hipStream_t st);                                                    \
+    void launch_fwdw_ens_nl##N(const FwdArgs* args, const int2* map, int nblocks, int p, size_t lds, hipStream_t st); \
+    void launch_invw_ens_nl##N(const InvArgs* args, const int2* map, int nblocks, int p, bool cyc, size_t lds, hipStream_t st); \
+    void set_wave_attrs_nl##N(int lim);
+OQG_WAVE_DECL(2)
+OQG_WAVE_DECL(3)
+OQG_WAVE_DECL(4)
+OQG_WAVE_DECL(5)
+OQG_WAVE_DECL(6)
+OQG_WAVE_DECL(7)
+OQG_WAVE_DECL(8)
+#undef OQG_WAVE_DECL
 void launch_suppress_ens(const SuppArgs* args, int nmembers, hipStream_t st);
 
 }  // namespace oqg

@@ -1,0 +1,610 @@
+// oqg_wave.hip -- the wavefront-transform forms of k_fwd and k_inv for one layer
+// count.  The file is compiled once per layer count (-DOQG_WAVE_NL=2..8), each
+// object defining the launchers launch_*_nl<NL>; oqg_transform.hip dispatches
+// to them.
+
+#include "oqg_inv_dev.h"
+
+#ifndef OQG_WAVE_NL
+#error "compile oqg_wave.hip with -DOQG_WAVE_NL=<layer count>"
+#endif
+
+namespace oqg {
+
+// k_fwdw and k_invw do the work of k_fwd and k_inv for a whole row pair in one
+// workgroup with one wavefront per mode: each wavefront runs its mode's
+// transform on its own (fft_wave, radix-16/8 plan, no block barrier inside),
+// which takes the block barriers out of the transform stages and the modes'
+// redundant reads of q out of k_fwd.  In k_invw one more wavefront computes
+// the correction coefficients from the column sums and the constraint state
+// while the others transform, and then writes the zonal boundary rows of the
+// first and last row pairs.  The arithmetic per point is that of k_fwd and
+// k_inv; only the transform's stage plan and the sums' order differ.
+
+constexpr int WFPT = 4;   // points per thread whose inputs k_fwdw loads before any is used
+
+template <int NL, int P>
+__device__ __forceinline__ void fwdw_block(const FwdArgs& a, int jp) {
+    extern __shared__ double2 wsm[];   // [NL][zpad_len(L)] transforms, then the twiddle table
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L, ZL = zpad_len(L);
+    const int nth = (int)blockDim.x;   // 64 NL
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    double2* tw = wsm + (size_t)NL * ZL;
+    const int r0 = 2 * jp, r1 = r0 + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    Staged<12, double> stw;
+    stage_load(stw, a.tw, 2 * twiddle_table_len(L));
+    const int first = g.cyclic ? 0 : 1, npts = s.nin;
+    double q0[WFPT][NL], q1[WFPT][NL], d0[WFPT], d1[WFPT];
+#pragma unroll
+    for (int u = 0; u < WFPT; u++) {
+        const int t = first + (int)threadIdx.x + nth * u;
+        const bool in = t < first + npts;
+        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            q0[u][k] = in ? a.q[k * g.fs + o0] : 0.0;
+            q1[u][k] = (in && has1) ? a.q[k * g.fs + o1] : 0.0;
+        }
+        d0[u] = (in && a.ddyn) ? a.ddyn[o0] : 0.0;
+        d1[u] = (in && has1 && a.ddyn) ? a.ddyn[o1] : 0.0;
+    }
+    const double by0 = g.beta * a.yprel[gj0], by1 = has1 ? g.beta * a.yprel[gj1] : 0.0;
+    double cm[NL][NL];   // ctl2m(k, m), the same in every thread
+#pragma unroll
+    for (int k = 0; k < NL; k++)
+#pragma unroll
+        for (int m = 0; m < NL; m++) cm[k][m] = a.dc->mx.ctl2m[k * MAXL + m];
+    // p_modes_rhs (modes.F90:259-264) of every mode at point t of both rows,
+    // into the transform buffers: the odd extension in a box, the packed rows
+    // in a channel
+    auto put = [&](int t, const double (&qa)[NL], const double (&qb)[NL], double da, double db) {
+        double la[NL], lb[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            la[k] = qa[k] - by0;
+            lb[k] = qb[k] - by1;
+            if (k == g.kt) {
+                la[k] = la[k] - da;
+                lb[k] = lb[k] - db;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double sa = 0.0, sb = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                sa += la[k] * cm[k][m];
+                sb += lb[k] * cm[k][m];
+            }
+            const double2 w = make_double2(g.fnot * sa, has1 ? g.fnot * sb : 0.0);
+            double2* z = wsm + (size_t)m * ZL;
+            z[zpad(t)] = w;
+            if (!g.cyclic) z[zpad(L - t)] = make_double2(-w.x, -w.y);
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < WFPT; u++) {
+        const int t = first + (int)threadIdx.x + nth * u;
+        if (t < first + npts) put(t, q0[u], q1[u], d0[u], d1[u]);
+    }
+    for (int t = first + (int)threadIdx.x + nth * WFPT; t < first + npts; t += nth) {
+        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
+        double qa[NL], qb[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            qa[k] = a.q[k * g.fs + o0];
+            qb[k] = has1 ? a.q[k * g.fs + o1] : 0.0;
+        }
+        put(t, qa, qb, a.ddyn ? a.ddyn[o0] : 0.0, (has1 && a.ddyn) ? a.ddyn[o1] : 0.0);
+    }
+    if (!g.cyclic && (int)threadIdx.x < NL) {
+        double2* z = wsm + (size_t)threadIdx.x * ZL;
+        z[zpad(0)] = make_double2(0.0, 0.0);
+        z[zpad(L / 2)] = make_double2(0.0, 0.0);
+    }
+    stage_store(stw, (double*)tw, a.tw, 2 * twiddle_table_len(L));
+    __syncthreads();
+    double2* z = wsm + (size_t)wv * ZL;
+    fft_wave_any<P, -1>(z, L, s.nstages, s.radix, tw, lane);
+    double* row0 = a.spec + ((long long)wv * s.nrow + r0) * s.kp;
+    double* row1 = row0 + s.kp;
+    if (!g.cyclic) {
+        // DST(x)_K = -Im Z_K (row r0), DST(y)_K = Re Z_K (row r1)
+        for (int K = lane + 1; K <= s.nin; K += 64) {
+            const double2 v = z[zpad(K)];
+            row0[K - 1] = -v.y;
+            if (has1) row1[K - 1] = v.x;
+        }
+    } else {
+        const int N = L;
+        for (int kk = lane; kk <= N / 2; kk += 64) {
+            const double2 A = z[zpad(kk)], B = z[zpad((N - kk) % N)];
+            const double xr = 0.5 * (A.x + B.x), xi = 0.5 * (A.y - B.y);
+            const double yr = 0.5 * (A.y + B.y), yi = 0.5 * (B.x - A.x);
+            *(double2*)(row0 + 2 * kk) = make_double2(xr, xi);
+            if (has1) *(double2*)(row1 + 2 * kk) = make_double2(yr, yi);
+        }
+    }
+}
+
+template <int NL, int P>
+__global__ __launch_bounds__(64 * NL) void k_fwdw(FwdArgs a) {
+    fwdw_block<NL, P>(a, xcd_row_pair(blockIdx.x, gridDim.x));
+}
+
+// Ensemble form: block b works on row pair map[b].y of member map[b].x.
+template <int NL, int P>
+__global__ __launch_bounds__(64 * NL) void k_fwdw_ens(const FwdArgs* __restrict__ args, const int2* __restrict__ map) {
+    const int2 e = map[blockIdx.x];
+    fwdw_block<NL, P>(args[e.x], e.y);
+}
+
+// The homogeneous correction of every mode at one point (homog.F90:323-327,
+// 362-364): box, from the point's homogeneous solutions h[m-1] (m >= 1);
+// channel, from its row's profiles hr = (bt, bc1(2..nl), bc2(2..nl)).
+template <int NL>
+__device__ __forceinline__ void homcor(bool cyc, const double* coef, const double* h, double (&hc)[NL]) {
+    if (!cyc) {
+        hc[0] = 0.0;
+#pragma unroll
+        for (int m = 1; m < NL; m++) hc[m] = coef[m - 1] * h[m - 1];
+    } else {
+        hc[0] = coef[0] * h[0];
+#pragma unroll
+        for (int m = 1; m < NL; m++) hc[m] = coef[1 + m] * h[m] + coef[1 + MAXL + m] * h[NL - 1 + m];
+    }
+}
+
+// p_k = sum_m inh_m ctm2l(m,k) + sum_m hc_m ctm2l(m,k) (modes.F90:268-284), in
+// the order of pmix.
+template <int NL>
+__device__ __forceinline__ void mix_point(const double (&inh)[NL], const double (&hc)[NL], const double* sctm2l,
+                                          double (&pk)[NL]) {
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < NL; m++) t += inh[m] * sctm2l[m * MAXL + k];
+        pk[k] = t;
+    }
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < NL; m++) t += hc[m] * sctm2l[m * MAXL + k];
+        pk[k] = pk[k] + t;
+    }
+}
+
+// ocqbdy (vorsubs.F90:58-114) at one boundary point, with beta*yprel(gj)
+// given: kind 0 zonal row, 1 box meridional column, 2 box corner.
+template <int NL>
+__device__ __forceinline__ void qbdy_at(const InvArgs& a, const double* samat, double qbfac, double by, int i, int gj,
+                                        int kind, const double (&pb)[NL], const double (&pin)[NL]) {
+    const Geo& g = a.g;
+    const long long o = (long long)gj * g.pitch + i;
+    const int reps = (kind == 2) ? 2 : 1;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        double qv = qbfac * (pin[k] - pb[k]);
+        double ap = 0.0;
+#pragma unroll
+        for (int l = 0; l < NL; l++) ap += samat[k * MAXL + l] * pb[l];
+        for (int r = 0; r < reps; r++) qv = qv - g.fnot * ap;
+        for (int r = 0; r < reps; r++) qv = qv + by;
+        if (k == g.kt && a.ddyn)
+            for (int r = 0; r < reps; r++) qv = qv + a.ddyn[o];
+        a.qnew[k * g.fs + o] = qv;
+    }
+}
+
+// The inverse transform's value of mode buffer z at column i of row sel,
+// without branches (every read is from a valid slot).
+template <bool CYC>
+__device__ __forceinline__ double inh_at(const Geo& g, const Spec& s, const double2* z, int i, int sel) {
+    if constexpr (!CYC) {
+        const bool in = i >= 1 && i <= s.nin;
+        const double2 v = z[zpad(in ? i : 0)];
+        return in ? (sel ? v.x : -v.y) : 0.0;
+    } else {
+        const double2 v = z[zpad(i == g.nxp - 1 ? 0 : i)];
+        return sel ? v.y : v.x;
+    }
+}
+
+constexpr int INV_EPU = 3;              // epilogue points per thread and row with prefetched inputs
+constexpr int HR = 1 + 2 * (MAXL - 1);   // a channel row's homogeneous profile values
+
+// Row pair jp.  CYC: a channel (else a box).
+template <int NL, int P, bool CYC>
+__device__ __forceinline__ void invw_block(const InvArgs& a, int jp) {
+    extern __shared__ double2 wsm[];   // [NL][zpad_len(L)] transforms, then the twiddle table
+    constexpr int NH = NL > 1 ? NL - 1 : 1;
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L, ZL = zpad_len(L);
+    const int nth = (int)blockDim.x;   // 64 (NL + 1)
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const bool helper = wv == NL;
+    double2* tw = wsm + (size_t)NL * ZL;
+    __shared__ DevConsts s_dc;
+    __shared__ double s_cin[C_LEN];
+    __shared__ double s_aux[A_LEN];
+    __shared__ double s_spk[2 * MAXL];
+    __shared__ double s_csum[2 * MAXL * CQ_N];
+    __shared__ double coef[3 * MAXL + 4];
+    __shared__ double s_hin[4 * MAXL];   // box: hom at columns 1 and nxp-2 of the block's rows, [sel][side][m]
+    __shared__ double s_hbc[4 * MAXL];   // box: hom at columns 1 and nxp-2 of the zonal rows, [zone][side][m]
+    __shared__ double s_hr[4][HR];       // channel: the profiles of rows gj0, gj1, 0 and nyp-1
+    const int r0 = 2 * jp, r1 = r0 + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    const int npairs = (s.nyi + 1) / 2;
+    const bool first = jp == 0, last = jp == npairs - 1;
+    const int zsel1 = has1 ? 1 : 0;   // the row of the last pair next to row nyp-1
+
+    // ---- every global load of the block, issued before any is used
+    const int nw_dc = (int)(sizeof(DevConsts) / 8);
+    Staged<2> st_dc;
+    stage_load(st_dc, a.dc, nw_dc);
+    Staged<12, double> stw;
+    stage_load(stw, a.twf, 2 * twiddle_table_len(L));
+    // transform waves: the spectra of mode wv on both rows
+    constexpr int SP = P / 2 + 1;
+    double sx0[CYC ? 1 : SP], sx1[CYC ? 1 : SP];
+    double2 cx0[CYC ? SP : 1], cx1[CYC ? SP : 1];
+    if (!helper) {
+        const double* row0 = a.spec + ((long long)wv * s.nrow + r0) * s.kp;
+        const double* row1 = row0 + s.kp;
+#pragma unroll
+        for (int u = 0; u < SP; u++) {
+            const int t = lane + 64 * u;
+            if constexpr (!CYC) {
+                const int tc = (t < s.nin) ? t : 0;
+                sx0[u] = row0[tc];
+                sx1[u] = row1[tc];
+            } else {
+                const int tc = (t <= L / 2) ? t : 0;
+                cx0[u] = *(const double2*)(row0 + 2 * tc);
+                cx1[u] = *(const double2*)(row1 + 2 * tc);
+            }
+        }
+    }
+    // helper wave: the integrals behind the correction coefficients and the
+    // small per-block inputs of the epilogue
+    double v[NL + 1];
+#pragma unroll
+    for (int t = 0; t <= NL; t++) v[t] = 0.0;
+    double cinv = 0.0, spkv = 0.0, hinv = 0.0, hbcv = 0.0, hrv[4] = {0.0, 0.0, 0.0, 0.0};
+    double txi_s = 0.0, txi_n = 0.0;
+    if (helper) {
+        if (lane < C_LEN) cinv = a.cin[lane];
+        if constexpr (CYC) {
+            txi_s = a.fcon[F_TXIS];
+            txi_n = a.fcon[F_TXIN];
+            if (lane < 2 * NL) {
+                const int m = lane >> 1, r = (lane & 1) ? s.nyi - 1 : 0;
+                spkv = a.spec[((long long)m * s.nrow + r) * s.kp];
+            }
+            // the row profiles (bt, bc1(2..nl), bc2(2..nl)) of rows gj0, gj1, 0, nyp-1
+            if (lane < 2 * NL - 1) {
+                const int rows[4] = {gj0, has1 ? gj1 : gj0, 0, g.nyp - 1};
+                const double* src = lane == 0 ? a.hom
+                                              : (lane < NL ? a.hom + g.nyp + (long long)(lane - 1) * g.nyp
+                                                           : a.hom + g.nyp + (long long)(NL - 1 + lane - NL) * g.nyp);
+#pragma unroll
+                for (int r = 0; r < 4; r++) hrv[r] = src[rows[r]];
+            }
+        } else {
+            if (lane < 4 * (NL - 1)) {
+                const int sel = lane / (2 * (NL - 1)), side = (lane / (NL - 1)) & 1, m = lane % (NL - 1);
+                const int gj = sel == 0 ? gj0 : gj1, i = side == 0 ? 1 : g.nxp - 2;
+                if (sel == 0 || has1) hinv = a.hom[(long long)m * g.fs + (long long)gj * g.pitch + i];
+                const int gz = sel == 0 ? 0 : g.nyp - 1;   // here sel is the zone
+                if ((sel == 0 && first) || (sel == 1 && last))
+                    hbcv = a.hom[(long long)m * g.fs + (long long)gz * g.pitch + i];
+            }
+            // ncol <= 32 P: at most P/2 columns per lane, all loaded before any is used
+            constexpr int NC = P / 2;
+            double cw[NC], cs[NC][NL];
+#pragma unroll
+            for (int u = 0; u < NC; u++) {
+                const int col = lane + 64 * u;
+                const bool ok = col < s.ncol;
+                cw[u] = ok ? a.colw[col] : 0.0;
+#pragma unroll
+                for (int m = 0; m < NL; m++) cs[u][m] = ok ? a.colsum[m * s.ncol + col] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < NC; u++)
+#pragma unroll
+                for (int m = 0; m < NL; m++) v[m] += cw[u] * cs[u][m];
+        }
+        constexpr int NE = 4;   // entrainment partials per lane loaded together; more by the loop
+        double ev[NE];
+#pragma unroll
+        for (int u = 0; u < NE; u++) ev[u] = (lane + 64 * u < a.nentpart) ? a.entpart[lane + 64 * u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < NE; u++) v[NL] += ev[u];
+        for (int t = lane + 64 * NE; t < a.nentpart; t += 64) v[NL] += a.entpart[t];
+    }
+    // every thread (box): the homogeneous solutions under its first epilogue
+    // points of both rows (i = tid + nth u) and, in the first and last row
+    // pairs, on the zonal boundary row next to the block
+    const int zrow = first ? 0 : (last ? g.nyp - 1 : -1);   // nyp >= 8: never both
+    const int zsel = first ? 0 : zsel1;
+    double hv[2][INV_EPU][NH], hz[INV_EPU][NH];
+#pragma unroll
+    for (int u = 0; u < INV_EPU; u++) {
+        const int i = (int)threadIdx.x + nth * u;
+        const bool in = !CYC && i < g.nxp;
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            const double* hm = a.hom + (long long)(m - 1) * g.fs + i;
+            hv[0][u][m - 1] = in ? hm[(long long)gj0 * g.pitch] : 0.0;
+            hv[1][u][m - 1] = (in && has1) ? hm[(long long)gj1 * g.pitch] : 0.0;
+            hz[u][m - 1] = (in && zrow >= 0) ? hm[(long long)(zrow > 0 ? zrow : 0) * g.pitch] : 0.0;
+        }
+    }
+    const double by0 = g.beta * a.yprel[gj0], by1 = has1 ? g.beta * a.yprel[gj1] : 0.0;
+    const double byz0 = g.beta * a.yprel[0], byz1 = g.beta * a.yprel[g.nyp - 1];
+
+    // ---- LDS: constants, twiddles, the transforms' inputs
+    stage_store(st_dc, &s_dc, a.dc, nw_dc);
+    stage_store(stw, (double*)tw, a.twf, 2 * twiddle_table_len(L));
+    const double fn = s.ftnorm;
+    if (!helper) {
+        double2* z = wsm + (size_t)wv * ZL;
+        if constexpr (!CYC) {
+            // odd extension of ftnorm*u for the two rows (inhomog.F90:202-203)
+            if (lane == 0) {
+                z[zpad(0)] = make_double2(0.0, 0.0);
+                z[zpad(L / 2)] = make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < SP; u++) {
+                const int t = lane + 64 * u;   // spectral index K-1
+                if (t < s.nin) {
+                    const double2 w = make_double2(fn * sx0[u], has1 ? fn * sx1[u] : 0.0);
+                    z[zpad(t + 1)] = w;
+                    z[zpad(L - t - 1)] = make_double2(-w.x, -w.y);
+                }
+            }
+        } else {
+            // Hermitian spectra of the two packed rows (inhomog.F90:246-247)
+            const int N = L, h = N / 2;
+#pragma unroll
+            for (int u = 0; u < SP; u++) {
+                const int kk = lane + 64 * u;
+                if (kk <= h) {
+                    double xr = fn * cx0[u].x, xi = fn * cx0[u].y;
+                    double yr = has1 ? fn * cx1[u].x : 0.0, yi = has1 ? fn * cx1[u].y : 0.0;
+                    if (kk == 0 || (N % 2 == 0 && kk == h)) { xi = 0.0; yi = 0.0; }
+                    z[zpad(kk)] = make_double2(xr - yi, xi + yr);
+                    if (kk != 0 && !(N % 2 == 0 && kk == h)) z[zpad(N - kk)] = make_double2(xr + yi, -xi + yr);
+                }
+            }
+        }
+    } else {
+        if (lane < C_LEN) s_cin[lane] = cinv;
+        if constexpr (CYC) {
+            if (lane < 2 * NL) s_spk[lane] = spkv;
+            if (lane < 2 * NL - 1) {
+                // lane 0: bt; lanes 1..NL-1: bc1 of mode lane; NL..2NL-2: bc2 of mode lane-NL+1 (0-based modes)
+#pragma unroll
+                for (int r = 0; r < 4; r++) s_hr[r][lane] = hrv[r];
+            }
+        } else if (lane < 4 * (NL - 1)) {
+            s_hin[lane] = hinv;
+            s_hbc[lane] = hbcv;
+        }
+    }
+    __syncthreads();
+
+    if (!helper) {
+        // ---- the inverse transform of mode wv
+        double2* z = wsm + (size_t)wv * ZL;
+        if constexpr (CYC) fft_wave_any<P, +1>(z, L, s.nstages, s.radix, tw, lane);   // c2r (inhomog.F90:247)
+        else fft_wave_any<P, -1>(z, L, s.nstages, s.radix, tw, lane);
+    } else {
+        // ---- correction coefficients (the same in every block)
+#pragma unroll
+        for (int t = 0; t <= NL; t++) v[t] = wave_sum(v[t]);
+        if constexpr (CYC) {
+            // the boundary integrals of each layer from k_vort's per-tile partials, in tile order
+            for (int idx = lane; idx < 2 * NL * CQ_N; idx += 64) {
+                const int ks = idx / CQ_N, qy = idx - ks * CQ_N;
+                const double* cp = a.cpart + (long long)ks * a.ctiles * CYC_SLOTS + qy;
+                double t = cp[0];
+                for (int u = 1; u < a.ctiles; u++) t += cp[u * CYC_SLOTS];
+                s_csum[idx] = t;
+            }
+            wave_lds_sync();
+            if (lane < NL) {
+                const int k = lane;
+                const double dx = g.dx;
+                const Phys& ph = s_dc.ph;
+                auto S = [&](int side, int q) { return s_csum[(k * 2 + side) * CQ_N + q]; };
+                s_aux[A_AP3S + k] = ph.ah2[k] * (S(0, CQ_D2) * dx) * ph.rdy;
+                s_aux[A_AP3N + k] = ph.ah2[k] * (S(1, CQ_D2) * dx) * ph.rdy;
+                s_aux[A_AP5S + k] = ph.ah4[k] * (S(0, CQ_D4) * dx) * ph.rdy;
+                s_aux[A_AP5N + k] = ph.ah4[k] * (S(1, CQ_D4) * dx) * ph.rdy;
+                const double aj5s = (S(0, CQ_AJ5) * dx) * ph.rdx2, aj9s = (S(0, CQ_AJ9) * dx) * ph.rdx2;
+                const double aj5n = (S(1, CQ_AJ5) * dx) * ph.rdx2, aj9n = (S(1, CQ_AJ9) * dx) * ph.rdx2;
+                s_aux[A_AJIS + k] = (aj5s + 2.0 * aj9s) * (1.0 / 6.0);
+                s_aux[A_AJIN + k] = -((aj5n + 2.0 * aj9n) * (1.0 / 6.0));
+                if (k >= 2) {   // layers 1 and 2 are written below
+                    s_aux[A_BES + k] = 0.0;
+                    s_aux[A_BEN + k] = 0.0;
+                }
+                if (k == g.kt) {
+                    s_aux[A_BDRS] = ph.cd_bdr * ((S(0, CQ_BD) * dx) * ph.rdy);
+                    s_aux[A_BDRN] = ph.cd_bdr * ((S(1, CQ_BD) * dx) * ph.rdy);
+                }
+                if (k == 0) {
+                    // int_Be_s/n (vorticity.F90:86-93): fohfac(2) in both northern terms
+                    const double dz = (double)ph.dz_sign;
+                    s_aux[A_BES + 0] = dz * ph.fohfac[0] * (S(0, CQ_E1) * dx);
+                    s_aux[A_BES + 1] = dz * ph.fohfac[0] * (S(0, CQ_E2) * dx);
+                    s_aux[A_BEN + 0] = -dz * ph.fohfac[1] * (S(1, CQ_E1) * dx);
+                    s_aux[A_BEN + 1] = -dz * ph.fohfac[1] * (S(1, CQ_E2) * dx);
+                    s_aux[A_TXIS] = txi_s;
+                    s_aux[A_TXIN] = txi_n;
+                }
+            }
+            wave_lds_sync();
+        }
+        if (lane == 0) step_coefficients<NL>(a, s_dc.mx, s_dc.bh, s_dc.ch, s_dc.ph, s_cin, s_aux, s_spk, v, coef, jp == 0);
+    }
+    __syncthreads();
+
+    // ---- p_new on the block's rows and boundary q on their meridional ends;
+    // the point's thread also writes the zonal boundary row next to it (first
+    // and last row pairs)
+    const double* s_ctm2l = s_dc.mx.ctm2l;
+    const double* s_amat = s_dc.mx.amat;
+    const double qbfac = s_dc.ph.qbfac;
+    const int nxp = g.nxp;
+    double* const pnew = a.pnew;
+    auto point = [&](int sel, int i, const double (&hp)[NH], const double (&hzp)[NH]) {
+        const int gj = sel == 0 ? gj0 : gj1;
+        double inh[NL], hc[NL], pk[NL];
+#pragma unroll
+        for (int m = 0; m < NL; m++) inh[m] = inh_at<CYC>(g, s, wsm + (size_t)m * ZL, i, sel);
+        if constexpr (CYC) homcor<NL>(true, coef, s_hr[sel], hc);
+        else homcor<NL>(false, coef, hp, hc);
+        mix_point<NL>(inh, hc, s_ctm2l, pk);
+        const long long o = (long long)gj * g.pitch + i;
+#pragma unroll
+        for (int k = 0; k < NL; k++) pnew[k * g.fs + o] = pk[k];
+        const bool wall = !CYC && (i == 0 || i == nxp - 1);
+        const int side = i == 0 ? 0 : 1;
+        double pin[NL];
+        if (wall) {
+            const int ii = i == 0 ? 1 : nxp - 2;
+#pragma unroll
+            for (int m = 0; m < NL; m++) inh[m] = inh_at<CYC>(g, s, wsm + (size_t)m * ZL, ii, sel);
+            homcor<NL>(false, coef, s_hin + (sel * 2 + side) * (NL - 1), hc);
+            mix_point<NL>(inh, hc, s_ctm2l, pin);
+            qbdy_at<NL>(a, s_amat, qbfac, sel == 0 ? by0 : by1, i, gj, 1, pk, pin);
+        }
+        if (zrow >= 0 && sel == zsel) {
+            const int zn = zrow == 0 ? 0 : 1;
+            double zero[NL], pb[NL];
+#pragma unroll
+            for (int m = 0; m < NL; m++) zero[m] = 0.0;
+            if constexpr (CYC) homcor<NL>(true, coef, s_hr[2 + zn], hc);
+            else homcor<NL>(false, coef, hzp, hc);
+            mix_point<NL>(zero, hc, s_ctm2l, pb);
+            const long long oz = (long long)zrow * g.pitch + i;
+#pragma unroll
+            for (int k = 0; k < NL; k++) pnew[k * g.fs + oz] = pb[k];
+            const double byz = zn == 0 ? byz0 : byz1;
+            if (wall) {
+                // corners take the x difference (vorsubs.F90:83-84 overwrite :79-80)
+                homcor<NL>(false, coef, s_hbc + (zn * 2 + side) * (NL - 1), hc);
+                mix_point<NL>(zero, hc, s_ctm2l, pin);
+                qbdy_at<NL>(a, s_amat, qbfac, byz, i, zrow, 2, pb, pin);
+            } else {
+                qbdy_at<NL>(a, s_amat, qbfac, byz, i, zrow, 0, pb, pk);
+            }
+        }
+    };
+#pragma unroll
+    for (int sel = 0; sel < 2; sel++) {
+        if (sel == 1 && !has1) break;
+#pragma unroll
+        for (int u = 0; u < INV_EPU; u++) {
+            const int i = (int)threadIdx.x + nth * u;
+            if (i < nxp) point(sel, i, hv[sel][u], hz[u]);
+        }
+        for (int i = (int)threadIdx.x + nth * INV_EPU; i < nxp; i += nth) {
+            const int gj = sel == 0 ? gj0 : gj1;
+            double h[NH], hzr[NH];
+#pragma unroll
+            for (int m = 1; m < NL; m++) {
+                const double* hm = a.hom + (long long)(m - 1) * g.fs + i;
+                h[m - 1] = CYC ? 0.0 : hm[(long long)gj * g.pitch];
+                hzr[m - 1] = (CYC || zrow < 0) ? 0.0 : hm[(long long)zrow * g.pitch];
+            }
+            point(sel, i, h, hzr);
+        }
+    }
+}
+
+template <int NL, int P, bool CYC>
+__global__ __launch_bounds__(64 * (NL + 1)) void k_invw(InvArgs a) {
+    invw_block<NL, P, CYC>(a, xcd_row_pair(blockIdx.x, gridDim.x));
+}
+
+template <int NL, int P, bool CYC>
+__global__ __launch_bounds__(64 * (NL + 1)) void k_invw_ens(const InvArgs* __restrict__ args,
+                                                            const int2* __restrict__ map) {
+    const int2 e = map[blockIdx.x];
+    invw_block<NL, P, CYC>(args[e.x], e.y);
+}
+
+// Launchers of this object's layer count, over the points per lane (4, 16, 32).
+#define OQG_WCAT2(a, b) a##b
+#define OQG_WCAT(a, b) OQG_WCAT2(a, b)
+#define OQG_WFN(name) OQG_WCAT(name, OQG_WAVE_NL)
+constexpr int WNL = OQG_WAVE_NL;
+
+#define OQG_P_SWITCH(PV, CALL)      \
+    switch (PV) {                   \
+        case 4: CALL(4); break;     \
+        case 16: CALL(16); break;   \
+        case 32: CALL(32); break;   \
+        default: break;             \
+    }
+
+void OQG_WFN(launch_fwdw_nl)(const FwdArgs& a, hipStream_t st) {
+    const int npairs = (a.s.nyi + 1) / 2;
+    const size_t lds = fftw_lds_bytes(a.s, WNL);
+#define FWDW_CALL(Pc) hipLaunchKernelGGL((k_fwdw<WNL, Pc>), dim3(npairs), dim3(64 * WNL), lds, st, a)
+    OQG_P_SWITCH(fftw_points(a.s), FWDW_CALL)
+#undef FWDW_CALL
+}
+
+void OQG_WFN(launch_invw_nl)(const InvArgs& a, hipStream_t st) {
+    const int npairs = (a.s.nyi + 1) / 2;
+    const size_t lds = fftw_lds_bytes(a.s, WNL);
+#define INVW_CALL(Pc)                                                                                        \
+    if (a.g.cyclic) hipLaunchKernelGGL((k_invw<WNL, Pc, true>), dim3(npairs), dim3(64 * (WNL + 1)), lds, st, a); \
+    else hipLaunchKernelGGL((k_invw<WNL, Pc, false>), dim3(npairs), dim3(64 * (WNL + 1)), lds, st, a);
+    OQG_P_SWITCH(fftw_points(a.s), INVW_CALL)
+#undef INVW_CALL
+}
+
+void OQG_WFN(launch_fwdw_ens_nl)(const FwdArgs* args, const int2* map, int nblocks, int p, size_t lds, hipStream_t st) {
+#define FWDWE_CALL(Pc) hipLaunchKernelGGL((k_fwdw_ens<WNL, Pc>), dim3(nblocks), dim3(64 * WNL), lds, st, args, map)
+    OQG_P_SWITCH(p, FWDWE_CALL)
+#undef FWDWE_CALL
+}
+
+void OQG_WFN(launch_invw_ens_nl)(const InvArgs* args, const int2* map, int nblocks, int p, bool cyc, size_t lds,
+                                 hipStream_t st) {
+#define INVWE_CALL(Pc)                                                                                          \
+    if (cyc) hipLaunchKernelGGL((k_invw_ens<WNL, Pc, true>), dim3(nblocks), dim3(64 * (WNL + 1)), lds, st, args, map); \
+    else hipLaunchKernelGGL((k_invw_ens<WNL, Pc, false>), dim3(nblocks), dim3(64 * (WNL + 1)), lds, st, args, map);
+    OQG_P_SWITCH(p, INVWE_CALL)
+#undef INVWE_CALL
+}
+
+void OQG_WFN(set_wave_attrs_nl)(int lim) {
+#define ATTR_P(Pc)                                                                                                    \
+    (void)hipFuncSetAttribute((const void*)k_fwdw<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);     \
+    (void)hipFuncSetAttribute((const void*)k_fwdw_ens<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+    (void)hipFuncSetAttribute((const void*)k_invw<WNL, Pc, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);     \
+    (void)hipFuncSetAttribute((const void*)k_invw<WNL, Pc, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);      \
+    (void)hipFuncSetAttribute((const void*)k_invw_ens<WNL, Pc, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+    (void)hipFuncSetAttribute((const void*)k_invw_ens<WNL, Pc, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    ATTR_P(4) ATTR_P(16) ATTR_P(32)
+#undef ATTR_P
+}
+
+}  // namespace oqg
