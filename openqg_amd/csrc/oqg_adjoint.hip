@@ -107,7 +107,8 @@ __device__ __forceinline__ double lapT_lds(const double* s, int W, int r, int c,
 //   lp_{n-1} = c2 (L^T)^2 mu - c4 (L^T)^3 mu - [k=kt] cdrag L^T mu
 //   lwekp += cwek mu_0,  lq_{n-1} = lq_{n+1} on the interior, 0 on the ring.
 // k_adj_vort tiles: 64 x 8 points of one layer per 256-thread block (two rows
-// per thread), 33 KB of LDS, so that four blocks share a CU.
+// per thread), 33 KB of LDS, so that four blocks share a CU (64 x 16 tiles,
+// two blocks per CU, measured 14.9 against 12.3 us at 257^2).
 constexpr int ADJ_TX = 64, ADJ_TY = 8;
 
 // Away from the boundary (2 <= i <= nxp-3, 2 <= j <= nyp-3, "deep" points)

@@ -121,6 +121,7 @@ struct oqg_ctx {
     double* scratch = nullptr;     // this device's address of the no-alloc scratch array
     hipStream_t own = nullptr;
     hipStream_t st = nullptr;
+    hipEvent_t ens_done = nullptr;   // end of the last ensemble run this basin took part in, not yet joined
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::string err;
     // host copies
@@ -154,6 +155,15 @@ struct oqg_ctx {
 };
 
 // ------------------------------------------------------------------ error helpers
+
+// Orders the basin's stream after the ensemble run it last took part in
+// (oqg_ens_run leaves the event for the member's next call to wait for).
+static void ctx_join(oqg_ctx* c) {
+    if (c && c->ens_done) {
+        (void)hipStreamWaitEvent(c->st, c->ens_done, 0);
+        c->ens_done = nullptr;
+    }
+}
 
 static int fail(oqg_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -932,6 +942,7 @@ void oqg_destroy(oqg_ctx* c) {
 const char* oqg_last_error(const oqg_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
 
 int oqg_set_state(oqg_ctx* c, const double* q, const double* qm, const double* p, const double* pm, int kind) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (c->g.cyclic && kind == OQG_HOST) {
         const double* fs[4] = {q, qm, p, pm};
@@ -953,6 +964,7 @@ int oqg_set_state(oqg_ctx* c, const double* q, const double* qm, const double* p
 }
 
 int oqg_get_state(oqg_ctx* c, double* q, double* qm, double* p, double* pm, int kind) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     const int cur = c->cur;
     if (q) if (int r = download_field(c, q, c->qbuf[cur], c->g.nl, kind)) return r;
@@ -964,6 +976,7 @@ int oqg_get_state(oqg_ctx* c, double* q, double* qm, double* p, double* pm, int 
 }
 
 int oqg_get_layer(oqg_ctx* c, int which, int k, double* out, int kind) {
+    ctx_join(c);
     if (!c || !out) return OQG_ERR_ARG;
     if (k < 1 || k > c->g.nl) return fail(c, OQG_ERR_ARG, "layer index out of range");
     const double* src;
@@ -982,6 +995,7 @@ int oqg_get_layer(oqg_ctx* c, int which, int k, double* out, int kind) {
 int oqg_constr_len(const oqg_ctx* c) { return c ? 2 * (c->g.nl - 1) + 4 * c->g.nl : 0; }
 
 int oqg_get_constr(oqg_ctx* c, double* v) {
+    ctx_join(c);
     if (!c || !v) return OQG_ERR_ARG;
     double h[C_LEN];
     HIPCHK(c, hipMemcpyAsync(h, c->cbuf[c->cur], sizeof h, hipMemcpyDeviceToHost, c->st));
@@ -998,6 +1012,7 @@ int oqg_get_constr(oqg_ctx* c, double* v) {
 }
 
 int oqg_set_constr(oqg_ctx* c, const double* v) {
+    ctx_join(c);
     if (!c || !v) return OQG_ERR_ARG;
     double h[C_LEN] = {};
     const int nl = c->g.nl;
@@ -1014,6 +1029,7 @@ int oqg_set_constr(oqg_ctx* c, const double* v) {
 }
 
 int oqg_init_state(oqg_ctx* c) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     const int cur = c->cur;
     for (int t = 0; t < 2; t++) {
@@ -1085,6 +1101,7 @@ int oqg_init_state(oqg_ctx* c) {
 }
 
 int oqg_set_forcing(oqg_ctx* c, const double* ent, int n_ent, const double* wekp, double txis, double txin, int kind) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     const hipMemcpyKind mk = kind == OQG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     const int nx = c->g.nxp, ny = c->g.nyp;
@@ -1118,6 +1135,7 @@ static int wind_buffers(oqg_ctx* c) {
 }
 
 int oqg_set_wind(oqg_ctx* c, const double* taux, const double* tauy, int kind) {
+    ctx_join(c);
     if (!c || !taux || !tauy) return c ? fail(c, OQG_ERR_ARG, "oqg_set_wind: null stress") : OQG_ERR_ARG;
     if (int r = wind_buffers(c)) return r;
     if (int r = upload_field(c, c->taux_d, taux, 1, kind)) return r;
@@ -1135,6 +1153,7 @@ int oqg_set_wind(oqg_ctx* c, const double* taux, const double* tauy, int kind) {
 }
 
 int oqg_get_forcing(oqg_ctx* c, double* wekp, double* txi, int kind) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (wekp)
         if (int r = download_field(c, wekp, c->wekp, 1, kind)) return r;
@@ -1150,6 +1169,7 @@ int oqg_get_forcing(oqg_ctx* c, double* wekp, double* txi, int kind) {
 }
 
 int oqg_get_wind(oqg_ctx* c, double* taux, double* tauy, int kind) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (!c->taux_d) return fail(c, OQG_ERR_ARG, "oqg_get_wind: the basin has no resident stress");
     if (taux)
@@ -1161,6 +1181,7 @@ int oqg_get_wind(oqg_ctx* c, double* taux, double* tauy, int kind) {
 }
 
 int oqg_get_ent(oqg_ctx* c, double* ent1, int kind) {
+    ctx_join(c);
     if (!c || !ent1) return c ? fail(c, OQG_ERR_ARG, "oqg_get_ent: null field") : OQG_ERR_ARG;
     if (int r = download_field(c, ent1, c->ent1, 1, kind)) return r;
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1252,6 +1273,7 @@ static void diag_finish(const oqg_ctx* c, const double* comb, double dt, double 
 extern "C" {
 
 int oqg_diagnose(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
+    ctx_join(c);
     if (!c || !out) return OQG_ERR_ARG;
     const int nl = c->g.nl, ny = c->g.nyp;
     if (!c->diag_part)
@@ -1270,6 +1292,7 @@ int oqg_diagnose(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
 }
 
 int oqg_diagnose_noalloc(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
+    ctx_join(c);
     if (!c || !out) return OQG_ERR_ARG;
     const int nl = c->g.nl, ny = c->g.nyp;
     const long long need = (long long)ny * DG_LEN + DG_COMB_LEN;
@@ -1296,6 +1319,7 @@ int oqg_diagnose_noalloc(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
 }
 
 int oqg_adj_wind(oqg_ctx* c, double* ltaux, double* ltauy, int kind) {
+    ctx_join(c);
     if (!c || !ltaux || !ltauy) return c ? fail(c, OQG_ERR_ARG, "oqg_adj_wind: null output") : OQG_ERR_ARG;
     if (!c->adj) return fail(c, OQG_ERR_STATE, "call oqg_adj_init first");
     if (int r = wind_buffers(c)) return r;
@@ -1313,6 +1337,7 @@ int oqg_adj_wind(oqg_ctx* c, double* ltaux, double* ltauy, int kind) {
 }
 
 int oqg_step(oqg_ctx* c, double tdt, const double* ent, int n_ent, const double* wekp, double txis, double txin) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if ((ent == nullptr) != (wekp == nullptr))
         return fail(c, OQG_ERR_ARG, "pass both ent and wekp, or neither to use the resident forcing");
@@ -1322,6 +1347,7 @@ int oqg_step(oqg_ctx* c, double tdt, const double* ent, int n_ent, const double*
 }
 
 int oqg_suppress(oqg_ctx* c) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     return enqueue_suppress(c);
 }
@@ -1438,20 +1464,24 @@ static int run_steps(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, bo
 extern "C" {
 
 int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
+    ctx_join(c);
     return run_steps(c, tdt, nsteps, step0, supp, false);
 }
 
 int oqg_plan_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
+    ctx_join(c);
     return run_steps(c, tdt, nsteps, step0, supp, true);
 }
 
 int oqg_sync(oqg_ctx* c) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
 
 int oqg_solve_inhomog(oqg_ctx* c, int m, const double* rhs, double* out, int kind) {
+    ctx_join(c);
     if (!c || !rhs || !out) return OQG_ERR_ARG;
     if (m < 1 || m > c->g.nl) return fail(c, OQG_ERR_ARG, "mode index out of range");
     if (int r = upload_field(c, c->raw[0], rhs, 1, kind)) return r;
@@ -1477,15 +1507,20 @@ int oqg_get_modes(oqg_ctx* c, double* amat, double* ctl2m, double* ctm2l, double
 }
 
 int oqg_set_stream(oqg_ctx* c, void* stream) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->st = stream ? static_cast<hipStream_t>(stream) : c->own;
     return 0;
 }
 
-void* oqg_get_stream(oqg_ctx* c) { return c ? static_cast<void*>(c->st) : nullptr; }
+void* oqg_get_stream(oqg_ctx* c) {
+    ctx_join(c);
+    return c ? static_cast<void*>(c->st) : nullptr;
+}
 
 int oqg_state_ptrs(oqg_ctx* c, double** q, double** qm, double** p, double** pm, size_t* pitch) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (q) *q = c->qbuf[c->cur];
     if (qm) *qm = c->qbuf[1 - c->cur];
@@ -1552,6 +1587,7 @@ static int time_kernels(oqg_ctx* c, double tdt, int reps, hipEvent_t* ev, double
 }
 
 int oqg_kernel_times(oqg_ctx* c, double tdt, int reps, double* ms, int max, char* names, int names_len) {
+    ctx_join(c);
     if (!c || reps < 1 || reps > 256 || !ms) return -OQG_ERR_ARG;
     hipEvent_t ev[2];
     for (auto& e : ev)
@@ -1705,6 +1741,7 @@ static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed, const SnapR
 extern "C" {
 
 int oqg_adj_init(oqg_ctx* c) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (c->g.cyclic) return fail(c, OQG_ERR_ARG, "the adjoint is implemented for box basins");
     if (c->adj) return 0;
@@ -1759,6 +1796,7 @@ int oqg_adj_init(oqg_ctx* c) {
 }
 
 int oqg_adj_set(oqg_ctx* c, const double* lq, const double* lqm, const double* lp, const double* lpm, int kind) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     double* dst[4] = {c->lq[c->cur], c->lq[1 - c->cur], c->lp[c->cur], c->lp[1 - c->cur]};
@@ -1775,6 +1813,7 @@ int oqg_adj_set(oqg_ctx* c, const double* lq, const double* lqm, const double* l
 }
 
 int oqg_adj_get(oqg_ctx* c, double* lq, double* lqm, double* lp, double* lpm, int kind) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     const double* src[4] = {c->lq[c->cur], c->lq[1 - c->cur], c->lp[c->cur], c->lp[1 - c->cur]};
@@ -1787,12 +1826,14 @@ int oqg_adj_get(oqg_ctx* c, double* lq, double* lqm, double* lp, double* lpm, in
 }
 
 int oqg_adj_step(oqg_ctx* c, double tdt, int suppressed) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     return enqueue_adj_step(c, tdt, suppressed != 0);
 }
 
 int oqg_adj_get_wekp(oqg_ctx* c, double* out, int kind) {
+    ctx_join(c);
     if (!c || !out) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     if (int r = download_field(c, out, c->lwek, 1, kind)) return r;
@@ -1801,6 +1842,7 @@ int oqg_adj_get_wekp(oqg_ctx* c, double* out, int kind) {
 }
 
 int oqg_adj_zero_wekp(oqg_ctx* c) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     HIPCHK(c, hipMemsetAsync(c->lwek, 0, (size_t)c->g.fs * sizeof(double), c->st));
@@ -1808,11 +1850,13 @@ int oqg_adj_zero_wekp(oqg_ctx* c) {
 }
 
 int oqg_invert(oqg_ctx* c) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     return enqueue_invert(c);
 }
 
 int oqg_adj_invert(oqg_ctx* c) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     const int cur = c->cur;
@@ -1858,6 +1902,7 @@ static int snap_sync_dev(oqg_ctx* c) {
 extern "C" {
 
 int oqg_snap_alloc(oqg_ctx* c, int nslots) {
+    ctx_join(c);
     if (!c || nslots < 0) return OQG_ERR_ARG;
     const size_t words = 4 * (size_t)c->g.nl * c->g.fs + 2 * C_LEN;
     const size_t had = c->snaps.size();
@@ -1920,6 +1965,7 @@ static bool suppressed_at(int nt, int supp) { return supp > 0 && (nt - 1) % supp
 extern "C" {
 
 int oqg_snap_save(oqg_ctx* c, int slot) {
+    ctx_join(c);
     if (!c || slot < 0 || slot >= (int)c->snaps.size()) return c ? fail(c, OQG_ERR_ARG, "snapshot slot out of range") : OQG_ERR_ARG;
     snap_copy(c, slot, true);
     HIPCHK(c, hipGetLastError());
@@ -1927,6 +1973,7 @@ int oqg_snap_save(oqg_ctx* c, int slot) {
 }
 
 int oqg_snap_load(oqg_ctx* c, int slot) {
+    ctx_join(c);
     if (!c || slot < 0 || slot >= (int)c->snaps.size()) return c ? fail(c, OQG_ERR_ARG, "snapshot slot out of range") : OQG_ERR_ARG;
     snap_copy(c, slot, false);
     HIPCHK(c, hipGetLastError());
@@ -1934,6 +1981,7 @@ int oqg_snap_load(oqg_ctx* c, int slot) {
 }
 
 int oqg_adj_record(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, int slot0) {
+    ctx_join(c);
     if (!c || nsteps < 0 || slot0 < 0) return OQG_ERR_ARG;
     if ((long long)slot0 + nsteps > (long long)c->snaps.size())
         return fail(c, OQG_ERR_ARG, "oqg_adj_record: fewer snapshot slots than steps");
@@ -1973,6 +2021,7 @@ int oqg_adj_record(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, int 
 }
 
 int oqg_adj_sweep(oqg_ctx* c, double tdt, int nsteps, int step0, int supp, int slot0) {
+    ctx_join(c);
     if (!c || nsteps < 0 || slot0 < 0) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     if ((long long)slot0 + nsteps > (long long)c->snaps.size())
@@ -2160,6 +2209,7 @@ int oqg_revolve_split(int nsteps, int free_slots) {
 }
 
 int oqg_adj_revolve_forward(oqg_ctx* c, double tdt, int nsteps, int supp, int free_slots) {
+    ctx_join(c);
     if (!c || nsteps < 1 || free_slots < 0) return OQG_ERR_ARG;
     if ((int)c->snaps.size() < free_slots + 1)
         return fail(c, OQG_ERR_ARG, "oqg_adj_revolve_forward: needs free_slots + 1 snapshot slots");
@@ -2186,6 +2236,7 @@ int oqg_adj_revolve_forward(oqg_ctx* c, double tdt, int nsteps, int supp, int fr
 }
 
 int oqg_adj_revolve_reverse(oqg_ctx* c, long long* forward_steps) {
+    ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     if (c->rv_n < 1) return fail(c, OQG_ERR_STATE, "oqg_adj_revolve_reverse: no forward sweep recorded");
@@ -2242,6 +2293,7 @@ struct EnsPlan {
 struct oqg_ens {
     std::vector<oqg_ctx*> m;
     hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;   // recorded after each run; the members' streams wait for it
     int dev = 0;
     std::string err;
     int schedule = 0;          // OQG_ENS_* bits
@@ -2392,7 +2444,7 @@ static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::ve
         EnsLaunch L;
         L.kind = ENS_YS;
         L.R = kv.first;
-        L.lds = std::max(kv.second.second, sizeof(double) * 8 * (size_t)ys_pitch(kv.first));
+        L.lds = std::max(kv.second.second, sizeof(double) * YS_CT * (size_t)ys_pitch(kv.first));
         if (int r = add_launch(L, kv.second.first)) return r;
     }
     for (auto& kv : inv) {
@@ -2457,7 +2509,10 @@ static int ens_run(oqg_ens* e, const double* tdt, const int* nsteps, int step0, 
     for (int i = 0; i < n; i++) {
         if (nsteps[i] < 0) return ens_fail(e, OQG_ERR_ARG, "negative step count");
         maxs = std::max(maxs, nsteps[i]);
-        ENSCHK(e, hipStreamSynchronize(e->m[i]->st));
+        oqg_ctx* c = e->m[i];
+        if (c->ens_done && c->ens_done != e->done) ENSCHK(e, hipStreamWaitEvent(e->st, c->ens_done, 0));   // another ensemble's run
+        c->ens_done = nullptr;
+        ENSCHK(e, hipStreamSynchronize(c->st));
         cur0[i] = e->m[i]->cur;
     }
     const int Pd = supp > 0 ? (supp % 2 == 0 ? supp : 2 * supp) : 2;
@@ -2502,6 +2557,12 @@ static int ens_run(oqg_ens* e, const double* tdt, const int* nsteps, int step0, 
         }
     }
     for (int i = 0; i < n; i++) e->m[i]->cur = cur0[i] ^ (nsteps[i] & 1);
+    // work later enqueued on a member's own stream starts after the ensemble's:
+    // the member's next call makes its stream wait for this event (ctx_join).
+    // Waiting here instead would put a barrier packet in a hardware queue that
+    // other streams may share, holding their work behind this run's end.
+    ENSCHK(e, hipEventRecord(e->done, e->st));
+    for (int i = 0; i < n; i++) e->m[i]->ens_done = e->done;
     return 0;
 }
 
@@ -2519,14 +2580,21 @@ int oqg_ens_create(oqg_ctx* const* members, int n, oqg_ens** out) {
     e->m.assign(members, members + n);
     e->dev = members[0]->dev;
     if (pool_take(&e->st) != hipSuccess) return fail(nullptr, OQG_ERR_HIP, "hipStreamCreate failed");
+    if (hipEventCreateWithFlags(&e->done, hipEventDisableTiming) != hipSuccess) {
+        pool_give(e->dev, e->st);
+        return fail(nullptr, OQG_ERR_HIP, "hipEventCreate failed");
+    }
     *out = e.release();
     return 0;
 }
 
 void oqg_ens_destroy(oqg_ens* e) {
     if (!e) return;
-    ens_clear_plans(e);
+    ens_clear_plans(e);   // synchronises the ensemble stream
+    for (oqg_ctx* c : e->m)
+        if (c->ens_done == e->done) c->ens_done = nullptr;
     pool_give(e->dev, e->st);
+    if (e->done) hipEventDestroy(e->done);
     delete e;
 }
 

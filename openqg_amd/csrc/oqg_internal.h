@@ -21,6 +21,10 @@ constexpr int MAX_FFT = 4096;        // longest x transform (complex points)
 constexpr int VORT_TX = 64;          // vorticity tile (x)
 constexpr int VORT_TY = 16;          // vorticity tile (y)
 constexpr int MAX_YROWS_PER_LANE = 16;  // y-solver: nyp-2 <= 64*16
+// Columns per y-solver workgroup (one wavefront each).  Two per workgroup
+// (768 workgroups at 513^2 instead of 192) measured slower: 12.1 against
+// 10.7 us.
+constexpr int YS_CT = 8;
 
 // Geometry and physics shared by every kernel (by value in kernel arguments).
 struct Geo {
@@ -449,6 +453,8 @@ void launch_invw_ens(const InvArgs* args, const int2* map, int nblocks, int nl, 
     void launch_invw_nl##N(const InvArgs& a, hipStream_t st);                                                    \
     void launch_fwdw_ens_nl##N(const FwdArgs* args, const int2* map, int nblocks, int p, size_t lds, hipStream_t st); \
     void launch_invw_ens_nl##N(const InvArgs* args, const int2* map, int nblocks, int p, bool cyc, size_t lds, hipStream_t st); \
+    void launch_fwdw_adj_nl##N(const FwdArgs& a, hipStream_t st);                                                \
+    void launch_invw_adj_nl##N(const InvArgs& a, hipStream_t st);                                                \
     void set_wave_attrs_nl##N(int lim);
 OQG_WAVE_DECL(2)
 OQG_WAVE_DECL(3)

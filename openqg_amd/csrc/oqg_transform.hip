@@ -24,7 +24,19 @@
 
 namespace oqg {
 
-constexpr int YS_CT = 8;   // columns per y-solver workgroup (one wavefront each)
+// Dispatch to the wavefront-transform objects of a layer count (oqg_wave.hip).
+#define OQG_WAVE_NL_SWITCH(NLV, CALL) \
+    switch (NLV) {                    \
+        case 2: CALL(2); break;       \
+        case 3: CALL(3); break;       \
+        case 4: CALL(4); break;       \
+        case 5: CALL(5); break;       \
+        case 6: CALL(6); break;       \
+        case 7: CALL(7); break;       \
+        case 8: CALL(8); break;       \
+        default: break;               \
+    }
+
 
 // ------------------------------------------------------------------ k_fwd
 
@@ -350,6 +362,12 @@ void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
 }
 
 void launch_fwd_adj(const FwdArgs& a, hipStream_t st) {
+    if (fftw_fits(a.s, a.g.nl)) {
+#define C(N) launch_fwdw_adj_nl##N(a, st)
+        OQG_WAVE_NL_SWITCH(a.g.nl, C)
+#undef C
+        return;
+    }
     const int npairs = (a.s.nyi + 1) / 2;
     const size_t lds = fwd_lds_bytes(a.s);
 #define FWDA_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd_adj<NLc, MBc>), dim3(npairs, a.g.nl), dim3(256), lds, st, a)
@@ -1164,18 +1182,6 @@ size_t fftw_lds_bytes(const Spec& s, int nl) {
 
 bool fftw_fits(const Spec& s, int nl) { return fftw_lds_bytes(s, nl) <= MAX_DYN_LDS; }
 
-#define OQG_WAVE_NL_SWITCH(NLV, CALL) \
-    switch (NLV) {                    \
-        case 2: CALL(2); break;       \
-        case 3: CALL(3); break;       \
-        case 4: CALL(4); break;       \
-        case 5: CALL(5); break;       \
-        case 6: CALL(6); break;       \
-        case 7: CALL(7); break;       \
-        case 8: CALL(8); break;       \
-        default: break;               \
-    }
-
 void launch_fwdw(const FwdArgs& a, hipStream_t st) {
 #define C(N) launch_fwdw_nl##N(a, st)
     OQG_WAVE_NL_SWITCH(a.g.nl, C)
@@ -1225,6 +1231,12 @@ void launch_inv(const InvArgs& a, hipStream_t st) {
 }
 
 void launch_inv_adj(const InvArgs& a, hipStream_t st) {
+    if (fftw_fits(a.s, a.g.nl)) {
+#define C(N) launch_invw_adj_nl##N(a, st)
+        OQG_WAVE_NL_SWITCH(a.g.nl, C)
+#undef C
+        return;
+    }
     const dim3 grid(inv_block_count(a.s)), block(FFT_GROUP * inv_group_count(a.g.nl));
     const size_t lds = inv_lds_bytes(a.s, a.g, a.g.nl);
 #define INVA_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv_adj<NLc, MBc>), grid, block, lds, st, a)

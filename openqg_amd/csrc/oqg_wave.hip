@@ -548,6 +548,369 @@ __global__ __launch_bounds__(64 * (NL + 1)) void k_invw_ens(const InvArgs* __res
     invw_block<NL, P, CYC>(args[e.x], e.y);
 }
 
+// ------------------------------------------------------------------ adjoint forms (box)
+//
+// The adjoint of the box inversion in the same layout.  k_fwdw_adj: the input
+// of mode m is w_m = sum_k ctm2l(m,k) lp_k, lp the adjoint of p completed by
+// the transpose of ocqbdy from the adjoint of q's boundary ring; the row
+// pair's sums of hom_m w_m (over its interior points, the ends of its rows and
+// its share of rows 0 and nyp-1) go to adj_partial[jp][m].  k_invw_adj: the
+// inverse transforms u_m, the constants s_m from the partial sums and
+// cdhoc^-T, and lq_k += f0 sum_m ctl2m(k,m) (u_m + s_m G_m) on the interior.
+
+template <int NL, int P>
+__device__ __forceinline__ void fwdw_adj_block(const FwdArgs& a, int jp) {
+    extern __shared__ double2 wsm[];
+    constexpr int NH = NL > 1 ? NL - 1 : 1;
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L, ZL = zpad_len(L);
+    const int nth = (int)blockDim.x;   // 64 NL
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    double2* tw = wsm + (size_t)NL * ZL;
+    __shared__ double s_red[(NL + 1) * 16];
+    const int r0 = 2 * jp, r1 = r0 + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    Staged<12, double> stw;
+    stage_load(stw, a.tw, 2 * twiddle_table_len(L));
+    const double qb = a.adj_qbfac;
+    const double* lq = a.adj_lq;
+    double cm[NL][NL];   // ctm2l(m, k)
+#pragma unroll
+    for (int m = 0; m < NL; m++)
+#pragma unroll
+        for (int k = 0; k < NL; k++) cm[m][k] = a.dc->mx.ctm2l[m * MAXL + k];
+    // lp, hom and, on the rows next to rows 0 and nyp-1, the adjoint of q on
+    // those rows (the ring neighbours the transpose of ocqbdy gathers), at this
+    // thread's first points of both rows, all loaded before any is used
+    const bool s0 = gj0 == 1, n0 = gj0 == g.nyp - 2, n1 = has1 && gj1 == g.nyp - 2;
+    double p0[WFPT][NL], p1[WFPT][NL], h0[WFPT][NH], h1[WFPT][NH], rq0[WFPT][NL], rq1[WFPT][NL];
+#pragma unroll
+    for (int u = 0; u < WFPT; u++) {
+        const int t = 1 + (int)threadIdx.x + nth * u;
+        const bool in = t <= s.nin;
+        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            p0[u][k] = in ? a.adj_lp[k * g.fs + o0] : 0.0;
+            p1[u][k] = (in && has1) ? a.adj_lp[k * g.fs + o1] : 0.0;
+            rq0[u][k] = (in && (s0 || n0)) ? lq[k * g.fs + o0 + (s0 ? -g.pitch : g.pitch)] : 0.0;
+            rq1[u][k] = (in && n1) ? lq[k * g.fs + o1 + g.pitch] : 0.0;
+        }
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            h0[u][m - 1] = in ? a.adj_hom[(long long)(m - 1) * g.fs + o0] : 0.0;
+            h1[u][m - 1] = (in && has1) ? a.adj_hom[(long long)(m - 1) * g.fs + o1] : 0.0;
+        }
+    }
+    double part[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) part[m] = 0.0;
+    // ring points of the inner products: columns 0 and nxp-1 of the block's
+    // rows, and its share of rows 0 and nyp-1 (2 nxp points dealt out evenly).
+    // A thread's first one is loaded with the batch above: lp and lq at the
+    // point, lq at the corner beside it when its inner point is this point
+    // (columns 1 and nxp-2 of rows 0 and nyp-1), and the homogeneous solutions.
+    const int npairs = (s.nyi + 1) / 2, nring = 2 * g.nxp;
+    const int rper = (nring + npairs - 1) / npairs;
+    const int rlo = min(nring, jp * rper), rhi = min(nring, rlo + rper);
+    const int nside = has1 ? 4 : 2;
+    const int nextra = nside + (rhi - rlo);
+    auto extra_point = [&](int e, int& i, int& gj) {
+        if (e < nside) {
+            gj = (e >> 1) ? gj1 : gj0;
+            i = (e & 1) ? g.nxp - 1 : 0;
+        } else {
+            const int idx = rlo + (e - nside);
+            gj = idx < g.nxp ? 0 : g.nyp - 1;
+            i = idx < g.nxp ? idx : idx - g.nxp;
+        }
+    };
+    {
+        const bool ehas = (int)threadIdx.x < nextra;
+        int ei = 0, ej = 0;
+        if (ehas) extra_point(threadIdx.x, ei, ej);
+        const bool erow = ej == 0 || ej == g.nyp - 1;
+        const bool ewc = ehas && erow && ei == 1, eec = ehas && erow && ei == g.nxp - 2;
+        const long long o = (long long)ej * g.pitch + ei;
+        double elp[NL], elq[NL], elc[NL], ehm[NH];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            elp[k] = ehas ? a.adj_lp[k * g.fs + o] : 0.0;
+            elq[k] = ehas ? lq[k * g.fs + o] : 0.0;
+            elc[k] = (ewc || eec) ? lq[k * g.fs + o + (ewc ? -1 : 1)] : 0.0;
+        }
+#pragma unroll
+        for (int m = 1; m < NL; m++) ehm[m - 1] = ehas ? a.adj_hom[(long long)(m - 1) * g.fs + o] : 0.0;
+        if (ehas) {
+            // bdy_gather at a ring point from the loaded values: the point's
+            // own term (rows and columns of ocqbdy twice at a corner), then
+            // the corner whose inner point it is
+            const Mix& mx = a.dc->mx;
+            const double r = ((ei == 0 || ei == g.nxp - 1) && erow) ? 2.0 : 1.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+#pragma unroll
+                for (int l = 0; l < NL; l++) {
+                    double c = -r * g.fnot * mx.amat[k * MAXL + l];
+                    if (l == k) c -= qb;
+                    elp[l] += elq[k] * c;
+                }
+            }
+            if (ewc || eec)
+#pragma unroll
+                for (int k = 0; k < NL; k++) elp[k] += qb * elc[k];
+#pragma unroll
+            for (int m = 1; m < NL; m++) {
+                double w = 0.0;
+#pragma unroll
+                for (int k = 0; k < NL; k++) w += cm[m][k] * elp[k];
+                part[m] += ehm[m - 1] * w;
+            }
+        }
+    }
+    for (int e = (int)threadIdx.x + nth; e < nextra; e += nth) {   // very wide, short basins
+        int i, gj;
+        extra_point(e, i, gj);
+        const long long o = (long long)gj * g.pitch + i;
+        double lp[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) lp[k] = a.adj_lp[k * g.fs + o];
+        bdy_gather<NL>(g, qb, a.dc->mx, lq, i, gj, lp);   // a ring point
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            double w = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) w += cm[m][k] * lp[k];
+            part[m] += a.adj_hom[(long long)(m - 1) * g.fs + o] * w;
+        }
+    }
+    // the transpose of ocqbdy at an interior point: the ring points whose
+    // inner point it is are its west (t = 1), east (t = nxp-2), south (gj = 1)
+    // and north (gj = nyp-2) neighbours, added in that order; rs/rn: the
+    // loaded south or north neighbours
+    auto gather = [&](int t, int gj, double (&lp)[NL], const double (&rs)[NL]) {
+        const long long o = (long long)gj * g.pitch + t;
+        if (t == 1)
+#pragma unroll
+            for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o - 1];
+        if (t == g.nxp - 2)
+#pragma unroll
+            for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o + 1];
+        if (gj == 1 || gj == g.nyp - 2)
+#pragma unroll
+            for (int k = 0; k < NL; k++) lp[k] += qb * rs[k];
+    };
+    auto put = [&](int t, double (&la)[NL], double (&lb)[NL], const double (&ha)[NH], const double (&hb)[NH],
+                   const double (&ra)[NL], const double (&rb)[NL]) {
+        gather(t, gj0, la, ra);
+        if (has1) gather(t, gj1, lb, rb);
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double wa = 0.0, wb = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                wa += cm[m][k] * la[k];
+                wb += cm[m][k] * lb[k];
+            }
+            if (!has1) wb = 0.0;
+            double2* z = wsm + (size_t)m * ZL;
+            z[zpad(t)] = make_double2(wa, wb);
+            z[zpad(L - t)] = make_double2(-wa, -wb);
+            if (m > 0) part[m] += ha[m - 1] * wa + hb[m - 1] * wb;
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < WFPT; u++) {
+        const int t = 1 + (int)threadIdx.x + nth * u;
+        if (t <= s.nin) put(t, p0[u], p1[u], h0[u], h1[u], rq0[u], rq1[u]);
+    }
+    for (int t = 1 + (int)threadIdx.x + nth * WFPT; t <= s.nin; t += nth) {
+        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
+        double la[NL], lb[NL], ha[NH], hb[NH], ra[NL], rb[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            la[k] = a.adj_lp[k * g.fs + o0];
+            lb[k] = has1 ? a.adj_lp[k * g.fs + o1] : 0.0;
+            ra[k] = (s0 || n0) ? lq[k * g.fs + o0 + (s0 ? -g.pitch : g.pitch)] : 0.0;
+            rb[k] = n1 ? lq[k * g.fs + o1 + g.pitch] : 0.0;
+        }
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            ha[m - 1] = a.adj_hom[(long long)(m - 1) * g.fs + o0];
+            hb[m - 1] = has1 ? a.adj_hom[(long long)(m - 1) * g.fs + o1] : 0.0;
+        }
+        put(t, la, lb, ha, hb, ra, rb);
+    }
+    if (threadIdx.x < NL) {
+        double2* z = wsm + (size_t)threadIdx.x * ZL;
+        z[zpad(0)] = make_double2(0.0, 0.0);
+        z[zpad(L / 2)] = make_double2(0.0, 0.0);
+    }
+    stage_store(stw, (double*)tw, a.tw, 2 * twiddle_table_len(L));
+    block_sum<NL>(part, s_red);   // its barriers also publish the transforms' inputs
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int m = 0; m < NL; m++) a.adj_partial[(long long)jp * MAXL + m] = part[m];
+    double2* z = wsm + (size_t)wv * ZL;
+    fft_wave_any<P, -1>(z, L, s.nstages, s.radix, tw, lane);
+    double* row0 = a.spec + ((long long)wv * s.nrow + r0) * s.kp;
+    double* row1 = row0 + s.kp;
+    for (int K = lane + 1; K <= s.nin; K += 64) {
+        const double2 v = z[zpad(K)];
+        row0[K - 1] = -v.y;
+        if (has1) row1[K - 1] = v.x;
+    }
+}
+
+template <int NL, int P>
+__global__ __launch_bounds__(64 * NL) void k_fwdw_adj(FwdArgs a) {
+    fwdw_adj_block<NL, P>(a, blockIdx.x);
+}
+
+template <int NL, int P>
+__device__ __forceinline__ void invw_adj_block(const InvArgs& a, int jp) {
+    extern __shared__ double2 wsm[];
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L, ZL = zpad_len(L);
+    const int nth = (int)blockDim.x;   // 64 (NL + 1)
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const bool helper = wv == NL;
+    double2* tw = wsm + (size_t)NL * ZL;
+    __shared__ double coef[MAXL];
+    __shared__ double s_ctl2m[MAXL * MAXL];
+    const int r0 = 2 * jp, r1 = r0 + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    Staged<12, double> stw;
+    stage_load(stw, a.twf, 2 * twiddle_table_len(L));
+    constexpr int SP = P / 2 + 1;
+    double sx0[SP], sx1[SP];
+    if (!helper) {
+        const double* row0 = a.spec + ((long long)wv * s.nrow + r0) * s.kp;
+        const double* row1 = row0 + s.kp;
+#pragma unroll
+        for (int u = 0; u < SP; u++) {
+            const int t = lane + 64 * u;
+            const int tc = (t < s.nin) ? t : 0;
+            sx0[u] = row0[tc];
+            sx1[u] = row1[tc];
+        }
+    }
+    // helper wave: the sums over row pairs of the forward kernel's partials, in row-pair order per lane
+    double la[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) la[m] = 0.0;
+    double ctlv = 0.0;
+    if (helper) {
+        if (lane < MAXL * MAXL) ctlv = a.dc->mx.ctl2m[lane];
+        for (int b = lane; b < a.adj_npart; b += 64)
+#pragma unroll
+            for (int m = 0; m < NL; m++) la[m] += a.adj_partial[(long long)b * MAXL + m];
+    }
+    // every thread: G and lq at its first interior points of both rows
+    double gv[2][INV_EPU][NL], lqv[2][INV_EPU][NL];
+#pragma unroll
+    for (int sel = 0; sel < 2; sel++)
+#pragma unroll
+        for (int u = 0; u < INV_EPU; u++) {
+            const int i = (int)threadIdx.x + nth * u;
+            const bool in = i >= 1 && i <= g.nxp - 2 && (sel == 0 || has1);
+            const long long o = (long long)(sel == 0 ? gj0 : gj1) * g.pitch + (in ? i : 0);
+#pragma unroll
+            for (int m = 0; m < NL; m++) {
+                gv[sel][u][m] = in ? a.adj_G[m * g.fs + o] : 0.0;
+                lqv[sel][u][m] = in ? a.adj_lq[m * g.fs + o] : 0.0;
+            }
+        }
+    stage_store(stw, (double*)tw, a.twf, 2 * twiddle_table_len(L));
+    const double fn = s.ftnorm;
+    if (!helper) {
+        double2* z = wsm + (size_t)wv * ZL;
+        if (lane == 0) {
+            z[zpad(0)] = make_double2(0.0, 0.0);
+            z[zpad(L / 2)] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < SP; u++) {
+            const int t = lane + 64 * u;
+            if (t < s.nin) {
+                const double2 w = make_double2(fn * sx0[u], has1 ? fn * sx1[u] : 0.0);
+                z[zpad(t + 1)] = w;
+                z[zpad(L - t - 1)] = make_double2(-w.x, -w.y);
+            }
+        }
+    } else if (lane < MAXL * MAXL) {
+        s_ctl2m[lane] = ctlv;
+    }
+    __syncthreads();
+    if (!helper) {
+        fft_wave_any<P, -1>(wsm + (size_t)wv * ZL, L, s.nstages, s.radix, tw, lane);
+    } else {
+#pragma unroll
+        for (int m = 0; m < NL; m++) la[m] = wave_sum(la[m]);
+        if (lane == 0) {
+            const Mix& mx = a.dc->mx;
+            double lr[NL];
+#pragma unroll
+            for (int l = 0; l < NL - 1; l++) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int j = 0; j < NL - 1; j++) sacc += a.adj_cinvT[l * MAXL + j] * la[j + 1];
+                lr[l] = sacc;
+            }
+#pragma unroll
+            for (int mm = 0; mm < NL; mm++) {
+                double sacc = 0.0;
+#pragma unroll
+                for (int l = 0; l < NL - 1; l++) sacc -= lr[l] * (mx.ctm2l[mm * MAXL + l + 1] - mx.ctm2l[mm * MAXL + l]);
+                coef[mm] = (sacc * g.dx) * g.dy;
+            }
+        }
+    }
+    __syncthreads();
+    // lq_k += f0 sum_m ctl2m(k,m) (u_m + s_m G_m) on the block's interior points
+    auto point = [&](int sel, int i, const double (&G)[NL], const double (&lqk)[NL]) {
+        const long long o = (long long)(sel == 0 ? gj0 : gj1) * g.pitch + i;
+        double r[NL];
+#pragma unroll
+        for (int m = 0; m < NL; m++) r[m] = inh_at<false>(g, s, wsm + (size_t)m * ZL, i, sel) + coef[m] * G[m];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < NL; m++) acc += s_ctl2m[k * MAXL + m] * r[m];
+            a.adj_lq[k * g.fs + o] = lqk[k] + g.fnot * acc;
+        }
+    };
+#pragma unroll
+    for (int sel = 0; sel < 2; sel++) {
+        if (sel == 1 && !has1) break;
+#pragma unroll
+        for (int u = 0; u < INV_EPU; u++) {
+            const int i = (int)threadIdx.x + nth * u;
+            if (i >= 1 && i <= g.nxp - 2) point(sel, i, gv[sel][u], lqv[sel][u]);
+        }
+        for (int i = (int)threadIdx.x + nth * INV_EPU; i <= g.nxp - 2; i += nth) {
+            const long long o = (long long)(sel == 0 ? gj0 : gj1) * g.pitch + i;
+            double G[NL], lqk[NL];
+#pragma unroll
+            for (int m = 0; m < NL; m++) {
+                G[m] = a.adj_G[m * g.fs + o];
+                lqk[m] = a.adj_lq[m * g.fs + o];
+            }
+            point(sel, i, G, lqk);
+        }
+    }
+}
+
+template <int NL, int P>
+__global__ __launch_bounds__(64 * (NL + 1)) void k_invw_adj(InvArgs a) {
+    invw_adj_block<NL, P>(a, blockIdx.x);
+}
+
 // Launchers of this object's layer count, over the points per lane (4, 16, 32).
 #define OQG_WCAT2(a, b) a##b
 #define OQG_WCAT(a, b) OQG_WCAT2(a, b)
@@ -595,6 +958,22 @@ void OQG_WFN(launch_invw_ens_nl)(const InvArgs* args, const int2* map, int nbloc
 #undef INVWE_CALL
 }
 
+void OQG_WFN(launch_fwdw_adj_nl)(const FwdArgs& a, hipStream_t st) {
+    const int npairs = (a.s.nyi + 1) / 2;
+    const size_t lds = fftw_lds_bytes(a.s, WNL);
+#define FWDWA_CALL(Pc) hipLaunchKernelGGL((k_fwdw_adj<WNL, Pc>), dim3(npairs), dim3(64 * WNL), lds, st, a)
+    OQG_P_SWITCH(fftw_points(a.s), FWDWA_CALL)
+#undef FWDWA_CALL
+}
+
+void OQG_WFN(launch_invw_adj_nl)(const InvArgs& a, hipStream_t st) {
+    const int npairs = (a.s.nyi + 1) / 2;
+    const size_t lds = fftw_lds_bytes(a.s, WNL);
+#define INVWA_CALL(Pc) hipLaunchKernelGGL((k_invw_adj<WNL, Pc>), dim3(npairs), dim3(64 * (WNL + 1)), lds, st, a)
+    OQG_P_SWITCH(fftw_points(a.s), INVWA_CALL)
+#undef INVWA_CALL
+}
+
 void OQG_WFN(set_wave_attrs_nl)(int lim) {
 #define ATTR_P(Pc)                                                                                                    \
     (void)hipFuncSetAttribute((const void*)k_fwdw<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);     \
@@ -603,6 +982,11 @@ void OQG_WFN(set_wave_attrs_nl)(int lim) {
     (void)hipFuncSetAttribute((const void*)k_invw<WNL, Pc, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);      \
     (void)hipFuncSetAttribute((const void*)k_invw_ens<WNL, Pc, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
     (void)hipFuncSetAttribute((const void*)k_invw_ens<WNL, Pc, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    ATTR_P(4) ATTR_P(16) ATTR_P(32)
+#undef ATTR_P
+#define ATTR_P(Pc)                                                                                                    \
+    (void)hipFuncSetAttribute((const void*)k_fwdw_adj<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);  \
+    (void)hipFuncSetAttribute((const void*)k_invw_adj<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     ATTR_P(4) ATTR_P(16) ATTR_P(32)
 #undef ATTR_P
 }

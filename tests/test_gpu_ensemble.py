@@ -112,3 +112,20 @@ def test_ensemble_groups_equal_members_stepped_alone(gpu, solo):
             assert np.array_equal(a, b), f"member {i} {name}"
         assert np.allclose(qg.get_constr(), con, rtol=1e-10, atol=0.0)
     grp.close()
+
+
+def test_member_read_after_run_without_sync(gpu):
+    """A member read on its own stream right after Ensemble.run (no sync) sees
+    the run's result: the run ends by making every member's stream wait for
+    the ensemble's."""
+    ws = _members()
+    nsteps = [20, 20, 20, 20, 20, 20]
+    ref = _alone(ws, nsteps, 25)
+    qgs = _make(ws)
+    ens = Ensemble(qgs)
+    ens.run([w.tdt for w in ws], nsteps, 0, 25)
+    for i, (qg, (st, _)) in enumerate(zip(qgs, ref)):
+        got = qg.get_state()
+        for name, a, b in zip(("q", "qm", "p", "pm"), got, st):
+            assert np.array_equal(a, b), f"member {i} {name}"
+    ens.close()
