@@ -1,10 +1,9 @@
-"""The QG-only run loop: clock, step cadence, validity checks and restart dumps.
+"""The run loop: clock, step cadence, validity checks and restart dumps.
 
 Mirrors `run` (src/openqg.F90:452-512) with `dynamic_step` (src/driver.F90:36-87)
 and the validity part of `diagnostic_step` (src/driver.F90:191-264) for an
 ocean basin, an atmosphere basin or both, each a `QG` whose forcing is
-resident on the device (`QG.set_forcing` / `QG.set_wind`).  The mixed layers
-and the coupled exchanges of `dynamic_step` are not part of this package, so
+resident on the device (`QG.set_forcing` / `QG.set_wind`).  Without a coupler
 the two basins do not interact and each advances on its own stream of steps:
 
 - atmosphere: one step of tdta = 2 dta every atmosphere step nt, then
@@ -16,6 +15,10 @@ Between two steps at which something happens on the host (a validity check,
 a monitor sample, a restart dump, the end of the run) every basin replays its
 steps as cached HIP graphs (`QG.run`), so the host touches the state only at
 those steps.  At the other steps the result equals stepping one at a time.
+
+With a coupler (`openqg_amd.coupled.Coupler`) the same cadence runs through
+`oqg_cpl_run`: both mixed layers and the exchanges of `dynamic_step` join the
+steps, replayed from one captured graph between host events.
 """
 from __future__ import annotations
 
@@ -114,7 +117,7 @@ class Basin:
 
 def run(clk: Clock, outdir: str, ocn: Optional[Basin] = None, atm: Optional[Basin] = None,
         monitor: Optional[Callable[[str, int, dict], None]] = None, coupler=None) -> int:
-    """The run loop of src/openqg.F90:479-512 for QG-only basins.
+    """The run loop of src/openqg.F90:479-512.
 
     Writes `<name>_qg_restart.nc` every clk.noutre steps and
     `<name>_qg_lastday.nc` at the end (src/openqg.F90:503-512).  When a
