@@ -47,16 +47,19 @@ __global__ void k_adj_add(double* dst, const double* src, long long n) {
         dst[t] += src[t];
 }
 
-// Segment blockIdx.y; two 16-byte vectors per thread and pass, loaded together.
+// One flat grid over all segments: segment s owns blocks [a.b0[s], a.b0[s+1]),
+// each thread two 16-byte vectors, loaded together (every segment starts on a
+// 16-byte boundary and holds an even number of doubles).
 __global__ __launch_bounds__(256) void k_copy_segs(CopySegs a) {
-    const int s = blockIdx.y;
-    const long long stride = 2LL * gridDim.x * blockDim.x;
+    int s = 0;
+    while (s + 1 < a.nseg && (int)blockIdx.x >= a.b0[s + 1]) s++;
     double* slot = a.slots ? a.slots[*a.base + a.t] : nullptr;
     const double2* src = (const double2*)a.src[s];
     double2* dst = (double2*)(slot ? slot + a.doff[s] : a.dst[s]);
     const long long n2 = a.n[s] >> 1;
-    for (long long t = blockIdx.x * 2LL * blockDim.x + threadIdx.x; t < n2; t += stride) {
-        const long long t1 = t + blockDim.x;
+    const long long t = ((long long)blockIdx.x - a.b0[s]) * 2LL * blockDim.x + threadIdx.x;
+    const long long t1 = t + blockDim.x;
+    if (t < n2) {
         const double2 v0 = src[t];
         const double2 v1 = src[t1 < n2 ? t1 : t];
         dst[t] = v0;
@@ -303,10 +306,15 @@ __global__ void k_set_int(int* p, int v) {
 
 void launch_set_int(int* p, int v, hipStream_t st) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, p, v); }
 
-void launch_copy_segs(const CopySegs& a, hipStream_t st) {
-    long long mx = 0;
-    for (int s = 0; s < a.nseg; s++) mx = a.n[s] > mx ? a.n[s] : mx;
-    hipLaunchKernelGGL(k_copy_segs, dim3(blocks_for((mx / 2 + 1) / 2), a.nseg), dim3(256), 0, st, a);
+void launch_copy_segs(const CopySegs& a0, hipStream_t st) {
+    CopySegs a = a0;
+    int nb = 0;
+    for (int s = 0; s < a.nseg; s++) {
+        a.b0[s] = nb;
+        nb += (int)((a.n[s] / 2 + 511) / 512);   // 512 vectors per block
+    }
+    a.b0[a.nseg] = nb;
+    hipLaunchKernelGGL(k_copy_segs, dim3(nb), dim3(256), 0, st, a);
 }
 
 void launch_adj_vort(const AdjVortArgs& a, hipStream_t st) {

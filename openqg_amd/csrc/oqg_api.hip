@@ -1416,9 +1416,11 @@ static RunSplit run_split(int nsteps, int step0, int P) {
 }
 
 // A head or tail stretch: its planned graph when oqg_plan_run captured one,
-// otherwise eager launches (a one-step graph measured slower than its four
-// launches, and arbitrary stretches, as in checkpointed sweeps, are not worth
-// a capture each).  Leaves c->cur advanced by n.
+// otherwise eager launches.  A one-step graph measured slower than its four
+// launches, and graphs of 2^k suppression-free steps for the stretches of a
+// checkpointed sweep measured no faster than eager launches (the sweep's
+// steps are bound by the kernels' own time, not by their launches).  Leaves
+// c->cur advanced by n.
 static int run_stretch(oqg_ctx* c, double tdt, int nt0, int n, int supp, int P, bool capture) {
     if (n <= 0) return 0;
     const GraphKey key{tdt, n, supp, c->cur, G_SEG, (nt0 - 1) % P};

@@ -399,6 +399,7 @@ struct CopySegs {
     const int* base;
     int t;
     long long doff[6];
+    int b0[7];   // first block of each segment (set by launch_copy_segs)
 };
 void launch_copy_segs(const CopySegs& a, hipStream_t st);
 void launch_set_int(int* p, int v, hipStream_t st);

@@ -360,3 +360,40 @@ def test_planned_run_equals_eager(gpu):
         for a, b in zip(outs[0][0], outs[1][0]):
             assert np.array_equal(a, b), (n, s0, supp)
         assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_unaligned_stretches_equal_eager(gpu):
+    """oqg_run over stretches that are not whole suppression periods (the
+    advances of a checkpointed sweep), chained from every phase and both
+    parities, must equal stepping one step at a time, bitwise, and so must
+    a second context running the same chain."""
+    w = W.small_box(65, 3, seed=11)
+    p = w.initial_p()
+    ent, wekp, txis, txin = w.forcing()
+    # (step0, n) pairs chained one after another: every phase of supp = 25,
+    # odd and even lengths, single steps, stretches across a whole period
+    runs = [(0, 3), (3, 1), (4, 30), (34, 47), (81, 2), (83, 19), (102, 64), (166, 5)]
+    supp = w.supp_every
+    ref = None
+    for mode in ("eager", "run", "run again"):
+        qg = w.make_qg()
+        qg.set_state(p=p, pm=p)
+        qg.init_state()
+        qg.set_forcing(ent, wekp, txis, txin)
+        for s0, n in runs:
+            if mode == "eager":
+                for nt in range(s0 + 1, s0 + n + 1):
+                    qg.step(w.tdt)
+                    if (nt - 1) % supp == 0:
+                        qg.suppress()
+            else:
+                qg.run(w.tdt, n, s0, supp)
+        qg.sync()
+        out = (qg.get_state(), qg.get_constr())
+        qg.close()
+        if ref is None:
+            ref = out
+            continue
+        for a, b in zip(ref[0], out[0]):
+            assert np.array_equal(a, b), mode
+        assert np.array_equal(ref[1], out[1]), mode
