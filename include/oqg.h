@@ -200,6 +200,12 @@ int oqg_adj_get(oqg_ctx* ctx, double* lq, double* lqm, double* lp, double* lpm, 
 int oqg_adj_step(oqg_ctx* ctx, double tdt, int suppressed);
 int oqg_adj_get_wekp(oqg_ctx* ctx, double* out, int kind);
 int oqg_adj_zero_wekp(oqg_ctx* ctx);
+/* Seeds the adjoint with the layer-1 kinetic energy of the loaded state,
+ * J = ekei(1) = 0.5 rho H_1 int_P_dA(u^2 + v^2), u = -dPdy_2bc(p_1)/f0,
+ * v = dPdx_2bc(p_1)/f0 (qg_monitor.F90:297-302,331-333): the adjoints of q,
+ * qm and pm are zeroed and that of p is dJ/dp (layer 1; 0 below), all on the
+ * device; *J (host) receives the objective (box basins). */
+int oqg_adj_seed_ke(oqg_ctx* ctx, double rho, double* J);
 
 /* Wind-stress forcing on the device: ekman_from_tau (src/coupler/ekman.F90:53-83)
  * with map_P_to_x_face / map_P_to_y_face and map_T_to_P
@@ -243,6 +249,7 @@ typedef struct oqg_diag {
     double et2m[OQG_MAX_LAYERS];    /* avg_P(eta_k^2) */
     double ctot;                    /* sum of circ */
     double nonfinite;               /* number of non-finite values of p and q */
+    double pint[OQG_MAX_LAYERS];    /* int_P_dA(p_k) (check_continuity's int_p, qg_monitor.F90:379) */
 } oqg_diag;
 
 int oqg_diagnose(oqg_ctx* ctx, double dt, double rho, oqg_diag* out);
@@ -259,6 +266,35 @@ int oqg_diagnose(oqg_ctx* ctx, double dt, double rho, oqg_diag* out);
  * calls that overlap on one GPU (from several threads or streams) race, so
  * serialise them. */
 int oqg_diagnose_noalloc(oqg_ctx* ctx, double dt, double rho, oqg_diag* out);
+
+/*
+ * The rest of qg_diagno's energetics (src/diagnostics/qg_monitor.F90:219-383),
+ * computed on the device from the resident state and forcing: the lagged
+ * geostrophic velocities ug = -dPdy_2bc(pm)/f0, vg = dPdx_2bc(pm)/f0, the
+ * current ones ugeos, vgeos from p, del2 = del2_P_bc and del4 = del2_P_bc of
+ * del2 (numerics.F90:410-513), integrals by int_P_dA / int_P_dx.  The
+ * reference declares vg (nxt,nyp,nl) (qg_monitor.F90:228) and assigns and
+ * passes (nxp,nyp) fields through it, so its v terms of ah2d, ah4d and btdg
+ * read past the array; here vg is (nxp,nyp), the shape of every field it is
+ * combined with.  ujeto is taken over rows 1..nyt as its declaration
+ * (:242) gives maxloc.  pket needs the previous sample's et2m and time: a
+ * caller forms it from two oqg_diagnose results (0.5 rho gp_k (et2m - et2m')
+ * / (tsec - tsec'), :266-272).
+ */
+typedef struct oqg_energy {
+    double ah2d[OQG_MAX_LAYERS];    /* -rho ah2_k H_k int(ugeos del2(ug) + vgeos del2(vg)) dA */
+    double ah4d[OQG_MAX_LAYERS];    /* rho ah4_k H_k int(ugeos del4(ug) + vgeos del4(vg)) dA */
+    double val[OQG_MAX_LAYERS];     /* max over rows j = 1..nyt of |int_P_dx(ugeos(:,j))| / xl */
+    double pos[OQG_MAX_LAYERS];     /* the first row j (1-based) where it is reached */
+    double btdg;                    /* 0.5 rho delek |f0| int(ug^2 + vg^2) dA in layer k_topo */
+    double utau;                    /* rho int(taux ugeos_1 + tauy vgeos_1) dA, the resident
+                                       stress of oqg_set_wind (0 before any) */
+    double wepm, wapm;              /* avg_P(wekp), avg_P(|wekp|) */
+    double entm, enam;              /* avg_P(ent_1), avg_P(|ent_1|), the resident entrainment */
+    double pken;                    /* rho gp_1 avg_P(eta_1 ent_1) (0 for one layer) */
+} oqg_energy;
+
+int oqg_diagnose_energy(oqg_ctx* ctx, double rho, oqg_energy* out);
 
 /* The inversion of one step applied to the current q (interior): p from q
  * (modes, Helmholtz solves, mass correction with the current dpi), q's

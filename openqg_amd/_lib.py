@@ -48,7 +48,14 @@ class OqgDiag(ctypes.Structure):
     """Mirror of `oqg_diag` (include/oqg.h)."""
 
     _fields_ = [(n, ctypes.c_double * MAX_LAYERS) for n in _DIAG_LAYER_FIELDS] + \
-               [("ctot", ctypes.c_double), ("nonfinite", ctypes.c_double)]
+               [("ctot", ctypes.c_double), ("nonfinite", ctypes.c_double), ("pint", ctypes.c_double * MAX_LAYERS)]
+
+
+class OqgEnergy(ctypes.Structure):
+    """Mirror of `oqg_energy` (include/oqg.h)."""
+
+    _fields_ = [(n, ctypes.c_double * MAX_LAYERS) for n in ("ah2d", "ah4d", "val", "pos")] + \
+               [(n, ctypes.c_double) for n in ("btdg", "utau", "wepm", "wapm", "entm", "enam", "pken")]
 
 
 class OqgOmlDesc(ctypes.Structure):
@@ -113,6 +120,8 @@ SIGNATURES = [
     ("oqg_get_wind", _i, [_vp, _vp, _vp, _i]),
     ("oqg_diagnose", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
     ("oqg_diagnose_noalloc", _i, [_vp, _d, _d, ctypes.POINTER(OqgDiag)]),
+    ("oqg_diagnose_energy", _i, [_vp, _d, ctypes.POINTER(OqgEnergy)]),
+    ("oqg_adj_seed_ke", _i, [_vp, _d, _dp]),
     ("oqg_invert", _i, [_vp]),
     ("oqg_adj_invert", _i, [_vp]),
     ("oqg_snap_alloc", _i, [_vp, _i]),

@@ -24,6 +24,7 @@ from typing import Optional
 
 import numpy as np
 
+from . import _lib
 from ._lib import OQG_HOST, check
 from .qg import QG
 
@@ -133,6 +134,13 @@ class AdjointState:
         keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (lq, lqm, lp, lpm)]
         ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in keep]
         self._c(self._lib.oqg_adj_set(self.qg._h, *ptrs, OQG_HOST), "oqg_adj_set")
+
+    def seed_ke(self, rho: float) -> float:
+        """J = ekei(1) of the loaded state and the adjoint seeded with dJ/dp
+        (oqg_adj_seed_ke, on the device)."""
+        J = ctypes.c_double()
+        self._c(self._lib.oqg_adj_seed_ke(self.qg._h, float(rho), ctypes.byref(J)), "oqg_adj_seed_ke")
+        return J.value
 
     def get(self):
         out = [np.empty(self.qg.shape) for _ in range(4)]
@@ -332,11 +340,7 @@ def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: fl
     adj.invert()
     traj = Trajectory(adj, tdt, nsteps, supp_every, snapshots)
     traj.forward()
-    p = qg.p
-    J, gp1 = ekei1(p[0], b.dx, b.dy, b.fnot, qg.bcco, rho, b.h[0])
-    lp = np.zeros(qg.shape)
-    lp[0] = gp1
-    adj.set(lp=lp)
+    J = adj.seed_ke(rho)
     adj.zero_wekp()
     traj.reverse()
     gtx, gty = adj.wind()     # the transpose of the wind forcing on the device
@@ -379,8 +383,9 @@ def objective(qg: QG, q0, taux, tauy, tdt: float, nsteps: int, supp_every: int =
     qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
     check(qg._lib.oqg_invert(qg._h), qg._h, "oqg_invert")
     qg.run(tdt, nsteps, 0, supp_every)
-    qg.sync()
-    return ekei1(qg.p[0], b.dx, b.dy, b.fnot, qg.bcco, rho, b.h[0])[0]
+    d = _lib.OqgDiag()
+    check(qg._lib.oqg_diagnose(qg._h, 0.5 * float(tdt), float(rho), ctypes.byref(d)), qg._h, "oqg_diagnose")
+    return d.ekei[0]   # the same device sums as the gradient's objective
 
 
 def make_autograd(qg: QG, tdt: float, nsteps: int, supp_every: int = 25, snapshots: Optional[int] = None):

@@ -67,6 +67,64 @@ __global__ __launch_bounds__(256) void k_copy_segs(CopySegs a) {
     }
 }
 
+// ------------------------------------------------------------------ k_ke_seed
+
+// dPdy_2bc / dPdx_2bc of a box field at (i, j) (numerics.F90:470-513).
+__device__ __forceinline__ double ke_dpdx(const double* P, long long pitch, int i, int j, const Geo& g, double ca) {
+    if (j == 0 || j == g.nyp - 1) return 0.0;
+    const long long o = (long long)j * pitch + i;
+    if (i >= 1 && i <= g.nxp - 2) return (P[o + 1] - P[o - 1]) / (2.0 * g.dx);
+    return i == 0 ? (P[o + 1] - P[o]) / (g.dx * ca) : (P[o] - P[o - 1]) / (g.dx * ca);
+}
+
+__device__ __forceinline__ double ke_dpdy(const double* P, long long pitch, int i, int j, const Geo& g, double ca) {
+    if (i == 0 || i == g.nxp - 1) return 0.0;
+    const long long o = (long long)j * pitch + i;
+    if (j >= 1 && j <= g.nyp - 2) return (P[o + pitch] - P[o - pitch]) / (2.0 * g.dy);
+    return j == 0 ? (P[o + pitch] - P[o]) / (g.dy * ca) : (P[o] - P[o - pitch]) / (g.dy * ca);
+}
+
+// dJ/dp_1 for J = 0.5 rho H_1 sum w (u^2 + v^2), in gather form: point (i, j)
+// collects, from every point whose derivative stencil reads it, the stencil's
+// coefficient times gu = rho H_1 w u (resp. gv = rho H_1 w v) there; then
+// grad = -(d/dy)^T gu / f0 + (d/dx)^T gv / f0.
+__global__ __launch_bounds__(256) void k_ke_seed(KeSeedArgs a) {
+    const Geo& g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= g.nxp) return;
+    const long long pitch = g.pitch;
+    const double ca = 0.5 * a.bcco + 1.0;
+    auto wgt = [&](int x, int y) {
+        const double wx = (x == 0 || x == g.nxp - 1) ? 0.5 : 1.0, wy = (y == 0 || y == g.nyp - 1) ? 0.5 : 1.0;
+        return wx * wy * g.dx * g.dy;
+    };
+    auto GU = [&](int x, int y) { return a.rhoh * wgt(x, y) * (-ke_dpdy(a.p, pitch, x, y, g, ca) / g.fnot); };
+    auto GV = [&](int x, int y) { return a.rhoh * wgt(x, y) * (ke_dpdx(a.p, pitch, x, y, g, ca) / g.fnot); };
+    const double cx = g.dx * ca, cy = g.dy * ca;
+    double tx = 0.0, ty = 0.0;
+    if (j != 0 && j != g.nyp - 1) {   // (d/dx)^T: rows 0 and nyp-1 of d/dx are zero
+        if (i >= 2) tx += GV(i - 1, j) / (2.0 * g.dx);
+        if (i <= g.nxp - 3) tx -= GV(i + 1, j) / (2.0 * g.dx);
+        if (i == 1) tx += GV(0, j) / cx;
+        if (i == 0) tx -= GV(0, j) / cx;
+        if (i == g.nxp - 1) tx += GV(g.nxp - 1, j) / cx;
+        if (i == g.nxp - 2) tx -= GV(g.nxp - 1, j) / cx;
+    }
+    if (i != 0 && i != g.nxp - 1) {   // (d/dy)^T: columns 0 and nxp-1 of d/dy are zero
+        if (j >= 2) ty += GU(i, j - 1) / (2.0 * g.dy);
+        if (j <= g.nyp - 3) ty -= GU(i, j + 1) / (2.0 * g.dy);
+        if (j == 1) ty += GU(i, 0) / cy;
+        if (j == 0) ty -= GU(i, 0) / cy;
+        if (j == g.nyp - 1) ty += GU(i, g.nyp - 1) / cy;
+        if (j == g.nyp - 2) ty -= GU(i, g.nyp - 1) / cy;
+    }
+    a.lp[(long long)j * pitch + i] = -ty / g.fnot + tx / g.fnot;
+}
+
+void launch_ke_seed(const KeSeedArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_ke_seed, dim3((a.g.nxp + 255) / 256, a.g.nyp), dim3(256), 0, st, a);
+}
+
 // ------------------------------------------------------------------ k_adj_vort
 
 // Coefficient that del2_P_bc at x (box) gives to f at x + o, for the five

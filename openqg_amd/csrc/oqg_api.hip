@@ -117,6 +117,8 @@ struct oqg_ctx {
     double* taux_d = nullptr;   // kinematic wind stress at p points (oqg_set_wind / oqg_adj_wind)
     double* tauy_d = nullptr;
     double* diag_part = nullptr;   // per-row diagnostics partials (oqg_diagnose)
+    double* en_d2 = nullptr;       // oqg_diagnose_energy: del2 of the lagged velocities (2 nl fields)
+    double* en_part = nullptr;     // and its per-row partials
     int dev = 0;                   // the HIP device the context was created on
     double* scratch = nullptr;     // this device's address of the no-alloc scratch array
     hipStream_t own = nullptr;
@@ -1266,8 +1268,48 @@ static void diag_finish(const oqg_ctx* c, const double* comb, double dt, double 
             out->etam[k] = (sum(3, k) * dA) / area;
             out->et2m[k] = (sum(4, k) * dA) / area;
         }
+        out->pint[k] = sum(1, k) * dA;
     }
     out->nonfinite = comb[DC_BAD];
+}
+
+// qg_diagno's energetics from the rows' partials, combined in row order.
+static void energy_finish(const oqg_ctx* c, const double* h, double rho, oqg_energy* out) {
+    const Geo& g = c->g;
+    const int nl = g.nl, ny = g.nyp;
+    const double dA = g.dx * g.dy, xl = (g.nxp - 1) * g.dx, area = xl * ((ny - 1) * g.dy);
+    std::memset(out, 0, sizeof *out);
+    auto total = [&](int slot) {
+        double s = 0.0;
+        for (int j = 0; j < ny; j++) s += h[(size_t)j * EN_LEN + slot];
+        return s;
+    };
+    for (int k = 0; k < nl; k++) {
+        const double u2diss = total(EN_U2 * MAXL + k) * dA, v2diss = total(EN_V2 * MAXL + k) * dA;
+        const double u4diss = total(EN_U4 * MAXL + k) * dA, v4diss = total(EN_V4 * MAXL + k) * dA;
+        out->ah2d[k] = -rho * c->d.ah2[k] * c->d.h[k] * (u2diss + v2diss);
+        out->ah4d[k] = rho * c->d.ah4[k] * c->d.h[k] * (u4diss + v4diss);
+        // maxloc / maxval of |int_P_dx(ugeos(:,j))| / xl over rows 1..nyt
+        double best = -1.0;
+        int pos = 0;
+        for (int j = 0; j < ny - 1; j++) {
+            const double ujet = h[(size_t)j * EN_LEN + EN_UJET * MAXL + k] * g.dx;
+            const double v = std::fabs(ujet) / xl;
+            if (v > best) {
+                best = v;
+                pos = j + 1;
+            }
+        }
+        out->val[k] = best;
+        out->pos[k] = pos;
+    }
+    out->btdg = 0.5 * rho * c->d.delek * std::fabs(g.fnot) * (total(EN_BTU) * dA + total(EN_BTV) * dA);
+    out->utau = rho * (total(EN_VTY) * dA + total(EN_UTX) * dA);
+    out->wepm = (total(EN_WEK) * dA) / area;
+    out->wapm = (total(EN_AWEK) * dA) / area;
+    out->entm = (total(EN_ENT) * dA) / area;
+    out->enam = (total(EN_AENT) * dA) / area;
+    out->pken = nl > 1 ? rho * c->d.gp[0] * ((total(EN_ETAENT) * dA) / area) : 0.0;
 }
 
 extern "C" {
@@ -1288,6 +1330,37 @@ int oqg_diagnose(oqg_ctx* c, double dt, double rho, oqg_diag* out) {
     double comb[DG_COMB_LEN];
     diag_combine_host(h.data(), ny, nl, comb);   // the rows in order (deterministic)
     diag_finish(c, comb, dt, rho, out);
+    return 0;
+}
+
+int oqg_diagnose_energy(oqg_ctx* c, double rho, oqg_energy* out) {
+    ctx_join(c);
+    if (!c || !out) return OQG_ERR_ARG;
+    const Geo& g = c->g;
+    if (!c->en_d2)
+        if (int r = dalloc(c, &c->en_d2, 2 * (size_t)g.nl * g.fs)) return r;
+    if (!c->en_part)
+        if (int r = dalloc(c, &c->en_part, (size_t)g.nyp * EN_LEN)) return r;
+    EnergyArgs a{};
+    a.g = g;
+    a.bcco = c->d.bcco;
+    a.gp0 = g.nl > 1 ? c->d.gp[0] : 1.0;
+    a.dz_sign = c->d.dz_sign;
+    a.p = c->pbuf[c->cur];
+    a.pm = c->pbuf[1 - c->cur];
+    a.wekp = c->wekp;
+    a.ent1 = c->ent1;
+    a.taux = c->taux_d;
+    a.tauy = c->taux_d ? c->tauy_d : nullptr;
+    a.d2u = c->en_d2;
+    a.d2v = c->en_d2 + (size_t)g.nl * g.fs;
+    a.partial = c->en_part;
+    launch_energy(a, c->st);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h((size_t)g.nyp * EN_LEN);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->en_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    energy_finish(c, h.data(), rho, out);
     return 0;
 }
 
@@ -1848,6 +1921,27 @@ int oqg_adj_zero_wekp(oqg_ctx* c) {
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     HIPCHK(c, hipMemsetAsync(c->lwek, 0, (size_t)c->g.fs * sizeof(double), c->st));
+    return 0;
+}
+
+int oqg_adj_seed_ke(oqg_ctx* c, double rho, double* J) {
+    ctx_join(c);
+    if (!c || !J) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    oqg_diag d;
+    if (int r = oqg_diagnose(c, 1.0, rho, &d)) return r;   // ekei from the device's row sums
+    *J = d.ekei[0];
+    const int cur = c->cur;
+    for (double* f : {c->lq[cur], c->lq[1 - cur], c->lp[cur], c->lp[1 - cur]})
+        HIPCHK(c, hipMemsetAsync(f, 0, c->fbytes, c->st));
+    KeSeedArgs a{};
+    a.g = c->g;
+    a.bcco = c->d.bcco;
+    a.rhoh = rho * c->d.h[0];
+    a.p = c->pbuf[cur];
+    a.lp = c->lp[cur];
+    launch_ke_seed(a, c->st);
+    HIPCHK(c, hipGetLastError());
     return 0;
 }
 

@@ -10,6 +10,8 @@
 // numerics.F90:291-309; qg_monitor.F90:507-569); and a count of non-finite
 // values of p and q (the check valids makes, valsubs.F90:46-139, extended to
 // NaN and Inf).  One block per row; the host combines the rows in order.
+// The energetics from the lagged velocities, the stress and the entrainment
+// (k_energy_d2, k_energy) follow the same row scheme.
 
 #include <mutex>
 
@@ -184,6 +186,158 @@ __global__ __launch_bounds__(256) void k_diag(DiagArgs a) {
 
 void launch_diag(const DiagArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_diag, dim3(a.g.nyp), dim3(256), 0, st, a);
+}
+
+// ------------------------------------------------------------------ energetics
+//
+// Point values of the reference's derivative operators, each written as its
+// whole-field form assigns that point (the row or column zeroed last wins):
+// dPdx_2bc, dPdy_2bc (numerics.F90:470-513), dP2dx2_bc, dP2dy2_bc (:427-468).
+// f(i, j) gives the field's value at any point of the basin.
+
+template <class F>
+__device__ __forceinline__ double en_dpdx(F f, int i, int j, const Geo& g, double ca) {
+    if (j == 0 || j == g.nyp - 1) return 0.0;
+    if (i >= 1 && i <= g.nxp - 2) return (f(i + 1, j) - f(i - 1, j)) / (2.0 * g.dx);
+    if (!g.cyclic) return i == 0 ? (f(1, j) - f(0, j)) / (g.dx * ca) : (f(i, j) - f(i - 1, j)) / (g.dx * ca);
+    return (f(1, j) - f(g.nxp - 2, j)) / (2.0 * g.dx);
+}
+
+template <class F>
+__device__ __forceinline__ double en_dpdy(F f, int i, int j, const Geo& g, double ca) {
+    if (!g.cyclic && (i == 0 || i == g.nxp - 1)) return 0.0;
+    if (j >= 1 && j <= g.nyp - 2) return (f(i, j + 1) - f(i, j - 1)) / (2.0 * g.dy);
+    if (j == 0) return (f(i, 1) - f(i, 0)) / (g.dy * ca);
+    return (f(i, j) - f(i, j - 1)) / (g.dy * ca);
+}
+
+// del2_P_bc at (i, j): dP2dx2_bc + dP2dy2_bc with alpha = bcco
+template <class F>
+__device__ __forceinline__ double en_del2(F f, int i, int j, const Geo& g, double alpha) {
+    const double bfac = alpha / (0.5 * alpha + 1.0);
+    double dxx = 0.0, dyy = 0.0;
+    if (j != 0 && j != g.nyp - 1) {
+        const double fac = 1.0 / (g.dx * g.dx);
+        if (i >= 1 && i <= g.nxp - 2) dxx = fac * ((f(i - 1, j) + f(i + 1, j)) - 2.0 * f(i, j));
+        else if (!g.cyclic) dxx = i == 0 ? fac * bfac * (f(1, j) - f(0, j)) : -fac * bfac * (f(i, j) - f(i - 1, j));
+        else dxx = fac * ((f(g.nxp - 2, j) + f(1, j)) - 2.0 * f(0, j));
+    }
+    if (g.cyclic || (i != 0 && i != g.nxp - 1)) {
+        const double fac = 1.0 / (g.dy * g.dy);
+        if (j >= 1 && j <= g.nyp - 2) dyy = fac * ((f(i, j - 1) + f(i, j + 1)) - 2.0 * f(i, j));
+        else if (j == 0) dyy = fac * bfac * (f(i, 1) - f(i, 0));
+        else dyy = -fac * bfac * (f(i, j) - f(i, j - 1));
+    }
+    return dxx + dyy;
+}
+
+// del2 of the lagged velocities, one point per thread: d2u = del2(ug), d2v =
+// del2(vg) with ug = -dPdy_2bc(pm)/f0, vg = dPdx_2bc(pm)/f0 evaluated at the
+// stencil's points.
+__global__ __launch_bounds__(256) void k_energy_d2(EnergyArgs a) {
+    const Geo& g = a.g;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= g.nxp) return;
+    const double ca = 0.5 * a.bcco + 1.0;
+    const double* pm = a.pm + k * g.fs;
+    const long long pitch = g.pitch;
+    auto P = [&](int x, int y) { return pm[(long long)y * pitch + x]; };
+    auto ug = [&](int x, int y) { return -en_dpdy(P, x, y, g, ca) / g.fnot; };
+    auto vg = [&](int x, int y) { return en_dpdx(P, x, y, g, ca) / g.fnot; };
+    const long long o = k * g.fs + (long long)j * pitch + i;
+    a.d2u[o] = en_del2(ug, i, j, g, a.bcco);
+    a.d2v[o] = en_del2(vg, i, j, g, a.bcco);
+}
+
+// One block per row j: the row's partials (EN_* slots), layers one after
+// another, block sums in a fixed order.
+__global__ __launch_bounds__(256) void k_energy(EnergyArgs a) {
+    const Geo& g = a.g;
+    const int nx = g.nxp, ny = g.nyp, nl = g.nl, j = blockIdx.x;
+    const long long pitch = g.pitch;
+    __shared__ double red[9 * 16];
+    const double wj = (j == 0 || j == ny - 1) ? 0.5 : 1.0;
+    const double ca = 0.5 * a.bcco + 1.0;
+    double* out = a.partial + (long long)j * EN_LEN;
+#pragma unroll 1
+    for (int k = 0; k < nl; k++) {
+        const double* p = a.p + k * g.fs;
+        const double* pm = a.pm + k * g.fs;
+        const double* d2u = a.d2u + k * g.fs;
+        const double* d2v = a.d2v + k * g.fs;
+        auto P = [&](int x, int y) { return p[(long long)y * pitch + x]; };
+        auto PM = [&](int x, int y) { return pm[(long long)y * pitch + x]; };
+        auto D2U = [&](int x, int y) { return d2u[(long long)y * pitch + x]; };
+        auto D2V = [&](int x, int y) { return d2v[(long long)y * pitch + x]; };
+        const bool first = k == 0, bottom = k == g.kt;
+        double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+            const double wi = (i == 0 || i == nx - 1) ? 0.5 : 1.0, w = wj * wi;
+            const long long o = (long long)j * pitch + i;
+            const double ugeos = -en_dpdy(P, i, j, g, ca) / g.fnot;
+            const double vgeos = en_dpdx(P, i, j, g, ca) / g.fnot;
+            const double d4u = en_del2(D2U, i, j, g, a.bcco);
+            const double d4v = en_del2(D2V, i, j, g, a.bcco);
+            v[0] += w * (ugeos * d2u[o]);
+            v[1] += w * (vgeos * d2v[o]);
+            v[2] += w * (ugeos * d4u);
+            v[3] += w * (vgeos * d4v);
+            v[4] += wi * ugeos;
+            if (first && a.taux) {
+                v[5] += w * (a.taux[o] * ugeos);
+                v[6] += w * (a.tauy[o] * vgeos);
+            }
+            if (bottom) {
+                const double ug = -en_dpdy(PM, i, j, g, ca) / g.fnot;
+                const double vg = en_dpdx(PM, i, j, g, ca) / g.fnot;
+                v[7] += w * (ug * ug);
+                v[8] += w * (vg * vg);
+            }
+        }
+        block_sum<9>(v, red);
+        if (threadIdx.x == 0) {
+            out[EN_U2 * MAXL + k] = v[0];
+            out[EN_V2 * MAXL + k] = v[1];
+            out[EN_U4 * MAXL + k] = v[2];
+            out[EN_V4 * MAXL + k] = v[3];
+            out[EN_UJET * MAXL + k] = v[4];
+            if (first) {
+                out[EN_UTX] = v[5];
+                out[EN_VTY] = v[6];
+            }
+            if (bottom) {
+                out[EN_BTU] = v[7];
+                out[EN_BTV] = v[8];
+            }
+        }
+    }
+    // the layer-independent sums: Ekman pumping, interface-1 entrainment and
+    // eta_1 ent_1 (zero eta for one layer)
+    double e[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+        const double w = wj * ((i == 0 || i == nx - 1) ? 0.5 : 1.0);
+        const long long o = (long long)j * pitch + i;
+        const double wk = a.wekp[o], en = a.ent1[o];
+        const double eta = nl > 1 ? a.dz_sign * (a.p[o] - a.p[g.fs + o]) / a.gp0 : 0.0;
+        e[0] += w * wk;
+        e[1] += w * fabs(wk);
+        e[2] += w * en;
+        e[3] += w * fabs(en);
+        e[4] += w * (eta * en);
+    }
+    block_sum<5>(e, red);
+    if (threadIdx.x == 0) {
+        out[EN_WEK] = e[0];
+        out[EN_AWEK] = e[1];
+        out[EN_ENT] = e[2];
+        out[EN_AENT] = e[3];
+        out[EN_ETAENT] = e[4];
+    }
+}
+
+void launch_energy(const EnergyArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(k_energy_d2, dim3((a.g.nxp + 255) / 256, a.g.nyp, a.g.nl), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_energy, dim3(a.g.nyp), dim3(256), 0, st, a);
 }
 
 }  // namespace oqg

@@ -260,6 +260,15 @@ struct AdjSuppArgs {
     double *lq, *lqm, *lp, *lpm;
 };
 
+struct KeSeedArgs {
+    Geo g;
+    double bcco;
+    double rhoh;        // rho H_1
+    const double* p;    // layer 1 of the loaded p
+    double* lp;         // layer 1 of its adjoint (written)
+};
+void launch_ke_seed(const KeSeedArgs& a, hipStream_t st);
+
 struct AdjVortArgs {
     Geo g;
     const Phys* ph;   // DevConsts::ph (kept out of the kernel arguments)
@@ -307,6 +316,31 @@ struct DiagArgs {
     double* partial;   // [nyp][DG_LEN]
 };
 void launch_diag(const DiagArgs& a, hipStream_t st);
+
+// The energetics of qg_diagno (k_energy_d2, k_energy): per-row partials
+// [nyp][EN_LEN]: per layer (MAXL slots each) the row sums of ugeos del2(ug),
+// vgeos del2(vg), ugeos del4(ug), vgeos del4(vg) with int_P_dA's weights and
+// the row's int_P_dx weights of ugeos (times dx on the host); then the
+// weighted row sums of taux ugeos_1, tauy vgeos_1, ug^2 and vg^2 in layer
+// k_topo, wekp, |wekp|, ent_1, |ent_1| and eta_1 ent_1.
+enum { EN_U2 = 0, EN_V2, EN_U4, EN_V4, EN_UJET, EN_NL };
+enum { EN_UTX = EN_NL * MAXL, EN_VTY, EN_BTU, EN_BTV, EN_WEK, EN_AWEK, EN_ENT, EN_AENT, EN_ETAENT, EN_LEN };
+struct EnergyArgs {
+    Geo g;
+    double bcco;
+    double gp0;          // gp_1 (eta_1), 1 for one layer
+    int dz_sign;
+    const double* p;     // current time level
+    const double* pm;    // previous time level
+    const double* wekp;
+    const double* ent1;
+    const double* taux;  // null: no resident stress
+    const double* tauy;
+    double* d2u;         // nl fields: del2_P_bc(ug), ug = -dPdy_2bc(pm)/f0
+    double* d2v;         // nl fields: del2_P_bc(vg), vg = dPdx_2bc(pm)/f0
+    double* partial;     // [nyp][EN_LEN]
+};
+void launch_energy(const EnergyArgs& a, hipStream_t st);
 
 // Ocean mixed layer (oqg_mixed.hip).  T fields are stored with the basin's
 // pitch, nyp-1 rows.

@@ -154,18 +154,46 @@ def run(clk: Clock, outdir: str, ocn: Optional[Basin] = None, atm: Optional[Basi
                 yield b.nocmon
         yield clk.noutre
 
+    last = {}   # per basin: (tsec, et2m) of its previous monitor sample
+
+    def sample(b, nt, d):
+        """A monitor sample: the state diagnostics, the rest of qg_diagno's
+        energetics (QG.diagnose_energy) and pket from the previous sample's
+        et2m (qg_monitor.F90:266-272, tsec = nt dta)."""
+        d = dict(d)
+        if hasattr(b.qg, "diagnose_energy"):
+            d.update(b.qg.diagnose_energy())
+        if "et2m" in d:
+            tsec = nt * clk.dta
+            et2 = np.asarray(d["et2m"])
+            prev = last.get(id(b))
+            if prev is not None and tsec > prev[0]:
+                d["pket"] = 0.5 * b.qg.rho * np.asarray(b.qg.gp) * (et2 - prev[1]) / (tsec - prev[0])
+            else:
+                d["pket"] = np.zeros_like(et2)
+            last[id(b)] = (tsec, et2)
+        return d
+
     def diagnostic_step(nt):
+        """driver.F90:191-264: valids for the atmosphere, then the ocean; a
+        failed check forces every active monitor's sample at this step
+        (force = .not. solnok, qg_monitor_step: qg_monitor.F90:134-141); the
+        qg monitors for the ocean, then the atmosphere; then the invalid
+        dump."""
         ntdone = nt - clk.nsteps0
         solnok = True
+        diags = {}
         for b in basins:
-            want_valid = b.valids is not None and ntdone % b.valids.nvalid == 0
-            want_mon = monitor is not None and b.nocmon > 0 and ntdone % b.nocmon == 0
-            if want_valid or want_mon:
-                d = b.qg.diagnose(dts[id(b)])
-                if want_valid and not b.valids.ok(d):
+            if b.valids is not None and ntdone % b.valids.nvalid == 0:
+                d = diags[id(b)] = b.qg.diagnose(dts[id(b)])
+                if not b.valids.ok(d):
                     solnok = False
-                if want_mon:
-                    monitor(b.name, nt, d)
+        force = not solnok
+        if monitor is not None:
+            for b in (ocn, atm):
+                if b is not None and b.nocmon > 0 and (force or ntdone % b.nocmon == 0):
+                    d = diags.get(id(b))
+                    monitor(b.name, nt, sample(b, nt, d if d is not None else b.qg.diagnose(dts[id(b)])))
         if not solnok:
             save_all("invalid.nc")
             raise InvalidSolution(f"invalid values at nt = {nt}, day {nt * clk.dta / SECDAY:.2f}")

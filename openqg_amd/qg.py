@@ -282,17 +282,31 @@ class QG:
             else:
                 out[name] = np.array(v[: nl - 1] if name in ("etam", "et2m") else v[:nl])
         if self.b.cyclic:
-            out["ermas"], out["emfr"] = self._continuity(out["pavg"], 2.0 * float(dt))
+            out["ermas"], out["emfr"] = self._continuity(out["pint"], 2.0 * float(dt))
         return out
 
-    def _continuity(self, pavg, tdt):
+    def diagnose_energy(self) -> dict:
+        """The rest of qg_diagno's energetics on the device
+        (oqg_diagnose_energy): ah2d, ah4d, the jet position and value per
+        layer; btdg, utau, wepm, wapm, entm, enam, pken."""
+        e = _lib.OqgEnergy()
+        check(self._lib.oqg_diagnose_energy(self._h, self.rho, ctypes.byref(e)), self._h, "oqg_diagnose_energy")
+        nl = self.b.nl
+        out = {}
+        for name, _ in _lib.OqgEnergy._fields_:
+            v = getattr(e, name)
+            out[name] = v if isinstance(v, float) else np.array(v[:nl])
+        out["pos"] = out["pos"].astype(int)
+        return out
+
+    def _continuity(self, int_p, tdt):
         """check_continuity (qg_monitor.F90:571-604, cyclic basins only): the
-        interface volume from the layer integrals of p (avg_P times the area)
-        against the constraint's dpi.  Returns (ermas, emfr) per interface."""
+        interface volume from the layer integrals int_P_dA(p_k) against the
+        constraint's dpi.  Returns (ermas, emfr) per interface."""
         b = self.b
         nl = b.nl
         area = b.xl * b.yl
-        int_p = np.asarray(pavg) * area
+        int_p = np.asarray(int_p)
         dpi = self.get_constr()[: nl - 1]
         ermas, emfr = np.zeros(nl - 1), np.zeros(nl - 1)
         for k in range(nl - 1):

@@ -92,6 +92,93 @@ def diagnose(q, p, dx, dy, fnot, bcco, cyclic, h, gp, dz_sign, rho, dt):
     return out
 
 
+def dP2dx2_bc(p, dx, alpha, cyclic):
+    """numerics.F90:427-448."""
+    d = np.zeros_like(p)
+    fac = 1.0 / (dx * dx)
+    d[:, 1:-1] = fac * (p[:, :-2] + p[:, 2:] - 2.0 * p[:, 1:-1])
+    if not cyclic:
+        d[:, 0] = fac * alpha / (0.5 * alpha + 1.0) * (p[:, 1] - p[:, 0])
+        d[:, -1] = -fac * alpha / (0.5 * alpha + 1.0) * (p[:, -1] - p[:, -2])
+    else:
+        d[:, 0] = fac * (p[:, -2] + p[:, 1] - 2.0 * p[:, 0])
+        d[:, -1] = d[:, 0]
+    d[0, :] = 0.0
+    d[-1, :] = 0.0
+    return d
+
+
+def dP2dy2_bc(p, dy, alpha, cyclic):
+    """numerics.F90:450-468."""
+    d = np.zeros_like(p)
+    fac = 1.0 / (dy * dy)
+    d[1:-1, :] = fac * (p[:-2, :] + p[2:, :] - 2.0 * p[1:-1, :])
+    d[0, :] = fac * alpha / (0.5 * alpha + 1.0) * (p[1, :] - p[0, :])
+    d[-1, :] = -fac * alpha / (0.5 * alpha + 1.0) * (p[-1, :] - p[-2, :])
+    if not cyclic:
+        d[:, 0] = 0.0
+        d[:, -1] = 0.0
+    return d
+
+
+def del2_P_bc(p, dx, dy, alpha, cyclic):
+    """del2_P_bc of one layer (numerics.F90:410-425)."""
+    return dP2dx2_bc(p, dx, alpha, cyclic) + dP2dy2_bc(p, dy, alpha, cyclic)
+
+
+def int_P_dx(row, dx):
+    """int_P_dx (numerics.F90:206-212)."""
+    return (0.5 * row[0] + row[1:-1].sum() + 0.5 * row[-1]) * dx
+
+
+def energetics(p, pm, wekp, ent1, taux, tauy, dx, dy, fnot, bcco, cyclic, h, gp, ah2, ah4, delek, kt,
+               dz_sign, rho):
+    """The parts of qg_diagno (qg_monitor.F90:219-383) beyond the state
+    diagnostics: Ekman and entrainment averages (:255-264), pken (:273-277),
+    utau (:304-306), ah2d, ah4d (:313-327), the jet position and value
+    (:334-343), btdg (:352-355), with the lagged velocities from pm.  vg is
+    taken (nxp, nyp) like every field it meets (the reference declares it
+    (nxt, nyp, nl), :228).  p, pm: (nl, nyp, nxp); wekp, ent1, taux, tauy:
+    (nyp, nxp); kt: 0-based k_topo."""
+    nl, nyp, nxp = p.shape
+    xl, yl = (nxp - 1) * dx, (nyp - 1) * dy
+    out = {"ah2d": np.zeros(nl), "ah4d": np.zeros(nl), "val": np.zeros(nl), "pos": np.zeros(nl, dtype=int)}
+    out["wepm"] = avg_P(wekp, dx, dy)
+    out["wapm"] = avg_P(np.abs(wekp), dx, dy)
+    out["entm"] = avg_P(ent1, dx, dy)
+    out["enam"] = avg_P(np.abs(ent1), dx, dy)
+    if nl > 1:
+        eta = dz_sign * (p[0] - p[1]) / gp[0]
+        out["pken"] = rho * gp[0] * avg_P(eta * ent1, dx, dy)
+    else:
+        out["pken"] = 0.0
+    ug = np.stack([-dPdy_2bc(pm[k], dy, bcco, cyclic) / fnot for k in range(nl)])
+    vg = np.stack([dPdx_2bc(pm[k], dx, bcco, cyclic) / fnot for k in range(nl)])
+    ugeos = np.stack([-dPdy_2bc(p[k], dy, bcco, cyclic) / fnot for k in range(nl)])
+    vgeos = np.stack([dPdx_2bc(p[k], dx, bcco, cyclic) / fnot for k in range(nl)])
+    utaux = int_P_dA(taux * ugeos[0], dx, dy)
+    vtauy = int_P_dA(tauy * vgeos[0], dx, dy)
+    out["utau"] = rho * (vtauy + utaux)
+    for k in range(nl):
+        d2u = del2_P_bc(ug[k], dx, dy, bcco, cyclic)
+        d4u = del2_P_bc(d2u, dx, dy, bcco, cyclic)
+        d2v = del2_P_bc(vg[k], dx, dy, bcco, cyclic)
+        d4v = del2_P_bc(d2v, dx, dy, bcco, cyclic)
+        u2diss = int_P_dA(ugeos[k] * d2u, dx, dy)
+        v2diss = int_P_dA(vgeos[k] * d2v, dx, dy)
+        out["ah2d"][k] = -rho * ah2[k] * h[k] * (u2diss + v2diss)
+        u4diss = int_P_dA(ugeos[k] * d4u, dx, dy)
+        v4diss = int_P_dA(vgeos[k] * d4v, dx, dy)
+        out["ah4d"][k] = rho * ah4[k] * h[k] * (u4diss + v4diss)
+        ujeto = np.array([abs(int_P_dx(ugeos[k, j], dx)) / xl for j in range(nyp - 1)])   # rows 1..nyt
+        out["pos"][k] = int(np.argmax(ujeto)) + 1
+        out["val"][k] = float(ujeto.max())
+    u2 = int_P_dA(ug[kt] ** 2, dx, dy)
+    v2 = int_P_dA(vg[kt] ** 2, dx, dy)
+    out["btdg"] = 0.5 * rho * delek * abs(fnot) * (u2 + v2)
+    return out
+
+
 def check_continuity(int_p, dpi, gp, dz_sign, xl, yl, tdt):
     """check_continuity (qg_monitor.F90:571-604), called for cyclic basins
     (:378-381): per interface k, ermas = dz_sign (int_p_k - int_p_k+1) - dpi_k

@@ -114,8 +114,14 @@ def test_ocean_steps_counts():
 def test_invalid_solution_stops_and_saves(tmp_path):
     clk = Clock(dta=1200.0, nstr=1, tini=0.0, trun=5.0 / 365.0, resday=1.0)
     o = FakeQG("ocn", bad_after=100)
+    seen = []
     with pytest.raises(InvalidSolution, match="nt = 108"):
-        run(clk, str(tmp_path), ocn=Basin("ocean", o, Valids(12, 1e9, 1e9)))
+        run(clk, str(tmp_path), ocn=Basin("ocean", o, Valids(12, 1e9, 1e9), nocmon=50),
+            monitor=lambda name, nt, d: seen.append((name, nt, d["pmax"][0])))
+    # the monitor's own cadence (0, 50, 100), then the sample the failed check
+    # forces at step 108, of the state that failed
+    assert [(n, t) for n, t, _ in seen] == [("ocean", 0), ("ocean", 50), ("ocean", 100), ("ocean", 108)]
+    assert seen[-1][2] == 1e30
     # the first check at or after 100 steps is at step 108; the state saved is that step's
     assert _read_p(os.path.join(tmp_path, "ocean_qg_invalid.nc")) == 108
     assert _read_p(os.path.join(tmp_path, "ocean_qg_restart.nc")) == 72

@@ -120,6 +120,25 @@ def _interior(a):
     return b
 
 
+@pytest.mark.parametrize("n", [33, 65])
+def test_device_ke_seed_matches_host_restatement(gpu, n):
+    """oqg_adj_seed_ke: J = ekei(1) from the device's row sums and dJ/dp_1 by
+    the gather form of the derivative transposes, against the host
+    restatement ekei1 (whose gradient is pinned by central differences on the
+    CPU); the other adjoint fields are zeroed."""
+    w = W.small_box(n, 3, seed=3)
+    qg, _, _ = _prepared(w, 5)
+    adj = AdjointState(qg)
+    adj.set(lq=np.ones(qg.shape), lqm=np.ones(qg.shape), lp=np.ones(qg.shape), lpm=np.ones(qg.shape))
+    J = adj.seed_ke(1000.0)
+    lq, lqm, lp, lpm = adj.get()
+    p = qg.get_state()[2]
+    Jh, gh = ekei1(p[0], w.dx, w.dx, w.fnot, qg.bcco, 1000.0, w.h[0])
+    assert abs(J - Jh) <= 1e-13 * abs(Jh)
+    assert np.abs(lp[0] - gh).max() <= 1e-13 * np.abs(gh).max()
+    assert not lp[1:].any() and not lq.any() and not lqm.any() and not lpm.any()
+
+
 def test_gradient_matches_finite_differences(gpu):
     w = W.small_box(33, 3)
     nsteps, supp = 10, 4

@@ -87,3 +87,51 @@ def test_continuity_check_matches_oracle(gpu):
         assert abs(d["ermas"][k] - ermas[k]) <= 1e-12 * esum, (k, d["ermas"], ermas)
         assert abs(d["emfr"][k] - emfr[k]) <= 4e-12 + 1e-9 * abs(emfr[k]), (k, d["emfr"], emfr)
     print(f"[diag] ermas {d['ermas']} emfr {d['emfr']}", flush=True)
+
+
+@pytest.mark.parametrize("make,steps", [(W.small_box, 12), (W.small_channel, 12), (W.c2_box513, 3)],
+                         ids=["box33", "chan33x17", "box513"])
+def test_energetics_match_oracle(gpu, make, steps):
+    """oqg_diagnose_energy against the numpy restatement of qg_diagno's
+    energetics (oracle/monitor.py) on a state evolved with lateral friction,
+    wind stress set on the device and seeded interface entrainment: the
+    oracle reads the device's own resident forcing and state."""
+    from dataclasses import replace
+
+    w = make()
+    w = replace(w, ah2=[150.0] * w.nl)
+    qg = w.make_qg()
+    p = w.initial_p()
+    qg.set_state(p=p, pm=p)
+    qg.init_state()
+    ent, wekp, txis, txin = w.forcing()
+    rng = np.random.default_rng(w.seed + 7)
+    ent = 1e-6 * rng.standard_normal(ent.shape)
+    if w.cyclic:
+        ent[..., -1] = ent[..., 0]
+    qg.set_forcing(ent, wekp, txis, txin)
+    tx = 1e-4 * (1.0 + rng.standard_normal((w.nyp, w.nxp)))
+    ty = 2e-5 * rng.standard_normal((w.nyp, w.nxp))
+    if w.cyclic:
+        tx[:, -1], ty[:, -1] = tx[:, 0], ty[:, 0]
+    qg.set_wind(tx, ty)
+    print(f"[energy] {w.name} after {steps} steps", flush=True)
+    qg.run(w.tdt, steps, 0, w.supp_every)
+    qg.sync()
+    e = qg.diagnose_energy()
+    _, _, p, pm = qg.get_state()
+    wk, _, _ = qg.get_forcing()
+    gx, gy = qg.get_wind()
+    ent1 = qg.get_ent()
+    dz = w.params()["dz_sign"]
+    kt = 0 if dz == 1 else w.nl - 1
+    o = M.energetics(p, pm, wk, ent1, gx, gy, w.dx, w.dx, w.fnot, qg.bcco, w.cyclic, w.h, w.gp, qg.ah2, qg.ah4,
+                     qg.delek, kt, dz, qg.rho)
+    for name in ("ah2d", "ah4d", "val", "btdg", "utau", "wepm", "wapm", "entm", "enam", "pken"):
+        a, b = np.atleast_1d(e[name]), np.atleast_1d(o[name])
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= 1e-9 * scale, (name, a, b)
+        if not (name == "btdg" and qg.delek == 0.0):   # the channel has no bottom Ekman layer
+            assert np.all(b != 0.0), name   # every quantity is exercised
+    assert np.array_equal(e["pos"], o["pos"]), (e["pos"], o["pos"])
+    print(f"[energy] ah4d {e['ah4d']} btdg {e['btdg']:.6e} utau {e['utau']:.6e}", flush=True)

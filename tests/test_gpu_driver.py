@@ -32,11 +32,22 @@ def test_run_loop_matches_single_steps(gpu, tmp_path):
     assert clk.dto == wo.dt and clk.dta == wa.dt
     print(f"[driver] {wo.name} + {wa.name}, {clk.ntsrun} steps", flush=True)
     o1, a1 = _make(wo), _make(wa)
-    seen = []
+    seen, samples = [], []
     nt = run(clk, str(tmp_path), ocn=Basin("ocean", o1, Valids(100, 1e12, 1e3), nocmon=144),
-             atm=Basin("atmos", a1, Valids(100, 1e12, 1e3)), monitor=lambda n, t, d: seen.append((n, t)))
+             atm=Basin("atmos", a1, Valids(100, 1e12, 1e3)),
+             monitor=lambda n, t, d: (seen.append((n, t)), samples.append(d)))
     assert nt == clk.ntsrun == 864
     assert seen == [("ocean", t) for t in range(0, 865, 144)]
+    # each sample carries the energetics and pket from the previous sample's et2m
+    for k, d in enumerate(samples):
+        assert {"ekei", "ah4d", "btdg", "utau", "pket"} <= set(d)
+        assert np.all(np.isfinite(d["ah4d"])) and d["btdg"] > 0.0
+        if k == 0:
+            assert np.all(d["pket"] == 0.0)
+        else:
+            dt = 144 * clk.dta
+            want = 0.5 * o1.rho * np.asarray(o1.gp) * (d["et2m"] - samples[k - 1]["et2m"]) / dt
+            np.testing.assert_allclose(d["pket"], want, rtol=1e-14, atol=0)
 
     o2, a2 = _make(wo), _make(wa)
     for t in range(1, clk.nsteps + 1):
