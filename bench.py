@@ -81,14 +81,19 @@ def _oracle_seconds(w, threads: int, nsteps: int, reps: int = 3) -> float:
     workload w (each sample from the same initial state, after one untimed step)."""
     from oracle import cpuref as cr
 
+    from openqg_amd import workloads as W
+
     cr.set_threads(threads)
     ent, wekp, txis, txin = w.forcing()
-    p = w.initial_p()
+    p = None if w.init == "splitmix" else w.initial_p()
     times = []
     for _ in range(reps):
         b = cr.Basin(w.params())
-        b.set_state(p=p, pm=p)
-        b.init_state()
+        if p is None:
+            b.start_from_q(W.initial_q(w))
+        else:
+            b.set_state(p=p, pm=p)
+            b.init_state()
         b.step(w.tdt, ent, wekp, txis, txin)   # first step (page-in, thread start) untimed
         b.suppress()
         t0 = time.perf_counter()
@@ -177,9 +182,7 @@ def ensemble_bench(members: int, days: float):
         qgs = []
         for w in ws:
             qg = w.make_qg()
-            p = w.initial_p()
-            qg.set_state(p=p, pm=p)
-            qg.init_state()
+            w.start(qg)
             ent, wekp, txis, txin = w.forcing()
             qg.set_forcing(ent, wekp, txis, txin)
             qg.sync()

@@ -285,6 +285,12 @@ class QG:
             out["ermas"], out["emfr"] = self._continuity(out["pint"], 2.0 * float(dt))
         return out
 
+    def invert(self):
+        """p from the current q (oqg_invert): modes, Helmholtz solves and the
+        mass correction with the current dpi; q's boundary PV by ocqbdy; then
+        qm = q and pm = p."""
+        check(self._lib.oqg_invert(self._h), self._h, "oqg_invert")
+
     def diagnose_energy(self) -> dict:
         """The rest of qg_diagno's energetics on the device
         (oqg_diagnose_energy): ah2d, ah4d, the jet position and value per

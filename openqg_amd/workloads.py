@@ -49,7 +49,7 @@ class Workload:
     tau0: float = 0.1
     fnot: float = F0
     beta: float = BETA
-    init: str = "white"       # "white": iid PV noise; "k3": k^-3 spectrum field
+    init: str = "white"       # "white": iid PV noise; "k3": k^-3 spectrum field; "splitmix": q from SplitMix64
     q_rms: float = 1.0e-7
     seed: int = 1
     supp_every: int = 25
@@ -85,7 +85,73 @@ class Workload:
 
     def initial_p(self):
         """Seeded p (= pm) whose PV anomaly has rms ~q_rms per layer."""
+        if self.init == "splitmix":
+            raise ValueError(f"{self.name} starts from q (Workload.start), not from a seeded p")
         return initial_p(self)
+
+    def start(self, qg):
+        """Put the initial state on the basin `qg`.  From a seeded p: p = pm,
+        then init_pv / init_constraint / init_mass_constr (QG.init_state).
+        init = "splitmix": q = qm = beta yprel + dq (dq = q_rms N(0, 1) from
+        normal_deviates(seed) on the interior points, x fastest, then rows, then
+        layers), p = pm = 0, dpi = 0, then p from one inversion (QG.invert)."""
+        if self.init == "splitmix":
+            z = np.zeros((self.nl, self.nyp, self.nxp))
+            q = initial_q(self)
+            qg.set_state(q=q, qm=q, p=z, pm=z)
+            qg.set_constr(np.zeros(len(qg.get_constr())))
+            qg.invert()
+        else:
+            p = self.initial_p()
+            qg.set_state(p=p, pm=p)
+            qg.init_state()
+
+
+# SplitMix64 (Steele, Lea and Flood, "Fast splittable pseudorandom number
+# generators", OOPSLA 2014; the finaliser constants of Stafford's Mix13).
+SM_GAMMA = 0x9E3779B97F4A7C15
+SM_M1 = 0xBF58476D1CE4E5B9
+SM_M2 = 0x94D049BB133111EB
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """The first n outputs (uint64) of SplitMix64 started from state `seed`:
+    state += gamma; z = state; z = (z ^ z >> 30) m1; z = (z ^ z >> 27) m2;
+    output z ^ z >> 31, all modulo 2^64."""
+    u = np.uint64
+    with np.errstate(over="ignore"):
+        s = u(seed % 2 ** 64) + np.arange(1, n + 1, dtype=np.uint64) * u(SM_GAMMA)
+        z = (s ^ (s >> u(30))) * u(SM_M1)
+        z = (z ^ (z >> u(27))) * u(SM_M2)
+    return z ^ (z >> u(31))
+
+
+def normal_deviates(seed: int, n: int) -> np.ndarray:
+    """n standard normal deviates by the Box-Muller transform of SplitMix64
+    uniforms u = (z >> 11) 2^-53 in [0, 1), taken in pairs (u1, u2):
+    r = sqrt(-2 ln(1 - u1)), then r cos(2 pi u2) and r sin(2 pi u2)."""
+    m = (n + 1) // 2
+    z = splitmix64(seed, 2 * m)
+    uni = (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    u1, u2 = uni[0::2], uni[1::2]
+    r = np.sqrt(-2.0 * np.log1p(-u1))
+    out = np.empty(2 * m)
+    out[0::2] = r * np.cos(2.0 * np.pi * u2)
+    out[1::2] = r * np.sin(2.0 * np.pi * u2)
+    return out[:n]
+
+
+def initial_q(w: Workload) -> np.ndarray:
+    """q = beta yprel + dq, dq = q_rms N(0, 1) on the interior points (x
+    fastest, then rows, then layers), beta yprel on the boundary ring (which
+    the inversion's ocqbdy replaces).  yprel = -yl/2 + j dy (box.F90:144)."""
+    nl, nyp, nxp = w.nl, w.nyp, w.nxp
+    yprel = -0.5 * (nyp - 1) * w.dx + np.arange(nyp) * w.dx
+    q = np.broadcast_to((w.beta * yprel)[None, :, None], (nl, nyp, nxp)).copy()
+    nyi, nxi = nyp - 2, nxp - 2
+    dq = w.q_rms * normal_deviates(w.seed, nl * nyi * nxi).reshape(nl, nyi, nxi)
+    q[:, 1:-1, 1:-1] += dq
+    return q
 
 
 def _dst1(x: np.ndarray, axis: int) -> np.ndarray:
@@ -154,8 +220,9 @@ def ocean_box(name: str, n: int, dx: float, ah4: float, dt: float, days: float, 
 
 
 def c1_box129() -> Workload:
-    """129^2 box at 30 km, ah4 = 2e10 m^4/s, dt = 1 h, 10 days; white PV noise 1e-7 (seed 1)."""
-    return ocean_box("box129_dx30km_3l", 129, 30.0e3, 2.0e10, 3600.0, 10.0, init="white", q_rms=1.0e-7, seed=1)
+    """129^2 box at 30 km, ah4 = 2e10 m^4/s, dt = 1 h, 10 days; q = beta yprel +
+    1e-7 N(0, 1) PV noise from SplitMix64 (seed 1), p by one inversion."""
+    return ocean_box("box129_dx30km_3l", 129, 30.0e3, 2.0e10, 3600.0, 10.0, init="splitmix", q_rms=1.0e-7, seed=1)
 
 
 def c2_box513() -> Workload:
@@ -197,14 +264,15 @@ def ensemble_member(i: int, base_seed: int = 3, days: float = 10.0) -> Workload:
     """Member i of the uneven double-gyre ensemble (seed base_seed + i): grid
     drawn from 65^2..513^2 over the same 3840 km basin, 2-4 layers, wind
     amplitude tau0 ~ U(0.05, 0.15) N/m^2, biharmonic viscosity ~ logU(1e9, 1e11)
-    m^4/s, white PV noise of rms 1e-7 1/s, the stable time step for its grid."""
+    m^4/s, PV noise 1e-7 N(0, 1) 1/s from SplitMix64 (seed base_seed + i) with
+    p by one inversion (as the 129^2 box), the stable time step for its grid."""
     rng = np.random.default_rng(base_seed + i)
     n = int(rng.choice(ENS_SIZES))
     nl = int(rng.choice([2, 3, 4]))
     tau0 = float(rng.uniform(0.05, 0.15))
     ah4 = float(10.0 ** rng.uniform(9.0, 11.0))
     dx = BASIN_L / (n - 1)
-    return ocean_box(f"ens{i:02d}_box{n}_{nl}l", n, dx, ah4, stable_dt(dx, ah4), days, nl=nl, init="white",
+    return ocean_box(f"ens{i:02d}_box{n}_{nl}l", n, dx, ah4, stable_dt(dx, ah4), days, nl=nl, init="splitmix",
                      q_rms=1.0e-7, seed=base_seed + i, tau0=tau0)
 
 

@@ -253,6 +253,17 @@ class Basin:
         lib().cr_solve_inhomog(self._h, int(m), _ptr(rhs), _ptr(out))
         return out
 
+    def start_from_q(self, q0):
+        """The initial state by one inversion: q = qm = q0, p = pm = 0 and a zero
+        constraint state, then one step with tdt = 0 (q_new = qm, dpi unchanged,
+        p by solve_pressure, q's ring by ocqbdy), then qm = q, pm = p."""
+        z = self._field()
+        self.set_state(q=q0, qm=q0, p=z, pm=z)
+        self.set_constr(np.zeros(len(self.get_constr())))
+        self.step(0.0, np.zeros((max(self.nl - 1, 1), self.nyp, self.nxp)), np.zeros((self.nyp, self.nxp)))
+        q, _, p, _ = self.get_state()
+        self.set_state(q=q, qm=q, p=p, pm=p)
+
     def del2(self, f):
         f = np.ascontiguousarray(f, dtype=np.float64)
         nl = 1 if f.ndim == 2 else f.shape[0]

@@ -42,9 +42,7 @@ def _constr_err(a, b):
 
 def _start(w):
     qg = w.make_qg()
-    p = w.initial_p()
-    qg.set_state(p=p, pm=p)
-    qg.init_state()
+    w.start(qg)
     fo = w.forcing()
     qg.set_forcing(*fo)
     return qg, fo
