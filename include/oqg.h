@@ -89,7 +89,11 @@ typedef struct oqg_ctx oqg_ctx;
 
 /* init_qg (src/qg/qg.F90:60-87): allocate device state, derive modes
  * (eigmod, modes.F90:46), Helmholtz tables (init_inhomog, inhomog.F90:71)
- * and homogeneous solutions (homsol, homog.F90:108).  State starts at zero. */
+ * and homogeneous solutions (homsol, homog.F90:108).  State starts at zero.
+ * The x transform length (box 2(nxp-1), channel nxp-1) must factor into 2, 3,
+ * 5, 7, 11 and 13 (FFTW takes any, inhomog.F90:103-122) and fit the LDS
+ * transforms (box nxp <= 1025, channel nxp <= 1025); lengths with a factor 7,
+ * 11 or 13 run the plain step only (no batched ensembles, no adjoint). */
 int oqg_create(const oqg_desc* desc, oqg_ctx** out);
 void oqg_destroy(oqg_ctx* ctx);
 

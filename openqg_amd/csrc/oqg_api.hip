@@ -657,16 +657,23 @@ static int setup(oqg_ctx* c) {
         auto plan = [&](int maxr, int& nst, int* radix) {
             int m = s.L;
             nst = 0;
-            for (int r : {16, 8, 4, 2, 3, 5})
-                if (r <= maxr || r == 2 || r == 3 || r == 5)
-                    while (m % r == 0) { radix[nst++] = r; m /= r; }
-            if (m != 1) return fail(c, OQG_ERR_ARG, "x transform length must factor into 2, 3 and 5");
+            for (int r : {16, 8, 4, 2, 3, 5, 7, 11, 13})
+                if (r <= maxr || r == 2 || r % 2 == 1)   // 16, 8, 4 up to the kernel's largest; all primes
+                    while (m % r == 0) {
+                        if (nst == MAX_RADIX_STAGES) return fail(c, OQG_ERR_ARG, "x transform plan too long");
+                        radix[nst++] = r;
+                        m /= r;
+                    }
+            if (m != 1)
+                return fail(c, OQG_ERR_ARG, "x transform length " + std::to_string(s.L) +
+                                                " must factor into 2, 3, 5, 7, 11 and 13");
             for (int st = 0; st < nst; st++)   // each thread holds at most 4 butterflies per stage
                 if (s.L / radix[st] > 4 * 256) return fail(c, OQG_ERR_ARG, "x transform too long for the LDS FFT");
             return 0;
         };
         if (int r = plan(FWD_MAX_RADIX, s.nstages, s.radix)) return r;
         if (int r = plan(INV_MAX_RADIX, s.instages, s.iradix)) return r;
+        s.prime = (s.L % 7 == 0 || s.L % 11 == 0 || s.L % 13 == 0) ? 1 : 0;
     }
     if (inv_lds_bytes(s, g, nl) > MAX_DYN_LDS || fwd_lds_bytes(s) > MAX_DYN_LDS || ysolve_lds_bytes(s) > MAX_DYN_LDS)
         return fail(c, OQG_ERR_ARG, "grid too large for the LDS-staged transforms");
@@ -760,6 +767,14 @@ static int setup(oqg_ctx* c) {
         for (int m = 1; m < 5 && L % 5 == 0; m++) {
             const size_t o = twiddle_roots5(L) + m - 1;
             twiddle_ld((long)m * (L / 5), L, &tw[2 * o], &tw[2 * o + 1]);
+        }
+        for (int R : {7, 11, 13}) {
+            if (L % R) continue;
+            const int base = R == 7 ? twiddle_roots7(L) : (R == 11 ? twiddle_roots11(L) : twiddle_roots13(L));
+            for (int m = 1; m < R; m++) {
+                const size_t o = base + m - 1;
+                twiddle_ld((long)m * (L / R), L, &tw[2 * o], &tw[2 * o + 1]);
+            }
         }
         HIPCHK(c, hipMemcpy(inv ? c->itw : c->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
     }
@@ -1819,6 +1834,8 @@ int oqg_adj_init(oqg_ctx* c) {
     ctx_join(c);
     if (!c) return OQG_ERR_ARG;
     if (c->g.cyclic) return fail(c, OQG_ERR_ARG, "the adjoint is implemented for box basins");
+    if (c->s.prime)
+        return fail(c, OQG_ERR_ARG, "the adjoint takes x transform lengths that factor into 2, 3 and 5");
     if (c->adj) return 0;
     const size_t fl = (size_t)c->g.nl * c->g.fs, f1 = (size_t)c->g.fs;
     int rc = 0;
@@ -2671,6 +2688,9 @@ int oqg_ens_create(oqg_ctx* const* members, int n, oqg_ens** out) {
         if (!members[i]) return fail(nullptr, OQG_ERR_ARG, "oqg_ens_create: null member");
         for (int j = 0; j < i; j++)
             if (members[j] == members[i]) return fail(nullptr, OQG_ERR_ARG, "oqg_ens_create: a basin appears twice");
+        if (members[i]->s.prime)
+            return fail(nullptr, OQG_ERR_ARG,
+                        "oqg_ens_create: batched members take x transform lengths that factor into 2, 3 and 5");
     }
     std::unique_ptr<oqg_ens> e(new oqg_ens);
     e->m.assign(members, members + n);

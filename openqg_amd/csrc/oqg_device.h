@@ -209,7 +209,8 @@ __device__ void block_sum(double (&v)[NV], double* red) {
 }
 
 // ---------------------------------------------------------------- FFT in LDS
-// Stockham autosort (Govindaraju et al. 2008) with mixed radix 2/3/4/5, one
+// Stockham autosort (Govindaraju et al. 2008) with mixed radix 2/3/4/5 (and
+// 8/16, and 7/11/13 in the PR instantiations), one
 // complex buffer of length L per transform; each of the FFT_GROUP threads of a
 // group holds up to FFT_MAXB butterflies in registers across the barrier that
 // separates a stage's reads from its writes.  Twiddles come from an LDS copy of
@@ -403,7 +404,7 @@ __device__ __forceinline__ void fft_stage(double2* buf, int LR, int Ns, const do
 // instantiated, so it costs the kernel no registers); gt = thread index within
 // its group; MB >= the most butterflies a thread owns in any stage.  Every
 // thread of the block must call this (it contains block barriers).
-template <int MB, int SIGN, int MAXR>
+template <int MB, int SIGN, int MAXR, bool PR = false>
 __device__ __forceinline__ void fft_lds(double2* buf, int L, int nst, const int* radix, const double2* tw, int gt) {
     int kk[MB], jj[MB];
 #pragma unroll
@@ -428,6 +429,16 @@ __device__ __forceinline__ void fft_lds(double2* buf, int L, int nst, const int*
             if (R == 8) {
                 fft_stage<8, MB, SIGN>(buf, LR, Ns, tws, nullptr, gt, kk, jj);
                 tws += 7 * Ns;
+                Ns *= R;
+                continue;
+            }
+        }
+        if constexpr (PR) {   // the odd primes 7, 11, 13 (plans with prime = 1 only)
+            if (R == 7 || R == 11 || R == 13) {
+                if (R == 7) fft_stage<7, MB, SIGN>(buf, LR, Ns, tws, tw + twiddle_roots7(L), gt, kk, jj);
+                else if (R == 11) fft_stage<11, MB, SIGN>(buf, LR, Ns, tws, tw + twiddle_roots11(L), gt, kk, jj);
+                else fft_stage<13, MB, SIGN>(buf, LR, Ns, tws, tw + twiddle_roots13(L), gt, kk, jj);
+                tws += (R - 1) * Ns;
                 Ns *= R;
                 continue;
             }

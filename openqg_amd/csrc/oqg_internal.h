@@ -52,6 +52,7 @@ struct Spec {
     int radix[MAX_RADIX_STAGES];
     int instages;   // plan of the inverse kernels, radix <= INV_MAX_RADIX
     int iradix[MAX_RADIX_STAGES];
+    int prime;      // the plans hold a radix 7, 11 or 13 (block kernels k_fwd_pr / k_inv_pr only)
     double a;       // 1/dy^2 (inhomog.F90:127)
     double ftnorm;  // 1/N (inhomog.F90:124)
 };
@@ -138,10 +139,14 @@ struct VortArgs {
 };
 
 // Entries of the twiddle table of a length-L transform plan (the stages'
-// twiddles, then the radix-3 and radix-5 roots at the two offsets below).
-__host__ __device__ inline int twiddle_table_len(int L) { return L + 8; }
+// twiddles, then the radix-3 and radix-5 roots at the two offsets below, then
+// those of the radices 7, 11 and 13 that divide L).
 __host__ __device__ inline int twiddle_roots3(int L) { return L - 1; }
 __host__ __device__ inline int twiddle_roots5(int L) { return L + 1; }
+__host__ __device__ inline int twiddle_roots7(int L) { return L + 8; }
+__host__ __device__ inline int twiddle_roots11(int L) { return twiddle_roots7(L) + (L % 7 == 0 ? 6 : 0); }
+__host__ __device__ inline int twiddle_roots13(int L) { return twiddle_roots11(L) + (L % 11 == 0 ? 10 : 0); }
+__host__ __device__ inline int twiddle_table_len(int L) { return twiddle_roots13(L) + (L % 13 == 0 ? 12 : 0); }
 
 // Column pitch (doubles) of the y solver's LDS tile for R rows per lane: room
 // for R row groups of stride 65, rounded up to 2 mod 16, which puts the 8

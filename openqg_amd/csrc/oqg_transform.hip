@@ -108,7 +108,7 @@ __device__ __forceinline__ void extra_point(const Geo& g, int e, int nside, int 
 // Row pair jp of mode mb (or of the raw field).  ADJ: the adjoint mode of
 // the box inversion (FwdArgs::adj_*), whose input of mode m is
 // w_m = sum_k ctm2l(m,k) lp_k.
-template <int NL, int MB, bool ADJ = false, bool STAGE_DOUBLES = ADJ>
+template <int NL, int MB, bool ADJ = false, bool STAGE_DOUBLES = ADJ, bool PR = false>
 __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
     extern __shared__ double2 zbuf[];   // [zpad_len(L)] transform, then the twiddle table
     const Geo& g = a.g;
@@ -280,7 +280,7 @@ __device__ __forceinline__ void fwd_block(const FwdArgs& a, int jp, int mb) {
         }
     }
     __syncthreads();
-    fft_lds<MB, -1, FWD_MAX_RADIX>(zbuf, s.L, s.nstages, s.radix, tw, threadIdx.x);
+    fft_lds<MB, -1, FWD_MAX_RADIX, PR>(zbuf, s.L, s.nstages, s.radix, tw, threadIdx.x);
     double* row0 = a.spec + ((long long)m * s.nrow + r0) * s.kp;
     double* row1 = row0 + s.kp;
     if (!g.cyclic) {
@@ -311,6 +311,14 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
 
 template <int NL, int MB>
 __global__ __launch_bounds__(256) void k_fwd_adj(FwdArgs a) { fwd_block<NL, MB, true>(a, blockIdx.x, blockIdx.y); }
+
+// Plans with a radix 7, 11 or 13 (transform lengths the wavefront forms and
+// the other instantiations do not take): four butterflies per thread cover
+// every such plan of the lengths the LDS FFT accepts.
+template <int NL>
+__global__ __launch_bounds__(256) void k_fwd_pr(FwdArgs a) {
+    fwd_block<NL, 4, false, false, true>(a, xcd_row_pair(blockIdx.x, gridDim.x), blockIdx.y);
+}
 
 // Ensemble form: block b works on block map[b].y (row pairs fastest, then
 // modes) of member map[b].x.
@@ -356,6 +364,12 @@ void launch_fwd(const FwdArgs& a, int nmodes, hipStream_t st) {
     }
     const int npairs = (a.s.nyi + 1) / 2;
     const size_t lds = fwd_lds_bytes(a.s);
+    if (a.s.prime) {
+#define C(N) hipLaunchKernelGGL((k_fwd_pr<N>), dim3(npairs, nmodes), dim3(256), lds, st, a)
+        OQG_WAVE_NL_SWITCH(a.g.nl, C)
+#undef C
+        return;
+    }
 #define FWD_CALL(NLc, MBc) hipLaunchKernelGGL((k_fwd<NLc, MBc>), dim3(npairs, nmodes), dim3(256), lds, st, a)
     OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s, false), FWD_CALL)
 #undef FWD_CALL
@@ -739,7 +753,7 @@ __device__ __forceinline__ void load_spec_pair(const InvArgs& a, int m, bool ok,
 }
 
 // Row pair jp.  ADJ: the adjoint mode of the box inversion (InvArgs::adj_*).
-template <int NL, int MB, bool ADJ = false>
+template <int NL, int MB, bool ADJ = false, bool PR = false>
 __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
     extern __shared__ double smem[];
     const Geo& g = a.g;
@@ -898,8 +912,8 @@ __device__ __forceinline__ void inv_block(const InvArgs& a, int jp) {
             }
         }
         __syncthreads();
-        if (g.cyclic) fft_lds<MB, +1, INV_MAX_RADIX>(z, s.L, s.instages, s.iradix, tw, gt);   // c2r (inhomog.F90:247)
-        else fft_lds<MB, -1, INV_MAX_RADIX>(z, s.L, s.instages, s.iradix, tw, gt);
+        if (g.cyclic) fft_lds<MB, +1, INV_MAX_RADIX, PR>(z, s.L, s.instages, s.iradix, tw, gt);   // c2r (inhomog.F90:247)
+        else fft_lds<MB, -1, INV_MAX_RADIX, PR>(z, s.L, s.instages, s.iradix, tw, gt);
         if (valid) {
             double* o0 = inh + (size_t)mi * 2 * g.pitch;
             double* o1 = o0 + g.pitch;
@@ -1164,6 +1178,11 @@ __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv_adj(Inv
     inv_block<NL, MB, true>(a, blockIdx.x);
 }
 
+template <int NL>
+__global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv_pr(InvArgs a) {
+    inv_block<NL, 4, false, true>(a, xcd_row_pair(blockIdx.x, gridDim.x));
+}
+
 // Ensemble form: block b works on row pair map[b].y of member map[b].x.
 template <int NL, int MB>
 __global__ __launch_bounds__(FFT_GROUP * inv_group_count(NL)) void k_inv_ens(const InvArgs* __restrict__ args,
@@ -1180,7 +1199,7 @@ size_t fftw_lds_bytes(const Spec& s, int nl) {
     return sizeof(double2) * ((size_t)nl * zpad_len(s.L) + twiddle_table_len(s.L));
 }
 
-bool fftw_fits(const Spec& s, int nl) { return fftw_lds_bytes(s, nl) <= MAX_DYN_LDS; }
+bool fftw_fits(const Spec& s, int nl) { return !s.prime && fftw_lds_bytes(s, nl) <= MAX_DYN_LDS; }
 
 void launch_fwdw(const FwdArgs& a, hipStream_t st) {
 #define C(N) launch_fwdw_nl##N(a, st)
@@ -1225,6 +1244,12 @@ void launch_inv(const InvArgs& a, hipStream_t st) {
     const int G = inv_groups(a);
     const dim3 grid(inv_block_count(a.s)), block(FFT_GROUP * G);
     const size_t lds = inv_lds_bytes(a.s, a.g, nm);
+    if (a.s.prime) {
+#define C(N) hipLaunchKernelGGL((k_inv_pr<N>), grid, block, lds, st, a)
+        OQG_WAVE_NL_SWITCH(a.g.nl, C)
+#undef C
+        return;
+    }
 #define INV_CALL(NLc, MBc) hipLaunchKernelGGL((k_inv<NLc, MBc>), grid, block, lds, st, a)
     OQG_NL_MB_SWITCH(a.g.nl, fft_mb(a.s, true), INV_CALL)
 #undef INV_CALL
@@ -1271,7 +1296,9 @@ void set_kernel_attrs() {
         ATTR_NL(2) ATTR_NL(3) ATTR_NL(4) ATTR_NL(5) ATTR_NL(6) ATTR_NL(7) ATTR_NL(8)
 #undef ATTR_NL
 #undef ATTR_NLMB
-#define C(N) set_wave_attrs_nl##N(lim);
+#define C(N) set_wave_attrs_nl##N(lim);                                                             \
+    (void)hipFuncSetAttribute((const void*)k_fwd_pr<N>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+    (void)hipFuncSetAttribute((const void*)k_inv_pr<N>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         C(2) C(3) C(4) C(5) C(6) C(7) C(8)
 #undef C
 #define ATTR_R(RR)                                                                                                 \

@@ -12,7 +12,7 @@ import pytest
 from conftest import rel_err
 from oracle import cpuref as cr
 from openqg_amd import workloads as W
-from openqg_amd import FLAG_YSOLVE_THOMAS
+from openqg_amd import FLAG_YSOLVE_THOMAS, OqgError
 
 pytestmark = pytest.mark.gpu
 
@@ -106,6 +106,31 @@ def test_step_parity_mixed_radix(gpu):
     60 = 4*3*5 in the channel): both root blocks of a plan's twiddle table."""
     _per_step(W.small_box(n=61), 6)
     _per_step(W.small_channel(nx=61, ny=31), 6)
+
+
+@pytest.mark.parametrize("kind,n", [("box", 57), ("box", 45), ("box", 40), ("chan", 53), ("chan", 57),
+                                    ("chan", 92)],
+                         ids=["box57_L112=16x7", "box45_L88=8x11", "box40_L78=2x3x13", "chan53_L52=4x13",
+                              "chan57_L56=8x7", "chan92_L91=7x13"])
+def test_step_parity_prime_lengths(gpu, kind, n):
+    """Transform lengths with a factor 7, 11 or 13 run the LDS block kernels
+    with those radices (FFTW's any-length transforms, inhomog.F90:103-122)."""
+    w = W.small_box(n=n) if kind == "box" else W.small_channel(nx=n, ny=25)
+    _per_step(w, 6)
+
+
+def test_prime_lengths_rejected_by_batches_and_adjoint(gpu):
+    """The batched ensemble and the adjoint take lengths of 2, 3 and 5 only:
+    both refuse a basin with a radix-7 plan with an error, not a wrong result."""
+    from openqg_amd.adjoint import AdjointState
+    from openqg_amd.ensemble import Ensemble
+
+    w = W.small_box(n=57)
+    qg = w.make_qg()
+    with pytest.raises(OqgError, match="factor into 2, 3 and 5"):
+        AdjointState(qg)
+    with pytest.raises(OqgError, match="factor into 2, 3 and 5"):
+        Ensemble([qg, W.small_box(n=33).make_qg()])
 
 
 def test_step_parity_rectangular_box(gpu):
