@@ -167,10 +167,12 @@ __device__ __forceinline__ double lapT_lds(const double* s, int W, int r, int c,
 //   lq_n += jfac J_1^T(mu; p_n),  lp_n += -jfac J_1^T(mu; q_n)   (J(q,p) = -J(p,q))
 //   lp_{n-1} = c2 (L^T)^2 mu - c4 (L^T)^3 mu - [k=kt] cdrag L^T mu
 //   lwekp += cwek mu_0,  lq_{n-1} = lq_{n+1} on the interior, 0 on the ring.
-// k_adj_vort tiles: 64 x 8 points of one layer per 256-thread block (two rows
-// per thread), 33 KB of LDS, so that four blocks share a CU (64 x 16 tiles,
-// two blocks per CU, measured 14.9 against 12.3 us at 257^2).
-constexpr int ADJ_TX = 64, ADJ_TY = 8;
+// k_adj_vort tiles: 64 x 4 points of one layer per 256-thread block (one row
+// per thread), 22 KB of LDS, seven blocks per CU.  The kernel is latency-bound:
+// shorter tiles shorten each block's chain of loads and LDS passes while every
+// block stays resident (a gradient at 257^2 took 271.0 ms with 64 x 4 tiles,
+// 274.3 ms with 64 x 8; 64 x 16 tiles measured 14.9 against 12.3 us for 64 x 8).
+constexpr int ADJ_TX = 64, ADJ_TY = 4;
 
 // Away from the boundary (2 <= i <= nxp-3, 2 <= j <= nyp-3, "deep" points)
 // every stencil point of lapT_lds is interior and L^T = L (the 5-point

@@ -87,7 +87,7 @@ struct oqg_ctx {
     int cur = 0;           // time-level parity
     double* hst = nullptr; // pinned staging of host field transfers (hst_words doubles)
     size_t hst_words = 0;
-    int tiles_x = 0, tiles_y = 0;
+    int tiles_x = 0, tiles_y = 0, vort_ty = VORT_TY;
     size_t fbytes = 0;     // bytes of one nl-layer field (pitch layout)
     // device memory
     double* qbuf[2] = {nullptr, nullptr};
@@ -516,6 +516,7 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     va.tdt = tdt;
     va.tiles_x = c->tiles_x;
     va.tiles_y = c->tiles_y;
+    va.ty = c->vort_ty;
     va.org_x = vort_org_x(c->g);
     va.cpart = c->g.cyclic ? c->cpart : nullptr;
     va.ent2s = c->ent2s;
@@ -699,7 +700,8 @@ static int setup(oqg_ctx* c) {
     const size_t nspec = (size_t)nl * s.nrow * s.kp;
     c->tiles_x = (vort_lim_x(g) - vort_org_x(g) + VORT_TX - 1) / VORT_TX;
     if (g.cyclic && c->tiles_x > CYC_MAX_TX) return fail(c, OQG_ERR_ARG, "channel too wide for the boundary-integral partials");
-    c->tiles_y = (nyp - 2 + VORT_TY - 1) / VORT_TY;
+    c->vort_ty = vort_rows(c->tiles_x, nyp, nl);
+    c->tiles_y = (nyp - 2 + c->vort_ty - 1) / c->vort_ty;
     int rc = 0;
     for (int t = 0; t < 2 && !rc; t++) {
         rc |= dalloc(c, &c->qbuf[t], fl);
@@ -2512,15 +2514,17 @@ static int ens_build_plan(oqg_ens* e, const std::vector<int>& act, const std::ve
         P->launches.push_back(L);
         return 0;
     };
-    {
+    for (int ty : {4, 8, 16}) {   // one stencil launch per tile height
         std::vector<int2> map;
         for (int i = 0; i < n; i++) {
             const oqg_ctx* c = e->m[P->members[i]];
+            if (c->vort_ty != ty) continue;
             const int nb = c->tiles_x * c->tiles_y * c->g.nl;
             for (int b = 0; b < nb; b++) map.push_back(make_int2(i, b));
         }
         EnsLaunch L;
         L.kind = ENS_VORT;
+        L.mb = ty;
         if (int r = add_launch(L, map)) return r;
     }
     // x-transform classes: (wavefront form?, layer count, points per lane or butterflies per thread)
@@ -2579,7 +2583,7 @@ static int ens_enqueue_step(oqg_ens* e, EnsPlan* P, int par, bool supp) {
     for (const EnsLaunch& L : P->launches) {
         switch (L.kind) {
             case ENS_VORT:
-                launch_vort_ens(P->va[par], L.map, L.nblocks, e->st);
+                launch_vort_ens(P->va[par], L.map, L.nblocks, L.mb, e->st);
                 break;
             case ENS_FWD: launch_fwd_ens(P->fa[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;
             case ENS_FWDW: launch_fwdw_ens(P->fa[par], L.map, L.nblocks, L.nl, L.mb, L.lds, e->st); break;

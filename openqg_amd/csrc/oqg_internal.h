@@ -19,7 +19,7 @@ constexpr int FWD_MAX_RADIX = 16;    // radix-16/8 butterflies: fewest LDS passe
 constexpr int INV_MAX_RADIX = 4;     // the inverse kernel's register budget keeps radix 4
 constexpr int MAX_FFT = 4096;        // longest x transform (complex points)
 constexpr int VORT_TX = 64;          // vorticity tile (x)
-constexpr int VORT_TY = 16;          // vorticity tile (y)
+constexpr int VORT_TY = 16;          // vorticity tile (y), the most rows; a basin takes 4, 8 or 16 (vort_rows)
 constexpr int MAX_YROWS_PER_LANE = 16;  // y-solver: nyp-2 <= 64*16
 // Columns per y-solver workgroup (one wavefront each).  Two per workgroup
 // (768 workgroups at 513^2 instead of 192) measured slower: 12.1 against
@@ -129,6 +129,7 @@ struct VortArgs {
     double* entpart;      // per-tile partial integrals of ent1 (layer-0 tiles)
     double tdt;
     int tiles_x, tiles_y;
+    int ty;               // tile rows (4, 8 or 16), the k_vort instantiation
     int org_x;            // first column of the tiles: 1 in a box, 0 in a channel
     // channel: the constraint integrals' partials (cpart, see CQ_*), the
     // layer-2 edge rows of ent and the resident forcing scalars (F_*)
@@ -473,7 +474,10 @@ void set_kernel_attrs();
 // members so that one launch shares the template parameters (layer count,
 // butterflies per thread, y-solver rows per lane); lds is the largest need of
 // the members in the launch.
-void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, hipStream_t st);
+void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, int ty, hipStream_t st);
+// Tile rows of k_vort for a basin with tiles_x tile columns, nyp rows and nl
+// layers: the fewest (4, 8, 16) whose blocks all fit on the device at once.
+int vort_rows(int tiles_x, int nyp, int nl);
 void launch_fwd_ens(const FwdArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st);
 void launch_ysolve_ens(const YArgs* args, const int2* map, int nblocks, int R, size_t lds, hipStream_t st);
 void launch_inv_ens(const InvArgs* args, const int2* map, int nblocks, int nl, int mb, size_t lds, hipStream_t st);

@@ -34,11 +34,12 @@ __device__ __forceinline__ double jac_int(const double* q, const double* p, int 
     return g.jfac * t;
 }
 
-// Tile (bx, by) of layer k.
-template <bool EDGE>
+// Tile (bx, by) of layer k, TY rows of 64 points.
+template <int TY, bool EDGE>
 __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int k, double* s_pm, double* s_d2,
                                           double* s_d4, double* s_q, double* s_p, double* s_red) {
-    constexpr int TX = VORT_TX, TY = VORT_TY;
+    constexpr int TX = VORT_TX;
+    static_assert(TY % 4 == 0 && TY <= 64, "four rows of 64 threads; the ring logic takes tx < TY");
     constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
     const Geo& g = a.g;
     const int tid = threadIdx.x;
@@ -266,8 +267,9 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
 
 // Tiles whose halo of 3 stays inside the domain without touching a boundary
 // point or crossing the channel's periodic seam take the branch-free body.
+template <int TY>
 __device__ __forceinline__ void vort_block(const VortArgs& a, int bx, int by, int k) {
-    constexpr int TX = VORT_TX, TY = VORT_TY;
+    constexpr int TX = VORT_TX;
     __shared__ double s_pm[(TY + 6) * (TX + 6)];
     __shared__ double s_d2[(TY + 4) * (TX + 4)];
     __shared__ double s_d4[(TY + 2) * (TX + 2)];
@@ -276,36 +278,65 @@ __device__ __forceinline__ void vort_block(const VortArgs& a, int bx, int by, in
     __shared__ double s_red[16];
     const int i0 = a.org_x + bx * TX, j0 = 1 + by * TY;
     const bool inner = i0 >= 3 && i0 + TX + 3 <= a.g.nxp && j0 >= 3 && j0 + TY + 3 <= a.g.nyp;
-    if (inner) vort_tile<false>(a, bx, by, k, s_pm, s_d2, s_d4, s_q, s_p, s_red);
-    else vort_tile<true>(a, bx, by, k, s_pm, s_d2, s_d4, s_q, s_p, s_red);
+    if (inner) vort_tile<TY, false>(a, bx, by, k, s_pm, s_d2, s_d4, s_q, s_p, s_red);
+    else vort_tile<TY, true>(a, bx, by, k, s_pm, s_d2, s_d4, s_q, s_p, s_red);
 }
 
 // Each XCD takes a contiguous run of tiles (whole rows of tiles), so the halo
 // lines that neighbouring tiles share are fetched from HBM into one L2.
+template <int TY>
 __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
     int bx, by, bz;
     xcd_block(bx, by, bz);
-    vort_block(a, bx, by, bz);
+    vort_block<TY>(a, bx, by, bz);
 }
 
 // Ensemble form: block b works on tile map[b].y of member map[b].x, whose
 // arguments are args[map[b].x] (tiles numbered x fastest, then y, then layer).
+template <int TY>
 __global__ __launch_bounds__(256) void k_vort_ens(const VortArgs* __restrict__ args, const int2* __restrict__ map) {
     const int2 e = map[blockIdx.x];
     const VortArgs& a = args[e.x];
     const int per_layer = a.tiles_x * a.tiles_y;
     const int k = e.y / per_layer, r = e.y - k * per_layer;
     const int by = r / a.tiles_x, bx = r - by * a.tiles_x;
-    vort_block(a, bx, by, k);
+    vort_block<TY>(a, bx, by, k);
+}
+
+// A tile's LDS (bytes): pm with halo 3, del2 with halo 2, del4, q and p with
+// halo 1, the reduction slots.
+static constexpr size_t vort_lds(int ty) {
+    return sizeof(double) * ((size_t)(ty + 6) * (VORT_TX + 6) + (size_t)(ty + 4) * (VORT_TX + 4) +
+                             3 * (size_t)(ty + 2) * (VORT_TX + 2) + 16);
+}
+
+// Shorter tiles shorten each block's chain of loads and LDS passes (the
+// kernel is latency-bound), at the price of more halo reads; they pay while
+// every block is resident at once: 64 x 4 tiles took k_vort from 10.4 to 6.1
+// us on the 97 x 49 channel and from 9.5 to 6.5 us on the 257^2 box, while
+// the 513^2 box stays with 64 x 16 (14.2 us against 14.7 us with 64 x 4).
+int vort_rows(int tiles_x, int nyp, int nl) {
+    constexpr int CUS = 256;
+    constexpr size_t LDS_PER_CU = 160 * 1024;
+    for (int ty : {4, 8}) {
+        const long long blocks = (long long)tiles_x * ((nyp - 2 + ty - 1) / ty) * nl;
+        const long long slots = (long long)CUS * (long long)(LDS_PER_CU / vort_lds(ty));
+        if (blocks <= slots) return ty;
+    }
+    return 16;
 }
 
 void launch_vort(const VortArgs& a, hipStream_t st) {
     dim3 grid(a.tiles_x, a.tiles_y, a.g.nl);
-    hipLaunchKernelGGL(k_vort, grid, dim3(256), 0, st, a);
+    if (a.ty == 4) hipLaunchKernelGGL(k_vort<4>, grid, dim3(256), 0, st, a);
+    else if (a.ty == 8) hipLaunchKernelGGL(k_vort<8>, grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_vort<16>, grid, dim3(256), 0, st, a);
 }
 
-void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_vort_ens, dim3(nblocks), dim3(256), 0, st, args, map);
+void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, int ty, hipStream_t st) {
+    if (ty == 4) hipLaunchKernelGGL(k_vort_ens<4>, dim3(nblocks), dim3(256), 0, st, args, map);
+    else if (ty == 8) hipLaunchKernelGGL(k_vort_ens<8>, dim3(nblocks), dim3(256), 0, st, args, map);
+    else hipLaunchKernelGGL(k_vort_ens<16>, dim3(nblocks), dim3(256), 0, st, args, map);
 }
 
 // ------------------------------------------------------------------ k_suppress
