@@ -127,39 +127,6 @@ void launch_ke_seed(const KeSeedArgs& a, hipStream_t st) {
 
 // ------------------------------------------------------------------ k_adj_vort
 
-// Coefficient that del2_P_bc at x (box) gives to f at x + o, for the five
-// offsets o of its stencil; zero elsewhere.  Mirrors del2_lds case by case.
-__device__ __forceinline__ double lap_coef(int xi, int xj, int oi, int oj, const Geo& g) {
-    if (xi < 0 || xi >= g.nxp || xj < 0 || xj >= g.nyp) return 0.0;
-    const bool xin = xi >= 1 && xi <= g.nxp - 2;
-    const bool yin = xj >= 1 && xj <= g.nyp - 2;
-    double c = 0.0;
-    if (oj == 0 && yin) {   // dP2dx2_bc, zero on rows 1 and nyp
-        if (xin) c += (oi == 0) ? -2.0 * g.fx : ((oi == 1 || oi == -1) ? g.fx : 0.0);
-        else if (xi == 0) c += (oi == 0) ? -g.cxl : ((oi == 1) ? g.cxl : 0.0);
-        else c += (oi == 0) ? g.cxr : ((oi == -1) ? -g.cxr : 0.0);
-    }
-    if (oi == 0 && xin) {   // dP2dy2_bc, zero on columns 1 and nxp
-        if (yin) c += (oj == 0) ? -2.0 * g.fy : ((oj == 1 || oj == -1) ? g.fy : 0.0);
-        else if (xj == 0) c += (oj == 0) ? -g.cyl : ((oj == 1) ? g.cyl : 0.0);
-        else c += (oj == 0) ? g.cyr : ((oj == -1) ? -g.cyr : 0.0);
-    }
-    return c;
-}
-
-// (L^T v)(a) = sum over the stencil neighbours x of a of v(x) * coef(x -> a).
-// s is an LDS tile of width W with a at (r, c) and global (ai, aj).
-__device__ __forceinline__ double lapT_lds(const double* s, int W, int r, int c, int ai, int aj, const Geo& g) {
-    double acc = 0.0;
-    const int off[5][2] = {{0, 0}, {-1, 0}, {1, 0}, {0, -1}, {0, 1}};   // x = a + off
-#pragma unroll
-    for (int n = 0; n < 5; n++) {
-        const int di = off[n][0], dj = off[n][1];
-        acc += s[(r + dj) * W + c + di] * lap_coef(ai + di, aj + dj, -di, -dj, g);
-    }
-    return acc;
-}
-
 // Forward (box interior points x): q_{n+1} = q_{n-1} + tdt F with
 //   F = jfac J(q_n, p_n) + c2 L^2 p_{n-1} - c4 L^3 p_{n-1} - [k=kt] cdrag L p_{n-1}
 //       + [k=0] (cent1 ent + cwek wekp) - [k=1] cent2 ent.
@@ -174,25 +141,37 @@ __device__ __forceinline__ double lapT_lds(const double* s, int W, int r, int c,
 // 274.3 ms with 64 x 8; 64 x 16 tiles measured 14.9 against 12.3 us for 64 x 8).
 constexpr int ADJ_TX = 64, ADJ_TY = 4;
 
-// Away from the boundary (2 <= i <= nxp-3, 2 <= j <= nyp-3, "deep" points)
-// every stencil point of lapT_lds is interior and L^T = L (the 5-point
-// Laplacian is symmetric); nearer the ring the boundary coefficients of
-// del2_P_bc make lapT_lds necessary.  The Jacobian needs no such case: mu
-// vanishes on the ring, so every nonzero term of the gather comes from an
-// interior point x, whose Arakawa coefficients are the full ones, and the
-// antisymmetry of the stencil (the coefficient of f(a) in J(f, h)(x) is minus
-// that of f(x) in J(f, h)(a), both built from the h values the two 3x3
-// neighbourhoods share) gives J_1(h)^T mu = -J(mu, h) at every point, the
-// ring included, whatever h holds outside the domain.
-__device__ __forceinline__ bool deep_point(int gi, int gj, const Geo& g) {
-    return gi >= 2 && gi <= g.nxp - 3 && gj >= 2 && gj <= g.nyp - 3;
+// (L^T v)(a) of del2_P_bc (box, numerics.F90:427-468): L = Dxx + Dyy, Dxx
+// zero on the rows 0 and nyp-1, Dyy zero on the columns 0 and nxp-1, so the
+// transpose at a gathers v(a) and v(a -/+ 1) with the coefficients Dxx has at
+// the columns a_i, a_i -/+ 1 towards a (when a's row is not a boundary row):
+// -2/dx^2 and 1/dx^2 inside, the mixed condition's cxl, cxr on columns 0 and
+// nxp-1; likewise in y.  Written as selects, so every point takes one path
+// (a per-point choice between the interior Laplacian and a general gather
+// cost 2.2 us per adjoint step at 257^2).  Neighbours outside the basin (a
+// tile's zero-filled halo) get coefficient 0.
+__device__ __forceinline__ double lapT_sel(const double* s, int W, int r, int c, int ai, int aj, const Geo& g) {
+    const bool xin = ai >= 1 && ai <= g.nxp - 2, yin = aj >= 1 && aj <= g.nyp - 2;
+    const bool xwin = ai - 1 >= 1 && ai - 1 <= g.nxp - 2, xein = ai + 1 >= 1 && ai + 1 <= g.nxp - 2;
+    const bool ysin = aj - 1 >= 1 && aj - 1 <= g.nyp - 2, ynin = aj + 1 >= 1 && aj + 1 <= g.nyp - 2;
+    const double sx = xin ? -2.0 * g.fx : (ai == 0 ? -g.cxl : g.cxr);
+    const double wx = (ai - 1 == 0) ? g.cxl : (xwin ? g.fx : 0.0);
+    const double ex = (ai + 1 == g.nxp - 1) ? -g.cxr : (xein ? g.fx : 0.0);
+    const double sy = yin ? -2.0 * g.fy : (aj == 0 ? -g.cyl : g.cyr);
+    const double ws = (aj - 1 == 0) ? g.cyl : (ysin ? g.fy : 0.0);
+    const double en = (aj + 1 == g.nyp - 1) ? -g.cyr : (ynin ? g.fy : 0.0);
+    const double* x = s + r * W + c;
+    const double tx = sx * x[0] + wx * x[-1] + ex * x[1];
+    const double ty = sy * x[0] + ws * x[-W] + en * x[W];
+    return (yin ? tx : 0.0) + (xin ? ty : 0.0);
 }
 
-// 5-point Laplacian (interior coefficients of del2_P_bc) at (r, c) of an LDS tile of width W.
-__device__ __forceinline__ double lap5_lds(const double* s, int W, int r, int c, const Geo& g) {
-    const double* x = s + r * W + c;
-    return g.fx * (x[-1] + x[1]) + g.fy * (x[-W] + x[W]) - 2.0 * (g.fx + g.fy) * x[0];
-}
+// The Jacobian transposes: mu vanishes on the ring, so every nonzero term of
+// the gather comes from an interior point x, whose Arakawa coefficients are
+// the full ones, and the antisymmetry of the stencil (the coefficient of f(a)
+// in J(f, h)(x) is minus that of f(x) in J(f, h)(a), both built from the h
+// values the two 3x3 neighbourhoods share) gives J_1(h)^T mu = -J(mu, h) at
+// every point, the ring included, whatever h holds outside the domain.
 
 // Arakawa sum J(f, h) before the factor jfac (J++ + J+x + Jx+, each times 3),
 // f and h LDS tiles of widths Wf, Wh with the point at (rf, cf) and (rh, ch).
@@ -213,12 +192,12 @@ __device__ __forceinline__ double arakawa_lds(const double* f, int Wf, int rf, i
 __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     constexpr int TX = ADJ_TX, TY = ADJ_TY;
     constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
-    constexpr int NMU = (H3 * W3 + 255) / 256, NQP = (H2 * W2 + 255) / 256, NOUT = TY / 4;
+    constexpr int NMU = (H3 * W3 + 255) / 256, NQP = (H1 * W1 + 255) / 256, NOUT = TY / 4;
     __shared__ double s_mu[H3 * W3];
     __shared__ double s_n1[H2 * W2];
     __shared__ double s_n2[H1 * W1];
-    __shared__ double s_q[H2 * W2];
-    __shared__ double s_p[H2 * W2];
+    __shared__ double s_q[H1 * W1];
+    __shared__ double s_p[H1 * W1];
     const Geo& g = a.g;
     const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
     int bx, by, k;
@@ -235,7 +214,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     }
     // every global load of the block is issued before any is used: the tile
     // of mu = tdt lq_{n+1} (interior only) with halo 3, q_n and p_n with halo
-    // 2 (from clamped, always valid offsets), and the outputs updated in place
+    // 1 (from clamped, always valid offsets), and the outputs updated in place
     double vmu[NMU], vq[NQP], vp[NQP];
 #pragma unroll
     for (int u = 0; u < NMU; u++) {
@@ -247,9 +226,9 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     }
 #pragma unroll
     for (int u = 0; u < NQP; u++) {
-        const int idx = tid + 256 * u, r = idx / W2, c = idx - r * W2;
-        const int gi = i0 - 2 + c, gj = j0 - 2 + r;
-        const bool ok = idx < H2 * W2 && gi >= 0 && gi < g.nxp && gj >= 0 && gj < g.nyp;
+        const int idx = tid + 256 * u, r = idx / W1, c = idx - r * W1;
+        const int gi = i0 - 1 + c, gj = j0 - 1 + r;
+        const bool ok = idx < H1 * W1 && gi >= 0 && gi < g.nxp && gj >= 0 && gj < g.nyp;
         const long long o = ok ? (long long)gj * g.pitch + gi : 0;
         const double q = qn[o], p = pn[o];
         vq[u] = ok ? q : 0.0;
@@ -274,7 +253,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
 #pragma unroll
     for (int u = 0; u < NQP; u++) {
         const int idx = tid + 256 * u;
-        if (idx < H2 * W2) {
+        if (idx < H1 * W1) {
             s_q[idx] = vq[u];
             s_p[idx] = vp[u];
         }
@@ -286,7 +265,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
         const int gi2 = i0 - 2 + c, gj2 = j0 - 2 + r;
         const bool in = gi2 >= 0 && gi2 < g.nxp && gj2 >= 0 && gj2 < g.nyp;
         double v = 0.0;
-        if (in) v = deep_point(gi2, gj2, g) ? lap5_lds(s_mu, W3, r + 1, c + 1, g) : lapT_lds(s_mu, W3, r + 1, c + 1, gi2, gj2, g);
+        if (in) v = lapT_sel(s_mu, W3, r + 1, c + 1, gi2, gj2, g);
         s_n1[idx] = v;
     }
     __syncthreads();
@@ -296,7 +275,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
         const int gi1 = i0 - 1 + c, gj1 = j0 - 1 + r;
         const bool in = gi1 >= 0 && gi1 < g.nxp && gj1 >= 0 && gj1 < g.nyp;
         double v = 0.0;
-        if (in) v = deep_point(gi1, gj1, g) ? lap5_lds(s_n1, W2, r + 1, c + 1, g) : lapT_lds(s_n1, W2, r + 1, c + 1, gi1, gj1, g);
+        if (in) v = lapT_sel(s_n1, W2, r + 1, c + 1, gi1, gj1, g);
         s_n2[idx] = v;
     }
     __syncthreads();
@@ -306,12 +285,11 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
         const int r = ty + 4 * u, gj = j0 + r;
         if (gi >= g.nxp || gj >= g.nyp) continue;
         const long long o = lo + (long long)gj * g.pitch + gi;
-        const double nu3 = deep_point(gi, gj, g) ? lap5_lds(s_n2, W1, r + 1, tx + 1, g)
-                                                 : lapT_lds(s_n2, W1, r + 1, tx + 1, gi, gj, g);
+        const double nu3 = lapT_sel(s_n2, W1, r + 1, tx + 1, gi, gj, g);
         // J_1(h)^T mu = -J(mu, h) at every point, the ring included (mu is 0
-        // there; see the note above deep_point)
-        const double jq = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_p, W2, r + 2, tx + 2);
-        const double jp = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_q, W2, r + 2, tx + 2);
+        // there; see the note above arakawa_lds)
+        const double jq = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_p, W1, r + 1, tx + 1);
+        const double jp = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_q, W1, r + 1, tx + 1);
         const double nu2 = s_n2[(r + 1) * W1 + tx + 1];
         const double nu1 = s_n1[(r + 2) * W2 + tx + 2];
         double lpm = c2 * nu2 - c4 * nu3;
