@@ -518,6 +518,9 @@ def main():
         "frac": round(ach / HBM_PEAK_GBS, 4),
         "traffic": load_traffic(dom, w.name),
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
+        # every kernel of the step against the same roof (the longest is the one above)
+        "kernel_frac": {k: round(algorithmic_bytes(w, k) / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                        for k, v in kt.items() if v > 0},
         "step_algorithmic_GBs": round(algorithmic_bytes(w, "step") / (ms_per_step * 1e-3) / 1e9, 1),
     }
     cpu = None
