@@ -90,9 +90,22 @@ struct oqg_ctx {
     int tiles_x = 0, tiles_y = 0, vort_ty = VORT_TY;
     size_t fbytes = 0;     // bytes of one nl-layer field (pitch layout)
     // device memory
+    // the dynamic state: q[0], q[1], p[0], p[1], c[0], c[1] in one allocation
+    // laid out as a snapshot slot (set_work), so that a checkpoint can take
+    // it over instead of copying it
     double* qbuf[2] = {nullptr, nullptr};
     double* pbuf[2] = {nullptr, nullptr};
     double* cbuf[2] = {nullptr, nullptr};
+    double* wbase = nullptr;        // the allocation qbuf/pbuf/cbuf point into
+    double* wbase_own = nullptr;    // the context's own one (wbase except inside a checkpointed sweep)
+    // Levels of the state that a step reads from elsewhere than qbuf/pbuf/cbuf
+    // (a checkpoint it starts from): the current level (cur) and the previous
+    // one; null q: the level is in the working buffers.
+    struct LevelRef {
+        const double* q = nullptr;
+        const double* p = nullptr;
+        const double* c = nullptr;
+    } ext_cur, ext_old;
     double* spec = nullptr;
     double* gamw = nullptr;
     double* colsum = nullptr;
@@ -150,7 +163,7 @@ struct oqg_ctx {
     std::vector<std::array<int, 4>> rv_left;
     std::array<int, 3> rv_tail{};
     int rv_n = -1, rv_supp = 0;
-    int rv_loaded = -1;   // reverse sweep: the slot the loaded state equals, or -1
+    bool rv_ref = false;   // reverse sweep: checkpoints taken and restored by reference
     double rv_tdt = 0.0;
     long long rv_steps = 0;
     std::map<std::pair<int, int>, int> rv_split;   // memo of revolve_split
@@ -500,15 +513,30 @@ struct StepArgs {
     InvArgs ia;
 };
 
+// The working buffers inside the slot-shaped allocation base.
+static void set_work(oqg_ctx* c, double* base) {
+    const long long fl = (long long)c->g.nl * c->g.fs;
+    c->wbase = base;
+    c->qbuf[0] = base;
+    c->qbuf[1] = base + fl;
+    c->pbuf[0] = base + 2 * fl;
+    c->pbuf[1] = base + 3 * fl;
+    c->cbuf[0] = base + 4 * fl;
+    c->cbuf[1] = base + 4 * fl + C_LEN;
+}
+
 static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     const int cur = c->cur, nxt = 1 - cur;
     std::memset(&a, 0, sizeof a);
     VortArgs& va = a.va;
     va.g = c->g;
     va.ph = &c->dconst->ph;
-    va.q = c->qbuf[cur];
-    va.p = c->pbuf[cur];
-    va.pm = c->pbuf[nxt];
+    // inputs: the current and previous levels (from a checkpoint when a step
+    // starts from one); outputs: the working buffers of the next level
+    va.q = c->ext_cur.q ? c->ext_cur.q : c->qbuf[cur];
+    va.p = c->ext_cur.q ? c->ext_cur.p : c->pbuf[cur];
+    va.pm = c->ext_old.q ? c->ext_old.p : c->pbuf[nxt];
+    va.qm = c->ext_old.q ? c->ext_old.q : c->qbuf[nxt];
     va.qnew = c->qbuf[nxt];
     va.ent1 = c->ent1;
     va.wekp = c->wekp;
@@ -542,7 +570,7 @@ static void make_step_args(oqg_ctx* c, double tdt, StepArgs& a) {
     ya.nmodes = c->g.nl;
     InvArgs& ia = a.ia;
     fill_inv_common(c, ia);
-    ia.cin = c->cbuf[cur];
+    ia.cin = c->ext_cur.q ? c->ext_cur.c : c->cbuf[cur];
     ia.cout = c->cbuf[nxt];
     ia.pnew = c->pbuf[nxt];
     ia.qnew = c->qbuf[nxt];
@@ -558,23 +586,27 @@ static void launch_kernel(oqg_ctx* c, const StepArgs& a, int k) {
     }
 }
 
-// One step_qg on resident forcing.
+// One step_qg on resident forcing.  The new level goes to the working
+// buffers; the level that was current becomes the previous one, wherever it is.
 static int enqueue_step(oqg_ctx* c, double tdt) {
     StepArgs a;
     make_step_args(c, tdt, a);
     for (int k = 0; k < NK; k++) launch_kernel(c, a, k);
     HIPCHK(c, hipGetLastError());
     c->cur = 1 - c->cur;
+    c->ext_old = c->ext_cur;
+    c->ext_cur = {};
     return 0;
 }
 
 static int enqueue_suppress(oqg_ctx* c) {
+    if (c->ext_cur.q) return fail(c, OQG_ERR_STATE, "suppression of a level the working buffers do not hold");
     SuppArgs sa{};
     sa.g = c->g;
     sa.q = c->qbuf[c->cur];
-    sa.qm = c->qbuf[1 - c->cur];
+    sa.qm = c->ext_old.q ? c->ext_old.q : c->qbuf[1 - c->cur];
     sa.p = c->pbuf[c->cur];
-    sa.pm = c->pbuf[1 - c->cur];
+    sa.pm = c->ext_old.q ? c->ext_old.p : c->pbuf[1 - c->cur];
     sa.c = c->cbuf[c->cur];
     launch_suppress(sa, c->st);
     HIPCHK(c, hipGetLastError());
@@ -702,13 +734,12 @@ static int setup(oqg_ctx* c) {
     if (g.cyclic && c->tiles_x > CYC_MAX_TX) return fail(c, OQG_ERR_ARG, "channel too wide for the boundary-integral partials");
     c->vort_ty = vort_rows(c->tiles_x, nyp, nl);
     c->tiles_y = (nyp - 2 + c->vort_ty - 1) / c->vort_ty;
-    int rc = 0;
-    for (int t = 0; t < 2 && !rc; t++) {
-        rc |= dalloc(c, &c->qbuf[t], fl);
-        rc |= dalloc(c, &c->pbuf[t], fl);
-        rc |= dalloc(c, &c->cbuf[t], C_LEN);
-        rc |= dalloc(c, &c->raw[t], f1);
+    int rc = dalloc(c, &c->wbase, 4 * fl + 2 * (size_t)C_LEN);
+    if (!rc) {
+        c->wbase_own = c->wbase;
+        set_work(c, c->wbase);
     }
+    for (int t = 0; t < 2 && !rc; t++) rc |= dalloc(c, &c->raw[t], f1);
     if (!rc) rc = dalloc(c, &c->spec, nspec);
     if (!rc && c->variant == 1) rc = dalloc(c, &c->gamw, nspec);
     if (!rc) rc = dalloc(c, &c->colsum, (size_t)nl * s.ncol);
@@ -1650,6 +1681,7 @@ static int time_kernels(oqg_ctx* c, double tdt, int reps, hipEvent_t* ev, double
         for (int r = 0; r < reps; r++) {
             StepArgs b = a;
             b.va.qnew = static_cast<double*>(tmp[r]);
+            b.va.qm = b.va.qnew;
             b.ya.spec = static_cast<double*>(tmp[reps + r]);
             launch_kernel(c, (k == KV || k == KY) ? b : a, k);
         }
@@ -2247,32 +2279,89 @@ static int adj_step_at(oqg_ctx* c, double tdt, bool suppressed, int slot) {
         const double* s = c->snaps[slot];
         src.q = s + (c->cur ? fl : 0);
         src.p = s + 2 * fl + (c->cur ? fl : 0);
+    } else if (c->ext_cur.q) {   // the loaded state's current level is a checkpoint's
+        src.q = c->ext_cur.q;
+        src.p = c->ext_cur.p;
     }
     return enqueue_adj_step(c, tdt, suppressed, src);
 }
 
+// Level t of the snapshot slot at s (q, p and the constraint block).
+static oqg_ctx::LevelRef slot_level(const oqg_ctx* c, const double* s, int t) {
+    const long long fl = (long long)c->g.nl * c->g.fs;
+    oqg_ctx::LevelRef r;
+    r.q = s + t * fl;
+    r.p = s + 2 * fl + t * fl;
+    r.c = s + 4 * fl + t * C_LEN;
+    return r;
+}
+
+// Copies into the working buffers the levels a step would read from a
+// checkpoint, so that they hold the whole state again.
+static void materialize(oqg_ctx* c) {
+    if (!c->ext_cur.q && !c->ext_old.q) return;
+    const long long fl = (long long)c->g.nl * c->g.fs;
+    CopySegs a{};
+    int n = 0;
+    auto add = [&](const double* src, double* dst, long long cnt) {
+        a.src[n] = src;
+        a.dst[n] = dst;
+        a.n[n] = cnt;
+        n++;
+    };
+    const int cur = c->cur;
+    if (c->ext_cur.q) {
+        add(c->ext_cur.q, c->qbuf[cur], fl);
+        add(c->ext_cur.p, c->pbuf[cur], fl);
+        add(c->ext_cur.c, c->cbuf[cur], C_LEN);
+    }
+    if (c->ext_old.q) {
+        add(c->ext_old.q, c->qbuf[1 - cur], fl);
+        add(c->ext_old.p, c->pbuf[1 - cur], fl);
+        add(c->ext_old.c, c->cbuf[1 - cur], C_LEN);
+    }
+    a.nseg = n;
+    launch_copy_segs(a, c->st);
+    c->ext_cur = {};
+    c->ext_old = {};
+}
+
+// Forward sweep: checkpoints are copies and the advances replay graphs.
+// Reverse sweep (rv_ref): a checkpoint takes over the working allocation
+// (the working buffers move to the slot's), a restore only points the next
+// step's inputs at the slot, and the advances launch their steps one by one
+// (the graphs hold the context's own buffers).  The first two steps after a
+// checkpoint read their inputs from it and write the working buffers, so no
+// state is copied: a copy of the 257^2 x 3 state took 4.4 us, about 2100 of
+// them a reverse sweep of 2160 steps with 32 slots.
 static int rv_advance(oqg_ctx* c, int lo, int hi) {
     if (hi <= lo) return 0;
     c->rv_steps += hi - lo;
-    c->rv_loaded = -1;
+    if (c->rv_ref) return run_eager(c, c->rv_tdt, lo + 1, hi - lo, c->rv_supp);
     return oqg_run(c, c->rv_tdt, hi - lo, lo, c->rv_supp);
 }
 
-// Checkpoint copies of the reverse sweep.  rv_loaded is the slot whose content
-// the loaded state still equals (adjoint steps leave the forward state alone),
-// so restoring the checkpoint just taken costs nothing.
 static void rv_save(oqg_ctx* c, int slot) {
-    snap_copy(c, slot, true);
-    c->rv_loaded = slot;
+    if (!c->rv_ref) {
+        snap_copy(c, slot, true);
+        return;
+    }
+    materialize(c);
+    std::swap(c->wbase, c->snaps[slot]);
+    set_work(c, c->wbase);
+    c->snap_cur[slot] = c->cur;
+    c->ext_cur = slot_level(c, c->snaps[slot], c->cur);
+    c->ext_old = slot_level(c, c->snaps[slot], 1 - c->cur);
 }
 
 static void rv_restore(oqg_ctx* c, int slot) {
-    if (c->rv_loaded == slot) {
-        c->cur = c->snap_cur[slot];
+    if (!c->rv_ref) {
+        snap_copy(c, slot, false);
         return;
     }
-    snap_copy(c, slot, false);
-    c->rv_loaded = slot;
+    c->cur = c->snap_cur[slot];
+    c->ext_cur = slot_level(c, c->snaps[slot], c->cur);
+    c->ext_old = slot_level(c, c->snaps[slot], 1 - c->cur);
 }
 
 // The state lo is in `slot`; the adjoint holds that of state hi; leave it at state lo.
@@ -2355,10 +2444,29 @@ int oqg_adj_revolve_reverse(oqg_ctx* c, long long* forward_steps) {
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     if (c->rv_n < 1) return fail(c, OQG_ERR_STATE, "oqg_adj_revolve_reverse: no forward sweep recorded");
-    c->rv_loaded = -1;   // the caller may have changed the state since the forward sweep
-    if (int r = rv_reverse(c, c->rv_tail[0], c->rv_n, c->rv_tail[1], c->rv_tail[2])) return r;
-    for (auto it = c->rv_left.rbegin(); it != c->rv_left.rend(); ++it)
-        if (int r = rv_reverse(c, (*it)[0], (*it)[1], (*it)[2], (*it)[3])) return r;
+    c->rv_ref = true;
+    int rc = rv_reverse(c, c->rv_tail[0], c->rv_n, c->rv_tail[1], c->rv_tail[2]);
+    for (auto it = c->rv_left.rbegin(); !rc && it != c->rv_left.rend(); ++it)
+        rc = rv_reverse(c, (*it)[0], (*it)[1], (*it)[2], (*it)[3]);
+    c->rv_ref = false;
+    // the context takes its own allocation back (slot 0, the initial state,
+    // never changed hands) and loads the initial state into it
+    c->ext_cur = {};
+    c->ext_old = {};
+    if (c->wbase != c->wbase_own) {
+        for (double*& sl : c->snaps)
+            if (sl == c->wbase_own) {
+                std::swap(sl, c->wbase);
+                break;
+            }
+        set_work(c, c->wbase);
+        if (c->snap_ptrs_dev) {
+            HIPCHK(c, hipMemcpyAsync(c->snap_ptrs_dev, c->snaps.data(), c->snaps.size() * sizeof(double*),
+                                     hipMemcpyHostToDevice, c->st));
+            HIPCHK(c, hipStreamSynchronize(c->st));
+        }
+    }
+    if (rc) return rc;
     snap_copy(c, 0, false);
     c->rv_n = -1;
     if (forward_steps) *forward_steps = c->rv_steps;

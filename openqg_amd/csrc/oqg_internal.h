@@ -123,7 +123,8 @@ struct VortArgs {
     const double* q;      // current q
     const double* p;      // current p
     const double* pm;     // previous p
-    double* qnew;         // in: qm, out: q_new (same buffer)
+    const double* qm;     // previous q (normally qnew's buffer: q_new is written over it)
+    double* qnew;         // out: q_new
     const double* ent1;   // (pitch layout) layer-1 entrainment
     const double* wekp;
     double* entpart;      // per-tile partial integrals of ent1 (layer-0 tiles)

@@ -95,7 +95,7 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
         const int gj = j0 + ty + 4 * u;
         const bool in = !EDGE || (gi < lim_x && gj < lim_y);
         const long long o2 = in ? (long long)gj * g.pitch + gi : 0;
-        rqm[u] = a.qnew[lo + o2];
+        rqm[u] = a.qm[lo + o2];
         rent[u] = (k <= 1) ? a.ent1[o2] : 0.0;
         rwek[u] = (k == 0) ? a.wekp[o2] : 0.0;
     }
