@@ -154,6 +154,9 @@ struct oqg_ctx {
     // state snapshots (oqg_snap_*): q[0..1], p[0..1], c[0..1] and the parity per slot
     std::vector<double*> snaps;
     std::vector<int> snap_cur;
+    // reverse sweep: a slot's level held by another slot (q null: its own),
+    // see rv_save
+    std::vector<std::array<LevelRef, 2>> snap_ref;
     // device copies for graph-replayed record/sweep stretches: the slot
     // pointers and the first slot of the stretch being replayed
     double** snap_ptrs_dev = nullptr;
@@ -2272,20 +2275,6 @@ static int rev_split(oqg_ctx* c, int n, int s) {
 
 // One adjoint step (eager launches): q_n, p_n from snapshot `slot`, or from
 // the loaded state when slot < 0.  c->cur must be the parity of that state.
-static int adj_step_at(oqg_ctx* c, double tdt, bool suppressed, int slot) {
-    SnapRef src;
-    if (slot >= 0) {
-        const long long fl = (long long)c->g.nl * c->g.fs;
-        const double* s = c->snaps[slot];
-        src.q = s + (c->cur ? fl : 0);
-        src.p = s + 2 * fl + (c->cur ? fl : 0);
-    } else if (c->ext_cur.q) {   // the loaded state's current level is a checkpoint's
-        src.q = c->ext_cur.q;
-        src.p = c->ext_cur.p;
-    }
-    return enqueue_adj_step(c, tdt, suppressed, src);
-}
-
 // Level t of the snapshot slot at s (q, p and the constraint block).
 static oqg_ctx::LevelRef slot_level(const oqg_ctx* c, const double* s, int t) {
     const long long fl = (long long)c->g.nl * c->g.fs;
@@ -2294,6 +2283,26 @@ static oqg_ctx::LevelRef slot_level(const oqg_ctx* c, const double* s, int t) {
     r.p = s + 2 * fl + t * fl;
     r.c = s + 4 * fl + t * C_LEN;
     return r;
+}
+
+// Where level t of checkpoint `slot` is: its own allocation, or (reverse
+// sweep) the slot that held it when the checkpoint was taken.
+static oqg_ctx::LevelRef slot_lvl(const oqg_ctx* c, int slot, int t) {
+    if (slot < (int)c->snap_ref.size() && c->snap_ref[slot][t].q) return c->snap_ref[slot][t];
+    return slot_level(c, c->snaps[slot], t);
+}
+
+static int adj_step_at(oqg_ctx* c, double tdt, bool suppressed, int slot) {
+    SnapRef src;
+    if (slot >= 0) {
+        const oqg_ctx::LevelRef l = slot_lvl(c, slot, c->cur);
+        src.q = l.q;
+        src.p = l.p;
+    } else if (c->ext_cur.q) {   // the loaded state's current level is a checkpoint's
+        src.q = c->ext_cur.q;
+        src.p = c->ext_cur.p;
+    }
+    return enqueue_adj_step(c, tdt, suppressed, src);
 }
 
 // Copies into the working buffers the levels a step would read from a
@@ -2333,7 +2342,11 @@ static void materialize(oqg_ctx* c) {
 // (the graphs hold the context's own buffers).  The first two steps after a
 // checkpoint read their inputs from it and write the working buffers, so no
 // state is copied: a copy of the 257^2 x 3 state took 4.4 us, about 2100 of
-// them a reverse sweep of 2160 steps with 32 slots.
+// them a reverse sweep of 2160 steps with 32 slots.  Most checkpoints follow a
+// single step from the previous one, whose current level is then the new
+// checkpoint's previous level: the new slot refers to it there.  That slot
+// is not written while the new one is in use (revolve's sub-sweeps use only
+// higher slots), so the reference stays valid.
 static int rv_advance(oqg_ctx* c, int lo, int hi) {
     if (hi <= lo) return 0;
     c->rv_steps += hi - lo;
@@ -2346,12 +2359,15 @@ static void rv_save(oqg_ctx* c, int slot) {
         snap_copy(c, slot, true);
         return;
     }
-    materialize(c);
+    if (c->ext_cur.q) materialize(c);   // (every checkpoint follows a step)
+    const oqg_ctx::LevelRef old = c->ext_old;   // the previous level, if a checkpoint holds it
     std::swap(c->wbase, c->snaps[slot]);
     set_work(c, c->wbase);
     c->snap_cur[slot] = c->cur;
-    c->ext_cur = slot_level(c, c->snaps[slot], c->cur);
-    c->ext_old = slot_level(c, c->snaps[slot], 1 - c->cur);
+    c->snap_ref[slot][c->cur] = {};
+    c->snap_ref[slot][1 - c->cur] = old;
+    c->ext_cur = slot_lvl(c, slot, c->cur);
+    c->ext_old = slot_lvl(c, slot, 1 - c->cur);
 }
 
 static void rv_restore(oqg_ctx* c, int slot) {
@@ -2360,8 +2376,8 @@ static void rv_restore(oqg_ctx* c, int slot) {
         return;
     }
     c->cur = c->snap_cur[slot];
-    c->ext_cur = slot_level(c, c->snaps[slot], c->cur);
-    c->ext_old = slot_level(c, c->snaps[slot], 1 - c->cur);
+    c->ext_cur = slot_lvl(c, slot, c->cur);
+    c->ext_old = slot_lvl(c, slot, 1 - c->cur);
 }
 
 // The state lo is in `slot`; the adjoint holds that of state hi; leave it at state lo.
@@ -2445,10 +2461,12 @@ int oqg_adj_revolve_reverse(oqg_ctx* c, long long* forward_steps) {
     if (int r = adj_require(c)) return r;
     if (c->rv_n < 1) return fail(c, OQG_ERR_STATE, "oqg_adj_revolve_reverse: no forward sweep recorded");
     c->rv_ref = true;
+    c->snap_ref.assign(c->snaps.size(), std::array<oqg_ctx::LevelRef, 2>{});
     int rc = rv_reverse(c, c->rv_tail[0], c->rv_n, c->rv_tail[1], c->rv_tail[2]);
     for (auto it = c->rv_left.rbegin(); !rc && it != c->rv_left.rend(); ++it)
         rc = rv_reverse(c, (*it)[0], (*it)[1], (*it)[2], (*it)[3]);
     c->rv_ref = false;
+    c->snap_ref.clear();
     // the context takes its own allocation back (slot 0, the initial state,
     // never changed hands) and loads the initial state into it
     c->ext_cur = {};
