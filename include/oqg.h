@@ -184,10 +184,10 @@ int oqg_kernel_times(oqg_ctx* ctx, double tdt, int reps, double* ms, int max, ch
                      int names_len);
 
 /*
- * Adjoint (reverse mode) of step_qg for box basins.  No counterpart in the
- * reference, which has no derivative code; these serve gradient studies of
- * the model (e.g. of a kinetic-energy diagnostic, qg_monitor.F90:331-333,
- * with respect to the wind stress and the initial PV).
+ * Adjoint (reverse mode) of step_qg, box and channel basins.  No counterpart
+ * in the reference, which has no derivative code; these serve gradient
+ * studies of the model (e.g. of a kinetic-energy diagnostic,
+ * qg_monitor.F90:331-333, with respect to the wind stress and the initial PV).
  *
  * The adjoint state mirrors the forward state: with the forward state
  * S = (q, qm, p, pm) loaded, oqg_adj_set/get address the adjoints of q, qm,
@@ -196,19 +196,35 @@ int oqg_kernel_times(oqg_ctx* ctx, double tdt, int reps, double* ms, int max, ch
  * to those of the state before it; the forward state before the step must be
  * loaded (e.g. from a snapshot).  The adjoint of the resident Ekman pumping
  * field accumulates across steps.  Asynchronous on the context's stream.
+ *
+ * A channel's state is periodic in x with column nxp-1 a copy of column 0:
+ * its adjoints are those of the periodic fields.  oqg_adj_set adds column
+ * nxp-1 of its inputs into column 0, and every adjoint the library returns
+ * holds 0 in column nxp-1 (a derivative with respect to a periodic point sits
+ * in its column 0..nxp-2).  A channel's adjoint state also holds the
+ * adjoints of the momentum-constraint scalars cs, cn, csp, cnp
+ * (constraint.F90:182-215): oqg_adj_set zeroes them, oqg_adj_set_constr /
+ * oqg_adj_get_constr address them in oqg_get_constr's packing (the dpi,
+ * dpip entries are ignored and read 0), and the adjoints of the forcing
+ * scalars txis, txin accumulate across steps beside that of wekp
+ * (oqg_adj_get_txi, 2 doubles on the host; cleared by oqg_adj_zero_wekp).
+ * oqg_adj_init rejects channels with nyp < 9.
  */
 int oqg_adj_init(oqg_ctx* ctx);
 int oqg_adj_set(oqg_ctx* ctx, const double* lq, const double* lqm, const double* lp, const double* lpm,
                 int kind);
 int oqg_adj_get(oqg_ctx* ctx, double* lq, double* lqm, double* lp, double* lpm, int kind);
+int oqg_adj_set_constr(oqg_ctx* ctx, const double* v);
+int oqg_adj_get_constr(oqg_ctx* ctx, double* v);
 int oqg_adj_step(oqg_ctx* ctx, double tdt, int suppressed);
 int oqg_adj_get_wekp(oqg_ctx* ctx, double* out, int kind);
+int oqg_adj_get_txi(oqg_ctx* ctx, double* out);
 int oqg_adj_zero_wekp(oqg_ctx* ctx);
 /* Seeds the adjoint with the layer-1 kinetic energy of the loaded state,
  * J = ekei(1) = 0.5 rho H_1 int_P_dA(u^2 + v^2), u = -dPdy_2bc(p_1)/f0,
  * v = dPdx_2bc(p_1)/f0 (qg_monitor.F90:297-302,331-333): the adjoints of q,
- * qm and pm are zeroed and that of p is dJ/dp (layer 1; 0 below), all on the
- * device; *J (host) receives the objective (box basins). */
+ * qm, pm and the constraint scalars are zeroed and that of p is dJ/dp
+ * (layer 1; 0 below), all on the device; *J (host) receives the objective. */
 int oqg_adj_seed_ke(oqg_ctx* ctx, double rho, double* J);
 
 /* Wind-stress forcing on the device: ekman_from_tau (src/coupler/ekman.F90:53-83)
@@ -216,9 +232,9 @@ int oqg_adj_seed_ke(oqg_ctx* ctx, double rho, double* J);
  * (src/glam/numerics.F90:64-107,271-289).  oqg_set_wind takes the kinematic
  * stress (taux, tauy) at p points, each (nxp,nyp), and sets the resident
  * Ekman pumping wekp (and, for a channel, txis/txin); the resident
- * entrainment is left as it is.  oqg_adj_wind (box) applies the transpose to
- * the accumulated adjoint of wekp (oqg_adj_get_wekp) and returns the adjoints
- * of taux and tauy. */
+ * entrainment is left as it is.  oqg_adj_wind applies the transpose to the
+ * accumulated adjoints of wekp (oqg_adj_get_wekp) and, for a channel, of
+ * txis and txin, and returns the adjoints of taux and tauy. */
 int oqg_set_wind(oqg_ctx* ctx, const double* taux, const double* tauy, int kind);
 int oqg_adj_wind(oqg_ctx* ctx, double* ltaux, double* ltauy, int kind);
 /* The resident forcing: wekp (nxp,nyp) and txi = {txis, txin} (host, 2

@@ -110,6 +110,9 @@ SIGNATURES = [
     ("oqg_adj_init", _i, [_vp]),
     ("oqg_adj_set", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
     ("oqg_adj_get", _i, [_vp, _vp, _vp, _vp, _vp, _i]),
+    ("oqg_adj_set_constr", _i, [_vp, _dp]),
+    ("oqg_adj_get_constr", _i, [_vp, _dp]),
+    ("oqg_adj_get_txi", _i, [_vp, _dp]),
     ("oqg_adj_step", _i, [_vp, _d, _i]),
     ("oqg_adj_get_wekp", _i, [_vp, _vp, _i]),
     ("oqg_adj_zero_wekp", _i, [_vp]),

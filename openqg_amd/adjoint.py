@@ -1,4 +1,5 @@
-"""Gradients of the QG model through the hand-written adjoint of step_qg (box).
+"""Gradients of the QG model through the hand-written adjoint of step_qg
+(box or zonally periodic channel).
 
 The objective is the layer-1 kinetic energy of qg_monitor (ekei(1),
 src/diagnostics/qg_monitor.F90:297-302,331-333) after N steps:
@@ -14,6 +15,12 @@ oqg_adj_step from them.  With `snapshots=None` every state is kept (HBM holds
 thousands of them; oqg_adj_record / oqg_adj_sweep run both sweeps from C);
 with a number, binomial checkpointing (Griewank's revolve schedule)
 recomputes the forward from the stored ones with graph-replayed oqg_run.
+
+In a channel the fields are periodic in x with column nxp-1 a copy of column
+0: the controls given keep that copy, and the gradients returned are those of
+the periodic fields (the derivative with respect to a periodic point sits in
+its column 0..nxp-2, column nxp-1 holds 0); the stress gradient includes the
+paths through the boundary stress integrals txis, txin.
 """
 from __future__ import annotations
 
@@ -108,7 +115,7 @@ def ekei1(p1: np.ndarray, dx: float, dy: float, fnot: float, bcco: float, rho: f
 # ------------------------------------------------------------------ device adjoint state
 
 class AdjointState:
-    """The adjoint buffers and state snapshots of one box basin."""
+    """The adjoint buffers and state snapshots of one basin."""
 
     def __init__(self, qg: QG):
         self.qg = qg
@@ -134,6 +141,25 @@ class AdjointState:
         keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (lq, lqm, lp, lpm)]
         ptrs = [None if a is None else a.ctypes.data_as(ctypes.c_void_p) for a in keep]
         self._c(self._lib.oqg_adj_set(self.qg._h, *ptrs, OQG_HOST), "oqg_adj_set")
+
+    def set_constr(self, v):
+        """The adjoints of the constraint scalars, in QG.get_constr's packing."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._c(self._lib.oqg_adj_set_constr(self.qg._h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                "oqg_adj_set_constr")
+
+    def get_constr(self) -> np.ndarray:
+        out = np.zeros(self._lib.oqg_constr_len(self.qg._h))
+        self._c(self._lib.oqg_adj_get_constr(self.qg._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                "oqg_adj_get_constr")
+        return out
+
+    def txi(self) -> np.ndarray:
+        """The accumulated adjoints of the channel's txis, txin."""
+        out = np.zeros(2)
+        self._c(self._lib.oqg_adj_get_txi(self.qg._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                "oqg_adj_get_txi")
+        return out
 
     def seed_ke(self, rho: float) -> float:
         """J = ekei(1) of the loaded state and the adjoint seeded with dJ/dp
@@ -329,8 +355,6 @@ def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: fl
     (nl, nyp, nxp; ring ignored) under the kinematic wind stress (taux, tauy);
     returns (J, dJ/dtaux, dJ/dtauy, dJ/dq0 (interior, ring 0), forward steps run)."""
     b = qg.b
-    if b.cyclic:
-        raise ValueError("the adjoint is implemented for box basins")
     adj = AdjointState(qg)
     ent = np.zeros((max(b.nl - 1, 1), b.nyp, b.nxp))
     qg.set_forcing(ent, None, 0.0, 0.0)

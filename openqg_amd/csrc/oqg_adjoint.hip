@@ -1,4 +1,4 @@
-// oqg_adjoint.hip -- reverse-mode (adjoint) kernels of one box step_qg.
+// oqg_adjoint.hip -- reverse-mode (adjoint) kernels of one step_qg (box or channel).
 //
 // The step maps S_n = (q_n, q_{n-1}, p_n, p_{n-1}) to S_{n+1}.  Given the
 // adjoint of S_{n+1} these kernels produce the adjoint of S_n (and accumulate
@@ -28,15 +28,22 @@ namespace oqg {
 
 // ------------------------------------------------------------------ k_adj_supp
 
-// q <- (q + qm)/2, p <- (p + pm)/2 transposed: lqm += lq/2, lq <- lq/2.
+// q <- (q + qm)/2, p <- (p + pm)/2 transposed: lqm += lq/2, lq <- lq/2;
+// in a channel also cs <- (cs + csp)/2, cn <- (cn + cnp)/2.
 __global__ void k_adj_supp(AdjSuppArgs a) {
     const long long n = (long long)a.g.nl * a.g.fs;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const long long t0 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    for (long long t = t0; t < n; t += (long long)gridDim.x * blockDim.x) {
         const double hq = 0.5 * a.lq[t], hp = 0.5 * a.lp[t];
         a.lqm[t] += hq;
         a.lq[t] = hq;
         a.lpm[t] += hp;
         a.lp[t] = hp;
+    }
+    if (a.lc && t0 < LC_LEN) {
+        const double h = 0.5 * a.lc[t0];
+        a.lcm[t0] += h;
+        a.lc[t0] = h;
     }
 }
 
@@ -121,8 +128,44 @@ __global__ __launch_bounds__(256) void k_ke_seed(KeSeedArgs a) {
     a.lp[(long long)j * pitch + i] = -ty / g.fnot + tx / g.fnot;
 }
 
+// Channel: on the torus of the nxp-1 distinct columns, where int_P_dA's x
+// weights are 1 and dPdx_2bc is the periodic centred difference (0 on rows 0
+// and nyp-1); column nxp-1 of the result is 0.
+__global__ __launch_bounds__(256) void k_ke_seed_cyc(KeSeedArgs a) {
+    const Geo& g = a.g;
+    const int L = g.nxp - 1, N = g.nyp - 1;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= g.nxp) return;
+    const long long pitch = g.pitch;
+    if (i == L) {
+        a.lp[(long long)j * pitch + i] = 0.0;
+        return;
+    }
+    const double ca = 0.5 * a.bcco + 1.0, cy = g.dy * ca;
+    auto wrap = [&](int x) { return x < 0 ? x + L : (x >= L ? x - L : x); };
+    auto P = [&](int x, int y) { return a.p[(long long)y * pitch + wrap(x)]; };
+    auto wgt = [&](int y) { return ((y == 0 || y == N) ? 0.5 : 1.0) * g.dx * g.dy; };
+    auto dpdx = [&](int x, int y) { return (y == 0 || y == N) ? 0.0 : (P(x + 1, y) - P(x - 1, y)) / (2.0 * g.dx); };
+    auto dpdy = [&](int x, int y) {
+        if (y >= 1 && y <= N - 1) return (P(x, y + 1) - P(x, y - 1)) / (2.0 * g.dy);
+        return y == 0 ? (P(x, 1) - P(x, 0)) / cy : (P(x, N) - P(x, N - 1)) / cy;
+    };
+    auto GU = [&](int x, int y) { return a.rhoh * wgt(y) * (-dpdy(x, y) / g.fnot); };
+    auto GV = [&](int x, int y) { return a.rhoh * wgt(y) * (dpdx(x, y) / g.fnot); };
+    double tx = 0.0, ty = 0.0;
+    if (j != 0 && j != N) tx = (GV(i - 1, j) - GV(i + 1, j)) / (2.0 * g.dx);
+    if (j >= 2) ty += GU(i, j - 1) / (2.0 * g.dy);
+    if (j <= N - 2) ty -= GU(i, j + 1) / (2.0 * g.dy);
+    if (j == 1) ty += GU(i, 0) / cy;
+    if (j == 0) ty -= GU(i, 0) / cy;
+    if (j == N) ty += GU(i, N) / cy;
+    if (j == N - 1) ty -= GU(i, N) / cy;
+    a.lp[(long long)j * pitch + i] = -ty / g.fnot + tx / g.fnot;
+}
+
 void launch_ke_seed(const KeSeedArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_ke_seed, dim3((a.g.nxp + 255) / 256, a.g.nyp), dim3(256), 0, st, a);
+    if (a.g.cyclic) hipLaunchKernelGGL(k_ke_seed_cyc, dim3((a.g.nxp + 255) / 256, a.g.nyp), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_ke_seed, dim3((a.g.nxp + 255) / 256, a.g.nyp), dim3(256), 0, st, a);
 }
 
 // ------------------------------------------------------------------ k_adj_vort
@@ -304,6 +347,174 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ k_adj_vort_cyc
+
+// Channel.  The state is periodic with period nxp-1 in x (column nxp-1 repeats
+// column 0), so the transposes are taken on the torus of the nxp-1 distinct
+// columns: every adjoint field holds the sensitivity of a periodic point in
+// its column 0..nxp-2, and 0 in column nxp-1.  mu = tdt lq_{n+1} vanishes on
+// the zonal boundary rows, which ocqbdy writes; on the torus the Jacobian
+// transposes are J_1(h)^T mu = -J(mu, h) as in a box.
+//
+// The momentum constraints add (vorticity.F90:57-62,84-101,
+// constraint.F90:182-215, numerics.F90:374-386), with ls = tdt lcs_{n+1},
+// ln = tdt lcn_{n+1} and sums over the torus:
+//   ajis = (1/12) sum_i (q(i,0) + 2 q(i,1)) (p(i+1,1) - p(i-1,1))
+//   ajin = -(1/12) sum_i (q(i,N) + 2 q(i,N-1)) (p(i+1,N-1) - p(i-1,N-1))
+//   ap3s, ap5s, ap3n, ap5n and the drag integrals: functions of the row sums
+//   of pm only (the periodic x difference sums to zero along a row), whose
+//   gradients vs3, vs5, vsb, vn3, vn5, vnb the host computes from the y part
+//   of del2_P_bc
+//   txis, txin: rhss_1 += tau_sign f0/h_1 txis, rhsn_1 -= tau_sign f0/h_1 txin.
+
+__device__ __forceinline__ int torus_col(int gi, int L) { return ((gi % L) + L) % L; }
+
+// (L^T v)(a) of the channel's del2_P_bc (numerics.F90:427-468): the periodic
+// x difference on rows 1..nyp-2, symmetric on the torus, and the y part with
+// its mixed condition on every column (as for a box).
+__device__ __forceinline__ double lapT_cyc(const double* s, int W, int r, int c, int aj, const Geo& g) {
+    const bool yin = aj >= 1 && aj <= g.nyp - 2;
+    const bool ysin = aj - 1 >= 1 && aj - 1 <= g.nyp - 2, ynin = aj + 1 >= 1 && aj + 1 <= g.nyp - 2;
+    const double sy = yin ? -2.0 * g.fy : (aj == 0 ? -g.cyl : g.cyr);
+    const double ws = (aj - 1 == 0) ? g.cyl : (ysin ? g.fy : 0.0);
+    const double en = (aj + 1 == g.nyp - 1) ? -g.cyr : (ynin ? g.fy : 0.0);
+    const double* x = s + r * W + c;
+    const double tx = g.fx * (x[-1] + x[1]) - 2.0 * g.fx * x[0];
+    const double ty = sy * x[0] + ws * x[-W] + en * x[W];
+    return (yin ? tx : 0.0) + ty;
+}
+
+__global__ __launch_bounds__(256) void k_adj_vort_cyc(AdjVortArgs a) {
+    constexpr int TX = ADJ_TX, TY = ADJ_TY;
+    constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
+    constexpr int NMU = (H3 * W3 + 255) / 256, NQP = (H1 * W1 + 255) / 256, NOUT = TY / 4;
+    __shared__ double s_mu[H3 * W3];
+    __shared__ double s_n1[H2 * W2];
+    __shared__ double s_n2[H1 * W1];
+    __shared__ double s_q[H1 * W1];
+    __shared__ double s_p[H1 * W1];
+    const Geo& g = a.g;
+    const int L = g.nxp - 1, N = g.nyp - 1;
+    const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+    const int k = blockIdx.z, i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
+    const long long lo = (long long)k * g.fs;
+    const double* ln = a.lq_np1 + lo;
+    const double* qn = a.q + lo;
+    const double* pn = a.p + lo;
+    if (a.slots) {
+        const double* sb = a.slots[*a.base + a.t];
+        qn = sb + a.qoff + lo;
+        pn = sb + a.poff + lo;
+    }
+    double vmu[NMU], vq[NQP], vp[NQP];
+#pragma unroll
+    for (int u = 0; u < NMU; u++) {
+        const int idx = tid + 256 * u, r = idx / W3, c = idx - r * W3;
+        const int gi = torus_col(i0 - 3 + c, L), gj = j0 - 3 + r;
+        const bool ok = idx < H3 * W3 && gj >= 1 && gj <= N - 1;
+        const double v = ln[ok ? (long long)gj * g.pitch + gi : 0];
+        vmu[u] = ok ? a.tdt * v : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NQP; u++) {
+        const int idx = tid + 256 * u, r = idx / W1, c = idx - r * W1;
+        const int gi = torus_col(i0 - 1 + c, L), gj = j0 - 1 + r;
+        const bool ok = idx < H1 * W1 && gj >= 0 && gj <= N;
+        const long long o = ok ? (long long)gj * g.pitch + gi : 0;
+        const double q = qn[o], p = pn[o];
+        vq[u] = ok ? q : 0.0;
+        vp[u] = ok ? p : 0.0;
+    }
+    const int gi = i0 + tx;
+    double olq[NOUT], olp[NOUT], olw[NOUT];
+#pragma unroll
+    for (int u = 0; u < NOUT; u++) {
+        const int gj = j0 + ty + 4 * u;
+        const bool ok = gi < L && gj <= N;
+        const long long o = ok ? (long long)gj * g.pitch + gi : 0;
+        olq[u] = a.lq_n[lo + o];
+        olp[u] = a.lp_n[lo + o];
+        olw[u] = a.lwek[o];
+    }
+    const int kb = g.nl - 1;
+    const double ls = a.tdt * a.lc[k], lnn = a.tdt * a.lc[MAXL + k];
+    const double lsb = a.tdt * a.lc[kb], lnb = a.tdt * a.lc[MAXL + kb];
+#pragma unroll
+    for (int u = 0; u < NMU; u++) {
+        const int idx = tid + 256 * u;
+        if (idx < H3 * W3) s_mu[idx] = vmu[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NQP; u++) {
+        const int idx = tid + 256 * u;
+        if (idx < H1 * W1) {
+            s_q[idx] = vq[u];
+            s_p[idx] = vp[u];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < H2 * W2; idx += 256) {
+        const int r = idx / W2, c = idx - r * W2;
+        const int gj2 = j0 - 2 + r;
+        s_n1[idx] = (gj2 >= 0 && gj2 <= N) ? lapT_cyc(s_mu, W3, r + 1, c + 1, gj2, g) : 0.0;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < H1 * W1; idx += 256) {
+        const int r = idx / W1, c = idx - r * W1;
+        const int gj1 = j0 - 1 + r;
+        s_n2[idx] = (gj1 >= 0 && gj1 <= N) ? lapT_cyc(s_n1, W2, r + 1, c + 1, gj1, g) : 0.0;
+    }
+    __syncthreads();
+    const Phys& ph = *a.ph;
+    const double c2 = ph.c2[k], c4 = ph.c4[k], rxy = g.dx / g.dy;
+    if (tid == 0 && blockIdx.x == 0 && blockIdx.y == 0 && k == 0) {
+        a.ltxi[0] += ph.tau_sign * ph.fohfac[0] * ls;
+        a.ltxi[1] -= ph.tau_sign * ph.fohfac[0] * lnn;
+    }
+#pragma unroll
+    for (int u = 0; u < NOUT; u++) {
+        const int r = ty + 4 * u, gj = j0 + r;
+        if (gi >= L || gj > N) continue;
+        const long long o = lo + (long long)gj * g.pitch + gi;
+        const double nu3 = lapT_cyc(s_n2, W1, r + 1, tx + 1, gj, g);
+        const double jq = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_p, W1, r + 1, tx + 1);
+        const double jp = -arakawa_lds(s_mu, W3, r + 3, tx + 3, s_q, W1, r + 1, tx + 1);
+        const double nu2 = s_n2[(r + 1) * W1 + tx + 1];
+        const double nu1 = s_n1[(r + 2) * W2 + tx + 2];
+        double lpm = c2 * nu2 - c4 * nu3;
+        if (k == g.kt) lpm -= ph.cdrag * nu1;
+        double dq = g.jfac * jq, dp = -g.jfac * jp;
+        // the momentum constraints' boundary integrals
+        const double* P = s_p + (r + 1) * W1 + tx + 1;   // the point in the q, p tiles
+        const double* Q = s_q + (r + 1) * W1 + tx + 1;
+        if (gj == 0) dq += (ls / 12.0) * (P[W1 + 1] - P[W1 - 1]);
+        if (gj == 1) {
+            dq += (2.0 * ls / 12.0) * (P[1] - P[-1]);
+            dp += (ls / 12.0) * ((Q[-W1 - 1] + 2.0 * Q[-1]) - (Q[-W1 + 1] + 2.0 * Q[1]));
+        }
+        if (gj == N) dq -= (lnn / 12.0) * (P[-W1 + 1] - P[-W1 - 1]);
+        if (gj == N - 1) {
+            dq -= (2.0 * lnn / 12.0) * (P[1] - P[-1]);
+            dp -= (lnn / 12.0) * ((Q[W1 - 1] + 2.0 * Q[-1]) - (Q[W1 + 1] + 2.0 * Q[1]));
+        }
+        if (gj <= 3) {
+            lpm += rxy * ls * (ph.ah4[k] * a.vs5[gj] - ph.ah2[k] * a.vs3[gj]);
+            if (k == g.kt) lpm += ph.fohfac[kb] * ph.cd_bdr * rxy * lsb * a.vsb[gj];
+        }
+        if (N - gj <= 3) {
+            const int d = N - gj;
+            lpm += rxy * lnn * (ph.ah2[k] * a.vn3[d] - ph.ah4[k] * a.vn5[d]);
+            if (k == g.kt) lpm -= ph.fohfac[kb] * ph.cd_bdr * rxy * lnb * a.vnb[d];
+        }
+        a.lq_n[o] = olq[u] + dq;
+        a.lp_n[o] = olp[u] + dp;
+        a.lp_nm1[o] = lpm;
+        const double mu = s_mu[(r + 3) * W3 + tx + 3];
+        if (k == 0) a.lwek[(long long)gj * g.pitch + gi] = olw[u] + ph.cwek * mu;
+        if (gj == 0 || gj == N) a.lq_np1[o] = 0.0;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 
 static int blocks_for(long long n) {
@@ -319,9 +530,9 @@ void launch_adj_add(double* dst, const double* src, long long n, hipStream_t st)
     hipLaunchKernelGGL(k_adj_add, dim3(blocks_for(n)), dim3(256), 0, st, dst, src, n);
 }
 
-// Zero the boundary ring of nl fields.
+// Zero the boundary ring of nl fields (a channel: rows 0 and nyp-1).
 __global__ void k_zero_ring(double* f, Geo g) {
-    const int nx = g.nxp, ny = g.nyp, per = 2 * nx + 2 * (ny - 2);
+    const int nx = g.nxp, ny = g.nyp, per = g.cyclic ? 2 * nx : 2 * nx + 2 * (ny - 2);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= per * g.nl) return;
     const int k = t / per, r = t - k * per;
@@ -334,8 +545,24 @@ __global__ void k_zero_ring(double* f, Geo g) {
 }
 
 void launch_zero_ring(double* f, const Geo& g, hipStream_t st) {
-    const int n = (2 * g.nxp + 2 * (g.nyp - 2)) * g.nl;
+    const int n = (g.cyclic ? 2 * g.nxp : 2 * g.nxp + 2 * (g.nyp - 2)) * g.nl;
     hipLaunchKernelGGL(k_zero_ring, dim3((n + 255) / 256), dim3(256), 0, st, f, g);
+}
+
+// Channel: an adjoint given on all nxp columns as the sensitivity to each
+// stored value becomes that of the periodic field: column nxp-1 (a copy of
+// column 0) is added into column 0 and cleared.
+__global__ void k_fold_dup(double* f, Geo g, int nl) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nl * g.nyp) return;
+    double* row = f + (long long)(t / g.nyp) * g.fs + (long long)(t % g.nyp) * g.pitch;
+    row[0] += row[g.nxp - 1];
+    row[g.nxp - 1] = 0.0;
+}
+
+void launch_fold_dup(double* f, const Geo& g, int nl, hipStream_t st) {
+    const int n = nl * g.nyp;
+    hipLaunchKernelGGL(k_fold_dup, dim3((n + 255) / 256), dim3(256), 0, st, f, g, nl);
 }
 
 __global__ void k_set_int(int* p, int v) {
@@ -356,6 +583,11 @@ void launch_copy_segs(const CopySegs& a0, hipStream_t st) {
 }
 
 void launch_adj_vort(const AdjVortArgs& a, hipStream_t st) {
+    if (a.g.cyclic) {
+        const dim3 grid((a.g.nxp - 1 + ADJ_TX - 1) / ADJ_TX, (a.g.nyp + ADJ_TY - 1) / ADJ_TY, a.g.nl);
+        hipLaunchKernelGGL(k_adj_vort_cyc, grid, dim3(256), 0, st, a);
+        return;
+    }
     const dim3 grid((a.g.nxp + ADJ_TX - 1) / ADJ_TX, (a.g.nyp + ADJ_TY - 1) / ADJ_TY, a.g.nl);
     hipLaunchKernelGGL(k_adj_vort, grid, dim3(256), 0, st, a);
 }

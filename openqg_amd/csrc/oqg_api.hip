@@ -148,9 +148,16 @@ struct oqg_ctx {
     double* lp[2] = {nullptr, nullptr};
     double* lwek = nullptr;
     double* adjW = nullptr;       // one field: the interior indicator
-    double* adjG = nullptr;       // nl fields: H_m^-1 of the interior indicator
+    double* adjG = nullptr;       // box: nl fields, H_m^-1 of the interior indicator; channel: 2 nl (unit rows)
     double* adjPart = nullptr;
     double* adjCinvT = nullptr;
+    // channel: adjoint constraint blocks (lcs, lcn) at both parities, a
+    // scratch block, the adjoints of txis and txin, and the gradients of the
+    // boundary integrals with respect to the row sums of pm (AdjVortArgs)
+    double* lc[2] = {nullptr, nullptr};
+    double* lc_scratch = nullptr;
+    double* ltxi = nullptr;
+    double adj_rows[6][4] = {};
     // state snapshots (oqg_snap_*): q[0..1], p[0..1], c[0..1] and the parity per slot
     std::vector<double*> snaps;
     std::vector<int> snap_cur;
@@ -1452,6 +1459,7 @@ int oqg_adj_wind(oqg_ctx* c, double* ltaux, double* ltauy, int kind) {
     a.taux = c->taux_d;
     a.tauy = c->tauy_d;
     a.lwek = c->lwek;
+    a.ltxi = c->ltxi;
     launch_ekman_adj(a, c->st);
     HIPCHK(c, hipGetLastError());
     if (int r = download_field(c, ltaux, c->taux_d, 1, kind)) return r;
@@ -1785,7 +1793,7 @@ static int enqueue_invert(oqg_ctx* c) {
 // in lq's ring is passed on to lp first (inside k_fwd_adj; lp itself is left
 // as it is).  Three launches: the transform kernels in their adjoint modes
 // around the forward y solve.
-static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp) {
+static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp, double* lc) {
     FwdArgs fa{};
     fa.g = c->g;
     fa.s = c->s;
@@ -1815,6 +1823,7 @@ static int enqueue_adj_inversion(oqg_ctx* c, double* lq, const double* lp) {
     ia.adj_partial = c->adjPart;
     ia.adj_npart = (c->s.nyi + 1) / 2;
     ia.adj_lq = lq;
+    ia.adj_lc = lc;
     launch_inv_adj(ia, c->st);
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -1841,9 +1850,13 @@ static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed, const SnapR
         s.lqm = c->lq[cur];
         s.lp = c->lp[cn];
         s.lpm = c->lp[cur];
+        if (c->g.cyclic) {
+            s.lc = c->lc[cn];
+            s.lcm = c->lc[cur];
+        }
         launch_adj_supp(s, c->st);
     }
-    if (int r = enqueue_adj_inversion(c, c->lq[cn], c->lp[cn])) return r;
+    if (int r = enqueue_adj_inversion(c, c->lq[cn], c->lp[cn], c->lc[cn])) return r;
     AdjVortArgs v{};
     v.g = c->g;
     v.ph = &c->dconst->ph;
@@ -1860,8 +1873,75 @@ static int enqueue_adj_step(oqg_ctx* c, double tdt, bool suppressed, const SnapR
     v.lp_nm1 = c->lp[cn];
     v.lwek = c->lwek;
     v.tdt = tdt;
+    if (c->g.cyclic) {
+        v.lc = c->lc[cn];
+        v.ltxi = c->ltxi;
+        for (int d = 0; d < 4; d++) {
+            v.vs3[d] = c->adj_rows[0][d];
+            v.vs5[d] = c->adj_rows[1][d];
+            v.vsb[d] = c->adj_rows[2][d];
+            v.vn3[d] = c->adj_rows[3][d];
+            v.vn5[d] = c->adj_rows[4][d];
+            v.vnb[d] = c->adj_rows[5][d];
+        }
+    }
     launch_adj_vort(v, c->st);
     HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+// Channel precompute of the adjoint: Gs_m, Gn_m = H_m^-1 of the unit rows 1
+// and nyp-2 (row-constant fields at adjG[side * nl + m]), and the gradients
+// of the boundary integrals with respect to the row sums P_j of pm
+// (vorticity.F90:57-62,95-101): with D the y part of del2_P_bc acting on a
+// column (numerics.F90:455-468) and es = e_1 - e_0, en = e_N - e_{N-1},
+//   ap3s = ah2 (dx/dy) es.D P, ap5s = ah4 (dx/dy) es.D D P, bdrins = cd (dx/dy) es.P
+// (north likewise with en), so the gradients are D^T es, D^T D^T es and es
+// (and with en), of which rows 0..3 (south) and N..N-3 (north) are nonzero.
+static int adj_init_channel(oqg_ctx* c) {
+    const Geo& g = c->g;
+    const int ny = g.nyp, N = ny - 1;
+    const size_t f1 = (size_t)g.fs;
+    std::vector<double> unit(f1, 0.0);
+    for (int side = 0; side < 2; side++) {
+        std::fill(unit.begin(), unit.end(), 0.0);
+        const int row = side == 0 ? 1 : N - 1;
+        for (int i = 0; i < g.nxp; i++) unit[(size_t)row * g.pitch + i] = 1.0;
+        HIPCHK(c, hipMemcpyAsync(c->adjW, unit.data(), f1 * sizeof(double), hipMemcpyHostToDevice, c->st));
+        for (int m = 0; m < g.nl; m++)
+            if (int r = enqueue_raw_solve(c, m, c->adjW, c->adjG + (size_t)(side * g.nl + m) * f1)) return r;
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    // (D^T v)(a) = sum_j D[j][a] v[j]
+    auto DT = [&](const std::vector<double>& v) {
+        std::vector<double> r(ny, 0.0);
+        auto D = [&](int j, int a) -> double {
+            if (j == 0) return a == 0 ? -g.cyl : (a == 1 ? g.cyl : 0.0);
+            if (j == N) return a == N ? g.cyr : (a == N - 1 ? -g.cyr : 0.0);
+            return a == j ? -2.0 * g.fy : ((a == j - 1 || a == j + 1) ? g.fy : 0.0);
+        };
+        for (int a = 0; a < ny; a++) {
+            double t = 0.0;
+            for (int j = std::max(0, a - 1); j <= std::min(N, a + 1); j++) t += D(j, a) * v[j];
+            r[a] = t;
+        }
+        return r;
+    };
+    std::vector<double> es(ny, 0.0), en(ny, 0.0);
+    es[0] = -1.0;
+    es[1] = 1.0;
+    en[N] = 1.0;
+    en[N - 1] = -1.0;
+    const std::vector<double> s3 = DT(es), s5 = DT(s3), n3 = DT(en), n5 = DT(n3);
+    for (int d = 0; d < 4; d++) {
+        c->adj_rows[0][d] = s3[d];
+        c->adj_rows[1][d] = s5[d];
+        c->adj_rows[2][d] = es[d];
+        c->adj_rows[3][d] = n3[N - d];
+        c->adj_rows[4][d] = n5[N - d];
+        c->adj_rows[5][d] = en[N - d];
+    }
+    c->adj = true;
     return 0;
 }
 
@@ -1870,11 +1950,13 @@ extern "C" {
 int oqg_adj_init(oqg_ctx* c) {
     ctx_join(c);
     if (!c) return OQG_ERR_ARG;
-    if (c->g.cyclic) return fail(c, OQG_ERR_ARG, "the adjoint is implemented for box basins");
     if (c->s.prime)
         return fail(c, OQG_ERR_ARG, "the adjoint takes x transform lengths that factor into 2, 3 and 5");
+    if (c->g.cyclic && (!fftw_fits(c->s, c->g.nl) || c->g.nyp < 9))
+        return fail(c, OQG_ERR_ARG, "the channel adjoint takes nyp >= 9 and transforms that fit the wavefront kernels");
     if (c->adj) return 0;
     const size_t fl = (size_t)c->g.nl * c->g.fs, f1 = (size_t)c->g.fs;
+    const size_t npairs = (size_t)((c->s.nyi + 1) / 2);
     int rc = 0;
     for (int t = 0; t < 2 && !rc; t++) {
         rc = dalloc(c, &c->lq[t], fl);
@@ -1882,10 +1964,18 @@ int oqg_adj_init(oqg_ctx* c) {
     }
     if (!rc) rc = dalloc(c, &c->lwek, f1);
     if (!rc) rc = dalloc(c, &c->adjW, f1);
-    if (!rc) rc = dalloc(c, &c->adjG, fl);
-    if (!rc) rc = dalloc(c, &c->adjPart, (size_t)((c->s.nyi + 1) / 2) * MAXL);   // [row pair][mode]
+    if (!rc) rc = dalloc(c, &c->adjG, c->g.cyclic ? 2 * fl : fl);
+    // [row pair][mode] (box), [row pair][profile] (channel)
+    if (!rc) rc = dalloc(c, &c->adjPart, npairs * (c->g.cyclic ? ADJ_CYC_NPROF : MAXL));
     if (!rc) rc = dalloc(c, &c->adjCinvT, MAXL * MAXL);
+    double* cblk = nullptr;
+    if (!rc) rc = dalloc(c, &cblk, 3 * LC_LEN + 8);
     if (rc) return rc;
+    c->lc[0] = cblk;
+    c->lc[1] = cblk + LC_LEN;
+    c->lc_scratch = cblk + 2 * LC_LEN;
+    c->ltxi = cblk + 3 * LC_LEN;
+    if (c->g.cyclic) return adj_init_channel(c);
     // (cdhoc^-1)^T by Gauss-Jordan in long double
     const int n = c->g.nl - 1;
     long double A[MAXL][2 * MAXL] = {};
@@ -1933,10 +2023,61 @@ int oqg_adj_set(oqg_ctx* c, const double* lq, const double* lqm, const double* l
     for (int t = 0; t < 4; t++) {
         if (src[t]) {
             if (int r = upload_field(c, dst[t], src[t], c->g.nl, kind)) return r;
+            if (c->g.cyclic) launch_fold_dup(dst[t], c->g, c->g.nl, c->st);
         } else {
             HIPCHK(c, hipMemsetAsync(dst[t], 0, c->fbytes, c->st));
         }
     }
+    HIPCHK(c, hipMemsetAsync(c->lc[0], 0, 2 * LC_LEN * sizeof(double), c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+// The adjoint constraint scalars in oqg_get_constr's packing; the dpi and dpip
+// entries are not state of the adjoint (the step's p does not depend on them
+// in a channel, and a box's dpi depends on the entrainment alone) and read 0.
+int oqg_adj_set_constr(oqg_ctx* c, const double* v) {
+    ctx_join(c);
+    if (!c || !v) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    const int nl = c->g.nl, o = 2 * (nl - 1);
+    double h[2][LC_LEN] = {};
+    for (int k = 0; k < nl; k++) {
+        h[0][k] = v[o + k];
+        h[0][MAXL + k] = v[o + nl + k];
+        h[1][k] = v[o + 2 * nl + k];
+        h[1][MAXL + k] = v[o + 3 * nl + k];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->lc[c->cur], h[0], sizeof h[0], hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->lc[1 - c->cur], h[1], sizeof h[1], hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int oqg_adj_get_constr(oqg_ctx* c, double* v) {
+    ctx_join(c);
+    if (!c || !v) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    const int nl = c->g.nl, o = 2 * (nl - 1);
+    double h[2][LC_LEN];
+    HIPCHK(c, hipMemcpyAsync(h[0], c->lc[c->cur], sizeof h[0], hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(h[1], c->lc[1 - c->cur], sizeof h[1], hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int t = 0; t < o; t++) v[t] = 0.0;
+    for (int k = 0; k < nl; k++) {
+        v[o + k] = h[0][k];
+        v[o + nl + k] = h[0][MAXL + k];
+        v[o + 2 * nl + k] = h[1][k];
+        v[o + 3 * nl + k] = h[1][MAXL + k];
+    }
+    return 0;
+}
+
+int oqg_adj_get_txi(oqg_ctx* c, double* out) {
+    ctx_join(c);
+    if (!c || !out) return OQG_ERR_ARG;
+    if (int r = adj_require(c)) return r;
+    HIPCHK(c, hipMemcpyAsync(out, c->ltxi, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
@@ -1975,6 +2116,7 @@ int oqg_adj_zero_wekp(oqg_ctx* c) {
     if (!c) return OQG_ERR_ARG;
     if (int r = adj_require(c)) return r;
     HIPCHK(c, hipMemsetAsync(c->lwek, 0, (size_t)c->g.fs * sizeof(double), c->st));
+    HIPCHK(c, hipMemsetAsync(c->ltxi, 0, 2 * sizeof(double), c->st));
     return 0;
 }
 
@@ -1988,6 +2130,7 @@ int oqg_adj_seed_ke(oqg_ctx* c, double rho, double* J) {
     const int cur = c->cur;
     for (double* f : {c->lq[cur], c->lq[1 - cur], c->lp[cur], c->lp[1 - cur]})
         HIPCHK(c, hipMemsetAsync(f, 0, c->fbytes, c->st));
+    HIPCHK(c, hipMemsetAsync(c->lc[0], 0, 2 * LC_LEN * sizeof(double), c->st));
     KeSeedArgs a{};
     a.g = c->g;
     a.bcco = c->d.bcco;
@@ -2013,7 +2156,9 @@ int oqg_adj_invert(oqg_ctx* c) {
     const long long n = (long long)c->g.nl * c->g.fs;
     launch_adj_add(c->lq[cur], c->lq[1 - cur], n, c->st);
     launch_adj_add(c->lp[cur], c->lp[1 - cur], n, c->st);
-    if (int r = enqueue_adj_inversion(c, c->lq[cur], c->lp[cur])) return r;
+    // the constraint state is an input of its own, not a function of q: its
+    // adjoint goes to scratch
+    if (int r = enqueue_adj_inversion(c, c->lq[cur], c->lp[cur], c->lc_scratch)) return r;
     // only the interior of q is an input of the inversion
     launch_zero_ring(c->lq[cur], c->g, c->st);
     HIPCHK(c, hipMemsetAsync(c->lq[1 - cur], 0, c->fbytes, c->st));

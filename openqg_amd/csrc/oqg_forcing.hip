@@ -6,8 +6,8 @@
 // the channel's boundary stress integrals txis/txin (int_P_dx of the two
 // outer row pairs).  One thread per p point recomputes the (at most four)
 // T-point values it averages, in the reference's order of operations.  The
-// transpose (for gradients with respect to the stress, box basins) is a
-// gather in the same style.
+// transposes (for gradients with respect to the stress) are gathers in the
+// same style.
 
 #include "oqg_device.h"
 
@@ -141,6 +141,55 @@ __global__ void k_ekman_adj(EkmanArgs a) {
     a.tauy[(long long)j * g.pitch + i] = ly;
 }
 
+// Channel transpose, on the torus of the nxp-1 distinct columns (column nxp-1
+// of the stress repeats column 0; the adjoint holds 0 there): p column i takes
+// T columns i-1 and i (mod nxp-1), T column I its x faces from taux columns I
+// and I+1 and its y faces from tauy columns I and I+1 (mod nxp-1); txis and
+// txin take 0.5 dx of taux on their two rows (ltxi[0], ltxi[1]).
+__global__ void k_ekman_adj_cyc(EkmanArgs a) {
+    const Geo& g = a.g;
+    const int L = g.nxp - 1, ny = g.nyp, mj = ny - 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.nxp * ny) return;
+    const int j = t / g.nxp, i = t - j * g.nxp;
+    const long long o = (long long)j * g.pitch + i;
+    if (i == L) {
+        a.taux[o] = 0.0;
+        a.tauy[o] = 0.0;
+        return;
+    }
+    auto wrap = [&](int x) { return x < 0 ? x + L : (x >= L ? x - L : x); };
+    auto lwek_t = [&](int I, int J) {
+        if (J < 0 || J > mj) return 0.0;
+        double s = 0.0;
+#pragma unroll
+        for (int dj = 0; dj <= 1; dj++) {
+            const int jj = J + dj;
+            const double w = (jj >= 1 && jj <= ny - 2) ? 0.25 : ((jj == 0 || jj == ny - 1) ? 0.5 : 0.0);
+#pragma unroll
+            for (int di = 0; di <= 1; di++) s += w * a.lwek[(long long)jj * g.pitch + wrap(I + di)];
+        }
+        return s;
+    };
+    const double cx = 1.0 / (g.dx * g.fnot), cy = 1.0 / (g.dy * g.fnot);
+    auto ltaux_face = [&](int I, int jf) {
+        double v = 0.0;
+        if (jf >= 1) v -= cy * lwek_t(I, jf - 1);
+        if (jf <= ny - 2) v += cy * lwek_t(I, jf);
+        return v;
+    };
+    auto ltauy_face = [&](int ifc, int J) {
+        if (J < 0 || J > mj) return 0.0;
+        return cx * lwek_t(wrap(ifc - 1), J) - cx * lwek_t(ifc, J);
+    };
+    double lx = 0.5 * ltaux_face(wrap(i - 1), j) + 0.5 * ltaux_face(i, j);
+    const double ly = 0.5 * ltauy_face(i, j - 1) + 0.5 * ltauy_face(i, j);
+    if (j <= 1) lx += 0.5 * g.dx * a.ltxi[0];
+    if (j >= ny - 2) lx += 0.5 * g.dx * a.ltxi[1];
+    a.taux[o] = lx;
+    a.tauy[o] = ly;
+}
+
 void launch_ekman(const EkmanArgs& a, hipStream_t st) {
     const int n = a.g.nxp * a.g.nyp;
     hipLaunchKernelGGL(k_ekman, dim3((n + 255) / 256), dim3(256), 0, st, a);
@@ -149,7 +198,8 @@ void launch_ekman(const EkmanArgs& a, hipStream_t st) {
 
 void launch_ekman_adj(const EkmanArgs& a, hipStream_t st) {
     const int n = a.g.nxp * a.g.nyp;
-    hipLaunchKernelGGL(k_ekman_adj, dim3((n + 255) / 256), dim3(256), 0, st, a);
+    if (a.g.cyclic) hipLaunchKernelGGL(k_ekman_adj_cyc, dim3((n + 255) / 256), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_ekman_adj, dim3((n + 255) / 256), dim3(256), 0, st, a);
 }
 
 }  // namespace oqg

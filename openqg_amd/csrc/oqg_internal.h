@@ -235,8 +235,18 @@ struct InvArgs {
     const double* adj_partial;
     int adj_npart;
     double* adj_lq;
+    // channel: adj_G holds H_m^-1 of the unit rows 1 and nyp-2 at fields
+    // [side * nl + m]; the adjoint of the new constraint state (cs at
+    // [0, MAXL), cn at [MAXL, 2 MAXL)) is accumulated into adj_lc by block 0
+    double* adj_lc;
     double tdt;
 };
+
+// Channel adjoints: stride of the per-row-pair partial sums of the inversion's
+// homogeneous profiles (bt, bc1 of modes 1..nl-1, bc2 of modes 1..nl-1).
+constexpr int ADJ_CYC_NPROF = 2 * MAXL;
+// the adjoint constraint block: lcs at [0, MAXL), lcn at [MAXL, 2 MAXL)
+constexpr int LC_LEN = 2 * MAXL;
 
 // Resident forcing scalars of a channel, read at run time so that a replayed
 // graph sees the latest forcing: [F_TXIS] txis, [F_TXIN] txin, [F_ENT2] 1 when
@@ -261,10 +271,11 @@ struct QcompArgs {
     double* q;
 };
 
-// Adjoint kernels (oqg_adjoint.hip), box basins.
+// Adjoint kernels (oqg_adjoint.hip).
 struct AdjSuppArgs {
     Geo g;
     double *lq, *lqm, *lp, *lpm;
+    double *lc, *lcm;   // channel: adjoint constraint blocks of cs/cn and csp/cnp (LC_LEN), else null
 };
 
 struct KeSeedArgs {
@@ -293,6 +304,14 @@ struct AdjVortArgs {
     const int* base;
     int t;
     long long qoff, poff;
+    // channel: the adjoint of the new constraint state (LC_LEN, read), the
+    // adjoints of txis and txin (+=, block 0), and the gradients of the
+    // boundary integrals ap3, ap5 and the drag integral with respect to the
+    // row sums of pm next to each zonal boundary ([d]: row d (south) or
+    // nyp-1-d (north)), before the factors ah2, ah4, cd_bdr and dx/dy
+    const double* lc;
+    double* ltxi;
+    double vs3[4], vs5[4], vsb[4], vn3[4], vn5[4], vnb[4];
 };
 
 void launch_adj_supp(const AdjSuppArgs& a, hipStream_t st);
@@ -305,6 +324,7 @@ struct EkmanArgs {
     double* wekp;         // out (k_ekman)
     double* fcon;         // out, channel: txis, txin (k_ekman_txi)
     const double* lwek;   // in (k_ekman_adj): adjoint of wekp
+    const double* ltxi;   // in (k_ekman_adj, channel): adjoints of txis, txin
 };
 void launch_ekman(const EkmanArgs& a, hipStream_t st);
 void launch_ekman_adj(const EkmanArgs& a, hipStream_t st);
@@ -445,6 +465,7 @@ struct CopySegs {
 void launch_copy_segs(const CopySegs& a, hipStream_t st);
 void launch_set_int(int* p, int v, hipStream_t st);
 void launch_zero_ring(double* f, const Geo& g, hipStream_t st);
+void launch_fold_dup(double* f, const Geo& g, int nl, hipStream_t st);
 
 void launch_adj_add(double* dst, const double* src, long long n, hipStream_t st);
 void launch_adj_vort(const AdjVortArgs& a, hipStream_t st);

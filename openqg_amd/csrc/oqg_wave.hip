@@ -911,6 +911,296 @@ __global__ __launch_bounds__(64 * (NL + 1)) void k_invw_adj(InvArgs a) {
     invw_adj_block<NL, P>(a, blockIdx.x);
 }
 
+// ---- the channel's inversion adjoint, on the torus of the nxp-1 distinct
+// columns (column nxp-1 of every adjoint field stays 0).  Forward: p_k =
+// sum_m ctm2l(m,k) (u_m + hc_m(j)), u_m = H_m^-1 r_m the periodic Helmholtz
+// solve of r_m = f0 sum_k ctl2m(k,m) (q_k - beta y - ddyn) on rows 1..nyp-2,
+// hc_m(j) = c_bt bt(j) (m = 0) or c_bc1(m) bc1_m(j) + c_bc2(m) bc2_m(j), the
+// coefficients from clhss_m = sum_k ctl2m(k,m) cs_k + (dx/dy) sum_i u_m(i,1)
+// and clhsn_m likewise (homog.F90:280-329), and ocqbdy on rows 0 and nyp-1.
+// Adjoint: w_m = sum_k ctm2l(m,k) lp_k (lp completed by ocqbdy's transpose);
+// the profile inner products sum_j prof(j) sum_i w_m(i,j) give the adjoints of
+// the coefficients and through them lclhss_m, lclhsn_m; lcs_k += sum_m
+// ctl2m(k,m) lclhss_m (lcn_k likewise); the adjoint of u_m is w_m plus
+// (dx/dy) lclhss_m on row 1 and (dx/dy) lclhsn_m on row nyp-2, and H_m is
+// symmetric, so lr_m = H_m^-1 w_m + (dx/dy)(lclhss_m Gs_m + lclhsn_m Gn_m)
+// with Gs_m, Gn_m = H_m^-1 of the unit rows; lq_k += f0 sum_m ctl2m(k,m) lr_m.
+
+// k_fwdw_adj, channel: the inputs w_m of both rows of the pair through the
+// forward real-to-complex transform, and the pair's profile inner products
+// (the pairs also share out the points of rows 0 and nyp-1) at
+// adj_partial[row pair * ADJ_CYC_NPROF + profile].
+template <int NL, int P>
+__device__ __forceinline__ void fwdw_adj_cyc_block(const FwdArgs& a, int jp) {
+    extern __shared__ double2 wsm[];
+    constexpr int NP = 2 * NL - 1;
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L, ZL = zpad_len(L), N = g.nyp - 1;
+    const int nth = (int)blockDim.x;   // 64 NL
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    double2* tw = wsm + (size_t)NL * ZL;
+    __shared__ double s_red[NP * 16];
+    const int r0 = 2 * jp, r1 = r0 + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    Staged<12, double> stw;
+    stage_load(stw, a.tw, 2 * twiddle_table_len(L));
+    const double qb = a.adj_qbfac;
+    const double* lq = a.adj_lq;
+    const Mix& mx = a.dc->mx;
+    double cm[NL][NL];   // ctm2l(m, k)
+#pragma unroll
+    for (int m = 0; m < NL; m++)
+#pragma unroll
+        for (int k = 0; k < NL; k++) cm[m][k] = mx.ctm2l[m * MAXL + k];
+    double h0[NP], h1[NP];
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+        h0[q] = a.adj_hom[(long long)q * g.nyp + gj0];
+        h1[q] = has1 ? a.adj_hom[(long long)q * g.nyp + gj1] : 0.0;
+    }
+    double part[NP];
+#pragma unroll
+    for (int q = 0; q < NP; q++) part[q] = 0.0;
+    auto prof_add = [&](const double (&w)[NL], const double (&h)[NP]) {
+        part[0] += h[0] * w[0];
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            part[m] += h[m] * w[m];
+            part[NL - 1 + m] += h[NL - 1 + m] * w[m];
+        }
+    };
+    for (int t = (int)threadIdx.x; t < L; t += nth) {
+        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
+        double la[NL], lb[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            la[k] = a.adj_lp[k * g.fs + o0];
+            lb[k] = has1 ? a.adj_lp[k * g.fs + o1] : 0.0;
+        }
+        // ocqbdy's fac (p(inner row) - p(boundary row)), transposed at the inner rows
+        if (gj0 == 1)
+#pragma unroll
+            for (int k = 0; k < NL; k++) la[k] += qb * lq[k * g.fs + t];
+        if (gj0 == N - 1)
+#pragma unroll
+            for (int k = 0; k < NL; k++) la[k] += qb * lq[k * g.fs + (long long)N * g.pitch + t];
+        if (has1 && gj1 == N - 1)
+#pragma unroll
+            for (int k = 0; k < NL; k++) lb[k] += qb * lq[k * g.fs + (long long)N * g.pitch + t];
+        double wa[NL], wb[NL];
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double sa = 0.0, sb = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                sa += cm[m][k] * la[k];
+                sb += cm[m][k] * lb[k];
+            }
+            wa[m] = sa;
+            wb[m] = has1 ? sb : 0.0;
+            wsm[(size_t)m * ZL + zpad(t)] = make_double2(wa[m], wb[m]);
+        }
+        prof_add(wa, h0);
+        if (has1) prof_add(wb, h1);
+    }
+    // rows 0 and nyp-1: 2 (nxp-1) points dealt out over the row pairs; p there
+    // is the homogeneous correction alone
+    const int npairs = (s.nyi + 1) / 2, nring = 2 * L;
+    const int rper = (nring + npairs - 1) / npairs;
+    const int rlo = min(nring, jp * rper), rhi = min(nring, rlo + rper);
+    for (int e = rlo + (int)threadIdx.x; e < rhi; e += nth) {
+        const int gj = e < L ? 0 : N, i = e < L ? e : e - L;
+        const long long o = (long long)gj * g.pitch + i;
+        double lp[NL], lqb[NL], hb[NP], w[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            lp[k] = a.adj_lp[k * g.fs + o];
+            lqb[k] = lq[k * g.fs + o];
+        }
+#pragma unroll
+        for (int q = 0; q < NP; q++) hb[q] = a.adj_hom[(long long)q * g.nyp + gj];
+        // ocqbdy at the point: q_k = fac (p_k(inner) - p_k) - f0 sum_l amat(k,l) p_l + ...
+#pragma unroll
+        for (int k = 0; k < NL; k++)
+#pragma unroll
+            for (int l = 0; l < NL; l++) {
+                double c = -g.fnot * mx.amat[k * MAXL + l];
+                if (l == k) c -= qb;
+                lp[l] += lqb[k] * c;
+            }
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double t = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) t += cm[m][k] * lp[k];
+            w[m] = t;
+        }
+        prof_add(w, hb);
+    }
+    stage_store(stw, (double*)tw, a.tw, 2 * twiddle_table_len(L));
+    block_sum<NP>(part, s_red);   // its barriers also publish the transforms' inputs
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < NP; q++) a.adj_partial[(long long)jp * ADJ_CYC_NPROF + q] = part[q];
+    double2* z = wsm + (size_t)wv * ZL;
+    fft_wave_any<P, -1>(z, L, s.nstages, s.radix, tw, lane);
+    double* row0 = a.spec + ((long long)wv * s.nrow + r0) * s.kp;
+    double* row1 = row0 + s.kp;
+    for (int kk = lane; kk <= L / 2; kk += 64) {
+        const double2 A = z[zpad(kk)], B = z[zpad((L - kk) % L)];
+        const double xr = 0.5 * (A.x + B.x), xi = 0.5 * (A.y - B.y);
+        const double yr = 0.5 * (A.y + B.y), yi = 0.5 * (B.x - A.x);
+        *(double2*)(row0 + 2 * kk) = make_double2(xr, xi);
+        if (has1) *(double2*)(row1 + 2 * kk) = make_double2(yr, yi);
+    }
+}
+
+template <int NL, int P>
+__global__ __launch_bounds__(64 * NL) void k_fwdw_adj_cyc(FwdArgs a) {
+    fwdw_adj_cyc_block<NL, P>(a, blockIdx.x);
+}
+
+// k_invw_adj, channel: the complex-to-real transforms of the solved spectra
+// while the helper wavefront turns the profile sums into lclhss, lclhsn (and
+// block 0 adds the constraint adjoints), then lq on the pair's rows.
+template <int NL, int P>
+__device__ __forceinline__ void invw_adj_cyc_block(const InvArgs& a, int jp) {
+    extern __shared__ double2 wsm[];
+    constexpr int NP = 2 * NL - 1;
+    const Geo& g = a.g;
+    const Spec& s = a.s;
+    const int L = s.L, ZL = zpad_len(L);
+    const int nth = (int)blockDim.x;   // 64 (NL + 1)
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const bool helper = wv == NL;
+    double2* tw = wsm + (size_t)NL * ZL;
+    __shared__ double coef[2 * MAXL];
+    __shared__ double s_ctl2m[MAXL * MAXL];
+    const int r0 = 2 * jp, r1 = r0 + 1;
+    const bool has1 = r1 < s.nyi;
+    const int gj0 = r0 + 1, gj1 = r1 + 1;
+    Staged<12, double> stw;
+    stage_load(stw, a.twf, 2 * twiddle_table_len(L));
+    constexpr int SP = P / 2 + 1;
+    double2 cx0[SP], cx1[SP];
+    if (!helper) {
+        const double* row0 = a.spec + ((long long)wv * s.nrow + r0) * s.kp;
+        const double* row1 = row0 + s.kp;
+#pragma unroll
+        for (int u = 0; u < SP; u++) {
+            const int kk = lane + 64 * u;
+            const int tc = (kk <= L / 2) ? kk : 0;
+            cx0[u] = *(const double2*)(row0 + 2 * tc);
+            cx1[u] = *(const double2*)(row1 + 2 * tc);
+        }
+    }
+    double pv[NP];
+#pragma unroll
+    for (int q = 0; q < NP; q++) pv[q] = 0.0;
+    double ctlv = 0.0;
+    if (helper) {
+        if (lane < MAXL * MAXL) ctlv = a.dc->mx.ctl2m[lane];
+        for (int b = lane; b < a.adj_npart; b += 64)
+#pragma unroll
+            for (int q = 0; q < NP; q++) pv[q] += a.adj_partial[(long long)b * ADJ_CYC_NPROF + q];
+    }
+    // every thread: Gs, Gn of both rows (row-constant fields, column 0)
+    double gs[2][NL], gn[2][NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) {
+        gs[0][m] = a.adj_G[(long long)m * g.fs + (long long)gj0 * g.pitch];
+        gn[0][m] = a.adj_G[(long long)(g.nl + m) * g.fs + (long long)gj0 * g.pitch];
+        gs[1][m] = has1 ? a.adj_G[(long long)m * g.fs + (long long)gj1 * g.pitch] : 0.0;
+        gn[1][m] = has1 ? a.adj_G[(long long)(g.nl + m) * g.fs + (long long)gj1 * g.pitch] : 0.0;
+    }
+    stage_store(stw, (double*)tw, a.twf, 2 * twiddle_table_len(L));
+    const double fn = s.ftnorm;
+    if (!helper) {
+        // Hermitian spectra of the two packed rows, as the forward inverse kernel
+        double2* z = wsm + (size_t)wv * ZL;
+        const int h = L / 2;
+#pragma unroll
+        for (int u = 0; u < SP; u++) {
+            const int kk = lane + 64 * u;
+            if (kk <= h) {
+                double xr = fn * cx0[u].x, xi = fn * cx0[u].y;
+                double yr = has1 ? fn * cx1[u].x : 0.0, yi = has1 ? fn * cx1[u].y : 0.0;
+                if (kk == 0 || (L % 2 == 0 && kk == h)) { xi = 0.0; yi = 0.0; }
+                z[zpad(kk)] = make_double2(xr - yi, xi + yr);
+                if (kk != 0 && !(L % 2 == 0 && kk == h)) z[zpad(L - kk)] = make_double2(xr + yi, -xi + yr);
+            }
+        }
+    } else if (lane < MAXL * MAXL) {
+        s_ctl2m[lane] = ctlv;
+    }
+    __syncthreads();
+    if (!helper) {
+        fft_wave_any<P, +1>(wsm + (size_t)wv * ZL, L, s.nstages, s.radix, tw, lane);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NP; q++) pv[q] = wave_sum(pv[q]);
+        if (lane == 0) {
+            const CycHom& ch = a.dc->ch;
+            double ls[NL], ln[NL];
+            ls[0] = ch.hbsi * pv[0];
+            ln[0] = 0.0;
+#pragma unroll
+            for (int m = 1; m < NL; m++) {
+                const double b1 = pv[m], b2 = pv[NL - 1 + m];
+                ls[m] = ch.hc2n[m] * b1 - ch.hc1n[m] * b2;
+                ln[m] = ch.hc1s[m] * b2 - ch.hc2s[m] * b1;
+            }
+            const double rxy = g.dx / g.dy;
+#pragma unroll
+            for (int m = 0; m < NL; m++) {
+                coef[m] = rxy * ls[m];
+                coef[MAXL + m] = rxy * ln[m];
+            }
+            if (jp == 0) {
+                const Mix& mx = a.dc->mx;
+#pragma unroll
+                for (int k = 0; k < NL; k++) {
+                    double ts = 0.0, tn = 0.0;
+#pragma unroll
+                    for (int m = 0; m < NL; m++) {
+                        ts += mx.ctl2m[k * MAXL + m] * ls[m];
+                        tn += mx.ctl2m[k * MAXL + m] * ln[m];
+                    }
+                    a.adj_lc[k] += ts;
+                    a.adj_lc[MAXL + k] += tn;
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sel = 0; sel < 2; sel++) {
+        if (sel == 1 && !has1) break;
+        const long long orow = (long long)(sel == 0 ? gj0 : gj1) * g.pitch;
+        for (int i = (int)threadIdx.x; i < L; i += nth) {
+            double r[NL];
+#pragma unroll
+            for (int m = 0; m < NL; m++)
+                r[m] = inh_at<true>(g, s, wsm + (size_t)m * ZL, i, sel) + coef[m] * gs[sel][m] + coef[MAXL + m] * gn[sel][m];
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int m = 0; m < NL; m++) acc += s_ctl2m[k * MAXL + m] * r[m];
+                double* d = a.adj_lq + k * g.fs + orow + i;
+                *d = *d + g.fnot * acc;
+            }
+        }
+    }
+}
+
+template <int NL, int P>
+__global__ __launch_bounds__(64 * (NL + 1)) void k_invw_adj_cyc(InvArgs a) {
+    invw_adj_cyc_block<NL, P>(a, blockIdx.x);
+}
+
 // Launchers of this object's layer count, over the points per lane (4, 16, 32).
 #define OQG_WCAT2(a, b) a##b
 #define OQG_WCAT(a, b) OQG_WCAT2(a, b)
@@ -961,7 +1251,9 @@ void OQG_WFN(launch_invw_ens_nl)(const InvArgs* args, const int2* map, int nbloc
 void OQG_WFN(launch_fwdw_adj_nl)(const FwdArgs& a, hipStream_t st) {
     const int npairs = (a.s.nyi + 1) / 2;
     const size_t lds = fftw_lds_bytes(a.s, WNL);
-#define FWDWA_CALL(Pc) hipLaunchKernelGGL((k_fwdw_adj<WNL, Pc>), dim3(npairs), dim3(64 * WNL), lds, st, a)
+#define FWDWA_CALL(Pc)                                                                                      \
+    if (a.g.cyclic) hipLaunchKernelGGL((k_fwdw_adj_cyc<WNL, Pc>), dim3(npairs), dim3(64 * WNL), lds, st, a); \
+    else hipLaunchKernelGGL((k_fwdw_adj<WNL, Pc>), dim3(npairs), dim3(64 * WNL), lds, st, a);
     OQG_P_SWITCH(fftw_points(a.s), FWDWA_CALL)
 #undef FWDWA_CALL
 }
@@ -969,7 +1261,9 @@ void OQG_WFN(launch_fwdw_adj_nl)(const FwdArgs& a, hipStream_t st) {
 void OQG_WFN(launch_invw_adj_nl)(const InvArgs& a, hipStream_t st) {
     const int npairs = (a.s.nyi + 1) / 2;
     const size_t lds = fftw_lds_bytes(a.s, WNL);
-#define INVWA_CALL(Pc) hipLaunchKernelGGL((k_invw_adj<WNL, Pc>), dim3(npairs), dim3(64 * (WNL + 1)), lds, st, a)
+#define INVWA_CALL(Pc)                                                                                              \
+    if (a.g.cyclic) hipLaunchKernelGGL((k_invw_adj_cyc<WNL, Pc>), dim3(npairs), dim3(64 * (WNL + 1)), lds, st, a); \
+    else hipLaunchKernelGGL((k_invw_adj<WNL, Pc>), dim3(npairs), dim3(64 * (WNL + 1)), lds, st, a);
     OQG_P_SWITCH(fftw_points(a.s), INVWA_CALL)
 #undef INVWA_CALL
 }
@@ -986,7 +1280,9 @@ void OQG_WFN(set_wave_attrs_nl)(int lim) {
 #undef ATTR_P
 #define ATTR_P(Pc)                                                                                                    \
     (void)hipFuncSetAttribute((const void*)k_fwdw_adj<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);  \
-    (void)hipFuncSetAttribute((const void*)k_invw_adj<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    (void)hipFuncSetAttribute((const void*)k_invw_adj<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);  \
+    (void)hipFuncSetAttribute((const void*)k_fwdw_adj_cyc<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim); \
+    (void)hipFuncSetAttribute((const void*)k_invw_adj_cyc<WNL, Pc>, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     ATTR_P(4) ATTR_P(16) ATTR_P(32)
 #undef ATTR_P
 }

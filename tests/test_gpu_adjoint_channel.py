@@ -1,0 +1,240 @@
+"""The hand-written adjoint of step_qg in a zonally periodic channel.
+
+A channel's state is periodic in x with column nxp-1 a copy of column 0, so
+perturbations keep that copy (column nxp-1 = column 0) and the adjoint is that
+of the periodic fields (the library adds column nxp-1 of an adjoint input into
+column 0 and returns 0 there).  Beyond the box's state the channel step
+carries the momentum-constraint scalars cs, cn, csp, cnp (constraint.F90:182-215),
+fed by boundary integrals of q, p and pm (vorticity.F90:57-62,84-101,
+numerics.F90:374-386) and by the forcing scalars txis, txin.  One step is
+still a quadratic map of (q, qm, p, pm, constraints, wekp, txis, txin), so a
+central difference is exact up to rounding and the dot-product test holds per
+input component.  Over many steps the gradient of ekei(1) is checked against
+central differences of the GPU model and of the CPU oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import cpuref as cr
+from oracle import monitor as mon
+from openqg_amd import workloads as W
+from openqg_amd.adjoint import AdjointState, gradient, objective
+from openqg_amd.forcing import double_gyre_tau, ekman_wekp
+
+pytestmark = pytest.mark.gpu
+
+DOT_TOL = 1e-9
+
+
+def _rms(a):
+    return float(np.sqrt(np.mean(a * a))) or 1.0
+
+
+def _periodic(a):
+    a = np.array(a, dtype=np.float64)
+    a[..., -1] = a[..., 0]
+    return a
+
+
+def _channel():
+    """33 x 17 three-layer channel with Laplacian friction and bottom drag, so
+    that every term of the momentum constraints is active."""
+    w = W.small_channel(33, 17, 3)
+    w.ah2 = [2.0e5] * 3
+    w.delek = 2.0
+    return w
+
+
+def _prepared(w, nsteps):
+    qg = w.make_qg()
+    qg.set_state(p=w.initial_p(), pm=w.initial_p())
+    qg.init_state()
+    ent, wekp, txis, txin = w.forcing()
+    qg.set_forcing(ent, wekp, txis, txin)
+    for nt in range(1, nsteps + 1):
+        qg.step(w.tdt)
+        if (nt - 1) % w.supp_every == 0:
+            qg.suppress()
+    return qg, ent, wekp, (txis, txin)
+
+
+def _fstep(qg, adj, S, C, wekp, txi, ent, tdt, suppressed):
+    adj.load(0)
+    qg.set_state(*S)
+    qg.set_constr(C)
+    qg.set_forcing(ent, wekp, txi[0], txi[1])
+    qg.step(tdt)
+    if suppressed:
+        qg.suppress()
+    return qg.get_state(), qg.get_constr()
+
+
+@pytest.mark.parametrize("suppressed", [False, True], ids=["plain", "suppressed"])
+def test_channel_one_step_dot_product(gpu, suppressed):
+    w = _channel()
+    qg, ent, wekp, txi = _prepared(w, 5)
+    adj = AdjointState(qg)
+    adj.alloc(1)
+    adj.save(0)
+    S, C = qg.get_state(), qg.get_constr()
+    nl, nc = w.nl, len(C)
+    cdyn = np.zeros(nc, bool)
+    cdyn[2 * (nl - 1):] = True   # cs, cn, csp, cnp (dpi, dpip do not enter a channel's p)
+    scale = [_rms(a) for a in S]
+    cscale = _rms(C[cdyn])
+    # the step is affine in wekp and txis, txin: large perturbations keep the
+    # central difference exact and lift the response above its rounding
+    wscale = 1e5 * _rms(wekp)
+    tscale = 1e5 * max(abs(txi[0]), abs(txi[1]), 1e-12)
+    rng = np.random.default_rng(21 + suppressed)
+    lam = [rng.standard_normal(qg.shape) / s for s in scale]
+    lam_c = np.where(cdyn, rng.standard_normal(nc) / cscale, 0.0)
+    print(f"[adjoint] channel one-step dot product, {w.name}, suppressed={suppressed}", flush=True)
+    _fstep(qg, adj, S, C, wekp, txi, ent, w.tdt, suppressed)
+    adj.set(*lam)
+    adj.set_constr(lam_c)
+    adj.zero_wekp()
+    adj.load(0)
+    adj.step(w.tdt, suppressed)
+    lS, lC = adj.get(), adj.get_constr()
+    lw, lt = adj.wekp(), adj.txi()
+    for a in lS:
+        assert not a[..., -1].any()
+    assert not lw[:, -1].any()
+    for comp in ("q", "qm", "p", "pm", "constr", "wekp", "txi", "all"):
+        d = [np.zeros(qg.shape) for _ in range(4)]
+        for c in range(4):
+            if comp in (("q", "qm", "p", "pm")[c], "all"):
+                d[c] = _periodic(scale[c] * rng.standard_normal(qg.shape))
+        dc = np.where(cdyn, cscale * rng.standard_normal(nc), 0.0) if comp in ("constr", "all") else np.zeros(nc)
+        dw = _periodic(wscale * rng.standard_normal(wekp.shape)) if comp in ("wekp", "all") else np.zeros_like(wekp)
+        dt_ = tscale * rng.standard_normal(2) if comp in ("txi", "all") else np.zeros(2)
+        eps = 1e-3
+        Fp, Cp = _fstep(qg, adj, [s + eps * x for s, x in zip(S, d)], C + eps * dc, wekp + eps * dw,
+                        (txi[0] + eps * dt_[0], txi[1] + eps * dt_[1]), ent, w.tdt, suppressed)
+        Fm, Cm = _fstep(qg, adj, [s - eps * x for s, x in zip(S, d)], C - eps * dc, wekp - eps * dw,
+                        (txi[0] - eps * dt_[0], txi[1] - eps * dt_[1]), ent, w.tdt, suppressed)
+        fd = (sum(float(np.sum(l * (a - b))) for l, a, b in zip(lam, Fp, Fm))
+              + float(np.sum(lam_c * (Cp - Cm)))) / (2 * eps)
+        ad = (sum(float(np.sum(l * x)) for l, x in zip(lS, d)) + float(np.sum(lC * dc))
+              + float(np.sum(lw * dw)) + float(np.sum(lt * dt_)))
+        err = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
+        print(f"[adjoint]   d={comp:6s} fd={fd:+.12e} adj={ad:+.12e} rel={err:.2e}", flush=True)
+        assert err <= DOT_TOL, (comp, fd, ad)
+    qg.set_forcing(ent, wekp, txi[0], txi[1])
+
+
+def _controls(w, qg):
+    """Initial PV of the seeded state and a periodic stress with both components."""
+    qg.set_state(p=w.initial_p(), pm=w.initial_p())
+    qg.init_state()
+    q0 = qg.get_state()[0]
+    taux, _ = double_gyre_tau(w.nyp, w.nxp, w.dx, w.tau0)
+    x = np.arange(w.nxp) / (w.nxp - 1)
+    y = np.arange(w.nyp) / (w.nyp - 1)
+    tauy = _periodic(0.3 * _rms(taux) * np.cos(np.pi * y)[:, None] * np.sin(2 * np.pi * x)[None, :])
+    return q0, _periodic(taux), tauy
+
+
+def _inner(a):
+    """Rows 1..nyp-2 (the inversion's input), periodic columns."""
+    b = np.zeros_like(a)
+    b[:, 1:-1, :] = a[:, 1:-1, :]
+    return _periodic(b)
+
+
+# J of the seeded channel state is ~1e19 and depends weakly on the stress over
+# a few steps: stress directions of 1e3 x rms(tau) put the difference of J well
+# above J's rounding (J is close to linear in the stress there, and the central
+# difference cancels the even-order terms)
+TAU_DIR = 1e3
+
+
+def _fd(qg, q0, tx, ty, d, w, nsteps, supp, rel=1e-5):
+    dq, dx_, dy_ = d
+    jp = objective(qg, q0 + rel * dq, tx + rel * dx_, ty + rel * dy_, w.tdt, nsteps, supp)
+    jm = objective(qg, q0 - rel * dq, tx - rel * dx_, ty - rel * dy_, w.tdt, nsteps, supp)
+    return (jp - jm) / (2 * rel)
+
+
+def test_channel_gradient_matches_finite_differences(gpu):
+    w = _channel()
+    nsteps, supp = 12, 5
+    qg = w.make_qg()
+    q0, tx, ty = _controls(w, qg)
+    J, gx, gy, gq, fsteps = gradient(qg, q0, tx, ty, w.tdt, nsteps, supp)
+    J2 = objective(qg, q0, tx, ty, w.tdt, nsteps, supp)
+    assert fsteps == nsteps
+    assert J > 0 and abs(J - J2) <= 1e-13 * J, (J, J2)
+    assert not gq[:, 0, :].any() and not gq[:, -1, :].any() and not gq[..., -1].any()
+    assert not gx[:, -1].any() and not gy[:, -1].any()
+    rng = np.random.default_rng(17)
+    dirs = {
+        "taux": (np.zeros_like(q0), _periodic(TAU_DIR * _rms(tx) * rng.standard_normal(tx.shape)), np.zeros_like(ty)),
+        "tauy": (np.zeros_like(q0), np.zeros_like(tx), _periodic(TAU_DIR * _rms(tx) * rng.standard_normal(ty.shape))),
+        "q0": (_inner(_rms(q0) * rng.standard_normal(q0.shape)), np.zeros_like(tx), np.zeros_like(ty)),
+    }
+    print(f"[adjoint] channel gradient vs central differences, {w.name}, {nsteps} steps, J={J:.6e}", flush=True)
+    for name, (dq, dx_, dy_) in dirs.items():
+        fd = _fd(qg, q0, tx, ty, (dq, dx_, dy_), w, nsteps, supp)
+        ad = float(np.sum(gq * dq) + np.sum(gx * dx_) + np.sum(gy * dy_))
+        err = abs(fd - ad) / abs(fd)
+        print(f"[adjoint]   {name:5s} fd={fd:+.10e} adj={ad:+.10e} rel={err:.2e}", flush=True)
+        assert err <= 1e-6, (name, fd, ad)
+
+
+def test_channel_checkpointed_gradient_equals_stored(gpu):
+    w = _channel()
+    nsteps, supp = 23, 5
+    qg = w.make_qg()
+    q0, tx, ty = _controls(w, qg)
+    ref = gradient(qg, q0, tx, ty, w.tdt, nsteps, supp, snapshots=None)
+    for s in (1, 4):
+        got = gradient(qg, q0, tx, ty, w.tdt, nsteps, supp, snapshots=s)
+        print(f"[adjoint] channel revolve with {s} snapshots: {got[4]} forward steps for {nsteps}", flush=True)
+        assert got[0] == ref[0]
+        for a, b in zip(got[1:4], ref[1:4]):
+            np.testing.assert_array_equal(a, b)
+
+
+def _oracle_objective(w, S0, c0, tx, ty, nsteps, supp, bcco):
+    o = cr.Basin(w.params())
+    o.set_state(*S0)
+    o.set_constr(c0)
+    wekp, txis, txin = ekman_wekp(w.nxp, w.nyp, w.dx, w.dx, w.fnot, True, tx, ty)
+    ent = np.zeros((max(w.nl - 1, 1), w.nyp, w.nxp))
+    for nt in range(1, nsteps + 1):
+        o.step(w.tdt, ent, wekp, txis, txin)
+        if (nt - 1) % supp == 0:
+            o.suppress()
+    q, _, p, _ = o.get_state()
+    d = mon.diagnose(q, p, w.dx, w.dx, w.fnot, bcco, True, w.h, w.gp, w.dz_sign, 1000.0, w.tdt)
+    return d["ekei"][0]
+
+
+def test_channel_stress_gradient_matches_oracle_perturbation(gpu):
+    """Reference gradient: central differences of the CPU oracle's channel
+    model, from the same initial state, in random periodic stress directions."""
+    w = _channel()
+    nsteps, supp = 10, 5
+    qg = w.make_qg()
+    q0, tx, ty = _controls(w, qg)
+    J, gx, gy, _, _ = gradient(qg, q0, tx, ty, w.tdt, nsteps, supp)
+    qg.set_state(q=q0, qm=q0, p=np.zeros(qg.shape), pm=np.zeros(qg.shape))
+    qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
+    AdjointState(qg).invert()
+    S0, c0 = qg.get_state(), qg.get_constr()
+    Jo = _oracle_objective(w, S0, c0, tx, ty, nsteps, supp, qg.bcco)
+    assert abs(Jo - J) <= 1e-10 * J, (Jo, J)
+    rng = np.random.default_rng(19)
+    s = TAU_DIR * _rms(tx)
+    for k in range(2):
+        dx_, dy_ = _periodic(s * rng.standard_normal(tx.shape)), _periodic(s * rng.standard_normal(ty.shape))
+        eps = 1e-5
+        jp = _oracle_objective(w, S0, c0, tx + eps * dx_, ty + eps * dy_, nsteps, supp, qg.bcco)
+        jm = _oracle_objective(w, S0, c0, tx - eps * dx_, ty - eps * dy_, nsteps, supp, qg.bcco)
+        fd = (jp - jm) / (2 * eps)
+        ad = float(np.sum(gx * dx_) + np.sum(gy * dy_))
+        err = abs(fd - ad) / abs(fd)
+        print(f"[adjoint] channel oracle direction {k}: fd={fd:+.10e} adj={ad:+.10e} rel={err:.2e}", flush=True)
+        assert err <= 1e-6
