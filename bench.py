@@ -18,7 +18,8 @@ Also reported:
                 257^2 3-layer box (forward + hand-written adjoint) against the
                 plain forward run, store-all and with 32 revolve snapshots;
                 and a batch of 16 perturbed members' gradients, one after
-                another and concurrently (one context and stream each);
+                another and concurrently (one context and stream each); the
+                same gradient of the 97 x 49 atmosphere channel;
   coupled       the coupled configuration for 30 days: the 385^2 ocean with its
                 100 m mixed layer and the 97 x 49 atmosphere with its 1000 m
                 mixed layer, exchanging stress, heat flux and entrainment every
@@ -280,13 +281,28 @@ def adjoint_bench(days: float, snapshots: int, batch: int = 16):
     """Gradient of the layer-1 kinetic energy after `days` of the 257^2 3-layer
     box with respect to the wind stress and the initial PV: forward + adjoint
     (store-all snapshots, and binomial checkpointing with `snapshots` slots)
-    against the plain forward run of the same steps (graph replay)."""
+    against the plain forward run of the same steps (graph replay); the same
+    for the 97 x 49 atmosphere channel (dt 200 s) under the `channel` key."""
     from openqg_amd import workloads as W
-    from openqg_amd.adjoint import gradient
-    from openqg_amd.forcing import double_gyre_tau
 
     w = W.c5_box257()
     n = int(round(days * 86400.0 / w.dt))
+    out = _gradient_timing(w, n, snapshots)
+    out["workload"] = f"{w.name}_{days:g}d_ke_gradient"
+    if batch > 1:
+        out["batch"] = adjoint_batch_bench(w, n, snapshots, batch)
+    # the atmosphere channel (97 x 49, dt 200 s) over the same days
+    wc = W.atmos_channel()
+    nc = int(round(days * 86400.0 / wc.dt))
+    out["channel"] = dict(workload=f"{wc.name}_{days:g}d_ke_gradient", **_gradient_timing(wc, nc, snapshots))
+    return out
+
+
+def _gradient_timing(w, n: int, snapshots: int):
+    """Forward run and gradients (store-all, revolve) of workload w over n steps."""
+    from openqg_amd.adjoint import gradient
+    from openqg_amd.forcing import double_gyre_tau
+
     qg = w.make_qg()
     p = w.initial_p()
     qg.set_state(p=p, pm=p)
@@ -302,7 +318,7 @@ def adjoint_bench(days: float, snapshots: int, batch: int = 16):
     qg.run(w.tdt, n, 0, w.supp_every)
     qg.sync()
     fwd = time.perf_counter() - t0
-    out = {"workload": f"{w.name}_{days:g}d_ke_gradient", "steps": n, "forward_s": round(fwd, 4)}
+    out = {"steps": n, "forward_s": round(fwd, 4)}
     for name, snaps in (("store_all", None), (f"revolve_{snapshots}", snapshots)):
         gradient(qg, q0, taux, tauy, w.tdt, n, w.supp_every, snaps)   # allocation, graph capture
         qg.sync()
@@ -312,8 +328,6 @@ def adjoint_bench(days: float, snapshots: int, batch: int = 16):
         out[name] = {"s": round(el, 4), "ratio_vs_forward": round(el / fwd, 3), "forward_steps": int(fsteps)}
     out["J"] = J
     qg.close()
-    if batch > 1:
-        out["batch"] = adjoint_batch_bench(w, n, snapshots, batch)
     return out
 
 
