@@ -174,6 +174,13 @@ class AdjointState:
                 "oqg_adj_get")
         return tuple(out)
 
+    def get_lq(self) -> np.ndarray:
+        """The adjoint of q alone."""
+        out = np.empty(self.qg.shape)
+        self._c(self._lib.oqg_adj_get(self.qg._h, out.ctypes.data_as(ctypes.c_void_p), None, None, None, OQG_HOST),
+                "oqg_adj_get")
+        return out
+
     def step(self, tdt: float, suppressed: bool):
         self._c(self._lib.oqg_adj_step(self.qg._h, float(tdt), int(bool(suppressed))), "oqg_adj_step")
 
@@ -359,7 +366,8 @@ def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: fl
     ent = np.zeros((max(b.nl - 1, 1), b.nyp, b.nxp))
     qg.set_forcing(ent, None, 0.0, 0.0)
     qg.set_wind(taux, tauy)   # wekp on the device
-    qg.set_state(q=q0, qm=q0, p=np.zeros(qg.shape), pm=np.zeros(qg.shape))
+    # the inversion writes p from q alone and then sets qm = q, pm = p
+    qg.set_state(q=q0)
     qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
     adj.invert()
     traj = Trajectory(adj, tdt, nsteps, supp_every, snapshots)
@@ -369,7 +377,7 @@ def gradient(qg: QG, q0: np.ndarray, taux: np.ndarray, tauy: np.ndarray, tdt: fl
     traj.reverse()
     gtx, gty = adj.wind()     # the transpose of the wind forcing on the device
     adj.adj_invert()
-    gq0 = adj.get()[0]
+    gq0 = adj.get_lq()
     return J, gtx, gty, gq0, traj.forward_steps
 
 
@@ -403,7 +411,7 @@ def objective(qg: QG, q0, taux, tauy, tdt: float, nsteps: int, supp_every: int =
     ent = np.zeros((max(b.nl - 1, 1), b.nyp, b.nxp))
     qg.set_forcing(ent, None, 0.0, 0.0)
     qg.set_wind(taux, tauy)
-    qg.set_state(q=q0, qm=q0, p=np.zeros(qg.shape), pm=np.zeros(qg.shape))
+    qg.set_state(q=q0)
     qg.set_constr(np.zeros(qg._lib.oqg_constr_len(qg._h)))
     check(qg._lib.oqg_invert(qg._h), qg._h, "oqg_invert")
     qg.run(tdt, nsteps, 0, supp_every)
