@@ -154,6 +154,18 @@ int oqg_plan_run(oqg_ctx* ctx, double tdt, int nsteps, int step0, int supp_every
 /* Wait for all work queued on the context's stream. */
 int oqg_sync(oqg_ctx* ctx);
 
+/* Tile height of the vorticity kernel (no counterpart in the reference, whose
+ * vort_step has no tiling): 4, 8 or 16 rows of 64 points; 0 restores the
+ * default rule (the fewest rows whose workgroups the device holds at once).
+ * The results do not depend on it, except for the summation order of the
+ * layer-1 entrainment integral.  oqg_tune times trial_steps steps of oqg_run
+ * (from step 1, suppression every supp_every) with each height, restores the
+ * state, keeps the fastest and returns it (negative on error); ms[] (3
+ * entries: 4, 8, 16 rows; may be NULL) receives the times. */
+int oqg_set_tile_rows(oqg_ctx* ctx, int rows);
+int oqg_get_tile_rows(const oqg_ctx* ctx);
+int oqg_tune(oqg_ctx* ctx, double tdt, int trial_steps, int supp_every, double* ms);
+
 /* solve_inhomog_eqn (inhomog.F90:156-168) for mode m = 1..nl: rhs and out are
  * (nxp,nyp) arrays.  Synchronous. */
 int oqg_solve_inhomog(oqg_ctx* ctx, int m, const double* rhs, double* out, int kind);
@@ -379,10 +391,12 @@ const char* oqg_ens_last_error(const oqg_ens* ens);
 int oqg_ens_run(oqg_ens* ens, const double* tdt, const int* nsteps, int step0, int supp_every);
 int oqg_ens_sync(oqg_ens* ens);
 
-/* Launch schedule (OQG_ENS_* bits); oqg_ens_tune picks the fastest one by
- * timing trial_steps steps of each on the members (whose state it restores)
- * and returns it (negative on error); ms[] (4 entries, may be NULL) receives
- * each schedule's time. */
+/* Launch schedule (OQG_ENS_* bits).  oqg_ens_tune times trial_steps steps
+ * on the members (whose state it restores) of every launch schedule with
+ * every tile height (the members' own default rule, then 4, 8 and 16 rows for
+ * all; oqg_set_tile_rows), keeps the fastest pair (the tile height set on
+ * the members) and returns its index t*4 + schedule (negative on error);
+ * ms[] (16 entries, [t*4 + schedule], may be NULL) receives the times. */
 int oqg_ens_set_schedule(oqg_ens* ens, int schedule);
 int oqg_ens_get_schedule(const oqg_ens* ens);
 int oqg_ens_tune(oqg_ens* ens, const double* tdt, int trial_steps, int supp_every, double* ms);

@@ -100,6 +100,9 @@ SIGNATURES = [
     ("oqg_run", _i, [_vp, _d, _i, _i, _i]),
     ("oqg_plan_run", _i, [_vp, _d, _i, _i, _i]),
     ("oqg_sync", _i, [_vp]),
+    ("oqg_set_tile_rows", _i, [_vp, _i]),
+    ("oqg_get_tile_rows", _i, [_vp]),
+    ("oqg_tune", _i, [_vp, _d, _i, _i, _dp]),
     ("oqg_solve_inhomog", _i, [_vp, _i, _vp, _vp, _i]),
     ("oqg_get_modes", _i, [_vp, _dp, _dp, _dp, _dp]),
     ("oqg_set_stream", _i, [_vp, _vp]),

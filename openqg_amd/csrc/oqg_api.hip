@@ -754,7 +754,8 @@ static int setup(oqg_ctx* c) {
     if (!rc && c->variant == 1) rc = dalloc(c, &c->gamw, nspec);
     if (!rc) rc = dalloc(c, &c->colsum, (size_t)nl * s.ncol);
     if (!rc) rc = dalloc(c, &c->colw, (size_t)s.ncol);
-    if (!rc) rc = dalloc(c, &c->entpart, (size_t)c->tiles_x * c->tiles_y);
+    // sized for the shortest tiles, so that the tile height can change (oqg_set_tile_rows)
+    if (!rc) rc = dalloc(c, &c->entpart, (size_t)c->tiles_x * ((nyp - 2 + 3) / 4));
     if (!rc && g.cyclic) rc = dalloc(c, &c->cpart, (size_t)2 * nl * c->tiles_x * CYC_SLOTS);
     if (!rc) rc = dalloc(c, &c->fcon, F_LEN);
     if (!rc) rc = dalloc(c, &c->ent1, f1);
@@ -1600,6 +1601,109 @@ extern "C" {
 int oqg_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
     ctx_join(c);
     return run_steps(c, tdt, nsteps, step0, supp, false);
+}
+
+}  // extern "C"
+
+// The vorticity kernel's tile height (4, 8 or 16 rows; 0: the occupancy rule
+// of vort_rows).  Captured graphs hold the launch shape, so they go.  Only the
+// order in which the per-tile entrainment integrals are summed depends on it.
+static int set_tile_rows(oqg_ctx* c, int rows) {
+    if (rows == 0) rows = vort_rows(c->tiles_x, c->g.nyp, c->g.nl);
+    if (rows != 4 && rows != 8 && rows != 16) return fail(c, OQG_ERR_ARG, "tile rows: 4, 8 or 16 (0: the default rule)");
+    if (rows == c->vort_ty) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (auto& kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    c->vort_ty = rows;
+    c->tiles_y = (c->g.nyp - 2 + rows - 1) / rows;
+    return 0;
+}
+
+// Device copies of a context's state (both time levels of q, p and the
+// constraint blocks, and the parity), for trial runs.
+struct StateKeep {
+    std::vector<void*> bufs;
+    int cur = 0;
+    int save(oqg_ctx* c) {
+        cur = c->cur;
+        for (int t = 0; t < 2; t++) {
+            const std::pair<const void*, size_t> parts[3] = {
+                {c->qbuf[t], c->fbytes}, {c->pbuf[t], c->fbytes}, {c->cbuf[t], C_LEN * sizeof(double)}};
+            for (const auto& pr : parts) {
+                void* d = nullptr;
+                if (hipMalloc(&d, pr.second) != hipSuccess) return OQG_ERR_HIP;
+                bufs.push_back(d);
+                if (hipMemcpyAsync(d, pr.first, pr.second, hipMemcpyDeviceToDevice, c->st) != hipSuccess) return OQG_ERR_HIP;
+            }
+        }
+        return 0;
+    }
+    void restore(oqg_ctx* c, hipStream_t st) {
+        for (int t = 0; t < 2; t++) {
+            hipMemcpyAsync(c->qbuf[t], bufs[t * 3 + 0], c->fbytes, hipMemcpyDeviceToDevice, st);
+            hipMemcpyAsync(c->pbuf[t], bufs[t * 3 + 1], c->fbytes, hipMemcpyDeviceToDevice, st);
+            hipMemcpyAsync(c->cbuf[t], bufs[t * 3 + 2], C_LEN * sizeof(double), hipMemcpyDeviceToDevice, st);
+        }
+        c->cur = cur;
+    }
+    void release(hipStream_t st) {
+        hipStreamSynchronize(st);
+        for (void* b : bufs) hipFree(b);
+        bufs.clear();
+    }
+};
+
+extern "C" {
+
+int oqg_set_tile_rows(oqg_ctx* c, int rows) {
+    ctx_join(c);
+    if (!c) return OQG_ERR_ARG;
+    return set_tile_rows(c, rows);
+}
+
+int oqg_get_tile_rows(const oqg_ctx* c) { return c ? c->vort_ty : -OQG_ERR_ARG; }
+
+int oqg_tune(oqg_ctx* c, double tdt, int trial_steps, int supp_every, double* ms) {
+    ctx_join(c);
+    if (!c || trial_steps < 1) return -OQG_ERR_ARG;
+    if (c->ext_cur.q || c->ext_old.q) return -fail(c, OQG_ERR_STATE, "oqg_tune: a checkpoint is loaded by reference");
+    StateKeep keep;
+    if (keep.save(c)) return (keep.release(c->st), -fail(c, OQG_ERR_HIP, "oqg_tune: could not keep the state"));
+    const int rows0 = c->vort_ty;
+    hipEvent_t ev[2];
+    hipEventCreate(&ev[0]);
+    hipEventCreate(&ev[1]);
+    int best = rows0, rc = 0;
+    double best_ms = 1e300;
+    const int opts[3] = {4, 8, 16};
+    for (int t = 0; t < 3 && !rc; t++) {
+        rc = set_tile_rows(c, opts[t]);
+        if (!rc) rc = run_steps(c, tdt, trial_steps, 0, supp_every, false);   // captures the graphs
+        keep.restore(c, c->st);
+        if (rc) break;
+        hipEventRecord(ev[0], c->st);
+        rc = run_steps(c, tdt, trial_steps, 0, supp_every, false);
+        hipEventRecord(ev[1], c->st);
+        hipEventSynchronize(ev[1]);
+        keep.restore(c, c->st);
+        float el = 0.f;
+        hipEventElapsedTime(&el, ev[0], ev[1]);
+        if (ms) ms[t] = el;
+        if (el < best_ms) {
+            best_ms = el;
+            best = opts[t];
+        }
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    keep.release(c->st);
+    if (rc) {
+        set_tile_rows(c, rows0);
+        return -rc;
+    }
+    if (int r = set_tile_rows(c, best)) return -r;
+    return best;
 }
 
 int oqg_plan_run(oqg_ctx* c, double tdt, int nsteps, int step0, int supp) {
@@ -3057,30 +3161,47 @@ int oqg_ens_tune(oqg_ens* e, const double* tdt, int trial_steps, int supp_every,
     int best = 0;
     double best_ms = 1e300;
     const int nsched = OQG_ENS_ONE_YSOLVE | OQG_ENS_LARGEST_FIRST;
+    const int tile_opt[4] = {0, 4, 8, 16};   // 0: each member's own rule
+    std::vector<int> rows0(n);
+    for (int i = 0; i < n; i++) rows0[i] = e->m[i]->vort_ty;
+    auto set_tiles = [&](int t) {
+        int r = 0;
+        for (int i = 0; i < n && !r; i++) r = set_tile_rows(e->m[i], tile_opt[t]);
+        ens_clear_plans(e);
+        return r;
+    };
     int rc = 0;
-    for (int sched = 0; sched <= nsched && !rc; sched++) {
-        oqg_ens_set_schedule(e, sched);
-        rc = ens_run(e, tdt, steps.data(), 0, supp_every);   // builds the plan and its graphs
-        restore();
-        if (rc) break;
-        hipEventRecord(ev[0], e->st);
-        rc = ens_run(e, tdt, steps.data(), 0, supp_every);
-        hipEventRecord(ev[1], e->st);
-        hipEventSynchronize(ev[1]);
-        restore();
-        float t = 0.f;
-        hipEventElapsedTime(&t, ev[0], ev[1]);
-        if (ms) ms[sched] = t;
-        if (t < best_ms) {
-            best_ms = t;
-            best = sched;
+    for (int t = 0; t < 4 && !rc; t++) {
+        rc = set_tiles(t);
+        for (int sched = 0; sched <= nsched && !rc; sched++) {
+            oqg_ens_set_schedule(e, sched);
+            rc = ens_run(e, tdt, steps.data(), 0, supp_every);   // builds the plan and its graphs
+            restore();
+            if (rc) break;
+            hipEventRecord(ev[0], e->st);
+            rc = ens_run(e, tdt, steps.data(), 0, supp_every);
+            hipEventRecord(ev[1], e->st);
+            hipEventSynchronize(ev[1]);
+            restore();
+            float el = 0.f;
+            hipEventElapsedTime(&el, ev[0], ev[1]);
+            if (ms) ms[t * (nsched + 1) + sched] = el;
+            if (el < best_ms) {
+                best_ms = el;
+                best = t * (nsched + 1) + sched;
+            }
         }
     }
     hipEventDestroy(ev[0]);
     hipEventDestroy(ev[1]);
     release();
-    if (rc) return -rc;
-    oqg_ens_set_schedule(e, best);
+    if (rc) {
+        for (int i = 0; i < n; i++) set_tile_rows(e->m[i], rows0[i]);
+        ens_clear_plans(e);
+        return -rc;
+    }
+    if (int r = set_tiles(best / (nsched + 1))) return -r;
+    oqg_ens_set_schedule(e, best % (nsched + 1));
     return best;
 }
 

@@ -60,16 +60,21 @@ class Ensemble:
     def schedule(self, value: int):
         self._check(self._lib.oqg_ens_set_schedule(self._h, int(value)), "oqg_ens_set_schedule")
 
+    TILE_OPTIONS = (0, 4, 8, 16)   # 0: each member's default rule
+
     def tune(self, tdt, trial_steps: int = 8, supp_every: int = 25):
-        """Time every launch schedule on the members (their state is restored),
-        keep the fastest; returns (schedule, [ms per schedule])."""
+        """Time every launch schedule with every vorticity tile height on the
+        members (their state is restored), keep the fastest pair; returns
+        (schedule, [ms] indexed t*4 + schedule) and sets `tile_option` to the
+        height chosen (TILE_OPTIONS[t])."""
         t, _ = self._arrays(tdt, 0, len(self.members))
-        ms = (ctypes.c_double * 4)()
+        ms = (ctypes.c_double * 16)()
         best = self._lib.oqg_ens_tune(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(trial_steps),
                                       int(supp_every), ms)
         if best < 0:
             self._check(-best, "oqg_ens_tune")
-        return int(best), list(ms)
+        self.tile_option = self.TILE_OPTIONS[best // 4]
+        return int(best % 4), list(ms)
 
     def close(self):
         if getattr(self, "_h", None) is not None:
@@ -133,7 +138,11 @@ class EnsembleGroups:
             e.sync()
 
     def tune(self, tdt, trial_steps: int = 8, supp_every: int = 25):
-        """Ensemble.tune for every group; returns [(schedule, [ms])] per group."""
+        """Ensemble.tune for every group and QG.tune for every solo member;
+        returns [(schedule, [ms])] per group."""
+        t_all = np.broadcast_to(np.asarray(tdt, dtype=np.float64), (len(self.members),))
+        for i in self.solo:
+            self.members[i].tune(float(t_all[i]), trial_steps, supp_every)
         return [e.tune(t, trial_steps, supp_every) for e, t in zip(self.parts, self._split(tdt))]
 
     def close(self):

@@ -398,6 +398,24 @@ class QG:
         keys = names.value.decode().split("\n")[:n]
         return {k: ms[i] for i, k in enumerate(keys)}
 
+    @property
+    def tile_rows(self) -> int:
+        """Rows of the vorticity kernel's 64-column tiles (4, 8 or 16)."""
+        return int(self._lib.oqg_get_tile_rows(self._h))
+
+    @tile_rows.setter
+    def tile_rows(self, rows: int):
+        check(self._lib.oqg_set_tile_rows(self._h, int(rows)), self._h, "oqg_set_tile_rows")
+
+    def tune(self, tdt: float, trial_steps: int = 50, supp_every: int = 25):
+        """Time `trial_steps` steps with each tile height (the state is
+        restored), keep the fastest; returns (rows, {rows: ms})."""
+        ms = (ctypes.c_double * 3)()
+        best = self._lib.oqg_tune(self._h, float(tdt), int(trial_steps), int(supp_every), ms)
+        if best < 0:
+            check(-best, self._h, "oqg_tune")
+        return int(best), dict(zip((4, 8, 16), list(ms)))
+
     def set_stream(self, stream_handle: int | None):
         check(self._lib.oqg_set_stream(self._h, ctypes.c_void_p(stream_handle) if stream_handle else None),
               self._h, "oqg_set_stream")

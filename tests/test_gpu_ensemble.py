@@ -79,7 +79,10 @@ def test_ensemble_runs_continue_and_tune_restores_state(gpu):
     ens.sync()
     before = [qg.get_state() for qg in qgs]
     best, ms = ens.tune(tdt, trial_steps=4, supp_every=3)
-    assert ens.schedule == best and len(ms) == 4 and all(t > 0 for t in ms)
+    assert ens.schedule == best and len(ms) == 16 and all(t > 0 for t in ms)
+    assert ens.tile_option in (0, 4, 8, 16)
+    if ens.tile_option:
+        assert all(qg.tile_rows == ens.tile_option for qg in qgs)
     for qg, b in zip(qgs, before):
         for x, y in zip(qg.get_state(), b):
             assert np.array_equal(x, y)

@@ -428,3 +428,37 @@ def test_unaligned_stretches_equal_eager(gpu):
         for a, b in zip(ref[0], out[0]):
             assert np.array_equal(a, b), mode
         assert np.array_equal(ref[1], out[1]), mode
+
+
+@pytest.mark.parametrize("make", [lambda: W.small_box(65, 3), lambda: W.small_channel(97, 49, 3)],
+                         ids=["box65", "chan97x49"])
+def test_tile_heights_agree_and_tune_restores_state(gpu, make):
+    """The vorticity kernel's tile height (4, 8, 16 rows) leaves every point's
+    arithmetic as it is: without entrainment the runs agree bitwise; with it
+    only the summation order of its layer-1 integral differs.  oqg_tune keeps
+    the state it times from and sets the fastest height."""
+    w = make()
+    print(f"[parity] tile heights, {w.name}", flush=True)
+    for ent_amp in (0.0, 1e-6):
+        runs = {}
+        for rows in (4, 8, 16):
+            qg, _, _ = _setup(w, ent_amp=ent_amp)
+            qg.tile_rows = rows
+            assert qg.tile_rows == rows
+            qg.run(w.tdt, 7, 0, 3)
+            runs[rows] = qg.get_state()
+            qg.close()
+        for rows in (8, 16):
+            for a, b in zip(runs[rows], runs[4]):
+                if ent_amp == 0.0:
+                    assert np.array_equal(a, b), rows
+                else:
+                    assert rel_err(a, b) <= 1e-13, rows
+    qg, _, _ = _setup(w)
+    qg.run(w.tdt, 3, 0, 3)
+    before = qg.get_state()
+    rows, ms = qg.tune(w.tdt, trial_steps=6, supp_every=3)
+    assert rows in (4, 8, 16) and qg.tile_rows == rows and all(t > 0 for t in ms.values())
+    for a, b in zip(qg.get_state(), before):
+        assert np.array_equal(a, b)
+    qg.close()
