@@ -277,6 +277,11 @@ def ensemble_bench(members: int, days: float):
     }
 
 
+def progress(msg: str):
+    """One line per finished leg on stderr (stdout carries the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def adjoint_bench(days: float, snapshots: int, batch: int = 16):
     """Gradient of the layer-1 kinetic energy after `days` of the 257^2 3-layer
     box with respect to the wind stress and the initial PV: forward + adjoint
@@ -289,12 +294,15 @@ def adjoint_bench(days: float, snapshots: int, batch: int = 16):
     n = int(round(days * 86400.0 / w.dt))
     out = _gradient_timing(w, n, snapshots)
     out["workload"] = f"{w.name}_{days:g}d_ke_gradient"
+    progress(f"adjoint {w.name}: revolve {out[f'revolve_{snapshots}']['ratio_vs_forward']}x the forward run")
     if batch > 1:
         out["batch"] = adjoint_batch_bench(w, n, snapshots, batch)
+        progress(f"adjoint batch: {out['batch']['s']} s")
     # the atmosphere channel (97 x 49, dt 200 s) over the same days
     wc = W.atmos_channel()
     nc = int(round(days * 86400.0 / wc.dt))
     out["channel"] = dict(workload=f"{wc.name}_{days:g}d_ke_gradient", **_gradient_timing(wc, nc, snapshots))
+    progress(f"adjoint {wc.name}: store-all {out['channel']['store_all']['ratio_vs_forward']}x the forward run")
     return out
 
 
@@ -537,13 +545,19 @@ def main():
                         for k, v in kt.items() if v > 0},
         "step_algorithmic_GBs": round(algorithmic_bytes(w, "step") / (ms_per_step * 1e-3) / 1e9, 1),
     }
+    progress(f"plain: {days_per_s:.1f} simulated days/s, {ms_per_step:.4f} ms/step")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_seconds)
+        progress(f"cpu baseline: {cpu['value']:.2f} simulated days/s")
     qg.close()
     ens = None if args.no_ensemble else ensemble_bench(args.ens_members, args.ens_days)
+    if ens:
+        progress(f"ensemble: {ens['value']} aggregate simulated days/s")
     adj = None if args.no_adjoint else adjoint_bench(args.adj_days, args.adj_snapshots, args.adj_batch)
     cpl = None if args.no_coupled else coupled_bench(30.0)
+    if cpl:
+        progress(f"coupled: {cpl['value']} simulated days/s")
     out = {
         "metric": METRIC,
         "value": round(days_per_s, 3),
