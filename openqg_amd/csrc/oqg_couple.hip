@@ -309,25 +309,52 @@ __global__ __launch_bounds__(256) void k_aml_step(CplArgs a) {
     const long long P = g.pitch;
     __shared__ double tl[4][AML_LH * AML_LW];   // ast, astm, hmixa, hmixam
     __shared__ double pl[3][AML_PH * AML_PW];   // p1, taux, tauy
+    // every global load of the block is issued before any is used (from
+    // always-valid offsets; the ones outside the basin are replaced by 0)
+    constexpr int NT = (AML_LH * AML_LW + 255) / 256, NPL = (AML_PH * AML_PW + 255) / 256;
     const double* tf[4] = {a.ast, a.astm, a.hmixa, a.hmixam};
-    for (int t = threadIdx.x; t < AML_LH * AML_LW; t += blockDim.x) {
+    const double* pf[3] = {a.pa1, a.tauxa, a.tauya};
+    double vt[NT][4], vp[NPL][3];
+    bool okt[NT], okp[NPL];
+#pragma unroll
+    for (int u = 0; u < NT; u++) {
+        const int t = threadIdx.x + 256 * u;
         const int ly = t / AML_LW, lx = t - ly * AML_LW;
         int i = i0 - AML_H + lx;
         const int j = j0 - AML_H + ly;
         if (g.cyclic) i = (i % nxt + nxt) % nxt;
-        const bool in = i >= 0 && i < nxt && j >= 0 && j < nyt;
-        const long long o = (long long)j * P + i;
+        okt[u] = t < AML_LH * AML_LW && i >= 0 && i < nxt && j >= 0 && j < nyt;
+        const long long o = okt[u] ? (long long)j * P + i : 0;
 #pragma unroll
-        for (int f = 0; f < 4; f++) tl[f][t] = in ? tf[f][o] : 0.0;
+        for (int f = 0; f < 4; f++) vt[u][f] = tf[f][o];
     }
-    const double* pf[3] = {a.pa1, a.tauxa, a.tauya};
-    for (int t = threadIdx.x; t < AML_PH * AML_PW; t += blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < NPL; u++) {
+        const int t = threadIdx.x + 256 * u;
         const int ly = t / AML_PW, lx = t - ly * AML_PW;
         const int i = i0 + lx, j = j0 + ly;
-        const bool in = i < g.nxp && j < g.nyp;
-        const long long o = (long long)j * P + i;
+        okp[u] = t < AML_PH * AML_PW && i < g.nxp && j < g.nyp;
+        const long long o = okp[u] ? (long long)j * P + i : 0;
 #pragma unroll
-        for (int f = 0; f < 3; f++) pl[f][t] = in ? pf[f][o] : 0.0;
+        for (int f = 0; f < 3; f++) vp[u][f] = pf[f][o];
+    }
+    const int pi = i0 + (threadIdx.x & 63), pj = j0 + (threadIdx.x >> 6);
+    const bool pout = pi < nxt && pj < nyt;
+    const long long po = pout ? (long long)pj * P + pi : 0;
+    const double fnet = a.fnetat[po], xc1 = a.xc1ast[po];
+#pragma unroll
+    for (int u = 0; u < NT; u++) {
+        const int t = threadIdx.x + 256 * u;
+        if (t < AML_LH * AML_LW)
+#pragma unroll
+            for (int f = 0; f < 4; f++) tl[f][t] = okt[u] ? vt[u][f] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NPL; u++) {
+        const int t = threadIdx.x + 256 * u;
+        if (t < AML_PH * AML_PW)
+#pragma unroll
+            for (int f = 0; f < 3; f++) pl[f][t] = okp[u] ? vp[u][f] : 0.0;
     }
     __syncthreads();
     const double hm = a.hma;
@@ -349,9 +376,9 @@ __global__ __launch_bounds__(256) void k_aml_step(CplArgs a) {
         dl[1][t] = in ? del2T_at(hmxm, i, j, nxt, nyt, g.cyclic, fx, fh.fluxn, fh.nval, fh.fluxs, fh.sval) : 0.0;
     }
     __syncthreads();
-    const int i = i0 + (threadIdx.x & 63), j = j0 + (threadIdx.x >> 6);
-    if (i >= nxt || j >= nyt) return;
-    const long long o = (long long)j * P + i;
+    const int i = pi, j = pj;
+    if (!pout) return;
+    const long long o = po;
     const PLds p1{pl[0], i0, j0}, tx{pl[1], i0, j0}, ty{pl[2], i0, j0};
     const DLds d2a{dl[0], i0, j0, nxt, g.cyclic}, d2h{dl[1], i0, j0, nxt, g.cyclic};
     const double tm = adv_diff_d2(g, fa, ast, d2a, p1, tx, ty, hm, a.tau_sign_a, i, j);
@@ -374,13 +401,13 @@ __global__ __launch_bounds__(256) void k_aml_step(CplArgs a) {
         dT_fix = 0.0;
     }
     const double hmainv = 1.0 / hm, rrcpat = 1.0 / a.rho_cp;
-    const double r = tm + rrcpat * a.fnetat[o] / hmixam - hmainv * wekt * astm0;
+    const double r = tm + rrcpat * fnet / hmixam - hmainv * wekt * astm0;
     const double an = astm0 + tdta * r + dT_fix;
     const double dtc = fmin(0.0, a.T1_rel - an);
     a.anew[o] = an + dtc;
     a.hnew[o] = hnew;
     // compute_atmos_entrainment's T-point terms (amlsubs.F90:303-309)
-    a.xfa[o] = a.xcexp * a.bface * (hmixam - hm) + a.dface * (a.xcexp * astm0 + (1.0 - a.xcexp) * a.xc1ast[o]) +
+    a.xfa[o] = a.xcexp * a.bface * (hmixam - hm) + a.dface * (a.xcexp * astm0 + (1.0 - a.xcexp) * xc1) +
                (a.xcexp / (tdta * a.dT_12)) * hmx(i, j) * dtc;
 }
 
