@@ -251,9 +251,30 @@ __global__ __launch_bounds__(256) void k_force_atm(CplArgs a) {
     const int n = a.ndxr, nxto = go.nxp - 1, nyto = go.nyp - 1;
     if (ia1 >= 0 && ja1 >= 0) {   // k_force_ocn wrote each cell's flux
         const int jo0 = ja1 * n, io0 = ia1 * n, jo1 = min(jo0 + n, nyto), io1 = min(io0 + n, nxto);
-        for (int jo = jo0; jo < jo1; jo++) {
-            const double* row = a.ocflux + (long long)jo * go.pitch;
-            for (int io = io0; io < io1; io++) f = f + ocfrac * row[io];
+        constexpr int FR = 4, FC = 16;   // rows per batch of loads, columns of a row held
+        if (io1 - io0 <= FC) {
+            // FR rows loaded together, then added in the same order
+            for (int jb = jo0; jb < jo1; jb += FR) {
+                double v[FR][FC];
+#pragma unroll
+                for (int r = 0; r < FR; r++)
+#pragma unroll
+                    for (int u = 0; u < FC; u++) {
+                        const bool ok = jb + r < jo1 && io0 + u < io1;
+                        const long long q = ok ? (long long)(jb + r) * go.pitch + io0 + u : 0;
+                        v[r][u] = a.ocflux[q];
+                    }
+#pragma unroll
+                for (int r = 0; r < FR; r++)
+#pragma unroll
+                    for (int u = 0; u < FC; u++)
+                        if (jb + r < jo1 && io0 + u < io1) f = f + ocfrac * v[r][u];
+            }
+        } else {
+            for (int jo = jo0; jo < jo1; jo++) {
+                const double* row = a.ocflux + (long long)jo * go.pitch;
+                for (int io = io0; io < io1; io++) f = f + ocfrac * row[io];
+            }
         }
     }
     // the thickness, topography and mixed-layer depth terms (forcing.F90:87-97)
