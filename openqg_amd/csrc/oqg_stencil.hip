@@ -34,12 +34,17 @@ __device__ __forceinline__ double jac_int(const double* q, const double* p, int 
     return g.jfac * t;
 }
 
+// Threads of a tile's workgroup: 512 for 16-row tiles (two output rows per
+// thread), 256 (four rows of 64) otherwise.
+__host__ __device__ constexpr int vort_threads(int ty) { return ty >= 16 ? 512 : 256; }
+
 // Tile (bx, by) of layer k, TY rows of 64 points.
 template <int TY, bool EDGE>
 __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int k, double* s_pm, double* s_d2,
                                           double* s_d4, double* s_q, double* s_p, double* s_red) {
     constexpr int TX = VORT_TX;
-    static_assert(TY % 4 == 0 && TY <= 64, "four rows of 64 threads; the ring logic takes tx < TY");
+    constexpr int NTH = vort_threads(TY), TR = NTH / 64;   // TR rows of 64 threads
+    static_assert(TY % TR == 0 && TY <= 64 && TR >= 4, "rows of 64 threads; the ring logic takes waves 0..3 and tx < TY");
     constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
     const Geo& g = a.g;
     const int tid = threadIdx.x;
@@ -56,12 +61,12 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
     // inputs of this thread's four output points.
     // Edge tiles load from clamped offsets and zero the out-of-basin values
     // when they are stored, so their loads carry no branches either.
-    constexpr int NPM = (H3 * W3 + 255) / 256, NQP = (H1 * W1 + 255) / 256, NOUT = TY / 4;
+    constexpr int NPM = (H3 * W3 + NTH - 1) / NTH, NQP = (H1 * W1 + NTH - 1) / NTH, NOUT = TY / TR;
     double rpm[NPM], rq[NQP], rp[NQP], rqm[NOUT], rent[NOUT], rwek[NOUT];
     bool okpm[NPM], okqp[NQP];
 #pragma unroll
     for (int t = 0; t < NPM; t++) {
-        const int idx = tid + 256 * t;
+        const int idx = tid + NTH * t;
         const int r = idx / W3, c = idx - r * W3;
         if constexpr (EDGE) {
             bool ok;
@@ -74,7 +79,7 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
     }
 #pragma unroll
     for (int t = 0; t < NQP; t++) {
-        const int idx = tid + 256 * t;
+        const int idx = tid + NTH * t;
         const int r = idx / W1, c = idx - r * W1;
         const bool in = idx < H1 * W1;
         if constexpr (EDGE) {
@@ -92,7 +97,7 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
     const int gi = i0 + tx;
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
-        const int gj = j0 + ty + 4 * u;
+        const int gj = j0 + ty + TR * u;
         const bool in = !EDGE || (gi < lim_x && gj < lim_y);
         const long long o2 = in ? (long long)gj * g.pitch + gi : 0;
         rqm[u] = a.qm[lo + o2];
@@ -141,12 +146,12 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
     }
 #pragma unroll
     for (int t = 0; t < NPM; t++) {
-        const int idx = tid + 256 * t;
+        const int idx = tid + NTH * t;
         if (idx < H3 * W3) s_pm[idx] = (!EDGE || okpm[t]) ? rpm[t] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < NQP; t++) {
-        const int idx = tid + 256 * t;
+        const int idx = tid + NTH * t;
         if (idx < H1 * W1) {
             s_q[idx] = (!EDGE || okqp[t]) ? rq[t] : 0.0;
             s_p[idx] = (!EDGE || okqp[t]) ? rp[t] : 0.0;
@@ -156,8 +161,8 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
     // Del^2 of pm on the tile with halo 2
     if constexpr (EDGE) {
 #pragma unroll
-        for (int t = 0; t < (H2 * W2 + 255) / 256; t++) {
-            const int idx = tid + 256 * t;
+        for (int t = 0; t < (H2 * W2 + NTH - 1) / NTH; t++) {
+            const int idx = tid + NTH * t;
             const int r = idx / W2, c = idx - r * W2;
             const int gj = j0 - 2 + r, gx = i0 - 2 + c;
             const int gxw = g.cyclic ? wrapx(gx, g) : gx;
@@ -167,18 +172,20 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < H2 / 4; u++) {
-            const int r = ty + 4 * u;
-            s_d2[r * W2 + tx] = lap_int(s_pm, W3, (r + 1) * W3 + tx + 1, g);
-            if (tx < W2 - 64) s_d2[r * W2 + tx + 64] = lap_int(s_pm, W3, (r + 1) * W3 + tx + 65, g);
+        for (int u = 0; u < (H2 + TR - 1) / TR; u++) {
+            const int r = ty + TR * u;
+            if (r < H2) {
+                s_d2[r * W2 + tx] = lap_int(s_pm, W3, (r + 1) * W3 + tx + 1, g);
+                if (tx < W2 - 64) s_d2[r * W2 + tx + 64] = lap_int(s_pm, W3, (r + 1) * W3 + tx + 65, g);
+            }
         }
     }
     __syncthreads();
     // Del^4 on the tile with halo 1
     if constexpr (EDGE) {
 #pragma unroll
-        for (int t = 0; t < (H1 * W1 + 255) / 256; t++) {
-            const int idx = tid + 256 * t;
+        for (int t = 0; t < (H1 * W1 + NTH - 1) / NTH; t++) {
+            const int idx = tid + NTH * t;
             const int r = idx / W1, c = idx - r * W1;
             const int gj = j0 - 1 + r, gx = i0 - 1 + c;
             const int gxw = g.cyclic ? wrapx(gx, g) : gx;
@@ -188,8 +195,8 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < (H1 + 3) / 4; u++) {
-            const int r = ty + 4 * u;
+        for (int u = 0; u < (H1 + TR - 1) / TR; u++) {
+            const int r = ty + TR * u;
             if (r < H1) {
                 s_d4[r * W1 + tx] = lap_int(s_d2, W2, (r + 1) * W2 + tx + 1, g);
                 if (tx < W1 - 64) s_d4[r * W1 + tx + 64] = lap_int(s_d2, W2, (r + 1) * W2 + tx + 65, g);
@@ -227,7 +234,7 @@ __device__ __forceinline__ void vort_tile(const VortArgs& a, int bx, int by, int
     double epart = 0.0;
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
-        const int r = ty + 4 * u;
+        const int r = ty + TR * u;
         const int gj = j0 + r;
         const bool out = !EDGE || (gi < lim_x && gj < lim_y);
         const int o1 = (r + 1) * W1 + tx + 1;
@@ -285,7 +292,7 @@ __device__ __forceinline__ void vort_block(const VortArgs& a, int bx, int by, in
 // Each XCD takes a contiguous run of tiles (whole rows of tiles), so the halo
 // lines that neighbouring tiles share are fetched from HBM into one L2.
 template <int TY>
-__global__ __launch_bounds__(256) void k_vort(VortArgs a) {
+__global__ __launch_bounds__(vort_threads(TY), TY >= 16 ? 6 : 1) void k_vort(VortArgs a) {
     int bx, by, bz;
     xcd_block(bx, by, bz);
     vort_block<TY>(a, bx, by, bz);
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(256) void k_vort(VortArgs a) {
 // Ensemble form: block b works on tile map[b].y of member map[b].x, whose
 // arguments are args[map[b].x] (tiles numbered x fastest, then y, then layer).
 template <int TY>
-__global__ __launch_bounds__(256) void k_vort_ens(const VortArgs* __restrict__ args, const int2* __restrict__ map) {
+__global__ __launch_bounds__(vort_threads(TY), TY >= 16 ? 6 : 1) void k_vort_ens(const VortArgs* __restrict__ args, const int2* __restrict__ map) {
     const int2 e = map[blockIdx.x];
     const VortArgs& a = args[e.x];
     const int per_layer = a.tiles_x * a.tiles_y;
@@ -330,13 +337,13 @@ void launch_vort(const VortArgs& a, hipStream_t st) {
     dim3 grid(a.tiles_x, a.tiles_y, a.g.nl);
     if (a.ty == 4) hipLaunchKernelGGL(k_vort<4>, grid, dim3(256), 0, st, a);
     else if (a.ty == 8) hipLaunchKernelGGL(k_vort<8>, grid, dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_vort<16>, grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_vort<16>, grid, dim3(vort_threads(16)), 0, st, a);
 }
 
 void launch_vort_ens(const VortArgs* args, const int2* map, int nblocks, int ty, hipStream_t st) {
     if (ty == 4) hipLaunchKernelGGL(k_vort_ens<4>, dim3(nblocks), dim3(256), 0, st, args, map);
     else if (ty == 8) hipLaunchKernelGGL(k_vort_ens<8>, dim3(nblocks), dim3(256), 0, st, args, map);
-    else hipLaunchKernelGGL(k_vort_ens<16>, dim3(nblocks), dim3(256), 0, st, args, map);
+    else hipLaunchKernelGGL(k_vort_ens<16>, dim3(nblocks), dim3(vort_threads(16)), 0, st, args, map);
 }
 
 // ------------------------------------------------------------------ k_suppress
