@@ -36,11 +36,11 @@ def _periodic(a):
     return a
 
 
-def _channel():
-    """33 x 17 three-layer channel with Laplacian friction and bottom drag, so
-    that every term of the momentum constraints is active."""
-    w = W.small_channel(33, 17, 3)
-    w.ah2 = [2.0e5] * 3
+def _channel(nx=33, ny=17, nl=3):
+    """A small channel with Laplacian friction and bottom drag, so that every
+    term of the momentum constraints is active."""
+    w = W.small_channel(nx, ny, nl)
+    w.ah2 = [2.0e5] * nl
     w.delek = 2.0
     return w
 
@@ -69,9 +69,11 @@ def _fstep(qg, adj, S, C, wekp, txi, ent, tdt, suppressed):
     return qg.get_state(), qg.get_constr()
 
 
-@pytest.mark.parametrize("suppressed", [False, True], ids=["plain", "suppressed"])
-def test_channel_one_step_dot_product(gpu, suppressed):
-    w = _channel()
+@pytest.mark.parametrize("suppressed,shape", [(False, (33, 17, 3)), (True, (33, 17, 3)), (False, (49, 21, 2)),
+                                               (True, (65, 25, 4)), (False, (97, 49, 3))],
+                         ids=["plain", "suppressed", "49x21x2", "65x25x4_suppressed", "97x49x3"])
+def test_channel_one_step_dot_product(gpu, suppressed, shape):
+    w = _channel(*shape)
     qg, ent, wekp, txi = _prepared(w, 5)
     adj = AdjointState(qg)
     adj.alloc(1)
