@@ -177,12 +177,15 @@ void launch_ke_seed(const KeSeedArgs& a, hipStream_t st) {
 //   lq_n += jfac J_1^T(mu; p_n),  lp_n += -jfac J_1^T(mu; q_n)   (J(q,p) = -J(p,q))
 //   lp_{n-1} = c2 (L^T)^2 mu - c4 (L^T)^3 mu - [k=kt] cdrag L^T mu
 //   lwekp += cwek mu_0,  lq_{n-1} = lq_{n+1} on the interior, 0 on the ring.
-// k_adj_vort tiles: 64 x 4 points of one layer per 256-thread block (one row
-// per thread), 22 KB of LDS, seven blocks per CU.  The kernel is latency-bound:
-// shorter tiles shorten each block's chain of loads and LDS passes while every
-// block stays resident (a gradient at 257^2 took 271.0 ms with 64 x 4 tiles,
-// 274.3 ms with 64 x 8; 64 x 16 tiles measured 14.9 against 12.3 us for 64 x 8).
+// The kernel is latency-bound: what counts is each thread's chain of loads
+// and LDS passes while every block stays resident.  k_adj_vort (box): 64 x 8
+// points of one layer per 512-thread block, one point per thread, 30 KB of
+// LDS, 76 VGPRs, three blocks per CU.  Reverse sweeps of the 257^2 gradient:
+// 190.1 ms with these, 191.0 ms with 64 x 4 tiles of 256 threads, 195.2 ms with
+// 64 x 2 of 128; two points per thread (64 x 8 tiles of 256 threads) were
+// slower still.  k_adj_vort_cyc (channel): 64 x 4 points per 256 threads.
 constexpr int ADJ_TX = 64, ADJ_TY = 4;
+constexpr int ADJB_TY = 8, ADJB_NTH = 512;
 
 // (L^T v)(a) of del2_P_bc (box, numerics.F90:427-468): L = Dxx + Dyy, Dxx
 // zero on the rows 0 and nyp-1, Dyy zero on the columns 0 and nxp-1, so the
@@ -232,10 +235,10 @@ __device__ __forceinline__ double arakawa_lds(const double* f, int Wf, int rf, i
     return jpp + jpx + jxp;
 }
 
-__global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
-    constexpr int TX = ADJ_TX, TY = ADJ_TY;
+__global__ __launch_bounds__(ADJB_NTH) void k_adj_vort(AdjVortArgs a) {
+    constexpr int TX = ADJ_TX, TY = ADJB_TY, NTH = ADJB_NTH, TR = NTH / 64;
     constexpr int W3 = TX + 6, H3 = TY + 6, W2 = TX + 4, H2 = TY + 4, W1 = TX + 2, H1 = TY + 2;
-    constexpr int NMU = (H3 * W3 + 255) / 256, NQP = (H1 * W1 + 255) / 256, NOUT = TY / 4;
+    constexpr int NMU = (H3 * W3 + NTH - 1) / NTH, NQP = (H1 * W1 + NTH - 1) / NTH, NOUT = TY / TR;
     __shared__ double s_mu[H3 * W3];
     __shared__ double s_n1[H2 * W2];
     __shared__ double s_n2[H1 * W1];
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     double vmu[NMU], vq[NQP], vp[NQP];
 #pragma unroll
     for (int u = 0; u < NMU; u++) {
-        const int idx = tid + 256 * u, r = idx / W3, c = idx - r * W3;
+        const int idx = tid + NTH * u, r = idx / W3, c = idx - r * W3;
         const int gi = i0 - 3 + c, gj = j0 - 3 + r;
         const bool ok = idx < H3 * W3 && gi >= 1 && gi <= g.nxp - 2 && gj >= 1 && gj <= g.nyp - 2;
         const double v = ln[ok ? (long long)gj * g.pitch + gi : 0];
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     }
 #pragma unroll
     for (int u = 0; u < NQP; u++) {
-        const int idx = tid + 256 * u, r = idx / W1, c = idx - r * W1;
+        const int idx = tid + NTH * u, r = idx / W1, c = idx - r * W1;
         const int gi = i0 - 1 + c, gj = j0 - 1 + r;
         const bool ok = idx < H1 * W1 && gi >= 0 && gi < g.nxp && gj >= 0 && gj < g.nyp;
         const long long o = ok ? (long long)gj * g.pitch + gi : 0;
@@ -281,7 +284,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     double olq[NOUT], olp[NOUT], olw[NOUT];
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
-        const int gj = j0 + ty + 4 * u;
+        const int gj = j0 + ty + TR * u;
         const bool ok = gi < g.nxp && gj < g.nyp;
         const long long o = ok ? (long long)gj * g.pitch + gi : 0;
         olq[u] = a.lq_n[lo + o];
@@ -290,12 +293,12 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     }
 #pragma unroll
     for (int u = 0; u < NMU; u++) {
-        const int idx = tid + 256 * u;
+        const int idx = tid + NTH * u;
         if (idx < H3 * W3) s_mu[idx] = vmu[u];
     }
 #pragma unroll
     for (int u = 0; u < NQP; u++) {
-        const int idx = tid + 256 * u;
+        const int idx = tid + NTH * u;
         if (idx < H1 * W1) {
             s_q[idx] = vq[u];
             s_p[idx] = vp[u];
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     }
     __syncthreads();
     // nu1 = L^T mu on the tile with halo 2
-    for (int idx = tid; idx < H2 * W2; idx += 256) {
+    for (int idx = tid; idx < H2 * W2; idx += NTH) {
         const int r = idx / W2, c = idx - r * W2;
         const int gi2 = i0 - 2 + c, gj2 = j0 - 2 + r;
         const bool in = gi2 >= 0 && gi2 < g.nxp && gj2 >= 0 && gj2 < g.nyp;
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     }
     __syncthreads();
     // nu2 = L^T nu1 on the tile with halo 1
-    for (int idx = tid; idx < H1 * W1; idx += 256) {
+    for (int idx = tid; idx < H1 * W1; idx += NTH) {
         const int r = idx / W1, c = idx - r * W1;
         const int gi1 = i0 - 1 + c, gj1 = j0 - 1 + r;
         const bool in = gi1 >= 0 && gi1 < g.nxp && gj1 >= 0 && gj1 < g.nyp;
@@ -325,7 +328,7 @@ __global__ __launch_bounds__(256) void k_adj_vort(AdjVortArgs a) {
     const double c2 = a.ph->c2[k], c4 = a.ph->c4[k];
 #pragma unroll
     for (int u = 0; u < NOUT; u++) {
-        const int r = ty + 4 * u, gj = j0 + r;
+        const int r = ty + TR * u, gj = j0 + r;
         if (gi >= g.nxp || gj >= g.nyp) continue;
         const long long o = lo + (long long)gj * g.pitch + gi;
         const double nu3 = lapT_sel(s_n2, W1, r + 1, tx + 1, gi, gj, g);
@@ -588,8 +591,8 @@ void launch_adj_vort(const AdjVortArgs& a, hipStream_t st) {
         hipLaunchKernelGGL(k_adj_vort_cyc, grid, dim3(256), 0, st, a);
         return;
     }
-    const dim3 grid((a.g.nxp + ADJ_TX - 1) / ADJ_TX, (a.g.nyp + ADJ_TY - 1) / ADJ_TY, a.g.nl);
-    hipLaunchKernelGGL(k_adj_vort, grid, dim3(256), 0, st, a);
+    const dim3 grid((a.g.nxp + ADJ_TX - 1) / ADJ_TX, (a.g.nyp + ADJB_TY - 1) / ADJB_TY, a.g.nl);
+    hipLaunchKernelGGL(k_adj_vort, grid, dim3(ADJB_NTH), 0, st, a);
 }
 
 }  // namespace oqg
