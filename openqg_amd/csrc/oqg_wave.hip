@@ -350,6 +350,19 @@ __device__ __forceinline__ void invw_block(const InvArgs& a, int jp) {
             hz[u][m - 1] = (in && zrow >= 0) ? hm[(long long)(zrow > 0 ? zrow : 0) * g.pitch] : 0.0;
         }
     }
+    // box: the ocqbdy work at the walls (items 0..3: row sel = e / 2, side
+    // e % 2) and at the corners of the zonal row next to the first or last
+    // row pair (items 4, 5) is done by the last six threads of the block, out
+    // of the unrolled per-point code (a large branch there, duplicated per
+    // point, cost 2.7 us of 13.7 at 513^2); each loads the homogeneous
+    // solutions at its point here
+    const int witem = nth - 1 - (int)threadIdx.x;
+    const bool wact = !CYC && witem < 6 && (witem < 2 || (witem < 4 ? has1 : zrow >= 0));
+    const int wsel = witem < 4 ? (witem >> 1) : 0, wside = witem & 1;
+    const int wi = wside ? g.nxp - 1 : 0, wj = witem < 4 ? (wsel ? gj1 : gj0) : (zrow > 0 ? zrow : 0);
+    double hw[NH];
+#pragma unroll
+    for (int m = 1; m < NL; m++) hw[m - 1] = wact ? a.hom[(long long)(m - 1) * g.fs + (long long)wj * g.pitch + wi] : 0.0;
     const double by0 = g.beta * a.yprel[gj0], by1 = has1 ? g.beta * a.yprel[gj1] : 0.0;
     const double byz0 = g.beta * a.yprel[0], byz1 = g.beta * a.yprel[g.nyp - 1];
 
@@ -470,9 +483,15 @@ __device__ __forceinline__ void invw_block(const InvArgs& a, int jp) {
     const double qbfac = s_dc.ph.qbfac;
     const int nxp = g.nxp;
     double* const pnew = a.pnew;
-    auto point = [&](int sel, int i, const double (&hp)[NH], const double (&hzp)[NH]) {
+    const int zn = zrow == 0 ? 0 : 1;
+    const double byz = zn == 0 ? byz0 : byz1;
+    double zero[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) zero[m] = 0.0;
+    // p_new at point i of row sel (every point, walls included)
+    auto main_point = [&](int sel, int i, const double (&hp)[NH], double (&pk)[NL]) {
         const int gj = sel == 0 ? gj0 : gj1;
-        double inh[NL], hc[NL], pk[NL];
+        double inh[NL], hc[NL];
 #pragma unroll
         for (int m = 0; m < NL; m++) inh[m] = inh_at<CYC>(g, s, wsm + (size_t)m * ZL, i, sel);
         if constexpr (CYC) homcor<NL>(true, coef, s_hr[sel], hc);
@@ -481,57 +500,77 @@ __device__ __forceinline__ void invw_block(const InvArgs& a, int jp) {
         const long long o = (long long)gj * g.pitch + i;
 #pragma unroll
         for (int k = 0; k < NL; k++) pnew[k * g.fs + o] = pk[k];
-        const bool wall = !CYC && (i == 0 || i == nxp - 1);
-        const int side = i == 0 ? 0 : 1;
-        double pin[NL];
-        if (wall) {
-            const int ii = i == 0 ? 1 : nxp - 2;
-#pragma unroll
-            for (int m = 0; m < NL; m++) inh[m] = inh_at<CYC>(g, s, wsm + (size_t)m * ZL, ii, sel);
-            homcor<NL>(false, coef, s_hin + (sel * 2 + side) * (NL - 1), hc);
-            mix_point<NL>(inh, hc, s_ctm2l, pin);
-            qbdy_at<NL>(a, s_amat, qbfac, sel == 0 ? by0 : by1, i, gj, 1, pk, pin);
-        }
-        if (zrow >= 0 && sel == zsel) {
-            const int zn = zrow == 0 ? 0 : 1;
-            double zero[NL], pb[NL];
-#pragma unroll
-            for (int m = 0; m < NL; m++) zero[m] = 0.0;
-            if constexpr (CYC) homcor<NL>(true, coef, s_hr[2 + zn], hc);
-            else homcor<NL>(false, coef, hzp, hc);
-            mix_point<NL>(zero, hc, s_ctm2l, pb);
-            const long long oz = (long long)zrow * g.pitch + i;
-#pragma unroll
-            for (int k = 0; k < NL; k++) pnew[k * g.fs + oz] = pb[k];
-            const double byz = zn == 0 ? byz0 : byz1;
-            if (wall) {
-                // corners take the x difference (vorsubs.F90:83-84 overwrite :79-80)
-                homcor<NL>(false, coef, s_hbc + (zn * 2 + side) * (NL - 1), hc);
-                mix_point<NL>(zero, hc, s_ctm2l, pin);
-                qbdy_at<NL>(a, s_amat, qbfac, byz, i, zrow, 2, pb, pin);
-            } else {
-                qbdy_at<NL>(a, s_amat, qbfac, byz, i, zrow, 0, pb, pk);
-            }
-        }
     };
+    // the zonal row next to the first and last row pairs: p (the homogeneous
+    // correction alone) and, away from a box's corners, its boundary q from the
+    // inner row's p at the same column (pin)
+    auto zone_point = [&](int i, const double (&hzp)[NH], const double (&pin)[NL]) {
+        double hc[NL], pb[NL];
+        if constexpr (CYC) homcor<NL>(true, coef, s_hr[2 + zn], hc);
+        else homcor<NL>(false, coef, hzp, hc);
+        mix_point<NL>(zero, hc, s_ctm2l, pb);
+        const long long oz = (long long)zrow * g.pitch + i;
+#pragma unroll
+        for (int k = 0; k < NL; k++) pnew[k * g.fs + oz] = pb[k];
+        if (CYC || (i != 0 && i != nxp - 1)) qbdy_at<NL>(a, s_amat, qbfac, byz, i, zrow, 0, pb, pin);
+    };
+    double pks[2][INV_EPU][NL] = {};
 #pragma unroll
     for (int sel = 0; sel < 2; sel++) {
         if (sel == 1 && !has1) break;
 #pragma unroll
         for (int u = 0; u < INV_EPU; u++) {
             const int i = (int)threadIdx.x + nth * u;
-            if (i < nxp) point(sel, i, hv[sel][u], hz[u]);
+            if (i < nxp) main_point(sel, i, hv[sel][u], pks[sel][u]);
         }
-        for (int i = (int)threadIdx.x + nth * INV_EPU; i < nxp; i += nth) {
-            const int gj = sel == 0 ? gj0 : gj1;
-            double h[NH], hzr[NH];
+    }
+    if (zrow >= 0) {
 #pragma unroll
-            for (int m = 1; m < NL; m++) {
-                const double* hm = a.hom + (long long)(m - 1) * g.fs + i;
-                h[m - 1] = CYC ? 0.0 : hm[(long long)gj * g.pitch];
-                hzr[m - 1] = (CYC || zrow < 0) ? 0.0 : hm[(long long)zrow * g.pitch];
-            }
-            point(sel, i, h, hzr);
+        for (int u = 0; u < INV_EPU; u++) {
+            const int i = (int)threadIdx.x + nth * u;
+            double pin[NL];
+#pragma unroll
+            for (int k = 0; k < NL; k++) pin[k] = zsel ? pks[1][u][k] : pks[0][u][k];
+            if (i < nxp) zone_point(i, hz[u], pin);
+        }
+    }
+    // wide basins: the points past the prefetched ones
+    for (int i = (int)threadIdx.x + nth * INV_EPU; i < nxp; i += nth) {
+        double h[2][NH], hzr[NH], pk[2][NL];
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            const double* hm = a.hom + (long long)(m - 1) * g.fs + i;
+            h[0][m - 1] = CYC ? 0.0 : hm[(long long)gj0 * g.pitch];
+            h[1][m - 1] = (CYC || !has1) ? 0.0 : hm[(long long)gj1 * g.pitch];
+            hzr[m - 1] = (CYC || zrow < 0) ? 0.0 : hm[(long long)zrow * g.pitch];
+        }
+        main_point(0, i, h[0], pk[0]);
+        if (has1) main_point(1, i, h[1], pk[1]);
+        if (zrow >= 0) {
+            double pz[NL];
+#pragma unroll
+            for (int k = 0; k < NL; k++) pz[k] = zsel ? pk[1][k] : pk[0][k];
+            zone_point(i, hzr, pz);
+        }
+    }
+    // box walls and corners (vorsubs.F90:79-84: the corners take the x
+    // difference): p at the wall point from the homogeneous correction (the
+    // transforms vanish there), p at its inner neighbour (columns 1, nxp-2)
+    if (wact) {
+        double hc[NL], pb[NL], pin[NL], inh[NL];
+        homcor<NL>(false, coef, hw, hc);
+        mix_point<NL>(zero, hc, s_ctm2l, pb);
+        if (witem < 4) {
+            const int ii = wside ? nxp - 2 : 1;
+#pragma unroll
+            for (int m = 0; m < NL; m++) inh[m] = inh_at<CYC>(g, s, wsm + (size_t)m * ZL, ii, wsel);
+            homcor<NL>(false, coef, s_hin + (wsel * 2 + wside) * (NL - 1), hc);
+            mix_point<NL>(inh, hc, s_ctm2l, pin);
+            qbdy_at<NL>(a, s_amat, qbfac, wsel == 0 ? by0 : by1, wi, wj, 1, pb, pin);
+        } else {
+            homcor<NL>(false, coef, s_hbc + (zn * 2 + wside) * (NL - 1), hc);
+            mix_point<NL>(zero, hc, s_ctm2l, pin);
+            qbdy_at<NL>(a, s_amat, qbfac, byz, wi, zrow, 2, pb, pin);
         }
     }
 }
