@@ -643,9 +643,81 @@ __device__ __forceinline__ void fwdw_adj_block(const FwdArgs& a, int jp) {
             h1[u][m - 1] = (in && has1) ? a.adj_hom[(long long)(m - 1) * g.fs + o1] : 0.0;
         }
     }
+    // ocqbdy's x difference at the walls, transposed: the inner points t = 1
+    // and nxp-2 of both rows take qb times the adjoint of the ring's q beside
+    // them.  Two threads (one per wall) load those ring values here and add
+    // their part after the points are written, out of the unrolled per-point
+    // code (a dependent load there sat on two wavefronts' paths).
+    const int cw = (int)threadIdx.x - (nth - 2);   // 0: west wall, 1: east wall, else none
+    const bool cact = cw >= 0;
+    const int ct = cw == 1 ? g.nxp - 2 : 1;
+    double cr0[NL], cr1[NL], ch0[NH], ch1[NH];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const long long o0 = (long long)gj0 * g.pitch + (cw == 1 ? g.nxp - 1 : 0);
+        const long long o1 = (long long)gj1 * g.pitch + (cw == 1 ? g.nxp - 1 : 0);
+        cr0[k] = cact ? lq[k * g.fs + o0] : 0.0;
+        cr1[k] = (cact && has1) ? lq[k * g.fs + o1] : 0.0;
+    }
+#pragma unroll
+    for (int m = 1; m < NL; m++) {
+        ch0[m - 1] = cact ? a.adj_hom[(long long)(m - 1) * g.fs + (long long)gj0 * g.pitch + ct] : 0.0;
+        ch1[m - 1] = (cact && has1) ? a.adj_hom[(long long)(m - 1) * g.fs + (long long)gj1 * g.pitch + ct] : 0.0;
+    }
     double part[NL];
 #pragma unroll
     for (int m = 0; m < NL; m++) part[m] = 0.0;
+    // the transpose of ocqbdy at an interior point: the ring points whose
+    // inner point it is are its west (t = 1), east (t = nxp-2), south (gj = 1)
+    // and north (gj = nyp-2) neighbours, added in that order; rs/rn: the
+    // loaded south or north neighbours
+    auto gather = [&](int t, int gj, double (&lp)[NL], const double (&rs)[NL]) {
+        if (gj == 1 || gj == g.nyp - 2)
+#pragma unroll
+            for (int k = 0; k < NL; k++) lp[k] += qb * rs[k];
+    };
+    auto put = [&](int t, double (&la)[NL], double (&lb)[NL], const double (&ha)[NH], const double (&hb)[NH],
+                   const double (&ra)[NL], const double (&rb)[NL]) {
+        gather(t, gj0, la, ra);
+        if (has1) gather(t, gj1, lb, rb);
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double wa = 0.0, wb = 0.0;
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                wa += cm[m][k] * la[k];
+                wb += cm[m][k] * lb[k];
+            }
+            if (!has1) wb = 0.0;
+            double2* z = wsm + (size_t)m * ZL;
+            z[zpad(t)] = make_double2(wa, wb);
+            z[zpad(L - t)] = make_double2(-wa, -wb);
+            if (m > 0) part[m] += ha[m - 1] * wa + hb[m - 1] * wb;
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < WFPT; u++) {
+        const int t = 1 + (int)threadIdx.x + nth * u;
+        if (t <= s.nin) put(t, p0[u], p1[u], h0[u], h1[u], rq0[u], rq1[u]);
+    }
+    for (int t = 1 + (int)threadIdx.x + nth * WFPT; t <= s.nin; t += nth) {
+        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
+        double la[NL], lb[NL], ha[NH], hb[NH], ra[NL], rb[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            la[k] = a.adj_lp[k * g.fs + o0];
+            lb[k] = has1 ? a.adj_lp[k * g.fs + o1] : 0.0;
+            ra[k] = (s0 || n0) ? lq[k * g.fs + o0 + (s0 ? -g.pitch : g.pitch)] : 0.0;
+            rb[k] = n1 ? lq[k * g.fs + o1 + g.pitch] : 0.0;
+        }
+#pragma unroll
+        for (int m = 1; m < NL; m++) {
+            ha[m - 1] = a.adj_hom[(long long)(m - 1) * g.fs + o0];
+            hb[m - 1] = has1 ? a.adj_hom[(long long)(m - 1) * g.fs + o1] : 0.0;
+        }
+        put(t, la, lb, ha, hb, ra, rb);
+    }
+    // (after the points, so that no wavefront waits on this rarer work first)
     // ring points of the inner products: columns 0 and nxp-1 of the block's
     // rows, and its share of rows 0 and nyp-1 (2 nxp points dealt out evenly).
     // A thread's first one is loaded with the batch above: lp and lq at the
@@ -725,73 +797,40 @@ __device__ __forceinline__ void fwdw_adj_block(const FwdArgs& a, int jp) {
             part[m] += a.adj_hom[(long long)(m - 1) * g.fs + o] * w;
         }
     }
-    // the transpose of ocqbdy at an interior point: the ring points whose
-    // inner point it is are its west (t = 1), east (t = nxp-2), south (gj = 1)
-    // and north (gj = nyp-2) neighbours, added in that order; rs/rn: the
-    // loaded south or north neighbours
-    auto gather = [&](int t, int gj, double (&lp)[NL], const double (&rs)[NL]) {
-        const long long o = (long long)gj * g.pitch + t;
-        if (t == 1)
-#pragma unroll
-            for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o - 1];
-        if (t == g.nxp - 2)
-#pragma unroll
-            for (int k = 0; k < NL; k++) lp[k] += qb * lq[k * g.fs + o + 1];
-        if (gj == 1 || gj == g.nyp - 2)
-#pragma unroll
-            for (int k = 0; k < NL; k++) lp[k] += qb * rs[k];
-    };
-    auto put = [&](int t, double (&la)[NL], double (&lb)[NL], const double (&ha)[NH], const double (&hb)[NH],
-                   const double (&ra)[NL], const double (&rb)[NL]) {
-        gather(t, gj0, la, ra);
-        if (has1) gather(t, gj1, lb, rb);
-#pragma unroll
-        for (int m = 0; m < NL; m++) {
-            double wa = 0.0, wb = 0.0;
-#pragma unroll
-            for (int k = 0; k < NL; k++) {
-                wa += cm[m][k] * la[k];
-                wb += cm[m][k] * lb[k];
-            }
-            if (!has1) wb = 0.0;
-            double2* z = wsm + (size_t)m * ZL;
-            z[zpad(t)] = make_double2(wa, wb);
-            z[zpad(L - t)] = make_double2(-wa, -wb);
-            if (m > 0) part[m] += ha[m - 1] * wa + hb[m - 1] * wb;
-        }
-    };
-#pragma unroll
-    for (int u = 0; u < WFPT; u++) {
-        const int t = 1 + (int)threadIdx.x + nth * u;
-        if (t <= s.nin) put(t, p0[u], p1[u], h0[u], h1[u], rq0[u], rq1[u]);
-    }
-    for (int t = 1 + (int)threadIdx.x + nth * WFPT; t <= s.nin; t += nth) {
-        const long long o0 = (long long)gj0 * g.pitch + t, o1 = (long long)gj1 * g.pitch + t;
-        double la[NL], lb[NL], ha[NH], hb[NH], ra[NL], rb[NL];
-#pragma unroll
-        for (int k = 0; k < NL; k++) {
-            la[k] = a.adj_lp[k * g.fs + o0];
-            lb[k] = has1 ? a.adj_lp[k * g.fs + o1] : 0.0;
-            ra[k] = (s0 || n0) ? lq[k * g.fs + o0 + (s0 ? -g.pitch : g.pitch)] : 0.0;
-            rb[k] = n1 ? lq[k * g.fs + o1 + g.pitch] : 0.0;
-        }
-#pragma unroll
-        for (int m = 1; m < NL; m++) {
-            ha[m - 1] = a.adj_hom[(long long)(m - 1) * g.fs + o0];
-            hb[m - 1] = has1 ? a.adj_hom[(long long)(m - 1) * g.fs + o1] : 0.0;
-        }
-        put(t, la, lb, ha, hb, ra, rb);
-    }
     if (threadIdx.x < NL) {
         double2* z = wsm + (size_t)threadIdx.x * ZL;
         z[zpad(0)] = make_double2(0.0, 0.0);
         z[zpad(L / 2)] = make_double2(0.0, 0.0);
     }
     stage_store(stw, (double*)tw, a.tw, 2 * twiddle_table_len(L));
+    // the walls' part: its modal input and its share of the inner products
+    double dw0[NL], dw1[NL];
+#pragma unroll
+    for (int m = 0; m < NL; m++) {
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            s0 += cm[m][k] * (qb * cr0[k]);
+            s1 += cm[m][k] * (qb * cr1[k]);
+        }
+        dw0[m] = s0;
+        dw1[m] = has1 ? s1 : 0.0;
+        if (m > 0) part[m] += ch0[m - 1] * dw0[m] + ch1[m - 1] * dw1[m];
+    }
     block_sum<NL>(part, s_red);   // its barriers also publish the transforms' inputs
     if (threadIdx.x == 0)
 #pragma unroll
         for (int m = 0; m < NL; m++) a.adj_partial[(long long)jp * MAXL + m] = part[m];
+    if (cact) {
+#pragma unroll
+        for (int m = 0; m < NL; m++) {
+            double2* zm = wsm + (size_t)m * ZL;
+            const double2 v = zm[zpad(ct)];
+            zm[zpad(ct)] = make_double2(v.x + dw0[m], v.y + dw1[m]);
+            zm[zpad(L - ct)] = make_double2(-(v.x + dw0[m]), -(v.y + dw1[m]));
+        }
+    }
+    __syncthreads();
     double2* z = wsm + (size_t)wv * ZL;
     fft_wave_any<P, -1>(z, L, s.nstages, s.radix, tw, lane);
     double* row0 = a.spec + ((long long)wv * s.nrow + r0) * s.kp;
