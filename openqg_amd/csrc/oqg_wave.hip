@@ -884,7 +884,21 @@ __device__ __forceinline__ void invw_adj_block(const InvArgs& a, int jp) {
     double ctlv = 0.0;
     if (helper) {
         if (lane < MAXL * MAXL) ctlv = a.dc->mx.ctl2m[lane];
-        for (int b = lane; b < a.adj_npart; b += 64)
+        // the first four per lane loaded together, then added in row-pair order
+        constexpr int NB = 4;
+        double pb[NB][NL];
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int b = lane + 64 * u;
+#pragma unroll
+            for (int m = 0; m < NL; m++) pb[u][m] = b < a.adj_npart ? a.adj_partial[(long long)b * MAXL + m] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+            if (lane + 64 * u < a.adj_npart)
+#pragma unroll
+                for (int m = 0; m < NL; m++) la[m] += pb[u][m];
+        for (int b = lane + 64 * NB; b < a.adj_npart; b += 64)
 #pragma unroll
             for (int m = 0; m < NL; m++) la[m] += a.adj_partial[(long long)b * MAXL + m];
     }
@@ -1180,7 +1194,20 @@ __device__ __forceinline__ void invw_adj_cyc_block(const InvArgs& a, int jp) {
     double ctlv = 0.0;
     if (helper) {
         if (lane < MAXL * MAXL) ctlv = a.dc->mx.ctl2m[lane];
-        for (int b = lane; b < a.adj_npart; b += 64)
+        constexpr int NB = 2;   // loaded together, then added in row-pair order
+        double pb[NB][NP];
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int b = lane + 64 * u;
+#pragma unroll
+            for (int q = 0; q < NP; q++) pb[u][q] = b < a.adj_npart ? a.adj_partial[(long long)b * ADJ_CYC_NPROF + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+            if (lane + 64 * u < a.adj_npart)
+#pragma unroll
+                for (int q = 0; q < NP; q++) pv[q] += pb[u][q];
+        for (int b = lane + 64 * NB; b < a.adj_npart; b += 64)
 #pragma unroll
             for (int q = 0; q < NP; q++) pv[q] += a.adj_partial[(long long)b * ADJ_CYC_NPROF + q];
     }
