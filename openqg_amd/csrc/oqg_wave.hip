@@ -667,6 +667,41 @@ __device__ __forceinline__ void fwdw_adj_block(const FwdArgs& a, int jp) {
     double part[NL];
 #pragma unroll
     for (int m = 0; m < NL; m++) part[m] = 0.0;
+    // ring points of the inner products: columns 0 and nxp-1 of the block's
+    // rows, and its share of rows 0 and nyp-1 (2 nxp points dealt out evenly).
+    // A thread's first one is loaded with the batch above: lp and lq at the
+    // point, lq at the corner beside it when its inner point is this point
+    // (columns 1 and nxp-2 of rows 0 and nyp-1), and the homogeneous solutions.
+    const int npairs = (s.nyi + 1) / 2, nring = 2 * g.nxp;
+    const int rper = (nring + npairs - 1) / npairs;
+    const int rlo = min(nring, jp * rper), rhi = min(nring, rlo + rper);
+    const int nside = has1 ? 4 : 2;
+    const int nextra = nside + (rhi - rlo);
+    auto extra_point = [&](int e, int& i, int& gj) {
+        if (e < nside) {
+            gj = (e >> 1) ? gj1 : gj0;
+            i = (e & 1) ? g.nxp - 1 : 0;
+        } else {
+            const int idx = rlo + (e - nside);
+            gj = idx < g.nxp ? 0 : g.nyp - 1;
+            i = idx < g.nxp ? idx : idx - g.nxp;
+        }
+    };
+    const bool ehas = (int)threadIdx.x < nextra;
+    int ei = 0, ej = 0;
+    if (ehas) extra_point(threadIdx.x, ei, ej);
+    const bool erow = ej == 0 || ej == g.nyp - 1;
+    const bool ewc = ehas && erow && ei == 1, eec = ehas && erow && ei == g.nxp - 2;
+    const long long eo = (long long)ej * g.pitch + ei;
+    double elp[NL], elq[NL], elc[NL], ehm[NH];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        elp[k] = ehas ? a.adj_lp[k * g.fs + eo] : 0.0;
+        elq[k] = ehas ? lq[k * g.fs + eo] : 0.0;
+        elc[k] = (ewc || eec) ? lq[k * g.fs + eo + (ewc ? -1 : 1)] : 0.0;
+    }
+#pragma unroll
+    for (int m = 1; m < NL; m++) ehm[m - 1] = ehas ? a.adj_hom[(long long)(m - 1) * g.fs + eo] : 0.0;
     // the transpose of ocqbdy at an interior point: the ring points whose
     // inner point it is are its west (t = 1), east (t = nxp-2), south (gj = 1)
     // and north (gj = nyp-2) neighbours, added in that order; rs/rn: the
@@ -717,68 +752,31 @@ __device__ __forceinline__ void fwdw_adj_block(const FwdArgs& a, int jp) {
         }
         put(t, la, lb, ha, hb, ra, rb);
     }
-    // (after the points, so that no wavefront waits on this rarer work first)
-    // ring points of the inner products: columns 0 and nxp-1 of the block's
-    // rows, and its share of rows 0 and nyp-1 (2 nxp points dealt out evenly).
-    // A thread's first one is loaded with the batch above: lp and lq at the
-    // point, lq at the corner beside it when its inner point is this point
-    // (columns 1 and nxp-2 of rows 0 and nyp-1), and the homogeneous solutions.
-    const int npairs = (s.nyi + 1) / 2, nring = 2 * g.nxp;
-    const int rper = (nring + npairs - 1) / npairs;
-    const int rlo = min(nring, jp * rper), rhi = min(nring, rlo + rper);
-    const int nside = has1 ? 4 : 2;
-    const int nextra = nside + (rhi - rlo);
-    auto extra_point = [&](int e, int& i, int& gj) {
-        if (e < nside) {
-            gj = (e >> 1) ? gj1 : gj0;
-            i = (e & 1) ? g.nxp - 1 : 0;
-        } else {
-            const int idx = rlo + (e - nside);
-            gj = idx < g.nxp ? 0 : g.nyp - 1;
-            i = idx < g.nxp ? idx : idx - g.nxp;
-        }
-    };
-    {
-        const bool ehas = (int)threadIdx.x < nextra;
-        int ei = 0, ej = 0;
-        if (ehas) extra_point(threadIdx.x, ei, ej);
-        const bool erow = ej == 0 || ej == g.nyp - 1;
-        const bool ewc = ehas && erow && ei == 1, eec = ehas && erow && ei == g.nxp - 2;
-        const long long o = (long long)ej * g.pitch + ei;
-        double elp[NL], elq[NL], elc[NL], ehm[NH];
+    // (the ring points' arithmetic after the block's own points)
+    if (ehas) {
+        // bdy_gather at a ring point from the loaded values: the point's
+        // own term (rows and columns of ocqbdy twice at a corner), then
+        // the corner whose inner point it is
+        const Mix& mx = a.dc->mx;
+        const double r = ((ei == 0 || ei == g.nxp - 1) && erow) ? 2.0 : 1.0;
 #pragma unroll
         for (int k = 0; k < NL; k++) {
-            elp[k] = ehas ? a.adj_lp[k * g.fs + o] : 0.0;
-            elq[k] = ehas ? lq[k * g.fs + o] : 0.0;
-            elc[k] = (ewc || eec) ? lq[k * g.fs + o + (ewc ? -1 : 1)] : 0.0;
+#pragma unroll
+            for (int l = 0; l < NL; l++) {
+                double c = -r * g.fnot * mx.amat[k * MAXL + l];
+                if (l == k) c -= qb;
+                elp[l] += elq[k] * c;
+            }
         }
+        if (ewc || eec)
 #pragma unroll
-        for (int m = 1; m < NL; m++) ehm[m - 1] = ehas ? a.adj_hom[(long long)(m - 1) * g.fs + o] : 0.0;
-        if (ehas) {
-            // bdy_gather at a ring point from the loaded values: the point's
-            // own term (rows and columns of ocqbdy twice at a corner), then
-            // the corner whose inner point it is
-            const Mix& mx = a.dc->mx;
-            const double r = ((ei == 0 || ei == g.nxp - 1) && erow) ? 2.0 : 1.0;
+            for (int k = 0; k < NL; k++) elp[k] += qb * elc[k];
 #pragma unroll
-            for (int k = 0; k < NL; k++) {
+        for (int m = 1; m < NL; m++) {
+            double w = 0.0;
 #pragma unroll
-                for (int l = 0; l < NL; l++) {
-                    double c = -r * g.fnot * mx.amat[k * MAXL + l];
-                    if (l == k) c -= qb;
-                    elp[l] += elq[k] * c;
-                }
-            }
-            if (ewc || eec)
-#pragma unroll
-                for (int k = 0; k < NL; k++) elp[k] += qb * elc[k];
-#pragma unroll
-            for (int m = 1; m < NL; m++) {
-                double w = 0.0;
-#pragma unroll
-                for (int k = 0; k < NL; k++) w += cm[m][k] * elp[k];
-                part[m] += ehm[m - 1] * w;
-            }
+            for (int k = 0; k < NL; k++) w += cm[m][k] * elp[k];
+            part[m] += ehm[m - 1] * w;
         }
     }
     for (int e = (int)threadIdx.x + nth; e < nextra; e += nth) {   // very wide, short basins
