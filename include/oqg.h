@@ -161,7 +161,9 @@ int oqg_sync(oqg_ctx* ctx);
  * layer-1 entrainment integral.  oqg_tune times trial_steps steps of oqg_run
  * (from step 1, suppression every supp_every) with each height, restores the
  * state, keeps the fastest and returns it (negative on error); ms[] (3
- * entries: 4, 8, 16 rows; may be NULL) receives the times. */
+ * entries: 4, 8, 16 rows; may be NULL) receives the times.  It allocates a
+ * copy of the state (both time levels) while it runs and recaptures the
+ * context's graphs for each height. */
 int oqg_set_tile_rows(oqg_ctx* ctx, int rows);
 int oqg_get_tile_rows(const oqg_ctx* ctx);
 int oqg_tune(oqg_ctx* ctx, double tdt, int trial_steps, int supp_every, double* ms);
@@ -396,7 +398,8 @@ int oqg_ens_sync(oqg_ens* ens);
  * every tile height (the members' own default rule, then 4, 8 and 16 rows for
  * all; oqg_set_tile_rows), keeps the fastest pair (the tile height set on
  * the members) and returns its index t*4 + schedule (negative on error);
- * ms[] (16 entries, [t*4 + schedule], may be NULL) receives the times. */
+ * ms[] (16 entries, [t*4 + schedule], may be NULL) receives the times.  It
+ * allocates a copy of every member's state while it runs. */
 int oqg_ens_set_schedule(oqg_ens* ens, int schedule);
 int oqg_ens_get_schedule(const oqg_ens* ens);
 int oqg_ens_tune(oqg_ens* ens, const double* tdt, int trial_steps, int supp_every, double* ms);
